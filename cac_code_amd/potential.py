@@ -1,0 +1,144 @@
+"""EAM and LJ potential set-up on the host.
+
+* ``johnson_cu_funcfl`` tabulates an analytic Johnson-type EAM for Cu
+  (exponential density, exponential pair term, Banerjea-Smith embedding) to
+  the reference's hybrid funcfl layout: F(rho), then rho(r), then r*phi(r)
+  (pair_eam.cpp:197-268, the z2r block used as-is at pair_eam.cpp:486-487).
+* ``read_funcfl`` / ``write_funcfl`` read and write that file
+  (pair_eam.cpp:197-268: line 1 comment, line 2 ``Z mass``, line 3
+  ``nrho drho nr dr cut``, then nrho + 2*nr values, any number per line).
+* ``lj_coeffs`` gives lj1..lj4 exactly as PairLJCut::init_one computes them
+  (pair_lj_cut.cpp:140-198, libm ``pow``).
+
+The spline tables themselves (file2array + array2spline) are built by the
+native library (``cac_eam_tables_from_funcfl``), so the device and the host
+share one set of bits.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+# Johnson, Phys. Rev. B 37, 3924 (1988) functional forms; parameters for Cu:
+# cohesive energy E_c, nearest-neighbour distance r_e = a/sqrt(2) at
+# a = 3.615 A, alpha = sqrt(9*Omega*B/E_c) with Omega = a^3/4 and
+# B = 1.38 Mbar, beta and gamma the density and pair decay exponents,
+# phi_e the pair energy at r_e, f_e the density scale.
+JOHNSON_CU = dict(E_c=3.54, r_e=3.615 / math.sqrt(2.0), alpha=5.08, beta=5.85,
+                  gamma=8.00, phi_e=0.59, f_e=0.30)
+CU_MASS = 63.546
+CU_A0 = 3.615
+
+
+def _taper(r, r_s, r_c):
+    """C2 switch: 1 below r_s, 0 above r_c, 1-10x^3+15x^4-6x^5 between."""
+    x = np.clip((np.asarray(r, dtype=np.float64) - r_s) / (r_c - r_s), 0.0, 1.0)
+    return 1.0 - x ** 3 * (10.0 - 15.0 * x + 6.0 * x * x)
+
+
+@dataclass
+class Funcfl:
+    """Contents of one hybrid funcfl file (pair_eam.cpp:197-268)."""
+    comment: str
+    z: int
+    mass: float
+    nrho: int
+    drho: float
+    nr: int
+    dr: float
+    cut: float
+    frho: np.ndarray  # [nrho], F at rho = m*drho
+    rhor: np.ndarray  # [nr], rho at r = m*dr
+    zr: np.ndarray    # [nr], r*phi(r) at r = m*dr
+
+
+def johnson_cu_funcfl(nrho: int = 10000, nr: int = 10000, rc: float = 4.95,
+                      taper_width: float = 0.5, rho_max_factor: float = 2.0,
+                      params: dict | None = None) -> Funcfl:
+    """Tabulate the analytic Johnson Cu EAM with a C2 taper on f and phi."""
+    p = dict(JOHNSON_CU)
+    if params:
+        p.update(params)
+    r_s = rc - taper_width
+    dr = rc / (nr - 1)
+    r = np.arange(nr, dtype=np.float64) * dr
+
+    def f_of(rr):
+        return p["f_e"] * np.exp(-p["beta"] * (rr / p["r_e"] - 1.0)) * _taper(rr, r_s, rc)
+
+    def phi_of(rr):
+        return p["phi_e"] * np.exp(-p["gamma"] * (rr / p["r_e"] - 1.0)) * _taper(rr, r_s, rc)
+
+    rhor = f_of(r)
+    zr = r * phi_of(r)
+    # equilibrium density of the perfect fcc lattice at a = CU_A0 (shells
+    # inside rc), used for the Banerjea-Smith reference density
+    shells = _fcc_shells(CU_A0, rc)
+    rho_e = float(sum(n * f_of(np.array([d]))[0] for d, n in shells))
+    rho_max = rho_max_factor * rho_e
+    drho = rho_max / (nrho - 1)
+    rho = np.arange(nrho, dtype=np.float64) * drho
+    ab = p["alpha"] / p["beta"]
+    gb = p["gamma"] / p["beta"]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        x = rho / rho_e
+        lnx = np.where(x > 0, np.log(np.where(x > 0, x, 1.0)), 0.0)
+        frho = -p["E_c"] * (1.0 - ab * lnx) * np.where(x > 0, x ** ab, 0.0) \
+            - 6.0 * p["phi_e"] * np.where(x > 0, x ** gb, 0.0)
+    frho[0] = 0.0
+    return Funcfl("Johnson-type analytic EAM for Cu (exp density, exp pair, Banerjea-Smith F)",
+                  29, CU_MASS, nrho, drho, nr, dr, rc, frho, rhor, zr)
+
+
+def _fcc_shells(a: float, rc: float):
+    """(distance, count) of fcc neighbour shells with distance < rc."""
+    n = int(math.ceil(rc / (a / 2.0))) + 1
+    rng = np.arange(-n, n + 1)
+    i, j, k = np.meshgrid(rng, rng, rng, indexing="ij")
+    keep = ((i + j + k) % 2 == 0) & ~((i == 0) & (j == 0) & (k == 0))
+    d2 = (i[keep] ** 2 + j[keep] ** 2 + k[keep] ** 2).astype(np.int64)
+    out = []
+    for s in np.unique(d2):
+        d = a / 2.0 * math.sqrt(float(s))
+        if d < rc:
+            out.append((d, int(np.count_nonzero(d2 == s))))
+    return out
+
+
+def write_funcfl(path: str, ff: Funcfl) -> None:
+    with open(path, "w") as fh:
+        fh.write(ff.comment.replace("\n", " ") + "\n")
+        fh.write(f"{ff.z} {ff.mass!r}\n")
+        fh.write(f"{ff.nrho} {ff.drho!r} {ff.nr} {ff.dr!r} {ff.cut!r}\n")
+        for arr in (ff.frho, ff.rhor, ff.zr):
+            vals = [repr(float(v)) for v in arr]
+            for s in range(0, len(vals), 5):
+                fh.write(" ".join(vals[s:s + 5]) + "\n")
+
+
+def read_funcfl(path: str) -> Funcfl:
+    """PairEAM::read_file + grab, pair_eam.cpp:197-268 (values parsed with
+    the same decimal-to-double conversion as atof)."""
+    with open(path) as fh:
+        comment = fh.readline().rstrip("\n")
+        w = fh.readline().split()
+        z, mass = int(w[0]), float(w[1])
+        w = fh.readline().split()
+        nrho, drho, nr, dr, cut = int(w[0]), float(w[1]), int(w[2]), float(w[3]), float(w[4])
+        vals = np.array(fh.read().split(), dtype=np.float64)
+    need = nrho + 2 * nr
+    if vals.size < need:
+        raise ValueError(f"{path}: expected {need} table values, found {vals.size}")
+    return Funcfl(comment, z, mass, nrho, drho, nr, dr, cut,
+                  vals[:nrho].copy(), vals[nrho:nrho + nr].copy(), vals[nrho + nr:need].copy())
+
+
+def lj_coeffs(epsilon: float, sigma: float, cut: float):
+    """PairLJCut::init_one, pair_lj_cut.cpp:150-153 (offset off)."""
+    lj1 = 48.0 * epsilon * math.pow(sigma, 12.0)
+    lj2 = 24.0 * epsilon * math.pow(sigma, 6.0)
+    lj3 = 4.0 * epsilon * math.pow(sigma, 12.0)
+    lj4 = 4.0 * epsilon * math.pow(sigma, 6.0)
+    return lj1, lj2, lj3, lj4, cut * cut
