@@ -1,0 +1,246 @@
+#!/usr/bin/env python3
+"""Throughput of the CAC force-and-neighbor path on one MI355X.
+
+One step = FixNVE initial integrate, rebuild check (and rebuild when an atom
+or node moved more than half the skin), EAM force on atoms and element
+integration points with the node scatter, FixNVE final integrate
+(verlet.cpp:149-219), all device-resident through libcac_hip.so.
+
+Metric: Matom-steps/s = (N_atoms + 8*N_elements) * steps / s / 1e6.
+
+Default workload: EAM Cu CAC, periodic box of 198^3 fcc cells (715.8 A),
+elements of 11 sites per edge with 8 corner integration points, atomistic
+core of dropped blocks (484,484 atoms, 22,964 elements), Johnson-type
+analytic EAM tabulated with nrho = nr = 10000, rc = 4.95 A, neighbor cutoff
+5.95 A, 300 K seeded velocities, dt = 1 fs.  Synthetic data, generated here.
+
+The JSON line carries the roofline of the dominant kernel (algorithmic
+bytes / measured average launch time, HIP events on the context stream) and
+the CPU oracle timed on this host on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+K_RC = 42  # fcc Cu neighbours inside rc = 4.95 A (12 + 6 + 24)
+
+# algorithmic bytes per unit for each kernel family
+ALG_BYTES = {
+    "atom_rho": ("atoms", 24 + 8 + 4 * K_RC),
+    "atom_force": ("atoms", 24 + 8 + 24 + 4 * K_RC),
+    "ip_erho": ("elements", 192 + 64 + 8 * 8 * K_RC),
+    "ip_force": ("elements", 192 + 64 + 192 + 8 * 8 * K_RC),
+}
+B_ATOM = 616
+B_ELEM = 7640
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_workload(name, seed):
+    from cac_code_amd.lattice import build_cac_system, eam_c1_system, lj_c0_system
+    if name == "eam_c1":
+        return eam_c1_system(seed=seed)
+    if name == "lj_c0":
+        return lj_c0_system(seed=seed)
+    if name == "small":
+        return build_cac_system(20, 5, 8.0, temperature=300.0, seed=seed)
+    raise ValueError(name)
+
+
+def make_pair(workload):
+    from cac_code_amd import PairEAM, PairLJCut, johnson_cu_funcfl
+    if workload == "lj_c0":
+        p = PairLJCut()
+        p.settings(["5.0"])
+        p.coeff(["1", "1", "0.4093", "2.338"])
+        return p, 6.0, 1.0, None
+    ff = johnson_cu_funcfl()
+    p = PairEAM()
+    p.settings([])
+    p.coeff(["*", "*", "Cu_johnson.eam"], funcfl=ff)
+    return p, 5.95, 1.0, ff
+
+
+def gpu_run(s, pair, cutneigh, skin, steps, warmup, precision, every, delay):
+    from cac_code_amd import Engine
+    eng = Engine(0)
+    eng.load_system(s)
+    eng.set_pair(pair)
+    eng.set_neighbor(cutneigh, skin=skin, every=every, delay=delay, check=True)
+    eng.build()
+    eng.force()
+    nb_warm = eng.run(warmup, precision=precision) if warmup else 0
+    eng.synchronize()
+    eng.profile(True)
+    t0 = time.perf_counter()
+    nb = eng.run(steps, precision=precision)
+    eng.synchronize()
+    t1 = time.perf_counter()
+    prof = {}
+    for name in ("atom_rho", "atom_force", "ip_erho", "ip_force", "scatter", "atom_neigh", "ip_neigh",
+                 "elem_neigh"):
+        ms, cnt = eng.profile_avg_ms(name)
+        if cnt:
+            prof[name] = (ms, cnt)
+    eng.profile(False)
+    return eng, t1 - t0, nb, nb_warm, prof
+
+
+def cpu_baseline(s, pair, ff, cutneigh, seconds, rebuild_rate):
+    """The CPU FP64 oracle on this host: one neighbor build plus as many full
+    steps (force on atoms and elements, both NVE halves) as fit in `seconds`;
+    the build is charged at the GPU run's rebuild rate."""
+    import numpy as np
+    from oracle import oracle as O
+    from cac_code_amd.lattice import FTM2V_METAL
+    O.build()
+    x = s.x.copy(); v = s.v.copy(); nodex = s.nodex.copy(); nodev = s.nodev.copy()
+    es = O.ElementSet(nodex, s.nae, s.intp_ijk, s.intp_w)
+    t0 = time.perf_counter()
+    anum, anb = O.atom_neighbors(x, s.box_lo, s.box_hi, cutneigh, 128)
+    inum, inb = O.ip_neighbors(es, s.box_lo, s.box_hi, cutneigh, 2, 128)
+    t_build = time.perf_counter() - t0
+    if ff is not None:
+        T = O.eam_tables(ff)
+        fa = lambda: O.eam_atoms(x, s.box_lo, s.box_hi, anum, anb, T)["f"]
+        fe = lambda: O.eam_elements(es, s.box_lo, s.box_hi, inum, inb, T)["nodef"]
+    else:
+        lj = pair.init_one(1, 1)
+        fa = lambda: O.lj_atoms(x, s.box_lo, s.box_hi, anum, anb, lj)["f"]
+        fe = lambda: O.lj_elements(es, s.box_lo, s.box_hi, inum, inb, lj)["nodef"]
+    dt, dtf = 0.001, 0.5 * 0.001 * FTM2V_METAL
+    m, me = s.mass, s.mass_e
+    f = fa(); nodef = fe()
+    xc = np.zeros((s.n_elements, 3))
+    nsteps = 0
+    t0 = time.perf_counter()
+    while True:
+        O.nve_atoms_initial(x, v, f, dt, dtf, m)
+        O.nve_nodes_initial(es.nodex, nodev, nodef, xc, dt, dtf, me)
+        f = fa(); nodef = fe()
+        O.nve_atoms_final(v, f, dtf, m)
+        O.nve_nodes_final(nodev, nodef, dtf, me)
+        nsteps += 1
+        if time.perf_counter() - t0 >= seconds:
+            break
+    t_step = (time.perf_counter() - t0) / nsteps
+    per_step = t_step + rebuild_rate * t_build
+    return dict(per_step=per_step, steps=nsteps, t_build=t_build, t_step=t_step, cores=O.num_threads())
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--workload", choices=["eam_c1", "lj_c0", "small"], default="eam_c1")
+    ap.add_argument("--seed", type=int, default=4928459)
+    ap.add_argument("--every", type=int, default=1)
+    ap.add_argument("--delay", type=int, default=10)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-variants", action="store_true", help="skip the mixed-precision line item")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU path.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from cac_code_amd import _lib
+    t0 = time.perf_counter()
+    s = make_workload(args.workload, args.seed)
+    pair, cutneigh, skin, ff = make_pair(args.workload)
+    log(f"[bench] workload {args.workload}: atoms={s.n_atoms} elements={s.n_elements} "
+        f"(built in {time.perf_counter() - t0:.1f}s)")
+    prec = _lib.PREC_FP64 if args.precision == "fp64" else _lib.PREC_MIXED
+    eng, secs, nbuilds, nbw, prof = gpu_run(s, pair, cutneigh, skin, args.steps, args.warmup, prec, args.every,
+                                            args.delay)
+    eng.close()
+    n_int = s.n_atoms + 8 * s.n_elements
+    n_rep = s.n_atoms + s.nae[0] * s.nae[1] * s.nae[2] * s.n_elements
+    value = n_int * args.steps / secs / 1e6
+    log(f"[bench] {args.precision}: {secs * 1e3 / args.steps:.3f} ms/step, {value:.1f} Matom-steps/s, "
+        f"{nbuilds} rebuilds in {args.steps} steps")
+    for k, (ms, c) in prof.items():
+        log(f"[bench]   {k:12s} {ms * 1e3:9.1f} us x {c}")
+
+    # dominant kernel by total time among those with an algorithmic byte count
+    dom = max((k for k in prof if k in ALG_BYTES), key=lambda k: prof[k][0] * prof[k][1], default=None)
+    roof = None
+    if dom:
+        unit, per = ALG_BYTES[dom]
+        units = s.n_atoms if unit == "atoms" else s.n_elements
+        ms = prof[dom][0]
+        ach = units * per / (ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "alg_bytes_per_launch": units * per, "avg_launch_us": round(ms * 1e3, 2)}
+    b_step = B_ATOM * s.n_atoms + B_ELEM * s.n_elements
+    step_roof = {"alg_bytes_per_step": b_step,
+                 "achieved_GBs": round(b_step * args.steps / secs / 1e9, 1),
+                 "frac": round(b_step * args.steps / secs / 1e9 / HBM_PEAK_GBS, 4)}
+
+    variants = {}
+    if not args.no_variants and args.precision == "fp64":
+        eng2, secs2, nb2, _, _ = gpu_run(s, pair, cutneigh, skin, args.steps, args.warmup, _lib.PREC_MIXED,
+                                         args.every, args.delay)
+        eng2.close()
+        variants["mixed"] = {"value": round(n_int * args.steps / secs2 / 1e6, 2),
+                             "ms_per_step": round(secs2 * 1e3 / args.steps, 4), "rebuilds": nb2}
+        log(f"[bench] mixed: {secs2 * 1e3 / args.steps:.3f} ms/step")
+
+    cpu = None
+    if not args.no_cpu:
+        rate = nbuilds / max(args.steps, 1)
+        log(f"[bench] cpu baseline (oracle), {args.cpu_seconds:.0f}s sample ...")
+        c = cpu_baseline(s, pair, ff, cutneigh, args.cpu_seconds, rate)
+        cpu = {"value": round(n_int / c["per_step"] / 1e6, 4), "unit": "Matom-steps/s", "cores": c["cores"],
+               "kind": "port",
+               "sample": f"{c['steps']} full FP64 steps of the same system on the oracle "
+                         f"({c['t_step']:.3f} s/step) plus one neighbor build ({c['t_build']:.2f} s) "
+                         f"charged at the GPU rebuild rate {rate:.4f}/step"}
+        log(f"[bench] cpu: {c['per_step']:.3f} s/step on {c['cores']} threads")
+
+    out = {
+        "metric": "Matom-steps/s on 1 MI355X (integrated particles: atoms + 8 x elements)",
+        "value": round(value, 2),
+        "unit": "Matom-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(secs * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32-pair/f64-accum",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "atoms": s.n_atoms, "elements": s.n_elements,
+                   "nae": list(s.nae), "integration_points": int(s.intp_w.size), "box_cells": s.cells,
+                   "represented_atoms": n_rep, "rebuilds": nbuilds, "neighbor": [cutneigh, skin],
+                   "neigh_modify": {"every": args.every, "delay": args.delay, "check": "yes"},
+                   "parallelism": "single GPU"},
+        "represented_Matom_steps_per_s": round(n_rep * args.steps / secs / 1e6, 1),
+        "roofline": roof,
+        "step_roofline": step_roof,
+        "variants": variants,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,97 @@
+"""ctypes binding of libcac_hip.so (the C-ABI in include/cac_hip.h).
+
+Loading fails loudly when the library is missing: there is no CPU fallback on
+the product path.  Build it with ``python -m cac_code_amd.build``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcac_hip.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "cac_hip.h")
+
+CAC_OK, CAC_ERR_ARG, CAC_ERR_HIP, CAC_ERR_OVERFLOW, CAC_ERR_BOX, CAC_ERR_DATA = range(6)
+PREC_FP64, PREC_MIXED = 0, 1
+
+_d = C.POINTER(C.c_double)
+_i = C.POINTER(C.c_int)
+_u32 = C.POINTER(C.c_uint32)
+_u64 = C.POINTER(C.c_uint64)
+_f = C.POINTER(C.c_float)
+_ctx = C.c_void_p
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "cac_ctx_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "cac_ctx_destroy": (C.c_int, [_ctx]),
+    "cac_last_error": (C.c_char_p, [_ctx]),
+    "cac_synchronize": (C.c_int, [_ctx]),
+    "cac_set_box": (C.c_int, [_ctx, _d, _d, _i]),
+    "cac_upload_atoms": (C.c_int, [_ctx, C.c_int, _d, _d, C.c_double]),
+    "cac_upload_elements": (C.c_int, [_ctx, C.c_int, _i, _d, _d, C.c_double]),
+    "cac_set_quadrature": (C.c_int, [_ctx, C.c_int, _i, _d]),
+    "cac_eam_tables_from_funcfl": (C.c_int, [C.c_int, C.c_double, C.c_int, C.c_double, _d, _d, _d,
+                                             _i, _d, _i, _d, _d, _d, _d]),
+    "cac_set_eam": (C.c_int, [_ctx, C.c_int, C.c_double, _d, C.c_int, C.c_double, _d, _d, C.c_double]),
+    "cac_set_lj": (C.c_int, [_ctx, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double]),
+    "cac_set_neighbor": (C.c_int, [_ctx, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "cac_build_neighbors": (C.c_int, [_ctx]),
+    "cac_decide": (C.c_int, [_ctx, _i]),
+    "cac_force": (C.c_int, [_ctx, C.c_int, C.c_int, C.c_int]),
+    "cac_nve_initial": (C.c_int, [_ctx, C.c_double, C.c_double]),
+    "cac_nve_final": (C.c_int, [_ctx, C.c_double]),
+    "cac_run": (C.c_int, [_ctx, C.c_int, C.c_double, C.c_double, C.c_int, _i]),
+    "cac_download_atoms": (C.c_int, [_ctx, _d, _d, _d]),
+    "cac_download_elements": (C.c_int, [_ctx, _d, _d, _d]),
+    "cac_get_energy_virial": (C.c_int, [_ctx, _d, _d]),
+    "cac_get_capacities": (C.c_int, [_ctx, _i, _i]),
+    "cac_get_atom_neighbors": (C.c_int, [_ctx, C.c_int, _i, _u32]),
+    "cac_get_ip_neighbors": (C.c_int, [_ctx, C.c_int, _i, _u64]),
+    "cac_get_rho": (C.c_int, [_ctx, _d, _d]),
+    "cac_get_erho": (C.c_int, [_ctx, _d]),
+    "cac_get_ip_forces": (C.c_int, [_ctx, _d]),
+    "cac_profile_enable": (C.c_int, [_ctx, C.c_int]),
+    "cac_profile_avg_ms": (C.c_double, [_ctx, C.c_char_p, _i]),
+    # learned element surrogate
+    "cac_mlp_create": (C.c_int, [_ctx, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64]),
+    "cac_mlp_set_weights": (C.c_int, [_ctx, C.c_int, _f, _f]),
+    "cac_mlp_get_weights": (C.c_int, [_ctx, C.c_int, _f, _f]),
+    "cac_mlp_forward": (C.c_int, [_ctx, C.c_int, _f, _f]),
+    "cac_mlp_train_step": (C.c_int, [_ctx, C.c_int, _f, _f, C.c_float, _f]),
+    "cac_mlp_grads": (C.c_int, [_ctx, C.c_int, _f, _f, _f, C.POINTER(_f)]),
+}
+
+
+class CacError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"CAC status {code}: {msg}")
+        self.code = code
+
+
+def header_symbols():
+    """Function names declared in include/cac_hip.h."""
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|double|const char \*)\s*\**\s*(cac_\w+)\s*\(", text, re.M)))
+
+
+_lib = None
+
+
+def lib():
+    """Load libcac_hip.so; raise if it is not built (no fallback path)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run `python -m cac_code_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name, None)
+            if fn is None:
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib

@@ -1,0 +1,766 @@
+// cac_api.hip -- C-ABI entry points of libcac_hip.so: context, uploads,
+// FixNVE kernels, energy/virial reduction and the device-resident Verlet
+// loop.  The force and neighbor kernels live in k_pair.hip and
+// k_neighbor.hip.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "cac_internal.h"
+
+namespace cac {
+
+int fail(cac_ctx *ctx, int code, const std::string &msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+int hip_fail(cac_ctx *ctx, hipError_t e, const char *where) {
+  char buf[512];
+  snprintf(buf, sizeof(buf), "HIP error %d (%s) at %s", (int)e, hipGetErrorString(e), where);
+  if (ctx) ctx->err = buf;
+  return CAC_ERR_HIP;
+}
+
+void timer_begin(cac_ctx *ctx, const char *name, hipEvent_t *start) {
+  *start = nullptr;
+  if (!ctx->profiling) return;
+  if (hipEventCreate(start) != hipSuccess) { *start = nullptr; return; }
+  hipEventRecord(*start, ctx->stream);
+  (void)name;
+}
+
+void timer_end(cac_ctx *ctx, const char *name, hipEvent_t start) {
+  if (!ctx->profiling || !start) return;
+  hipEvent_t stop;
+  if (hipEventCreate(&stop) != hipSuccess) return;
+  hipEventRecord(stop, ctx->stream);
+  ctx->timers[name].pending.emplace_back(start, stop);
+}
+
+namespace {
+
+constexpr int kBlock = 256;
+inline unsigned grid_for(long n, int block = kBlock) { return (unsigned)((n + block - 1) / block); }
+
+// ---------------------------------------------------------------- FixNVE
+// FixNVE::initial_integrate, fix_nve.cpp:54-98: v += dtfm*f; x += dtv*v.
+__global__ void k_nve_atoms_initial(int n, double *__restrict__ x4, double *__restrict__ v,
+                                    const double *__restrict__ f, double dtv, double dtfm) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int d = 0; d < 3; ++d) {
+    double vv = v[(size_t)d * n + i] + dtfm * f[(size_t)d * n + i];
+    v[(size_t)d * n + i] = vv;
+    x4[4 * i + d] += dtv * vv;
+  }
+}
+
+// FixNVE::final_integrate, fix_nve.cpp:181-212.
+__global__ void k_nve_atoms_final(int n, double *__restrict__ v, const double *__restrict__ f, double dtfm) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int d = 0; d < 3; ++d) v[(size_t)d * n + i] += dtfm * f[(size_t)d * n + i];
+}
+
+// FixNVE::einitial_integrate, fix_nve.cpp:100-157: nodes with the lumped
+// mass m_e/8, then the element center as the node mean.
+__global__ void k_nve_nodes_initial(int ne, double *__restrict__ nodex, double *__restrict__ nodev,
+                                    const double *__restrict__ nodef, double *__restrict__ xc, double dtv,
+                                    double dtfm) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ne) return;
+  const size_t b = 24 * (size_t)i;
+  double c[3] = {0.0, 0.0, 0.0};
+  for (int k = 0; k < 8; ++k)
+    for (int d = 0; d < 3; ++d) {
+      double vv = nodev[b + 3 * k + d] + dtfm * nodef[b + 3 * k + d];
+      nodev[b + 3 * k + d] = vv;
+      double xx = nodex[b + 3 * k + d] + dtv * vv;
+      nodex[b + 3 * k + d] = xx;
+    }
+  for (int k = 0; k < 8; ++k)
+    for (int d = 0; d < 3; ++d) c[d] += nodex[b + 3 * k + d];
+  for (int d = 0; d < 3; ++d) xc[3 * i + d] = c[d] / 8;
+}
+
+// FixNVE::efinal_integrate, fix_nve.cpp:216-242.
+__global__ void k_nve_nodes_final(long nq, double *__restrict__ nodev, const double *__restrict__ nodef,
+                                  double dtfm) {
+  long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < nq) nodev[q] += dtfm * nodef[q];
+}
+
+// ---------------------------------------------------------------- reductions
+// Deterministic sums: block b of the grid reduces array b with a fixed
+// stride pattern and a fixed LDS tree.
+struct RedArgs {
+  const double *src[14];
+  long len[14];
+};
+
+__global__ void k_reduce(RedArgs a, double *__restrict__ out) {
+  __shared__ double s[1024];
+  const int q = blockIdx.x;
+  const double *src = a.src[q];
+  const long len = a.len[q];
+  double acc = 0.0;
+  for (long i = threadIdx.x; i < len; i += blockDim.x) acc += src[i];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[q] = s[0];
+}
+
+int upload_tables(cac_ctx *ctx, int nrho, double drho, const double *frho_spline, int nr, double dr,
+                  const double *rhor_spline, const double *z2r_spline, double cutforcesq) {
+  EamTables &t = ctx->eam;
+  std::vector<double> rz((size_t)(nr + 1) * 16, 0.0), fr((size_t)(nrho + 1) * 8, 0.0);
+  std::vector<float> rzf((size_t)(nr + 1) * 16, 0.0f), frf((size_t)(nrho + 1) * 8, 0.0f);
+  for (int m = 0; m <= nr; ++m)
+    for (int c = 0; c < 7; ++c) {
+      rz[16 * (size_t)m + c] = rhor_spline[7 * (size_t)m + c];
+      rz[16 * (size_t)m + 8 + c] = z2r_spline[7 * (size_t)m + c];
+      rzf[16 * (size_t)m + c] = (float)rhor_spline[7 * (size_t)m + c];
+      rzf[16 * (size_t)m + 8 + c] = (float)z2r_spline[7 * (size_t)m + c];
+    }
+  for (int m = 0; m <= nrho; ++m)
+    for (int c = 0; c < 7; ++c) {
+      fr[8 * (size_t)m + c] = frho_spline[7 * (size_t)m + c];
+      frf[8 * (size_t)m + c] = (float)frho_spline[7 * (size_t)m + c];
+    }
+  int rc;
+  if ((rc = ensure(ctx, t.b_rz, rz.size(), &t.rz)) || (rc = ensure(ctx, t.b_fr, fr.size(), &t.fr)) ||
+      (rc = ensure(ctx, t.b_rz_f, rzf.size(), &t.rz_f)) || (rc = ensure(ctx, t.b_fr_f, frf.size(), &t.fr_f)))
+    return rc;
+  CAC_HIP(ctx, hipMemcpy(t.rz, rz.data(), rz.size() * sizeof(double), hipMemcpyHostToDevice));
+  CAC_HIP(ctx, hipMemcpy(t.fr, fr.data(), fr.size() * sizeof(double), hipMemcpyHostToDevice));
+  CAC_HIP(ctx, hipMemcpy(t.rz_f, rzf.data(), rzf.size() * sizeof(float), hipMemcpyHostToDevice));
+  CAC_HIP(ctx, hipMemcpy(t.fr_f, frf.data(), frf.size() * sizeof(float), hipMemcpyHostToDevice));
+  t.nrho = nrho;
+  t.nr = nr;
+  t.rdrho = 1.0 / drho;
+  t.rdr = 1.0 / dr;
+  t.cutforcesq = cutforcesq;
+  t.rhomax = (nrho - 1) / t.rdrho;
+  return CAC_OK;
+}
+
+int read_flags(cac_ctx *ctx) {
+  CAC_HIP(ctx, hipMemcpyAsync(ctx->hflags, ctx->dflags, 8 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return CAC_OK;
+}
+
+int clear_flags(cac_ctx *ctx) {
+  CAC_HIP(ctx, hipMemsetAsync(ctx->dflags, 0, 8 * sizeof(int), ctx->stream));
+  return CAC_OK;
+}
+
+int ready_for_force(cac_ctx *ctx) {
+  if (!ctx->box_set) return fail(ctx, CAC_ERR_ARG, "cac_set_box not called");
+  if (ctx->pair == CAC_PAIR_NONE) return fail(ctx, CAC_ERR_ARG, "no pair style set");
+  if (ctx->ne > 0 && !ctx->quad_set) return fail(ctx, CAC_ERR_ARG, "cac_set_quadrature not called");
+  if (!ctx->built) return fail(ctx, CAC_ERR_ARG, "neighbor lists not built");
+  return CAC_OK;
+}
+
+int alloc_ev(cac_ctx *ctx, int eflag, int vflag) {
+  int rc;
+  if (eflag && ctx->n && (rc = ensure(ctx, ctx->b_eatom, (size_t)ctx->n, &ctx->eatom))) return rc;
+  if (vflag && ctx->n && (rc = ensure(ctx, ctx->b_vatom, 6 * (size_t)ctx->n, &ctx->vatom))) return rc;
+  const size_t nip = (size_t)ctx->ne * ctx->nint;
+  if (eflag && nip && (rc = ensure(ctx, ctx->b_eip, nip, &ctx->eip))) return rc;
+  if (vflag && nip && (rc = ensure(ctx, ctx->b_vip, 6 * nip, &ctx->vip))) return rc;
+  return CAC_OK;
+}
+
+int force_impl(cac_ctx *ctx, int eflag, int vflag, int precision) {
+  int rc;
+  if ((rc = alloc_ev(ctx, eflag, vflag))) return rc;
+  if (ctx->pair == CAC_PAIR_EAM) {
+    if ((rc = launch_eam_atoms(ctx, eflag, vflag, precision))) return rc;
+    if ((rc = launch_eam_elements(ctx, eflag, vflag, precision))) return rc;
+  } else {
+    if ((rc = launch_lj_atoms(ctx, eflag, vflag, precision))) return rc;
+    if ((rc = launch_lj_elements(ctx, eflag, vflag, precision))) return rc;
+  }
+  ctx->have_ev = false;
+  if (eflag || vflag) return launch_reduce_ev(ctx, eflag, vflag);
+  return CAC_OK;
+}
+
+int build_impl(cac_ctx *ctx) {
+  int rc;
+  if (!ctx->neigh_set) return fail(ctx, CAC_ERR_ARG, "cac_set_neighbor not called");
+  if (!ctx->box_set) return fail(ctx, CAC_ERR_ARG, "cac_set_box not called");
+  if (ctx->ne > 0 && !ctx->quad_set) return fail(ctx, CAC_ERR_ARG, "cac_set_quadrature not called");
+  if ((rc = launch_pbc_and_centers(ctx))) return rc;
+  // regrow-and-retry on capacity overflow (the reference aborts instead,
+  // neighbor.cpp:619-620, element_neighbor.cpp:71-72)
+  for (int attempt = 0; attempt < 4; ++attempt) {
+    if ((rc = clear_flags(ctx))) return rc;
+    if ((rc = launch_atom_neighbors(ctx))) return rc;
+    if ((rc = launch_element_neighbors(ctx))) return rc;  // synchronizes
+    if ((rc = read_flags(ctx))) return rc;
+    bool again = false;
+    if (ctx->hflags[0]) { ctx->maxneigh = ctx->hflags[1] + 16; again = true; }
+    if (ctx->hflags[2]) { ctx->maxelist = ctx->hflags[3] + 16; again = true; }
+    if (again) continue;
+    if ((rc = launch_ip_neighbors(ctx))) return rc;
+    if ((rc = read_flags(ctx))) return rc;
+    if (ctx->hflags[4]) { ctx->maxnip = ctx->hflags[5] + 16; continue; }
+    ctx->built = true;
+    ctx->ago = 0;
+    ctx->ncalls++;
+    return CAC_OK;
+  }
+  return fail(ctx, CAC_ERR_OVERFLOW, "neighbor capacity still exceeded after regrowth");
+}
+
+int decide_impl(cac_ctx *ctx, int *rebuild) {
+  // Neighbor::decide, neighbor.cpp:1150-1163
+  ctx->ago++;
+  *rebuild = 0;
+  if (ctx->ago >= ctx->delay && ctx->ago % ctx->every == 0) {
+    if (!ctx->check) { *rebuild = 1; return CAC_OK; }
+    int rc;
+    if ((rc = clear_flags(ctx))) return rc;
+    if ((rc = launch_check_displacement(ctx))) return rc;
+    if ((rc = read_flags(ctx))) return rc;
+    *rebuild = ctx->hflags[6] ? 1 : 0;
+  }
+  return CAC_OK;
+}
+
+int nve_initial_impl(cac_ctx *ctx, double dtv, double dtf) {
+  if (ctx->n)
+    hipLaunchKernelGGL(k_nve_atoms_initial, dim3(grid_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->n, ctx->x4,
+                       ctx->v, ctx->f, dtv, dtf / ctx->mass);
+  if (ctx->ne) {
+    double inv_npe = 1 / (double)8;
+    hipLaunchKernelGGL(k_nve_nodes_initial, dim3(grid_for(ctx->ne)), dim3(kBlock), 0, ctx->stream, ctx->ne,
+                       ctx->nodex, ctx->nodev, ctx->nodef, ctx->xc, dtv, dtf / (inv_npe * ctx->mass_e));
+  }
+  CAC_HIP(ctx, hipGetLastError());
+  return CAC_OK;
+}
+
+int nve_final_impl(cac_ctx *ctx, double dtf) {
+  if (ctx->n)
+    hipLaunchKernelGGL(k_nve_atoms_final, dim3(grid_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->n, ctx->v,
+                       ctx->f, dtf / ctx->mass);
+  if (ctx->ne) {
+    double inv_npe = 1 / (double)8;
+    long nq = 24L * ctx->ne;
+    hipLaunchKernelGGL(k_nve_nodes_final, dim3(grid_for(nq)), dim3(kBlock), 0, ctx->stream, nq, ctx->nodev,
+                       ctx->nodef, dtf / (inv_npe * ctx->mass_e));
+  }
+  CAC_HIP(ctx, hipGetLastError());
+  return CAC_OK;
+}
+
+}  // namespace
+
+int launch_reduce_ev(cac_ctx *ctx, int eflag, int vflag) {
+  RedArgs a;
+  int nq = 0;
+  const long n = ctx->n, nip = (long)ctx->ne * ctx->nint;
+  // slots: 0 atom energy, 1 IP energy, 2..7 atom virial, 8..13 IP virial
+  for (int q = 0; q < 14; ++q) {
+    a.src[q] = ctx->x4;
+    a.len[q] = 0;
+  }
+  if (eflag) {
+    if (n) { a.src[0] = ctx->eatom; a.len[0] = n; }
+    if (nip) { a.src[1] = ctx->eip; a.len[1] = nip; }
+  }
+  if (vflag)
+    for (int c = 0; c < 6; ++c) {
+      if (n) { a.src[2 + c] = ctx->vatom + c * n; a.len[2 + c] = n; }
+      if (nip) { a.src[8 + c] = ctx->vip + c * nip; a.len[8 + c] = nip; }
+    }
+  nq = 14;
+  int rc;
+  if ((rc = ensure(ctx, ctx->b_red, 64 * 16, &ctx->red))) return rc;
+  hipLaunchKernelGGL(k_reduce, dim3(nq), dim3(1024), 0, ctx->stream, a, ctx->red + 16);
+  double h[14];
+  CAC_HIP(ctx, hipMemcpyAsync(h, ctx->red + 16, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+  CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->eng[0] = eflag ? h[0] : 0.0;
+  ctx->eng[1] = eflag ? h[1] : 0.0;
+  for (int c = 0; c < 6; ++c) ctx->virial[c] = vflag ? h[2 + c] + h[8 + c] : 0.0;
+  ctx->have_ev = true;
+  return CAC_OK;
+}
+
+}  // namespace cac
+
+using namespace cac;
+
+// ============================================================== C-ABI
+extern "C" {
+
+int cac_ctx_create(int device, cac_ctx **out) {
+  if (!out) return CAC_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return CAC_ERR_HIP;
+  if (device < 0 || device >= ndev) return CAC_ERR_ARG;
+  if (hipSetDevice(device) != hipSuccess) return CAC_ERR_HIP;
+  cac_ctx *ctx = new cac_ctx();
+  ctx->device = device;
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return CAC_ERR_HIP;
+  }
+  if (ensure(ctx, ctx->b_dflags, 8, &ctx->dflags) ||
+      hipHostMalloc((void **)&ctx->hflags, 8 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+    cac_ctx_destroy(ctx);
+    return CAC_ERR_HIP;
+  }
+  *out = ctx;
+  return CAC_OK;
+}
+
+int cac_ctx_destroy(cac_ctx *ctx) {
+  if (!ctx) return CAC_OK;
+  hipSetDevice(ctx->device);
+  if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  DevBuf *bufs[] = {&ctx->b_x4, &ctx->b_v, &ctx->b_f, &ctx->b_xhold, &ctx->b_rho, &ctx->b_eatom, &ctx->b_vatom,
+                    &ctx->b_aneigh, &ctx->b_anum, &ctx->b_akey, &ctx->b_aval, &ctx->b_akey2, &ctx->b_aval2,
+                    &ctx->b_acs, &ctx->b_ace, &ctx->b_sorttmp, &ctx->b_nodex, &ctx->b_nodev, &ctx->b_nodef,
+                    &ctx->b_nodexhold, &ctx->b_xc, &ctx->b_spa_site, &ctx->b_spa_int, &ctx->b_weight, &ctx->b_sbc,
+                    &ctx->b_erad, &ctx->b_eneigh, &ctx->b_enumb, &ctx->b_ekey, &ctx->b_eval, &ctx->b_ekey2,
+                    &ctx->b_eval2, &ctx->b_ecs, &ctx->b_ece, &ctx->b_ipneigh, &ctx->b_ipnum, &ctx->b_erho,
+                    &ctx->b_ipf, &ctx->b_eip, &ctx->b_vip, &ctx->b_red, &ctx->b_dflags, &ctx->eam.b_rz,
+                    &ctx->eam.b_fr, &ctx->eam.b_rz_f, &ctx->eam.b_fr_f};
+  for (DevBuf *b : bufs)
+    if (b->p) hipFree(b->p);
+  if (ctx->hflags) hipHostFree(ctx->hflags);
+  for (auto &kv : ctx->timers)
+    for (auto &pr : kv.second.pending) {
+      hipEventDestroy(pr.first);
+      hipEventDestroy(pr.second);
+    }
+  if (ctx->stream) hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return CAC_OK;
+}
+
+const char *cac_last_error(const cac_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int cac_synchronize(cac_ctx *ctx) {
+  if (!ctx) return CAC_ERR_ARG;
+  CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return CAC_OK;
+}
+
+int cac_set_box(cac_ctx *ctx, const double lo[3], const double hi[3], const int periodic[3]) {
+  if (!ctx || !lo || !hi) return CAC_ERR_ARG;
+  for (int d = 0; d < 3; ++d) {
+    if (periodic && !periodic[d]) return fail(ctx, CAC_ERR_ARG, "only fully periodic boxes are supported");
+    if (!(hi[d] > lo[d])) return fail(ctx, CAC_ERR_ARG, "box hi must exceed lo");
+    ctx->lo[d] = lo[d];
+    ctx->hi[d] = hi[d];
+    ctx->prd[d] = hi[d] - lo[d];
+  }
+  ctx->box_set = true;
+  ctx->built = false;
+  return CAC_OK;
+}
+
+int cac_upload_atoms(cac_ctx *ctx, int n, const double *x, const double *v, double mass) {
+  if (!ctx || n < 0 || (n > 0 && !x) || !(mass > 0)) return CAC_ERR_ARG;
+  if (n >= (1 << 27)) return fail(ctx, CAC_ERR_ARG, "at most 2^27-1 atoms");
+  hipSetDevice(ctx->device);
+  ctx->n = n;
+  ctx->mass = mass;
+  int rc;
+  if ((rc = ensure(ctx, ctx->b_x4, 4 * (size_t)n, &ctx->x4)) || (rc = ensure(ctx, ctx->b_v, 3 * (size_t)n, &ctx->v)) ||
+      (rc = ensure(ctx, ctx->b_f, 3 * (size_t)n, &ctx->f)) || (rc = ensure(ctx, ctx->b_rho, (size_t)n, &ctx->rho)) ||
+      (rc = ensure(ctx, ctx->b_xhold, 3 * (size_t)n, &ctx->xhold)))
+    return rc;
+  std::vector<double> x4((size_t)4 * n), vs((size_t)3 * n, 0.0);
+  for (int i = 0; i < n; ++i) {
+    for (int d = 0; d < 3; ++d) {
+      x4[4 * (size_t)i + d] = x[3 * (size_t)i + d];
+      if (v) vs[(size_t)d * n + i] = v[3 * (size_t)i + d];
+    }
+    x4[4 * (size_t)i + 3] = 0.0;
+  }
+  if (n) {
+    CAC_HIP(ctx, hipMemcpy(ctx->x4, x4.data(), x4.size() * sizeof(double), hipMemcpyHostToDevice));
+    CAC_HIP(ctx, hipMemcpy(ctx->v, vs.data(), vs.size() * sizeof(double), hipMemcpyHostToDevice));
+    CAC_HIP(ctx, hipMemset(ctx->f, 0, 3 * (size_t)n * sizeof(double)));
+  }
+  ctx->built = false;
+  return CAC_OK;
+}
+
+int cac_upload_elements(cac_ctx *ctx, int ne, const int nae[3], const double *nodex, const double *nodev,
+                        double mass_e) {
+  if (!ctx || ne < 0 || !nae || (ne > 0 && !nodex) || !(mass_e > 0)) return CAC_ERR_ARG;
+  if (ne >= (1 << 27)) return fail(ctx, CAC_ERR_ARG, "at most 2^27-1 elements");
+  for (int d = 0; d < 3; ++d)
+    if (nae[d] < 2 || nae[d] > kSubBlock * 8 * 4)
+      return fail(ctx, CAC_ERR_ARG, "nae must be in [2, 128] per edge");  // element.cpp:1104-1107
+  if ((long)nae[0] * nae[1] * nae[2] >= 65536) return fail(ctx, CAC_ERR_ARG, "at most 65535 sites per element");
+  {
+    int nsb = 1;
+    for (int d = 0; d < 3; ++d) nsb *= (nae[d] + kSubBlock - 1) / kSubBlock;
+    if (nsb > kMaxSubBlocks) return fail(ctx, CAC_ERR_ARG, "element too large (sub-block limit)");
+  }
+  hipSetDevice(ctx->device);
+  const bool new_type = (nae[0] != ctx->nae[0] || nae[1] != ctx->nae[1] || nae[2] != ctx->nae[2]);
+  ctx->ne = ne;
+  for (int d = 0; d < 3; ++d) ctx->nae[d] = nae[d];
+  ctx->nsite = nae[0] * nae[1] * nae[2];
+  ctx->mass_e = mass_e;
+  int rc;
+  const size_t nq = 24 * (size_t)ne;
+  if ((rc = ensure(ctx, ctx->b_nodex, nq, &ctx->nodex)) || (rc = ensure(ctx, ctx->b_nodev, nq, &ctx->nodev)) ||
+      (rc = ensure(ctx, ctx->b_nodef, nq, &ctx->nodef)) || (rc = ensure(ctx, ctx->b_nodexhold, nq, &ctx->nodexhold)) ||
+      (rc = ensure(ctx, ctx->b_xc, 3 * (size_t)ne, &ctx->xc)) || (rc = ensure(ctx, ctx->b_erho, 8 * (size_t)ne, &ctx->erho)))
+    return rc;
+  if (ne) {
+    CAC_HIP(ctx, hipMemcpy(ctx->nodex, nodex, nq * sizeof(double), hipMemcpyHostToDevice));
+    if (nodev)
+      CAC_HIP(ctx, hipMemcpy(ctx->nodev, nodev, nq * sizeof(double), hipMemcpyHostToDevice));
+    else
+      CAC_HIP(ctx, hipMemset(ctx->nodev, 0, nq * sizeof(double)));
+    CAC_HIP(ctx, hipMemset(ctx->nodef, 0, nq * sizeof(double)));
+  }
+  // site shape functions, Element::inter_ele + shape_array_inter_ele
+  // (element.cpp:1361-1423): parent coordinate 2/(n-1)*i - 1, x outermost.
+  {
+    std::vector<double> spa((size_t)ctx->nsite * 8);
+    const double inx = 2 / ((double)nae[0] - 1), iny = 2 / ((double)nae[1] - 1), inz = 2 / ((double)nae[2] - 1);
+    size_t s = 0;
+    for (int ii = 0; ii < nae[0]; ++ii)
+      for (int jj = 0; jj < nae[1]; ++jj)
+        for (int kk = 0; kk < nae[2]; ++kk, ++s) {
+          const double px = inx * (double)ii - 1, py = iny * (double)jj - 1, pz = inz * (double)kk - 1;
+          double *N = &spa[8 * s];
+          N[0] = 0.125 * (1 - px) * (1 - py) * (1 - pz);
+          N[1] = 0.125 * (1 + px) * (1 - py) * (1 - pz);
+          N[2] = 0.125 * (1 + px) * (1 + py) * (1 - pz);
+          N[3] = 0.125 * (1 - px) * (1 + py) * (1 - pz);
+          N[4] = 0.125 * (1 - px) * (1 - py) * (1 + pz);
+          N[5] = 0.125 * (1 + px) * (1 - py) * (1 + pz);
+          N[6] = 0.125 * (1 + px) * (1 + py) * (1 + pz);
+          N[7] = 0.125 * (1 - px) * (1 + py) * (1 + pz);
+        }
+    if ((rc = ensure(ctx, ctx->b_spa_site, spa.size(), &ctx->spa_site))) return rc;
+    CAC_HIP(ctx, hipMemcpy(ctx->spa_site, spa.data(), spa.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  if (new_type) ctx->quad_set = false;
+  ctx->built = false;
+  return CAC_OK;
+}
+
+int cac_set_quadrature(cac_ctx *ctx, int nint, const int *ijk, const double *w) {
+  if (!ctx || nint <= 0 || nint > kMaxNint || !ijk || !w) return CAC_ERR_ARG;
+  if (ctx->nae[0] < 2) return fail(ctx, CAC_ERR_ARG, "upload elements before the quadrature");
+  hipSetDevice(ctx->device);
+  // Element::intp_pcoord (element.cpp:1170-1191) + shape_array_int
+  // (element.cpp:1326-1354)
+  std::vector<double> spa((size_t)nint * 8);
+  const double dx = 2 / ((double)ctx->nae[0] - 1), dy = 2 / ((double)ctx->nae[1] - 1),
+               dz = 2 / ((double)ctx->nae[2] - 1);
+  for (int g = 0; g < nint; ++g) {
+    for (int d = 0; d < 3; ++d)
+      if (ijk[3 * g + d] < 0 || ijk[3 * g + d] >= ctx->nae[d])
+        return fail(ctx, CAC_ERR_DATA, "integration point lattice index outside the element");
+    const double px = dx * ((double)ijk[3 * g]) - 1, py = dy * ((double)ijk[3 * g + 1]) - 1,
+                 pz = dz * ((double)ijk[3 * g + 2]) - 1;
+    double *N = &spa[8 * (size_t)g];
+    N[0] = 0.125 * (1 - px) * (1 - py) * (1 - pz);
+    N[1] = 0.125 * (1 + px) * (1 - py) * (1 - pz);
+    N[2] = 0.125 * (1 + px) * (1 + py) * (1 - pz);
+    N[3] = 0.125 * (1 - px) * (1 + py) * (1 - pz);
+    N[4] = 0.125 * (1 - px) * (1 - py) * (1 + pz);
+    N[5] = 0.125 * (1 + px) * (1 - py) * (1 + pz);
+    N[6] = 0.125 * (1 + px) * (1 + py) * (1 + pz);
+    N[7] = 0.125 * (1 - px) * (1 + py) * (1 + pz);
+  }
+  // Element::find_node_list, element.cpp:1296-1319 (TOL 0.01)
+  for (int k = 0; k < 8; ++k) {
+    int g = 0;
+    for (; g < nint; ++g)
+      if (std::fabs(spa[8 * (size_t)g + k] - 1.0) < 0.01) break;
+    if (g == nint) return fail(ctx, CAC_ERR_DATA, "didn't contain all node integration point");
+    ctx->node_list[k] = g;
+  }
+  int rc;
+  if ((rc = ensure(ctx, ctx->b_spa_int, spa.size(), &ctx->spa_int)) ||
+      (rc = ensure(ctx, ctx->b_weight, (size_t)nint, &ctx->weight)))
+    return rc;
+  CAC_HIP(ctx, hipMemcpy(ctx->spa_int, spa.data(), spa.size() * sizeof(double), hipMemcpyHostToDevice));
+  CAC_HIP(ctx, hipMemcpy(ctx->weight, w, nint * sizeof(double), hipMemcpyHostToDevice));
+  ctx->h_spa_int = spa;
+  ctx->h_weight.assign(w, w + nint);
+  ctx->nint = nint;
+  const size_t nip = (size_t)ctx->ne * nint;
+  if ((rc = ensure(ctx, ctx->b_ipf, 3 * nip, &ctx->ipf))) return rc;
+  ctx->quad_set = true;
+  ctx->built = false;
+  return CAC_OK;
+}
+
+int cac_set_eam(cac_ctx *ctx, int nrho, double drho, const double *frho_spline, int nr, double dr,
+                const double *rhor_spline, const double *z2r_spline, double cutforcesq) {
+  if (!ctx || nrho < 3 || nr < 3 || !(drho > 0) || !(dr > 0) || !frho_spline || !rhor_spline || !z2r_spline ||
+      !(cutforcesq > 0))
+    return CAC_ERR_ARG;
+  hipSetDevice(ctx->device);
+  int rc = upload_tables(ctx, nrho, drho, frho_spline, nr, dr, rhor_spline, z2r_spline, cutforcesq);
+  if (rc) return rc;
+  ctx->pair = CAC_PAIR_EAM;
+  return CAC_OK;
+}
+
+int cac_set_lj(cac_ctx *ctx, double lj1, double lj2, double lj3, double lj4, double cutsq) {
+  if (!ctx || !(cutsq > 0)) return CAC_ERR_ARG;
+  ctx->lj.lj1 = lj1;
+  ctx->lj.lj2 = lj2;
+  ctx->lj.lj3 = lj3;
+  ctx->lj.lj4 = lj4;
+  ctx->lj.cutsq = cutsq;
+  ctx->pair = CAC_PAIR_LJ;
+  return CAC_OK;
+}
+
+int cac_set_neighbor(cac_ctx *ctx, double cutneigh, double triggersq, int every, int delay, int check, int maxneigh,
+                     int maxnip) {
+  if (!ctx || !(cutneigh > 0) || triggersq < 0 || every <= 0 || delay < 0) return CAC_ERR_ARG;
+  if (delay > 0 && delay % every != 0)
+    return fail(ctx, CAC_ERR_ARG, "Neighbor delay must be 0 or multiple of every setting");  // neighbor.cpp:309
+  ctx->cutneigh = cutneigh;
+  ctx->triggersq = triggersq;
+  ctx->every = every;
+  ctx->delay = delay;
+  ctx->check = check ? 1 : 0;
+  if (maxneigh > 0) ctx->maxneigh = maxneigh;
+  if (maxnip > 0) ctx->maxnip = maxnip;
+  ctx->neigh_set = true;
+  ctx->built = false;
+  return CAC_OK;
+}
+
+int cac_build_neighbors(cac_ctx *ctx) {
+  if (!ctx) return CAC_ERR_ARG;
+  hipSetDevice(ctx->device);
+  return build_impl(ctx);
+}
+
+int cac_decide(cac_ctx *ctx, int *rebuild) {
+  if (!ctx || !rebuild) return CAC_ERR_ARG;
+  if (!ctx->built) return fail(ctx, CAC_ERR_ARG, "neighbor lists not built");
+  hipSetDevice(ctx->device);
+  return decide_impl(ctx, rebuild);
+}
+
+int cac_force(cac_ctx *ctx, int eflag, int vflag, int precision) {
+  if (!ctx || (precision != CAC_PREC_FP64 && precision != CAC_PREC_MIXED)) return CAC_ERR_ARG;
+  hipSetDevice(ctx->device);
+  int rc = ready_for_force(ctx);
+  if (rc) return rc;
+  return force_impl(ctx, eflag, vflag, precision);
+}
+
+int cac_nve_initial(cac_ctx *ctx, double dtv, double dtf) {
+  if (!ctx) return CAC_ERR_ARG;
+  hipSetDevice(ctx->device);
+  return nve_initial_impl(ctx, dtv, dtf);
+}
+
+int cac_nve_final(cac_ctx *ctx, double dtf) {
+  if (!ctx) return CAC_ERR_ARG;
+  hipSetDevice(ctx->device);
+  return nve_final_impl(ctx, dtf);
+}
+
+int cac_run(cac_ctx *ctx, int nsteps, double dtv, double dtf, int precision, int *nbuilds) {
+  if (!ctx || nsteps < 0 || (precision != CAC_PREC_FP64 && precision != CAC_PREC_MIXED)) return CAC_ERR_ARG;
+  hipSetDevice(ctx->device);
+  int rc = ready_for_force(ctx);
+  if (rc) return rc;
+  int builds = 0;
+  // Verlet::run, verlet.cpp:149-219
+  for (int step = 0; step < nsteps; ++step) {
+    if ((rc = nve_initial_impl(ctx, dtv, dtf))) return rc;
+    int rebuild = 0;
+    if ((rc = decide_impl(ctx, &rebuild))) return rc;
+    if (rebuild) {
+      if ((rc = build_impl(ctx))) return rc;
+      ++builds;
+    }
+    if ((rc = force_impl(ctx, 0, 0, precision))) return rc;
+    if ((rc = nve_final_impl(ctx, dtf))) return rc;
+  }
+  if (nbuilds) *nbuilds = builds;
+  return CAC_OK;
+}
+
+int cac_download_atoms(cac_ctx *ctx, double *x, double *v, double *f) {
+  if (!ctx) return CAC_ERR_ARG;
+  hipSetDevice(ctx->device);
+  const int n = ctx->n;
+  if (n == 0) return CAC_OK;
+  CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (x) {
+    std::vector<double> x4((size_t)4 * n);
+    CAC_HIP(ctx, hipMemcpy(x4.data(), ctx->x4, x4.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; ++i)
+      for (int d = 0; d < 3; ++d) x[3 * (size_t)i + d] = x4[4 * (size_t)i + d];
+  }
+  std::vector<double> soa((size_t)3 * n);
+  const double *srcs[2] = {ctx->v, ctx->f};
+  double *dsts[2] = {v, f};
+  for (int q = 0; q < 2; ++q) {
+    if (!dsts[q]) continue;
+    CAC_HIP(ctx, hipMemcpy(soa.data(), srcs[q], soa.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; ++i)
+      for (int d = 0; d < 3; ++d) dsts[q][3 * (size_t)i + d] = soa[(size_t)d * n + i];
+  }
+  return CAC_OK;
+}
+
+int cac_download_elements(cac_ctx *ctx, double *nodex, double *nodev, double *nodef) {
+  if (!ctx) return CAC_ERR_ARG;
+  hipSetDevice(ctx->device);
+  const size_t nq = 24 * (size_t)ctx->ne;
+  if (nq == 0) return CAC_OK;
+  CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (nodex) CAC_HIP(ctx, hipMemcpy(nodex, ctx->nodex, nq * sizeof(double), hipMemcpyDeviceToHost));
+  if (nodev) CAC_HIP(ctx, hipMemcpy(nodev, ctx->nodev, nq * sizeof(double), hipMemcpyDeviceToHost));
+  if (nodef) CAC_HIP(ctx, hipMemcpy(nodef, ctx->nodef, nq * sizeof(double), hipMemcpyDeviceToHost));
+  return CAC_OK;
+}
+
+int cac_get_energy_virial(cac_ctx *ctx, double eng[2], double virial[6]) {
+  if (!ctx) return CAC_ERR_ARG;
+  if (!ctx->have_ev) return fail(ctx, CAC_ERR_ARG, "no energy/virial tallied: call cac_force with eflag/vflag");
+  if (eng) { eng[0] = ctx->eng[0]; eng[1] = ctx->eng[1]; }
+  if (virial)
+    for (int c = 0; c < 6; ++c) virial[c] = ctx->virial[c];
+  return CAC_OK;
+}
+
+int cac_get_capacities(cac_ctx *ctx, int *maxneigh, int *maxnip) {
+  if (!ctx) return CAC_ERR_ARG;
+  if (maxneigh) *maxneigh = ctx->maxneigh;
+  if (maxnip) *maxnip = ctx->maxnip;
+  return CAC_OK;
+}
+
+int cac_get_atom_neighbors(cac_ctx *ctx, int maxneigh, int *numneigh, uint32_t *neigh) {
+  if (!ctx || !numneigh || !neigh) return CAC_ERR_ARG;
+  if (!ctx->built) return fail(ctx, CAC_ERR_ARG, "neighbor lists not built");
+  if (maxneigh < ctx->maxneigh) return fail(ctx, CAC_ERR_ARG, "capacity below the device list stride");
+  hipSetDevice(ctx->device);
+  const int n = ctx->n;
+  if (n == 0) return CAC_OK;
+  CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  std::vector<uint32_t> cm((size_t)ctx->maxneigh * n);
+  CAC_HIP(ctx, hipMemcpy(numneigh, ctx->anum, n * sizeof(int), hipMemcpyDeviceToHost));
+  CAC_HIP(ctx, hipMemcpy(cm.data(), ctx->aneigh, cm.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  for (int i = 0; i < n; ++i)
+    for (int k = 0; k < numneigh[i] && k < ctx->maxneigh; ++k)
+      neigh[(size_t)i * maxneigh + k] = cm[(size_t)k * n + i];
+  return CAC_OK;
+}
+
+int cac_get_ip_neighbors(cac_ctx *ctx, int maxnip, int *numneigh, uint64_t *neigh) {
+  if (!ctx || !numneigh || !neigh) return CAC_ERR_ARG;
+  if (!ctx->built) return fail(ctx, CAC_ERR_ARG, "neighbor lists not built");
+  if (maxnip < ctx->maxnip) return fail(ctx, CAC_ERR_ARG, "capacity below the device list stride");
+  hipSetDevice(ctx->device);
+  const size_t nip = (size_t)ctx->ne * ctx->nint;
+  if (nip == 0) return CAC_OK;
+  CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  std::vector<uint64_t> cm((size_t)ctx->maxnip * nip);
+  CAC_HIP(ctx, hipMemcpy(numneigh, ctx->ipnum, nip * sizeof(int), hipMemcpyDeviceToHost));
+  CAC_HIP(ctx, hipMemcpy(cm.data(), ctx->ipneigh, cm.size() * sizeof(uint64_t), hipMemcpyDeviceToHost));
+  for (size_t p = 0; p < nip; ++p)
+    for (int k = 0; k < numneigh[p] && k < ctx->maxnip; ++k) neigh[p * maxnip + k] = cm[(size_t)k * nip + p];
+  return CAC_OK;
+}
+
+int cac_get_rho(cac_ctx *ctx, double *rho, double *fp) {
+  if (!ctx) return CAC_ERR_ARG;
+  hipSetDevice(ctx->device);
+  const int n = ctx->n;
+  if (n == 0) return CAC_OK;
+  CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (rho) CAC_HIP(ctx, hipMemcpy(rho, ctx->rho, n * sizeof(double), hipMemcpyDeviceToHost));
+  if (fp) {
+    std::vector<double> x4((size_t)4 * n);
+    CAC_HIP(ctx, hipMemcpy(x4.data(), ctx->x4, x4.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; ++i) fp[i] = x4[4 * (size_t)i + 3];
+  }
+  return CAC_OK;
+}
+
+int cac_get_erho(cac_ctx *ctx, double *erho) {
+  if (!ctx || !erho) return CAC_ERR_ARG;
+  hipSetDevice(ctx->device);
+  if (ctx->ne == 0) return CAC_OK;
+  CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  CAC_HIP(ctx, hipMemcpy(erho, ctx->erho, 8 * (size_t)ctx->ne * sizeof(double), hipMemcpyDeviceToHost));
+  return CAC_OK;
+}
+
+int cac_get_ip_forces(cac_ctx *ctx, double *ipf) {
+  if (!ctx || !ipf) return CAC_ERR_ARG;
+  hipSetDevice(ctx->device);
+  const size_t nip = (size_t)ctx->ne * ctx->nint;
+  if (nip == 0) return CAC_OK;
+  CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  std::vector<double> soa(3 * nip);
+  CAC_HIP(ctx, hipMemcpy(soa.data(), ctx->ipf, soa.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (size_t p = 0; p < nip; ++p)
+    for (int d = 0; d < 3; ++d) ipf[3 * p + d] = soa[d * nip + p];
+  return CAC_OK;
+}
+
+int cac_profile_enable(cac_ctx *ctx, int on) {
+  if (!ctx) return CAC_ERR_ARG;
+  hipStreamSynchronize(ctx->stream);
+  for (auto &kv : ctx->timers) {
+    for (auto &pr : kv.second.pending) {
+      hipEventDestroy(pr.first);
+      hipEventDestroy(pr.second);
+    }
+  }
+  ctx->timers.clear();
+  ctx->profiling = on != 0;
+  return CAC_OK;
+}
+
+double cac_profile_avg_ms(cac_ctx *ctx, const char *name, int *count) {
+  if (count) *count = 0;
+  if (!ctx || !name) return -1.0;
+  auto it = ctx->timers.find(name);
+  if (it == ctx->timers.end()) return -1.0;
+  Timer &t = it->second;
+  hipStreamSynchronize(ctx->stream);
+  for (auto &pr : t.pending) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) {
+      t.total_ms += ms;
+      t.count++;
+    }
+    hipEventDestroy(pr.first);
+    hipEventDestroy(pr.second);
+  }
+  t.pending.clear();
+  if (count) *count = t.count;
+  return t.count ? t.total_ms / t.count : -1.0;
+}
+
+}  // extern "C"

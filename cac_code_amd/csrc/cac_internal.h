@@ -1,0 +1,192 @@
+// cac_internal.h -- device context and launcher declarations shared by the
+// translation units of libcac_hip.so.  Not part of the C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/cac_hip.h"
+
+namespace cac {
+
+// Grow-only device buffer.
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+
+// Sub-block edge (sites) used to prune the integration-point site search: a
+// 4x4x4 block holds 64 sites, one per lane of a wavefront.
+constexpr int kSubBlock = 4;
+constexpr int kMaxSubBlocks = 512;   // nae <= 32 per edge
+constexpr int kMaxNint = 64;         // integration points per element
+constexpr int kImgSelf = 13;
+
+struct Timer {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+  double total_ms = 0.0;
+  int count = 0;
+};
+
+struct EamTables {
+  int nrho = 0, nr = 0;
+  double rdrho = 0, rdr = 0, cutforcesq = 0, rhomax = 0;
+  // r-knot rows [(nr+1)][16]: rhor c0..c6 in 0..6, z2r c0..c6 in 8..14;
+  // rho-knot rows [(nrho+1)][8]: frho c0..c6
+  double *rz = nullptr, *fr = nullptr;
+  float *rz_f = nullptr, *fr_f = nullptr;
+  DevBuf b_rz, b_fr, b_rz_f, b_fr_f;
+};
+
+struct LjParams {
+  double lj1 = 0, lj2 = 0, lj3 = 0, lj4 = 0, cutsq = 0;
+};
+
+}  // namespace cac
+
+struct cac_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  // box (fully periodic)
+  bool box_set = false;
+  double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0}, prd[3] = {0, 0, 0};
+
+  // ---- atoms: x4 = (x, y, z, fp) per atom; v, f, xhold SoA [3][n]
+  int n = 0;
+  double mass = 1.0;
+  double *x4 = nullptr, *v = nullptr, *f = nullptr, *xhold = nullptr, *rho = nullptr;
+  double *eatom = nullptr, *vatom = nullptr;  // [n], [6][n]
+  cac::DevBuf b_x4, b_v, b_f, b_xhold, b_rho, b_eatom, b_vatom;
+  // atom neighbor list, column-major [maxneigh][n]
+  int maxneigh = 128;
+  uint32_t *aneigh = nullptr;
+  int *anum = nullptr;
+  cac::DevBuf b_aneigh, b_anum;
+  // atom cells
+  int acell[3] = {0, 0, 0};
+  cac::DevBuf b_akey, b_aval, b_akey2, b_aval2, b_acs, b_ace, b_sorttmp;
+
+  // ---- elements (one type)
+  int ne = 0, nae[3] = {0, 0, 0}, nsite = 0;
+  double mass_e = 1.0;
+  double *nodex = nullptr, *nodev = nullptr, *nodef = nullptr, *nodexhold = nullptr, *xc = nullptr;
+  cac::DevBuf b_nodex, b_nodev, b_nodef, b_nodexhold, b_xc;
+  double *spa_site = nullptr;  // [nsite][8]
+  cac::DevBuf b_spa_site;
+  int nint = 0;
+  double *spa_int = nullptr, *weight = nullptr;  // [nint][8], [nint]
+  cac::DevBuf b_spa_int, b_weight;
+  std::vector<double> h_spa_int, h_weight;
+  int node_list[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  bool quad_set = false;
+  // sub-blocks: per element per sub-block (cx, cy, cz, radius); element (cx, cy, cz, R)
+  int nsb[3] = {0, 0, 0}, nsb_total = 0;
+  double *sbc = nullptr, *erad = nullptr;
+  cac::DevBuf b_sbc, b_erad;
+  // element neighbor list, column-major [maxelist][ne]
+  int maxelist = 192;
+  uint32_t *eneigh = nullptr;
+  int *enumb = nullptr;
+  cac::DevBuf b_eneigh, b_enumb;
+  int ecell[3] = {0, 0, 0};
+  cac::DevBuf b_ekey, b_eval, b_ekey2, b_eval2, b_ecs, b_ece;
+  // integration-point neighbor list, column-major [maxnip][nip]
+  int maxnip = 128;
+  uint64_t *ipneigh = nullptr;
+  int *ipnum = nullptr;
+  cac::DevBuf b_ipneigh, b_ipnum;
+  double *erho = nullptr, *ipf = nullptr, *eip = nullptr, *vip = nullptr;  // [ne*8], [3][nip], [nip], [6][nip]
+  cac::DevBuf b_erho, b_ipf, b_eip, b_vip;
+
+  // ---- potential
+  int pair = CAC_PAIR_NONE;
+  cac::EamTables eam;
+  cac::LjParams lj;
+
+  // ---- neighbor settings / state
+  double cutneigh = 0.0, triggersq = 0.0;
+  int every = 1, delay = 0, check = 1;
+  bool neigh_set = false, built = false;
+  int ago = 0;
+  long ncalls = 0;
+
+  // ---- energy / virial of the last force call
+  bool have_ev = false;
+  double eng[2] = {0, 0}, virial[6] = {0, 0, 0, 0, 0, 0};
+  double *red = nullptr;  // reduction scratch [64 blocks][16]
+  cac::DevBuf b_red;
+
+  // ---- flags: [0] overflow atoms, [1] max atom count, [2] overflow elist,
+  // [3] max elist count, [4] overflow ip, [5] max ip count, [6] rebuild flag
+  int *dflags = nullptr;
+  int *hflags = nullptr;  // pinned mirror
+  cac::DevBuf b_dflags;
+
+  // ---- profiling
+  bool profiling = false;
+  std::map<std::string, cac::Timer> timers;
+};
+
+namespace cac {
+
+// error helpers (return status, store message in ctx)
+int fail(cac_ctx *ctx, int code, const std::string &msg);
+int hip_fail(cac_ctx *ctx, hipError_t e, const char *where);
+#define CAC_HIP(ctx, call)                                     \
+  do {                                                         \
+    hipError_t e_ = (call);                                    \
+    if (e_ != hipSuccess) return ::cac::hip_fail(ctx, e_, #call); \
+  } while (0)
+
+template <class T>
+int ensure(cac_ctx *ctx, DevBuf &b, size_t count, T **out) {
+  size_t need = count * sizeof(T);
+  if (need == 0) need = sizeof(T);
+  if (b.bytes < need) {
+    if (b.p) {
+      hipError_t e = hipFree(b.p);
+      if (e != hipSuccess) return hip_fail(ctx, e, "hipFree");
+      b.p = nullptr;
+      b.bytes = 0;
+    }
+    hipError_t e = hipMalloc(&b.p, need);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc");
+    b.bytes = need;
+  }
+  *out = static_cast<T *>(b.p);
+  return CAC_OK;
+}
+
+// timing of kernel families on the context stream
+void timer_begin(cac_ctx *ctx, const char *name, hipEvent_t *start);
+void timer_end(cac_ctx *ctx, const char *name, hipEvent_t start);
+
+// ---- launchers (k_neighbor.hip)
+int launch_pbc_and_centers(cac_ctx *ctx);
+int launch_atom_neighbors(cac_ctx *ctx);
+int launch_element_neighbors(cac_ctx *ctx);
+int launch_ip_neighbors(cac_ctx *ctx);
+int launch_check_displacement(cac_ctx *ctx);
+
+// ---- launchers (k_pair.hip)
+int launch_eam_atoms(cac_ctx *ctx, int eflag, int vflag, int precision);
+int launch_eam_elements(cac_ctx *ctx, int eflag, int vflag, int precision);
+int launch_lj_atoms(cac_ctx *ctx, int eflag, int vflag, int precision);
+int launch_lj_elements(cac_ctx *ctx, int eflag, int vflag, int precision);
+int launch_scatter(cac_ctx *ctx);
+
+// ---- launchers (cac_api.hip)
+int launch_reduce_ev(cac_ctx *ctx, int eflag, int vflag);
+
+// host table construction (tables.cpp)
+int eam_regrid_spline(int nrho_f, double drho_f, int nr_f, double dr_f, const double *frho_f,
+                      const double *rhor_f, const double *zr_f, int *nrho, double *drho, int *nr,
+                      double *dr, double *frho_spline, double *rhor_spline, double *z2r_spline);
+
+}  // namespace cac

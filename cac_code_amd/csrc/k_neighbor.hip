@@ -1,0 +1,632 @@
+// k_neighbor.hip -- periodic wrap, cell binning and the three neighbor builds
+// of the CAC path on gfx950: atoms (full list), elements (candidate element
+// images), and integration points (interpolated sites of neighbor elements).
+//
+// Bit-exactness: this translation unit is compiled with -ffp-contract=off.
+// Image coordinates are x + s*prd (one add per component), the value the
+// reference's ghost copies carry (atom_vec_atomic.cpp:202-221,
+// element.cpp:933-953); site positions are sum_k N_k*X'_k in node order
+// (full_node_neighbor.cpp:141-159); rsq is summed x, y, z left to right and
+// compared with <= (neighbor.cpp:609, full_node_neighbor.cpp:167,216).
+// Search pruning (cells, element radii, sub-block radii) is conservative by
+// construction, so the sets equal the brute-force sets.
+#include <hipcub/hipcub.hpp>
+
+#include "cac_internal.h"
+
+namespace cac {
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ void img_shift(int img, int s[3]) {
+  s[0] = img % 3 - 1;
+  s[1] = (img / 3) % 3 - 1;
+  s[2] = img / 9 - 1;
+}
+
+__device__ __forceinline__ double shifted(double x, int s, double prd) {
+  return s ? x + (double)s * prd : x;
+}
+
+struct Box3 {
+  double lo[3], hi[3], prd[3];
+};
+
+Box3 box_of(const cac_ctx *ctx) {
+  Box3 b;
+  for (int d = 0; d < 3; ++d) {
+    b.lo[d] = ctx->lo[d];
+    b.hi[d] = ctx->hi[d];
+    b.prd[d] = ctx->prd[d];
+  }
+  return b;
+}
+
+struct Grid3 {
+  int n[3];
+  double inv[3];
+};
+
+// ---------------------------------------------------------------- pbc
+// Domain::pbc, domain.cpp:535-633.
+__global__ void k_pbc_atoms(int n, double *__restrict__ x4, Box3 b) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int d = 0; d < 3; ++d) {
+    double x = x4[4 * i + d];
+    if (x < b.lo[d]) x += b.prd[d];
+    if (x >= b.hi[d]) {
+      x -= b.prd[d];
+      x = x > b.lo[d] ? x : b.lo[d];
+    }
+    x4[4 * i + d] = x;
+  }
+}
+
+// Element centers (node mean, fix_nve.cpp:139-153) and Domain::elepbc
+// (domain.cpp:647-745; the nodes move with the center, fixing the x-hi sign
+// slip at domain.cpp:682).
+__global__ void k_pbc_elements(int ne, double *__restrict__ nodex, double *__restrict__ xc, Box3 b) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ne) return;
+  double *X = nodex + 24 * (size_t)i;
+  for (int d = 0; d < 3; ++d) {
+    double c = 0.0;
+    for (int k = 0; k < 8; ++k) c += X[3 * k + d];
+    c = c / 8;
+    if (c < b.lo[d]) {
+      c += b.prd[d];
+      for (int k = 0; k < 8; ++k) X[3 * k + d] += b.prd[d];
+    }
+    if (c >= b.hi[d]) {
+      c -= b.prd[d];
+      for (int k = 0; k < 8; ++k) X[3 * k + d] -= b.prd[d];
+      c = c > b.lo[d] ? c : b.lo[d];
+    }
+    xc[3 * i + d] = c;
+  }
+}
+
+// ---------------------------------------------------------------- cells
+__device__ __forceinline__ int cell_coord(double x, double lo, double inv, int n) {
+  int k = (int)floor((x - lo) * inv);
+  return k < 0 ? 0 : (k >= n ? n - 1 : k);
+}
+
+__global__ void k_cell_keys(int n, const double *__restrict__ p, int stride, Box3 b, Grid3 g,
+                            uint32_t *__restrict__ key, uint32_t *__restrict__ val) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int c0 = cell_coord(p[stride * i + 0], b.lo[0], g.inv[0], g.n[0]);
+  int c1 = cell_coord(p[stride * i + 1], b.lo[1], g.inv[1], g.n[1]);
+  int c2 = cell_coord(p[stride * i + 2], b.lo[2], g.inv[2], g.n[2]);
+  key[i] = (uint32_t)((c2 * g.n[1] + c1) * g.n[0] + c0);
+  val[i] = (uint32_t)i;
+}
+
+__global__ void k_fill_int(int n, int *__restrict__ a, int v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+__global__ void k_cell_bounds(int n, const uint32_t *__restrict__ key, int *__restrict__ cs,
+                              int *__restrict__ ce) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  uint32_t k = key[p];
+  if (p == 0 || key[p - 1] != k) cs[k] = p;
+  if (p == n - 1 || key[p + 1] != k) ce[k] = p + 1;
+}
+
+// ---------------------------------------------------------------- atoms
+// Full list (Neighbor::half_bin_no_newton, neighbor.cpp:546-625): every
+// (j, image) != (i, self) with rsq <= cutneighsq.  Candidates come from the
+// 27 cells around i's cell (cell edge >= cutneigh, >= 3 cells per dim), each
+// cell's atoms in ascending index, so the list order is deterministic.
+__global__ void k_atom_neigh(int n, const double *__restrict__ x4, Box3 b, Grid3 g,
+                             const uint32_t *__restrict__ sorted, const int *__restrict__ cs,
+                             const int *__restrict__ ce, double cutsq, int maxneigh,
+                             uint32_t *__restrict__ neigh, int *__restrict__ num,
+                             double *__restrict__ xhold, int *__restrict__ flags) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double xi = x4[4 * i], yi = x4[4 * i + 1], zi = x4[4 * i + 2];
+  const int ci[3] = {cell_coord(xi, b.lo[0], g.inv[0], g.n[0]), cell_coord(yi, b.lo[1], g.inv[1], g.n[1]),
+                     cell_coord(zi, b.lo[2], g.inv[2], g.n[2])};
+  int cnt = 0;
+  for (int dz = -1; dz <= 1; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        int cc[3] = {ci[0] + dx, ci[1] + dy, ci[2] + dz}, s[3];
+        for (int d = 0; d < 3; ++d) {
+          s[d] = 0;
+          if (cc[d] < 0) { cc[d] += g.n[d]; s[d] = -1; }
+          else if (cc[d] >= g.n[d]) { cc[d] -= g.n[d]; s[d] = 1; }
+        }
+        const int img = (s[0] + 1) + 3 * (s[1] + 1) + 9 * (s[2] + 1);
+        const int c = (cc[2] * g.n[1] + cc[1]) * g.n[0] + cc[0];
+        const int pe = ce[c];
+        for (int q = cs[c]; q < pe; ++q) {
+          const int j = (int)sorted[q];
+          if (j == i && img == kImgSelf) continue;
+          double delx = xi - shifted(x4[4 * j], s[0], b.prd[0]);
+          double dely = yi - shifted(x4[4 * j + 1], s[1], b.prd[1]);
+          double delz = zi - shifted(x4[4 * j + 2], s[2], b.prd[2]);
+          double rsq = delx * delx + dely * dely + delz * delz;
+          if (rsq <= cutsq) {
+            if (cnt < maxneigh) neigh[(size_t)cnt * n + i] = (uint32_t)j | ((uint32_t)img << 27);
+            ++cnt;
+          }
+        }
+      }
+  num[i] = cnt;
+  if (cnt > maxneigh) {
+    atomicOr(&flags[0], 1);
+    atomicMax(&flags[1], cnt);
+  }
+  xhold[i] = xi;
+  xhold[n + i] = yi;
+  xhold[2 * (size_t)n + i] = zi;
+}
+
+// ---------------------------------------------------------------- elements
+// Geometry at rebuild: element radius R = max_k |X_k - c| (every site is a
+// convex combination of the nodes, so it lies within R of the center) and,
+// per 4x4x4 sub-block, the center and radius of its 8 corner sites (the
+// trilinear map restricted to the sub-block is trilinear in the block's own
+// coordinates, so its sites are convex combinations of those corners).
+__global__ void k_elem_radius(int ne, const double *__restrict__ nodex, const double *__restrict__ xc,
+                              double *__restrict__ erad, unsigned long long *__restrict__ rmax_bits) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ne) return;
+  const double *X = nodex + 24 * (size_t)i;
+  double c[3] = {0.0, 0.0, 0.0};
+  for (int k = 0; k < 8; ++k)
+    for (int d = 0; d < 3; ++d) c[d] += X[3 * k + d];
+  for (int d = 0; d < 3; ++d) c[d] = c[d] / 8;
+  double r2 = 0.0;
+  for (int k = 0; k < 8; ++k) {
+    double dx = X[3 * k] - c[0], dy = X[3 * k + 1] - c[1], dz = X[3 * k + 2] - c[2];
+    double q = dx * dx + dy * dy + dz * dz;
+    r2 = q > r2 ? q : r2;
+  }
+  double R = sqrt(r2) * (1.0 + 1e-12) + 1e-9;
+  erad[4 * i] = c[0];
+  erad[4 * i + 1] = c[1];
+  erad[4 * i + 2] = c[2];
+  erad[4 * i + 3] = R;
+  atomicMax(rmax_bits, (unsigned long long)__double_as_longlong(R));
+}
+
+struct SubBlockGeom {
+  int nae[3];
+  int nsb[3];
+};
+
+__device__ __forceinline__ void sb_range(int b, int nae, int *lo, int *hi) {
+  *lo = b * kSubBlock;
+  int h = *lo + kSubBlock - 1;
+  *hi = h < nae - 1 ? h : nae - 1;
+}
+
+__device__ __forceinline__ void interp_site(const double *__restrict__ N, const double X[24], double out[3]) {
+  double a = 0.0, b = 0.0, c = 0.0;
+  for (int k = 0; k < 8; ++k) {
+    a += N[k] * X[3 * k];
+    b += N[k] * X[3 * k + 1];
+    c += N[k] * X[3 * k + 2];
+  }
+  out[0] = a;
+  out[1] = b;
+  out[2] = c;
+}
+
+__global__ void k_subblocks(int ne, SubBlockGeom sg, const double *__restrict__ nodex,
+                            const double *__restrict__ spa_site, double *__restrict__ sbc) {
+  const int nsbt = sg.nsb[0] * sg.nsb[1] * sg.nsb[2];
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)ne * nsbt) return;
+  const int i = (int)(t / nsbt), bidx = (int)(t % nsbt);
+  const int ba = bidx / (sg.nsb[1] * sg.nsb[2]);
+  const int bb = (bidx / sg.nsb[2]) % sg.nsb[1];
+  const int bc = bidx % sg.nsb[2];
+  int lo[3], hi[3];
+  sb_range(ba, sg.nae[0], &lo[0], &hi[0]);
+  sb_range(bb, sg.nae[1], &lo[1], &hi[1]);
+  sb_range(bc, sg.nae[2], &lo[2], &hi[2]);
+  double X[24];
+  for (int q = 0; q < 24; ++q) X[q] = nodex[24 * (size_t)i + q];
+  double P[8][3];
+  double c[3] = {0.0, 0.0, 0.0};
+  for (int k = 0; k < 8; ++k) {
+    int a = (k & 1) ? hi[0] : lo[0];
+    int bq = (k & 2) ? hi[1] : lo[1];
+    int cq = (k & 4) ? hi[2] : lo[2];
+    int site = (a * sg.nae[1] + bq) * sg.nae[2] + cq;
+    interp_site(spa_site + 8 * site, X, P[k]);
+    for (int d = 0; d < 3; ++d) c[d] += P[k][d];
+  }
+  for (int d = 0; d < 3; ++d) c[d] = c[d] / 8;
+  double r2 = 0.0;
+  for (int k = 0; k < 8; ++k) {
+    double dx = P[k][0] - c[0], dy = P[k][1] - c[1], dz = P[k][2] - c[2];
+    double q = dx * dx + dy * dy + dz * dz;
+    r2 = q > r2 ? q : r2;
+  }
+  double* o = sbc + 4 * (size_t)t;
+  o[0] = c[0];
+  o[1] = c[1];
+  o[2] = c[2];
+  o[3] = sqrt(r2) * (1.0 + 1e-12) + 1e-9;
+}
+
+// Candidate element images of element i: every (j, image) != (i, self)
+// with |c_i - c_j'| <= cutneigh + R_i + R_j.  Any site within cutneigh of
+// an integration point of i lies in such an element (triangle inequality),
+// so this list is a superset of what element_neighbor.cpp:37-66 needs.
+__device__ __forceinline__ bool elem_pair(const double *__restrict__ erad, int i, int j, int img,
+                                          const Box3 &b, double cutneigh) {
+  if (j == i && img == kImgSelf) return false;
+  int s[3];
+  img_shift(img, s);
+  double dx = erad[4 * i] - shifted(erad[4 * j], s[0], b.prd[0]);
+  double dy = erad[4 * i + 1] - shifted(erad[4 * j + 1], s[1], b.prd[1]);
+  double dz = erad[4 * i + 2] - shifted(erad[4 * j + 2], s[2], b.prd[2]);
+  double reach = cutneigh + erad[4 * i + 3] + erad[4 * j + 3];
+  return dx * dx + dy * dy + dz * dz <= reach * reach;
+}
+
+__global__ void k_elem_neigh_cells(int ne, const double *__restrict__ erad, Box3 b, Grid3 g,
+                                   const uint32_t *__restrict__ sorted, const int *__restrict__ cs,
+                                   const int *__restrict__ ce, double cutneigh, int maxelist,
+                                   uint32_t *__restrict__ eneigh, int *__restrict__ enumb,
+                                   int *__restrict__ flags) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ne) return;
+  const int ci[3] = {cell_coord(erad[4 * i], b.lo[0], g.inv[0], g.n[0]),
+                     cell_coord(erad[4 * i + 1], b.lo[1], g.inv[1], g.n[1]),
+                     cell_coord(erad[4 * i + 2], b.lo[2], g.inv[2], g.n[2])};
+  int cnt = 0;
+  for (int dz = -1; dz <= 1; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        int cc[3] = {ci[0] + dx, ci[1] + dy, ci[2] + dz}, s[3];
+        for (int d = 0; d < 3; ++d) {
+          s[d] = 0;
+          if (cc[d] < 0) { cc[d] += g.n[d]; s[d] = -1; }
+          else if (cc[d] >= g.n[d]) { cc[d] -= g.n[d]; s[d] = 1; }
+        }
+        const int img = (s[0] + 1) + 3 * (s[1] + 1) + 9 * (s[2] + 1);
+        const int c = (cc[2] * g.n[1] + cc[1]) * g.n[0] + cc[0];
+        for (int q = cs[c]; q < ce[c]; ++q) {
+          const int j = (int)sorted[q];
+          if (elem_pair(erad, i, j, img, b, cutneigh)) {
+            if (cnt < maxelist) eneigh[(size_t)cnt * ne + i] = (uint32_t)j | ((uint32_t)img << 27);
+            ++cnt;
+          }
+        }
+      }
+  enumb[i] = cnt;
+  if (cnt > maxelist) {
+    atomicOr(&flags[2], 1);
+    atomicMax(&flags[3], cnt);
+  }
+}
+
+// Small boxes (fewer than 3 element cells per dim): all j, all 27 images.
+__global__ void k_elem_neigh_brute(int ne, const double *__restrict__ erad, Box3 b, double cutneigh,
+                                   int maxelist, uint32_t *__restrict__ eneigh, int *__restrict__ enumb,
+                                   int *__restrict__ flags) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ne) return;
+  int cnt = 0;
+  for (int img = 0; img < 27; ++img)
+    for (int j = 0; j < ne; ++j)
+      if (elem_pair(erad, i, j, img, b, cutneigh)) {
+        if (cnt < maxelist) eneigh[(size_t)cnt * ne + i] = (uint32_t)j | ((uint32_t)img << 27);
+        ++cnt;
+      }
+  enumb[i] = cnt;
+  if (cnt > maxelist) {
+    atomicOr(&flags[2], 1);
+    atomicMax(&flags[3], cnt);
+  }
+}
+
+// Integration-point site lists (Neighbor::node_neighbor,
+// full_node_neighbor.cpp:19-239): for integration point g of element i, every
+// site of every candidate element image with rsq <= cutneighsq, then the
+// sites of i itself with rsq > 1e-4 (full_node_neighbor.cpp:216), which drops
+// the integration point's own site.  One thread per integration point.
+struct IpGeom {
+  int ne, nint, nae[3], nsb[3], nsbt;
+};
+
+__global__ void k_ip_neigh(IpGeom ig, const double *__restrict__ nodex, const double *__restrict__ spa_int,
+                           const double *__restrict__ spa_site, const double *__restrict__ erad,
+                           const double *__restrict__ sbc, const uint32_t *__restrict__ eneigh,
+                           const int *__restrict__ enumb, Box3 b, double cutneigh, int maxnip,
+                           uint64_t *__restrict__ ipneigh, int *__restrict__ ipnum, int *__restrict__ flags) {
+  const int nip = ig.ne * ig.nint;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nip) return;
+  const int i = p / ig.nint, g = p % ig.nint;
+  const double cutsq = cutneigh * cutneigh;
+  double ip[3];
+  {
+    double Xi[24];
+    for (int q = 0; q < 24; ++q) Xi[q] = nodex[24 * (size_t)i + q];
+    interp_site(spa_int + 8 * g, Xi, ip);
+  }
+  int cnt = 0;
+  const int nel = enumb[i];
+  for (int l = 0; l <= nel; ++l) {
+    int j, img;
+    if (l < nel) {
+      uint32_t e = eneigh[(size_t)l * ig.ne + i];
+      j = (int)(e & 0x7FFFFFFu);
+      img = (int)(e >> 27);
+    } else {
+      j = i;
+      img = kImgSelf;
+    }
+    const bool self = (j == i && img == kImgSelf);
+    int s[3];
+    img_shift(img, s);
+    {
+      double dx = ip[0] - shifted(erad[4 * j], s[0], b.prd[0]);
+      double dy = ip[1] - shifted(erad[4 * j + 1], s[1], b.prd[1]);
+      double dz = ip[2] - shifted(erad[4 * j + 2], s[2], b.prd[2]);
+      double reach = cutneigh + erad[4 * j + 3];
+      if (dx * dx + dy * dy + dz * dz > reach * reach) continue;
+    }
+    double X[24];
+    for (int k = 0; k < 8; ++k)
+      for (int d = 0; d < 3; ++d) X[3 * k + d] = shifted(nodex[24 * (size_t)j + 3 * k + d], s[d], b.prd[d]);
+    for (int sbi = 0; sbi < ig.nsbt; ++sbi) {
+      const double *sb = sbc + 4 * ((size_t)j * ig.nsbt + sbi);
+      double dx = ip[0] - shifted(sb[0], s[0], b.prd[0]);
+      double dy = ip[1] - shifted(sb[1], s[1], b.prd[1]);
+      double dz = ip[2] - shifted(sb[2], s[2], b.prd[2]);
+      double reach = cutneigh + sb[3];
+      if (dx * dx + dy * dy + dz * dz > reach * reach) continue;
+      const int ba = sbi / (ig.nsb[1] * ig.nsb[2]);
+      const int bb = (sbi / ig.nsb[2]) % ig.nsb[1];
+      const int bc = sbi % ig.nsb[2];
+      int lo[3], hi[3];
+      sb_range(ba, ig.nae[0], &lo[0], &hi[0]);
+      sb_range(bb, ig.nae[1], &lo[1], &hi[1]);
+      sb_range(bc, ig.nae[2], &lo[2], &hi[2]);
+      for (int a = lo[0]; a <= hi[0]; ++a)
+        for (int bq = lo[1]; bq <= hi[1]; ++bq)
+          for (int cq = lo[2]; cq <= hi[2]; ++cq) {
+            const int site = (a * ig.nae[1] + bq) * ig.nae[2] + cq;
+            double xs[3];
+            interp_site(spa_site + 8 * site, X, xs);
+            double delx = xs[0] - ip[0], dely = xs[1] - ip[1], delz = xs[2] - ip[2];
+            double rsq = delx * delx + dely * dely + delz * delz;
+            if (rsq <= cutsq && (!self || rsq > 1.0e-4)) {
+              if (cnt < maxnip)
+                ipneigh[(size_t)cnt * nip + p] =
+                    (uint64_t)j | ((uint64_t)img << 27) | ((uint64_t)site << 32);
+              ++cnt;
+            }
+          }
+    }
+  }
+  ipnum[p] = cnt;
+  if (cnt > maxnip) {
+    atomicOr(&flags[4], 1);
+    atomicMax(&flags[5], cnt);
+  }
+}
+
+// Neighbor::check_distance, neighbor.cpp:1176-1217: flag if any atom or node
+// moved more than sqrt(triggersq) since the last build.
+__global__ void k_check_disp(int n, const double *__restrict__ x4, const double *__restrict__ xhold, int ne,
+                             const double *__restrict__ nodex, const double *__restrict__ nodexhold,
+                             double triggersq, int *__restrict__ flags) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  bool hit = false;
+  if (t < n) {
+    double dx = x4[4 * t] - xhold[t];
+    double dy = x4[4 * t + 1] - xhold[n + t];
+    double dz = x4[4 * t + 2] - xhold[2 * (size_t)n + t];
+    hit = dx * dx + dy * dy + dz * dz > triggersq;
+  } else if (t < n + 8 * ne) {
+    size_t q = (size_t)(t - n);
+    double dx = nodex[3 * q] - nodexhold[3 * q];
+    double dy = nodex[3 * q + 1] - nodexhold[3 * q + 1];
+    double dz = nodex[3 * q + 2] - nodexhold[3 * q + 2];
+    hit = dx * dx + dy * dy + dz * dz > triggersq;
+  }
+  if (hit) atomicOr(&flags[6], 1);
+}
+
+inline unsigned grid_for(long n, int block = kBlock) { return (unsigned)((n + block - 1) / block); }
+
+// Cells of edge >= cut*1.0001 over the box; 0 if fewer than 3 per dim.
+bool make_grid(const cac_ctx *ctx, double cut, Grid3 *g) {
+  bool ok = true;
+  for (int d = 0; d < 3; ++d) {
+    g->n[d] = (int)(ctx->prd[d] / (cut * 1.0001));
+    if (g->n[d] < 3) ok = false;
+    g->inv[d] = g->n[d] > 0 ? g->n[d] / ctx->prd[d] : 0.0;
+  }
+  return ok;
+}
+
+// Bin points (stride doubles apart) into g: sorted point ids and per-cell
+// [start, end) ranges, ascending id within a cell (stable radix sort).
+int bin_points(cac_ctx *ctx, int np, const double *pts, int stride, const Grid3 &g, cac::DevBuf &bk,
+               cac::DevBuf &bv, cac::DevBuf &bk2, cac::DevBuf &bv2, cac::DevBuf &bcs, cac::DevBuf &bce,
+               uint32_t **sorted_out, int **cs_out, int **ce_out) {
+  uint32_t *key, *val, *key2, *val2;
+  int *cs, *ce;
+  const long ncell = (long)g.n[0] * g.n[1] * g.n[2];
+  int rc;
+  if ((rc = ensure(ctx, bk, np, &key)) || (rc = ensure(ctx, bv, np, &val)) || (rc = ensure(ctx, bk2, np, &key2)) ||
+      (rc = ensure(ctx, bv2, np, &val2)) || (rc = ensure(ctx, bcs, ncell, &cs)) || (rc = ensure(ctx, bce, ncell, &ce)))
+    return rc;
+  Box3 b = box_of(ctx);
+  if (np > 0) {
+    hipLaunchKernelGGL(k_cell_keys, dim3(grid_for(np)), dim3(kBlock), 0, ctx->stream, np, pts, stride, b, g, key, val);
+    int bits = 1;
+    while ((1L << bits) < ncell) ++bits;
+    size_t tmp_bytes = 0;
+    CAC_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, key, key2, val, val2, np, 0, bits,
+                                                    ctx->stream));
+    char *tmp;
+    if ((rc = ensure(ctx, ctx->b_sorttmp, tmp_bytes, &tmp))) return rc;
+    CAC_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, key, key2, val, val2, np, 0, bits,
+                                                    ctx->stream));
+  }
+  hipLaunchKernelGGL(k_fill_int, dim3(grid_for(ncell)), dim3(kBlock), 0, ctx->stream, (int)ncell, cs, 0);
+  hipLaunchKernelGGL(k_fill_int, dim3(grid_for(ncell)), dim3(kBlock), 0, ctx->stream, (int)ncell, ce, 0);
+  if (np > 0)
+    hipLaunchKernelGGL(k_cell_bounds, dim3(grid_for(np)), dim3(kBlock), 0, ctx->stream, np, key2, cs, ce);
+  CAC_HIP(ctx, hipGetLastError());
+  *sorted_out = val2;
+  *cs_out = cs;
+  *ce_out = ce;
+  return CAC_OK;
+}
+
+}  // namespace
+
+int launch_pbc_and_centers(cac_ctx *ctx) {
+  Box3 b = box_of(ctx);
+  if (ctx->n > 0)
+    hipLaunchKernelGGL(k_pbc_atoms, dim3(grid_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->n, ctx->x4, b);
+  if (ctx->ne > 0)
+    hipLaunchKernelGGL(k_pbc_elements, dim3(grid_for(ctx->ne)), dim3(kBlock), 0, ctx->stream, ctx->ne,
+                       ctx->nodex, ctx->xc, b);
+  CAC_HIP(ctx, hipGetLastError());
+  return CAC_OK;
+}
+
+int launch_atom_neighbors(cac_ctx *ctx) {
+  const int n = ctx->n;
+  if (n == 0) return CAC_OK;
+  Grid3 g;
+  if (!make_grid(ctx, ctx->cutneigh, &g))
+    return fail(ctx, CAC_ERR_BOX, "box must be at least 3 neighbor cutoffs wide in every dimension");
+  for (int d = 0; d < 3; ++d) ctx->acell[d] = g.n[d];
+  uint32_t *sorted;
+  int *cs, *ce, rc;
+  if ((rc = bin_points(ctx, n, ctx->x4, 4, g, ctx->b_akey, ctx->b_aval, ctx->b_akey2, ctx->b_aval2, ctx->b_acs,
+                       ctx->b_ace, &sorted, &cs, &ce)))
+    return rc;
+  if ((rc = ensure(ctx, ctx->b_aneigh, (size_t)ctx->maxneigh * n, &ctx->aneigh)) ||
+      (rc = ensure(ctx, ctx->b_anum, n, &ctx->anum)) || (rc = ensure(ctx, ctx->b_xhold, 3 * (size_t)n, &ctx->xhold)))
+    return rc;
+  hipEvent_t t0;
+  timer_begin(ctx, "atom_neigh", &t0);
+  hipLaunchKernelGGL(k_atom_neigh, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, n, ctx->x4, box_of(ctx), g,
+                     sorted, cs, ce, ctx->cutneigh * ctx->cutneigh, ctx->maxneigh, ctx->aneigh, ctx->anum,
+                     ctx->xhold, ctx->dflags);
+  timer_end(ctx, "atom_neigh", t0);
+  CAC_HIP(ctx, hipGetLastError());
+  return CAC_OK;
+}
+
+int launch_element_neighbors(cac_ctx *ctx) {
+  const int ne = ctx->ne;
+  if (ne == 0) return CAC_OK;
+  int rc;
+  unsigned long long *rmax_bits;
+  if ((rc = ensure(ctx, ctx->b_erad, 4 * (size_t)ne, &ctx->erad))) return rc;
+  // scratch for the radius max lives in the reduction buffer
+  if ((rc = ensure(ctx, ctx->b_red, 64 * 16, &ctx->red))) return rc;
+  rmax_bits = reinterpret_cast<unsigned long long *>(ctx->red);
+  CAC_HIP(ctx, hipMemsetAsync(rmax_bits, 0, sizeof(unsigned long long), ctx->stream));
+  hipLaunchKernelGGL(k_elem_radius, dim3(grid_for(ne)), dim3(kBlock), 0, ctx->stream, ne, ctx->nodex, ctx->xc,
+                     ctx->erad, rmax_bits);
+  // sub-block geometry
+  for (int d = 0; d < 3; ++d) ctx->nsb[d] = (ctx->nae[d] + kSubBlock - 1) / kSubBlock;
+  ctx->nsb_total = ctx->nsb[0] * ctx->nsb[1] * ctx->nsb[2];
+  if ((rc = ensure(ctx, ctx->b_sbc, 4 * (size_t)ne * ctx->nsb_total, &ctx->sbc))) return rc;
+  SubBlockGeom sg;
+  for (int d = 0; d < 3; ++d) {
+    sg.nae[d] = ctx->nae[d];
+    sg.nsb[d] = ctx->nsb[d];
+  }
+  hipLaunchKernelGGL(k_subblocks, dim3(grid_for((long)ne * ctx->nsb_total)), dim3(kBlock), 0, ctx->stream, ne, sg,
+                     ctx->nodex, ctx->spa_site, ctx->sbc);
+  unsigned long long hbits = 0;
+  CAC_HIP(ctx, hipMemcpyAsync(&hbits, rmax_bits, sizeof(hbits), hipMemcpyDeviceToHost, ctx->stream));
+  CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  double rmax;
+  memcpy(&rmax, &hbits, sizeof(rmax));
+  if ((rc = ensure(ctx, ctx->b_eneigh, (size_t)ctx->maxelist * ne, &ctx->eneigh)) ||
+      (rc = ensure(ctx, ctx->b_enumb, ne, &ctx->enumb)))
+    return rc;
+  Grid3 g;
+  const double ecut = ctx->cutneigh + 2.0 * rmax;
+  hipEvent_t t0;
+  if (make_grid(ctx, ecut, &g)) {
+    for (int d = 0; d < 3; ++d) ctx->ecell[d] = g.n[d];
+    uint32_t *sorted;
+    int *cs, *ce;
+    if ((rc = bin_points(ctx, ne, ctx->erad, 4, g, ctx->b_ekey, ctx->b_eval, ctx->b_ekey2, ctx->b_eval2, ctx->b_ecs,
+                         ctx->b_ece, &sorted, &cs, &ce)))
+      return rc;
+    timer_begin(ctx, "elem_neigh", &t0);
+    hipLaunchKernelGGL(k_elem_neigh_cells, dim3(grid_for(ne)), dim3(kBlock), 0, ctx->stream, ne, ctx->erad,
+                       box_of(ctx), g, sorted, cs, ce, ctx->cutneigh, ctx->maxelist, ctx->eneigh, ctx->enumb,
+                       ctx->dflags);
+    timer_end(ctx, "elem_neigh", t0);
+  } else {
+    for (int d = 0; d < 3; ++d) ctx->ecell[d] = 0;
+    for (int d = 0; d < 3; ++d)
+      if (ctx->prd[d] < ctx->cutneigh)
+        return fail(ctx, CAC_ERR_BOX, "box narrower than the neighbor cutoff");
+    timer_begin(ctx, "elem_neigh", &t0);
+    hipLaunchKernelGGL(k_elem_neigh_brute, dim3(grid_for(ne)), dim3(kBlock), 0, ctx->stream, ne, ctx->erad,
+                       box_of(ctx), ctx->cutneigh, ctx->maxelist, ctx->eneigh, ctx->enumb, ctx->dflags);
+    timer_end(ctx, "elem_neigh", t0);
+  }
+  CAC_HIP(ctx, hipGetLastError());
+  return CAC_OK;
+}
+
+int launch_ip_neighbors(cac_ctx *ctx) {
+  const int ne = ctx->ne;
+  if (ne == 0) return CAC_OK;
+  const int nip = ne * ctx->nint;
+  int rc;
+  if ((rc = ensure(ctx, ctx->b_ipneigh, (size_t)ctx->maxnip * nip, &ctx->ipneigh)) ||
+      (rc = ensure(ctx, ctx->b_ipnum, nip, &ctx->ipnum)) ||
+      (rc = ensure(ctx, ctx->b_nodexhold, 24 * (size_t)ne, &ctx->nodexhold)))
+    return rc;
+  IpGeom ig;
+  ig.ne = ne;
+  ig.nint = ctx->nint;
+  for (int d = 0; d < 3; ++d) {
+    ig.nae[d] = ctx->nae[d];
+    ig.nsb[d] = ctx->nsb[d];
+  }
+  ig.nsbt = ctx->nsb_total;
+  hipEvent_t t0;
+  timer_begin(ctx, "ip_neigh", &t0);
+  hipLaunchKernelGGL(k_ip_neigh, dim3(grid_for(nip, 64)), dim3(64), 0, ctx->stream, ig, ctx->nodex, ctx->spa_int,
+                     ctx->spa_site, ctx->erad, ctx->sbc, ctx->eneigh, ctx->enumb, box_of(ctx), ctx->cutneigh,
+                     ctx->maxnip, ctx->ipneigh, ctx->ipnum, ctx->dflags);
+  timer_end(ctx, "ip_neigh", t0);
+  CAC_HIP(ctx, hipMemcpyAsync(ctx->nodexhold, ctx->nodex, sizeof(double) * 24 * (size_t)ne,
+                              hipMemcpyDeviceToDevice, ctx->stream));
+  CAC_HIP(ctx, hipGetLastError());
+  return CAC_OK;
+}
+
+int launch_check_displacement(cac_ctx *ctx) {
+  long tot = (long)ctx->n + 8L * ctx->ne;
+  if (tot == 0) return CAC_OK;
+  hipLaunchKernelGGL(k_check_disp, dim3(grid_for(tot)), dim3(kBlock), 0, ctx->stream, ctx->n, ctx->x4, ctx->xhold,
+                     ctx->ne, ctx->nodex, ctx->nodexhold, ctx->triggersq, ctx->dflags);
+  CAC_HIP(ctx, hipGetLastError());
+  return CAC_OK;
+}
+
+}  // namespace cac

@@ -1,0 +1,316 @@
+"""Host-side mirror of the reference's pair-style and run interfaces on top of
+the C-ABI (include/cac_hip.h).
+
+* ``PairEAM`` / ``PairLJCut`` follow ``pair_style eam`` and ``pair_style
+  lj/cut`` of the reference: ``settings(args)`` (pair_eam.cpp:109-114,
+  pair_lj_cut.cpp:74-88), ``coeff(args)`` (pair_eam.cpp:120-160,
+  pair_lj_cut.cpp:94-121), ``init_one(i, j)`` (pair_eam.cpp:288-304,
+  pair_lj_cut.cpp:140-198) and ``compute``/``ecompute`` (pair.h:96-97),
+  with the same argument meaning and the same error conditions.
+* ``Engine`` owns one device context: box, atoms, elements, integration
+  points, neighbor settings (``neighbor`` / ``neigh_modify``), force calls
+  and the Verlet/FixNVE loop (verlet.cpp:149-219, fix_nve.cpp:40-242).
+
+Everything numeric runs in libcac_hip.so; this module only moves arguments.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from .lattice import CACSystem, FTM2V_METAL
+from .potential import Funcfl, lj_coeffs, read_funcfl
+
+
+def _p(a, t=_lib._d):
+    return a.ctypes.data_as(t)
+
+
+def _check(ctx, rc):
+    if rc != _lib.CAC_OK:
+        msg = _lib.lib().cac_last_error(ctx).decode() if ctx else "no context"
+        raise _lib.CacError(rc, msg)
+
+
+def _bounds(s: str, ntypes: int):
+    """Force::bounds, force.cpp:158-180: 'N', '*', 'N*', '*N', 'M*N'."""
+    if s == "*":
+        return 1, ntypes
+    if "*" not in s:
+        v = int(s)
+        return v, v
+    a, b = s.split("*", 1)
+    lo = int(a) if a else 1
+    hi = int(b) if b else ntypes
+    return lo, hi
+
+
+class PairError(ValueError):
+    pass
+
+
+class PairEAM:
+    """pair_style eam: one funcfl file per atom type (pair_eam.cpp:120-160)."""
+    style = "eam"
+
+    def __init__(self, ntypes: int = 1):
+        self.ntypes = ntypes
+        self.funcfl: Optional[Funcfl] = None
+        self.mass: Optional[float] = None
+        self.tables = None
+
+    def settings(self, args):
+        if len(args) > 0:
+            raise PairError("Illegal pair_style command")          # pair_eam.cpp:111
+
+    def coeff(self, args, funcfl: Optional[Funcfl] = None):
+        if len(args) != 3:
+            raise PairError("Incorrect args for pair coefficients")  # pair_eam.cpp:124
+        ilo, ihi = _bounds(args[0], self.ntypes)
+        jlo, jhi = _bounds(args[1], self.ntypes)
+        if not (ilo <= 1 <= ihi and jlo <= 1 <= jhi) or ihi > self.ntypes:
+            raise PairError("Incorrect args for pair coefficients")
+        ff = funcfl if funcfl is not None else read_funcfl(args[2])
+        self.funcfl = ff
+        self.mass = ff.mass                                          # pair_eam.cpp:159-160
+        self.tables = eam_tables(ff)
+
+    def init_one(self, i: int, j: int) -> float:
+        if self.funcfl is None:
+            raise PairError("All pair coeffs are not set")
+        return self.funcfl.cut                                       # pair_eam.cpp:297-301
+
+    def install(self, eng: "Engine"):
+        t = self.tables
+        cut = self.init_one(1, 1)
+        _check(eng.ctx, _lib.lib().cac_set_eam(eng.ctx, t["nrho"], t["drho"], _p(t["frho_spline"]), t["nr"],
+                                               t["dr"], _p(t["rhor_spline"]), _p(t["z2r_spline"]), cut * cut))
+
+
+class PairLJCut:
+    """pair_style lj/cut cut (pair_lj_cut.cpp:74-198)."""
+    style = "lj/cut"
+
+    def __init__(self, ntypes: int = 1):
+        self.ntypes = ntypes
+        self.cut_global = None
+        self.eps = self.sigma = self.cut = None
+
+    def settings(self, args):
+        if len(args) != 1:
+            raise PairError("Illegal pair_style command")
+        self.cut_global = float(args[0])
+
+    def coeff(self, args):
+        if len(args) < 4 or len(args) > 5:
+            raise PairError("Incorrect args for pair coefficients")  # pair_lj_cut.cpp:96-97
+        _bounds(args[0], self.ntypes)
+        _bounds(args[1], self.ntypes)
+        self.eps = float(args[2])
+        self.sigma = float(args[3])
+        self.cut = float(args[4]) if len(args) == 5 else self.cut_global
+
+    def init_one(self, i: int, j: int):
+        if self.eps is None:
+            raise PairError("All pair coeffs are not set")
+        return lj_coeffs(self.eps, self.sigma, self.cut)
+
+    def install(self, eng: "Engine"):
+        lj1, lj2, lj3, lj4, cutsq = self.init_one(1, 1)
+        _check(eng.ctx, _lib.lib().cac_set_lj(eng.ctx, lj1, lj2, lj3, lj4, cutsq))
+
+
+def eam_tables(ff: Funcfl):
+    """file2array + array2spline in the native library (pair_eam.cpp:311-580)."""
+    L = _lib.lib()
+    nrho, nr = C.c_int(), C.c_int()
+    drho, dr = C.c_double(), C.c_double()
+    fr, rr, zr = (np.ascontiguousarray(a, dtype=np.float64) for a in (ff.frho, ff.rhor, ff.zr))
+    rc = L.cac_eam_tables_from_funcfl(ff.nrho, ff.drho, ff.nr, ff.dr, _p(fr), _p(rr), _p(zr), C.byref(nrho),
+                                      C.byref(drho), C.byref(nr), C.byref(dr), None, None, None)
+    if rc:
+        raise PairError("bad EAM file")
+    out = dict(nrho=nrho.value, nr=nr.value, drho=drho.value, dr=dr.value, cutforcesq=ff.cut * ff.cut)
+    out["frho_spline"] = np.zeros((nrho.value + 1, 7))
+    out["rhor_spline"] = np.zeros((nr.value + 1, 7))
+    out["z2r_spline"] = np.zeros((nr.value + 1, 7))
+    rc = L.cac_eam_tables_from_funcfl(ff.nrho, ff.drho, ff.nr, ff.dr, _p(fr), _p(rr), _p(zr), C.byref(nrho),
+                                      C.byref(drho), C.byref(nr), C.byref(dr), _p(out["frho_spline"]),
+                                      _p(out["rhor_spline"]), _p(out["z2r_spline"]))
+    if rc:
+        raise PairError("bad EAM file")
+    return out
+
+
+class Engine:
+    """One device context (cac_ctx) and the run state around it."""
+
+    def __init__(self, device: int = 0):
+        self.L = _lib.lib()
+        h = C.c_void_p()
+        rc = self.L.cac_ctx_create(device, C.byref(h))
+        if rc != _lib.CAC_OK:
+            raise _lib.CacError(rc, "cac_ctx_create failed (no HIP device?)")
+        self.ctx = h
+        self.n = 0
+        self.ne = 0
+        self.nint = 0
+        self.pair = None
+        self.sys: Optional[CACSystem] = None
+
+    def close(self):
+        if self.ctx:
+            self.L.cac_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- set-up
+    def set_box(self, lo, hi):
+        lo = np.ascontiguousarray(lo, dtype=np.float64)
+        hi = np.ascontiguousarray(hi, dtype=np.float64)
+        per = np.ones(3, dtype=np.int32)
+        _check(self.ctx, self.L.cac_set_box(self.ctx, _p(lo), _p(hi), _p(per, _lib._i)))
+
+    def upload_atoms(self, x, v, mass):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        vv = None if v is None else np.ascontiguousarray(v, dtype=np.float64)
+        self.n = x.shape[0]
+        _check(self.ctx, self.L.cac_upload_atoms(self.ctx, self.n, _p(x), _p(vv) if vv is not None else None, mass))
+
+    def upload_elements(self, nodex, nodev, nae, mass_e, intp_ijk, intp_w):
+        nodex = np.ascontiguousarray(nodex, dtype=np.float64)
+        nv = None if nodev is None else np.ascontiguousarray(nodev, dtype=np.float64)
+        nae_c = np.ascontiguousarray(nae, dtype=np.int32)
+        self.ne = nodex.shape[0]
+        _check(self.ctx, self.L.cac_upload_elements(self.ctx, self.ne, _p(nae_c, _lib._i), _p(nodex),
+                                                    _p(nv) if nv is not None else None, mass_e))
+        ijk = np.ascontiguousarray(intp_ijk, dtype=np.int32)
+        w = np.ascontiguousarray(intp_w, dtype=np.float64)
+        self.nint = ijk.shape[0]
+        _check(self.ctx, self.L.cac_set_quadrature(self.ctx, self.nint, _p(ijk, _lib._i), _p(w)))
+
+    def load_system(self, s: CACSystem, mass: Optional[float] = None):
+        self.sys = s
+        self.set_box(s.box_lo, s.box_hi)
+        m = s.mass if mass is None else mass
+        self.mass = m
+        self.upload_atoms(s.x, s.v, m)
+        mass_e = float(s.nae[0] * s.nae[1] * s.nae[2]) * m
+        self.mass_e = mass_e
+        self.upload_elements(s.nodex, s.nodev, s.nae, mass_e, s.intp_ijk, s.intp_w)
+
+    def set_pair(self, pair):
+        self.pair = pair
+        pair.install(self)
+
+    def set_neighbor(self, cutneigh: float, skin: Optional[float] = None, every: int = 1, delay: int = 0,
+                     check: bool = True, maxneigh: int = 0, maxnip: int = 0):
+        """neighbor <cutneigh> bin + neigh_modify.  The rebuild trigger is half
+        of the true skin (cutneigh - force cutoff); the reference takes half of
+        the whole cutoff (neighbor.cpp:334, 357-358), which would let pairs
+        enter the force cutoff unlisted."""
+        if skin is None:
+            skin = cutneigh
+        triggersq = 0.25 * skin * skin
+        _check(self.ctx, self.L.cac_set_neighbor(self.ctx, cutneigh, triggersq, every, delay, 1 if check else 0,
+                                                 maxneigh, maxnip))
+
+    # ---- actions
+    def build(self):
+        _check(self.ctx, self.L.cac_build_neighbors(self.ctx))
+
+    def decide(self) -> bool:
+        flag = C.c_int()
+        _check(self.ctx, self.L.cac_decide(self.ctx, C.byref(flag)))
+        return bool(flag.value)
+
+    def force(self, eflag=False, vflag=False, precision=_lib.PREC_FP64):
+        _check(self.ctx, self.L.cac_force(self.ctx, int(eflag), int(vflag), precision))
+
+    def nve_initial(self, dtv, dtf):
+        _check(self.ctx, self.L.cac_nve_initial(self.ctx, dtv, dtf))
+
+    def nve_final(self, dtf):
+        _check(self.ctx, self.L.cac_nve_final(self.ctx, dtf))
+
+    def run(self, nsteps: int, dt: float = 0.001, precision=_lib.PREC_FP64, ftm2v: float = FTM2V_METAL) -> int:
+        nb = C.c_int()
+        dtf = 0.5 * dt * ftm2v                                  # fix_nve.cpp:43
+        _check(self.ctx, self.L.cac_run(self.ctx, nsteps, dt, dtf, precision, C.byref(nb)))
+        return nb.value
+
+    def synchronize(self):
+        _check(self.ctx, self.L.cac_synchronize(self.ctx))
+
+    # ---- results
+    def atoms(self):
+        x = np.zeros((self.n, 3)); v = np.zeros((self.n, 3)); f = np.zeros((self.n, 3))
+        _check(self.ctx, self.L.cac_download_atoms(self.ctx, _p(x), _p(v), _p(f)))
+        return x, v, f
+
+    def elements(self):
+        a = np.zeros((self.ne, 8, 3)); b = np.zeros((self.ne, 8, 3)); c = np.zeros((self.ne, 8, 3))
+        _check(self.ctx, self.L.cac_download_elements(self.ctx, _p(a), _p(b), _p(c)))
+        return a, b, c
+
+    def energy_virial(self):
+        e = np.zeros(2); w = np.zeros(6)
+        _check(self.ctx, self.L.cac_get_energy_virial(self.ctx, _p(e), _p(w)))
+        return e, w
+
+    def capacities(self):
+        a, b = C.c_int(), C.c_int()
+        _check(self.ctx, self.L.cac_get_capacities(self.ctx, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def atom_neighbors(self):
+        maxn, _ = self.capacities()
+        num = np.zeros(self.n, dtype=np.int32)
+        nb = np.zeros((max(self.n, 1), maxn), dtype=np.uint32)
+        _check(self.ctx, self.L.cac_get_atom_neighbors(self.ctx, maxn, _p(num, _lib._i), _p(nb, _lib._u32)))
+        return num, nb[:self.n]
+
+    def ip_neighbors(self):
+        _, maxn = self.capacities()
+        nip = self.ne * self.nint
+        num = np.zeros(nip, dtype=np.int32)
+        nb = np.zeros((max(nip, 1), maxn), dtype=np.uint64)
+        _check(self.ctx, self.L.cac_get_ip_neighbors(self.ctx, maxn, _p(num, _lib._i), _p(nb, _lib._u64)))
+        return num, nb[:nip]
+
+    def rho(self):
+        r = np.zeros(self.n); fp = np.zeros(self.n)
+        _check(self.ctx, self.L.cac_get_rho(self.ctx, _p(r), _p(fp)))
+        return r, fp
+
+    def erho(self):
+        e = np.zeros((self.ne, 8))
+        _check(self.ctx, self.L.cac_get_erho(self.ctx, _p(e)))
+        return e
+
+    def ip_forces(self):
+        f = np.zeros((self.ne * self.nint, 3))
+        _check(self.ctx, self.L.cac_get_ip_forces(self.ctx, _p(f)))
+        return f
+
+    def profile(self, on: bool = True):
+        _check(self.ctx, self.L.cac_profile_enable(self.ctx, int(on)))
+
+    def profile_avg_ms(self, name: str):
+        cnt = C.c_int()
+        ms = self.L.cac_profile_avg_ms(self.ctx, name.encode(), C.byref(cnt))
+        return ms, cnt.value

@@ -1,0 +1,162 @@
+/*
+ * cac_hip.h -- C-ABI of the MI355X-native CAC force-and-neighbor path.
+ *
+ * One context per host thread, one HIP stream per context, all device
+ * buffers owned by the context.  Plain pointers and sizes only; host arrays
+ * stay the owner of record for I/O, the device copy is authoritative while a
+ * run is in progress.  Every entry point returns 0 (CAC_OK) or a CAC_ERR_*
+ * status; cac_last_error() returns the message.  The reference has no return
+ * codes: a host style built on this ABI maps a non-zero status to its
+ * Error::all (error.cpp:72-91).
+ *
+ * Each declaration names the reference interface it replaces.  Reference
+ * paths are relative to the fangyuanxiao/CAC_Code tree.
+ *
+ * Neighbor entry encodings (cac_get_atom_neighbors / cac_get_ip_neighbors):
+ *   atom entry (uint32): j | (img << 27)
+ *   IP entry   (uint64): j | (img << 27) | ((uint64)site << 32)
+ * img = (sx+1) + 3*(sy+1) + 9*(sz+1), s in {-1,0,1}^3 the periodic image of
+ * j (position x_j + s*prd); img 13 is the unshifted copy.
+ */
+#ifndef CAC_HIP_H
+#define CAC_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cac_ctx cac_ctx;
+
+enum {
+  CAC_OK = 0,
+  CAC_ERR_ARG = 1,      /* bad argument or call order */
+  CAC_ERR_HIP = 2,      /* HIP runtime error */
+  CAC_ERR_OVERFLOW = 3, /* neighbor capacity exceeded (after regrow attempts) */
+  CAC_ERR_BOX = 4,      /* box too small for the cutoffs */
+  CAC_ERR_DATA = 5      /* inconsistent element data */
+};
+
+enum { CAC_PREC_FP64 = 0, CAC_PREC_MIXED = 1 };
+enum { CAC_PAIR_NONE = 0, CAC_PAIR_EAM = 1, CAC_PAIR_LJ = 2 };
+
+/* ---- context ------------------------------------------------------------
+ * Replaces the CAC god-object set-up (cac.cpp:157-213) for this path. */
+int cac_ctx_create(int device, cac_ctx **out);
+int cac_ctx_destroy(cac_ctx *ctx);
+const char *cac_last_error(const cac_ctx *ctx);
+int cac_synchronize(cac_ctx *ctx);
+
+/* Domain::set_initial_box / set_global_box (domain.cpp), fully periodic only
+ * (the C0-C4 decks use "boundary p p p"). */
+int cac_set_box(cac_ctx *ctx, const double lo[3], const double hi[3], const int periodic[3]);
+
+/* Atom container upload, replaces AtomVecAtomic::data_atom + grow
+ * (atom_vec_atomic.cpp:41-71).  x, v: [n][3] AoS like Atom::x (atom.h:37);
+ * v may be NULL (zero).  One atom type of mass `mass` (Atom::mass[1]). */
+int cac_upload_atoms(cac_ctx *ctx, int n, const double *x, const double *v, double mass);
+
+/* Element container upload, replaces Element::data_elements/data_nodes/
+ * set_nae (element.cpp:195-294, 478-540, 1086-1110).  nodex, nodev:
+ * [ne][8][3] like Element::nodex (element.h:25); nodev may be NULL.
+ * mass_e = nae0*nae1*nae2*mass (element.cpp:1104). */
+int cac_upload_elements(cac_ctx *ctx, int ne, const int nae[3], const double *nodex,
+                        const double *nodev, double mass_e);
+
+/* Integration points of the element type, replaces the Numint/Intp sections
+ * (element.cpp:1113-1166) plus intp_pcoord / shape_array_int /
+ * find_node_list (element.cpp:1170-1191, 1326-1354, 1296-1319).  ijk: [nint][3]
+ * lattice indices, w: [nint] weights.  Fails with CAC_ERR_DATA if a node has
+ * no coincident integration point. */
+int cac_set_quadrature(cac_ctx *ctx, int nint, const int *ijk, const double *w);
+
+/* Host-side table construction shared by every path: PairEAM::file2array +
+ * array2spline for one funcfl file (pair_eam.cpp:311-580).  Inputs are the
+ * file arrays (0-based, value at r = m*dr is f[m]); outputs are 1-based spline
+ * arrays [(n+1)][7] with n = *nrho_out / *nr_out; pass NULL outputs to query
+ * the sizes first. */
+int cac_eam_tables_from_funcfl(int nrho_f, double drho_f, int nr_f, double dr_f,
+                               const double *frho_f, const double *rhor_f, const double *zr_f,
+                               int *nrho_out, double *drho_out, int *nr_out, double *dr_out,
+                               double *frho_spline, double *rhor_spline, double *z2r_spline);
+
+/* pair_style eam + pair_coeff (pair_eam.cpp:124-160, init_style :274-283,
+ * init_one :288-304): upload the spline tables (rows 1..n of 7) and
+ * cutforcesq. */
+int cac_set_eam(cac_ctx *ctx, int nrho, double drho, const double *frho_spline,
+                int nr, double dr, const double *rhor_spline, const double *z2r_spline,
+                double cutforcesq);
+
+/* pair_style lj/cut + pair_coeff (pair_lj_cut.cpp:96-198): lj1..lj4 from
+ * PairLJCut::init_one and cutsq = cut^2. */
+int cac_set_lj(cac_ctx *ctx, double lj1, double lj2, double lj3, double lj4, double cutsq);
+
+/* neighbor + neigh_modify (neighbor.cpp:194-262, init :297-481).
+ * cutneigh is the full list cutoff (the reference's "skin" value, which it
+ * uses as the whole cutoff, neighbor.cpp:357-358).  triggersq: squared
+ * displacement that forces a rebuild (neighbor.cpp:334).  every/delay/check
+ * as in Neighbor::decide (neighbor.cpp:1150-1163).  maxneigh / maxnip:
+ * initial per-atom / per-integration-point capacities (grown on overflow). */
+int cac_set_neighbor(cac_ctx *ctx, double cutneigh, double triggersq, int every, int delay,
+                     int check, int maxneigh, int maxnip);
+
+/* Neighbor rebuild: Domain::pbc/elepbc + Neighbor::build/ebuild/nbuild
+ * (verlet.cpp:173-193; neighbor.cpp:546-625, element_neighbor.cpp:11-75,
+ * full_node_neighbor.cpp:19-239).  Regrows capacities on overflow. */
+int cac_build_neighbors(cac_ctx *ctx);
+
+/* Neighbor::decide + check_distance (neighbor.cpp:1150-1217). */
+int cac_decide(cac_ctx *ctx, int *rebuild);
+
+/* Pair::compute (atoms) + Pair::ecompute (elements), pair.h:96-97:
+ * PairEAM::compute/ecompute (pair_eam.cpp:584-746, 761-1137) or
+ * PairLJCut::compute/ecompute (pair_lj_cut.cpp:227-288, 295-442), after
+ * Verlet::force_clear/node_force_clear (verlet.cpp:105-144).  eflag/vflag
+ * enable the energy/virial tallies (build-defined; the reference has none).
+ * precision: CAC_PREC_FP64 or CAC_PREC_MIXED (FP32 pair/embedding math,
+ * FP64 accumulation). */
+int cac_force(cac_ctx *ctx, int eflag, int vflag, int precision);
+
+/* FixNVE::initial_integrate + einitial_integrate (fix_nve.cpp:54-157) and
+ * FixNVE::final_integrate + efinal_integrate (fix_nve.cpp:181-242);
+ * dtf = 0.5*dt*ftm2v (fix_nve.cpp:43). */
+int cac_nve_initial(cac_ctx *ctx, double dtv, double dtf);
+int cac_nve_final(cac_ctx *ctx, double dtf);
+
+/* Verlet::run (verlet.cpp:149-219) for nsteps with FixNVE, device-resident:
+ * initial integrate, decide/rebuild, force, final integrate.  *nbuilds (may
+ * be NULL) receives the number of rebuilds. */
+int cac_run(cac_ctx *ctx, int nsteps, double dtv, double dtf, int precision, int *nbuilds);
+
+/* Device -> host copies (Atom::x/v/f, Element::nodex/nodev/nodef layouts). */
+int cac_download_atoms(cac_ctx *ctx, double *x, double *v, double *f);
+int cac_download_elements(cac_ctx *ctx, double *nodex, double *nodev, double *nodef);
+
+/* Energy and virial of the last cac_force with eflag/vflag set:
+ * eng[0] atoms, eng[1] elements (integration-point weighted);
+ * virial[6] = xx yy zz xy xz yz summed over both (build-defined, not in the
+ * reference: integrate.cpp:97-114 forces eflag = vflag = 0). */
+int cac_get_energy_virial(cac_ctx *ctx, double eng[2], double virial[6]);
+
+/* Parity hooks: neighbor lists (row-major [n][maxneigh] / [ne*nint][maxnip],
+ * device order), per-atom rho/fp, per-node erho ([ne][8]), per-integration-
+ * point force ([ne*nint][3]).  Return CAC_ERR_ARG if the capacity given is
+ * below the device list length (see cac_get_capacities). */
+int cac_get_capacities(cac_ctx *ctx, int *maxneigh, int *maxnip);
+int cac_get_atom_neighbors(cac_ctx *ctx, int maxneigh, int *numneigh, uint32_t *neigh);
+int cac_get_ip_neighbors(cac_ctx *ctx, int maxnip, int *numneigh, uint64_t *neigh);
+int cac_get_rho(cac_ctx *ctx, double *rho, double *fp);
+int cac_get_erho(cac_ctx *ctx, double *erho);
+int cac_get_ip_forces(cac_ctx *ctx, double *ipf);
+
+/* Timing hook: average duration (ms) of the named kernel family over the
+ * launches recorded since the last reset, measured with HIP events on the
+ * context stream ("atom_force", "ip_force", ...); returns -1 if unknown. */
+int cac_profile_enable(cac_ctx *ctx, int on);
+double cac_profile_avg_ms(cac_ctx *ctx, const char *name, int *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,130 @@
+"""GPU parity: libcac_hip.so against the CPU FP64 oracle on the same inputs.
+
+Bar: neighbor sets bit-exact (compared sorted); FP64 rho, fp, erho, forces,
+energy and virial within 1e-10 norm-relative (max |gpu - cpu| / max |cpu|).
+"""
+import numpy as np
+import pytest
+
+from cac_code_amd import Engine, PairEAM, PairLJCut
+from cac_code_amd.lattice import build_cac_system
+from cac_code_amd.potential import lj_coeffs
+
+from gpu_common import (CUTNEIGH_EAM, assert_same_sets, oracle_state, rel_err, small_system)
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def make_engine(s, pair, cutneigh, maxneigh=0, maxnip=0):
+    eng = Engine(0)
+    eng.load_system(s)
+    eng.set_pair(pair)
+    eng.set_neighbor(cutneigh, every=1, delay=0, check=True, maxneigh=maxneigh, maxnip=maxnip)
+    eng.build()
+    return eng
+
+
+def eam_pair(cu_funcfl):
+    p = PairEAM()
+    p.settings([])
+    p.coeff(["1", "1", "Cu_johnson.eam"], funcfl=cu_funcfl)
+    return p
+
+
+SYSTEMS = {
+    # 10^3-cell box, nae=5, brute-force element-pair path (box < 3 element reaches)
+    "small": lambda: small_system(seed=7, L=10, nae=5, core=4.0),
+    # no atoms at all
+    "elements_only": lambda: small_system(seed=11, L=10, nae=5, core=0.0),
+    # 20^3 cells, nae=5, cell path for elements
+    "medium": lambda: small_system(seed=3, L=20, nae=5, core=8.0, sigma=0.08),
+    # finest mesh nae=2 (every site a node)
+    "finest": lambda: small_system(seed=5, L=8, nae=2, core=3.0, sigma=0.03),
+}
+
+
+@pytest.mark.parametrize("name", list(SYSTEMS))
+def test_neighbor_sets_bitexact(name, orc, cu_funcfl):
+    s = SYSTEMS[name]()
+    print(f"[neighbors] {name}: atoms={s.n_atoms} elements={s.n_elements}", flush=True)
+    ref = oracle_state(orc, s, CUTNEIGH_EAM)
+    eng = make_engine(s, eam_pair(cu_funcfl), CUTNEIGH_EAM)
+    try:
+        if s.n_atoms:
+            num, nb = eng.atom_neighbors()
+            assert_same_sets(num, nb, ref["anum"], ref["anb"], "atoms")
+        if s.n_elements:
+            num, nb = eng.ip_neighbors()
+            assert_same_sets(num, nb, ref["inum"], ref["inb"], "integration points")
+    finally:
+        eng.close()
+
+
+@pytest.mark.parametrize("name", list(SYSTEMS))
+def test_eam_fp64_parity(name, orc, cu_funcfl, cu_tables):
+    s = SYSTEMS[name]()
+    print(f"[eam fp64] {name}: atoms={s.n_atoms} elements={s.n_elements}", flush=True)
+    ref = oracle_state(orc, s, CUTNEIGH_EAM, tables=cu_tables)
+    eng = make_engine(s, eam_pair(cu_funcfl), CUTNEIGH_EAM)
+    try:
+        eng.force(eflag=True, vflag=True)
+        e, w = eng.energy_virial()
+        if s.n_atoms:
+            rho, fp = eng.rho()
+            _, _, f = eng.atoms()
+            a = ref["atoms"]
+            assert rel_err(rho, a["rho"]) < TOL
+            assert rel_err(fp, a["fp"]) < TOL
+            assert rel_err(f, a["f"]) < TOL
+            assert abs(e[0] - a["eatom"].sum()) <= TOL * abs(a["eatom"].sum())
+        if s.n_elements:
+            el = ref["elems"]
+            assert rel_err(eng.erho(), el["erho"]) < TOL
+            assert rel_err(eng.ip_forces(), el["ipf"]) < TOL
+            _, _, nodef = eng.elements()
+            assert rel_err(nodef, el["nodef"]) < TOL
+            assert abs(e[1] - el["eip"].sum()) <= TOL * abs(el["eip"].sum())
+        vref = np.zeros(6)
+        if s.n_atoms:
+            vref += ref["atoms"]["vatom"].sum(axis=0)
+        if s.n_elements:
+            vref += ref["elems"]["vip"].sum(axis=0)
+        assert rel_err(w, vref) < TOL
+    finally:
+        eng.close()
+
+
+@pytest.mark.parametrize("name", ["small", "medium"])
+def test_lj_fp64_parity(name, orc):
+    s = SYSTEMS[name]()
+    print(f"[lj fp64] {name}", flush=True)
+    lj = lj_coeffs(0.4093, 2.338, 5.0)
+    ref = oracle_state(orc, s, 6.0, lj=lj)
+    p = PairLJCut()
+    p.settings(["5.0"])
+    p.coeff(["1", "1", "0.4093", "2.338"])
+    eng = make_engine(s, p, 6.0)
+    try:
+        eng.force(eflag=True, vflag=True)
+        e, w = eng.energy_virial()
+        _, _, f = eng.atoms()
+        assert rel_err(f, ref["atoms"]["f"]) < TOL
+        assert rel_err(eng.ip_forces(), ref["elems"]["ipf"]) < TOL
+        _, _, nodef = eng.elements()
+        assert rel_err(nodef, ref["elems"]["nodef"]) < TOL
+        assert abs(e[0] - ref["atoms"]["eatom"].sum()) <= TOL * abs(ref["atoms"]["eatom"].sum())
+        assert abs(e[1] - ref["elems"]["eip"].sum()) <= TOL * abs(ref["elems"]["eip"].sum())
+        vref = ref["atoms"]["vatom"].sum(axis=0) + ref["elems"]["vip"].sum(axis=0)
+        assert rel_err(w, vref) < TOL
+    finally:
+        eng.close()
+
+
+def test_tables_shared_bits(orc, cu_funcfl, cu_tables):
+    """The library's host table builder equals the oracle's bit for bit."""
+    from cac_code_amd import eam_tables
+    t = eam_tables(cu_funcfl)
+    for k in ("frho_spline", "rhor_spline", "z2r_spline"):
+        assert np.array_equal(t[k], cu_tables[k]), k
