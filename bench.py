@@ -89,8 +89,8 @@ def gpu_run(s, pair, cutneigh, skin, steps, warmup, precision, every, delay):
     eng.synchronize()
     t1 = time.perf_counter()
     prof = {}
-    for name in ("atom_rho", "atom_force", "ip_erho", "ip_force", "scatter", "atom_neigh", "ip_neigh",
-                 "elem_neigh"):
+    for name in ("atom_rho", "atom_force", "site_pos", "ip_erho", "site_fp", "ip_force", "scatter", "atom_neigh",
+                 "ip_neigh", "elem_neigh"):
         ms, cnt = eng.profile_avg_ms(name)
         if cnt:
             prof[name] = (ms, cnt)

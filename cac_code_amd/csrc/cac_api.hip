@@ -168,13 +168,16 @@ int ready_for_force(cac_ctx *ctx) {
   return CAC_OK;
 }
 
+// The tally kernels write energy and virial together when either is asked.
 int alloc_ev(cac_ctx *ctx, int eflag, int vflag) {
+  if (!eflag && !vflag) return CAC_OK;
   int rc;
-  if (eflag && ctx->n && (rc = ensure(ctx, ctx->b_eatom, (size_t)ctx->n, &ctx->eatom))) return rc;
-  if (vflag && ctx->n && (rc = ensure(ctx, ctx->b_vatom, 6 * (size_t)ctx->n, &ctx->vatom))) return rc;
+  if (ctx->n && ((rc = ensure(ctx, ctx->b_eatom, (size_t)ctx->n, &ctx->eatom)) ||
+                 (rc = ensure(ctx, ctx->b_vatom, 6 * (size_t)ctx->n, &ctx->vatom))))
+    return rc;
   const size_t nip = (size_t)ctx->ne * ctx->nint;
-  if (eflag && nip && (rc = ensure(ctx, ctx->b_eip, nip, &ctx->eip))) return rc;
-  if (vflag && nip && (rc = ensure(ctx, ctx->b_vip, 6 * nip, &ctx->vip))) return rc;
+  if (nip && ((rc = ensure(ctx, ctx->b_eip, nip, &ctx->eip)) || (rc = ensure(ctx, ctx->b_vip, 6 * nip, &ctx->vip))))
+    return rc;
   return CAC_OK;
 }
 
@@ -213,6 +216,7 @@ int build_impl(cac_ctx *ctx) {
     if ((rc = launch_ip_neighbors(ctx))) return rc;
     if ((rc = read_flags(ctx))) return rc;
     if (ctx->hflags[4]) { ctx->maxnip = ctx->hflags[5] + 16; continue; }
+    if ((rc = launch_active_sites(ctx))) return rc;
     ctx->built = true;
     ctx->ago = 0;
     ctx->ncalls++;
@@ -336,6 +340,8 @@ int cac_ctx_destroy(cac_ctx *ctx) {
                     &ctx->b_nodexhold, &ctx->b_xc, &ctx->b_spa_site, &ctx->b_spa_int, &ctx->b_weight, &ctx->b_sbc,
                     &ctx->b_erad, &ctx->b_eneigh, &ctx->b_enumb, &ctx->b_ekey, &ctx->b_eval, &ctx->b_ekey2,
                     &ctx->b_eval2, &ctx->b_ecs, &ctx->b_ece, &ctx->b_ipneigh, &ctx->b_ipnum, &ctx->b_erho,
+                    &ctx->b_iptmp, &ctx->b_bitmap, &ctx->b_wprefix, &ctx->b_ecount, &ctx->b_estart, &ctx->b_asite,
+                    &ctx->b_xs4, &ctx->b_scantmp,
                     &ctx->b_ipf, &ctx->b_eip, &ctx->b_vip, &ctx->b_red, &ctx->b_dflags, &ctx->eam.b_rz,
                     &ctx->eam.b_fr, &ctx->eam.b_rz_f, &ctx->eam.b_fr_f};
   for (DevBuf *b : bufs)
@@ -666,12 +672,13 @@ int cac_get_atom_neighbors(cac_ctx *ctx, int maxneigh, int *numneigh, uint32_t *
   const int n = ctx->n;
   if (n == 0) return CAC_OK;
   CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  std::vector<uint32_t> cm((size_t)ctx->maxneigh * n);
+  const int M = ctx->maxneigh;
+  std::vector<uint32_t> cm((size_t)M * round_up(n, kRowGroup));
   CAC_HIP(ctx, hipMemcpy(numneigh, ctx->anum, n * sizeof(int), hipMemcpyDeviceToHost));
   CAC_HIP(ctx, hipMemcpy(cm.data(), ctx->aneigh, cm.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
   for (int i = 0; i < n; ++i)
-    for (int k = 0; k < numneigh[i] && k < ctx->maxneigh; ++k)
-      neigh[(size_t)i * maxneigh + k] = cm[(size_t)k * n + i];
+    for (int k = 0; k < numneigh[i] && k < M; ++k)
+      neigh[(size_t)i * maxneigh + k] = cm[((size_t)(i >> 3) * M + k) * kRowGroup + (i & 7)];
   return CAC_OK;
 }
 
@@ -683,11 +690,19 @@ int cac_get_ip_neighbors(cac_ctx *ctx, int maxnip, int *numneigh, uint64_t *neig
   const size_t nip = (size_t)ctx->ne * ctx->nint;
   if (nip == 0) return CAC_OK;
   CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  std::vector<uint64_t> cm((size_t)ctx->maxnip * nip);
+  const int M = ctx->maxnip;
+  std::vector<uint32_t> cm((size_t)M * round_up((int)nip, kRowGroup));
+  std::vector<int2> as((size_t)ctx->nactive);
   CAC_HIP(ctx, hipMemcpy(numneigh, ctx->ipnum, nip * sizeof(int), hipMemcpyDeviceToHost));
-  CAC_HIP(ctx, hipMemcpy(cm.data(), ctx->ipneigh, cm.size() * sizeof(uint64_t), hipMemcpyDeviceToHost));
+  CAC_HIP(ctx, hipMemcpy(cm.data(), ctx->ipneigh, cm.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  if (ctx->nactive)
+    CAC_HIP(ctx, hipMemcpy(as.data(), ctx->asite, as.size() * sizeof(int2), hipMemcpyDeviceToHost));
   for (size_t p = 0; p < nip; ++p)
-    for (int k = 0; k < numneigh[p] && k < ctx->maxnip; ++k) neigh[p * maxnip + k] = cm[(size_t)k * nip + p];
+    for (int k = 0; k < numneigh[p] && k < M; ++k) {
+      const uint32_t e = cm[((p >> 3) * M + k) * kRowGroup + (p & 7)];
+      const int2 js = as[e & 0x7FFFFFFu];
+      neigh[p * maxnip + k] = (uint64_t)js.x | ((uint64_t)(e >> 27) << 27) | ((uint64_t)js.y << 32);
+    }
   return CAC_OK;
 }
 

@@ -25,6 +25,27 @@ constexpr int kSubBlock = 4;
 constexpr int kMaxSubBlocks = 512;   // nae <= 32 per edge
 constexpr int kMaxNint = 64;         // integration points per element
 constexpr int kImgSelf = 13;
+// Lanes cooperating on one atom / node / integration point in the pair
+// passes, and the row-group size of the blocked neighbor layout: entry k of
+// row r lives at ((r/8)*M + k)*8 + r%8, so a wavefront serving 8 rows with
+// 8 lanes each reads 64 consecutive entries per step.
+constexpr int kLanes = 8;
+constexpr int kRowGroup = 8;
+
+// Workgroups are handed to the 8 XCDs of an MI355X round-robin.  Kernels
+// launched with xcd_grid() call xcd_block() to remap: XCD x then processes
+// the contiguous logical blocks [x*q, (x+1)*q), so its L2 serves one spatial
+// region of the (spatially ordered) particle arrays instead of all of them.
+__device__ __forceinline__ int xcd_block() {
+  const int q = (int)(gridDim.x >> 3);
+  const int b = (int)blockIdx.x;
+  return (b & 7) * q + (b >> 3);
+}
+inline unsigned xcd_grid(long nthreads, int block) {
+  long nb = (nthreads + block - 1) / block;
+  return (unsigned)(((nb + 7) / 8) * 8);
+}
+inline int round_up(int v, int m) { return ((v + m - 1) / m) * m; }
 
 struct Timer {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
@@ -63,7 +84,7 @@ struct cac_ctx {
   double *x4 = nullptr, *v = nullptr, *f = nullptr, *xhold = nullptr, *rho = nullptr;
   double *eatom = nullptr, *vatom = nullptr;  // [n], [6][n]
   cac::DevBuf b_x4, b_v, b_f, b_xhold, b_rho, b_eatom, b_vatom;
-  // atom neighbor list, column-major [maxneigh][n]
+  // atom neighbor list, blocked: entry k of atom i at ((i/8)*maxneigh + k)*8 + i%8
   int maxneigh = 128;
   uint32_t *aneigh = nullptr;
   int *anum = nullptr;
@@ -96,11 +117,24 @@ struct cac_ctx {
   cac::DevBuf b_eneigh, b_enumb;
   int ecell[3] = {0, 0, 0};
   cac::DevBuf b_ekey, b_eval, b_ekey2, b_eval2, b_ecs, b_ece;
-  // integration-point neighbor list, column-major [maxnip][nip]
+  // integration-point neighbor list: raw (j | img<<27 | site<<32) entries
+  // from the search, then compacted to (active-site id | img<<27) in the
+  // blocked layout; maxnip is a multiple of kLanes
   int maxnip = 128;
-  uint64_t *ipneigh = nullptr;
+  uint64_t *iptmp = nullptr;
+  uint32_t *ipneigh = nullptr;
   int *ipnum = nullptr;
-  cac::DevBuf b_ipneigh, b_ipnum;
+  cac::DevBuf b_iptmp, b_ipneigh, b_ipnum;
+  // active sites: the (element, site) pairs some list refers to; per step
+  // xs4 = (x, y, z, fp) of each, interpolated once
+  int nactive = 0, nwords = 0;
+  uint32_t *bitmap = nullptr;   // [ne][nwords] referenced-site bits
+  int *wprefix = nullptr;       // [ne][nwords] first active id of each word
+  int *ecount = nullptr;        // [ne+1] active sites per element
+  int *estart = nullptr;        // [ne+1] first active id per element
+  int2 *asite = nullptr;        // [nactive] (element, site)
+  double *xs4 = nullptr;        // [nactive][4]
+  cac::DevBuf b_bitmap, b_wprefix, b_ecount, b_estart, b_asite, b_xs4, b_scantmp;
   double *erho = nullptr, *ipf = nullptr, *eip = nullptr, *vip = nullptr;  // [ne*8], [3][nip], [nip], [6][nip]
   cac::DevBuf b_erho, b_ipf, b_eip, b_vip;
 
@@ -172,6 +206,7 @@ int launch_pbc_and_centers(cac_ctx *ctx);
 int launch_atom_neighbors(cac_ctx *ctx);
 int launch_element_neighbors(cac_ctx *ctx);
 int launch_ip_neighbors(cac_ctx *ctx);
+int launch_active_sites(cac_ctx *ctx);
 int launch_check_displacement(cac_ctx *ctx);
 
 // ---- launchers (k_pair.hip)

@@ -124,13 +124,15 @@ __global__ void k_cell_bounds(int n, const uint32_t *__restrict__ key, int *__re
 // (j, image) != (i, self) with rsq <= cutneighsq.  Candidates come from the
 // 27 cells around i's cell (cell edge >= cutneigh, >= 3 cells per dim), each
 // cell's atoms in ascending index, so the list order is deterministic.
+// Entry k of atom i is stored at ((i/8)*maxneigh + k)*8 + i%8.
 __global__ void k_atom_neigh(int n, const double *__restrict__ x4, Box3 b, Grid3 g,
                              const uint32_t *__restrict__ sorted, const int *__restrict__ cs,
                              const int *__restrict__ ce, double cutsq, int maxneigh,
                              uint32_t *__restrict__ neigh, int *__restrict__ num,
                              double *__restrict__ xhold, int *__restrict__ flags) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
   if (i >= n) return;
+  uint32_t *row = neigh + (size_t)(i >> 3) * maxneigh * kRowGroup + (i & 7);
   const double xi = x4[4 * i], yi = x4[4 * i + 1], zi = x4[4 * i + 2];
   const int ci[3] = {cell_coord(xi, b.lo[0], g.inv[0], g.n[0]), cell_coord(yi, b.lo[1], g.inv[1], g.n[1]),
                      cell_coord(zi, b.lo[2], g.inv[2], g.n[2])};
@@ -155,7 +157,7 @@ __global__ void k_atom_neigh(int n, const double *__restrict__ x4, Box3 b, Grid3
           double delz = zi - shifted(x4[4 * j + 2], s[2], b.prd[2]);
           double rsq = delx * delx + dely * dely + delz * delz;
           if (rsq <= cutsq) {
-            if (cnt < maxneigh) neigh[(size_t)cnt * n + i] = (uint32_t)j | ((uint32_t)img << 27);
+            if (cnt < maxneigh) row[(size_t)cnt * kRowGroup] = (uint32_t)j | ((uint32_t)img << 27);
             ++cnt;
           }
         }
@@ -338,19 +340,30 @@ __global__ void k_elem_neigh_brute(int ne, const double *__restrict__ erad, Box3
 // full_node_neighbor.cpp:19-239): for integration point g of element i, every
 // site of every candidate element image with rsq <= cutneighsq, then the
 // sites of i itself with rsq > 1e-4 (full_node_neighbor.cpp:216), which drops
-// the integration point's own site.  One thread per integration point.
+// the integration point's own site.
+//
+// One wavefront per integration point.  Lanes test 64 candidate elements at
+// a time against the element radius; for each survivor (wave-uniform loop)
+// lanes test its sub-blocks, and for each surviving 4x4x4 sub-block every
+// lane takes one site.  Accepted sites are compacted in lane order with a
+// ballot, so the list order is deterministic.  Each accepted (element, site)
+// is also marked in the element's referenced-site bitmap.
 struct IpGeom {
-  int ne, nint, nae[3], nsb[3], nsbt;
+  int ne, nint, nae[3], nsb[3], nsbt, nwords;
 };
 
-__global__ void k_ip_neigh(IpGeom ig, const double *__restrict__ nodex, const double *__restrict__ spa_int,
-                           const double *__restrict__ spa_site, const double *__restrict__ erad,
-                           const double *__restrict__ sbc, const uint32_t *__restrict__ eneigh,
-                           const int *__restrict__ enumb, Box3 b, double cutneigh, int maxnip,
-                           uint64_t *__restrict__ ipneigh, int *__restrict__ ipnum, int *__restrict__ flags) {
+__global__ __launch_bounds__(256) void k_ip_neigh(IpGeom ig, const double *__restrict__ nodex,
+                                                  const double *__restrict__ spa_int,
+                                                  const double *__restrict__ spa_site,
+                                                  const double *__restrict__ erad, const double *__restrict__ sbc,
+                                                  const uint32_t *__restrict__ eneigh, const int *__restrict__ enumb,
+                                                  Box3 b, double cutneigh, int maxnip, uint64_t *__restrict__ tmp,
+                                                  int *__restrict__ ipnum, uint32_t *__restrict__ bitmap,
+                                                  int *__restrict__ flags) {
   const int nip = ig.ne * ig.nint;
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= nip) return;
+  const int lane = threadIdx.x & 63;
+  const int p = (xcd_block() * (int)blockDim.x + (int)threadIdx.x) >> 6;
+  if (p >= nip) return;  // whole wavefront
   const int i = p / ig.nint, g = p % ig.nint;
   const double cutsq = cutneigh * cutneigh;
   double ip[3];
@@ -360,66 +373,137 @@ __global__ void k_ip_neigh(IpGeom ig, const double *__restrict__ nodex, const do
     interp_site(spa_int + 8 * g, Xi, ip);
   }
   int cnt = 0;
-  const int nel = enumb[i];
-  for (int l = 0; l <= nel; ++l) {
-    int j, img;
+  const int nel = enumb[i] + 1;  // candidates, then the element itself
+  for (int l0 = 0; l0 < nel; l0 += 64) {
+    const int l = l0 + lane;
+    int j = i, img = kImgSelf;
+    bool keep = false;
     if (l < nel) {
-      uint32_t e = eneigh[(size_t)l * ig.ne + i];
-      j = (int)(e & 0x7FFFFFFu);
-      img = (int)(e >> 27);
-    } else {
-      j = i;
-      img = kImgSelf;
+      if (l < nel - 1) {
+        const uint32_t e = eneigh[(size_t)l * ig.ne + i];
+        j = (int)(e & 0x7FFFFFFu);
+        img = (int)(e >> 27);
+      }
+      int s[3];
+      img_shift(img, s);
+      const double dx = ip[0] - shifted(erad[4 * j], s[0], b.prd[0]);
+      const double dy = ip[1] - shifted(erad[4 * j + 1], s[1], b.prd[1]);
+      const double dz = ip[2] - shifted(erad[4 * j + 2], s[2], b.prd[2]);
+      const double reach = cutneigh + erad[4 * j + 3];
+      keep = dx * dx + dy * dy + dz * dz <= reach * reach;
     }
-    const bool self = (j == i && img == kImgSelf);
-    int s[3];
-    img_shift(img, s);
-    {
-      double dx = ip[0] - shifted(erad[4 * j], s[0], b.prd[0]);
-      double dy = ip[1] - shifted(erad[4 * j + 1], s[1], b.prd[1]);
-      double dz = ip[2] - shifted(erad[4 * j + 2], s[2], b.prd[2]);
-      double reach = cutneigh + erad[4 * j + 3];
-      if (dx * dx + dy * dy + dz * dz > reach * reach) continue;
-    }
-    double X[24];
-    for (int k = 0; k < 8; ++k)
-      for (int d = 0; d < 3; ++d) X[3 * k + d] = shifted(nodex[24 * (size_t)j + 3 * k + d], s[d], b.prd[d]);
-    for (int sbi = 0; sbi < ig.nsbt; ++sbi) {
-      const double *sb = sbc + 4 * ((size_t)j * ig.nsbt + sbi);
-      double dx = ip[0] - shifted(sb[0], s[0], b.prd[0]);
-      double dy = ip[1] - shifted(sb[1], s[1], b.prd[1]);
-      double dz = ip[2] - shifted(sb[2], s[2], b.prd[2]);
-      double reach = cutneigh + sb[3];
-      if (dx * dx + dy * dy + dz * dz > reach * reach) continue;
-      const int ba = sbi / (ig.nsb[1] * ig.nsb[2]);
-      const int bb = (sbi / ig.nsb[2]) % ig.nsb[1];
-      const int bc = sbi % ig.nsb[2];
-      int lo[3], hi[3];
-      sb_range(ba, ig.nae[0], &lo[0], &hi[0]);
-      sb_range(bb, ig.nae[1], &lo[1], &hi[1]);
-      sb_range(bc, ig.nae[2], &lo[2], &hi[2]);
-      for (int a = lo[0]; a <= hi[0]; ++a)
-        for (int bq = lo[1]; bq <= hi[1]; ++bq)
-          for (int cq = lo[2]; cq <= hi[2]; ++cq) {
-            const int site = (a * ig.nae[1] + bq) * ig.nae[2] + cq;
+    unsigned long long cm = __ballot(keep);
+    while (cm) {
+      const int src = __ffsll((long long)cm) - 1;
+      cm &= cm - 1;
+      const int cj = __shfl(j, src);
+      const int cimg = __shfl(img, src);
+      const bool self = (cj == i && cimg == kImgSelf);
+      int s[3];
+      img_shift(cimg, s);
+      double X[24];
+      for (int k = 0; k < 8; ++k)
+        for (int d = 0; d < 3; ++d) X[3 * k + d] = shifted(nodex[24 * (size_t)cj + 3 * k + d], s[d], b.prd[d]);
+      for (int b0 = 0; b0 < ig.nsbt; b0 += 64) {
+        const int sbi = b0 + lane;
+        bool sk = false;
+        if (sbi < ig.nsbt) {
+          const double *sb = sbc + 4 * ((size_t)cj * ig.nsbt + sbi);
+          const double dx = ip[0] - shifted(sb[0], s[0], b.prd[0]);
+          const double dy = ip[1] - shifted(sb[1], s[1], b.prd[1]);
+          const double dz = ip[2] - shifted(sb[2], s[2], b.prd[2]);
+          const double reach = cutneigh + sb[3];
+          sk = dx * dx + dy * dy + dz * dz <= reach * reach;
+        }
+        unsigned long long sm = __ballot(sk);
+        while (sm) {
+          const int sq = b0 + __ffsll((long long)sm) - 1;
+          sm &= sm - 1;
+          const int ba = sq / (ig.nsb[1] * ig.nsb[2]);
+          const int bb = (sq / ig.nsb[2]) % ig.nsb[1];
+          const int bc = sq % ig.nsb[2];
+          int lo[3], hi[3];
+          sb_range(ba, ig.nae[0], &lo[0], &hi[0]);
+          sb_range(bb, ig.nae[1], &lo[1], &hi[1]);
+          sb_range(bc, ig.nae[2], &lo[2], &hi[2]);
+          const int a = lo[0] + (lane >> 4), bq = lo[1] + ((lane >> 2) & 3), cq = lo[2] + (lane & 3);
+          bool acc = false;
+          int site = 0;
+          if (a <= hi[0] && bq <= hi[1] && cq <= hi[2]) {
+            site = (a * ig.nae[1] + bq) * ig.nae[2] + cq;
             double xs[3];
             interp_site(spa_site + 8 * site, X, xs);
-            double delx = xs[0] - ip[0], dely = xs[1] - ip[1], delz = xs[2] - ip[2];
-            double rsq = delx * delx + dely * dely + delz * delz;
-            if (rsq <= cutsq && (!self || rsq > 1.0e-4)) {
-              if (cnt < maxnip)
-                ipneigh[(size_t)cnt * nip + p] =
-                    (uint64_t)j | ((uint64_t)img << 27) | ((uint64_t)site << 32);
-              ++cnt;
-            }
+            const double delx = xs[0] - ip[0], dely = xs[1] - ip[1], delz = xs[2] - ip[2];
+            const double rsq = delx * delx + dely * dely + delz * delz;
+            acc = rsq <= cutsq && (!self || rsq > 1.0e-4);
           }
+          const unsigned long long am = __ballot(acc);
+          if (acc) {
+            const int pos = cnt + __popcll(am & ((1ull << lane) - 1ull));
+            if (pos < maxnip)
+              tmp[(size_t)p * maxnip + pos] = (uint64_t)cj | ((uint64_t)cimg << 27) | ((uint64_t)site << 32);
+            atomicOr(&bitmap[(size_t)cj * ig.nwords + (site >> 5)], 1u << (site & 31));
+          }
+          cnt += __popcll(am);
+        }
+      }
     }
   }
-  ipnum[p] = cnt;
-  if (cnt > maxnip) {
-    atomicOr(&flags[4], 1);
-    atomicMax(&flags[5], cnt);
+  if (lane == 0) {
+    ipnum[p] = cnt;
+    if (cnt > maxnip) {
+      atomicOr(&flags[4], 1);
+      atomicMax(&flags[5], cnt);
+    }
   }
+}
+
+// Active-site numbering: per element, the referenced sites in site order;
+// elements in index order.  wprefix[j][w] = first id of word w.
+__global__ void k_active_count(int ne, int nwords, const uint32_t *__restrict__ bitmap, int *__restrict__ wprefix,
+                               int *__restrict__ ecount) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ne) return;
+  int run = 0;
+  for (int w = 0; w < nwords; ++w) {
+    wprefix[(size_t)j * nwords + w] = run;
+    run += __popc(bitmap[(size_t)j * nwords + w]);
+  }
+  ecount[j] = run;
+}
+
+__global__ void k_active_finalize(int ne, int nwords, const uint32_t *__restrict__ bitmap,
+                                  const int *__restrict__ estart, int *__restrict__ wprefix,
+                                  int2 *__restrict__ asite) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ne) return;
+  const int base = estart[j];
+  int id = base;
+  for (int w = 0; w < nwords; ++w) {
+    wprefix[(size_t)j * nwords + w] += base;
+    uint32_t bits = bitmap[(size_t)j * nwords + w];
+    while (bits) {
+      const int bit = __ffs(bits) - 1;
+      bits &= bits - 1;
+      asite[id++] = make_int2(j, w * 32 + bit);
+    }
+  }
+}
+
+// Raw (element, image, site) entries -> (active id | image << 27) in the
+// blocked layout.
+__global__ void k_ip_translate(int nip, int maxnip, const uint64_t *__restrict__ tmp, const int *__restrict__ ipnum,
+                               int nwords, const uint32_t *__restrict__ bitmap, const int *__restrict__ wprefix,
+                               uint32_t *__restrict__ out) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)nip * maxnip) return;
+  const int p = (int)(t / maxnip), k = (int)(t % maxnip);
+  if (k >= ipnum[p]) return;
+  const uint64_t e = tmp[t];
+  const int j = (int)(e & 0x7FFFFFFu), img = (int)((e >> 27) & 31), site = (int)(e >> 32);
+  const size_t w = (size_t)j * nwords + (site >> 5);
+  const int aid = wprefix[w] + __popc(bitmap[w] & ((1u << (site & 31)) - 1u));
+  out[((size_t)(p >> 3) * maxnip + k) * kRowGroup + (p & 7)] = (uint32_t)aid | ((uint32_t)img << 27);
 }
 
 // Neighbor::check_distance, neighbor.cpp:1176-1217: flag if any atom or node
@@ -518,13 +602,15 @@ int launch_atom_neighbors(cac_ctx *ctx) {
   if ((rc = bin_points(ctx, n, ctx->x4, 4, g, ctx->b_akey, ctx->b_aval, ctx->b_akey2, ctx->b_aval2, ctx->b_acs,
                        ctx->b_ace, &sorted, &cs, &ce)))
     return rc;
-  if ((rc = ensure(ctx, ctx->b_aneigh, (size_t)ctx->maxneigh * n, &ctx->aneigh)) ||
-      (rc = ensure(ctx, ctx->b_anum, n, &ctx->anum)) || (rc = ensure(ctx, ctx->b_xhold, 3 * (size_t)n, &ctx->xhold)))
+  ctx->maxneigh = round_up(ctx->maxneigh, kLanes);
+  const size_t rows = (size_t)round_up(n, kRowGroup);
+  if ((rc = ensure(ctx, ctx->b_aneigh, (size_t)ctx->maxneigh * rows, &ctx->aneigh)) ||
+      (rc = ensure(ctx, ctx->b_anum, rows, &ctx->anum)) || (rc = ensure(ctx, ctx->b_xhold, 3 * (size_t)n, &ctx->xhold)))
     return rc;
   hipEvent_t t0;
   timer_begin(ctx, "atom_neigh", &t0);
-  hipLaunchKernelGGL(k_atom_neigh, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, n, ctx->x4, box_of(ctx), g,
-                     sorted, cs, ce, ctx->cutneigh * ctx->cutneigh, ctx->maxneigh, ctx->aneigh, ctx->anum,
+  hipLaunchKernelGGL(k_atom_neigh, dim3(xcd_grid(n, kBlock)), dim3(kBlock), 0, ctx->stream, n, ctx->x4, box_of(ctx),
+                     g, sorted, cs, ce, ctx->cutneigh * ctx->cutneigh, ctx->maxneigh, ctx->aneigh, ctx->anum,
                      ctx->xhold, ctx->dflags);
   timer_end(ctx, "atom_neigh", t0);
   CAC_HIP(ctx, hipGetLastError());
@@ -596,10 +682,16 @@ int launch_ip_neighbors(cac_ctx *ctx) {
   if (ne == 0) return CAC_OK;
   const int nip = ne * ctx->nint;
   int rc;
-  if ((rc = ensure(ctx, ctx->b_ipneigh, (size_t)ctx->maxnip * nip, &ctx->ipneigh)) ||
-      (rc = ensure(ctx, ctx->b_ipnum, nip, &ctx->ipnum)) ||
+  ctx->maxnip = round_up(ctx->maxnip, kLanes);
+  ctx->nwords = (ctx->nsite + 31) / 32;
+  const size_t nbits = (size_t)ne * ctx->nwords;
+  if ((rc = ensure(ctx, ctx->b_iptmp, (size_t)ctx->maxnip * nip, &ctx->iptmp)) ||
+      (rc = ensure(ctx, ctx->b_ipnum, (size_t)round_up(nip, kRowGroup), &ctx->ipnum)) ||
+      (rc = ensure(ctx, ctx->b_bitmap, nbits, &ctx->bitmap)) ||
       (rc = ensure(ctx, ctx->b_nodexhold, 24 * (size_t)ne, &ctx->nodexhold)))
     return rc;
+  hipLaunchKernelGGL(k_fill_int, dim3(grid_for((long)nbits)), dim3(kBlock), 0, ctx->stream, (int)nbits,
+                     reinterpret_cast<int *>(ctx->bitmap), 0);
   IpGeom ig;
   ig.ne = ne;
   ig.nint = ctx->nint;
@@ -608,14 +700,54 @@ int launch_ip_neighbors(cac_ctx *ctx) {
     ig.nsb[d] = ctx->nsb[d];
   }
   ig.nsbt = ctx->nsb_total;
+  ig.nwords = ctx->nwords;
   hipEvent_t t0;
   timer_begin(ctx, "ip_neigh", &t0);
-  hipLaunchKernelGGL(k_ip_neigh, dim3(grid_for(nip, 64)), dim3(64), 0, ctx->stream, ig, ctx->nodex, ctx->spa_int,
-                     ctx->spa_site, ctx->erad, ctx->sbc, ctx->eneigh, ctx->enumb, box_of(ctx), ctx->cutneigh,
-                     ctx->maxnip, ctx->ipneigh, ctx->ipnum, ctx->dflags);
+  hipLaunchKernelGGL(k_ip_neigh, dim3(xcd_grid(64L * nip, kBlock)), dim3(kBlock), 0, ctx->stream, ig, ctx->nodex,
+                     ctx->spa_int, ctx->spa_site, ctx->erad, ctx->sbc, ctx->eneigh, ctx->enumb, box_of(ctx),
+                     ctx->cutneigh, ctx->maxnip, ctx->iptmp, ctx->ipnum, ctx->bitmap, ctx->dflags);
   timer_end(ctx, "ip_neigh", t0);
   CAC_HIP(ctx, hipMemcpyAsync(ctx->nodexhold, ctx->nodex, sizeof(double) * 24 * (size_t)ne,
                               hipMemcpyDeviceToDevice, ctx->stream));
+  CAC_HIP(ctx, hipGetLastError());
+  return CAC_OK;
+}
+
+// After a successful search: number the referenced sites, build the
+// active-site table and rewrite the lists as (active id | image << 27).
+int launch_active_sites(cac_ctx *ctx) {
+  const int ne = ctx->ne;
+  if (ne == 0) return CAC_OK;
+  const int nip = ne * ctx->nint;
+  const int nw = ctx->nwords;
+  int rc;
+  if ((rc = ensure(ctx, ctx->b_wprefix, (size_t)ne * nw, &ctx->wprefix)) ||
+      (rc = ensure(ctx, ctx->b_ecount, (size_t)ne + 1, &ctx->ecount)) ||
+      (rc = ensure(ctx, ctx->b_estart, (size_t)ne + 1, &ctx->estart)))
+    return rc;
+  int *estart = ctx->estart;
+  hipLaunchKernelGGL(k_active_count, dim3(grid_for(ne)), dim3(kBlock), 0, ctx->stream, ne, nw, ctx->bitmap,
+                     ctx->wprefix, ctx->ecount);
+  size_t tmp_bytes = 0;
+  CAC_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, ctx->ecount, estart, ne + 1, ctx->stream));
+  char *tmp;
+  if ((rc = ensure(ctx, ctx->b_scantmp, tmp_bytes, &tmp))) return rc;
+  // ecount[ne] = 0 so estart[ne] is the total
+  CAC_HIP(ctx, hipMemsetAsync(ctx->ecount + ne, 0, sizeof(int), ctx->stream));
+  CAC_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, ctx->ecount, estart, ne + 1, ctx->stream));
+  int total = 0;
+  CAC_HIP(ctx, hipMemcpyAsync(&total, estart + ne, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->nactive = total;
+  if ((rc = ensure(ctx, ctx->b_asite, (size_t)total, &ctx->asite)) ||
+      (rc = ensure(ctx, ctx->b_xs4, 4 * (size_t)total, &ctx->xs4)) ||
+      (rc = ensure(ctx, ctx->b_ipneigh, (size_t)ctx->maxnip * round_up(nip, kRowGroup), &ctx->ipneigh)))
+    return rc;
+  hipLaunchKernelGGL(k_active_finalize, dim3(grid_for(ne)), dim3(kBlock), 0, ctx->stream, ne, nw, ctx->bitmap,
+                     estart, ctx->wprefix, ctx->asite);
+  const long tot = (long)nip * ctx->maxnip;
+  hipLaunchKernelGGL(k_ip_translate, dim3(grid_for(tot)), dim3(kBlock), 0, ctx->stream, nip, ctx->maxnip, ctx->iptmp,
+                     ctx->ipnum, nw, ctx->bitmap, ctx->wprefix, ctx->ipneigh);
   CAC_HIP(ctx, hipGetLastError());
   return CAC_OK;
 }

@@ -5,14 +5,25 @@
 // 927-1137; pair_lj_cut.cpp:227-288, 295-442), compiled with
 // -ffp-contract=off.  Results differ from the CPU oracle only through the
 // order of the neighbor sums.
-// MIXED path (Prec = float): r, the spline polynomials, F'(rho) and the pair
+// MIXED path (P = float): r, the spline polynomials, F'(rho) and the pair
 // force are evaluated in FP32; positions, site interpolation, rho, force,
 // energy and virial sums stay FP64.
 //
-// Table layout on the device: one 16-wide row per r-knot, rhor c0..c6 in
-// slots 0..6 and z2r c0..c6 in slots 8..14, so the force pass reads both
-// splines of a pair from one 128-byte line (64 bytes in FP32); frho rows are
-// 8 wide.
+// Work mapping: 8 lanes per atom / node / integration point.  Lane l takes
+// list entries l, l+8, ...; the blocked list layout makes the 64 lanes of a
+// wavefront read 64 consecutive entries per iteration.  Partial sums are
+// combined with a fixed xor butterfly (every lane ends with the same bits),
+// so results are reproducible run to run.
+//
+// Element path: the sites that some integration-point list refers to (the
+// "active sites", numbered at rebuild) are interpolated once per step
+// (k_site_pos: x = sum_k N_k X_k; k_site_fp: rho = sum_k N_k erho_k and
+// F'(rho)), so the erho and force passes gather 32 bytes per entry, like the
+// atom passes.  The image of a neighbor site is x_site + s*prd.
+//
+// Tables on the device: one 16-wide row per r-knot, rhor c0..c6 in slots
+// 0..6 and z2r c0..c6 in slots 8..14, so a pair reads both splines from one
+// 128-byte line (64 bytes in FP32); frho rows are 8 wide.
 #include "cac_internal.h"
 
 namespace cac {
@@ -93,346 +104,443 @@ __device__ __forceinline__ double sqrt_p<double>(double rsq) { return sqrt(rsq);
 template <>
 __device__ __forceinline__ float sqrt_p<float>(double rsq) { return sqrtf((float)rsq); }
 
+// Sum over the 8 lanes of a group; the xor butterfly pairs the same partial
+// sums on every lane, so all lanes hold identical bits.
+__device__ __forceinline__ double group_sum(double v) {
+  v += __shfl_xor(v, 1, kLanes);
+  v += __shfl_xor(v, 2, kLanes);
+  v += __shfl_xor(v, 4, kLanes);
+  return v;
+}
+
+__device__ __forceinline__ const uint32_t *list_row(const uint32_t *base, int r, int M) {
+  return base + (size_t)(r >> 3) * M * kRowGroup + (r & 7);
+}
+
+struct Pt {
+  double x, y, z;
+};
+
+__device__ __forceinline__ Pt load_xyz(const double *__restrict__ x4, int j, const int s[3], const Prd &prd) {
+  const double2 xy = reinterpret_cast<const double2 *>(x4)[2 * (size_t)j];
+  const double z = x4[4 * (size_t)j + 2];
+  return Pt{shifted(xy.x, s[0], prd.v[0]), shifted(xy.y, s[1], prd.v[1]), shifted(z, s[2], prd.v[2])};
+}
+
+// position and fourth slot (fp) of particle j, image s
+__device__ __forceinline__ Pt load_xyzw(const double *__restrict__ x4, int j, const int s[3], const Prd &prd,
+                                        double *w) {
+  const double2 xy = reinterpret_cast<const double2 *>(x4)[2 * (size_t)j];
+  const double2 zw = reinterpret_cast<const double2 *>(x4)[2 * (size_t)j + 1];
+  *w = zw.y;
+  return Pt{shifted(xy.x, s[0], prd.v[0]), shifted(xy.y, s[1], prd.v[1]), shifted(zw.x, s[2], prd.v[2])};
+}
+
+struct Virial {
+  double v[6];
+  __device__ void zero() {
+    for (int c = 0; c < 6; ++c) v[c] = 0.0;
+  }
+  __device__ void add(double dx, double dy, double dz, double fpair) {
+    v[0] += 0.5 * dx * dx * fpair;
+    v[1] += 0.5 * dy * dy * fpair;
+    v[2] += 0.5 * dz * dz * fpair;
+    v[3] += 0.5 * dx * dy * fpair;
+    v[4] += 0.5 * dx * dz * fpair;
+    v[5] += 0.5 * dy * dz * fpair;
+  }
+  __device__ void reduce() {
+    for (int c = 0; c < 6; ++c) v[c] = group_sum(v[c]);
+  }
+  __device__ void store(double *out, size_t stride, size_t idx, double w) const {
+    for (int c = 0; c < 6; ++c) out[c * stride + idx] = w * v[c];
+  }
+};
+
+// EAM pair terms shared by the atom and integration-point force passes,
+// pair_eam.cpp:706-738 / 1083-1120.
+template <class P>
+__device__ __forceinline__ double eam_fpair(const Tabs<P> &t, double rsq, P fpi, P fpj, P *phi_out) {
+  P r = sqrt_p<P>(rsq), p;
+  const P *c = rrow<P>(t, r, &p);
+  P rhoip = (c[0] * p + c[1]) * p + c[2];
+  P z2p = (c[8] * p + c[9]) * p + c[10];
+  P z2 = ((c[11] * p + c[12]) * p + c[13]) * p + c[14];
+  P recip = P(1.0) / r;
+  P phi = z2 * recip;
+  P phip = z2p * recip - phi * recip;
+  P psip = fpi * rhoip + fpj * rhoip + phip;
+  *phi_out = phi;
+  return (double)(-psip * recip);
+}
+
 // ------------------------------------------------------------- atoms, EAM
 // rho pass + fp pass fused (pair_eam.cpp:629-680); fp stored in x4.w, which
 // the force pass reads for ghosts too (forward_comm_pair, :683).
-template <class P>
-__global__ void k_eam_atom_rho(int n, double *__restrict__ x4, Prd prd, const uint32_t *__restrict__ neigh,
-                               const int *__restrict__ num, Tabs<P> t, double *__restrict__ rho_out,
-                               double *__restrict__ eatom) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double xi = x4[4 * i], yi = x4[4 * i + 1], zi = x4[4 * i + 2];
-  const int jn = num[i];
+template <class P, bool EV>
+__global__ __launch_bounds__(kBlock) void k_eam_atom_rho(int n, double *__restrict__ x4, Prd prd,
+                                                         const uint32_t *__restrict__ neigh, int M,
+                                                         const int *__restrict__ num, Tabs<P> t,
+                                                         double *__restrict__ rho_out, double *__restrict__ eatom) {
+  const int gt = xcd_block() * kBlock + (int)threadIdx.x;
+  const int i = gt >> 3, l = gt & 7;
   double rho = 0.0;
-  for (int k = 0; k < jn; ++k) {
-    uint32_t e = neigh[(size_t)k * n + i];
-    int j = (int)(e & 0x7FFFFFFu), s[3];
-    img_shift((int)(e >> 27), s);
-    double delx = xi - shifted(x4[4 * j], s[0], prd.v[0]);
-    double dely = yi - shifted(x4[4 * j + 1], s[1], prd.v[1]);
-    double delz = zi - shifted(x4[4 * j + 2], s[2], prd.v[2]);
-    double rsq = delx * delx + dely * dely + delz * delz;
-    if (rsq < t.cutforcesq) {
-      P p;
-      const P *c = rrow<P>(t, sqrt_p<P>(rsq), &p);
-      rho += (double)(((c[3] * p + c[4]) * p + c[5]) * p + c[6]);
-    }
-  }
-  rho_out[i] = rho;
-  double fp = (double)embed_fp<P>(t, rho);
-  x4[4 * i + 3] = fp;
-  if (eatom) eatom[i] = embed_energy<P>(t, rho, fp);
-}
-
-// force pass without Newton's third law, pair_eam.cpp:688-745
-template <class P>
-__global__ void k_eam_atom_force(int n, const double *__restrict__ x4, Prd prd, const uint32_t *__restrict__ neigh,
-                                 const int *__restrict__ num, Tabs<P> t, double *__restrict__ f,
-                                 double *__restrict__ eatom, double *__restrict__ vatom) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double xi = x4[4 * i], yi = x4[4 * i + 1], zi = x4[4 * i + 2];
-  const P fpi = (P)x4[4 * i + 3];
-  const int jn = num[i];
-  double fx = 0.0, fy = 0.0, fz = 0.0, ep = 0.0;
-  double v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
-  for (int k = 0; k < jn; ++k) {
-    uint32_t e = neigh[(size_t)k * n + i];
-    int j = (int)(e & 0x7FFFFFFu), s[3];
-    img_shift((int)(e >> 27), s);
-    double delx = xi - shifted(x4[4 * j], s[0], prd.v[0]);
-    double dely = yi - shifted(x4[4 * j + 1], s[1], prd.v[1]);
-    double delz = zi - shifted(x4[4 * j + 2], s[2], prd.v[2]);
-    double rsq = delx * delx + dely * dely + delz * delz;
-    if (rsq < t.cutforcesq) {
-      P r = sqrt_p<P>(rsq), p;
-      const P *c = rrow<P>(t, r, &p);
-      P rhoip = (c[0] * p + c[1]) * p + c[2];
-      P z2p = (c[8] * p + c[9]) * p + c[10];
-      P z2 = ((c[11] * p + c[12]) * p + c[13]) * p + c[14];
-      P recip = P(1.0) / r;
-      P phi = z2 * recip;
-      P phip = z2p * recip - phi * recip;
-      P psip = fpi * rhoip + (P)x4[4 * j + 3] * rhoip + phip;
-      double fpair = (double)(-psip * recip);
-      fx += delx * fpair;
-      fy += dely * fpair;
-      fz += delz * fpair;
-      if (eatom) ep += 0.5 * (double)phi;
-      if (vatom) {
-        v0 += 0.5 * delx * delx * fpair;
-        v1 += 0.5 * dely * dely * fpair;
-        v2 += 0.5 * delz * delz * fpair;
-        v3 += 0.5 * delx * dely * fpair;
-        v4 += 0.5 * delx * delz * fpair;
-        v5 += 0.5 * dely * delz * fpair;
+  if (i < n) {
+    const double xi = x4[4 * (size_t)i], yi = x4[4 * (size_t)i + 1], zi = x4[4 * (size_t)i + 2];
+    const int jn = num[i];
+    const uint32_t *row = list_row(neigh, i, M);
+    for (int k = l; k < jn; k += kLanes) {
+      const uint32_t e = row[(size_t)k * kRowGroup];
+      int s[3];
+      img_shift((int)(e >> 27), s);
+      const Pt xj = load_xyz(x4, (int)(e & 0x7FFFFFFu), s, prd);
+      const double delx = xi - xj.x, dely = yi - xj.y, delz = zi - xj.z;
+      const double rsq = delx * delx + dely * dely + delz * delz;
+      if (rsq < t.cutforcesq) {
+        P p;
+        const P *c = rrow<P>(t, sqrt_p<P>(rsq), &p);
+        rho += (double)(((c[3] * p + c[4]) * p + c[5]) * p + c[6]);
       }
     }
   }
-  f[i] = fx;
-  f[n + i] = fy;
-  f[2 * (size_t)n + i] = fz;
-  if (eatom) eatom[i] += ep;
-  if (vatom) {
-    vatom[i] = v0;
-    vatom[(size_t)n + i] = v1;
-    vatom[2 * (size_t)n + i] = v2;
-    vatom[3 * (size_t)n + i] = v3;
-    vatom[4 * (size_t)n + i] = v4;
-    vatom[5 * (size_t)n + i] = v5;
+  rho = group_sum(rho);
+  if (i < n && l == 0) {
+    rho_out[i] = rho;
+    const double fp = (double)embed_fp<P>(t, rho);
+    x4[4 * (size_t)i + 3] = fp;
+    if (EV) eatom[i] = embed_energy<P>(t, rho, fp);
+  }
+}
+
+// force pass without Newton's third law, pair_eam.cpp:688-745
+template <class P, bool EV>
+__global__ __launch_bounds__(kBlock) void k_eam_atom_force(int n, const double *__restrict__ x4, Prd prd,
+                                                           const uint32_t *__restrict__ neigh, int M,
+                                                           const int *__restrict__ num, Tabs<P> t,
+                                                           double *__restrict__ f, double *__restrict__ eatom,
+                                                           double *__restrict__ vatom) {
+  const int gt = xcd_block() * kBlock + (int)threadIdx.x;
+  const int i = gt >> 3, l = gt & 7;
+  double fx = 0.0, fy = 0.0, fz = 0.0, ep = 0.0;
+  Virial vir;
+  vir.zero();
+  if (i < n) {
+    const double xi = x4[4 * (size_t)i], yi = x4[4 * (size_t)i + 1], zi = x4[4 * (size_t)i + 2];
+    const P fpi = (P)x4[4 * (size_t)i + 3];
+    const int jn = num[i];
+    const uint32_t *row = list_row(neigh, i, M);
+    for (int k = l; k < jn; k += kLanes) {
+      const uint32_t e = row[(size_t)k * kRowGroup];
+      int s[3];
+      img_shift((int)(e >> 27), s);
+      double fpj;
+      const Pt xj = load_xyzw(x4, (int)(e & 0x7FFFFFFu), s, prd, &fpj);
+      const double delx = xi - xj.x, dely = yi - xj.y, delz = zi - xj.z;
+      const double rsq = delx * delx + dely * dely + delz * delz;
+      if (rsq < t.cutforcesq) {
+        P phi;
+        const double fpair = eam_fpair<P>(t, rsq, fpi, (P)fpj, &phi);
+        fx += delx * fpair;
+        fy += dely * fpair;
+        fz += delz * fpair;
+        if (EV) {
+          ep += 0.5 * (double)phi;
+          vir.add(delx, dely, delz, fpair);
+        }
+      }
+    }
+  }
+  fx = group_sum(fx);
+  fy = group_sum(fy);
+  fz = group_sum(fz);
+  if (EV) {
+    ep = group_sum(ep);
+    vir.reduce();
+  }
+  if (i < n && l == 0) {
+    f[i] = fx;
+    f[(size_t)n + i] = fy;
+    f[2 * (size_t)n + i] = fz;
+    if (EV) {
+      eatom[i] += ep;
+      vir.store(vatom, (size_t)n, (size_t)i, 1.0);
+    }
   }
 }
 
 // ------------------------------------------------------------- atoms, LJ
-template <class P>
-__global__ void k_lj_atom_force(int n, const double *__restrict__ x4, Prd prd, const uint32_t *__restrict__ neigh,
-                                const int *__restrict__ num, LjParams lj, double *__restrict__ f,
-                                double *__restrict__ eatom, double *__restrict__ vatom) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double xi = x4[4 * i], yi = x4[4 * i + 1], zi = x4[4 * i + 2];
+// PairLJCut::compute, pair_lj_cut.cpp:255-287
+template <class P, bool EV>
+__global__ __launch_bounds__(kBlock) void k_lj_atom_force(int n, const double *__restrict__ x4, Prd prd,
+                                                          const uint32_t *__restrict__ neigh, int M,
+                                                          const int *__restrict__ num, LjParams lj,
+                                                          double *__restrict__ f, double *__restrict__ eatom,
+                                                          double *__restrict__ vatom) {
+  const int gt = xcd_block() * kBlock + (int)threadIdx.x;
+  const int i = gt >> 3, l = gt & 7;
   const P lj1 = (P)lj.lj1, lj2 = (P)lj.lj2, lj3 = (P)lj.lj3, lj4 = (P)lj.lj4;
-  const int jn = num[i];
   double fx = 0.0, fy = 0.0, fz = 0.0, ep = 0.0;
-  double v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
-  for (int k = 0; k < jn; ++k) {
-    uint32_t e = neigh[(size_t)k * n + i];
-    int j = (int)(e & 0x7FFFFFFu), s[3];
-    img_shift((int)(e >> 27), s);
-    double delx = xi - shifted(x4[4 * j], s[0], prd.v[0]);
-    double dely = yi - shifted(x4[4 * j + 1], s[1], prd.v[1]);
-    double delz = zi - shifted(x4[4 * j + 2], s[2], prd.v[2]);
-    double rsq = delx * delx + dely * dely + delz * delz;
-    if (rsq < lj.cutsq) {
-      P r2inv = P(1.0) / (P)rsq;
-      P r6inv = r2inv * r2inv * r2inv;
-      P forcelj = r6inv * (lj1 * r6inv - lj2);
-      double fpair = (double)(P(1.0) * forcelj * r2inv);
-      fx += delx * fpair;
-      fy += dely * fpair;
-      fz += delz * fpair;
-      if (eatom) ep += 0.5 * (double)(r6inv * (lj3 * r6inv - lj4));
-      if (vatom) {
-        v0 += 0.5 * delx * delx * fpair;
-        v1 += 0.5 * dely * dely * fpair;
-        v2 += 0.5 * delz * delz * fpair;
-        v3 += 0.5 * delx * dely * fpair;
-        v4 += 0.5 * delx * delz * fpair;
-        v5 += 0.5 * dely * delz * fpair;
+  Virial vir;
+  vir.zero();
+  if (i < n) {
+    const double xi = x4[4 * (size_t)i], yi = x4[4 * (size_t)i + 1], zi = x4[4 * (size_t)i + 2];
+    const int jn = num[i];
+    const uint32_t *row = list_row(neigh, i, M);
+    for (int k = l; k < jn; k += kLanes) {
+      const uint32_t e = row[(size_t)k * kRowGroup];
+      int s[3];
+      img_shift((int)(e >> 27), s);
+      const Pt xj = load_xyz(x4, (int)(e & 0x7FFFFFFu), s, prd);
+      const double delx = xi - xj.x, dely = yi - xj.y, delz = zi - xj.z;
+      const double rsq = delx * delx + dely * dely + delz * delz;
+      if (rsq < lj.cutsq) {
+        P r2inv = P(1.0) / (P)rsq;
+        P r6inv = r2inv * r2inv * r2inv;
+        P forcelj = r6inv * (lj1 * r6inv - lj2);
+        double fpair = (double)(P(1.0) * forcelj * r2inv);
+        fx += delx * fpair;
+        fy += dely * fpair;
+        fz += delz * fpair;
+        if (EV) {
+          ep += 0.5 * (double)(r6inv * (lj3 * r6inv - lj4));
+          vir.add(delx, dely, delz, fpair);
+        }
       }
     }
   }
-  f[i] = fx;
-  f[n + i] = fy;
-  f[2 * (size_t)n + i] = fz;
-  if (eatom) eatom[i] = ep;
-  if (vatom) {
-    vatom[i] = v0;
-    vatom[(size_t)n + i] = v1;
-    vatom[2 * (size_t)n + i] = v2;
-    vatom[3 * (size_t)n + i] = v3;
-    vatom[4 * (size_t)n + i] = v4;
-    vatom[5 * (size_t)n + i] = v5;
+  fx = group_sum(fx);
+  fy = group_sum(fy);
+  fz = group_sum(fz);
+  if (EV) {
+    ep = group_sum(ep);
+    vir.reduce();
+  }
+  if (i < n && l == 0) {
+    f[i] = fx;
+    f[(size_t)n + i] = fy;
+    f[2 * (size_t)n + i] = fz;
+    if (EV) {
+      eatom[i] = ep;
+      vir.store(vatom, (size_t)n, (size_t)i, 1.0);
+    }
   }
 }
 
 // ------------------------------------------------------------- elements
 struct ElemArgs {
-  int ne, nint, nip;
-  const double *nodex;     // [ne][8][3]
-  const double *spa_site;  // [nsite][8]
-  const double *spa_int;   // [nint][8]
-  const double *weight;    // [nint]
-  const uint64_t *ipneigh; // [maxnip][nip]
-  const int *ipnum;        // [nip]
+  int ne, nint, nip, nactive, M;
+  const double *nodex;       // [ne][8][3]
+  const double *spa_site;    // [nsite][8]
+  const double *spa_int;     // [nint][8]
+  const double *weight;      // [nint]
+  const uint32_t *ipneigh;   // blocked, entries (active id | img << 27)
+  const int *ipnum;          // [nip]
+  const int2 *asite;         // [nactive] (element, site)
   int node_list[8];
 };
 
-__device__ __forceinline__ void load_shifted_nodes(const double *__restrict__ nodex, int j, int img, const Prd &prd,
-                                                   double X[24]) {
-  int s[3];
-  img_shift(img, s);
-  const double *src = nodex + 24 * (size_t)j;
-  for (int k = 0; k < 8; ++k)
-    for (int d = 0; d < 3; ++d) X[3 * k + d] = shifted(src[3 * k + d], s[d], prd.v[d]);
+// Active-site positions: sum_k N_k(site) X_k in node order
+// (pair_eam.cpp:1071-1076, element.cpp:1251-1271).
+__global__ __launch_bounds__(kBlock) void k_site_pos(ElemArgs a, double *__restrict__ xs4) {
+  const int q = xcd_block() * kBlock + (int)threadIdx.x;
+  if (q >= a.nactive) return;
+  const int2 js = a.asite[q];
+  const double *X = a.nodex + 24 * (size_t)js.x;
+  const double *N = a.spa_site + 8 * (size_t)js.y;
+  double x = 0.0, y = 0.0, z = 0.0;
+  for (int k = 0; k < 8; ++k) {
+    x += N[k] * X[3 * k];
+    y += N[k] * X[3 * k + 1];
+    z += N[k] * X[3 * k + 2];
+  }
+  reinterpret_cast<double2 *>(xs4)[2 * (size_t)q] = make_double2(x, y);
+  xs4[4 * (size_t)q + 2] = z;
+}
+
+// Active-site density and embedding derivative: rho_s = sum_k N_k erho_k,
+// F'(rho_s) (pair_eam.cpp:1077, 1089-1096).
+template <class P>
+__global__ __launch_bounds__(kBlock) void k_site_fp(ElemArgs a, Tabs<P> t, const double *__restrict__ erho,
+                                                    double *__restrict__ xs4) {
+  const int q = xcd_block() * kBlock + (int)threadIdx.x;
+  if (q >= a.nactive) return;
+  const int2 js = a.asite[q];
+  const double *E = erho + 8 * (size_t)js.x;
+  const double *N = a.spa_site + 8 * (size_t)js.y;
+  double rho = 0.0;
+  for (int k = 0; k < 8; ++k) rho += N[k] * E[k];
+  xs4[4 * (size_t)q + 3] = (double)embed_fp<P>(t, rho);
 }
 
 // Node density, pair_eam.cpp:927-986: erho[i][k] from the list of the
 // integration point at node k, measured from the node itself.
 template <class P>
-__global__ void k_eam_erho(ElemArgs a, Prd prd, Tabs<P> t, double *__restrict__ erho) {
-  int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= 8 * a.ne) return;
-  const int i = q >> 3, k = q & 7;
-  const double ix = a.nodex[24 * (size_t)i + 3 * k], iy = a.nodex[24 * (size_t)i + 3 * k + 1],
-               iz = a.nodex[24 * (size_t)i + 3 * k + 2];
-  const int p = i * a.nint + a.node_list[k];
-  const int jn = a.ipnum[p];
+__global__ __launch_bounds__(kBlock) void k_eam_erho(ElemArgs a, Prd prd, Tabs<P> t, const double *__restrict__ xs4,
+                                                     double *__restrict__ erho) {
+  const int gt = xcd_block() * kBlock + (int)threadIdx.x;
+  const int q = gt >> 3, l = gt & 7;
   double acc = 0.0;
-  for (int l = 0; l < jn; ++l) {
-    uint64_t e = a.ipneigh[(size_t)l * a.nip + p];
-    const int j = (int)(e & 0x7FFFFFFu), img = (int)((e >> 27) & 31), site = (int)(e >> 32);
-    double X[24];
-    load_shifted_nodes(a.nodex, j, img, prd, X);
-    const double *N = a.spa_site + 8 * (size_t)site;
-    double jx = 0.0, jy = 0.0, jz = 0.0;
-    for (int kk = 0; kk < 8; ++kk) {
-      jx += N[kk] * X[3 * kk];
-      jy += N[kk] * X[3 * kk + 1];
-      jz += N[kk] * X[3 * kk + 2];
-    }
-    double delx = ix - jx, dely = iy - jy, delz = iz - jz;
-    double rsq = delx * delx + dely * dely + delz * delz;
-    if (rsq < t.cutforcesq) {
-      P pp;
-      const P *c = rrow<P>(t, sqrt_p<P>(rsq), &pp);
-      acc += (double)(((c[3] * pp + c[4]) * pp + c[5]) * pp + c[6]);
+  if (q < 8 * a.ne) {
+    const int i = q >> 3, k = q & 7;
+    const double ix = a.nodex[24 * (size_t)i + 3 * k], iy = a.nodex[24 * (size_t)i + 3 * k + 1],
+                 iz = a.nodex[24 * (size_t)i + 3 * k + 2];
+    const int p = i * a.nint + a.node_list[k];
+    const int jn = a.ipnum[p];
+    const uint32_t *row = list_row(a.ipneigh, p, a.M);
+    for (int kk = l; kk < jn; kk += kLanes) {
+      const uint32_t e = row[(size_t)kk * kRowGroup];
+      int s[3];
+      img_shift((int)(e >> 27), s);
+      const Pt xj = load_xyz(xs4, (int)(e & 0x7FFFFFFu), s, prd);
+      const double delx = ix - xj.x, dely = iy - xj.y, delz = iz - xj.z;
+      const double rsq = delx * delx + dely * dely + delz * delz;
+      if (rsq < t.cutforcesq) {
+        P pp;
+        const P *c = rrow<P>(t, sqrt_p<P>(rsq), &pp);
+        acc += (double)(((c[3] * pp + c[4]) * pp + c[5]) * pp + c[6]);
+      }
     }
   }
-  erho[q] = acc;
+  acc = group_sum(acc);
+  if (q < 8 * a.ne && l == 0) erho[q] = acc;
 }
 
 // Integration-point force, pair_eam.cpp:1013-1129.  Writes f_g to ipf
 // ([3][nip]) and the weighted energy / virial tallies.
-template <class P>
-__global__ void k_eam_ip_force(ElemArgs a, Prd prd, Tabs<P> t, const double *__restrict__ erho,
-                               double *__restrict__ ipf, double *__restrict__ eip, double *__restrict__ vip) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= a.nip) return;
-  const int i = p / a.nint, g = p % a.nint;
-  const double *Ng = a.spa_int + 8 * g;
-  double ix = 0.0, iy = 0.0, iz = 0.0, rhoi = 0.0;
-  for (int kk = 0; kk < 8; ++kk) {
-    ix += Ng[kk] * a.nodex[24 * (size_t)i + 3 * kk];
-    iy += Ng[kk] * a.nodex[24 * (size_t)i + 3 * kk + 1];
-    iz += Ng[kk] * a.nodex[24 * (size_t)i + 3 * kk + 2];
-    rhoi += Ng[kk] * erho[8 * (size_t)i + kk];
-  }
-  const P fpi = embed_fp<P>(t, rhoi);
-  const int jn = a.ipnum[p];
-  double fx = 0.0, fy = 0.0, fz = 0.0, ep = 0.0;
-  double v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
-  for (int l = 0; l < jn; ++l) {
-    uint64_t e = a.ipneigh[(size_t)l * a.nip + p];
-    const int j = (int)(e & 0x7FFFFFFu), img = (int)((e >> 27) & 31), site = (int)(e >> 32);
-    double X[24];
-    load_shifted_nodes(a.nodex, j, img, prd, X);
-    const double *N = a.spa_site + 8 * (size_t)site;
-    const double *ej = erho + 8 * (size_t)j;
-    double jx = 0.0, jy = 0.0, jz = 0.0, rhoj = 0.0;
+template <class P, bool EV>
+__global__ __launch_bounds__(kBlock) void k_eam_ip_force(ElemArgs a, Prd prd, Tabs<P> t,
+                                                         const double *__restrict__ erho,
+                                                         const double *__restrict__ xs4, double *__restrict__ ipf,
+                                                         double *__restrict__ eip, double *__restrict__ vip) {
+  const int gt = xcd_block() * kBlock + (int)threadIdx.x;
+  const int p = gt >> 3, l = gt & 7;
+  double fx = 0.0, fy = 0.0, fz = 0.0, ep = 0.0, rhoi = 0.0;
+  P fpi = P(0);
+  Virial vir;
+  vir.zero();
+  if (p < a.nip) {
+    const int i = p / a.nint, g = p % a.nint;
+    const double *Ng = a.spa_int + 8 * g;
+    double ix = 0.0, iy = 0.0, iz = 0.0;
     for (int kk = 0; kk < 8; ++kk) {
-      jx += N[kk] * X[3 * kk];
-      jy += N[kk] * X[3 * kk + 1];
-      jz += N[kk] * X[3 * kk + 2];
-      rhoj += N[kk] * ej[kk];
+      ix += Ng[kk] * a.nodex[24 * (size_t)i + 3 * kk];
+      iy += Ng[kk] * a.nodex[24 * (size_t)i + 3 * kk + 1];
+      iz += Ng[kk] * a.nodex[24 * (size_t)i + 3 * kk + 2];
+      rhoi += Ng[kk] * erho[8 * (size_t)i + kk];
     }
-    double delx = ix - jx, dely = iy - jy, delz = iz - jz;
-    double rsq = delx * delx + dely * dely + delz * delz;
-    if (rsq < t.cutforcesq) {
-      P fpj = embed_fp<P>(t, rhoj);
-      P r = sqrt_p<P>(rsq), pp;
-      const P *c = rrow<P>(t, r, &pp);
-      P rhoip = (c[0] * pp + c[1]) * pp + c[2];
-      P z2p = (c[8] * pp + c[9]) * pp + c[10];
-      P z2 = ((c[11] * pp + c[12]) * pp + c[13]) * pp + c[14];
-      P recip = P(1.0) / r;
-      P phi = z2 * recip;
-      P phip = z2p * recip - phi * recip;
-      P psip = (fpi + fpj) * rhoip + phip;
-      double fpair = (double)(-psip * recip);
-      fx += delx * fpair;
-      fy += dely * fpair;
-      fz += delz * fpair;
-      if (eip) ep += 0.5 * (double)phi;
-      if (vip) {
-        v0 += 0.5 * delx * delx * fpair;
-        v1 += 0.5 * dely * dely * fpair;
-        v2 += 0.5 * delz * delz * fpair;
-        v3 += 0.5 * delx * dely * fpair;
-        v4 += 0.5 * delx * delz * fpair;
-        v5 += 0.5 * dely * delz * fpair;
+    fpi = embed_fp<P>(t, rhoi);
+    const int jn = a.ipnum[p];
+    const uint32_t *row = list_row(a.ipneigh, p, a.M);
+    for (int kk = l; kk < jn; kk += kLanes) {
+      const uint32_t e = row[(size_t)kk * kRowGroup];
+      int s[3];
+      img_shift((int)(e >> 27), s);
+      double fpj;
+      const Pt xj = load_xyzw(xs4, (int)(e & 0x7FFFFFFu), s, prd, &fpj);
+      const double delx = ix - xj.x, dely = iy - xj.y, delz = iz - xj.z;
+      const double rsq = delx * delx + dely * dely + delz * delz;
+      if (rsq < t.cutforcesq) {
+        P phi;
+        // psip = (fpi + fpj)*rhoip + phip, pair_eam.cpp:1118
+        P r = sqrt_p<P>(rsq), pp;
+        const P *c = rrow<P>(t, r, &pp);
+        P rhoip = (c[0] * pp + c[1]) * pp + c[2];
+        P z2p = (c[8] * pp + c[9]) * pp + c[10];
+        P z2 = ((c[11] * pp + c[12]) * pp + c[13]) * pp + c[14];
+        P recip = P(1.0) / r;
+        phi = z2 * recip;
+        P phip = z2p * recip - phi * recip;
+        P psip = (fpi + (P)fpj) * rhoip + phip;
+        const double fpair = (double)(-psip * recip);
+        fx += delx * fpair;
+        fy += dely * fpair;
+        fz += delz * fpair;
+        if (EV) {
+          ep += 0.5 * (double)phi;
+          vir.add(delx, dely, delz, fpair);
+        }
       }
     }
   }
-  ipf[p] = fx;
-  ipf[(size_t)a.nip + p] = fy;
-  ipf[2 * (size_t)a.nip + p] = fz;
-  const double w = a.weight[g];
-  if (eip) eip[p] = w * (embed_energy<P>(t, rhoi, (double)fpi) + ep);
-  if (vip) {
-    vip[p] = w * v0;
-    vip[(size_t)a.nip + p] = w * v1;
-    vip[2 * (size_t)a.nip + p] = w * v2;
-    vip[3 * (size_t)a.nip + p] = w * v3;
-    vip[4 * (size_t)a.nip + p] = w * v4;
-    vip[5 * (size_t)a.nip + p] = w * v5;
+  fx = group_sum(fx);
+  fy = group_sum(fy);
+  fz = group_sum(fz);
+  if (EV) {
+    ep = group_sum(ep);
+    vir.reduce();
+  }
+  if (p < a.nip && l == 0) {
+    ipf[p] = fx;
+    ipf[(size_t)a.nip + p] = fy;
+    ipf[2 * (size_t)a.nip + p] = fz;
+    if (EV) {
+      const double w = a.weight[p % a.nint];
+      eip[p] = w * (embed_energy<P>(t, rhoi, (double)fpi) + ep);
+      vir.store(vip, (size_t)a.nip, (size_t)p, w);
+    }
   }
 }
 
 // LJ integration-point force, pair_lj_cut.cpp:355-422.
-template <class P>
-__global__ void k_lj_ip_force(ElemArgs a, Prd prd, LjParams lj, double *__restrict__ ipf, double *__restrict__ eip,
-                              double *__restrict__ vip) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= a.nip) return;
-  const int i = p / a.nint, g = p % a.nint;
-  const double *Ng = a.spa_int + 8 * g;
-  double ix = 0.0, iy = 0.0, iz = 0.0;
-  for (int kk = 0; kk < 8; ++kk) {
-    ix += Ng[kk] * a.nodex[24 * (size_t)i + 3 * kk];
-    iy += Ng[kk] * a.nodex[24 * (size_t)i + 3 * kk + 1];
-    iz += Ng[kk] * a.nodex[24 * (size_t)i + 3 * kk + 2];
-  }
+template <class P, bool EV>
+__global__ __launch_bounds__(kBlock) void k_lj_ip_force(ElemArgs a, Prd prd, LjParams lj,
+                                                        const double *__restrict__ xs4, double *__restrict__ ipf,
+                                                        double *__restrict__ eip, double *__restrict__ vip) {
+  const int gt = xcd_block() * kBlock + (int)threadIdx.x;
+  const int p = gt >> 3, l = gt & 7;
   const P lj1 = (P)lj.lj1, lj2 = (P)lj.lj2, lj3 = (P)lj.lj3, lj4 = (P)lj.lj4;
-  const int jn = a.ipnum[p];
   double fx = 0.0, fy = 0.0, fz = 0.0, ep = 0.0;
-  double v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
-  for (int l = 0; l < jn; ++l) {
-    uint64_t e = a.ipneigh[(size_t)l * a.nip + p];
-    const int j = (int)(e & 0x7FFFFFFu), img = (int)((e >> 27) & 31), site = (int)(e >> 32);
-    double X[24];
-    load_shifted_nodes(a.nodex, j, img, prd, X);
-    const double *N = a.spa_site + 8 * (size_t)site;
-    double jx = 0.0, jy = 0.0, jz = 0.0;
+  Virial vir;
+  vir.zero();
+  if (p < a.nip) {
+    const int i = p / a.nint, g = p % a.nint;
+    const double *Ng = a.spa_int + 8 * g;
+    double ix = 0.0, iy = 0.0, iz = 0.0;
     for (int kk = 0; kk < 8; ++kk) {
-      jx += N[kk] * X[3 * kk];
-      jy += N[kk] * X[3 * kk + 1];
-      jz += N[kk] * X[3 * kk + 2];
+      ix += Ng[kk] * a.nodex[24 * (size_t)i + 3 * kk];
+      iy += Ng[kk] * a.nodex[24 * (size_t)i + 3 * kk + 1];
+      iz += Ng[kk] * a.nodex[24 * (size_t)i + 3 * kk + 2];
     }
-    double delx = ix - jx, dely = iy - jy, delz = iz - jz;
-    double rsq = delx * delx + dely * dely + delz * delz;
-    if (rsq < lj.cutsq) {
-      P r2inv = P(1.0) / (P)rsq;
-      P r6inv = r2inv * r2inv * r2inv;
-      P forcelj = r6inv * (lj1 * r6inv - lj2);
-      double fpair = (double)(forcelj * r2inv);
-      fx += delx * fpair;
-      fy += dely * fpair;
-      fz += delz * fpair;
-      if (eip) ep += 0.5 * (double)(r6inv * (lj3 * r6inv - lj4));
-      if (vip) {
-        v0 += 0.5 * delx * delx * fpair;
-        v1 += 0.5 * dely * dely * fpair;
-        v2 += 0.5 * delz * delz * fpair;
-        v3 += 0.5 * delx * dely * fpair;
-        v4 += 0.5 * delx * delz * fpair;
-        v5 += 0.5 * dely * delz * fpair;
+    const int jn = a.ipnum[p];
+    const uint32_t *row = list_row(a.ipneigh, p, a.M);
+    for (int kk = l; kk < jn; kk += kLanes) {
+      const uint32_t e = row[(size_t)kk * kRowGroup];
+      int s[3];
+      img_shift((int)(e >> 27), s);
+      const Pt xj = load_xyz(xs4, (int)(e & 0x7FFFFFFu), s, prd);
+      const double delx = ix - xj.x, dely = iy - xj.y, delz = iz - xj.z;
+      const double rsq = delx * delx + dely * dely + delz * delz;
+      if (rsq < lj.cutsq) {
+        P r2inv = P(1.0) / (P)rsq;
+        P r6inv = r2inv * r2inv * r2inv;
+        P forcelj = r6inv * (lj1 * r6inv - lj2);
+        const double fpair = (double)(forcelj * r2inv);
+        fx += delx * fpair;
+        fy += dely * fpair;
+        fz += delz * fpair;
+        if (EV) {
+          ep += 0.5 * (double)(r6inv * (lj3 * r6inv - lj4));
+          vir.add(delx, dely, delz, fpair);
+        }
       }
     }
   }
-  ipf[p] = fx;
-  ipf[(size_t)a.nip + p] = fy;
-  ipf[2 * (size_t)a.nip + p] = fz;
-  const double w = a.weight[g];
-  if (eip) eip[p] = w * ep;
-  if (vip) {
-    vip[p] = w * v0;
-    vip[(size_t)a.nip + p] = w * v1;
-    vip[2 * (size_t)a.nip + p] = w * v2;
-    vip[3 * (size_t)a.nip + p] = w * v3;
-    vip[4 * (size_t)a.nip + p] = w * v4;
-    vip[5 * (size_t)a.nip + p] = w * v5;
+  fx = group_sum(fx);
+  fy = group_sum(fy);
+  fz = group_sum(fz);
+  if (EV) {
+    ep = group_sum(ep);
+    vir.reduce();
+  }
+  if (p < a.nip && l == 0) {
+    ipf[p] = fx;
+    ipf[(size_t)a.nip + p] = fy;
+    ipf[2 * (size_t)a.nip + p] = fz;
+    if (EV) {
+      const double w = a.weight[p % a.nint];
+      eip[p] = w * ep;
+      vir.store(vip, (size_t)a.nip, (size_t)p, w);
+    }
   }
 }
 
@@ -499,70 +607,91 @@ ElemArgs elem_args(const cac_ctx *ctx) {
   a.ne = ctx->ne;
   a.nint = ctx->nint;
   a.nip = ctx->ne * ctx->nint;
+  a.nactive = ctx->nactive;
+  a.M = ctx->maxnip;
   a.nodex = ctx->nodex;
   a.spa_site = ctx->spa_site;
   a.spa_int = ctx->spa_int;
   a.weight = ctx->weight;
   a.ipneigh = ctx->ipneigh;
   a.ipnum = ctx->ipnum;
+  a.asite = ctx->asite;
   for (int k = 0; k < 8; ++k) a.node_list[k] = ctx->node_list[k];
   return a;
 }
 
-template <class P>
-int eam_atoms_t(cac_ctx *ctx, int eflag, int vflag) {
+template <class P, bool EV>
+int eam_atoms_t(cac_ctx *ctx) {
   const int n = ctx->n;
   Tabs<P> t = tabs_of<P>(ctx);
+  const unsigned grid = xcd_grid((long)n * kLanes, kBlock);
   hipEvent_t t0;
   timer_begin(ctx, "atom_rho", &t0);
-  hipLaunchKernelGGL(k_eam_atom_rho<P>, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, n, ctx->x4, prd_of(ctx),
-                     ctx->aneigh, ctx->anum, t, ctx->rho, eflag ? ctx->eatom : nullptr);
+  hipLaunchKernelGGL((k_eam_atom_rho<P, EV>), dim3(grid), dim3(kBlock), 0, ctx->stream, n, ctx->x4, prd_of(ctx),
+                     ctx->aneigh, ctx->maxneigh, ctx->anum, t, ctx->rho, ctx->eatom);
   timer_end(ctx, "atom_rho", t0);
   timer_begin(ctx, "atom_force", &t0);
-  hipLaunchKernelGGL(k_eam_atom_force<P>, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, n, ctx->x4, prd_of(ctx),
-                     ctx->aneigh, ctx->anum, t, ctx->f, eflag ? ctx->eatom : nullptr, vflag ? ctx->vatom : nullptr);
+  hipLaunchKernelGGL((k_eam_atom_force<P, EV>), dim3(grid), dim3(kBlock), 0, ctx->stream, n, ctx->x4, prd_of(ctx),
+                     ctx->aneigh, ctx->maxneigh, ctx->anum, t, ctx->f, ctx->eatom, ctx->vatom);
   timer_end(ctx, "atom_force", t0);
   CAC_HIP(ctx, hipGetLastError());
   return CAC_OK;
 }
 
-template <class P>
-int eam_elements_t(cac_ctx *ctx, int eflag, int vflag) {
+int site_positions(cac_ctx *ctx, const ElemArgs &a) {
+  if (a.nactive == 0) return CAC_OK;
+  hipEvent_t t0;
+  timer_begin(ctx, "site_pos", &t0);
+  hipLaunchKernelGGL(k_site_pos, dim3(xcd_grid(a.nactive, kBlock)), dim3(kBlock), 0, ctx->stream, a, ctx->xs4);
+  timer_end(ctx, "site_pos", t0);
+  return CAC_OK;
+}
+
+template <class P, bool EV>
+int eam_elements_t(cac_ctx *ctx) {
   ElemArgs a = elem_args(ctx);
   Tabs<P> t = tabs_of<P>(ctx);
+  site_positions(ctx, a);
   hipEvent_t t0;
   timer_begin(ctx, "ip_erho", &t0);
-  hipLaunchKernelGGL(k_eam_erho<P>, dim3(grid_for(8L * a.ne)), dim3(kBlock), 0, ctx->stream, a, prd_of(ctx), t,
-                     ctx->erho);
+  hipLaunchKernelGGL((k_eam_erho<P>), dim3(xcd_grid(8L * a.ne * kLanes, kBlock)), dim3(kBlock), 0, ctx->stream, a,
+                     prd_of(ctx), t, ctx->xs4, ctx->erho);
   timer_end(ctx, "ip_erho", t0);
+  if (a.nactive) {
+    timer_begin(ctx, "site_fp", &t0);
+    hipLaunchKernelGGL((k_site_fp<P>), dim3(xcd_grid(a.nactive, kBlock)), dim3(kBlock), 0, ctx->stream, a, t,
+                       ctx->erho, ctx->xs4);
+    timer_end(ctx, "site_fp", t0);
+  }
   timer_begin(ctx, "ip_force", &t0);
-  hipLaunchKernelGGL(k_eam_ip_force<P>, dim3(grid_for(a.nip)), dim3(kBlock), 0, ctx->stream, a, prd_of(ctx), t,
-                     ctx->erho, ctx->ipf, eflag ? ctx->eip : nullptr, vflag ? ctx->vip : nullptr);
+  hipLaunchKernelGGL((k_eam_ip_force<P, EV>), dim3(xcd_grid((long)a.nip * kLanes, kBlock)), dim3(kBlock), 0,
+                     ctx->stream, a, prd_of(ctx), t, ctx->erho, ctx->xs4, ctx->ipf, ctx->eip, ctx->vip);
   timer_end(ctx, "ip_force", t0);
   CAC_HIP(ctx, hipGetLastError());
   return launch_scatter(ctx);
 }
 
-template <class P>
-int lj_atoms_t(cac_ctx *ctx, int eflag, int vflag) {
+template <class P, bool EV>
+int lj_atoms_t(cac_ctx *ctx) {
   const int n = ctx->n;
   hipEvent_t t0;
   timer_begin(ctx, "atom_force", &t0);
-  hipLaunchKernelGGL(k_lj_atom_force<P>, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, n, ctx->x4, prd_of(ctx),
-                     ctx->aneigh, ctx->anum, ctx->lj, ctx->f, eflag ? ctx->eatom : nullptr,
-                     vflag ? ctx->vatom : nullptr);
+  hipLaunchKernelGGL((k_lj_atom_force<P, EV>), dim3(xcd_grid((long)n * kLanes, kBlock)), dim3(kBlock), 0,
+                     ctx->stream, n, ctx->x4, prd_of(ctx), ctx->aneigh, ctx->maxneigh, ctx->anum, ctx->lj, ctx->f,
+                     ctx->eatom, ctx->vatom);
   timer_end(ctx, "atom_force", t0);
   CAC_HIP(ctx, hipGetLastError());
   return CAC_OK;
 }
 
-template <class P>
-int lj_elements_t(cac_ctx *ctx, int eflag, int vflag) {
+template <class P, bool EV>
+int lj_elements_t(cac_ctx *ctx) {
   ElemArgs a = elem_args(ctx);
+  site_positions(ctx, a);
   hipEvent_t t0;
   timer_begin(ctx, "ip_force", &t0);
-  hipLaunchKernelGGL(k_lj_ip_force<P>, dim3(grid_for(a.nip)), dim3(kBlock), 0, ctx->stream, a, prd_of(ctx), ctx->lj,
-                     ctx->ipf, eflag ? ctx->eip : nullptr, vflag ? ctx->vip : nullptr);
+  hipLaunchKernelGGL((k_lj_ip_force<P, EV>), dim3(xcd_grid((long)a.nip * kLanes, kBlock)), dim3(kBlock), 0,
+                     ctx->stream, a, prd_of(ctx), ctx->lj, ctx->xs4, ctx->ipf, ctx->eip, ctx->vip);
   timer_end(ctx, "ip_force", t0);
   CAC_HIP(ctx, hipGetLastError());
   return launch_scatter(ctx);
@@ -580,26 +709,28 @@ int launch_scatter(cac_ctx *ctx) {
   return CAC_OK;
 }
 
+#define CAC_DISPATCH(fn, ctx, ev, precision)                                      \
+  ((precision) == CAC_PREC_MIXED ? ((ev) ? fn<float, true>(ctx) : fn<float, false>(ctx)) \
+                                 : ((ev) ? fn<double, true>(ctx) : fn<double, false>(ctx)))
+
 int launch_eam_atoms(cac_ctx *ctx, int eflag, int vflag, int precision) {
   if (ctx->n == 0) return CAC_OK;
-  return precision == CAC_PREC_MIXED ? eam_atoms_t<float>(ctx, eflag, vflag) : eam_atoms_t<double>(ctx, eflag, vflag);
+  return CAC_DISPATCH(eam_atoms_t, ctx, eflag || vflag, precision);
 }
 
 int launch_eam_elements(cac_ctx *ctx, int eflag, int vflag, int precision) {
   if (ctx->ne == 0) return CAC_OK;
-  return precision == CAC_PREC_MIXED ? eam_elements_t<float>(ctx, eflag, vflag)
-                                     : eam_elements_t<double>(ctx, eflag, vflag);
+  return CAC_DISPATCH(eam_elements_t, ctx, eflag || vflag, precision);
 }
 
 int launch_lj_atoms(cac_ctx *ctx, int eflag, int vflag, int precision) {
   if (ctx->n == 0) return CAC_OK;
-  return precision == CAC_PREC_MIXED ? lj_atoms_t<float>(ctx, eflag, vflag) : lj_atoms_t<double>(ctx, eflag, vflag);
+  return CAC_DISPATCH(lj_atoms_t, ctx, eflag || vflag, precision);
 }
 
 int launch_lj_elements(cac_ctx *ctx, int eflag, int vflag, int precision) {
   if (ctx->ne == 0) return CAC_OK;
-  return precision == CAC_PREC_MIXED ? lj_elements_t<float>(ctx, eflag, vflag)
-                                     : lj_elements_t<double>(ctx, eflag, vflag);
+  return CAC_DISPATCH(lj_elements_t, ctx, eflag || vflag, precision);
 }
 
 }  // namespace cac
