@@ -118,26 +118,32 @@ __global__ void k_reduce(RedArgs a, double *__restrict__ out) {
 int upload_tables(cac_ctx *ctx, int nrho, double drho, const double *frho_spline, int nr, double dr,
                   const double *rhor_spline, const double *z2r_spline, double cutforcesq) {
   EamTables &t = ctx->eam;
-  std::vector<double> rz((size_t)(nr + 1) * 16, 0.0), fr((size_t)(nrho + 1) * 8, 0.0);
-  std::vector<float> rzf((size_t)(nr + 1) * 16, 0.0f), frf((size_t)(nrho + 1) * 8, 0.0f);
-  for (int m = 0; m <= nr; ++m)
-    for (int c = 0; c < 7; ++c) {
-      rz[16 * (size_t)m + c] = rhor_spline[7 * (size_t)m + c];
-      rz[16 * (size_t)m + 8 + c] = z2r_spline[7 * (size_t)m + c];
-      rzf[16 * (size_t)m + c] = (float)rhor_spline[7 * (size_t)m + c];
-      rzf[16 * (size_t)m + 8 + c] = (float)z2r_spline[7 * (size_t)m + c];
+  const size_t nrow = (size_t)nr + 1, nrrow = (size_t)nrho + 1;
+  std::vector<double> rv(nrow * 4), rz(nrow * 8), fr(nrrow * 8, 0.0);
+  std::vector<float> rvf(nrow * 4), rzf(nrow * 8), frf(nrrow * 8, 0.0f);
+  for (size_t m = 0; m < nrow; ++m)
+    for (int c = 0; c < 4; ++c) {
+      rv[4 * m + c] = rhor_spline[7 * m + 3 + c];
+      rz[8 * m + c] = rhor_spline[7 * m + 3 + c];
+      rz[8 * m + 4 + c] = z2r_spline[7 * m + 3 + c];
+      rvf[4 * m + c] = (float)rv[4 * m + c];
+      rzf[8 * m + c] = (float)rz[8 * m + c];
+      rzf[8 * m + 4 + c] = (float)rz[8 * m + 4 + c];
     }
-  for (int m = 0; m <= nrho; ++m)
+  for (size_t m = 0; m < nrrow; ++m)
     for (int c = 0; c < 7; ++c) {
-      fr[8 * (size_t)m + c] = frho_spline[7 * (size_t)m + c];
-      frf[8 * (size_t)m + c] = (float)frho_spline[7 * (size_t)m + c];
+      fr[8 * m + c] = frho_spline[7 * m + c];
+      frf[8 * m + c] = (float)frho_spline[7 * m + c];
     }
   int rc;
-  if ((rc = ensure(ctx, t.b_rz, rz.size(), &t.rz)) || (rc = ensure(ctx, t.b_fr, fr.size(), &t.fr)) ||
+  if ((rc = ensure(ctx, t.b_rv, rv.size(), &t.rv)) || (rc = ensure(ctx, t.b_rz, rz.size(), &t.rz)) ||
+      (rc = ensure(ctx, t.b_fr, fr.size(), &t.fr)) || (rc = ensure(ctx, t.b_rv_f, rvf.size(), &t.rv_f)) ||
       (rc = ensure(ctx, t.b_rz_f, rzf.size(), &t.rz_f)) || (rc = ensure(ctx, t.b_fr_f, frf.size(), &t.fr_f)))
     return rc;
+  CAC_HIP(ctx, hipMemcpy(t.rv, rv.data(), rv.size() * sizeof(double), hipMemcpyHostToDevice));
   CAC_HIP(ctx, hipMemcpy(t.rz, rz.data(), rz.size() * sizeof(double), hipMemcpyHostToDevice));
   CAC_HIP(ctx, hipMemcpy(t.fr, fr.data(), fr.size() * sizeof(double), hipMemcpyHostToDevice));
+  CAC_HIP(ctx, hipMemcpy(t.rv_f, rvf.data(), rvf.size() * sizeof(float), hipMemcpyHostToDevice));
   CAC_HIP(ctx, hipMemcpy(t.rz_f, rzf.data(), rzf.size() * sizeof(float), hipMemcpyHostToDevice));
   CAC_HIP(ctx, hipMemcpy(t.fr_f, frf.data(), frf.size() * sizeof(float), hipMemcpyHostToDevice));
   t.nrho = nrho;
@@ -343,7 +349,7 @@ int cac_ctx_destroy(cac_ctx *ctx) {
                     &ctx->b_iptmp, &ctx->b_bitmap, &ctx->b_wprefix, &ctx->b_ecount, &ctx->b_estart, &ctx->b_asite,
                     &ctx->b_xs4, &ctx->b_scantmp,
                     &ctx->b_ipf, &ctx->b_eip, &ctx->b_vip, &ctx->b_red, &ctx->b_dflags, &ctx->eam.b_rz,
-                    &ctx->eam.b_fr, &ctx->eam.b_rz_f, &ctx->eam.b_fr_f};
+                    &ctx->eam.b_fr, &ctx->eam.b_rz_f, &ctx->eam.b_fr_f, &ctx->eam.b_rv, &ctx->eam.b_rv_f};
   for (DevBuf *b : bufs)
     if (b->p) hipFree(b->p);
   if (ctx->hflags) hipHostFree(ctx->hflags);

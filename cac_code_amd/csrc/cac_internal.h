@@ -56,11 +56,13 @@ struct Timer {
 struct EamTables {
   int nrho = 0, nr = 0;
   double rdrho = 0, rdr = 0, cutforcesq = 0, rhomax = 0;
-  // r-knot rows [(nr+1)][16]: rhor c0..c6 in 0..6, z2r c0..c6 in 8..14;
-  // rho-knot rows [(nrho+1)][8]: frho c0..c6
-  double *rz = nullptr, *fr = nullptr;
-  float *rz_f = nullptr, *fr_f = nullptr;
-  DevBuf b_rz, b_fr, b_rz_f, b_fr_f;
+  // r-knot rows: rv [(nr+1)][4] = rhor value coefficients c3..c6 (density
+  // pass); rz [(nr+1)][8] = rhor c3..c6, z2r c3..c6 (force pass, derivative
+  // polynomial formed from the value coefficients).  rho-knot rows fr
+  // [(nrho+1)][8] = frho c0..c6.
+  double *rv = nullptr, *rz = nullptr, *fr = nullptr;
+  float *rv_f = nullptr, *rz_f = nullptr, *fr_f = nullptr;
+  DevBuf b_rv, b_rz, b_fr, b_rv_f, b_rz_f, b_fr_f;
 };
 
 struct LjParams {

@@ -21,9 +21,12 @@
 // F'(rho)), so the erho and force passes gather 32 bytes per entry, like the
 // atom passes.  The image of a neighbor site is x_site + s*prd.
 //
-// Tables on the device: one 16-wide row per r-knot, rhor c0..c6 in slots
-// 0..6 and z2r c0..c6 in slots 8..14, so a pair reads both splines from one
-// 128-byte line (64 bytes in FP32); frho rows are 8 wide.
+// Tables on the device: per r-knot, the density pass reads the 4 value
+// coefficients of rhor (32 bytes, 16 in FP32) and the force pass the 4 value
+// coefficients of rhor and of z2r (64 bytes); the derivative polynomial is
+// formed from them, (3*c3*p + 2*c4)*p + c5 times 1/dr, which is the
+// reference's (c0*p + c1)*p + c2 with c0 = 3*c3/dr, c1 = 2*c4/dr,
+// c2 = c5/dr (pair_eam.cpp:574-578).  frho rows keep all 7 coefficients.
 #include "cac_internal.h"
 
 namespace cac {
@@ -47,23 +50,42 @@ struct Prd {
 
 template <class P>
 struct Tabs {
-  const P *rz;  // [(nr+1)][16]
+  const P *rv;  // [(nr+1)][4]  rhor c3..c6
+  const P *rz;  // [(nr+1)][8]  rhor c3..c6, z2r c3..c6
   const P *fr;  // [(nrho+1)][8]
   int nr, nrho;
   P rdr, rdrho;
   double cutforcesq, rhomax;
 };
 
-// r-table row and fraction, pair_eam.cpp:649-653
+// r-table knot and fraction, pair_eam.cpp:649-653
 template <class P>
-__device__ __forceinline__ const P *rrow(const Tabs<P> &t, P r, P *pout) {
+__device__ __forceinline__ int rknot(const Tabs<P> &t, P r, P *pout) {
   P p = r * t.rdr + P(1.0);
   int m = (int)p;
   m = m < t.nr - 1 ? m : t.nr - 1;
   p -= (P)m;
   p = p < P(1.0) ? p : P(1.0);
   *pout = p;
-  return t.rz + 16 * (size_t)m;
+  return m;
+}
+
+// rho(r): ((c3*p + c4)*p + c5)*p + c6, pair_eam.cpp:655
+template <class P>
+__device__ __forceinline__ P rho_of_r(const Tabs<P> &t, P r) {
+  P p;
+  const P *c = t.rv + 4 * (size_t)rknot<P>(t, r, &p);
+  return ((c[0] * p + c[1]) * p + c[2]) * p + c[3];
+}
+
+// rho'(r), z2'(r), z2(r) of one pair, pair_eam.cpp:720-726
+template <class P>
+__device__ __forceinline__ void rz_of_r(const Tabs<P> &t, P r, P *rhoip, P *z2p, P *z2) {
+  P p;
+  const P *c = t.rz + 8 * (size_t)rknot<P>(t, r, &p);
+  *rhoip = ((P(3.0) * c[0] * p + P(2.0) * c[1]) * p + c[2]) * t.rdr;
+  *z2p = ((P(3.0) * c[4] * p + P(2.0) * c[5]) * p + c[6]) * t.rdr;
+  *z2 = ((c[4] * p + c[5]) * p + c[6]) * p + c[7];
 }
 
 // rho-table row and fraction, pair_eam.cpp:672-676
@@ -121,18 +143,44 @@ struct Pt {
   double x, y, z;
 };
 
-__device__ __forceinline__ Pt load_xyz(const double *__restrict__ x4, int j, const int s[3], const Prd &prd) {
-  const double2 xy = reinterpret_cast<const double2 *>(x4)[2 * (size_t)j];
-  const double z = x4[4 * (size_t)j + 2];
+// Entries per lane fetched together: the list words and then the neighbor
+// gathers of kU entries are issued back to back before any arithmetic, so a
+// lane keeps several memory requests in flight instead of one dependent chain.
+constexpr int kU = 4;
+constexpr uint32_t kNoEntry = 0xFFFFFFFFu;  // image field 31: never a real entry
+
+template <int U>
+__device__ __forceinline__ void fetch_entries(const uint32_t *__restrict__ row, int k0, int jn, uint32_t e[U]) {
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int k = k0 + u * kLanes;
+    e[u] = k < jn ? row[(size_t)k * kRowGroup] : kNoEntry;
+  }
+}
+
+// Position of list entry e = (j | img << 27): x_j, or x_j + s*prd for a
+// periodic image.
+__device__ __forceinline__ Pt load_xyz(const double *__restrict__ x4, uint32_t e, const Prd &prd) {
+  const size_t j = e & 0x7FFFFFFu;
+  const int img = (int)(e >> 27);
+  const double2 xy = reinterpret_cast<const double2 *>(x4)[2 * j];
+  const double z = x4[4 * j + 2];
+  if (img == kImgSelf) return Pt{xy.x, xy.y, z};
+  int s[3];
+  img_shift(img, s);
   return Pt{shifted(xy.x, s[0], prd.v[0]), shifted(xy.y, s[1], prd.v[1]), shifted(z, s[2], prd.v[2])};
 }
 
-// position and fourth slot (fp) of particle j, image s
-__device__ __forceinline__ Pt load_xyzw(const double *__restrict__ x4, int j, const int s[3], const Prd &prd,
-                                        double *w) {
-  const double2 xy = reinterpret_cast<const double2 *>(x4)[2 * (size_t)j];
-  const double2 zw = reinterpret_cast<const double2 *>(x4)[2 * (size_t)j + 1];
+// position and fourth slot (fp) of list entry e
+__device__ __forceinline__ Pt load_xyzw(const double *__restrict__ x4, uint32_t e, const Prd &prd, double *w) {
+  const size_t j = e & 0x7FFFFFFu;
+  const int img = (int)(e >> 27);
+  const double2 xy = reinterpret_cast<const double2 *>(x4)[2 * j];
+  const double2 zw = reinterpret_cast<const double2 *>(x4)[2 * j + 1];
   *w = zw.y;
+  if (img == kImgSelf) return Pt{xy.x, xy.y, zw.x};
+  int s[3];
+  img_shift(img, s);
   return Pt{shifted(xy.x, s[0], prd.v[0]), shifted(xy.y, s[1], prd.v[1]), shifted(zw.x, s[2], prd.v[2])};
 }
 
@@ -161,11 +209,8 @@ struct Virial {
 // pair_eam.cpp:706-738 / 1083-1120.
 template <class P>
 __device__ __forceinline__ double eam_fpair(const Tabs<P> &t, double rsq, P fpi, P fpj, P *phi_out) {
-  P r = sqrt_p<P>(rsq), p;
-  const P *c = rrow<P>(t, r, &p);
-  P rhoip = (c[0] * p + c[1]) * p + c[2];
-  P z2p = (c[8] * p + c[9]) * p + c[10];
-  P z2 = ((c[11] * p + c[12]) * p + c[13]) * p + c[14];
+  P r = sqrt_p<P>(rsq), rhoip, z2p, z2;
+  rz_of_r<P>(t, r, &rhoip, &z2p, &z2);
   P recip = P(1.0) / r;
   P phi = z2 * recip;
   P phip = z2p * recip - phi * recip;
@@ -189,17 +234,19 @@ __global__ __launch_bounds__(kBlock) void k_eam_atom_rho(int n, double *__restri
     const double xi = x4[4 * (size_t)i], yi = x4[4 * (size_t)i + 1], zi = x4[4 * (size_t)i + 2];
     const int jn = num[i];
     const uint32_t *row = list_row(neigh, i, M);
-    for (int k = l; k < jn; k += kLanes) {
-      const uint32_t e = row[(size_t)k * kRowGroup];
-      int s[3];
-      img_shift((int)(e >> 27), s);
-      const Pt xj = load_xyz(x4, (int)(e & 0x7FFFFFFu), s, prd);
-      const double delx = xi - xj.x, dely = yi - xj.y, delz = zi - xj.z;
-      const double rsq = delx * delx + dely * dely + delz * delz;
-      if (rsq < t.cutforcesq) {
-        P p;
-        const P *c = rrow<P>(t, sqrt_p<P>(rsq), &p);
-        rho += (double)(((c[3] * p + c[4]) * p + c[5]) * p + c[6]);
+    for (int k0 = l; k0 < jn; k0 += kU * kLanes) {
+      uint32_t e[kU];
+      fetch_entries<kU>(row, k0, jn, e);
+      Pt xj[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u)
+        if (e[u] != kNoEntry) xj[u] = load_xyz(x4, e[u], prd);
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        if (e[u] == kNoEntry) continue;
+        const double delx = xi - xj[u].x, dely = yi - xj[u].y, delz = zi - xj[u].z;
+        const double rsq = delx * delx + dely * dely + delz * delz;
+        if (rsq < t.cutforcesq) rho += (double)rho_of_r<P>(t, sqrt_p<P>(rsq));
       }
     }
   }
@@ -229,12 +276,19 @@ __global__ __launch_bounds__(kBlock) void k_eam_atom_force(int n, const double *
     const P fpi = (P)x4[4 * (size_t)i + 3];
     const int jn = num[i];
     const uint32_t *row = list_row(neigh, i, M);
-    for (int k = l; k < jn; k += kLanes) {
-      const uint32_t e = row[(size_t)k * kRowGroup];
-      int s[3];
-      img_shift((int)(e >> 27), s);
-      double fpj;
-      const Pt xj = load_xyzw(x4, (int)(e & 0x7FFFFFFu), s, prd, &fpj);
+    for (int k0 = l; k0 < jn; k0 += kU * kLanes) {
+      uint32_t e[kU];
+      fetch_entries<kU>(row, k0, jn, e);
+      Pt xjv[kU];
+      double fpjv[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u)
+        if (e[u] != kNoEntry) xjv[u] = load_xyzw(x4, e[u], prd, &fpjv[u]);
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+      if (e[u] == kNoEntry) continue;
+      const Pt xj = xjv[u];
+      const double fpj = fpjv[u];
       const double delx = xi - xj.x, dely = yi - xj.y, delz = zi - xj.z;
       const double rsq = delx * delx + dely * dely + delz * delz;
       if (rsq < t.cutforcesq) {
@@ -247,6 +301,7 @@ __global__ __launch_bounds__(kBlock) void k_eam_atom_force(int n, const double *
           ep += 0.5 * (double)phi;
           vir.add(delx, dely, delz, fpair);
         }
+      }
       }
     }
   }
@@ -288,9 +343,7 @@ __global__ __launch_bounds__(kBlock) void k_lj_atom_force(int n, const double *_
     const uint32_t *row = list_row(neigh, i, M);
     for (int k = l; k < jn; k += kLanes) {
       const uint32_t e = row[(size_t)k * kRowGroup];
-      int s[3];
-      img_shift((int)(e >> 27), s);
-      const Pt xj = load_xyz(x4, (int)(e & 0x7FFFFFFu), s, prd);
+      const Pt xj = load_xyz(x4, e, prd);
       const double delx = xi - xj.x, dely = yi - xj.y, delz = zi - xj.z;
       const double rsq = delx * delx + dely * dely + delz * delz;
       if (rsq < lj.cutsq) {
@@ -387,17 +440,19 @@ __global__ __launch_bounds__(kBlock) void k_eam_erho(ElemArgs a, Prd prd, Tabs<P
     const int p = i * a.nint + a.node_list[k];
     const int jn = a.ipnum[p];
     const uint32_t *row = list_row(a.ipneigh, p, a.M);
-    for (int kk = l; kk < jn; kk += kLanes) {
-      const uint32_t e = row[(size_t)kk * kRowGroup];
-      int s[3];
-      img_shift((int)(e >> 27), s);
-      const Pt xj = load_xyz(xs4, (int)(e & 0x7FFFFFFu), s, prd);
-      const double delx = ix - xj.x, dely = iy - xj.y, delz = iz - xj.z;
-      const double rsq = delx * delx + dely * dely + delz * delz;
-      if (rsq < t.cutforcesq) {
-        P pp;
-        const P *c = rrow<P>(t, sqrt_p<P>(rsq), &pp);
-        acc += (double)(((c[3] * pp + c[4]) * pp + c[5]) * pp + c[6]);
+    for (int k0 = l; k0 < jn; k0 += kU * kLanes) {
+      uint32_t e[kU];
+      fetch_entries<kU>(row, k0, jn, e);
+      Pt xj[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u)
+        if (e[u] != kNoEntry) xj[u] = load_xyz(xs4, e[u], prd);
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        if (e[u] == kNoEntry) continue;
+        const double delx = ix - xj[u].x, dely = iy - xj[u].y, delz = iz - xj[u].z;
+        const double rsq = delx * delx + dely * dely + delz * delz;
+        if (rsq < t.cutforcesq) acc += (double)rho_of_r<P>(t, sqrt_p<P>(rsq));
       }
     }
   }
@@ -431,22 +486,26 @@ __global__ __launch_bounds__(kBlock) void k_eam_ip_force(ElemArgs a, Prd prd, Ta
     fpi = embed_fp<P>(t, rhoi);
     const int jn = a.ipnum[p];
     const uint32_t *row = list_row(a.ipneigh, p, a.M);
-    for (int kk = l; kk < jn; kk += kLanes) {
-      const uint32_t e = row[(size_t)kk * kRowGroup];
-      int s[3];
-      img_shift((int)(e >> 27), s);
-      double fpj;
-      const Pt xj = load_xyzw(xs4, (int)(e & 0x7FFFFFFu), s, prd, &fpj);
+    for (int k0 = l; k0 < jn; k0 += kU * kLanes) {
+      uint32_t e[kU];
+      fetch_entries<kU>(row, k0, jn, e);
+      Pt xjv[kU];
+      double fpjv[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u)
+        if (e[u] != kNoEntry) xjv[u] = load_xyzw(xs4, e[u], prd, &fpjv[u]);
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+      if (e[u] == kNoEntry) continue;
+      const Pt xj = xjv[u];
+      const double fpj = fpjv[u];
       const double delx = ix - xj.x, dely = iy - xj.y, delz = iz - xj.z;
       const double rsq = delx * delx + dely * dely + delz * delz;
       if (rsq < t.cutforcesq) {
         P phi;
         // psip = (fpi + fpj)*rhoip + phip, pair_eam.cpp:1118
-        P r = sqrt_p<P>(rsq), pp;
-        const P *c = rrow<P>(t, r, &pp);
-        P rhoip = (c[0] * pp + c[1]) * pp + c[2];
-        P z2p = (c[8] * pp + c[9]) * pp + c[10];
-        P z2 = ((c[11] * pp + c[12]) * pp + c[13]) * pp + c[14];
+        P r = sqrt_p<P>(rsq), rhoip, z2p, z2;
+        rz_of_r<P>(t, r, &rhoip, &z2p, &z2);
         P recip = P(1.0) / r;
         phi = z2 * recip;
         P phip = z2p * recip - phi * recip;
@@ -459,6 +518,7 @@ __global__ __launch_bounds__(kBlock) void k_eam_ip_force(ElemArgs a, Prd prd, Ta
           ep += 0.5 * (double)phi;
           vir.add(delx, dely, delz, fpair);
         }
+      }
       }
     }
   }
@@ -505,9 +565,7 @@ __global__ __launch_bounds__(kBlock) void k_lj_ip_force(ElemArgs a, Prd prd, LjP
     const uint32_t *row = list_row(a.ipneigh, p, a.M);
     for (int kk = l; kk < jn; kk += kLanes) {
       const uint32_t e = row[(size_t)kk * kRowGroup];
-      int s[3];
-      img_shift((int)(e >> 27), s);
-      const Pt xj = load_xyz(xs4, (int)(e & 0x7FFFFFFu), s, prd);
+      const Pt xj = load_xyz(xs4, e, prd);
       const double delx = ix - xj.x, dely = iy - xj.y, delz = iz - xj.z;
       const double rsq = delx * delx + dely * dely + delz * delz;
       if (rsq < lj.cutsq) {
@@ -578,6 +636,7 @@ Tabs<P> tabs_of(const cac_ctx *ctx);
 template <>
 Tabs<double> tabs_of<double>(const cac_ctx *ctx) {
   Tabs<double> t;
+  t.rv = ctx->eam.rv;
   t.rz = ctx->eam.rz;
   t.fr = ctx->eam.fr;
   t.nr = ctx->eam.nr;
@@ -591,6 +650,7 @@ Tabs<double> tabs_of<double>(const cac_ctx *ctx) {
 template <>
 Tabs<float> tabs_of<float>(const cac_ctx *ctx) {
   Tabs<float> t;
+  t.rv = ctx->eam.rv_f;
   t.rz = ctx->eam.rz_f;
   t.fr = ctx->eam.fr_f;
   t.nr = ctx->eam.nr;
