@@ -54,45 +54,56 @@ class Funcfl:
     zr: np.ndarray    # [nr], r*phi(r) at r = m*dr
 
 
-def johnson_cu_funcfl(nrho: int = 10000, nr: int = 10000, rc: float = 4.95,
-                      taper_width: float = 0.5, rho_max_factor: float = 2.0,
-                      params: dict | None = None) -> Funcfl:
-    """Tabulate the analytic Johnson Cu EAM with a C2 taper on f and phi."""
+def johnson_cu_functions(rc: float = 4.95, taper_width: float = 0.5, params: dict | None = None):
+    """Analytic density f(r), pair phi(r) and embedding F(rho) of the
+    Johnson-type Cu model (with the C2 taper on f and phi), plus rho_e."""
     p = dict(JOHNSON_CU)
     if params:
         p.update(params)
     r_s = rc - taper_width
-    dr = rc / (nr - 1)
-    r = np.arange(nr, dtype=np.float64) * dr
 
     def f_of(rr):
+        rr = np.asarray(rr, dtype=np.float64)
         return p["f_e"] * np.exp(-p["beta"] * (rr / p["r_e"] - 1.0)) * _taper(rr, r_s, rc)
 
     def phi_of(rr):
+        rr = np.asarray(rr, dtype=np.float64)
         return p["phi_e"] * np.exp(-p["gamma"] * (rr / p["r_e"] - 1.0)) * _taper(rr, r_s, rc)
 
-    rhor = f_of(r)
-    zr = r * phi_of(r)
     # equilibrium density of the perfect fcc lattice at a = CU_A0 (shells
-    # inside rc), used for the Banerjea-Smith reference density
-    shells = _fcc_shells(CU_A0, rc)
-    rho_e = float(sum(n * f_of(np.array([d]))[0] for d, n in shells))
-    rho_max = rho_max_factor * rho_e
-    drho = rho_max / (nrho - 1)
-    rho = np.arange(nrho, dtype=np.float64) * drho
+    # inside rc): the Banerjea-Smith reference density
+    rho_e = float(sum(n * float(f_of(d)) for d, n in fcc_shells(CU_A0, rc)))
     ab = p["alpha"] / p["beta"]
     gb = p["gamma"] / p["beta"]
-    with np.errstate(divide="ignore", invalid="ignore"):
-        x = rho / rho_e
-        lnx = np.where(x > 0, np.log(np.where(x > 0, x, 1.0)), 0.0)
-        frho = -p["E_c"] * (1.0 - ab * lnx) * np.where(x > 0, x ** ab, 0.0) \
-            - 6.0 * p["phi_e"] * np.where(x > 0, x ** gb, 0.0)
-    frho[0] = 0.0
+
+    def F_of(rho):
+        rho = np.asarray(rho, dtype=np.float64)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            x = rho / rho_e
+            pos = x > 0
+            xs = np.where(pos, x, 1.0)
+            val = -p["E_c"] * (1.0 - ab * np.log(xs)) * xs ** ab - 6.0 * p["phi_e"] * xs ** gb
+        return np.where(pos, val, 0.0)
+
+    return f_of, phi_of, F_of, rho_e
+
+
+def johnson_cu_funcfl(nrho: int = 10000, nr: int = 10000, rc: float = 4.95,
+                      taper_width: float = 0.5, rho_max_factor: float = 2.0,
+                      params: dict | None = None) -> Funcfl:
+    """Tabulate the analytic Johnson Cu EAM with a C2 taper on f and phi."""
+    f_of, phi_of, F_of, rho_e = johnson_cu_functions(rc, taper_width, params)
+    dr = rc / (nr - 1)
+    r = np.arange(nr, dtype=np.float64) * dr
+    rhor = f_of(r)
+    zr = r * phi_of(r)
+    drho = rho_max_factor * rho_e / (nrho - 1)
+    frho = F_of(np.arange(nrho, dtype=np.float64) * drho)
     return Funcfl("Johnson-type analytic EAM for Cu (exp density, exp pair, Banerjea-Smith F)",
                   29, CU_MASS, nrho, drho, nr, dr, rc, frho, rhor, zr)
 
 
-def _fcc_shells(a: float, rc: float):
+def fcc_shells(a: float, rc: float):
     """(distance, count) of fcc neighbour shells with distance < rc."""
     n = int(math.ceil(rc / (a / 2.0))) + 1
     rng = np.arange(-n, n + 1)

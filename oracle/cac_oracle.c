@@ -13,7 +13,9 @@
  * fix is named at the spot:
  *   - element neighbor search is complete (the reference bins elements with
  *     the atom stencil, neighbor.cpp:963-975, which cannot reach cut_all);
- *   - elepbc subtracts the period at x-hi (domain.cpp:682 adds it).
+ *   - elepbc subtracts the period at x-hi (domain.cpp:682 adds it);
+ *   - the element / sub-element pruning radii are true bounds (see
+ *     orc_ip_neighbors), so the integration-point lists are complete.
  */
 #include "cac_oracle.h"
 
@@ -406,6 +408,7 @@ typedef struct {
   int *nsite;          /* sites per sub-element */
   int *site;           /* [nsub][nsite_total] site ids, list_inter_subele */
   double *spa_center;  /* [nsub][8] Spa_center_subele */
+  double *spa_corner;  /* [nsub][8][8] N at the sub-element's 8 corners */
 } subele_t;
 
 /* Element::specify_sub_ele_pc / count_sub_element / shape_array_inter_subele,
@@ -423,6 +426,7 @@ static int subele_build(subele_t *sb, const int nae[3], int nsplit) {
   sb->nsite = (int *)calloc(nsub, sizeof(int));
   sb->site = (int *)malloc(sizeof(int) * (size_t)nsub * ns);
   sb->spa_center = (double *)malloc(sizeof(double) * 8 * nsub);
+  sb->spa_corner = (double *)malloc(sizeof(double) * 64 * nsub);
   int in = 0, total = 0;
   for (int ii = 0; ii < nsplit; ii++)
     for (int jj = 0; jj < nsplit; jj++)
@@ -438,6 +442,7 @@ static int subele_build(subele_t *sb, const int nae[3], int nsplit) {
           for (int d = 0; d < 3; d++) cpc[d] += sp[l][d];
         for (int d = 0; d < 3; d++) cpc[d] = cpc[d] / 8;
         orc_shape(cpc[0], cpc[1], cpc[2], sb->spa_center + 8 * in);
+        for (int l = 0; l < 8; l++) orc_shape(sp[l][0], sp[l][1], sp[l][2], sb->spa_corner + 64 * in + 8 * l);
         double xlo = sp[0][0], xhi = sp[1][0], ylo = sp[0][1], yhi = sp[3][1], zlo = sp[0][2], zhi = sp[4][2];
         if (xlo < -0.99) xlo = -1.01;
         if (xhi > 0.99) xhi = 1.01;
@@ -457,6 +462,7 @@ static int subele_build(subele_t *sb, const int nae[3], int nsplit) {
 }
 
 static void subele_free(subele_t *sb) {
+  free(sb->spa_corner);
   free(sb->nsite);
   free(sb->site);
   free(sb->spa_center);
@@ -493,23 +499,91 @@ typedef struct {
   int j, img;
 } eref_t;
 
+/* Pruning radii.  The reference prunes with cut_ele = cn + max_size/2 and
+ * cut_subele = cn + max_size/(2*nsplit) (neighbor.cpp:284-293), which are not
+ * bounds once elements are distorted: with jittered nodes it misses sites at
+ * the cutoff (tests/test_oracle_neighbors.py).  Fixed here: each element
+ * (sub-element) prunes with cn + the largest distance from its center to its
+ * own corners, a true bound since every site is a convex combination of the
+ * corners (trilinear weights are >= 0 and sum to 1).  Centers are the
+ * reference's (node mean; Spa_center_subele image of the sub-cube center,
+ * which equals the mean of the sub-cube corner images). */
+static double corner_radius(const double *Nc /* [8][8] */, const double X[8][3], const double c[3]) {
+  double r2 = 0.0;
+  for (int l = 0; l < 8; l++) {
+    double p[3];
+    interp3(Nc + 8 * l, X, p);
+    double dx = p[0] - c[0], dy = p[1] - c[1], dz = p[2] - c[2];
+    double q = dx * dx + dy * dy + dz * dz;
+    r2 = MAXD(r2, q);
+  }
+  return sqrt(r2) * (1.0 + 1e-12) + 1e-9;
+}
+
+static double node_radius(const double X[8][3], const double c[3]) {
+  double r2 = 0.0;
+  for (int k = 0; k < 8; k++) {
+    double dx = X[k][0] - c[0], dy = X[k][1] - c[1], dz = X[k][2] - c[2];
+    double q = dx * dx + dy * dy + dz * dz;
+    r2 = MAXD(r2, q);
+  }
+  return sqrt(r2) * (1.0 + 1e-12) + 1e-9;
+}
+
+/* Sites of element image (j, img) within cutneigh of ip, reference order:
+ * sub-elements, then their sites (full_node_neighbor.cpp:127-170,176-224). */
+static int scan_element(const orc_elems *el, const subele_t *sb, int j, int img, int self, const double ip[3],
+                        const double prd[3], double cutneigh, double Rj, const double cj[3], int n, int maxnip,
+                        uint64_t *out) {
+  double cutsq = cutneigh * cutneigh;
+  double npdis = (ip[0] - cj[0]) * (ip[0] - cj[0]) + (ip[1] - cj[1]) * (ip[1] - cj[1]) +
+                 (ip[2] - cj[2]) * (ip[2] - cj[2]);
+  double ce = cutneigh + Rj;
+  if (npdis > ce * ce) return n;
+  double X[8][3];
+  shifted_nodes(el->nodex, j, img, prd, X);
+  for (int isub = 0; isub < sb->nsub; isub++) {
+    double sc[3];
+    interp3(sb->spa_center + 8 * isub, X, sc);
+    double nps = (ip[0] - sc[0]) * (ip[0] - sc[0]) + (ip[1] - sc[1]) * (ip[1] - sc[1]) +
+                 (ip[2] - sc[2]) * (ip[2] - sc[2]);
+    double cs = cutneigh + corner_radius(sb->spa_corner + 64 * isub, X, sc);
+    if (nps > cs * cs) continue;
+    for (int mm = 0; mm < sb->nsite[isub]; mm++) {
+      int site = sb->site[(size_t)isub * el->nsite + mm];
+      double sx[3];
+      interp3(el->spa_site + 8 * site, X, sx);
+      double delx = sx[0] - ip[0], dely = sx[1] - ip[1], delz = sx[2] - ip[2];
+      double rsq = delx * delx + dely * dely + delz * delz;
+      if (rsq <= cutsq && (!self || rsq > 1.0e-4)) {
+        if (n < maxnip) out[n] = (uint64_t)j | ((uint64_t)img << 27) | ((uint64_t)site << 32);
+        n++;
+      }
+    }
+  }
+  return n;
+}
+
 int orc_ip_neighbors(const orc_elems *el, const orc_box *box, double cutneigh, int nsplit,
                      int maxnip, int *numneigh, uint64_t *neigh) {
   int ne = el->ne, nint = el->nint;
   double prd[3] = {box->hi[0] - box->lo[0], box->hi[1] - box->lo[1], box->hi[2] - box->lo[2]};
   subele_t sb;
   if (subele_build(&sb, el->nae, nsplit)) { subele_free(&sb); return -2; }
-  double max_size = orc_element_max_size(ne, el->nodex);
-  /* Neighbor::setup_cut, neighbor.cpp:284-293 */
-  double cut_all = cutneigh + max_size;
-  double cut_allsq = cut_all * cut_all;
-  double cut_ele = cutneigh + max_size / 2;
-  double cut_elesq = cut_ele * cut_ele;
-  double cut_subele = cutneigh + max_size / (2 * nsplit);
-  double cut_subelesq = cut_subele * cut_subele;
-  double cutsq = cutneigh * cutneigh;
   double *xc = (double *)malloc(sizeof(double) * 3 * (ne > 0 ? ne : 1));
+  double *R = (double *)malloc(sizeof(double) * (ne > 0 ? ne : 1));
   orc_element_centers(ne, el->nodex, xc);
+  double Rmax = 0.0;
+  for (int i = 0; i < ne; i++) {
+    double X[8][3];
+    shifted_nodes(el->nodex, i, ORC_IMG_SELF, prd, X);
+    R[i] = node_radius(X, xc + 3 * i);
+    Rmax = MAXD(Rmax, R[i]);
+  }
+  /* element list reach (reference: cut_all = cn + max_size): two elements can
+   * share an in-range (integration point, site) pair only if their centers
+   * are within cn + R_i + R_j */
+  double cut_all = cutneigh + 2.0 * Rmax;
   cells_t c;
   cells_build(&c, ne, xc, box, cut_all);
   int overflow = 0;
@@ -517,7 +591,9 @@ int orc_ip_neighbors(const orc_elems *el, const orc_box *box, double cutneigh, i
 #pragma omp parallel for schedule(dynamic, 4) reduction(| : overflow)
   for (int i = 0; i < ne; i++) {
     /* element_neighbor.cpp:37-66: every other element (local or ghost image)
-     * whose center lies within cut_all of i's center.  Complete search. */
+     * whose center is in reach of i's center.  Complete search (the reference
+     * bins elements with the atom stencil, neighbor.cpp:963-975, which cannot
+     * reach cut_all). */
     int cap = 256, nel = 0;
     eref_t *elist = (eref_t *)malloc(sizeof(eref_t) * cap);
 #define ELEM_TEST(j, img)                                                                  \
@@ -529,7 +605,8 @@ int orc_ip_neighbors(const orc_elems *el, const orc_box *box, double cutneigh, i
     double dy = xc[3 * i + 1] - shifted(xc[3 * (j) + 1], s_[1], prd[1]);                   \
     double dz = xc[3 * i + 2] - shifted(xc[3 * (j) + 2], s_[2], prd[2]);                   \
     double rsq = dx * dx + dy * dy + dz * dz;                                              \
-    if (rsq <= cut_allsq) {                                                                \
+    double reach = cutneigh + R[i] + R[(j)];                                               \
+    if (rsq <= reach * reach) {                                                            \
       if (nel == cap) { cap *= 2; elist = (eref_t *)realloc(elist, sizeof(eref_t) * cap); } \
       elist[nel].j = (j);                                                                  \
       elist[nel].img = (img);                                                              \
@@ -566,57 +643,13 @@ int orc_ip_neighbors(const orc_elems *el, const orc_box *box, double cutneigh, i
       for (int l = 0; l < nel; l++) {
         int j = elist[l].j, img = elist[l].img, s[3];
         img_shift(img, s);
-        double xj = shifted(xc[3 * j + 0], s[0], prd[0]);
-        double yj = shifted(xc[3 * j + 1], s[1], prd[1]);
-        double zj = shifted(xc[3 * j + 2], s[2], prd[2]);
-        double npdis = (ip[0] - xj) * (ip[0] - xj) + (ip[1] - yj) * (ip[1] - yj) + (ip[2] - zj) * (ip[2] - zj);
-        if (npdis > cut_elesq) continue;
-        double X[8][3];
-        shifted_nodes(el->nodex, j, img, prd, X);
-        for (int isub = 0; isub < sb.nsub; isub++) {
-          double sc[3];
-          interp3(sb.spa_center + 8 * isub, X, sc);
-          double nps = (ip[0] - sc[0]) * (ip[0] - sc[0]) + (ip[1] - sc[1]) * (ip[1] - sc[1]) +
-                       (ip[2] - sc[2]) * (ip[2] - sc[2]);
-          if (nps > cut_subelesq) continue;
-          for (int mm = 0; mm < sb.nsite[isub]; mm++) {
-            int site = sb.site[(size_t)isub * el->nsite + mm];
-            double sx[3];
-            interp3(el->spa_site + 8 * site, X, sx);
-            double delx = sx[0] - ip[0], dely = sx[1] - ip[1], delz = sx[2] - ip[2];
-            double rsq = delx * delx + dely * dely + delz * delz;
-            if (rsq <= cutsq) {
-              if (n < maxnip) out[n] = (uint64_t)j | ((uint64_t)img << 27) | ((uint64_t)site << 32);
-              n++;
-            }
-          }
-        }
+        double cj[3] = {shifted(xc[3 * j + 0], s[0], prd[0]), shifted(xc[3 * j + 1], s[1], prd[1]),
+                        shifted(xc[3 * j + 2], s[2], prd[2])};
+        n = scan_element(el, &sb, j, img, 0, ip, prd, cutneigh, R[j], cj, n, maxnip, out);
       }
-      /* self element, full_node_neighbor.cpp:176-224: rsq > 1e-4 drops the
-       * integration point's own site */
-      {
-        double X[8][3];
-        shifted_nodes(el->nodex, i, ORC_IMG_SELF, prd, X);
-        for (int isub = 0; isub < sb.nsub; isub++) {
-          double sc[3];
-          interp3(sb.spa_center + 8 * isub, X, sc);
-          double nps = (ip[0] - sc[0]) * (ip[0] - sc[0]) + (ip[1] - sc[1]) * (ip[1] - sc[1]) +
-                       (ip[2] - sc[2]) * (ip[2] - sc[2]);
-          if (nps > cut_subelesq) continue;
-          for (int mm = 0; mm < sb.nsite[isub]; mm++) {
-            int site = sb.site[(size_t)isub * el->nsite + mm];
-            double sx[3];
-            interp3(el->spa_site + 8 * site, X, sx);
-            double delx = sx[0] - ip[0], dely = sx[1] - ip[1], delz = sx[2] - ip[2];
-            double rsq = delx * delx + dely * dely + delz * delz;
-            if (rsq <= cutsq && rsq > 1.0e-4) {
-              if (n < maxnip)
-                out[n] = (uint64_t)i | ((uint64_t)ORC_IMG_SELF << 27) | ((uint64_t)site << 32);
-              n++;
-            }
-          }
-        }
-      }
+      /* self element last, full_node_neighbor.cpp:176-224: rsq > 1e-4 drops
+       * the integration point's own site */
+      n = scan_element(el, &sb, i, ORC_IMG_SELF, 1, ip, prd, cutneigh, R[i], xc + 3 * i, n, maxnip, out);
       if (n > maxnip) overflow = 1;
       numneigh[p] = n;
       if (n <= maxnip) qsort(out, n, sizeof(uint64_t), cmp_u64);
@@ -625,6 +658,7 @@ int orc_ip_neighbors(const orc_elems *el, const orc_box *box, double cutneigh, i
   }
   cells_free(&c);
   free(xc);
+  free(R);
   subele_free(&sb);
   return overflow ? -1 : 0;
 }

@@ -1,0 +1,104 @@
+"""Pins for FixNVE (fix_nve.cpp:40-242) and the periodic wraps
+(domain.cpp:535-745): exact quadratic under constant force, time
+reversibility, lumped node mass, center update, and pbc."""
+import numpy as np
+
+from cac_code_amd.lattice import FTM2V_METAL, build_cac_system
+from cac_code_amd.potential import lj_coeffs
+
+from gpu_common import jitter
+
+LJ = lj_coeffs(0.4093, 2.338, 5.0)
+
+
+def test_constant_force_is_exact_quadratic(orc):
+    rng = np.random.default_rng(0)
+    n, m, dt = 50, 63.546, 0.001
+    dtf = 0.5 * dt * FTM2V_METAL
+    x0 = rng.uniform(0, 10, (n, 3)); v0 = rng.normal(size=(n, 3)); f = rng.normal(size=(n, 3))
+    x, v = x0.copy(), v0.copy()
+    steps = 25
+    for _ in range(steps):
+        orc.nve_atoms_initial(x, v, f, dt, dtf, m)
+        orc.nve_atoms_final(v, f, dtf, m)
+    t = steps * dt
+    a = f * FTM2V_METAL / m
+    assert np.allclose(x, x0 + v0 * t + 0.5 * a * t * t, rtol=0, atol=1e-12)
+    assert np.allclose(v, v0 + a * t, rtol=0, atol=1e-12)
+
+
+def test_node_lumped_mass_and_center(orc):
+    rng = np.random.default_rng(1)
+    ne, me, dt = 7, 125 * 63.546, 0.001
+    dtf = 0.5 * dt * FTM2V_METAL
+    nodex = rng.uniform(0, 10, (ne, 8, 3)); nodev = rng.normal(size=(ne, 8, 3)); nodef = rng.normal(size=(ne, 8, 3))
+    xc = np.zeros((ne, 3))
+    x0, v0 = nodex.copy(), nodev.copy()
+    orc.nve_nodes_initial(nodex, nodev, nodef, xc, dt, dtf, me)
+    a = nodef * FTM2V_METAL / (me / 8)            # node mass m_e/8 (fix_nve.cpp:124)
+    assert np.allclose(nodev, v0 + 0.5 * a * dt, atol=1e-13)
+    assert np.allclose(nodex, x0 + (v0 + 0.5 * a * dt) * dt, atol=1e-13)
+    assert np.allclose(xc, nodex.mean(axis=1), atol=1e-13)
+
+
+def _lj_forces(orc, s, x):
+    num, nb = orc.atom_neighbors(x, s.box_lo, s.box_hi, 6.0, 200)
+    return orc.lj_atoms(x, s.box_lo, s.box_hi, num, nb, LJ)
+
+
+def test_time_reversibility(orc):
+    s = jitter(build_cac_system(6, 2, 6.0, temperature=50.0, seed=3), 0.05, 3)
+    dt, dtf, m = 0.001, 0.5 * 0.001 * FTM2V_METAL, s.mass
+    x, v = s.x.copy(), s.v.copy()
+    f = _lj_forces(orc, s, x)["f"]
+    for _ in range(30):
+        orc.nve_atoms_initial(x, v, f, dt, dtf, m)
+        f = _lj_forces(orc, s, x)["f"]
+        orc.nve_atoms_final(v, f, dtf, m)
+    v = -v
+    for _ in range(30):
+        orc.nve_atoms_initial(x, v, f, dt, dtf, m)
+        f = _lj_forces(orc, s, x)["f"]
+        orc.nve_atoms_final(v, f, dtf, m)
+    assert np.max(np.abs(x - s.x)) < 1e-9
+    assert np.max(np.abs(-v - s.v)) < 1e-7
+
+
+def test_energy_conservation_eam(orc, cu_tables):
+    """Velocity Verlet on the tapered EAM conserves the build-defined total
+    energy: 200 steps at 300 K drift far below kT per atom."""
+    s = jitter(build_cac_system(6, 2, 6.0, temperature=300.0, seed=8), 0.02, 8)
+    dt, dtf, m = 0.001, 0.5 * 0.001 * FTM2V_METAL, s.mass
+    x, v = s.x.copy(), s.v.copy()
+
+    def forces(x):
+        num, nb = orc.atom_neighbors(x, s.box_lo, s.box_hi, 5.95, 200)
+        return orc.eam_atoms(x, s.box_lo, s.box_hi, num, nb, cu_tables)
+
+    def etot(x, v, r):
+        return r["eatom"].sum() + 0.5 * m * (v * v).sum() / FTM2V_METAL
+
+    r = forces(x)
+    e0 = etot(x, v, r)
+    es = []
+    for _ in range(200):
+        orc.nve_atoms_initial(x, v, r["f"], dt, dtf, m)
+        r = forces(x)
+        orc.nve_atoms_final(v, r["f"], dtf, m)
+        es.append(etot(x, v, r))
+    drift = abs(es[-1] - e0) / s.n_atoms
+    assert drift < 1e-5, drift                      # kT(300 K) = 0.026 eV
+
+
+def test_pbc_wraps_and_moves_nodes(orc):
+    lo, hi = np.zeros(3), np.full(3, 10.0)
+    x = np.array([[-0.5, 5.0, 10.0], [10.25, -3.0, 4.0]])
+    orc.pbc_atoms(x, lo, hi)
+    assert np.array_equal(x, [[9.5, 5.0, 0.0], [0.25, 7.0, 4.0]])
+    nodex = np.tile(np.array([[10.5, 1.0, 1.0]]), (1, 8, 1)) + np.arange(8)[None, :, None] * 0.01
+    xc = nodex.mean(axis=1)
+    orc.pbc_elements(xc, nodex, lo, hi)
+    # the nodes follow the center (the reference adds the period at x-hi,
+    # domain.cpp:682; fixed)
+    assert np.allclose(xc[0, 0], 0.535)
+    assert np.allclose(nodex[0, :, 0], 0.5 + np.arange(8) * 0.01)
