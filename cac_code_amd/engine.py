@@ -224,7 +224,14 @@ class Engine:
         the whole cutoff (neighbor.cpp:334, 357-358), which would let pairs
         enter the force cutoff unlisted."""
         if skin is None:
-            skin = cutneigh
+            cut = 0.0
+            if isinstance(self.pair, PairEAM) and self.pair.funcfl is not None:
+                cut = self.pair.init_one(1, 1)
+            elif isinstance(self.pair, PairLJCut) and self.pair.cut is not None:
+                cut = self.pair.cut
+            skin = cutneigh - cut
+            if skin <= 0.0:
+                raise PairError("neighbor cutoff must exceed the force cutoff")
         triggersq = 0.25 * skin * skin
         _check(self.ctx, self.L.cac_set_neighbor(self.ctx, cutneigh, triggersq, every, delay, 1 if check else 0,
                                                  maxneigh, maxnip))
