@@ -223,6 +223,7 @@ int build_impl(cac_ctx *ctx) {
     if ((rc = read_flags(ctx))) return rc;
     if (ctx->hflags[4]) { ctx->maxnip = ctx->hflags[5] + 16; continue; }
     if ((rc = launch_active_sites(ctx))) return rc;
+    if (ctx->surr.enabled && (rc = surrogate_classify(ctx))) return rc;
     ctx->built = true;
     ctx->ago = 0;
     ctx->ncalls++;
@@ -340,6 +341,8 @@ int cac_ctx_destroy(cac_ctx *ctx) {
   if (!ctx) return CAC_OK;
   hipSetDevice(ctx->device);
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  mlp_destroy(ctx);
+  surrogate_destroy(ctx);
   DevBuf *bufs[] = {&ctx->b_x4, &ctx->b_v, &ctx->b_f, &ctx->b_xhold, &ctx->b_rho, &ctx->b_eatom, &ctx->b_vatom,
                     &ctx->b_aneigh, &ctx->b_anum, &ctx->b_akey, &ctx->b_aval, &ctx->b_akey2, &ctx->b_aval2,
                     &ctx->b_acs, &ctx->b_ace, &ctx->b_sorttmp, &ctx->b_nodex, &ctx->b_nodev, &ctx->b_nodef,

@@ -65,6 +65,27 @@ struct EamTables {
   DevBuf b_rv, b_rz, b_fr, b_rv_f, b_rz_f, b_fr_f;
 };
 
+struct MlpState;
+
+// Learned element surrogate (k_surrogate.hip): reference nodes, mesh
+// adjacency, scales, and the element / active-site subsets of the plain
+// path for fallback elements.
+struct SurrState {
+  bool setup = false, enabled = false, ready = false;
+  double strain_limit = 0.02;
+  double *r0 = nullptr;       // [ne][8][3] reference node offsets from the element centroid
+  double *d0 = nullptr;       // [ne][26][3] reference neighbor-centroid offsets
+  double *c0 = nullptr;       // [ne][56][3] reference offsets of the neighbor nodes meeting each corner
+  float *lin_A = nullptr, *lin_b = nullptr;  // [24][270], [24] linear part of the prediction
+  int *nbr = nullptr;         // [ne][26] mesh neighbors, -1 if absent
+  float *inv_in = nullptr;    // [270] 1 / input scale
+  float *out_scale = nullptr; // [24]
+  double *ubar = nullptr;     // [ne][3] current element centroids (per step)
+  int nF = 0, nR = 0, nR2 = 0, nS = 0, nsR = 0, nsR2 = 0, ninterface = 0, nstrained = 0;
+  int *elemF = nullptr, *elemR = nullptr, *elemR2 = nullptr, *elemS = nullptr, *siteR = nullptr, *siteR2 = nullptr;
+  DevBuf b_X0, b_d0, b_c0, b_A, b_b, b_xf, b_nbr, b_inv_in, b_out, b_cls, b_mark, b_F, b_S, b_R, b_R2, b_sR, b_sR2, b_tmp, b_ubar, b_feat;
+};
+
 struct LjParams {
   double lj1 = 0, lj2 = 0, lj3 = 0, lj4 = 0, cutsq = 0;
 };
@@ -164,6 +185,10 @@ struct cac_ctx {
   int *hflags = nullptr;  // pinned mirror
   cac::DevBuf b_dflags;
 
+  // ---- learned element surrogate (k_mlp.hip, k_surrogate.hip)
+  cac::MlpState *mlp = nullptr;
+  cac::SurrState surr;
+
   // ---- profiling
   bool profiling = false;
   std::map<std::string, cac::Timer> timers;
@@ -220,6 +245,12 @@ int launch_scatter(cac_ctx *ctx);
 
 // ---- launchers (cac_api.hip)
 int launch_reduce_ev(cac_ctx *ctx, int eflag, int vflag);
+
+// ---- MLP and surrogate (k_mlp.hip, k_surrogate.hip)
+void mlp_destroy(cac_ctx *ctx);
+int surrogate_classify(cac_ctx *ctx);
+int surrogate_forces(cac_ctx *ctx);
+void surrogate_destroy(cac_ctx *ctx);
 
 // host table construction (tables.cpp)
 int eam_regrid_spline(int nrho_f, double drho_f, int nr_f, double dr_f, const double *frho_f,

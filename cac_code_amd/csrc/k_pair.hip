@@ -390,13 +390,24 @@ struct ElemArgs {
   const int *ipnum;          // [nip]
   const int2 *asite;         // [nactive] (element, site)
   int node_list[8];
+  // optional subsets (surrogate mode): elements for the erho / force /
+  // scatter passes, active sites for the site passes; null = all
+  const int *elist;
+  int nelist;
+  const int *slist;
+  int nslist;
+  __device__ __forceinline__ int n_elem() const { return elist ? nelist : ne; }
+  __device__ __forceinline__ int elem(int o) const { return elist ? elist[o] : o; }
+  __device__ __forceinline__ int n_site() const { return slist ? nslist : nactive; }
+  __device__ __forceinline__ int site(int o) const { return slist ? slist[o] : o; }
 };
 
 // Active-site positions: sum_k N_k(site) X_k in node order
 // (pair_eam.cpp:1071-1076, element.cpp:1251-1271).
 __global__ __launch_bounds__(kBlock) void k_site_pos(ElemArgs a, double *__restrict__ xs4) {
-  const int q = xcd_block() * kBlock + (int)threadIdx.x;
-  if (q >= a.nactive) return;
+  const int o = xcd_block() * kBlock + (int)threadIdx.x;
+  if (o >= a.n_site()) return;
+  const int q = a.site(o);
   const int2 js = a.asite[q];
   const double *X = a.nodex + 24 * (size_t)js.x;
   const double *N = a.spa_site + 8 * (size_t)js.y;
@@ -415,8 +426,9 @@ __global__ __launch_bounds__(kBlock) void k_site_pos(ElemArgs a, double *__restr
 template <class P>
 __global__ __launch_bounds__(kBlock) void k_site_fp(ElemArgs a, Tabs<P> t, const double *__restrict__ erho,
                                                     double *__restrict__ xs4) {
-  const int q = xcd_block() * kBlock + (int)threadIdx.x;
-  if (q >= a.nactive) return;
+  const int o = xcd_block() * kBlock + (int)threadIdx.x;
+  if (o >= a.n_site()) return;
+  const int q = a.site(o);
   const int2 js = a.asite[q];
   const double *E = erho + 8 * (size_t)js.x;
   const double *N = a.spa_site + 8 * (size_t)js.y;
@@ -431,10 +443,12 @@ template <class P>
 __global__ __launch_bounds__(kBlock) void k_eam_erho(ElemArgs a, Prd prd, Tabs<P> t, const double *__restrict__ xs4,
                                                      double *__restrict__ erho) {
   const int gt = xcd_block() * kBlock + (int)threadIdx.x;
-  const int q = gt >> 3, l = gt & 7;
+  const int o = gt >> 3, l = gt & 7;
+  const bool live = o < 8 * a.n_elem();
+  const int i = live ? a.elem(o >> 3) : 0, k = o & 7;
+  const int q = 8 * i + k;
   double acc = 0.0;
-  if (q < 8 * a.ne) {
-    const int i = q >> 3, k = q & 7;
+  if (live) {
     const double ix = a.nodex[24 * (size_t)i + 3 * k], iy = a.nodex[24 * (size_t)i + 3 * k + 1],
                  iz = a.nodex[24 * (size_t)i + 3 * k + 2];
     const int p = i * a.nint + a.node_list[k];
@@ -457,7 +471,7 @@ __global__ __launch_bounds__(kBlock) void k_eam_erho(ElemArgs a, Prd prd, Tabs<P
     }
   }
   acc = group_sum(acc);
-  if (q < 8 * a.ne && l == 0) erho[q] = acc;
+  if (live && l == 0) erho[q] = acc;
 }
 
 // Integration-point force, pair_eam.cpp:1013-1129.  Writes f_g to ipf
@@ -468,13 +482,15 @@ __global__ __launch_bounds__(kBlock) void k_eam_ip_force(ElemArgs a, Prd prd, Ta
                                                          const double *__restrict__ xs4, double *__restrict__ ipf,
                                                          double *__restrict__ eip, double *__restrict__ vip) {
   const int gt = xcd_block() * kBlock + (int)threadIdx.x;
-  const int p = gt >> 3, l = gt & 7;
+  const int o = gt >> 3, l = gt & 7;
+  const bool live = o < a.n_elem() * a.nint;
+  const int i = live ? a.elem(o / a.nint) : 0, g = o % a.nint;
+  const int p = i * a.nint + g;
   double fx = 0.0, fy = 0.0, fz = 0.0, ep = 0.0, rhoi = 0.0;
   P fpi = P(0);
   Virial vir;
   vir.zero();
-  if (p < a.nip) {
-    const int i = p / a.nint, g = p % a.nint;
+  if (live) {
     const double *Ng = a.spa_int + 8 * g;
     double ix = 0.0, iy = 0.0, iz = 0.0;
     for (int kk = 0; kk < 8; ++kk) {
@@ -529,7 +545,7 @@ __global__ __launch_bounds__(kBlock) void k_eam_ip_force(ElemArgs a, Prd prd, Ta
     ep = group_sum(ep);
     vir.reduce();
   }
-  if (p < a.nip && l == 0) {
+  if (live && l == 0) {
     ipf[p] = fx;
     ipf[(size_t)a.nip + p] = fy;
     ipf[2 * (size_t)a.nip + p] = fz;
@@ -547,13 +563,15 @@ __global__ __launch_bounds__(kBlock) void k_lj_ip_force(ElemArgs a, Prd prd, LjP
                                                         const double *__restrict__ xs4, double *__restrict__ ipf,
                                                         double *__restrict__ eip, double *__restrict__ vip) {
   const int gt = xcd_block() * kBlock + (int)threadIdx.x;
-  const int p = gt >> 3, l = gt & 7;
+  const int o = gt >> 3, l = gt & 7;
+  const bool live = o < a.n_elem() * a.nint;
+  const int i = live ? a.elem(o / a.nint) : 0, g = o % a.nint;
+  const int p = i * a.nint + g;
   const P lj1 = (P)lj.lj1, lj2 = (P)lj.lj2, lj3 = (P)lj.lj3, lj4 = (P)lj.lj4;
   double fx = 0.0, fy = 0.0, fz = 0.0, ep = 0.0;
   Virial vir;
   vir.zero();
-  if (p < a.nip) {
-    const int i = p / a.nint, g = p % a.nint;
+  if (live) {
     const double *Ng = a.spa_int + 8 * g;
     double ix = 0.0, iy = 0.0, iz = 0.0;
     for (int kk = 0; kk < 8; ++kk) {
@@ -590,7 +608,7 @@ __global__ __launch_bounds__(kBlock) void k_lj_ip_force(ElemArgs a, Prd prd, LjP
     ep = group_sum(ep);
     vir.reduce();
   }
-  if (p < a.nip && l == 0) {
+  if (live && l == 0) {
     ipf[p] = fx;
     ipf[(size_t)a.nip + p] = fy;
     ipf[2 * (size_t)a.nip + p] = fz;
@@ -607,9 +625,10 @@ __global__ __launch_bounds__(kBlock) void k_lj_ip_force(ElemArgs a, Prd prd, LjP
 // from the cleared force (Verlet::node_force_clear, verlet.cpp:131-144).
 // One thread per node, no atomics, fixed order.
 __global__ void k_scatter(ElemArgs a, const double *__restrict__ ipf, double *__restrict__ nodef) {
-  int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= 8 * a.ne) return;
-  const int i = q >> 3, k = q & 7;
+  const int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= 8 * a.n_elem()) return;
+  const int i = a.elem(o >> 3), k = o & 7;
+  const int q = 8 * i + k;
   double fx = 0.0, fy = 0.0, fz = 0.0;
   for (int g = 0; g < a.nint; ++g) {
     const int p = i * a.nint + g;
@@ -677,6 +696,10 @@ ElemArgs elem_args(const cac_ctx *ctx) {
   a.ipnum = ctx->ipnum;
   a.asite = ctx->asite;
   for (int k = 0; k < 8; ++k) a.node_list[k] = ctx->node_list[k];
+  a.elist = nullptr;
+  a.nelist = 0;
+  a.slist = nullptr;
+  a.nslist = 0;
   return a;
 }
 
@@ -698,37 +721,75 @@ int eam_atoms_t(cac_ctx *ctx) {
   return CAC_OK;
 }
 
+long host_nsite(const ElemArgs &a) { return a.slist ? a.nslist : a.nactive; }
+long host_nelem(const ElemArgs &a) { return a.elist ? a.nelist : a.ne; }
+
 int site_positions(cac_ctx *ctx, const ElemArgs &a) {
-  if (a.nactive == 0) return CAC_OK;
+  if (host_nsite(a) == 0) return CAC_OK;
   hipEvent_t t0;
   timer_begin(ctx, "site_pos", &t0);
-  hipLaunchKernelGGL(k_site_pos, dim3(xcd_grid(a.nactive, kBlock)), dim3(kBlock), 0, ctx->stream, a, ctx->xs4);
+  hipLaunchKernelGGL(k_site_pos, dim3(xcd_grid(host_nsite(a), kBlock)), dim3(kBlock), 0, ctx->stream, a, ctx->xs4);
   timer_end(ctx, "site_pos", t0);
+  return CAC_OK;
+}
+
+ElemArgs with_elems(ElemArgs a, const int *el, int n) {
+  a.elist = el;
+  a.nelist = n;
+  return a;
+}
+ElemArgs with_sites(ElemArgs a, const int *sl, int n) {
+  a.slist = sl;
+  a.nslist = n;
+  return a;
+}
+
+// Element EAM passes over optional subsets: site positions over `spos`,
+// node densities over `erho_el`, site F' over `sfp`, forces and scatter over
+// `force_el` (all elements when a subset is null).
+template <class P, bool EV>
+int eam_elements_sub(cac_ctx *ctx, const ElemArgs &base, const ElemArgs &spos, const ElemArgs &erho_el,
+                     const ElemArgs &sfp, const ElemArgs &force_el) {
+  Tabs<P> t = tabs_of<P>(ctx);
+  site_positions(ctx, spos);
+  hipEvent_t t0;
+  timer_begin(ctx, "ip_erho", &t0);
+  if (host_nelem(erho_el))
+    hipLaunchKernelGGL((k_eam_erho<P>), dim3(xcd_grid(8L * host_nelem(erho_el) * kLanes, kBlock)), dim3(kBlock), 0,
+                       ctx->stream, erho_el, prd_of(ctx), t, ctx->xs4, ctx->erho);
+  timer_end(ctx, "ip_erho", t0);
+  if (host_nsite(sfp)) {
+    timer_begin(ctx, "site_fp", &t0);
+    hipLaunchKernelGGL((k_site_fp<P>), dim3(xcd_grid(host_nsite(sfp), kBlock)), dim3(kBlock), 0, ctx->stream, sfp, t,
+                       ctx->erho, ctx->xs4);
+    timer_end(ctx, "site_fp", t0);
+  }
+  const long nf = host_nelem(force_el);
+  if (nf) {
+    timer_begin(ctx, "ip_force", &t0);
+    hipLaunchKernelGGL((k_eam_ip_force<P, EV>), dim3(xcd_grid(nf * base.nint * kLanes, kBlock)), dim3(kBlock), 0,
+                       ctx->stream, force_el, prd_of(ctx), t, ctx->erho, ctx->xs4, ctx->ipf, ctx->eip, ctx->vip);
+    timer_end(ctx, "ip_force", t0);
+    timer_begin(ctx, "scatter", &t0);
+    hipLaunchKernelGGL(k_scatter, dim3(grid_for(8L * nf)), dim3(kBlock), 0, ctx->stream, force_el, ctx->ipf,
+                       ctx->nodef);
+    timer_end(ctx, "scatter", t0);
+  }
+  CAC_HIP(ctx, hipGetLastError());
   return CAC_OK;
 }
 
 template <class P, bool EV>
 int eam_elements_t(cac_ctx *ctx) {
   ElemArgs a = elem_args(ctx);
-  Tabs<P> t = tabs_of<P>(ctx);
-  site_positions(ctx, a);
-  hipEvent_t t0;
-  timer_begin(ctx, "ip_erho", &t0);
-  hipLaunchKernelGGL((k_eam_erho<P>), dim3(xcd_grid(8L * a.ne * kLanes, kBlock)), dim3(kBlock), 0, ctx->stream, a,
-                     prd_of(ctx), t, ctx->xs4, ctx->erho);
-  timer_end(ctx, "ip_erho", t0);
-  if (a.nactive) {
-    timer_begin(ctx, "site_fp", &t0);
-    hipLaunchKernelGGL((k_site_fp<P>), dim3(xcd_grid(a.nactive, kBlock)), dim3(kBlock), 0, ctx->stream, a, t,
-                       ctx->erho, ctx->xs4);
-    timer_end(ctx, "site_fp", t0);
+  const SurrState &S = ctx->surr;
+  if (S.enabled && S.ready && !EV) {
+    int rc = eam_elements_sub<P, false>(ctx, a, with_sites(a, S.siteR2, S.nsR2), with_elems(a, S.elemR, S.nR),
+                                        with_sites(a, S.siteR, S.nsR), with_elems(a, S.elemF, S.nF));
+    if (rc) return rc;
+    return surrogate_forces(ctx);
   }
-  timer_begin(ctx, "ip_force", &t0);
-  hipLaunchKernelGGL((k_eam_ip_force<P, EV>), dim3(xcd_grid((long)a.nip * kLanes, kBlock)), dim3(kBlock), 0,
-                     ctx->stream, a, prd_of(ctx), t, ctx->erho, ctx->xs4, ctx->ipf, ctx->eip, ctx->vip);
-  timer_end(ctx, "ip_force", t0);
-  CAC_HIP(ctx, hipGetLastError());
-  return launch_scatter(ctx);
+  return eam_elements_sub<P, EV>(ctx, a, a, a, a, a);
 }
 
 template <class P, bool EV>

@@ -145,7 +145,91 @@ def eam_tables(ff: Funcfl):
     return out
 
 
-class Engine:
+class _MlpMixin:
+    """Device MLP and surrogate entry points of Engine (cac_mlp_*, cac_surrogate_*)."""
+
+    def mlp_create(self, nin=270, nhidden=256, nlayers=3, nout=24, seed=0):
+        _check(self.ctx, self.L.cac_mlp_create(self.ctx, nin, nhidden, nlayers, nout, seed))
+        self.mlp_shape = (nin, nhidden, nlayers, nout)
+
+    def _layer_dims(self):
+        nin, nh, nl, nout = self.mlp_shape
+        dims = [nin] + [nh] * nl + [nout]
+        return [(dims[l + 1], dims[l]) for l in range(nl + 1)]
+
+    def mlp_get_weights(self):
+        out = []
+        for l, (o, i) in enumerate(self._layer_dims()):
+            w = np.zeros((o, i), dtype=np.float32); b = np.zeros(o, dtype=np.float32)
+            _check(self.ctx, self.L.cac_mlp_get_weights(self.ctx, l, _p(w, _lib._f), _p(b, _lib._f)))
+            out.append((w, b))
+        return out
+
+    def mlp_set_weights(self, weights):
+        for l, (w, b) in enumerate(weights):
+            w = np.ascontiguousarray(w, dtype=np.float32); b = np.ascontiguousarray(b, dtype=np.float32)
+            _check(self.ctx, self.L.cac_mlp_set_weights(self.ctx, l, _p(w, _lib._f), _p(b, _lib._f)))
+
+    def mlp_forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros((x.shape[0], self.mlp_shape[3]), dtype=np.float32)
+        _check(self.ctx, self.L.cac_mlp_forward(self.ctx, x.shape[0], _p(x, _lib._f), _p(y, _lib._f)))
+        return y
+
+    def mlp_grads(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32); t = np.ascontiguousarray(t, dtype=np.float32)
+        dims = self._layer_dims()
+        bufs = [np.zeros(o * i + o, dtype=np.float32) for o, i in dims]
+        arr = (_lib._f * len(bufs))(*[_p(b, _lib._f) for b in bufs])
+        loss = C.c_float()
+        _check(self.ctx, self.L.cac_mlp_grads(self.ctx, x.shape[0], _p(x, _lib._f), _p(t, _lib._f), C.byref(loss),
+                                              arr))
+        grads = [(b[:o * i].reshape(o, i), b[o * i:]) for b, (o, i) in zip(bufs, dims)]
+        return loss.value, grads
+
+    def mlp_train_step(self, x, t, lr):
+        x = np.ascontiguousarray(x, dtype=np.float32); t = np.ascontiguousarray(t, dtype=np.float32)
+        loss = C.c_float()
+        _check(self.ctx, self.L.cac_mlp_train_step(self.ctx, x.shape[0], _p(x, _lib._f), _p(t, _lib._f), lr,
+                                                   C.byref(loss)))
+        return loss.value
+
+    def mlp_set_dataset(self, x, t):
+        self._ds = (np.ascontiguousarray(x, dtype=np.float32), np.ascontiguousarray(t, dtype=np.float32))
+        _check(self.ctx, self.L.cac_mlp_set_dataset(self.ctx, self._ds[0].shape[0], _p(self._ds[0], _lib._f),
+                                                    _p(self._ds[1], _lib._f)))
+
+    def mlp_train(self, epochs, batch, lr, seed=0):
+        loss = C.c_float()
+        _check(self.ctx, self.L.cac_mlp_train(self.ctx, epochs, batch, lr, seed, C.byref(loss)))
+        return loss.value
+
+    def surrogate_setup(self, X0, nbr, in_scale, out_scale, strain_limit=0.02):
+        X0 = np.ascontiguousarray(X0, dtype=np.float64); nbr = np.ascontiguousarray(nbr, dtype=np.int32)
+        a = np.ascontiguousarray(in_scale, dtype=np.float32); b = np.ascontiguousarray(out_scale, dtype=np.float32)
+        _check(self.ctx, self.L.cac_surrogate_setup(self.ctx, _p(X0), _p(nbr, _lib._i), _p(a, _lib._f),
+                                                    _p(b, _lib._f), strain_limit))
+
+    def surrogate_set_linear(self, A, b):
+        A = np.ascontiguousarray(A, dtype=np.float32); b = np.ascontiguousarray(b, dtype=np.float32)
+        _check(self.ctx, self.L.cac_surrogate_set_linear(self.ctx, _p(A, _lib._f), _p(b, _lib._f)))
+
+    def surrogate_enable(self, on=True):
+        _check(self.ctx, self.L.cac_surrogate_enable(self.ctx, int(on)))
+
+    def surrogate_features(self):
+        f = np.zeros((self.ne, 270), dtype=np.float32); c = np.zeros(self.ne, dtype=np.int32)
+        _check(self.ctx, self.L.cac_surrogate_features(self.ctx, _p(f, _lib._f), _p(c, _lib._i)))
+        return f, c
+
+    def surrogate_info(self):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        _check(self.ctx, self.L.cac_surrogate_info(self.ctx, C.byref(a), C.byref(b), C.byref(c)))
+        return dict(interface=a.value, strained=b.value, surrogate=c.value)
+
+
+
+class Engine(_MlpMixin):
     """One device context (cac_ctx) and the run state around it."""
 
     def __init__(self, device: int = 0):
@@ -321,3 +405,4 @@ class Engine:
         cnt = C.c_int()
         ms = self.L.cac_profile_avg_ms(self.ctx, name.encode(), C.byref(cnt))
         return ms, cnt.value
+

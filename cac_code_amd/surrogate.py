@@ -1,0 +1,268 @@
+"""Learned element surrogate: data from the plain integration-point path,
+training with the device MLP, and deployment on the force path.
+
+Inputs (270, computed on the device by k_surrogate.hip): the 24
+element-frame nodal displacements, the 78 neighbor-centroid displacements and
+the 168 displacements of the 7 neighbor nodes that meet each of the 8 corners.
+Outputs: the element's 24 nodal forces.  Prediction = A x + b (FP32, fitted
+by least squares in FP64) + MLP(x), the MLP 270 -> 256 -> 256 -> 256 -> 24,
+SiLU, BF16 weights and activations, FP32 accumulation (k_mlp.hip), trained
+on the residual of the linear part.
+
+Why not the 24 + 78 inputs alone: a node's force depends on the nodes of the
+7 neighbor elements that meet it (the corner integration point's sites lie in
+those elements), so under independent nodal thermal motion the force is not
+a function of the 102 inputs: a least-squares fit on them leaves ~70% of the
+nodal force unexplained at 300 K, and ~5% with the corner-node inputs added;
+the MLP then fits the nonlinear remainder.
+
+Training data: periodic all-element fcc boxes of the same element type under
+random normal strains up to `strain_max` (an orthogonal periodic box cannot
+carry shear), periodic long-wave displacement modes (local strain variation,
+including shear), and per-node Gaussian noise of the equipartition amplitude
+sqrt(kB T / k_node) at `temperature`, with k_node the diagonal stiffness of
+one node measured on the plain path; targets are the FP64 nodal forces of the
+plain path on the same device.  Elements outside the envelope (interface
+elements, strain above the limit) fall back to the plain path at run time.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .engine import Engine, PairEAM
+from .lattice import CACSystem, build_cac_system
+
+NIN, NHID, NLAYERS, NOUT = 270, 256, 3, 24
+BOLTZ_METAL = 8.617343e-5
+
+
+def mesh_neighbors(s: CACSystem) -> np.ndarray:
+    """[ne][26] element indices of the 26 neighboring blocks (di, dj, dk in
+    {-1,0,1}^3 minus 0, i outermost), -1 where the block is not an element.
+    Blocks are matched through their centroids wrapped into the box (doubled
+    a/2 integer units, as lattice.fcc_box_blocks enumerates them)."""
+    if s.blocks is None:
+        raise ValueError("system has no block indices (not built by the mesher)")
+    nae, L = s.nae[0], s.cells
+    lim = 4 * L
+
+    def key(B):
+        cx = (2 * nae * (B[..., 0] + B[..., 1]) + 2 * (nae - 1) + 1) % lim
+        cy = (2 * nae * (B[..., 0] + B[..., 2]) + 2 * (nae - 1) + 1) % lim
+        cz = (2 * nae * (B[..., 1] + B[..., 2]) + 2 * (nae - 1) + 1) % lim
+        return (cx.astype(np.int64) * lim + cy) * lim + cz
+
+    k0 = key(s.blocks)
+    order = np.argsort(k0)
+    ks = k0[order]
+    offs = np.array([(a, b, c) for a in (-1, 0, 1) for b in (-1, 0, 1) for c in (-1, 0, 1)
+                     if (a, b, c) != (0, 0, 0)], dtype=np.int64)
+    kn = key(s.blocks[:, None, :] + offs[None, :, :])
+    pos = np.clip(np.searchsorted(ks, kn), 0, len(ks) - 1)
+    nbr = np.where(ks[pos] == kn, order[pos], -1).astype(np.int32)
+    return nbr
+
+
+def _periodic_modes(rng, box, nmodes, strain):
+    """Displacement field sum_m A_m sin(k_m . x + phi_m), |k| = 2pi/L along
+    random lattice directions, amplitude giving local strain ~ strain."""
+    L = box
+    modes = []
+    for _ in range(nmodes):
+        n = rng.integers(-1, 2, 3)
+        if not n.any():
+            n[rng.integers(0, 3)] = 1
+        k = 2 * np.pi * n / L
+        amp = rng.normal(size=3)
+        amp *= strain * rng.uniform(0, 1) / (np.linalg.norm(amp) * np.linalg.norm(k))
+        modes.append((k, amp, rng.uniform(0, 2 * np.pi)))
+    return modes
+
+
+def _apply(x, modes):
+    u = np.zeros_like(x)
+    for k, amp, phi in modes:
+        u += np.sin(x @ k + phi)[..., None] * amp
+    return u
+
+
+@dataclass
+class Dataset:
+    x: np.ndarray           # [n][270] raw features
+    t: np.ndarray           # [n][24] nodal forces (eV/A)
+    meta: dict = field(default_factory=dict)
+
+
+def _plain_nodef(eng, base, box_hi, nodex, cutneigh):
+    eng.set_box(base.box_lo, box_hi)
+    eng.upload_elements(nodex, None, base.nae, base.mass_e, base.intp_ijk, base.intp_w)
+    eng.upload_atoms(np.zeros((0, 3)), None, base.mass)
+    eng.set_neighbor(cutneigh, skin=1.0)
+    eng.build()
+    eng.force()
+    return eng.elements()[2]
+
+
+def node_stiffness(eng, base, cutneigh: float = 5.95, h: float = 0.01) -> float:
+    """Mean diagonal stiffness (eV/A^2) of one node of the unstrained box:
+    central differences of its own force under +-h moves along x, y, z."""
+    X0 = base.nodex
+    k = []
+    for d in range(3):
+        fs = []
+        for sgn in (1.0, -1.0):
+            X = X0.copy()
+            X[0, 0, d] += sgn * h
+            fs.append(_plain_nodef(eng, base, base.box_hi, X, cutneigh)[0, 0, d])
+        k.append(-(fs[0] - fs[1]) / (2 * h))
+    return float(np.mean(k))
+
+
+def generate_dataset(pair: PairEAM, n_samples: int, L: int = 66, nae: int = 11, seed: int = 1234,
+                     strain_max: float = 0.02, mode_strain: float = 0.004, temperature: float = 300.0,
+                     cutneigh: float = 5.95, device: int = 0, log=None) -> Dataset:
+    """Plain-path (FP64, GPU) nodal forces of strained, thermally displaced
+    all-element boxes; features from the device feature kernel.  Each
+    configuration draws its strain magnitude uniformly in [0, strain_max] and
+    its noise amplitude uniformly in [0.25, 1.5] x the equipartition one."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    base = build_cac_system(L, nae, 0.0)
+    ne = base.n_elements
+    nbr = mesh_neighbors(base)
+    X0 = base.nodex.copy()
+    prd = base.box_hi - base.box_lo
+    xs, ts = [], []
+    t0 = time.perf_counter()
+    ncfg = 0
+    with Engine(device) as eng:
+        eng.load_system(base)
+        eng.set_pair(pair)
+        knode = node_stiffness(eng, base, cutneigh)
+        sigma = float(np.sqrt(BOLTZ_METAL * temperature / knode))
+        ones = np.ones(NIN, dtype=np.float32)
+        eng.surrogate_setup(X0, nbr, ones, np.ones(NOUT, dtype=np.float32), strain_limit=10.0)
+        total = 0
+        while total < n_samples:
+            eps = rng.uniform(-1, 1, 3)
+            eps *= strain_max * rng.uniform(0, 1) / max(np.abs(eps).max(), 1e-12)
+            F = 1.0 + eps
+            modes = _periodic_modes(rng, prd, 3, mode_strain)
+            noise = sigma * rng.uniform(0.25, 1.5)
+            nodex = X0 * F + _apply(X0, modes) * F + noise * rng.standard_normal(X0.shape)
+            nodef = _plain_nodef(eng, base, base.box_lo + prd * F, nodex, cutneigh)
+            f, _ = eng.surrogate_features()
+            xs.append(f)
+            ts.append(nodef.reshape(ne, 24).astype(np.float32))
+            total += ne
+            ncfg += 1
+    x = np.concatenate(xs)[:n_samples]
+    t = np.concatenate(ts)[:n_samples]
+    if log:
+        log(f"[surrogate] dataset: {x.shape[0]} samples from {ncfg} configurations in {time.perf_counter() - t0:.1f}s"
+            f" (k_node {knode:.1f} eV/A^2, sigma {sigma:.4f} A at {temperature} K)")
+    return Dataset(x, t, dict(L=L, nae=nae, seed=seed, strain_max=strain_max, mode_strain=mode_strain,
+                              temperature=temperature, k_node=knode, sigma=sigma, configurations=ncfg))
+
+
+@dataclass
+class Model:
+    weights: list           # MLP layers [(W [out][in], b [out])], FP32 masters
+    in_scale: np.ndarray    # [270]
+    out_scale: np.ndarray   # [24]
+    lin_A: np.ndarray       # [24][270] linear part, normalized units
+    lin_b: np.ndarray       # [24]
+    history: list = field(default_factory=list)
+
+
+def fit_linear(xn: np.ndarray, tn: np.ndarray):
+    """Least-squares A, b minimizing |xn A^T + b - tn| (FP64 normal equations
+    on centered data)."""
+    xm = xn.mean(axis=0, dtype=np.float64)
+    tm = tn.mean(axis=0, dtype=np.float64)
+    G = np.zeros((xn.shape[1], xn.shape[1]))
+    R = np.zeros((xn.shape[1], tn.shape[1]))
+    for i in range(0, len(xn), 65536):
+        xc = xn[i:i + 65536].astype(np.float64) - xm
+        G += xc.T @ xc
+        R += xc.T @ (tn[i:i + 65536].astype(np.float64) - tm)
+    G += 1e-9 * np.trace(G) / len(G) * np.eye(len(G))
+    A = np.linalg.solve(G, R).T
+    return A, tm - A @ xm
+
+
+def train(ds: Dataset, epochs: int = 60, batch: int = 1024, lr: float = 1e-3, seed: int = 0, device: int = 0,
+          log=None, lr_decay_to: float = 0.05) -> Model:
+    """Linear least squares, then Adam on the device MLP over the residual
+    (scaled to unit RMS per output while training, the scale folded into the
+    last layer after); the rate falls geometrically from lr to
+    lr*lr_decay_to over the epochs."""
+    in_scale = np.sqrt(np.mean(ds.x.astype(np.float64) ** 2, axis=0)).astype(np.float32)
+    in_scale = np.maximum(in_scale, 1e-6 * max(float(in_scale.max()), 1e-12)).astype(np.float32)
+    out_scale = np.sqrt(np.mean(ds.t.astype(np.float64) ** 2, axis=0)).astype(np.float32)
+    out_scale = np.maximum(out_scale, 1e-6 * max(float(out_scale.max()), 1e-12)).astype(np.float32)
+    xn = ds.x / in_scale
+    tn = ds.t / out_scale
+    A, b = fit_linear(xn, tn)
+    A = A.astype(np.float32)
+    b = b.astype(np.float32)
+    res = tn - (xn @ A.T + b)
+    r_scale = np.sqrt(np.mean(res.astype(np.float64) ** 2, axis=0))
+    r_scale = np.maximum(r_scale, 1e-6 * max(float(r_scale.max()), 1e-12))
+    if log:
+        log(f"[surrogate] linear part: train rel RMS residual {float(np.sqrt(np.mean(res ** 2))):.4f}")
+    tn = (res / r_scale).astype(np.float32)
+    hist = []
+    t0 = time.perf_counter()
+    with Engine(device) as eng:
+        eng.mlp_create(NIN, NHID, NLAYERS, NOUT, seed)
+        eng.mlp_set_dataset(xn, tn)
+        chunk = max(1, epochs // 10)
+        done = 0
+        while done < epochs:
+            e = min(chunk, epochs - done)
+            rate = lr * (lr_decay_to ** (done / max(epochs - 1, 1)))
+            loss = eng.mlp_train(e, batch, rate, seed + done)
+            done += e
+            hist.append((done, rate, loss))
+            if log:
+                log(f"[surrogate] epoch {done}/{epochs} lr {rate:.2e} loss {loss:.3e}")
+        w = eng.mlp_get_weights()
+    W, bl = w[-1]
+    w[-1] = ((W * r_scale[:, None]).astype(np.float32), (bl * r_scale).astype(np.float32))
+    if log:
+        log(f"[surrogate] trained in {time.perf_counter() - t0:.1f}s")
+    return Model(w, in_scale, out_scale, A, b, hist)
+
+
+def predict(model: Model, x: np.ndarray, device: int = 0, linear_only: bool = False) -> np.ndarray:
+    """Nodal forces [n][24] for raw features x [n][270]: the linear part on
+    the host (FP32), the MLP on the device."""
+    xn = (x / model.in_scale).astype(np.float32)
+    y = xn @ model.lin_A.T + model.lin_b
+    if not linear_only:
+        with Engine(device) as eng:
+            eng.mlp_create(NIN, NHID, NLAYERS, NOUT, 0)
+            eng.mlp_set_weights(model.weights)
+            y = y + eng.mlp_forward(xn)
+    return y * model.out_scale
+
+
+def rel_rms(pred, ref):
+    pred = np.asarray(pred, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    return float(np.sqrt(np.mean((pred - ref) ** 2)) / np.sqrt(np.mean(ref ** 2)))
+
+
+def attach(eng: Engine, model: Model, s: CACSystem, strain_limit: float = 0.02):
+    """Load the model into eng's context and switch its element forces to
+    the surrogate (fallback elements keep the plain path)."""
+    eng.mlp_create(NIN, NHID, NLAYERS, NOUT, 0)
+    eng.mlp_set_weights(model.weights)
+    eng.surrogate_setup(s.nodex if getattr(s, "X0", None) is None else s.X0, mesh_neighbors(s), model.in_scale,
+                        model.out_scale, strain_limit)
+    eng.surrogate_set_linear(model.lin_A, model.lin_b)
+    eng.surrogate_enable(True)
+    return eng.surrogate_info()

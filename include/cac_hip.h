@@ -156,6 +156,43 @@ int cac_get_ip_forces(cac_ctx *ctx, double *ipf);
 int cac_profile_enable(cac_ctx *ctx, int on);
 double cac_profile_avg_ms(cac_ctx *ctx, const char *name, int *count);
 
+/* ---- learned element surrogate (no counterpart in the reference) --------
+ * A BF16-MFMA multilayer perceptron in -> nhidden (x nlayers, SiLU) -> out
+ * with FP32 accumulation and FP32 master weights, trained on the device with
+ * Adam (lr given, beta 0.9/0.999, eps 1e-8) on the mean-squared error.
+ * Weights are [out][in] row-major FP32 on the host side.  Inputs and targets
+ * are FP32 [n][in] / [n][out]; the caller normalizes them. */
+int cac_mlp_create(cac_ctx *ctx, int nin, int nhidden, int nlayers, int nout, uint64_t seed);
+int cac_mlp_set_weights(cac_ctx *ctx, int layer, const float *w, const float *b);
+int cac_mlp_get_weights(cac_ctx *ctx, int layer, float *w, float *b);
+int cac_mlp_forward(cac_ctx *ctx, int n, const float *x, float *y);
+/* loss and gradients of one batch, no update: grads[l] = dW_l [out][in]
+ * then db_l [out]; NULL entries are skipped */
+int cac_mlp_grads(cac_ctx *ctx, int n, const float *x, const float *t, float *loss, float **grads);
+int cac_mlp_train_step(cac_ctx *ctx, int n, const float *x, const float *t, float lr, float *loss);
+int cac_mlp_set_dataset(cac_ctx *ctx, int n, const float *x, const float *t);
+int cac_mlp_train(cac_ctx *ctx, int epochs, int batch, float lr, uint64_t seed, float *loss);
+
+/* Surrogate on the force path (no counterpart in the reference).  X0:
+ * [ne][8][3] reference node positions; nbr: [ne][26] mesh neighbors of each
+ * element (-1 where the neighbor block is not an element); in_scale [270],
+ * out_scale [24] normalize the features and nodal forces.  Features: 24
+ * element-frame nodal displacements, 78 neighbor-centroid displacements, 168
+ * displacements of the neighbor nodes meeting each corner.  Prediction:
+ * A x + b (cac_surrogate_set_linear, [24][270] FP32) plus the MLP (270 -> 24).
+ * While enabled, cac_force (EAM, eflag = vflag = 0) takes the nodal forces of
+ * every element from the surrogate except fallback elements (interface, or
+ * Green-Lagrange strain above strain_limit), which keep the plain
+ * integration-point path; classification is refreshed at each rebuild.
+ * cac_surrogate_features writes the raw features [ne][270] and the class
+ * per element (0 surrogate, 1 interface, 2 strained). */
+int cac_surrogate_setup(cac_ctx *ctx, const double *X0, const int *nbr, const float *in_scale,
+                        const float *out_scale, double strain_limit);
+int cac_surrogate_set_linear(cac_ctx *ctx, const float *A, const float *b);
+int cac_surrogate_enable(cac_ctx *ctx, int on);
+int cac_surrogate_features(cac_ctx *ctx, float *feats, int *cls);
+int cac_surrogate_info(cac_ctx *ctx, int *ninterface, int *nstrained, int *nsurrogate);
+
 #ifdef __cplusplus
 }
 #endif
