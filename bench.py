@@ -42,6 +42,9 @@ ALG_BYTES = {
 }
 B_ATOM = 616
 B_ELEM = 7640
+# HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by the
+# copy-kernel calibration), for the kernels profiled on this workload
+TRAFFIC = {}
 
 
 def log(msg):
@@ -98,14 +101,16 @@ def gpu_run(s, pair, cutneigh, skin, steps, warmup, precision, every, delay):
     return eng, t1 - t0, nb, nb_warm, prof
 
 
-def cpu_baseline(s, pair, ff, cutneigh, seconds, rebuild_rate):
-    """The CPU FP64 oracle on this host: one neighbor build plus as many full
-    steps (force on atoms and elements, both NVE halves) as fit in `seconds`;
-    the build is charged at the GPU run's rebuild rate."""
+def cpu_baseline(s, pair, ff, cutneigh, threads, warm, timed, rebuild_rate):
+    """The CPU FP64 oracle on this host with `threads` OpenMP threads: one
+    neighbor build, `warm` untimed and `timed` timed full steps (force on
+    atoms and elements, both NVE halves); the build is charged at the GPU
+    run's rebuild rate."""
     import numpy as np
     from oracle import oracle as O
     from cac_code_amd.lattice import FTM2V_METAL
     O.build()
+    O.set_num_threads(threads)
     x = s.x.copy(); v = s.v.copy(); nodex = s.nodex.copy(); nodev = s.nodev.copy()
     es = O.ElementSet(nodex, s.nae, s.intp_ijk, s.intp_w)
     t0 = time.perf_counter()
@@ -124,35 +129,158 @@ def cpu_baseline(s, pair, ff, cutneigh, seconds, rebuild_rate):
     m, me = s.mass, s.mass_e
     f = fa(); nodef = fe()
     xc = np.zeros((s.n_elements, 3))
-    nsteps = 0
-    t0 = time.perf_counter()
-    while True:
+    for i in range(warm + timed):
+        if i == warm:
+            t0 = time.perf_counter()
         O.nve_atoms_initial(x, v, f, dt, dtf, m)
         O.nve_nodes_initial(es.nodex, nodev, nodef, xc, dt, dtf, me)
         f = fa(); nodef = fe()
         O.nve_atoms_final(v, f, dtf, m)
         O.nve_nodes_final(nodev, nodef, dtf, me)
-        nsteps += 1
-        if time.perf_counter() - t0 >= seconds:
-            break
-    t_step = (time.perf_counter() - t0) / nsteps
+    t_step = (time.perf_counter() - t0) / timed
     per_step = t_step + rebuild_rate * t_build
-    return dict(per_step=per_step, steps=nsteps, t_build=t_build, t_step=t_step, cores=O.num_threads())
+    return dict(per_step=per_step, steps=timed, warm=warm, t_build=t_build, t_step=t_step,
+                cores=O.num_threads())
+
+
+def host_info():
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count()
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity": affinity,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
+def new_engine(s, pair, cutneigh, skin, every, delay):
+    from cac_code_amd import Engine
+    eng = Engine(0)
+    eng.load_system(s)
+    eng.set_pair(pair)
+    eng.set_neighbor(cutneigh, skin=skin, every=every, delay=delay, check=True)
+    eng.build()
+    eng.force()
+    return eng
+
+
+def _err(test, ref):
+    """Relative RMS and max of a force difference: ||test - ref|| / ||ref||
+    over all components, max_i |test_i - ref_i| / RMS_i |ref_i|."""
+    import numpy as np
+    d = (test - ref).reshape(-1, 3)
+    r = ref.reshape(-1, 3)
+    rms = float(np.sqrt(np.mean(np.sum(r * r, axis=1))))
+    if rms == 0.0:
+        return {"rel_rms": None, "rel_max": None}
+    return {"rel_rms": float(np.sqrt(np.mean(np.sum(d * d, axis=1)))) / rms,
+            "rel_max": float(np.sqrt(np.max(np.sum(d * d, axis=1)))) / rms}
+
+
+def accuracy(s, pair, cutneigh, skin, every, delay, at_steps, model=None):
+    """Along an FP64 trajectory: at each step in at_steps, the mixed (and,
+    with a model, the mixed + surrogate) forces against the FP64 forces on
+    the same configuration."""
+    import numpy as np
+    from cac_code_amd import _lib
+    from cac_code_amd import surrogate as S
+    eng = new_engine(s, pair, cutneigh, skin, every, delay)
+    if model is not None:
+        S.attach(eng, model, s)
+        eng.surrogate_enable(False)
+    out = []
+    done = 0
+    for step in at_steps:
+        if step > done:
+            eng.run(step - done)
+            done = step
+        eng.force(precision=_lib.PREC_FP64)
+        _, _, fa = eng.atoms()
+        _, _, fn = eng.elements()
+        eng.force(eflag=True, precision=_lib.PREC_FP64)
+        e64 = float(eng.energy_virial()[0].sum())
+        eng.force(eflag=True, precision=_lib.PREC_MIXED)
+        emx = float(eng.energy_virial()[0].sum())
+        eng.force(precision=_lib.PREC_MIXED)
+        _, _, fam = eng.atoms()
+        _, _, fnm = eng.elements()
+        rec = {"step": step, "mixed": {"atoms": _err(fam, fa), "nodes": _err(fnm, fn),
+                                       "energy_rel": abs(emx - e64) / abs(e64), "energy_fp64_eV": e64}}
+        if model is not None:
+            eng.surrogate_enable(True)
+            eng.force(precision=_lib.PREC_MIXED)
+            _, _, fas = eng.atoms()
+            _, _, fns = eng.elements()
+            _, cls = eng.surrogate_features()
+            info = eng.surrogate_info()
+            eng.surrogate_enable(False)
+            sm = cls == 0
+            rec["surrogate"] = {"atoms": _err(fas, fa), "nodes": _err(fns, fn),
+                                "surrogate_nodes": _err(fns[sm], fn[sm]) if sm.any() else None,
+                                "fallback_fraction": float(1.0 - sm.mean()), **info}
+        eng.force(precision=_lib.PREC_FP64)      # the trajectory continues on FP64 forces
+        out.append(rec)
+        log(f"[bench] accuracy at step {step}: {json.dumps(rec)}")
+    eng.close()
+    return out
+
+
+def drift(s, pair, cutneigh, skin, every, delay, nsteps, precision, chunk=1000):
+    """Total energy (build-defined PE + KE) every `chunk` steps of an NVE run."""
+    from cac_code_amd.lattice import kinetic_energy
+    eng = new_engine(s, pair, cutneigh, skin, every, delay)
+    n_int = s.n_atoms + 8 * s.n_elements
+
+    def etot():
+        eng.force(eflag=True, precision=precision)
+        pe = float(eng.energy_virial()[0].sum())
+        _, v, _ = eng.atoms()
+        _, nv, _ = eng.elements()
+        return pe + kinetic_energy(s, v, nv)
+
+    e0 = etot()
+    series = [0.0]
+    done = 0
+    t0 = time.perf_counter()
+    while done < nsteps:
+        k = min(chunk, nsteps - done)
+        eng.run(k, precision=precision)
+        done += k
+        series.append((etot() - e0) / n_int)
+    eng.close()
+    return {"steps": nsteps, "e0_eV": e0, "drift_eV_per_particle": series[-1],
+            "max_abs_eV_per_particle": max(abs(x) for x in series),
+            "every": chunk, "series_eV_per_particle": series, "seconds": round(time.perf_counter() - t0, 1)}
 
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--workload", choices=["eam_c1", "lj_c0", "small"], default="eam_c1")
     ap.add_argument("--seed", type=int, default=4928459)
     ap.add_argument("--every", type=int, default=1)
     ap.add_argument("--delay", type=int, default=10)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-warmup", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-variants", action="store_true", help="skip the mixed-precision line item")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the mixed-precision and surrogate lines, accuracy and drift")
+    ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--surrogate-samples", type=int, default=200000)
+    ap.add_argument("--surrogate-epochs", type=int, default=400)
+    ap.add_argument("--error-steps", default="0,100,1000")
+    ap.add_argument("--drift-steps", type=int, default=10000)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -199,33 +327,54 @@ def main():
         ms = prof[dom][0]
         ach = units * per / (ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": TRAFFIC.get(dom),
                 "alg_bytes_per_launch": units * per, "avg_launch_us": round(ms * 1e3, 2)}
     b_step = B_ATOM * s.n_atoms + B_ELEM * s.n_elements
     step_roof = {"alg_bytes_per_step": b_step,
                  "achieved_GBs": round(b_step * args.steps / secs / 1e9, 1),
                  "frac": round(b_step * args.steps / secs / 1e9 / HBM_PEAK_GBS, 4)}
 
-    variants = {}
+    variants, acc = {}, {}
     if not args.no_variants and args.precision == "fp64":
-        eng2, secs2, nb2, _, _ = gpu_run(s, pair, cutneigh, skin, args.steps, args.warmup, _lib.PREC_MIXED,
-                                         args.every, args.delay)
+        eng2, secs2, nb2, _, prof2 = gpu_run(s, pair, cutneigh, skin, args.steps, args.warmup, _lib.PREC_MIXED,
+                                             args.every, args.delay)
         eng2.close()
         variants["mixed"] = {"value": round(n_int * args.steps / secs2 / 1e6, 2),
-                             "ms_per_step": round(secs2 * 1e3 / args.steps, 4), "rebuilds": nb2}
+                             "ms_per_step": round(secs2 * 1e3 / args.steps, 4), "rebuilds": nb2,
+                             "kernels_us": {k: round(v[0] * 1e3, 1) for k, v in prof2.items()}}
         log(f"[bench] mixed: {secs2 * 1e3 / args.steps:.3f} ms/step")
+        model = None
+        if args.workload == "eam_c1" and not args.no_surrogate:
+            model, variants["surrogate"] = surrogate_variant(s, pair, cutneigh, skin, args, n_int, secs)
+        steps = [int(x) for x in args.error_steps.split(",") if x.strip()]
+        acc["forces"] = accuracy(s, pair, cutneigh, skin, args.every, args.delay, steps, model)
+        if args.drift_steps:
+            acc["drift_fp64"] = drift(s, pair, cutneigh, skin, args.every, args.delay, args.drift_steps,
+                                      _lib.PREC_FP64)
+            acc["drift_mixed"] = drift(s, pair, cutneigh, skin, args.every, args.delay, args.drift_steps,
+                                       _lib.PREC_MIXED)
+            log(f"[bench] drift fp64 {acc['drift_fp64']['drift_eV_per_particle']:.3e}, mixed "
+                f"{acc['drift_mixed']['drift_eV_per_particle']:.3e} eV/particle over {args.drift_steps} steps")
 
     cpu = None
     if not args.no_cpu:
         rate = nbuilds / max(args.steps, 1)
-        log(f"[bench] cpu baseline (oracle), {args.cpu_seconds:.0f}s sample ...")
-        c = cpu_baseline(s, pair, ff, cutneigh, args.cpu_seconds, rate)
-        cpu = {"value": round(n_int / c["per_step"] / 1e6, 4), "unit": "Matom-steps/s", "cores": c["cores"],
+        host = host_info()
+        nthr = host["affinity"] or 1
+        if host["omp_num_threads"]:
+            nthr = min(nthr, int(host["omp_num_threads"]))
+        log(f"[bench] cpu baseline (oracle): {args.cpu_warmup}+{args.cpu_steps} steps on 1 and {nthr} threads ...")
+        c1 = cpu_baseline(s, pair, ff, cutneigh, 1, args.cpu_warmup, args.cpu_steps, rate)
+        cn = cpu_baseline(s, pair, ff, cutneigh, nthr, args.cpu_warmup, args.cpu_steps, rate)
+        cpu = {"value": round(n_int / cn["per_step"] / 1e6, 4), "unit": "Matom-steps/s", "cores": cn["cores"],
                "kind": "port",
-               "sample": f"{c['steps']} full FP64 steps of the same system on the oracle "
-                         f"({c['t_step']:.3f} s/step) plus one neighbor build ({c['t_build']:.2f} s) "
-                         f"charged at the GPU rebuild rate {rate:.4f}/step"}
-        log(f"[bench] cpu: {c['per_step']:.3f} s/step on {c['cores']} threads")
+               "sample": f"{args.cpu_warmup} warm-up + {cn['steps']} timed full FP64 steps of the same system on "
+                         f"the oracle ({cn['t_step']:.3f} s/step) plus one neighbor build ({cn['t_build']:.2f} s) "
+                         f"charged at the GPU rebuild rate {rate:.4f}/step",
+               "single_thread": {"value": round(n_int / c1["per_step"] / 1e6, 4), "t_step": round(c1["t_step"], 3),
+                                 "t_build": round(c1["t_build"], 2)},
+               "host": host}
+        log(f"[bench] cpu: {cn['per_step']:.3f} s/step on {cn['cores']} threads, {c1['per_step']:.3f} s/step on 1")
 
     out = {
         "metric": "Matom-steps/s on 1 MI355X (integrated particles: atoms + 8 x elements)",
@@ -244,14 +393,71 @@ def main():
                    "nae": list(s.nae), "integration_points": int(s.intp_w.size), "box_cells": s.cells,
                    "represented_atoms": n_rep, "rebuilds": nbuilds, "neighbor": [cutneigh, skin],
                    "neigh_modify": {"every": args.every, "delay": args.delay, "check": "yes"},
-                   "parallelism": "single GPU"},
+                   "seed": args.seed, "parallelism": "single GPU"},
         "represented_Matom_steps_per_s": round(n_rep * args.steps / secs / 1e6, 1),
         "roofline": roof,
         "step_roofline": step_roof,
         "variants": variants,
+        "accuracy": acc,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def surrogate_variant(s, pair, cutneigh, skin, args, n_int, secs_fp64):
+    """Train the element surrogate (dataset seed 1234, init seed 0, Adam
+    lr 1e-3) and time mixed precision + surrogate on the same workload."""
+    from cac_code_amd import _lib
+    from cac_code_amd import surrogate as S
+    t0 = time.perf_counter()
+    ds = S.generate_dataset(pair, args.surrogate_samples, seed=1234, log=log)
+    held = S.generate_dataset(pair, 20000, seed=77)
+    model = S.train(ds, epochs=args.surrogate_epochs, batch=1024, lr=1e-3, seed=0, log=log)
+    held_err = S.rel_rms(S.predict(model, held.x), held.t)
+    held_fit = S.rel_rms(S.predict(model, held.x, structured_only=True), held.t)
+    t_train = time.perf_counter() - t0
+    log(f"[bench] surrogate trained in {t_train:.0f}s: held-out rel RMS {held_err:.5f} "
+        f"(least-squares part alone {held_fit:.5f})")
+    eng = new_engine(s, pair, cutneigh, skin, args.every, args.delay)
+    info = S.attach(eng, model, s)
+    eng.force(precision=_lib.PREC_MIXED)
+    nb_w = eng.run(args.warmup, precision=_lib.PREC_MIXED) if args.warmup else 0
+    eng.synchronize()
+    eng.profile(True)
+    t1 = time.perf_counter()
+    nb = eng.run(args.steps, precision=_lib.PREC_MIXED)
+    eng.synchronize()
+    secs = time.perf_counter() - t1
+    prof = {}
+    for name in ("atom_rho", "atom_force", "site_pos", "ip_erho", "site_fp", "ip_force", "scatter", "surr_feat",
+                 "surr_mlp", "surr_fit", "atom_neigh", "ip_neigh", "elem_neigh"):
+        ms, cnt = eng.profile_avg_ms(name)
+        if cnt:
+            prof[name] = round(ms * 1e3, 1)
+    eng.profile(False)
+    info = eng.surrogate_info()
+    eng.close()
+    ne = s.n_elements
+    f_fb = 1.0 - info["surrogate"] / ne
+    b_step = B_ATOM * s.n_atoms + ne * ((1.0 - f_fb) * 2568 + f_fb * B_ELEM)
+    mlp_flop = 2.0 * (S.NIN * S.NHID + (S.NLAYERS - 1) * S.NHID * S.NHID + S.NHID * S.NOUT) * info["surrogate"]
+    mlp_roof = None
+    if "surr_mlp" in prof:
+        tf = mlp_flop / (prof["surr_mlp"] * 1e-6) / 1e12
+        mlp_roof = {"bound": "mfma", "kernel": "surr_mlp (3 hidden + output GEMMs, BF16 MFMA)",
+                    "achieved": round(tf, 1), "peak": 2500.0, "unit": "TFLOP/s", "frac": round(tf / 2500.0, 4),
+                    "flop_per_launch": mlp_flop}
+    log(f"[bench] mixed+surrogate: {secs * 1e3 / args.steps:.3f} ms/step, fallback fraction {f_fb:.3f}, "
+        f"{prof}")
+    return model, {"value": round(n_int * args.steps / secs / 1e6, 2),
+                   "ms_per_step": round(secs * 1e3 / args.steps, 4),
+                   "speedup_vs_fp64": round(secs_fp64 / secs, 3), "rebuilds": nb,
+                   "fallback_fraction": round(f_fb, 4), **info,
+                   "alg_bytes_per_step": b_step, "kernels_us": prof, "mlp_roofline": mlp_roof,
+                   "training": {"samples": args.surrogate_samples, "epochs": args.surrogate_epochs,
+                                "dataset_seed": 1234, "init_seed": 0, "lr": 1e-3, "seconds": round(t_train, 1),
+                                "heldout_rel_rms": round(held_err, 6), "heldout_least_squares_only": round(held_fit, 6),
+                                "k_node_eV_A2": round(ds.meta["k_node"], 2), "sigma_A": round(ds.meta["sigma"], 5)}}
 
 
 if __name__ == "__main__":

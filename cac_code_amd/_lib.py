@@ -65,7 +65,7 @@ SIGNATURES = {
     "cac_mlp_set_dataset": (C.c_int, [_ctx, C.c_int, _f, _f]),
     "cac_mlp_train": (C.c_int, [_ctx, C.c_int, C.c_int, C.c_float, C.c_uint64, _f]),
     "cac_surrogate_setup": (C.c_int, [_ctx, _d, _i, _f, _f, C.c_double]),
-    "cac_surrogate_set_linear": (C.c_int, [_ctx, _f, _f]),
+    "cac_surrogate_set_fit": (C.c_int, [_ctx, _f, _f, _f]),
     "cac_surrogate_enable": (C.c_int, [_ctx, C.c_int]),
     "cac_surrogate_features": (C.c_int, [_ctx, _f, _i]),
     "cac_surrogate_info": (C.c_int, [_ctx, _i, _i, _i]),

@@ -73,17 +73,19 @@ struct MlpState;
 struct SurrState {
   bool setup = false, enabled = false, ready = false;
   double strain_limit = 0.02;
-  double *r0 = nullptr;       // [ne][8][3] reference node offsets from the element centroid
-  double *d0 = nullptr;       // [ne][26][3] reference neighbor-centroid offsets
-  double *c0 = nullptr;       // [ne][56][3] reference offsets of the neighbor nodes meeting each corner
-  float *lin_A = nullptr, *lin_b = nullptr;  // [24][270], [24] linear part of the prediction
+  double *ref = nullptr;      // [nref][279] reference geometry classes (r0, d0, c0, G0^-1)
+  int *rcls = nullptr;        // [ne] reference class of each element
+  int nref = 0;
+  std::vector<int> hrcls, hnbr;  // host copies (classification builds the row metadata from them)
+  int *meta = nullptr;        // [nS][28] surrogate rows: element, class, 26 neighbors
+  float *fit_A = nullptr, *fit_Q = nullptr, *fit_b = nullptr;  // [24][279], [24][561], [24] least-squares part
   int *nbr = nullptr;         // [ne][26] mesh neighbors, -1 if absent
-  float *inv_in = nullptr;    // [270] 1 / input scale
+  float *inv_in = nullptr;    // [279] 1 / input scale
   float *out_scale = nullptr; // [24]
   double *ubar = nullptr;     // [ne][3] current element centroids (per step)
   int nF = 0, nR = 0, nR2 = 0, nS = 0, nsR = 0, nsR2 = 0, ninterface = 0, nstrained = 0;
   int *elemF = nullptr, *elemR = nullptr, *elemR2 = nullptr, *elemS = nullptr, *siteR = nullptr, *siteR2 = nullptr;
-  DevBuf b_X0, b_d0, b_c0, b_A, b_b, b_xf, b_nbr, b_inv_in, b_out, b_cls, b_mark, b_F, b_S, b_R, b_R2, b_sR, b_sR2, b_tmp, b_ubar, b_feat;
+  DevBuf b_meta, b_ref, b_rcls, b_A, b_Q, b_b, b_xf, b_nbr, b_inv_in, b_out, b_cls, b_mark, b_F, b_S, b_R, b_R2, b_sR, b_sR2, b_tmp, b_ubar, b_feat;
 };
 
 struct LjParams {

@@ -138,6 +138,121 @@ __global__ __launch_bounds__(kBlock) void k_gemm_tn(int M, int N, int K, const u
   }
 }
 
+// Fused inference for the surrogate shape (3 hidden layers of 256, output
+// padded to 32, input padded to at most kFusedIn): one block of 256 threads
+// carries 64 rows through every layer with the activations in LDS, so only
+// the BF16 input rows, the weights (L2-resident) and the FP32 output touch
+// memory.  Wave w owns output columns [64w, 64w + 64) of each hidden layer
+// (2 x 2 MFMA tiles); waves 0 and 1 own the two 32-row tiles of the output
+// layer.  Row strides are padded by 8 BF16 (16 B) against LDS bank
+// conflicts of the 16-byte fragment reads.
+constexpr int kFusedRows = 64, kFusedH = 256, kFusedIn = 288, kFusedOut = 32;
+constexpr int kLdA = kFusedIn + 8, kLdB = kFusedH + 8;
+
+struct FusedW {
+  const uint16_t *w[4];
+  const float *b[4];
+};
+
+__device__ __forceinline__ void fused_layer(const uint16_t *src, int lds, int K, const uint16_t *__restrict__ W,
+                                            const float *__restrict__ bias, uint16_t *dst, int ldd, int wave,
+                                            int lane) {
+  const int kofs = 8 * (lane >> 5), r = lane & 31;
+  const uint16_t *a0p = src + r * lds + kofs, *a1p = src + (32 + r) * lds + kofs;
+  const int c0 = 64 * wave;
+  const uint16_t *b0p = W + (size_t)(c0 + r) * K + kofs, *b1p = W + (size_t)(c0 + 32 + r) * K + kofs;
+  f32x16 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0f;
+  // weight fragments are prefetched two k-steps ahead (K is a multiple of 32)
+  bf16x8 b0 = *reinterpret_cast<const bf16x8 *>(b0p), b1 = *reinterpret_cast<const bf16x8 *>(b1p);
+  bf16x8 g0 = *reinterpret_cast<const bf16x8 *>(b0p + 16), g1 = *reinterpret_cast<const bf16x8 *>(b1p + 16);
+  for (int k0 = 0; k0 < K; k0 += 32) {
+    bf16x8 n0 = b0, n1 = b1, m0 = g0, m1 = g1;
+    if (k0 + 32 < K) {
+      n0 = *reinterpret_cast<const bf16x8 *>(b0p + k0 + 32);
+      n1 = *reinterpret_cast<const bf16x8 *>(b1p + k0 + 32);
+      m0 = *reinterpret_cast<const bf16x8 *>(b0p + k0 + 48);
+      m1 = *reinterpret_cast<const bf16x8 *>(b1p + k0 + 48);
+    }
+    bf16x8 a0 = *reinterpret_cast<const bf16x8 *>(a0p + k0);
+    bf16x8 a1 = *reinterpret_cast<const bf16x8 *>(a1p + k0);
+    acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b0, acc[0][0], 0, 0, 0);
+    acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b1, acc[0][1], 0, 0, 0);
+    acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b0, acc[1][0], 0, 0, 0);
+    acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b1, acc[1][1], 0, 0, 0);
+    a0 = *reinterpret_cast<const bf16x8 *>(a0p + k0 + 16);
+    a1 = *reinterpret_cast<const bf16x8 *>(a1p + k0 + 16);
+    acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, g0, acc[0][0], 0, 0, 0);
+    acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, g1, acc[0][1], 0, 0, 0);
+    acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, g0, acc[1][0], 0, 0, 0);
+    acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, g1, acc[1][1], 0, 0, 0);
+    b0 = n0; b1 = n1; g0 = m0; g1 = m1;
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int col = c0 + 32 * j + r;
+      const float bc = bias[col];
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int row = 32 * i + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+        const float z = acc[i][j][v] + bc;
+        dst[row * ldd + col] = f2bf(z * sigm(z));
+      }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_mlp_infer_fused(int n, int inp, const uint16_t *__restrict__ x,
+                                                             FusedW fw, float *__restrict__ y) {
+  __shared__ uint16_t bufA[kFusedRows * kLdA];
+  __shared__ uint16_t bufB[kFusedRows * kLdB];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int row0 = blockIdx.x * kFusedRows;
+  // stage the input rows (16-byte chunks; rows past n are zero)
+  const int chunks = inp / 8;
+  for (int i = threadIdx.x; i < kFusedRows * chunks; i += kBlock) {
+    const int rr = i / chunks, c = i % chunks;
+    bf16x8 v;
+    if (row0 + rr < n) {
+      v = *reinterpret_cast<const bf16x8 *>(x + (size_t)(row0 + rr) * inp + 8 * c);
+    } else {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) v[j] = (__bf16)0.0f;
+    }
+    *reinterpret_cast<bf16x8 *>(bufA + rr * kLdA + 8 * c) = v;
+  }
+  __syncthreads();
+  fused_layer(bufA, kLdA, inp, fw.w[0], fw.b[0], bufB, kLdB, wave, lane);
+  __syncthreads();
+  fused_layer(bufB, kLdB, kFusedH, fw.w[1], fw.b[1], bufA, kLdA, wave, lane);
+  __syncthreads();
+  fused_layer(bufA, kLdA, kFusedH, fw.w[2], fw.b[2], bufB, kLdB, wave, lane);
+  __syncthreads();
+  if (wave < 2) {
+    const int kofs = 8 * (lane >> 5), r = lane & 31;
+    const uint16_t *ap = bufB + (32 * wave + r) * kLdB + kofs;
+    const uint16_t *bp = fw.w[3] + (size_t)r * kFusedH + kofs;
+    f32x16 acc;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[v] = 0.0f;
+    for (int k0 = 0; k0 < kFusedH; k0 += 16)
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(*reinterpret_cast<const bf16x8 *>(ap + k0),
+                                                    *reinterpret_cast<const bf16x8 *>(bp + k0), acc, 0, 0, 0);
+    const float bc = fw.b[3][r];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const int row = row0 + 32 * wave + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+      if (row < n) y[(size_t)row * kFusedOut + r] = acc[v] + bc;
+    }
+  }
+}
+
 // Mean-squared-error gradient of the output layer: dY = 2 (Y - T) / (n*out)
 // over the true rows and columns, zero in the padding; per-block partial
 // loss sums in fixed order.
@@ -396,11 +511,26 @@ void mlp_destroy(cac_ctx *ctx) {
   ctx->mlp = nullptr;
 }
 
-// Device-side inference on rows already staged in ctx->mlp->x (used by the
-// surrogate force path): returns the output buffer [cap][outp].
-const float *mlp_forward_staged(cac_ctx *ctx) {
-  forward(ctx, false);
-  return ctx->mlp->y;
+bool fused_shape(const MlpState *s) {
+  return s->nlayer == 3 && s->nhid == kFusedH && s->L[0].inp <= kFusedIn && s->L.back().outp == kFusedOut;
+}
+
+// Device-side inference on the first n rows staged in ctx->mlp->x (the
+// surrogate force path): returns the output buffer [n][outp].
+const float *mlp_forward_staged(cac_ctx *ctx, int n) {
+  MlpState *s = ctx->mlp;
+  if (fused_shape(s)) {
+    FusedW fw;
+    for (int l = 0; l < 4; ++l) {
+      fw.w[l] = s->L[l].wb;
+      fw.b[l] = s->L[l].b;
+    }
+    hipLaunchKernelGGL(k_mlp_infer_fused, dim3(grid_for(n, kFusedRows)), dim3(kBlock), 0, ctx->stream, n,
+                       s->L[0].inp, s->x, fw, s->y);
+  } else {
+    forward(ctx, false);
+  }
+  return s->y;
 }
 
 uint16_t *mlp_input_buffer(cac_ctx *ctx, int n, int *ld) {

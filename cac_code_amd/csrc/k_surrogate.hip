@@ -1,20 +1,23 @@
 // k_surrogate.hip -- learned element surrogate on the force path.
 //
-// Features of element e (270):
-//   [0, 24)    the nodal displacements in the element's frame,
-//              (X_k - Xbar) - (X0_k - X0bar) (own centroid displacement removed);
-//   [24, 102)  the centroid displacements of the 26 mesh neighbors relative to
-//              its own, mi(Xbar_j - Xbar) - mi0(X0bar_j - X0bar);
-//   [102, 270) for each corner k, the displacements (same relative form) of
-//              the 7 neighbor nodes that meet at that corner: the integration
-//              point at corner k sees mostly the sites next to those nodes, so
-//              without them the nodal force under independent nodal motion is
-//              not a function of the inputs.
-// mi / mi0 are the minimum image in the current / reference box, so a
-// homogeneous strain of the periodic box gives the same features on every
-// element.  Each feature is divided by in_scale[f].  The prediction of the
-// 24 nodal forces (divided by out_scale[c]) is A x + b (FP32, fitted by
-// least squares) plus the MLP's output.
+// Frame of element e: its centroid Xbar and the affine map F_e that best fits
+// its nodes, F_e = (sum_k (X_k - Xbar) r0_k^T) G0^-1 with r0_k = X0_k - X0bar
+// the reference offsets and G0 = sum_k r0_k r0_k^T.  Features (279, groups of
+// three), all deviations from that affine map, so a homogeneous deformation
+// of the periodic box gives zero deviations on every element:
+//   [0, 24)     own nodes:            (X_k - Xbar) - F_e r0_k
+//   [24, 102)   26 neighbor centroids: mi(Xbar_j - Xbar) - F_e d0_m
+//   [102, 270)  for each corner k, the 7 neighbor nodes that meet there:
+//               mi(X_{j,n} - Xbar) - F_e c0_q
+//   [270, 279)  strain F_e - I, row-major
+// with mi the minimum image in the current box and d0, c0 the reference
+// offsets (minimum image in the reference box).  Each feature is divided by
+// in_scale[f] (x below).  Prediction of nodal force c = 3k + d, divided by
+// out_scale[c]:
+//   b_c + A_c . x + sum_{i<=j} Q_c[ij] z_i z_j + MLP(x)_c
+// where z (33) is corner k's local input (own node k, its 7 corner
+// neighbors, the strain), A, Q, b are FP32 least-squares coefficients and
+// the MLP (BF16 MFMA, k_mlp.hip) is trained on what they leave.
 //
 // Fallback to the plain integration-point path (classified at every
 // neighbor rebuild): interface elements (a mesh neighbor missing, i.e. the
@@ -24,9 +27,10 @@
 //   F  = fallback elements                (force passes, scatter)
 //   R  = F + candidate neighbors of F     (node densities, site F')
 //   R2 = R + candidate neighbors of R     (site positions)
-// and the MLP covers S = all elements not in F.
+// and the surrogate covers S = all elements not in F.
 #include <algorithm>
 #include <cmath>
+#include <map>
 #include <vector>
 
 #include <hipcub/hipcub.hpp>
@@ -35,7 +39,7 @@
 
 namespace cac {
 
-const float *mlp_forward_staged(cac_ctx *ctx);
+const float *mlp_forward_staged(cac_ctx *ctx, int n);
 uint16_t *mlp_input_buffer(cac_ctx *ctx, int n, int *ld);
 int mlp_out_ld(cac_ctx *ctx);
 bool mlp_shape_ok(cac_ctx *ctx, int nin, int nout);
@@ -43,7 +47,10 @@ bool mlp_shape_ok(cac_ctx *ctx, int nin, int nout);
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kNFeat = 270, kNOut = 24;
+constexpr int kNFeat = CAC_SURR_NFEAT, kNOut = CAC_SURR_NOUT, kNLoc = 33, kNQuad = CAC_SURR_NQUAD;
+constexpr int kGroups = kNFeat / 3;
+static_assert(kNQuad == kNLoc * (kNLoc + 1) / 2, "quadratic term count");
+constexpr int kPredElems = 64;  // elements per prediction block (one per lane)
 
 // (neighbor slot, neighbor node) of the 7 nodes meeting corner k, k-major
 struct CornerTab {
@@ -66,65 +73,109 @@ __device__ __forceinline__ uint16_t to_bf16(float f) {
   return (uint16_t)(u >> 16);
 }
 
+// Reference geometry of an element, deduplicated into classes (all interior
+// elements of a structured mesh share one): row of kRefLen doubles
+//   [0, 24)    r0   node offsets X0_k - X0bar
+//   [24, 102)  d0   neighbor-centroid offsets (minimum image, reference box)
+//   [102, 270) c0   corner-neighbor node offsets (same)
+//   [270, 279) G0^-1, G0 = sum_k r0_k r0_k^T
+constexpr int kRefLen = 279, kRefD0 = 24, kRefC0 = 102, kRefG = 270;
+
 // current centroid of every element (its nodes are contiguous: elepbc moves
-// them together)
+// them together); one thread per element and direction
 __global__ void k_surr_centroid(int ne, const double *__restrict__ nodex, double *__restrict__ xbar) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= ne) return;
-  for (int d = 0; d < 3; ++d) {
-    double acc = 0.0;
-    for (int k = 0; k < 8; ++k) acc += nodex[24 * (size_t)e + 3 * k + d];
-    xbar[3 * (size_t)e + d] = acc / 8;
-  }
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= 3 * ne) return;
+  const int e = q / 3, d = q % 3;
+  double acc = 0.0;
+  for (int k = 0; k < 8; ++k) acc += nodex[24 * (size_t)e + 3 * k + d];
+  xbar[q] = acc / 8;
 }
 
-// feature group g of element e (groups of 3 consecutive features, 90 per
-// element): g < 8 own node g, g < 34 neighbor centroid g - 8, else corner
-// node q = g - 34.  r0 [ne][24] are the reference node offsets X0_k - X0bar,
-// d0 [ne][78] the reference neighbor-centroid offsets, c0 [ne][168] the
-// reference corner-node offsets; emit(c, v) receives feature c = 3g + d.
-constexpr int kGroups = kNFeat / 3;
+struct FeatArgs {
+  const double *nodex, *xbar, *ref;
+  const int *nbr, *rcls;
+  CornerTab ct;
+  Prd3 prd;
+};
 
-template <class Emit>
-__device__ __forceinline__ void feature_group(int e, int g, const double *__restrict__ nodex,
-                                              const double *__restrict__ r0, const double *__restrict__ d0,
-                                              const double *__restrict__ c0, const double *__restrict__ xbar,
-                                              const int *__restrict__ nbr, const CornerTab &ct, const Prd3 &prd,
-                                              Emit emit) {
-  const double *xb = xbar + 3 * (size_t)e;
-  if (g < 8) {
-    for (int d = 0; d < 3; ++d)
-      emit(3 * g + d, (float)((nodex[24 * (size_t)e + 3 * g + d] - xb[d]) - r0[24 * (size_t)e + 3 * g + d]));
-  } else if (g < 34) {
-    const int m = g - 8, j = nbr[26 * (size_t)e + m];
-    for (int d = 0; d < 3; ++d)
-      emit(3 * g + d, j >= 0 ? (float)(min_image(xbar[3 * (size_t)j + d] - xb[d], prd.v[d]) -
-                                       d0[78 * (size_t)e + 3 * m + d])
-                             : 0.0f);
-  } else {
-    const int q = g - 34, j = nbr[26 * (size_t)e + ct.m[q]];
-    for (int d = 0; d < 3; ++d)
-      emit(3 * g + d, j >= 0 ? (float)(min_image(nodex[24 * (size_t)j + 3 * ct.node[q] + d] - xb[d], prd.v[d]) -
-                                       c0[168 * (size_t)e + 3 * q + d])
-                             : 0.0f);
+// Per surrogate row, built at classification: element, reference class and
+// the 26 mesh neighbors, so the feature kernel's data loads depend on one
+// index load only.
+constexpr int kMeta = 28;
+
+// affine map F (row-major) of element e from its current nodes:
+// F = (sum_k (X_k - Xbar) r0_k^T) G0^-1; every lane of the wavefront
+// computes it from the same (broadcast) loads
+__device__ __forceinline__ void element_frame(int e, const FeatArgs &a, const double *ref, double F[9]) {
+  const double *X = a.nodex + 24 * (size_t)e;
+  const double *xb = a.xbar + 3 * (size_t)e;
+  double M[9] = {};
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const double u0 = X[3 * k] - xb[0], u1 = X[3 * k + 1] - xb[1], u2 = X[3 * k + 2] - xb[2];
+    const double s0 = ref[3 * k], s1 = ref[3 * k + 1], s2 = ref[3 * k + 2];
+    M[0] += u0 * s0; M[1] += u0 * s1; M[2] += u0 * s2;
+    M[3] += u1 * s0; M[4] += u1 * s1; M[5] += u1 * s2;
+    M[6] += u2 * s0; M[7] += u2 * s1; M[8] += u2 * s2;
   }
+  const double *G = ref + kRefG;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) F[3 * i + j] = M[3 * i] * G[j] + M[3 * i + 1] * G[3 + j] + M[3 * i + 2] * G[6 + j];
+}
+
+// feature group g of element e: g < 8 own node g, g < 34 neighbor centroid
+// g - 8, g < 90 corner node q = g - 34, else strain row g - 90; emit(c, v)
+// receives feature c = 3g + d
+template <class Emit>
+__device__ __forceinline__ void feature_group(int e, int g, const FeatArgs &a, const int *nb, const double *ref,
+                                              const double F[9], Emit emit) {
+  const double *xb = a.xbar + 3 * (size_t)e;
+  double v[3];
+  const double *r = nullptr;
+  bool have = true;
+  if (g < 8) {
+    for (int d = 0; d < 3; ++d) v[d] = a.nodex[24 * (size_t)e + 3 * g + d] - xb[d];
+    r = ref + 3 * g;
+  } else if (g < 34) {
+    const int m = g - 8, j = nb[m];
+    have = j >= 0;
+    if (have)
+      for (int d = 0; d < 3; ++d) v[d] = min_image(a.xbar[3 * (size_t)j + d] - xb[d], a.prd.v[d]);
+    r = ref + kRefD0 + 3 * m;
+  } else if (g < 90) {
+    const int q = g - 34, j = nb[a.ct.m[q]];
+    have = j >= 0;
+    if (have)
+      for (int d = 0; d < 3; ++d)
+        v[d] = min_image(a.nodex[24 * (size_t)j + 3 * a.ct.node[q] + d] - xb[d], a.prd.v[d]);
+    r = ref + kRefC0 + 3 * q;
+  } else {
+    const int i = g - 90;
+    for (int d = 0; d < 3; ++d) emit(3 * g + d, (float)(F[3 * i + d] - (i == d ? 1.0 : 0.0)));
+    return;
+  }
+  for (int d = 0; d < 3; ++d)
+    emit(3 * g + d, have ? (float)(v[d] - (F[3 * d] * r[0] + F[3 * d + 1] * r[1] + F[3 * d + 2] * r[2])) : 0.0f);
 }
 
 // normalized features of the surrogate elements: BF16 into the MLP input,
-// FP32 into xf for the linear part; one wavefront per element
-__global__ void k_surr_features(int ns, const int *__restrict__ elemS, const double *__restrict__ nodex,
-                                const double *__restrict__ r0, const double *__restrict__ d0,
-                                const double *__restrict__ c0, const double *__restrict__ xbar,
-                                const int *__restrict__ nbr, CornerTab ct, Prd3 prd,
-                                const float *__restrict__ inv_in, uint16_t *__restrict__ x, int ldx,
-                                float *__restrict__ xf) {
+// FP32 into xf for the least-squares part; one wavefront per element
+__global__ void k_surr_features(int ns, const int *__restrict__ meta, FeatArgs a, const float *__restrict__ inv_in,
+                                uint16_t *__restrict__ x, int ldx, float *__restrict__ xf) {
   const int r = blockIdx.x * (kBlock / 64) + (int)(threadIdx.x >> 6);
   if (r >= ns) return;
-  const int e = elemS[r];
+  const int *mr = meta + kMeta * (size_t)r;
+  const int e = mr[0];
+  const double *ref = a.ref + kRefLen * (size_t)mr[1];
+  double F[9];
+  element_frame(e, a, ref, F);
   uint16_t *row = x + (size_t)r * ldx;
   float *rowf = xf + (size_t)r * kNFeat;
   for (int g = threadIdx.x & 63; g < kGroups; g += 64)
-    feature_group(e, g, nodex, r0, d0, c0, xbar, nbr, ct, prd, [&](int c, float v) {
+    feature_group(e, g, a, mr + 2, ref, F, [&](int c, float v) {
       v *= inv_in[c];
       row[c] = to_bf16(v);
       rowf[c] = v;
@@ -132,22 +183,111 @@ __global__ void k_surr_features(int ns, const int *__restrict__ elemS, const dou
 }
 
 // raw FP32 features of every element (dataset generation, tests)
-__global__ void k_surr_features_all(int ne, const double *__restrict__ nodex, const double *__restrict__ r0,
-                                    const double *__restrict__ d0, const double *__restrict__ c0,
-                                    const double *__restrict__ xbar, const int *__restrict__ nbr, CornerTab ct,
-                                    Prd3 prd, float *__restrict__ out) {
+__global__ void k_surr_features_all(int ne, FeatArgs a, float *__restrict__ out) {
   const int e = blockIdx.x * (kBlock / 64) + (int)(threadIdx.x >> 6);
   if (e >= ne) return;
   float *row = out + kNFeat * (size_t)e;
+  const double *ref = a.ref + kRefLen * (size_t)a.rcls[e];
+  double F[9];
+  element_frame(e, a, ref, F);
   for (int g = threadIdx.x & 63; g < kGroups; g += 64)
-    feature_group(e, g, nodex, r0, d0, c0, xbar, nbr, ct, prd, [&](int c, float v) { row[c] = v; });
+    feature_group(e, g, a, a.nbr + 26 * (size_t)e, ref, F, [&](int c, float v) { row[c] = v; });
+}
+
+// local input i of corner k: own node k, its 7 corner neighbors, the strain
+__device__ __forceinline__ int local_feature(int k, int i) {
+  return i < 3 ? 3 * k + i : (i < 24 ? 102 + 21 * k + (i - 3) : 270 + (i - 24));
+}
+
+// nodal forces of the surrogate elements.  Block: 64 elements (one per
+// lane) x 8 nodes (one per wavefront).  The elements' features and all
+// least-squares coefficients are staged in LDS; the node index is
+// wave-uniform, so the coefficient reads are same-address (broadcast) LDS
+// reads, 4 floats at a time from rows padded to 16 bytes.
+constexpr int kLdAq = 280, kLdQq = 564;  // padded coefficient rows (floats)
+
+__global__ void __launch_bounds__(512) k_surr_predict(int ns, const int *__restrict__ elemS,
+                                                      const float *__restrict__ xf, const float *__restrict__ y,
+                                                      int ldy, const float *__restrict__ A,
+                                                      const float *__restrict__ Q, const float *__restrict__ b,
+                                                      const float *__restrict__ out_scale,
+                                                      double *__restrict__ nodef) {
+  __shared__ __align__(16) float xs[kPredElems * kNFeat];
+  __shared__ __align__(16) float as[kNOut * kLdAq];
+  __shared__ __align__(16) float qs[kNOut * kLdQq];
+  const int r0 = blockIdx.x * kPredElems;
+  const int nrow = min(kPredElems, ns - r0);
+  // stage with independent 16-byte loads, all issued before the LDS stores
+  // (64 rows of 279 floats start on a 16-byte boundary; the last block's
+  // tail is copied by scalars); A and Q arrive padded to kLdAq / kLdQq
+  {
+    constexpr int kX4 = kPredElems * kNFeat / 4, kA4 = kNOut * kLdAq / 4, kQ4 = kNOut * kLdQq / 4;
+    constexpr int kPer = (kX4 + kA4 + kQ4 + 511) / 512;
+    const int nx4 = nrow * kNFeat / 4;
+    const float4 *sx = reinterpret_cast<const float4 *>(xf + (size_t)r0 * kNFeat);
+    const float4 *sa = reinterpret_cast<const float4 *>(A), *sq = reinterpret_cast<const float4 *>(Q);
+    float4 v[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int i = threadIdx.x + 512 * u;
+      if (i < nx4) v[u] = sx[i];
+      else if (i >= kX4 && i < kX4 + kA4) v[u] = sa[i - kX4];
+      else if (i >= kX4 + kA4 && i < kX4 + kA4 + kQ4) v[u] = sq[i - kX4 - kA4];
+    }
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int i = threadIdx.x + 512 * u;
+      if (i < nx4) reinterpret_cast<float4 *>(xs)[i] = v[u];
+      else if (i >= kX4 && i < kX4 + kA4) reinterpret_cast<float4 *>(as)[i - kX4] = v[u];
+      else if (i >= kX4 + kA4 && i < kX4 + kA4 + kQ4) reinterpret_cast<float4 *>(qs)[i - kX4 - kA4] = v[u];
+    }
+    for (int i = 4 * nx4 + (int)threadIdx.x; i < nrow * kNFeat; i += 512) xs[i] = xf[(size_t)r0 * kNFeat + i];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int k = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  if (lane >= nrow) return;
+  const float *x = xs + lane * kNFeat;
+  const float4 *A0 = reinterpret_cast<const float4 *>(as + 3 * k * kLdAq);
+  const float4 *A1 = A0 + kLdAq / 4, *A2 = A1 + kLdAq / 4;
+  float a0 = b[3 * k], a1 = b[3 * k + 1], a2 = b[3 * k + 2];
+  for (int f4 = 0; f4 < kLdAq / 4; ++f4) {
+    const float4 c0 = A0[f4], c1 = A1[f4], c2 = A2[f4];
+    const int f = 4 * f4;
+    const float x0 = x[f], x1 = x[f + 1], x2 = x[f + 2], x3 = f + 3 < kNFeat ? x[f + 3] : 0.0f;
+    a0 = fmaf(c0.x, x0, a0); a0 = fmaf(c0.y, x1, a0); a0 = fmaf(c0.z, x2, a0); a0 = fmaf(c0.w, x3, a0);
+    a1 = fmaf(c1.x, x0, a1); a1 = fmaf(c1.y, x1, a1); a1 = fmaf(c1.z, x2, a1); a1 = fmaf(c1.w, x3, a1);
+    a2 = fmaf(c2.x, x0, a2); a2 = fmaf(c2.y, x1, a2); a2 = fmaf(c2.z, x2, a2); a2 = fmaf(c2.w, x3, a2);
+  }
+  float z[kNLoc];
+#pragma unroll
+  for (int i = 0; i < kNLoc; ++i) z[i] = x[local_feature(k, i)];
+  const float *Q0 = qs + 3 * k * kLdQq, *Q1 = Q0 + kLdQq, *Q2 = Q1 + kLdQq;
+  int t = 0;
+#pragma unroll
+  for (int i = 0; i < kNLoc; ++i)
+#pragma unroll
+    for (int j = i; j < kNLoc; ++j, ++t) {
+      const float p = z[i] * z[j];
+      a0 = fmaf(Q0[t], p, a0);
+      a1 = fmaf(Q1[t], p, a1);
+      a2 = fmaf(Q2[t], p, a2);
+    }
+  const int r = r0 + lane;
+  const float *yr = y + (size_t)r * ldy + 3 * k;
+  double *out = nodef + kNOut * (size_t)elemS[r] + 3 * k;
+  out[0] = ((double)a0 + (double)yr[0]) * (double)out_scale[3 * k];
+  out[1] = ((double)a1 + (double)yr[1]) * (double)out_scale[3 * k + 1];
+  out[2] = ((double)a2 + (double)yr[2]) * (double)out_scale[3 * k + 2];
 }
 
 // fallback class: 1 interface, 2 strained, 0 surrogate
-__global__ void k_surr_classify(int ne, const double *__restrict__ nodex, const double *__restrict__ X0,
-                                const int *__restrict__ nbr, double strain_limit, int *__restrict__ cls) {
+__global__ void k_surr_classify(int ne, const double *__restrict__ nodex, const double *__restrict__ ref,
+                                const int *__restrict__ rcls, const int *__restrict__ nbr, double strain_limit,
+                                int *__restrict__ cls) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= ne) return;
+  const double *r0 = ref + kRefLen * (size_t)rcls[e];
   int c = 0;
   for (int m = 0; m < 26; ++m)
     if (nbr[26 * (size_t)e + m] < 0) c = 1;
@@ -162,9 +302,9 @@ __global__ void k_surr_classify(int ne, const double *__restrict__ nodex, const 
       for (int d = 0; d < 3; ++d) {
         double s = 0.0, s0 = 0.0;
         for (int q = 0; q < 4; ++q) {
-          const size_t i0 = 24 * (size_t)e + 3 * pr[a][q][0] + d, i1 = 24 * (size_t)e + 3 * pr[a][q][1] + d;
-          s += nodex[i1] - nodex[i0];
-          s0 += X0[i1] - X0[i0];
+          const int k0 = 3 * pr[a][q][0] + d, k1 = 3 * pr[a][q][1] + d;
+          s += nodex[24 * (size_t)e + k1] - nodex[24 * (size_t)e + k0];
+          s0 += r0[k1] - r0[k0];
         }
         E[d][a] = s / 4;
         E0[d][a] = s0 / 4;
@@ -211,22 +351,6 @@ __global__ void k_mark_neighbors(int n, const int *__restrict__ src, int ne, con
   for (int l = 0; l < enumb[i]; ++l) mark[eneigh[(size_t)l * ne + i] & 0x7FFFFFFu] = 1;
 }
 
-// nodal forces of the surrogate elements: (A x + b + MLP(x)) * out_scale,
-// the linear term in FP32 (one thread per element and output)
-__global__ void k_surr_scatter(int ns, const int *__restrict__ elemS, const float *__restrict__ y, int ldy,
-                               const float *__restrict__ xf, const float *__restrict__ A,
-                               const float *__restrict__ b, const float *__restrict__ out_scale,
-                               double *__restrict__ nodef) {
-  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= (long)ns * kNOut) return;
-  const int r = (int)(q / kNOut), c = (int)(q % kNOut);
-  const float *x = xf + (size_t)r * kNFeat;
-  const float *a = A + (size_t)c * kNFeat;
-  float acc = b[c];
-  for (int f = 0; f < kNFeat; ++f) acc = fmaf(a[f], x[f], acc);
-  nodef[kNOut * (size_t)elemS[r] + c] = ((double)acc + (double)y[(size_t)r * ldy + c]) * (double)out_scale[c];
-}
-
 CornerTab corner_table() {
   // node corner flags (element.cpp:1375-1382 order) and the 26 neighbor
   // offsets (i outermost); neighbor d meets corner k iff each d_i is 0 or
@@ -269,6 +393,17 @@ Prd3 prd_of(const cac_ctx *ctx) {
   return p;
 }
 
+// current centroids of all elements, then the feature arguments
+int frames(cac_ctx *ctx, FeatArgs *a) {
+  SurrState &S = ctx->surr;
+  const int ne = ctx->ne;
+  int rc;
+  if ((rc = ensure(ctx, S.b_ubar, 3 * (size_t)ne, &S.ubar))) return rc;
+  hipLaunchKernelGGL(k_surr_centroid, dim3(grid_for(3L * ne)), dim3(kBlock), 0, ctx->stream, ne, ctx->nodex, S.ubar);
+  *a = FeatArgs{ctx->nodex, S.ubar, S.ref, S.nbr, S.rcls, corner_table(), prd_of(ctx)};
+  return CAC_OK;
+}
+
 int upload_list(cac_ctx *ctx, DevBuf &b, const std::vector<int> &v, int **out) {
   int rc = ensure(ctx, b, std::max<size_t>(v.size(), 1), out);
   if (rc) return rc;
@@ -287,8 +422,8 @@ int surrogate_classify(cac_ctx *ctx) {
   int rc;
   int *cls, *mark;
   if ((rc = ensure(ctx, S.b_cls, (size_t)ne, &cls)) || (rc = ensure(ctx, S.b_mark, (size_t)ne, &mark))) return rc;
-  hipLaunchKernelGGL(k_surr_classify, dim3(grid_for(ne)), dim3(kBlock), 0, ctx->stream, ne, ctx->nodex, S.r0, S.nbr,
-                     S.strain_limit, cls);
+  hipLaunchKernelGGL(k_surr_classify, dim3(grid_for(ne)), dim3(kBlock), 0, ctx->stream, ne, ctx->nodex, S.ref, S.rcls,
+                     S.nbr, S.strain_limit, cls);
   std::vector<int> hcls(ne);
   CAC_HIP(ctx, hipMemcpyAsync(hcls.data(), cls, ne * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -305,6 +440,16 @@ int surrogate_classify(cac_ctx *ctx) {
   S.nF = (int)F.size();
   S.nS = (int)Sl.size();
   if ((rc = upload_list(ctx, S.b_F, F, &S.elemF)) || (rc = upload_list(ctx, S.b_S, Sl, &S.elemS))) return rc;
+  {
+    std::vector<int> meta((size_t)kMeta * std::max<size_t>(Sl.size(), 1), 0);
+    for (size_t r = 0; r < Sl.size(); ++r) {
+      const int e = Sl[r];
+      meta[kMeta * r] = e;
+      meta[kMeta * r + 1] = S.hrcls[e];
+      std::copy(S.hnbr.begin() + 26 * (size_t)e, S.hnbr.begin() + 26 * (size_t)(e + 1), meta.begin() + kMeta * r + 2);
+    }
+    if ((rc = upload_list(ctx, S.b_meta, meta, &S.meta))) return rc;
+  }
   // R = F + neighbors, R2 = R + neighbors (device marking)
   auto expand = [&](const std::vector<int> &src, std::vector<int> &dst, DevBuf &b, int **dptr) -> int {
     int *dsrc;
@@ -346,32 +491,34 @@ int surrogate_forces(cac_ctx *ctx) {
   SurrState &S = ctx->surr;
   if (S.nS == 0) return CAC_OK;
   int rc;
-  if ((rc = ensure(ctx, S.b_ubar, 3 * (size_t)ctx->ne, &S.ubar))) return rc;
   hipEvent_t t0;
   timer_begin(ctx, "surr_feat", &t0);
-  hipLaunchKernelGGL(k_surr_centroid, dim3(grid_for(ctx->ne)), dim3(kBlock), 0, ctx->stream, ctx->ne, ctx->nodex,
-                     S.ubar);
+  FeatArgs fa;
+  if ((rc = frames(ctx, &fa))) return rc;
   int ldx = 0;
   uint16_t *x = mlp_input_buffer(ctx, S.nS, &ldx);
   if (!x) return fail(ctx, CAC_ERR_HIP, "surrogate input buffer");
   float *xf;
   if ((rc = ensure(ctx, S.b_xf, (size_t)S.nS * kNFeat, &xf))) return rc;
-  hipLaunchKernelGGL(k_surr_features, dim3((S.nS + kBlock / 64 - 1) / (kBlock / 64)), dim3(kBlock), 0, ctx->stream, S.nS, S.elemS, ctx->nodex,
-                     S.r0, S.d0, S.c0, S.ubar, S.nbr, corner_table(), prd_of(ctx), S.inv_in, x, ldx, xf);
+  hipLaunchKernelGGL(k_surr_features, dim3(grid_for(S.nS, kBlock / 64)), dim3(kBlock), 0, ctx->stream, S.nS,
+                     S.meta, fa, S.inv_in, x, ldx, xf);
   timer_end(ctx, "surr_feat", t0);
   timer_begin(ctx, "surr_mlp", &t0);
-  const float *y = mlp_forward_staged(ctx);
+  const float *y = mlp_forward_staged(ctx, S.nS);
   timer_end(ctx, "surr_mlp", t0);
-  hipLaunchKernelGGL(k_surr_scatter, dim3(grid_for((long)S.nS * kNOut)), dim3(kBlock), 0, ctx->stream, S.nS,
-                     S.elemS, y, mlp_out_ld(ctx), xf, S.lin_A, S.lin_b, S.out_scale, ctx->nodef);
+  timer_begin(ctx, "surr_fit", &t0);
+  hipLaunchKernelGGL(k_surr_predict, dim3(grid_for(S.nS, kPredElems)), dim3(8 * 64), 0, ctx->stream, S.nS,
+                     S.elemS, xf, y, mlp_out_ld(ctx), S.fit_A, S.fit_Q, S.fit_b, S.out_scale, ctx->nodef);
+  timer_end(ctx, "surr_fit", t0);
   CAC_HIP(ctx, hipGetLastError());
   return CAC_OK;
 }
 
 void surrogate_destroy(cac_ctx *ctx) {
   SurrState &S = ctx->surr;
-  DevBuf *bufs[] = {&S.b_X0, &S.b_d0, &S.b_c0, &S.b_A, &S.b_b, &S.b_xf, &S.b_nbr, &S.b_inv_in, &S.b_out, &S.b_cls, &S.b_mark, &S.b_F, &S.b_S, &S.b_R, &S.b_R2,
-                    &S.b_sR, &S.b_sR2, &S.b_tmp, &S.b_ubar, &S.b_feat};
+  DevBuf *bufs[] = {&S.b_ref, &S.b_rcls, &S.b_A,    &S.b_Q,  &S.b_b,   &S.b_xf, &S.b_nbr, &S.b_inv_in,
+                    &S.b_out, &S.b_cls,  &S.b_mark, &S.b_F,  &S.b_S,   &S.b_R,  &S.b_R2,  &S.b_sR,
+                    &S.b_sR2, &S.b_tmp,  &S.b_ubar, &S.b_feat, &S.b_meta};
   for (DevBuf *b : bufs)
     if (b->p) hipFree(b->p);
 }
@@ -390,52 +537,88 @@ int cac_surrogate_setup(cac_ctx *ctx, const double *X0, const int *nbr, const fl
   SurrState &S = ctx->surr;
   const int ne = ctx->ne;
   int rc;
-  if ((rc = ensure(ctx, S.b_X0, 24 * (size_t)ne, &S.r0)) || (rc = ensure(ctx, S.b_d0, 78 * (size_t)ne, &S.d0)) ||
-      (rc = ensure(ctx, S.b_c0, 168 * (size_t)ne, &S.c0)) || (rc = ensure(ctx, S.b_nbr, 26 * (size_t)ne, &S.nbr)) ||
+  if ((rc = ensure(ctx, S.b_nbr, 26 * (size_t)ne, &S.nbr)) || (rc = ensure(ctx, S.b_rcls, (size_t)ne, &S.rcls)) ||
       (rc = ensure(ctx, S.b_inv_in, kNFeat, &S.inv_in)) || (rc = ensure(ctx, S.b_out, kNOut, &S.out_scale)) ||
-      (rc = ensure(ctx, S.b_A, (size_t)kNOut * kNFeat, &S.lin_A)) || (rc = ensure(ctx, S.b_b, kNOut, &S.lin_b)))
+      (rc = ensure(ctx, S.b_A, (size_t)kNOut * kLdAq, &S.fit_A)) ||
+      (rc = ensure(ctx, S.b_Q, (size_t)kNOut * kLdQq, &S.fit_Q)) || (rc = ensure(ctx, S.b_b, kNOut, &S.fit_b)))
     return rc;
-  CAC_HIP(ctx, hipMemset(S.lin_A, 0, (size_t)kNOut * kNFeat * sizeof(float)));
-  CAC_HIP(ctx, hipMemset(S.lin_b, 0, kNOut * sizeof(float)));
-  // reference offsets: nodes from their centroid, neighbor centroids from the
-  // element's centroid (minimum image in the box set now, the reference box)
-  std::vector<double> cen(3 * (size_t)ne), r0(24 * (size_t)ne), d0(78 * (size_t)ne, 0.0);
+  CAC_HIP(ctx, hipMemset(S.fit_A, 0, (size_t)kNOut * kLdAq * sizeof(float)));
+  CAC_HIP(ctx, hipMemset(S.fit_Q, 0, (size_t)kNOut * kLdQq * sizeof(float)));
+  CAC_HIP(ctx, hipMemset(S.fit_b, 0, kNOut * sizeof(float)));
+  // reference rows (minimum image in the box set now, the reference box),
+  // deduplicated into classes: rows equal to 1e-9 A share one class
+  const CornerTab ct = corner_table();
+  std::vector<double> cen(3 * (size_t)ne);
   for (int e = 0; e < ne; ++e)
     for (int d = 0; d < 3; ++d) {
       double acc = 0.0;
       for (int k = 0; k < 8; ++k) acc += X0[24 * (size_t)e + 3 * k + d];
       cen[3 * (size_t)e + d] = acc / 8;
-      for (int k = 0; k < 8; ++k) r0[24 * (size_t)e + 3 * k + d] = X0[24 * (size_t)e + 3 * k + d] - acc / 8;
     }
-  for (int e = 0; e < ne; ++e)
+  auto mi = [&](double dd, int d) { return dd - ctx->prd[d] * std::rint(dd / ctx->prd[d]); };
+  std::vector<double> table;
+  std::vector<int> rcls(ne);
+  std::map<std::vector<long long>, int> index;
+  std::vector<double> row(kRefLen);
+  std::vector<long long> key(kRefLen);
+  for (int e = 0; e < ne; ++e) {
+    std::fill(row.begin(), row.end(), 0.0);
+    for (int k = 0; k < 8; ++k)
+      for (int d = 0; d < 3; ++d) row[3 * k + d] = X0[24 * (size_t)e + 3 * k + d] - cen[3 * (size_t)e + d];
     for (int m = 0; m < 26; ++m) {
       const int j = nbr[26 * (size_t)e + m];
       if (j < -1 || j >= ne) return fail(ctx, CAC_ERR_ARG, "neighbor table entry out of range");
-      if (j < 0) continue;
-      for (int d = 0; d < 3; ++d) {
-        const double dd = cen[3 * (size_t)j + d] - cen[3 * (size_t)e + d];
-        d0[78 * (size_t)e + 3 * m + d] = dd - ctx->prd[d] * std::rint(dd / ctx->prd[d]);
-      }
+      if (j >= 0)
+        for (int d = 0; d < 3; ++d)
+          row[kRefD0 + 3 * m + d] = mi(cen[3 * (size_t)j + d] - cen[3 * (size_t)e + d], d);
     }
-  const CornerTab ct = corner_table();
-  std::vector<double> c0(168 * (size_t)ne, 0.0);
-  for (int e = 0; e < ne; ++e)
     for (int q = 0; q < 56; ++q) {
       const int j = nbr[26 * (size_t)e + ct.m[q]];
-      if (j < 0) continue;
-      for (int d = 0; d < 3; ++d) {
-        const double dd = X0[24 * (size_t)j + 3 * ct.node[q] + d] - cen[3 * (size_t)e + d];
-        c0[168 * (size_t)e + 3 * q + d] = dd - ctx->prd[d] * std::rint(dd / ctx->prd[d]);
-      }
+      if (j >= 0)
+        for (int d = 0; d < 3; ++d)
+          row[kRefC0 + 3 * q + d] = mi(X0[24 * (size_t)j + 3 * ct.node[q] + d] - cen[3 * (size_t)e + d], d);
     }
+    // G0^-1, G0 = sum_k r0_k r0_k^T (positive definite for a non-degenerate element)
+    double G[3][3] = {};
+    for (int k = 0; k < 8; ++k)
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) G[i][j] += row[3 * k + i] * row[3 * k + j];
+    const double det = G[0][0] * (G[1][1] * G[2][2] - G[1][2] * G[2][1]) -
+                       G[0][1] * (G[1][0] * G[2][2] - G[1][2] * G[2][0]) +
+                       G[0][2] * (G[1][0] * G[2][1] - G[1][1] * G[2][0]);
+    if (!(std::fabs(det) > 0)) return fail(ctx, CAC_ERR_ARG, "degenerate reference element");
+    double *I = row.data() + kRefG;
+    I[0] = (G[1][1] * G[2][2] - G[1][2] * G[2][1]) / det;
+    I[1] = (G[0][2] * G[2][1] - G[0][1] * G[2][2]) / det;
+    I[2] = (G[0][1] * G[1][2] - G[0][2] * G[1][1]) / det;
+    I[3] = (G[1][2] * G[2][0] - G[1][0] * G[2][2]) / det;
+    I[4] = (G[0][0] * G[2][2] - G[0][2] * G[2][0]) / det;
+    I[5] = (G[0][2] * G[1][0] - G[0][0] * G[1][2]) / det;
+    I[6] = (G[1][0] * G[2][1] - G[1][1] * G[2][0]) / det;
+    I[7] = (G[0][1] * G[2][0] - G[0][0] * G[2][1]) / det;
+    I[8] = (G[0][0] * G[1][1] - G[0][1] * G[1][0]) / det;
+    for (int c = 0; c < kRefG; ++c) key[c] = std::llround(row[c] * 1e9);
+    for (int c = kRefG; c < kRefLen; ++c) key[c] = std::llround(row[c] * 1e15);
+    auto it = index.find(key);
+    if (it == index.end()) {
+      it = index.emplace(key, (int)(table.size() / kRefLen)).first;
+      table.insert(table.end(), row.begin(), row.end());
+    }
+    rcls[e] = it->second;
+  }
+  S.nref = (int)(table.size() / kRefLen);
+  double *dref;
+  if ((rc = ensure(ctx, S.b_ref, table.size(), &dref))) return rc;
+  S.ref = dref;
   std::vector<float> inv(kNFeat);
   for (int c = 0; c < kNFeat; ++c) {
     if (!(in_scale[c] > 0)) return fail(ctx, CAC_ERR_ARG, "input scales must be positive");
     inv[c] = 1.0f / in_scale[c];
   }
-  CAC_HIP(ctx, hipMemcpy(S.r0, r0.data(), r0.size() * sizeof(double), hipMemcpyHostToDevice));
-  CAC_HIP(ctx, hipMemcpy(S.d0, d0.data(), d0.size() * sizeof(double), hipMemcpyHostToDevice));
-  CAC_HIP(ctx, hipMemcpy(S.c0, c0.data(), c0.size() * sizeof(double), hipMemcpyHostToDevice));
+  CAC_HIP(ctx, hipMemcpy(S.ref, table.data(), table.size() * sizeof(double), hipMemcpyHostToDevice));
+  CAC_HIP(ctx, hipMemcpy(S.rcls, rcls.data(), rcls.size() * sizeof(int), hipMemcpyHostToDevice));
+  S.hrcls = rcls;
+  S.hnbr.assign(nbr, nbr + 26 * (size_t)ne);
   CAC_HIP(ctx, hipMemcpy(S.nbr, nbr, 26 * (size_t)ne * sizeof(int), hipMemcpyHostToDevice));
   CAC_HIP(ctx, hipMemcpy(S.inv_in, inv.data(), kNFeat * sizeof(float), hipMemcpyHostToDevice));
   CAC_HIP(ctx, hipMemcpy(S.out_scale, out_scale, kNOut * sizeof(float), hipMemcpyHostToDevice));
@@ -445,13 +628,20 @@ int cac_surrogate_setup(cac_ctx *ctx, const double *X0, const int *nbr, const fl
   return CAC_OK;
 }
 
-int cac_surrogate_set_linear(cac_ctx *ctx, const float *A, const float *b) {
-  if (!ctx || !A || !b) return CAC_ERR_ARG;
+int cac_surrogate_set_fit(cac_ctx *ctx, const float *A, const float *Q, const float *b) {
+  if (!ctx || !A || !Q || !b) return CAC_ERR_ARG;
   SurrState &S = ctx->surr;
   if (!S.setup) return fail(ctx, CAC_ERR_ARG, "cac_surrogate_setup not called");
   hipSetDevice(ctx->device);
-  CAC_HIP(ctx, hipMemcpy(S.lin_A, A, (size_t)kNOut * kNFeat * sizeof(float), hipMemcpyHostToDevice));
-  CAC_HIP(ctx, hipMemcpy(S.lin_b, b, kNOut * sizeof(float), hipMemcpyHostToDevice));
+  // rows padded to 16 bytes (kLdAq, kLdQq floats) for the staging loads
+  std::vector<float> ap((size_t)kNOut * kLdAq, 0.0f), qp((size_t)kNOut * kLdQq, 0.0f);
+  for (int c = 0; c < kNOut; ++c) {
+    std::copy(A + (size_t)c * kNFeat, A + (size_t)(c + 1) * kNFeat, ap.begin() + (size_t)c * kLdAq);
+    std::copy(Q + (size_t)c * kNQuad, Q + (size_t)(c + 1) * kNQuad, qp.begin() + (size_t)c * kLdQq);
+  }
+  CAC_HIP(ctx, hipMemcpy(S.fit_A, ap.data(), ap.size() * sizeof(float), hipMemcpyHostToDevice));
+  CAC_HIP(ctx, hipMemcpy(S.fit_Q, qp.data(), qp.size() * sizeof(float), hipMemcpyHostToDevice));
+  CAC_HIP(ctx, hipMemcpy(S.fit_b, b, kNOut * sizeof(float), hipMemcpyHostToDevice));
   return CAC_OK;
 }
 
@@ -462,7 +652,7 @@ int cac_surrogate_enable(cac_ctx *ctx, int on) {
   if (on) {
     if (!S.setup) return fail(ctx, CAC_ERR_ARG, "cac_surrogate_setup not called");
     if (!ctx->mlp) return fail(ctx, CAC_ERR_ARG, "cac_mlp_create not called");
-    if (!mlp_shape_ok(ctx, kNFeat, kNOut)) return fail(ctx, CAC_ERR_ARG, "the MLP must map 270 features to 24 forces");
+    if (!mlp_shape_ok(ctx, kNFeat, kNOut)) return fail(ctx, CAC_ERR_ARG, "the MLP must map 279 features to 24 forces");
     if (ctx->pair != CAC_PAIR_EAM) return fail(ctx, CAC_ERR_ARG, "the surrogate is trained for pair_style eam");
   }
   S.enabled = on != 0;
@@ -480,14 +670,12 @@ int cac_surrogate_features(cac_ctx *ctx, float *feats, int *cls) {
   int rc;
   float *f;
   int *c;
-  if ((rc = ensure(ctx, S.b_ubar, 3 * (size_t)ne, &S.ubar)) || (rc = ensure(ctx, S.b_feat, kNFeat * (size_t)ne, &f)) ||
-      (rc = ensure(ctx, S.b_cls, (size_t)ne, &c)))
-    return rc;
-  hipLaunchKernelGGL(k_surr_centroid, dim3(grid_for(ne)), dim3(kBlock), 0, ctx->stream, ne, ctx->nodex, S.ubar);
-  hipLaunchKernelGGL(k_surr_features_all, dim3((ne + kBlock / 64 - 1) / (kBlock / 64)), dim3(kBlock), 0, ctx->stream, ne, ctx->nodex, S.r0,
-                     S.d0, S.c0, S.ubar, S.nbr, corner_table(), prd_of(ctx), f);
-  hipLaunchKernelGGL(k_surr_classify, dim3(grid_for(ne)), dim3(kBlock), 0, ctx->stream, ne, ctx->nodex, S.r0, S.nbr,
-                     S.strain_limit, c);
+  if ((rc = ensure(ctx, S.b_feat, kNFeat * (size_t)ne, &f)) || (rc = ensure(ctx, S.b_cls, (size_t)ne, &c))) return rc;
+  FeatArgs fa;
+  if ((rc = frames(ctx, &fa))) return rc;
+  hipLaunchKernelGGL(k_surr_features_all, dim3(grid_for(ne, kBlock / 64)), dim3(kBlock), 0, ctx->stream, ne, fa, f);
+  hipLaunchKernelGGL(k_surr_classify, dim3(grid_for(ne)), dim3(kBlock), 0, ctx->stream, ne, ctx->nodex, S.ref, S.rcls,
+                     S.nbr, S.strain_limit, c);
   CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (feats) CAC_HIP(ctx, hipMemcpy(feats, f, kNFeat * (size_t)ne * sizeof(float), hipMemcpyDeviceToHost));
   if (cls) CAC_HIP(ctx, hipMemcpy(cls, c, (size_t)ne * sizeof(int), hipMemcpyDeviceToHost));

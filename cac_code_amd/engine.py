@@ -24,6 +24,8 @@ from . import _lib
 from .lattice import CACSystem, FTM2V_METAL
 from .potential import Funcfl, lj_coeffs, read_funcfl
 
+SURR_NFEAT, SURR_NOUT, SURR_NQUAD = 279, 24, 561   # CAC_SURR_* in cac_hip.h
+
 
 def _p(a, t=_lib._d):
     return a.ctypes.data_as(t)
@@ -148,7 +150,8 @@ def eam_tables(ff: Funcfl):
 class _MlpMixin:
     """Device MLP and surrogate entry points of Engine (cac_mlp_*, cac_surrogate_*)."""
 
-    def mlp_create(self, nin=270, nhidden=256, nlayers=3, nout=24, seed=0):
+    def mlp_create(self, nin=None, nhidden=256, nlayers=3, nout=24, seed=0):
+        nin = SURR_NFEAT if nin is None else nin
         _check(self.ctx, self.L.cac_mlp_create(self.ctx, nin, nhidden, nlayers, nout, seed))
         self.mlp_shape = (nin, nhidden, nlayers, nout)
 
@@ -210,15 +213,16 @@ class _MlpMixin:
         _check(self.ctx, self.L.cac_surrogate_setup(self.ctx, _p(X0), _p(nbr, _lib._i), _p(a, _lib._f),
                                                     _p(b, _lib._f), strain_limit))
 
-    def surrogate_set_linear(self, A, b):
-        A = np.ascontiguousarray(A, dtype=np.float32); b = np.ascontiguousarray(b, dtype=np.float32)
-        _check(self.ctx, self.L.cac_surrogate_set_linear(self.ctx, _p(A, _lib._f), _p(b, _lib._f)))
+    def surrogate_set_fit(self, A, Q, b):
+        A = np.ascontiguousarray(A, dtype=np.float32); Q = np.ascontiguousarray(Q, dtype=np.float32)
+        b = np.ascontiguousarray(b, dtype=np.float32)
+        _check(self.ctx, self.L.cac_surrogate_set_fit(self.ctx, _p(A, _lib._f), _p(Q, _lib._f), _p(b, _lib._f)))
 
     def surrogate_enable(self, on=True):
         _check(self.ctx, self.L.cac_surrogate_enable(self.ctx, int(on)))
 
     def surrogate_features(self):
-        f = np.zeros((self.ne, 270), dtype=np.float32); c = np.zeros(self.ne, dtype=np.int32)
+        f = np.zeros((self.ne, SURR_NFEAT), dtype=np.float32); c = np.zeros(self.ne, dtype=np.int32)
         _check(self.ctx, self.L.cac_surrogate_features(self.ctx, _p(f, _lib._f), _p(c, _lib._i)))
         return f, c
 

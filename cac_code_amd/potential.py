@@ -54,7 +54,15 @@ class Funcfl:
     zr: np.ndarray    # [nr], r*phi(r) at r = m*dr
 
 
-def johnson_cu_functions(rc: float = 4.95, taper_width: float = 0.5, params: dict | None = None):
+# Taper window [rc - 0.25, rc] = [4.70, 4.95] A: every fcc shell stays out of
+# it under |strain| <= 2% (3rd shell a*sqrt(3/2) = 4.427 A -> at most 4.52 A;
+# 4th shell a*sqrt(2) = 5.112 A -> at least 5.01 A), so the force constants of
+# the lattice are analytic over that range.  A 0.5 A window would start at
+# 4.45 A, on top of the 3rd shell, where the switch's third derivative jumps.
+TAPER_WIDTH = 0.25
+
+
+def johnson_cu_functions(rc: float = 4.95, taper_width: float = TAPER_WIDTH, params: dict | None = None):
     """Analytic density f(r), pair phi(r) and embedding F(rho) of the
     Johnson-type Cu model (with the C2 taper on f and phi), plus rho_e."""
     p = dict(JOHNSON_CU)
@@ -89,7 +97,7 @@ def johnson_cu_functions(rc: float = 4.95, taper_width: float = 0.5, params: dic
 
 
 def johnson_cu_funcfl(nrho: int = 10000, nr: int = 10000, rc: float = 4.95,
-                      taper_width: float = 0.5, rho_max_factor: float = 2.0,
+                      taper_width: float = TAPER_WIDTH, rho_max_factor: float = 2.0,
                       params: dict | None = None) -> Funcfl:
     """Tabulate the analytic Johnson Cu EAM with a C2 taper on f and phi."""
     f_of, phi_of, F_of, rho_e = johnson_cu_functions(rc, taper_width, params)

@@ -175,20 +175,28 @@ int cac_mlp_train(cac_ctx *ctx, int epochs, int batch, float lr, uint64_t seed, 
 
 /* Surrogate on the force path (no counterpart in the reference).  X0:
  * [ne][8][3] reference node positions; nbr: [ne][26] mesh neighbors of each
- * element (-1 where the neighbor block is not an element); in_scale [270],
- * out_scale [24] normalize the features and nodal forces.  Features: 24
- * element-frame nodal displacements, 78 neighbor-centroid displacements, 168
- * displacements of the neighbor nodes meeting each corner.  Prediction:
- * A x + b (cac_surrogate_set_linear, [24][270] FP32) plus the MLP (270 -> 24).
- * While enabled, cac_force (EAM, eflag = vflag = 0) takes the nodal forces of
- * every element from the surrogate except fallback elements (interface, or
- * Green-Lagrange strain above strain_limit), which keep the plain
- * integration-point path; classification is refreshed at each rebuild.
- * cac_surrogate_features writes the raw features [ne][270] and the class
- * per element (0 surrogate, 1 interface, 2 strained). */
+ * element (-1 where the neighbor block is not an element); in_scale
+ * [CAC_SURR_NFEAT], out_scale [CAC_SURR_NOUT] normalize the features and the
+ * nodal forces.  Features: deviations from the element's best-fit affine map
+ * of its 8 nodes (24), of the 26 neighbor centroids (78) and of the 7
+ * neighbor nodes meeting each corner (168), then the strain F - I (9).
+ * Prediction of nodal force c = 3k + d (normalized):
+ *   b[c] + A[c] . x + sum_{i<=j} Q[c][ij] z_i z_j + MLP(x)[c]
+ * with z the 33 local inputs of corner k (own node k, its 7 corner
+ * neighbors, the strain); A, Q, b are set by cac_surrogate_set_fit (FP32),
+ * the MLP (CAC_SURR_NFEAT -> 24) by cac_mlp_set_weights.  While enabled,
+ * cac_force (EAM, eflag = vflag = 0) takes the nodal forces of every element
+ * from the surrogate except fallback elements (interface, or Green-Lagrange
+ * strain above strain_limit), which keep the plain integration-point path;
+ * classification is refreshed at each rebuild.  cac_surrogate_features
+ * writes the raw features [ne][CAC_SURR_NFEAT] and the class per element
+ * (0 surrogate, 1 interface, 2 strained). */
+#define CAC_SURR_NFEAT 279
+#define CAC_SURR_NOUT 24
+#define CAC_SURR_NQUAD 561
 int cac_surrogate_setup(cac_ctx *ctx, const double *X0, const int *nbr, const float *in_scale,
                         const float *out_scale, double strain_limit);
-int cac_surrogate_set_linear(cac_ctx *ctx, const float *A, const float *b);
+int cac_surrogate_set_fit(cac_ctx *ctx, const float *A, const float *Q, const float *b);
 int cac_surrogate_enable(cac_ctx *ctx, int on);
 int cac_surrogate_features(cac_ctx *ctx, float *feats, int *cls);
 int cac_surrogate_info(cac_ctx *ctx, int *ninterface, int *nstrained, int *nsurrogate);

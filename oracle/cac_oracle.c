@@ -37,6 +37,14 @@ int orc_num_threads(void) {
 #endif
 }
 
+void orc_set_num_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
 static void img_shift(int img, int s[3]) {
   s[0] = img % 3 - 1;
   s[1] = (img / 3) % 3 - 1;

@@ -121,6 +121,7 @@ void orc_pbc_elements(int ne, double *xc, double *nodex, int *image, const orc_b
 int  orc_max_displacement_exceeds(int n, const double *x, const double *xhold, double triggersq);
 
 int orc_num_threads(void);
+void orc_set_num_threads(int n);
 
 #ifdef __cplusplus
 }

@@ -312,3 +312,7 @@ def displacement_exceeds(x, xhold, triggersq):
 
 def num_threads():
     return int(lib().orc_num_threads())
+
+
+def set_num_threads(n: int):
+    lib().orc_set_num_threads(C.c_int(n))

@@ -53,3 +53,13 @@ def test_product_never_imports_oracle():
             if f.endswith((".py", ".hip", ".cpp", ".h")):
                 text = open(os.path.join(dirpath, f)).read()
                 assert not pat.search(text), f
+
+
+def test_surrogate_constants_match_header():
+    import re
+    from cac_code_amd import engine, surrogate
+    text = open(_lib.HEADER).read()
+    vals = {k: int(v) for k, v in re.findall(r"#define (CAC_SURR_\w+) (\d+)", text)}
+    assert (engine.SURR_NFEAT, engine.SURR_NOUT, engine.SURR_NQUAD) == (
+        vals["CAC_SURR_NFEAT"], vals["CAC_SURR_NOUT"], vals["CAC_SURR_NQUAD"])
+    assert surrogate.NIN == engine.SURR_NFEAT and surrogate.NOUT == engine.SURR_NOUT

@@ -43,9 +43,12 @@ def test_mlp_forward_matches_oracle(n):
         y = eng.mlp_forward(x)
     yr, _ = M.forward(w, x, "bf16")
     # FP32 accumulation on the device vs float64 here, on identical BF16
-    # operands: agreement to FP32 accumulation error
+    # operands.  Hidden activations are rounded to BF16 between layers: where
+    # the two sums fall on either side of a BF16 rounding boundary (a few in
+    # 1e4 activations) the rounded activations differ by one BF16 ulp
+    # (2^-8 relative), which sets the RMS bound; the max bound is a few ulps.
     assert np.max(np.abs(y - yr)) <= 2e-3 * np.max(np.abs(yr)), np.max(np.abs(y - yr))
-    assert np.sqrt(np.mean((y - yr) ** 2)) <= 2e-4 * np.sqrt(np.mean(yr ** 2))
+    assert np.sqrt(np.mean((y - yr) ** 2)) <= 5e-4 * np.sqrt(np.mean(yr ** 2))
 
 
 def test_mlp_gradients_match_oracle():
@@ -79,84 +82,96 @@ def test_mlp_training_reduces_loss():
     assert np.isfinite(l2)
 
 
-def test_features_closed_form_under_strain(pair):
+def _features_numpy(s, X0, X, box, nbr):
+    """The 279 features restated in numpy: the affine map by least squares,
+    corner nodes found by nearest-node search around each corner."""
+    prd0 = s.box_hi - s.box_lo
+    ne = len(X)
+    c0, c = X0.mean(axis=1), X.mean(axis=1)
+    r0 = X0 - c0[:, None]
+    G0 = np.einsum("eki,ekj->eij", r0, r0)
+    Fe = np.einsum("eki,ekj->eij", X - c[:, None], r0) @ np.linalg.inv(G0)
+    dev = lambda v, v0: v - np.einsum("eij,e...j->e...i", Fe, v0)
+    mi = lambda v, p: v - p * np.rint(v / p)
+    own = dev(X - c[:, None], r0).reshape(ne, 24)
+    cent = dev(mi(c[nbr] - c[:, None, :], box), mi(c0[nbr] - c0[:, None, :], prd0)).reshape(ne, 78)
+    offs = [(a, b, cc) for a in (-1, 0, 1) for b in (-1, 0, 1) for cc in (-1, 0, 1) if (a, b, cc) != (0, 0, 0)]
+    axes = np.stack([X0[0, 1] - X0[0, 0], X0[0, 3] - X0[0, 0], X0[0, 4] - X0[0, 0]])
+    corner = []
+    for k in range(8):
+        toward = np.sign(r0[0, k] @ np.linalg.inv(axes))
+        ms = [m for m, o in enumerate(offs) if all(oi == 0 or oi == ti for oi, ti in zip(o, toward))]
+        assert len(ms) == 7
+        for m in ms:
+            j = nbr[:, m]
+            v = mi(X0[j] - X0[np.arange(ne), k][:, None, :], prd0)
+            dist = np.linalg.norm(v, axis=2)
+            n = np.argmin(dist, axis=1)
+            # a block touching the corner has a node within a few lattice
+            # vectors of it (one primitive vector per shifted axis)
+            assert np.all(dist.min(axis=1) < 3.615 * np.sqrt(3) + 1e-6)
+            corner.append(dev(mi(X[j, n] - c, box)[:, None, :], mi(X0[j, n] - c0, prd0)[:, None, :])[:, 0])
+    corner = np.stack(corner, axis=1).reshape(ne, 168)
+    strain = (Fe - np.eye(3)).reshape(ne, 9)
+    return np.concatenate([own, cent, corner, strain], axis=1)
+
+
+def test_features_match_numpy_restatement(pair):
     s = build_cac_system(33, 11, 0.0)
     X0 = s.nodex.copy()
     nbr = S.mesh_neighbors(s)
+    prd = s.box_hi - s.box_lo
     eps = np.array([0.01, -0.015, 0.005])
+    rng = np.random.default_rng(8)
     with Engine(0) as eng:
         eng.load_system(s)
         eng.set_pair(pair)
         eng.surrogate_setup(X0, nbr, np.ones(NIN), np.ones(24))
         f, cls = eng.surrogate_features()
         assert np.max(np.abs(f)) < 1e-9 and np.all(cls == 0)
-        eng.set_box(s.box_lo, s.box_lo + (s.box_hi - s.box_lo) * (1 + eps))
+        # homogeneous strain: zero deviations, strain features diag(eps)
+        box = prd * (1 + eps)
+        eng.set_box(s.box_lo, s.box_lo + box)
         eng.upload_elements(X0 * (1 + eps), None, s.nae, s.mass_e, s.intp_ijk, s.intp_w)
         f, cls = eng.surrogate_features()
-    r0 = X0 - X0.mean(axis=1, keepdims=True)
-    assert np.allclose(f[:, :24], (r0 * eps).reshape(-1, 24), atol=1e-5)
-    cen = X0.mean(axis=1)
-    prd = s.box_hi - s.box_lo
-    d = cen[nbr] - cen[:, None, :]
-    d -= prd * np.rint(d / prd)
-    assert np.allclose(f[:, 24:102], (d * eps).reshape(-1, 78), atol=1e-5)
-    # corner inputs: for own corner k and each of the 7 neighbor blocks that
-    # touch it (offset components 0 or toward the corner), the neighbor's
-    # node nearest to corner k, as an offset from the own centroid
-    offs = [(a, b, c) for a in (-1, 0, 1) for b in (-1, 0, 1) for c in (-1, 0, 1) if (a, b, c) != (0, 0, 0)]
-    expect = []
-    for k in range(8):
-        toward = np.sign(r0[0, k] @ np.linalg.inv(_block_axes(s)))
-        ms = [m for m, o in enumerate(offs) if all(oi == 0 or oi == ti for oi, ti in zip(o, toward))]
-        assert len(ms) == 7
-        for m in ms:
-            j = nbr[:, m]
-            v = X0[j] - X0[np.arange(len(j)), k][:, None, :]
-            v -= prd * np.rint(v / prd)
-            dist = np.linalg.norm(v, axis=2)
-            node = np.argmin(dist, axis=1)
-            # a block touching the corner has a node within a few lattice
-            # vectors of it (one primitive vector per shifted axis, <= a*sqrt(3))
-            assert np.all(dist.min(axis=1) < 3.615 * np.sqrt(3) + 1e-6)
-            c = X0[j, node] - cen
-            c -= prd * np.rint(c / prd)
-            expect.append(c * eps)
-    expect = np.stack(expect, axis=1).reshape(-1, 168)
-    assert np.allclose(f[:, 102:], expect, atol=1e-5)
-    assert np.all(cls == 0)                     # |eps| < 2%
+        assert np.max(np.abs(f[:, :270])) < 1e-9
+        assert np.allclose(f[:, 270:], np.diag(eps).reshape(1, 9), atol=1e-12)
+        assert np.all(cls == 0)                     # |eps| < 2%
+        # strain plus independent nodal displacements
+        X = X0 * (1 + eps) + 0.02 * rng.standard_normal(X0.shape)
+        eng.upload_elements(X, None, s.nae, s.mass_e, s.intp_ijk, s.intp_w)
+        f, _ = eng.surrogate_features()
+    ref = _features_numpy(s, X0, X, box, nbr)
+    assert np.max(np.abs(f - ref)) < 1e-6, np.max(np.abs(f - ref))
+    assert np.max(np.abs(ref[:, 102:270])) > 1e-2  # the corner deviations are exercised
     with Engine(0) as eng:
         eng.load_system(s)
         eng.set_pair(pair)
         eng.surrogate_setup(X0, nbr, np.ones(NIN), np.ones(24), strain_limit=0.012)
-        eng.set_box(s.box_lo, s.box_lo + (s.box_hi - s.box_lo) * (1 + eps))
+        eng.set_box(s.box_lo, s.box_lo + box)
         eng.upload_elements(X0 * (1 + eps), None, s.nae, s.mass_e, s.intp_ijk, s.intp_w)
         _, cls = eng.surrogate_features()
     assert np.all(cls == 2)                     # 1.5% > 1.2% limit: strained
 
 
-def _block_axes(s):
-    """Rows: the element's three edge vectors (node 0 to nodes 1, 3, 4)."""
-    X = s.nodex[0]
-    return np.stack([X[1] - X[0], X[3] - X[0], X[4] - X[0]])
-
-
 @pytest.fixture(scope="module")
 def small_model(pair):
-    ds = S.generate_dataset(pair, 12000, L=33, seed=21, log=print)
+    ds = S.generate_dataset(pair, 24000, L=33, seed=21, log=print)
     model = S.train(ds, epochs=40, batch=512, lr=1e-3, log=print)
     return ds, model
 
 
 def test_trained_surrogate_gain(pair, small_model):
-    """On held-out plain-path nodal forces, the linear part alone explains
-    most of the force and the trained network removes much of what is left."""
+    """On held-out plain-path nodal forces, the least-squares part explains
+    all but a few percent and the trained network removes a good part of
+    what is left."""
     ds, model = small_model
-    held = S.generate_dataset(pair, 3000, L=33, seed=99)
+    held = S.generate_dataset(pair, 4000, L=33, seed=99)
     err = S.rel_rms(S.predict(model, held.x), held.t)
-    err_lin = S.rel_rms(S.predict(model, held.x, linear_only=True), held.t)
-    print(f"[surrogate] held-out rel RMS nodal-force error {err:.4f} (linear part alone {err_lin:.4f}, zero 1.0)")
-    assert err_lin < 0.2
-    assert err < 0.7 * err_lin
+    err_fit = S.rel_rms(S.predict(model, held.x, structured_only=True), held.t)
+    print(f"[surrogate] held-out rel RMS nodal-force error {err:.5f} (least-squares part alone {err_fit:.5f})")
+    assert err_fit < 0.05
+    assert err < 0.8 * err_fit
 
 
 def test_fallback_elements_keep_plain_forces(pair, small_model):
