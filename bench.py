@@ -76,14 +76,11 @@ def make_pair(workload):
     return p, 5.95, 1.0, ff
 
 
+COUPLED = True  # atom-element coupling (set from --no-coupling in main)
+
+
 def gpu_run(s, pair, cutneigh, skin, steps, warmup, precision, every, delay):
-    from cac_code_amd import Engine
-    eng = Engine(0)
-    eng.load_system(s)
-    eng.set_pair(pair)
-    eng.set_neighbor(cutneigh, skin=skin, every=every, delay=delay, check=True)
-    eng.build()
-    eng.force()
+    eng = new_engine(s, pair, cutneigh, skin, every, delay)
     nb_warm = eng.run(warmup, precision=precision) if warmup else 0
     eng.synchronize()
     eng.profile(True)
@@ -93,7 +90,7 @@ def gpu_run(s, pair, cutneigh, skin, steps, warmup, precision, every, delay):
     t1 = time.perf_counter()
     prof = {}
     for name in ("atom_rho", "atom_force", "site_pos", "ip_erho", "site_fp", "ip_force", "scatter", "atom_neigh",
-                 "ip_neigh", "elem_neigh"):
+                 "ip_neigh", "elem_neigh", "atom_sites"):
         ms, cnt = eng.profile_avg_ms(name)
         if cnt:
             prof[name] = (ms, cnt)
@@ -114,15 +111,28 @@ def cpu_baseline(s, pair, ff, cutneigh, threads, warm, timed, rebuild_rate):
     x = s.x.copy(); v = s.v.copy(); nodex = s.nodex.copy(); nodev = s.nodev.copy()
     es = O.ElementSet(nodex, s.nae, s.intp_ijk, s.intp_w)
     t0 = time.perf_counter()
-    anum, anb = O.atom_neighbors(x, s.box_lo, s.box_hi, cutneigh, 128)
-    inum, inb = O.ip_neighbors(es, s.box_lo, s.box_hi, cutneigh, 2, 128)
+    if COUPLED:
+        anum, anb, inum, inb = O.coupled_neighbors(x, es, s.box_lo, s.box_hi, cutneigh, 2, 128, 128)
+    else:
+        anum, anb = O.atom_neighbors(x, s.box_lo, s.box_hi, cutneigh, 128)
+        inum, inb = O.ip_neighbors(es, s.box_lo, s.box_hi, cutneigh, 2, 128)
     t_build = time.perf_counter() - t0
-    if ff is not None:
-        T = O.eam_tables(ff)
+    lj = pair.init_one(1, 1) if ff is None else None
+    T = O.eam_tables(ff) if ff is not None else None
+    if COUPLED:
+        last = {}
+
+        def fa():
+            fn = O.eam_coupled if ff is not None else O.lj_coupled
+            r = fn(x, es, s.box_lo, s.box_hi, anum, anb, inum, inb, T if ff is not None else lj)
+            last["nodef"] = r["nodef"]
+            return r["f"]
+
+        fe = lambda: last["nodef"]
+    elif ff is not None:
         fa = lambda: O.eam_atoms(x, s.box_lo, s.box_hi, anum, anb, T)["f"]
         fe = lambda: O.eam_elements(es, s.box_lo, s.box_hi, inum, inb, T)["nodef"]
     else:
-        lj = pair.init_one(1, 1)
         fa = lambda: O.lj_atoms(x, s.box_lo, s.box_hi, anum, anb, lj)["f"]
         fe = lambda: O.lj_elements(es, s.box_lo, s.box_hi, inum, inb, lj)["nodef"]
     dt, dtf = 0.001, 0.5 * 0.001 * FTM2V_METAL
@@ -167,22 +177,26 @@ def new_engine(s, pair, cutneigh, skin, every, delay):
     eng.load_system(s)
     eng.set_pair(pair)
     eng.set_neighbor(cutneigh, skin=skin, every=every, delay=delay, check=True)
+    eng.set_coupling(COUPLED)
     eng.build()
     eng.force()
     return eng
 
 
 def _err(test, ref):
-    """Relative RMS and max of a force difference: ||test - ref|| / ||ref||
-    over all components, max_i |test_i - ref_i| / RMS_i |ref_i|."""
+    """Force difference: absolute RMS (eV/A) of |test - ref| per particle,
+    RMS |ref|, and relative RMS / max (divided by RMS |ref|; meaningless
+    where the reference forces vanish, e.g. the perfect lattice at step 0)."""
     import numpy as np
     d = (test - ref).reshape(-1, 3)
     r = ref.reshape(-1, 3)
     rms = float(np.sqrt(np.mean(np.sum(r * r, axis=1))))
-    if rms == 0.0:
-        return {"rel_rms": None, "rel_max": None}
-    return {"rel_rms": float(np.sqrt(np.mean(np.sum(d * d, axis=1)))) / rms,
-            "rel_max": float(np.sqrt(np.max(np.sum(d * d, axis=1)))) / rms}
+    drms = float(np.sqrt(np.mean(np.sum(d * d, axis=1))))
+    dmax = float(np.sqrt(np.max(np.sum(d * d, axis=1))))
+    out = {"abs_rms_eV_A": drms, "ref_rms_eV_A": rms}
+    if rms > 0.0:
+        out.update(rel_rms=drms / rms, rel_max=dmax / rms)
+    return out
 
 
 def accuracy(s, pair, cutneigh, skin, every, delay, at_steps, model=None):
@@ -233,8 +247,11 @@ def accuracy(s, pair, cutneigh, skin, every, delay, at_steps, model=None):
     return out
 
 
-def drift(s, pair, cutneigh, skin, every, delay, nsteps, precision, chunk=1000):
-    """Total energy (build-defined PE + KE) every `chunk` steps of an NVE run."""
+def drift(s, pair, cutneigh, skin, every, delay, nsteps, precision, chunk=1000, limit=0.5):
+    """Total energy (build-defined PE + KE) every `chunk` steps of an NVE run,
+    stopped once |E - E0| exceeds `limit` eV per integrated particle (the
+    force-based CAC forces are not the gradient of an energy, so the
+    element region and the atom-element interface can heat up)."""
     from cac_code_amd.lattice import kinetic_energy
     eng = new_engine(s, pair, cutneigh, skin, every, delay)
     n_int = s.n_atoms + 8 * s.n_elements
@@ -249,14 +266,20 @@ def drift(s, pair, cutneigh, skin, every, delay, nsteps, precision, chunk=1000):
     e0 = etot()
     series = [0.0]
     done = 0
+    diverged = None
     t0 = time.perf_counter()
-    while done < nsteps:
+    while done < nsteps and diverged is None:
         k = min(chunk, nsteps - done)
-        eng.run(k, precision=precision)
+        nb = eng.run(k, precision=precision)
         done += k
         series.append((etot() - e0) / n_int)
+        log(f"[bench] drift prec={precision} step {done}: {series[-1]:+.3e} eV/particle, {nb} rebuilds, "
+            f"{time.perf_counter() - t0:.1f}s")
+        if abs(series[-1]) > limit:
+            diverged = done
     eng.close()
-    return {"steps": nsteps, "e0_eV": e0, "drift_eV_per_particle": series[-1],
+    return {"steps": done, "steps_asked": nsteps, "diverged_at": diverged, "limit_eV_per_particle": limit,
+            "e0_eV": e0, "drift_eV_per_particle": series[-1],
             "max_abs_eV_per_particle": max(abs(x) for x in series),
             "every": chunk, "series_eV_per_particle": series, "seconds": round(time.perf_counter() - t0, 1)}
 
@@ -273,7 +296,10 @@ def main():
     ap.add_argument("--delay", type=int, default=10)
     ap.add_argument("--cpu-warmup", type=int, default=2)
     ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--cpu-steps-1t", type=int, default=3, help="timed CPU steps of the single-thread sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-coupling", action="store_true",
+                    help="atoms and elements do not interact (the reference's behaviour)")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the mixed-precision and surrogate lines, accuracy and drift")
     ap.add_argument("--no-surrogate", action="store_true")
@@ -301,6 +327,8 @@ def main():
         sys.exit(2)
 
     from cac_code_amd import _lib
+    global COUPLED
+    COUPLED = not args.no_coupling
     t0 = time.perf_counter()
     s = make_workload(args.workload, args.seed)
     pair, cutneigh, skin, ff = make_pair(args.workload)
@@ -363,16 +391,20 @@ def main():
         nthr = host["affinity"] or 1
         if host["omp_num_threads"]:
             nthr = min(nthr, int(host["omp_num_threads"]))
-        log(f"[bench] cpu baseline (oracle): {args.cpu_warmup}+{args.cpu_steps} steps on 1 and {nthr} threads ...")
-        c1 = cpu_baseline(s, pair, ff, cutneigh, 1, args.cpu_warmup, args.cpu_steps, rate)
+        log(f"[bench] cpu baseline (oracle): {args.cpu_warmup}+{args.cpu_steps} steps on {nthr} threads ...")
         cn = cpu_baseline(s, pair, ff, cutneigh, nthr, args.cpu_warmup, args.cpu_steps, rate)
+        log(f"[bench] cpu: {cn['per_step']:.3f} s/step on {cn['cores']} threads; 1+{args.cpu_steps_1t} steps on 1 "
+            f"thread ...")
+        c1 = cpu_baseline(s, pair, ff, cutneigh, 1, 1, args.cpu_steps_1t, rate)
         cpu = {"value": round(n_int / cn["per_step"] / 1e6, 4), "unit": "Matom-steps/s", "cores": cn["cores"],
                "kind": "port",
                "sample": f"{args.cpu_warmup} warm-up + {cn['steps']} timed full FP64 steps of the same system on "
                          f"the oracle ({cn['t_step']:.3f} s/step) plus one neighbor build ({cn['t_build']:.2f} s) "
                          f"charged at the GPU rebuild rate {rate:.4f}/step",
                "single_thread": {"value": round(n_int / c1["per_step"] / 1e6, 4), "t_step": round(c1["t_step"], 3),
-                                 "t_build": round(c1["t_build"], 2)},
+                                 "t_build": round(c1["t_build"], 2),
+                                 "sample": f"1 warm-up + {c1['steps']} timed steps, one neighbor build"},
+               "coupling": COUPLED,
                "host": host}
         log(f"[bench] cpu: {cn['per_step']:.3f} s/step on {cn['cores']} threads, {c1['per_step']:.3f} s/step on 1")
 
@@ -393,6 +425,7 @@ def main():
                    "nae": list(s.nae), "integration_points": int(s.intp_w.size), "box_cells": s.cells,
                    "represented_atoms": n_rep, "rebuilds": nbuilds, "neighbor": [cutneigh, skin],
                    "neigh_modify": {"every": args.every, "delay": args.delay, "check": "yes"},
+                   "coupling": COUPLED,
                    "seed": args.seed, "parallelism": "single GPU"},
         "represented_Matom_steps_per_s": round(n_rep * args.steps / secs / 1e6, 1),
         "roofline": roof,
@@ -430,7 +463,7 @@ def surrogate_variant(s, pair, cutneigh, skin, args, n_int, secs_fp64):
     secs = time.perf_counter() - t1
     prof = {}
     for name in ("atom_rho", "atom_force", "site_pos", "ip_erho", "site_fp", "ip_force", "scatter", "surr_feat",
-                 "surr_mlp", "surr_fit", "atom_neigh", "ip_neigh", "elem_neigh"):
+                 "surr_mlp", "surr_fit", "atom_neigh", "ip_neigh", "elem_neigh", "atom_sites"):
         ms, cnt = eng.profile_avg_ms(name)
         if cnt:
             prof[name] = round(ms * 1e3, 1)

@@ -15,6 +15,7 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "cac_hip.h")
 
 CAC_OK, CAC_ERR_ARG, CAC_ERR_HIP, CAC_ERR_OVERFLOW, CAC_ERR_BOX, CAC_ERR_DATA = range(6)
 PREC_FP64, PREC_MIXED = 0, 1
+ATOM_SITE = 0xFFFFFFFF  # CAC_ATOM_SITE: site field of a list entry naming an atom
 
 _d = C.POINTER(C.c_double)
 _i = C.POINTER(C.c_int)
@@ -38,6 +39,7 @@ SIGNATURES = {
     "cac_set_eam": (C.c_int, [_ctx, C.c_int, C.c_double, _d, C.c_int, C.c_double, _d, _d, C.c_double]),
     "cac_set_lj": (C.c_int, [_ctx, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double]),
     "cac_set_neighbor": (C.c_int, [_ctx, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "cac_set_coupling": (C.c_int, [_ctx, C.c_int]),
     "cac_build_neighbors": (C.c_int, [_ctx]),
     "cac_decide": (C.c_int, [_ctx, _i]),
     "cac_force": (C.c_int, [_ctx, C.c_int, C.c_int, C.c_int]),
@@ -48,7 +50,7 @@ SIGNATURES = {
     "cac_download_elements": (C.c_int, [_ctx, _d, _d, _d]),
     "cac_get_energy_virial": (C.c_int, [_ctx, _d, _d]),
     "cac_get_capacities": (C.c_int, [_ctx, _i, _i]),
-    "cac_get_atom_neighbors": (C.c_int, [_ctx, C.c_int, _i, _u32]),
+    "cac_get_atom_neighbors": (C.c_int, [_ctx, C.c_int, _i, _u64]),
     "cac_get_ip_neighbors": (C.c_int, [_ctx, C.c_int, _i, _u64]),
     "cac_get_rho": (C.c_int, [_ctx, _d, _d]),
     "cac_get_erho": (C.c_int, [_ctx, _d]),

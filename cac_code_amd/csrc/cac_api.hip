@@ -191,11 +191,9 @@ int force_impl(cac_ctx *ctx, int eflag, int vflag, int precision) {
   int rc;
   if ((rc = alloc_ev(ctx, eflag, vflag))) return rc;
   if (ctx->pair == CAC_PAIR_EAM) {
-    if ((rc = launch_eam_atoms(ctx, eflag, vflag, precision))) return rc;
-    if ((rc = launch_eam_elements(ctx, eflag, vflag, precision))) return rc;
+    if ((rc = launch_eam(ctx, eflag, vflag, precision))) return rc;
   } else {
-    if ((rc = launch_lj_atoms(ctx, eflag, vflag, precision))) return rc;
-    if ((rc = launch_lj_elements(ctx, eflag, vflag, precision))) return rc;
+    if ((rc = launch_lj(ctx, eflag, vflag, precision))) return rc;
   }
   ctx->have_ev = false;
   if (eflag || vflag) return launch_reduce_ev(ctx, eflag, vflag);
@@ -220,8 +218,11 @@ int build_impl(cac_ctx *ctx) {
     if (ctx->hflags[2]) { ctx->maxelist = ctx->hflags[3] + 16; again = true; }
     if (again) continue;
     if ((rc = launch_ip_neighbors(ctx))) return rc;
+    if ((rc = launch_atom_sites(ctx))) return rc;  // coupled: element sites of atoms
     if ((rc = read_flags(ctx))) return rc;
-    if (ctx->hflags[4]) { ctx->maxnip = ctx->hflags[5] + 16; continue; }
+    if (ctx->hflags[0]) { ctx->maxneigh = ctx->hflags[1] + 16; again = true; }
+    if (ctx->hflags[4]) { ctx->maxnip = ctx->hflags[5] + 16; again = true; }
+    if (again) continue;
     if ((rc = launch_active_sites(ctx))) return rc;
     if (ctx->surr.enabled && (rc = surrogate_classify(ctx))) return rc;
     ctx->built = true;
@@ -328,7 +329,7 @@ int cac_ctx_create(int device, cac_ctx **out) {
     delete ctx;
     return CAC_ERR_HIP;
   }
-  if (ensure(ctx, ctx->b_dflags, 8, &ctx->dflags) ||
+  if (ensure(ctx, ctx->b_dflags, 16, &ctx->dflags) ||
       hipHostMalloc((void **)&ctx->hflags, 8 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
     cac_ctx_destroy(ctx);
     return CAC_ERR_HIP;
@@ -350,7 +351,8 @@ int cac_ctx_destroy(cac_ctx *ctx) {
                     &ctx->b_erad, &ctx->b_eneigh, &ctx->b_enumb, &ctx->b_ekey, &ctx->b_eval, &ctx->b_ekey2,
                     &ctx->b_eval2, &ctx->b_ecs, &ctx->b_ece, &ctx->b_ipneigh, &ctx->b_ipnum, &ctx->b_erho,
                     &ctx->b_iptmp, &ctx->b_bitmap, &ctx->b_wprefix, &ctx->b_ecount, &ctx->b_estart, &ctx->b_asite,
-                    &ctx->b_xs4, &ctx->b_scantmp,
+                    &ctx->b_scantmp, &ctx->b_iflag, &ctx->b_ipos, &ctx->b_iface, &ctx->b_asnum, &ctx->b_atmp,
+                    &ctx->b_etouch,
                     &ctx->b_ipf, &ctx->b_eip, &ctx->b_vip, &ctx->b_red, &ctx->b_dflags, &ctx->eam.b_rz,
                     &ctx->eam.b_fr, &ctx->eam.b_rz_f, &ctx->eam.b_fr_f, &ctx->eam.b_rv, &ctx->eam.b_rv_f};
   for (DevBuf *b : bufs)
@@ -566,6 +568,13 @@ int cac_set_neighbor(cac_ctx *ctx, double cutneigh, double triggersq, int every,
   return CAC_OK;
 }
 
+int cac_set_coupling(cac_ctx *ctx, int on) {
+  if (!ctx) return CAC_ERR_ARG;
+  if (ctx->coupled != (on != 0)) ctx->built = false;
+  ctx->coupled = on != 0;
+  return CAC_OK;
+}
+
 int cac_build_neighbors(cac_ctx *ctx) {
   if (!ctx) return CAC_ERR_ARG;
   hipSetDevice(ctx->device);
@@ -673,22 +682,48 @@ int cac_get_capacities(cac_ctx *ctx, int *maxneigh, int *maxnip) {
   return CAC_OK;
 }
 
-int cac_get_atom_neighbors(cac_ctx *ctx, int maxneigh, int *numneigh, uint32_t *neigh) {
+}  // extern "C"
+
+namespace {
+// Blocked device rows of particle entries -> row-major host rows of the
+// ABI's uint64 entries (atom j, or element/site of an active site).
+int download_rows(cac_ctx *ctx, const uint32_t *dev, const int *dnum, size_t rows, int M, int cap, int *numneigh,
+                  uint64_t *neigh) {
+  std::vector<uint32_t> cm((size_t)M * round_up((int)rows, kRowGroup));
+  std::vector<int2> as((size_t)ctx->nactive);
+  CAC_HIP(ctx, hipMemcpy(numneigh, dnum, rows * sizeof(int), hipMemcpyDeviceToHost));
+  CAC_HIP(ctx, hipMemcpy(cm.data(), dev, cm.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  if (ctx->nactive)
+    CAC_HIP(ctx, hipMemcpy(as.data(), ctx->asite, as.size() * sizeof(int2), hipMemcpyDeviceToHost));
+  const uint32_t n = (uint32_t)ctx->n;
+  for (size_t p = 0; p < rows; ++p)
+    for (int k = 0; k < numneigh[p] && k < M; ++k) {
+      const uint32_t e = cm[((p >> 3) * M + k) * kRowGroup + (p & 7)];
+      const uint32_t idx = e & 0x7FFFFFFu;
+      const uint64_t img = (uint64_t)(e >> 27) << 27;
+      if (idx < n) {
+        neigh[p * cap + k] = (uint64_t)idx | img | ((uint64_t)CAC_ATOM_SITE << 32);
+      } else {
+        if (idx - n >= (uint32_t)as.size())
+          return fail(ctx, CAC_ERR_DATA, "list entry " + std::to_string(e) + " beyond the active sites");
+        const int2 js = as[idx - n];
+        neigh[p * cap + k] = (uint64_t)js.x | img | ((uint64_t)js.y << 32);
+      }
+    }
+  return CAC_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int cac_get_atom_neighbors(cac_ctx *ctx, int maxneigh, int *numneigh, uint64_t *neigh) {
   if (!ctx || !numneigh || !neigh) return CAC_ERR_ARG;
   if (!ctx->built) return fail(ctx, CAC_ERR_ARG, "neighbor lists not built");
   if (maxneigh < ctx->maxneigh) return fail(ctx, CAC_ERR_ARG, "capacity below the device list stride");
   hipSetDevice(ctx->device);
-  const int n = ctx->n;
-  if (n == 0) return CAC_OK;
+  if (ctx->n == 0) return CAC_OK;
   CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  const int M = ctx->maxneigh;
-  std::vector<uint32_t> cm((size_t)M * round_up(n, kRowGroup));
-  CAC_HIP(ctx, hipMemcpy(numneigh, ctx->anum, n * sizeof(int), hipMemcpyDeviceToHost));
-  CAC_HIP(ctx, hipMemcpy(cm.data(), ctx->aneigh, cm.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
-  for (int i = 0; i < n; ++i)
-    for (int k = 0; k < numneigh[i] && k < M; ++k)
-      neigh[(size_t)i * maxneigh + k] = cm[((size_t)(i >> 3) * M + k) * kRowGroup + (i & 7)];
-  return CAC_OK;
+  return download_rows(ctx, ctx->aneigh, ctx->anum, (size_t)ctx->n, ctx->maxneigh, maxneigh, numneigh, neigh);
 }
 
 int cac_get_ip_neighbors(cac_ctx *ctx, int maxnip, int *numneigh, uint64_t *neigh) {
@@ -699,20 +734,7 @@ int cac_get_ip_neighbors(cac_ctx *ctx, int maxnip, int *numneigh, uint64_t *neig
   const size_t nip = (size_t)ctx->ne * ctx->nint;
   if (nip == 0) return CAC_OK;
   CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  const int M = ctx->maxnip;
-  std::vector<uint32_t> cm((size_t)M * round_up((int)nip, kRowGroup));
-  std::vector<int2> as((size_t)ctx->nactive);
-  CAC_HIP(ctx, hipMemcpy(numneigh, ctx->ipnum, nip * sizeof(int), hipMemcpyDeviceToHost));
-  CAC_HIP(ctx, hipMemcpy(cm.data(), ctx->ipneigh, cm.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
-  if (ctx->nactive)
-    CAC_HIP(ctx, hipMemcpy(as.data(), ctx->asite, as.size() * sizeof(int2), hipMemcpyDeviceToHost));
-  for (size_t p = 0; p < nip; ++p)
-    for (int k = 0; k < numneigh[p] && k < M; ++k) {
-      const uint32_t e = cm[((p >> 3) * M + k) * kRowGroup + (p & 7)];
-      const int2 js = as[e & 0x7FFFFFFu];
-      neigh[p * maxnip + k] = (uint64_t)js.x | ((uint64_t)(e >> 27) << 27) | ((uint64_t)js.y << 32);
-    }
-  return CAC_OK;
+  return download_rows(ctx, ctx->ipneigh, ctx->ipnum, nip, ctx->maxnip, maxnip, numneigh, neigh);
 }
 
 int cac_get_rho(cac_ctx *ctx, double *rho, double *fp) {

@@ -25,6 +25,8 @@ constexpr int kSubBlock = 4;
 constexpr int kMaxSubBlocks = 512;   // nae <= 32 per edge
 constexpr int kMaxNint = 64;         // integration points per element
 constexpr int kImgSelf = 13;
+// site field of a raw list entry that names an atom (CAC_ATOM_SITE)
+constexpr uint32_t kAtomSite = 0xFFFFFFFFu;
 // Lanes cooperating on one atom / node / integration point in the pair
 // passes, and the row-group size of the blocked neighbor layout: entry k of
 // row r lives at ((r/8)*M + k)*8 + r%8, so a wavefront serving 8 rows with
@@ -103,7 +105,10 @@ struct cac_ctx {
   bool box_set = false;
   double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0}, prd[3] = {0, 0, 0};
 
-  // ---- atoms: x4 = (x, y, z, fp) per atom; v, f, xhold SoA [3][n]
+  // ---- particles: x4 = (x, y, z, fp) rows, the n atoms first, then the
+  // nactive active element sites (xs4 = x4 + 4n).  Every neighbor entry
+  // indexes this array, so a pass gathers atoms and sites alike.
+  // v, f, xhold of the atoms are SoA [3][n].
   int n = 0;
   double mass = 1.0;
   double *x4 = nullptr, *v = nullptr, *f = nullptr, *xhold = nullptr, *rho = nullptr;
@@ -114,9 +119,21 @@ struct cac_ctx {
   uint32_t *aneigh = nullptr;
   int *anum = nullptr;
   cac::DevBuf b_aneigh, b_anum;
-  // atom cells
+  // atom cells (kept from the last build: the coupled integration-point
+  // search reads them)
   int acell[3] = {0, 0, 0};
+  uint32_t *abin = nullptr;
+  int *abin_s = nullptr, *abin_e = nullptr;
   cac::DevBuf b_akey, b_aval, b_akey2, b_aval2, b_acs, b_ace, b_sorttmp;
+  // atom-element coupling (build-defined, off by default = the reference):
+  // atom lists also hold element sites, integration-point lists also atoms.
+  // Interface atoms are those within cutneigh + R_j of some element center.
+  bool coupled = false;
+  int niface = 0;
+  int *iflag = nullptr, *ipos = nullptr, *iface = nullptr, *asnum = nullptr;
+  uint64_t *atmp = nullptr;  // [niface][maxneigh] raw (element | img<<27 | site<<32)
+  int *etouch = nullptr;     // [ne] element has an atom in a list or a site in an atom list
+  cac::DevBuf b_iflag, b_ipos, b_iface, b_asnum, b_atmp, b_etouch;
 
   // ---- elements (one type)
   int ne = 0, nae[3] = {0, 0, 0}, nsite = 0;
@@ -140,11 +157,14 @@ struct cac_ctx {
   uint32_t *eneigh = nullptr;
   int *enumb = nullptr;
   cac::DevBuf b_eneigh, b_enumb;
-  int ecell[3] = {0, 0, 0};
+  int ecell[3] = {0, 0, 0};  // 0: box too small for element cells (brute force)
+  uint32_t *ebin = nullptr;
+  int *ebin_s = nullptr, *ebin_e = nullptr;
   cac::DevBuf b_ekey, b_eval, b_ekey2, b_eval2, b_ecs, b_ece;
   // integration-point neighbor list: raw (j | img<<27 | site<<32) entries
-  // from the search, then compacted to (active-site id | img<<27) in the
-  // blocked layout; maxnip is a multiple of kLanes
+  // from the search (site = kAtomSite for an atom j), then rewritten as
+  // particle entries (row of x4 | img<<27: an atom j, or n + active-site id)
+  // in the blocked layout; maxnip is a multiple of kLanes
   int maxnip = 128;
   uint64_t *iptmp = nullptr;
   uint32_t *ipneigh = nullptr;
@@ -158,8 +178,8 @@ struct cac_ctx {
   int *ecount = nullptr;        // [ne+1] active sites per element
   int *estart = nullptr;        // [ne+1] first active id per element
   int2 *asite = nullptr;        // [nactive] (element, site)
-  double *xs4 = nullptr;        // [nactive][4]
-  cac::DevBuf b_bitmap, b_wprefix, b_ecount, b_estart, b_asite, b_xs4, b_scantmp;
+  double *xs4 = nullptr;        // x4 + 4n: [nactive][4]
+  cac::DevBuf b_bitmap, b_wprefix, b_ecount, b_estart, b_asite, b_scantmp;
   double *erho = nullptr, *ipf = nullptr, *eip = nullptr, *vip = nullptr;  // [ne*8], [3][nip], [nip], [6][nip]
   cac::DevBuf b_erho, b_ipf, b_eip, b_vip;
 
@@ -182,7 +202,8 @@ struct cac_ctx {
   cac::DevBuf b_red;
 
   // ---- flags: [0] overflow atoms, [1] max atom count, [2] overflow elist,
-  // [3] max elist count, [4] overflow ip, [5] max ip count, [6] rebuild flag
+  // [3] max elist count, [4] overflow ip, [5] max ip count, [6] rebuild flag,
+  // [7..10] first out-of-range list entry (set, row, slot, entry)
   int *dflags = nullptr;
   int *hflags = nullptr;  // pinned mirror
   cac::DevBuf b_dflags;
@@ -226,6 +247,33 @@ int ensure(cac_ctx *ctx, DevBuf &b, size_t count, T **out) {
   return CAC_OK;
 }
 
+// Grow b to hold count T, keeping the first keep elements (stream-ordered
+// copy into the new allocation).
+template <class T>
+int ensure_keep(cac_ctx *ctx, DevBuf &b, size_t count, size_t keep, T **out) {
+  size_t need = count * sizeof(T);
+  if (need == 0) need = sizeof(T);
+  if (b.bytes < need) {
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, need);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc");
+    if (b.p && keep) {
+      e = hipMemcpyAsync(p, b.p, keep * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream);
+      if (e != hipSuccess) return hip_fail(ctx, e, "hipMemcpyAsync");
+      e = hipStreamSynchronize(ctx->stream);
+      if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamSynchronize");
+    }
+    if (b.p) {
+      e = hipFree(b.p);
+      if (e != hipSuccess) return hip_fail(ctx, e, "hipFree");
+    }
+    b.p = p;
+    b.bytes = need;
+  }
+  *out = static_cast<T *>(b.p);
+  return CAC_OK;
+}
+
 // timing of kernel families on the context stream
 void timer_begin(cac_ctx *ctx, const char *name, hipEvent_t *start);
 void timer_end(cac_ctx *ctx, const char *name, hipEvent_t start);
@@ -235,15 +283,15 @@ int launch_pbc_and_centers(cac_ctx *ctx);
 int launch_atom_neighbors(cac_ctx *ctx);
 int launch_element_neighbors(cac_ctx *ctx);
 int launch_ip_neighbors(cac_ctx *ctx);
+int launch_atom_sites(cac_ctx *ctx);
 int launch_active_sites(cac_ctx *ctx);
+int validate_lists(cac_ctx *ctx);
 int launch_check_displacement(cac_ctx *ctx);
 
-// ---- launchers (k_pair.hip)
-int launch_eam_atoms(cac_ctx *ctx, int eflag, int vflag, int precision);
-int launch_eam_elements(cac_ctx *ctx, int eflag, int vflag, int precision);
-int launch_lj_atoms(cac_ctx *ctx, int eflag, int vflag, int precision);
-int launch_lj_elements(cac_ctx *ctx, int eflag, int vflag, int precision);
-int launch_scatter(cac_ctx *ctx);
+// ---- launchers (k_pair.hip): all passes of one force call, atoms and
+// elements interleaved in the order the coupled lists need
+int launch_eam(cac_ctx *ctx, int eflag, int vflag, int precision);
+int launch_lj(cac_ctx *ctx, int eflag, int vflag, int precision);
 
 // ---- launchers (cac_api.hip)
 int launch_reduce_ev(cac_ctx *ctx, int eflag, int vflag);

@@ -352,6 +352,149 @@ struct IpGeom {
   int ne, nint, nae[3], nsb[3], nsbt, nwords;
 };
 
+// Raw entry store as two 32-bit halves (lo = j | img << 27, hi = site).
+// Written this way on purpose: hipcc (ROCm 7.2, gfx950) folds
+// `(uint64_t)lo | ((uint64_t)0xFFFFFFFF << 32)` into a store whose high
+// dword is the sign extension of j instead of all ones, which turned atom
+// entries into site entries.
+__device__ __forceinline__ void put_raw(uint64_t *__restrict__ out, int pos, uint32_t lo, uint32_t hi) {
+  reinterpret_cast<uint2 *>(out)[pos] = make_uint2(lo, hi);
+}
+
+// Sites of element image (cj, cimg) within cutneigh of point q: the
+// element's sub-blocks are tested lane-parallel against their radii, then the
+// (up to) 64 sites of each surviving 4x4x4 sub-block are taken one per lane.
+// Accepted sites are compacted in lane order into out[cnt..] (raw entries
+// cj | cimg << 27 | site << 32, capacity cap) and marked in cj's
+// referenced-site bitmap.  `self` applies the rsq > 1e-4 exclusion of the
+// own element (full_node_neighbor.cpp:216).  Wave-uniform; returns the new
+// count.
+__device__ int scan_sites(const IpGeom &ig, const double *__restrict__ nodex, const double *__restrict__ spa_site,
+                          const double *__restrict__ sbc, const Box3 &b, int cj, int cimg, bool self,
+                          const double q[3], double cutneigh, int cnt, int cap, uint64_t *__restrict__ out,
+                          uint32_t *__restrict__ bitmap, int lane) {
+  const double cutsq = cutneigh * cutneigh;
+  int s[3];
+  img_shift(cimg, s);
+  double X[24];
+  for (int k = 0; k < 8; ++k)
+    for (int d = 0; d < 3; ++d) X[3 * k + d] = shifted(nodex[24 * (size_t)cj + 3 * k + d], s[d], b.prd[d]);
+  for (int b0 = 0; b0 < ig.nsbt; b0 += 64) {
+    const int sbi = b0 + lane;
+    bool sk = false;
+    if (sbi < ig.nsbt) {
+      const double *sb = sbc + 4 * ((size_t)cj * ig.nsbt + sbi);
+      const double dx = q[0] - shifted(sb[0], s[0], b.prd[0]);
+      const double dy = q[1] - shifted(sb[1], s[1], b.prd[1]);
+      const double dz = q[2] - shifted(sb[2], s[2], b.prd[2]);
+      const double reach = cutneigh + sb[3];
+      sk = dx * dx + dy * dy + dz * dz <= reach * reach;
+    }
+    unsigned long long sm = __ballot(sk);
+    while (sm) {
+      const int sq = b0 + __ffsll((long long)sm) - 1;
+      sm &= sm - 1;
+      const int ba = sq / (ig.nsb[1] * ig.nsb[2]);
+      const int bb = (sq / ig.nsb[2]) % ig.nsb[1];
+      const int bc = sq % ig.nsb[2];
+      int lo[3], hi[3];
+      sb_range(ba, ig.nae[0], &lo[0], &hi[0]);
+      sb_range(bb, ig.nae[1], &lo[1], &hi[1]);
+      sb_range(bc, ig.nae[2], &lo[2], &hi[2]);
+      const int a = lo[0] + (lane >> 4), bq = lo[1] + ((lane >> 2) & 3), cq = lo[2] + (lane & 3);
+      bool acc = false;
+      int site = 0;
+      if (a <= hi[0] && bq <= hi[1] && cq <= hi[2]) {
+        site = (a * ig.nae[1] + bq) * ig.nae[2] + cq;
+        double xs[3];
+        interp_site(spa_site + 8 * site, X, xs);
+        const double delx = xs[0] - q[0], dely = xs[1] - q[1], delz = xs[2] - q[2];
+        const double rsq = delx * delx + dely * dely + delz * delz;
+        acc = rsq <= cutsq && (!self || rsq > 1.0e-4);
+      }
+      const unsigned long long am = __ballot(acc);
+      if (acc) {
+        const int pos = cnt + __popcll(am & ((1ull << lane) - 1ull));
+        if (pos < cap) put_raw(out, pos, (uint32_t)cj | ((uint32_t)cimg << 27), (uint32_t)site);
+        atomicOr(&bitmap[(size_t)cj * ig.nwords + (site >> 5)], 1u << (site & 31));
+      }
+      cnt += __popcll(am);
+    }
+  }
+  return cnt;
+}
+
+// Atom cells for the coupled searches.
+struct AtomBins {
+  int n;
+  const double *x4;
+  Grid3 g;
+  const uint32_t *bin;
+  const int *bs, *be;
+};
+
+// Atoms within cutneigh of point p (coupled integration-point lists).  p may
+// lie outside the box (element nodes follow the element center): the cells
+// around its wrapped position q = p - t*prd are searched, and an atom found
+// with cell image s is image s + t of p.  Lanes take the atoms of one cell at
+// a time; accepted atoms are compacted in lane order as raw entries
+// j | img << 27 | kAtomSite << 32.  Wave-uniform; returns the new count.
+__device__ int scan_atoms(const AtomBins &ab, const Box3 &b, const double p[3], double cutneigh, int cnt, int cap,
+                          uint64_t *__restrict__ out, int lane) {
+  const double cutsq = cutneigh * cutneigh;
+  int t[3], ci[3];
+  for (int d = 0; d < 3; ++d) {
+    t[d] = p[d] < b.lo[d] ? -1 : (p[d] >= b.hi[d] ? 1 : 0);
+    const double q = p[d] - (double)t[d] * b.prd[d];
+    ci[d] = cell_coord(q, b.lo[d], ab.g.inv[d], ab.g.n[d]);
+  }
+  for (int dz = -1; dz <= 1; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        int cc[3] = {ci[0] + dx, ci[1] + dy, ci[2] + dz}, s[3];
+        bool far = false;
+        for (int d = 0; d < 3; ++d) {
+          s[d] = t[d];
+          if (cc[d] < 0) { cc[d] += ab.g.n[d]; s[d] -= 1; }
+          else if (cc[d] >= ab.g.n[d]) { cc[d] -= ab.g.n[d]; s[d] += 1; }
+          far |= s[d] < -1 || s[d] > 1;
+        }
+        if (far) continue;  // beyond one period: out of reach when prd > 2*(cutneigh + R)
+        const int img = (s[0] + 1) + 3 * (s[1] + 1) + 9 * (s[2] + 1);
+        const int c = (cc[2] * ab.g.n[1] + cc[1]) * ab.g.n[0] + cc[0];
+        const int qs = ab.bs[c], qe = ab.be[c];
+        for (int q0 = qs; q0 < qe; q0 += 64) {
+          const int qq = q0 + lane;
+          bool acc = false;
+          int j = 0;
+          if (qq < qe) {
+            j = (int)ab.bin[qq];
+            const double delx = shifted(ab.x4[4 * (size_t)j], s[0], b.prd[0]) - p[0];
+            const double dely = shifted(ab.x4[4 * (size_t)j + 1], s[1], b.prd[1]) - p[1];
+            const double delz = shifted(ab.x4[4 * (size_t)j + 2], s[2], b.prd[2]) - p[2];
+            acc = delx * delx + dely * dely + delz * delz <= cutsq;
+          }
+          const unsigned long long am = __ballot(acc);
+          if (acc) {
+            const int pos = cnt + __popcll(am & ((1ull << lane) - 1ull));
+            if (pos < cap) put_raw(out, pos, (uint32_t)j | ((uint32_t)img << 27), kAtomSite);
+          }
+          cnt += __popcll(am);
+        }
+      }
+  return cnt;
+}
+
+// Integration-point lists (Neighbor::node_neighbor,
+// full_node_neighbor.cpp:19-239): for integration point g of element i, every
+// site of every candidate element image with rsq <= cutneighsq, then the
+// sites of i itself with rsq > 1e-4 (full_node_neighbor.cpp:216), which drops
+// the integration point's own site; with coupling, then the atoms within
+// cutneigh.
+//
+// One wavefront per integration point.  Lanes test 64 candidate elements at
+// a time against the element radius; each survivor is scanned by scan_sites.
+// The list order is deterministic.
 __global__ __launch_bounds__(256) void k_ip_neigh(IpGeom ig, const double *__restrict__ nodex,
                                                   const double *__restrict__ spa_int,
                                                   const double *__restrict__ spa_site,
@@ -359,19 +502,20 @@ __global__ __launch_bounds__(256) void k_ip_neigh(IpGeom ig, const double *__res
                                                   const uint32_t *__restrict__ eneigh, const int *__restrict__ enumb,
                                                   Box3 b, double cutneigh, int maxnip, uint64_t *__restrict__ tmp,
                                                   int *__restrict__ ipnum, uint32_t *__restrict__ bitmap,
-                                                  int *__restrict__ flags) {
+                                                  int *__restrict__ flags, bool coupled, AtomBins ab,
+                                                  int *__restrict__ etouch) {
   const int nip = ig.ne * ig.nint;
   const int lane = threadIdx.x & 63;
   const int p = (xcd_block() * (int)blockDim.x + (int)threadIdx.x) >> 6;
   if (p >= nip) return;  // whole wavefront
   const int i = p / ig.nint, g = p % ig.nint;
-  const double cutsq = cutneigh * cutneigh;
   double ip[3];
   {
     double Xi[24];
     for (int q = 0; q < 24; ++q) Xi[q] = nodex[24 * (size_t)i + q];
     interp_site(spa_int + 8 * g, Xi, ip);
   }
+  uint64_t *out = tmp + (size_t)p * maxnip;
   int cnt = 0;
   const int nel = enumb[i] + 1;  // candidates, then the element itself
   for (int l0 = 0; l0 < nel; l0 += 64) {
@@ -398,62 +542,165 @@ __global__ __launch_bounds__(256) void k_ip_neigh(IpGeom ig, const double *__res
       cm &= cm - 1;
       const int cj = __shfl(j, src);
       const int cimg = __shfl(img, src);
-      const bool self = (cj == i && cimg == kImgSelf);
-      int s[3];
-      img_shift(cimg, s);
-      double X[24];
-      for (int k = 0; k < 8; ++k)
-        for (int d = 0; d < 3; ++d) X[3 * k + d] = shifted(nodex[24 * (size_t)cj + 3 * k + d], s[d], b.prd[d]);
-      for (int b0 = 0; b0 < ig.nsbt; b0 += 64) {
-        const int sbi = b0 + lane;
-        bool sk = false;
-        if (sbi < ig.nsbt) {
-          const double *sb = sbc + 4 * ((size_t)cj * ig.nsbt + sbi);
-          const double dx = ip[0] - shifted(sb[0], s[0], b.prd[0]);
-          const double dy = ip[1] - shifted(sb[1], s[1], b.prd[1]);
-          const double dz = ip[2] - shifted(sb[2], s[2], b.prd[2]);
-          const double reach = cutneigh + sb[3];
-          sk = dx * dx + dy * dy + dz * dz <= reach * reach;
-        }
-        unsigned long long sm = __ballot(sk);
-        while (sm) {
-          const int sq = b0 + __ffsll((long long)sm) - 1;
-          sm &= sm - 1;
-          const int ba = sq / (ig.nsb[1] * ig.nsb[2]);
-          const int bb = (sq / ig.nsb[2]) % ig.nsb[1];
-          const int bc = sq % ig.nsb[2];
-          int lo[3], hi[3];
-          sb_range(ba, ig.nae[0], &lo[0], &hi[0]);
-          sb_range(bb, ig.nae[1], &lo[1], &hi[1]);
-          sb_range(bc, ig.nae[2], &lo[2], &hi[2]);
-          const int a = lo[0] + (lane >> 4), bq = lo[1] + ((lane >> 2) & 3), cq = lo[2] + (lane & 3);
-          bool acc = false;
-          int site = 0;
-          if (a <= hi[0] && bq <= hi[1] && cq <= hi[2]) {
-            site = (a * ig.nae[1] + bq) * ig.nae[2] + cq;
-            double xs[3];
-            interp_site(spa_site + 8 * site, X, xs);
-            const double delx = xs[0] - ip[0], dely = xs[1] - ip[1], delz = xs[2] - ip[2];
-            const double rsq = delx * delx + dely * dely + delz * delz;
-            acc = rsq <= cutsq && (!self || rsq > 1.0e-4);
-          }
-          const unsigned long long am = __ballot(acc);
-          if (acc) {
-            const int pos = cnt + __popcll(am & ((1ull << lane) - 1ull));
-            if (pos < maxnip)
-              tmp[(size_t)p * maxnip + pos] = (uint64_t)cj | ((uint64_t)cimg << 27) | ((uint64_t)site << 32);
-            atomicOr(&bitmap[(size_t)cj * ig.nwords + (site >> 5)], 1u << (site & 31));
-          }
-          cnt += __popcll(am);
-        }
-      }
+      cnt = scan_sites(ig, nodex, spa_site, sbc, b, cj, cimg, cj == i && cimg == kImgSelf, ip, cutneigh, cnt, maxnip,
+                       out, bitmap, lane);
     }
+  }
+  if (coupled) {
+    const int c0 = cnt;
+    cnt = scan_atoms(ab, b, ip, cutneigh, cnt, maxnip, out, lane);
+    if (lane == 0 && cnt > c0) etouch[i] = 1;
   }
   if (lane == 0) {
     ipnum[p] = cnt;
     if (cnt > maxnip) {
       atomicOr(&flags[4], 1);
       atomicMax(&flags[5], cnt);
+    }
+  }
+}
+
+// Element images whose center is within cutneigh + R_j of point q, found
+// through the element cells (or, for boxes under 3 element cells per
+// dimension, all elements and images); fn(j, img) is called wave-uniformly
+// for each, in cell order then ascending index.  Lanes test 64 candidates at
+// a time.
+struct ElemBins {
+  int ne;
+  const double *erad;
+  bool grid;
+  Grid3 g;
+  const uint32_t *bin;
+  const int *bs, *be;
+};
+
+template <class Fn>
+__device__ void for_elements_near(const ElemBins &eb, const Box3 &b, const double q[3], double cutneigh, int lane,
+                                  Fn fn) {
+  auto test = [&](int j, int img) {
+    int s[3];
+    img_shift(img, s);
+    const double dx = q[0] - shifted(eb.erad[4 * j], s[0], b.prd[0]);
+    const double dy = q[1] - shifted(eb.erad[4 * j + 1], s[1], b.prd[1]);
+    const double dz = q[2] - shifted(eb.erad[4 * j + 2], s[2], b.prd[2]);
+    const double reach = cutneigh + eb.erad[4 * j + 3];
+    return dx * dx + dy * dy + dz * dz <= reach * reach;
+  };
+  auto run = [&](bool keep, int j, int img) {
+    unsigned long long cm = __ballot(keep);
+    while (cm) {
+      const int src = __ffsll((long long)cm) - 1;
+      cm &= cm - 1;
+      fn(__shfl(j, src), __shfl(img, src));
+    }
+  };
+  if (eb.grid) {
+    const int ci[3] = {cell_coord(q[0], b.lo[0], eb.g.inv[0], eb.g.n[0]),
+                       cell_coord(q[1], b.lo[1], eb.g.inv[1], eb.g.n[1]),
+                       cell_coord(q[2], b.lo[2], eb.g.inv[2], eb.g.n[2])};
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          int cc[3] = {ci[0] + dx, ci[1] + dy, ci[2] + dz}, s[3];
+          for (int d = 0; d < 3; ++d) {
+            s[d] = 0;
+            if (cc[d] < 0) { cc[d] += eb.g.n[d]; s[d] = -1; }
+            else if (cc[d] >= eb.g.n[d]) { cc[d] -= eb.g.n[d]; s[d] = 1; }
+          }
+          const int img = (s[0] + 1) + 3 * (s[1] + 1) + 9 * (s[2] + 1);
+          const int c = (cc[2] * eb.g.n[1] + cc[1]) * eb.g.n[0] + cc[0];
+          const int qs = eb.bs[c], qe = eb.be[c];
+          for (int q0 = qs; q0 < qe; q0 += 64) {
+            const int qq = q0 + lane;
+            const int j = qq < qe ? (int)eb.bin[qq] : 0;
+            run(qq < qe && test(j, img), j, img);
+          }
+        }
+  } else {
+    for (int img = 0; img < 27; ++img)
+      for (int j0 = 0; j0 < eb.ne; j0 += 64) {
+        const int j = j0 + lane;
+        run(j < eb.ne && test(j, img), j, img);
+      }
+  }
+}
+
+// Coupled atom lists, step 1: flag the atoms that some element image can
+// reach (|x - c_j'| <= cutneigh + R_j).  One thread per atom.
+__global__ void k_atom_iface(int n, const double *__restrict__ x4, ElemBins eb, Box3 b, double cutneigh,
+                             int *__restrict__ iflag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double q[3] = {x4[4 * (size_t)i], x4[4 * (size_t)i + 1], x4[4 * (size_t)i + 2]};
+  auto hit = [&](int j, int img) {
+    int s[3];
+    img_shift(img, s);
+    const double dx = q[0] - shifted(eb.erad[4 * j], s[0], b.prd[0]);
+    const double dy = q[1] - shifted(eb.erad[4 * j + 1], s[1], b.prd[1]);
+    const double dz = q[2] - shifted(eb.erad[4 * j + 2], s[2], b.prd[2]);
+    const double reach = cutneigh + eb.erad[4 * j + 3];
+    return dx * dx + dy * dy + dz * dz <= reach * reach;
+  };
+  int found = 0;
+  if (eb.grid) {
+    const int ci[3] = {cell_coord(q[0], b.lo[0], eb.g.inv[0], eb.g.n[0]),
+                       cell_coord(q[1], b.lo[1], eb.g.inv[1], eb.g.n[1]),
+                       cell_coord(q[2], b.lo[2], eb.g.inv[2], eb.g.n[2])};
+    for (int dz = -1; dz <= 1 && !found; ++dz)
+      for (int dy = -1; dy <= 1 && !found; ++dy)
+        for (int dx = -1; dx <= 1 && !found; ++dx) {
+          int cc[3] = {ci[0] + dx, ci[1] + dy, ci[2] + dz}, s[3];
+          for (int d = 0; d < 3; ++d) {
+            s[d] = 0;
+            if (cc[d] < 0) { cc[d] += eb.g.n[d]; s[d] = -1; }
+            else if (cc[d] >= eb.g.n[d]) { cc[d] -= eb.g.n[d]; s[d] = 1; }
+          }
+          const int img = (s[0] + 1) + 3 * (s[1] + 1) + 9 * (s[2] + 1);
+          const int c = (cc[2] * eb.g.n[1] + cc[1]) * eb.g.n[0] + cc[0];
+          for (int qq = eb.bs[c]; qq < eb.be[c] && !found; ++qq) found = hit((int)eb.bin[qq], img);
+        }
+  } else {
+    for (int img = 0; img < 27 && !found; ++img)
+      for (int j = 0; j < eb.ne && !found; ++j) found = hit(j, img);
+  }
+  iflag[i] = found;
+}
+
+__global__ void k_iface_compact(int n, const int *__restrict__ iflag, const int *__restrict__ ipos,
+                                int *__restrict__ iface) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && iflag[i]) iface[ipos[i]] = i;
+}
+
+// Coupled atom lists, step 2: one wavefront per flagged atom collects the
+// element sites within cutneigh (raw entries, after the atom's own atom
+// entries, capacity maxneigh in total).
+__global__ __launch_bounds__(256) void k_atom_sites(int niface, const int *__restrict__ iface,
+                                                    const double *__restrict__ x4, const int *__restrict__ anum,
+                                                    IpGeom ig, const double *__restrict__ nodex,
+                                                    const double *__restrict__ spa_site,
+                                                    const double *__restrict__ sbc, ElemBins eb, Box3 b,
+                                                    double cutneigh, int maxneigh, uint64_t *__restrict__ atmp,
+                                                    int *__restrict__ asnum, uint32_t *__restrict__ bitmap,
+                                                    int *__restrict__ etouch, int *__restrict__ flags) {
+  const int lane = threadIdx.x & 63;
+  const int r = (xcd_block() * (int)blockDim.x + (int)threadIdx.x) >> 6;
+  if (r >= niface) return;  // whole wavefront
+  const int i = iface[r];
+  const double q[3] = {x4[4 * (size_t)i], x4[4 * (size_t)i + 1], x4[4 * (size_t)i + 2]};
+  uint64_t *out = atmp + (size_t)r * maxneigh;
+  int cnt = 0;
+  for_elements_near(eb, b, q, cutneigh, lane, [&](int cj, int cimg) {
+    const int c0 = cnt;
+    cnt = scan_sites(ig, nodex, spa_site, sbc, b, cj, cimg, false, q, cutneigh, cnt, maxneigh, out, bitmap, lane);
+    if (lane == 0 && cnt > c0) etouch[cj] = 1;
+  });
+  if (lane == 0) {
+    asnum[r] = cnt;
+    const int tot = anum[i] + cnt;
+    if (tot > maxneigh) {
+      atomicOr(&flags[0], 1);
+      atomicMax(&flags[1], tot);
     }
   }
 }
@@ -490,20 +737,80 @@ __global__ void k_active_finalize(int ne, int nwords, const uint32_t *__restrict
   }
 }
 
-// Raw (element, image, site) entries -> (active id | image << 27) in the
-// blocked layout.
+// Raw entry -> particle entry (row of x4 | image << 27): an atom j keeps
+// its index, site `site` of element j becomes n + its active id.
+__device__ __forceinline__ uint32_t particle_entry(uint64_t e, int n, int nwords, const uint32_t *__restrict__ bitmap,
+                                                   const int *__restrict__ wprefix) {
+  const int j = (int)(e & 0x7FFFFFFu);
+  const uint32_t img = (uint32_t)((e >> 27) & 31);
+  const uint32_t site = (uint32_t)(e >> 32);
+  if (site == kAtomSite) return (uint32_t)j | (img << 27);
+  const size_t w = (size_t)j * nwords + (site >> 5);
+  const int aid = wprefix[w] + __popc(bitmap[w] & ((1u << (site & 31)) - 1u));
+  return (uint32_t)(n + aid) | (img << 27);
+}
+
+// Integration-point lists: raw entries -> particle entries in the blocked
+// layout.
 __global__ void k_ip_translate(int nip, int maxnip, const uint64_t *__restrict__ tmp, const int *__restrict__ ipnum,
-                               int nwords, const uint32_t *__restrict__ bitmap, const int *__restrict__ wprefix,
-                               uint32_t *__restrict__ out) {
+                               int n, int nwords, const uint32_t *__restrict__ bitmap,
+                               const int *__restrict__ wprefix, uint32_t *__restrict__ out) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)nip * maxnip) return;
   const int p = (int)(t / maxnip), k = (int)(t % maxnip);
   if (k >= ipnum[p]) return;
-  const uint64_t e = tmp[t];
-  const int j = (int)(e & 0x7FFFFFFu), img = (int)((e >> 27) & 31), site = (int)(e >> 32);
-  const size_t w = (size_t)j * nwords + (site >> 5);
-  const int aid = wprefix[w] + __popc(bitmap[w] & ((1u << (site & 31)) - 1u));
-  out[((size_t)(p >> 3) * maxnip + k) * kRowGroup + (p & 7)] = (uint32_t)aid | ((uint32_t)img << 27);
+  out[((size_t)(p >> 3) * maxnip + k) * kRowGroup + (p & 7)] = particle_entry(tmp[t], n, nwords, bitmap, wprefix);
+}
+
+// Coupled atom lists: the raw site entries of flagged atom r go after the
+// atom's atom entries.
+__global__ void k_atom_site_translate(int niface, int maxneigh, const int *__restrict__ iface,
+                                      const uint64_t *__restrict__ atmp, const int *__restrict__ asnum,
+                                      const int *__restrict__ anum, int n, int nwords,
+                                      const uint32_t *__restrict__ bitmap, const int *__restrict__ wprefix,
+                                      uint32_t *__restrict__ aneigh) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)niface * maxneigh) return;
+  const int r = (int)(t / maxneigh), k = (int)(t % maxneigh);
+  if (k >= asnum[r]) return;
+  const int i = iface[r];
+  const int pos = anum[i] + k;
+  aneigh[((size_t)(i >> 3) * maxneigh + pos) * kRowGroup + (i & 7)] = particle_entry(atmp[t], n, nwords, bitmap,
+                                                                                       wprefix);
+}
+
+__global__ void k_atom_site_count(int niface, const int *__restrict__ iface, const int *__restrict__ asnum,
+                                  int *__restrict__ anum) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < niface) anum[iface[r]] += asnum[r];
+}
+
+// Every entry of a blocked list must name a row of x4 (< nrow) with an image
+// code < 27: the pair passes gather with it.  The first bad entry found is
+// reported as (row, slot, entry) in flags[8..10] with flags[7] set.
+__global__ void k_validate_list(int rows, int M, const uint32_t *__restrict__ list, const int *__restrict__ num,
+                                uint32_t nrow, int *__restrict__ flags) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)rows * M) return;
+  const int p = (int)(t / M), k = (int)(t % M);
+  const int c = num[p];
+  if (c < 0 || c > M) {
+    if (k == 0 && atomicCAS(&flags[7], 0, 1) == 0) {
+      flags[8] = p;
+      flags[9] = -1;
+      flags[10] = c;
+    }
+    return;
+  }
+  if (k >= c) return;
+  const uint32_t e = list[((size_t)(p >> 3) * M + k) * kRowGroup + (p & 7)];
+  if ((e & 0x7FFFFFFu) >= nrow || (e >> 27) >= 27u) {
+    if (atomicCAS(&flags[7], 0, 1) == 0) {
+      flags[8] = p;
+      flags[9] = k;
+      flags[10] = (int)e;
+    }
+  }
 }
 
 // Neighbor::check_distance, neighbor.cpp:1176-1217: flag if any atom or node
@@ -577,6 +884,48 @@ int bin_points(cac_ctx *ctx, int np, const double *pts, int stride, const Grid3 
   return CAC_OK;
 }
 
+AtomBins atom_bins(const cac_ctx *ctx) {
+  AtomBins ab;
+  ab.n = ctx->n;
+  ab.x4 = ctx->x4;
+  for (int d = 0; d < 3; ++d) {
+    ab.g.n[d] = ctx->acell[d];
+    ab.g.inv[d] = ctx->acell[d] > 0 ? ctx->acell[d] / ctx->prd[d] : 0.0;
+  }
+  ab.bin = ctx->abin;
+  ab.bs = ctx->abin_s;
+  ab.be = ctx->abin_e;
+  return ab;
+}
+
+ElemBins elem_bins(const cac_ctx *ctx) {
+  ElemBins eb;
+  eb.ne = ctx->ne;
+  eb.erad = ctx->erad;
+  eb.grid = ctx->ecell[0] > 0;
+  for (int d = 0; d < 3; ++d) {
+    eb.g.n[d] = ctx->ecell[d];
+    eb.g.inv[d] = ctx->ecell[d] > 0 ? ctx->ecell[d] / ctx->prd[d] : 0.0;
+  }
+  eb.bin = ctx->ebin;
+  eb.bs = ctx->ebin_s;
+  eb.be = ctx->ebin_e;
+  return eb;
+}
+
+IpGeom ip_geom(const cac_ctx *ctx) {
+  IpGeom ig;
+  ig.ne = ctx->ne;
+  ig.nint = ctx->nint;
+  for (int d = 0; d < 3; ++d) {
+    ig.nae[d] = ctx->nae[d];
+    ig.nsb[d] = ctx->nsb[d];
+  }
+  ig.nsbt = ctx->nsb_total;
+  ig.nwords = ctx->nwords;
+  return ig;
+}
+
 }  // namespace
 
 int launch_pbc_and_centers(cac_ctx *ctx) {
@@ -607,6 +956,9 @@ int launch_atom_neighbors(cac_ctx *ctx) {
   if ((rc = ensure(ctx, ctx->b_aneigh, (size_t)ctx->maxneigh * rows, &ctx->aneigh)) ||
       (rc = ensure(ctx, ctx->b_anum, rows, &ctx->anum)) || (rc = ensure(ctx, ctx->b_xhold, 3 * (size_t)n, &ctx->xhold)))
     return rc;
+  ctx->abin = sorted;
+  ctx->abin_s = cs;
+  ctx->abin_e = ce;
   hipEvent_t t0;
   timer_begin(ctx, "atom_neigh", &t0);
   hipLaunchKernelGGL(k_atom_neigh, dim3(xcd_grid(n, kBlock)), dim3(kBlock), 0, ctx->stream, n, ctx->x4, box_of(ctx),
@@ -658,6 +1010,9 @@ int launch_element_neighbors(cac_ctx *ctx) {
     if ((rc = bin_points(ctx, ne, ctx->erad, 4, g, ctx->b_ekey, ctx->b_eval, ctx->b_ekey2, ctx->b_eval2, ctx->b_ecs,
                          ctx->b_ece, &sorted, &cs, &ce)))
       return rc;
+    ctx->ebin = sorted;
+    ctx->ebin_s = cs;
+    ctx->ebin_e = ce;
     timer_begin(ctx, "elem_neigh", &t0);
     hipLaunchKernelGGL(k_elem_neigh_cells, dim3(grid_for(ne)), dim3(kBlock), 0, ctx->stream, ne, ctx->erad,
                        box_of(ctx), g, sorted, cs, ce, ctx->cutneigh, ctx->maxelist, ctx->eneigh, ctx->enumb,
@@ -665,6 +1020,8 @@ int launch_element_neighbors(cac_ctx *ctx) {
     timer_end(ctx, "elem_neigh", t0);
   } else {
     for (int d = 0; d < 3; ++d) ctx->ecell[d] = 0;
+    ctx->ebin = nullptr;
+    ctx->ebin_s = ctx->ebin_e = nullptr;
     for (int d = 0; d < 3; ++d)
       if (ctx->prd[d] < ctx->cutneigh)
         return fail(ctx, CAC_ERR_BOX, "box narrower than the neighbor cutoff");
@@ -688,10 +1045,13 @@ int launch_ip_neighbors(cac_ctx *ctx) {
   if ((rc = ensure(ctx, ctx->b_iptmp, (size_t)ctx->maxnip * nip, &ctx->iptmp)) ||
       (rc = ensure(ctx, ctx->b_ipnum, (size_t)round_up(nip, kRowGroup), &ctx->ipnum)) ||
       (rc = ensure(ctx, ctx->b_bitmap, nbits, &ctx->bitmap)) ||
-      (rc = ensure(ctx, ctx->b_nodexhold, 24 * (size_t)ne, &ctx->nodexhold)))
+      (rc = ensure(ctx, ctx->b_nodexhold, 24 * (size_t)ne, &ctx->nodexhold)) ||
+      (rc = ensure(ctx, ctx->b_etouch, (size_t)ne, &ctx->etouch)))
     return rc;
   hipLaunchKernelGGL(k_fill_int, dim3(grid_for((long)nbits)), dim3(kBlock), 0, ctx->stream, (int)nbits,
                      reinterpret_cast<int *>(ctx->bitmap), 0);
+  hipLaunchKernelGGL(k_fill_int, dim3(grid_for(ne)), dim3(kBlock), 0, ctx->stream, ne, ctx->etouch, 0);
+  const bool coupled = ctx->coupled && ctx->n > 0;
   IpGeom ig;
   ig.ne = ne;
   ig.nint = ctx->nint;
@@ -705,7 +1065,8 @@ int launch_ip_neighbors(cac_ctx *ctx) {
   timer_begin(ctx, "ip_neigh", &t0);
   hipLaunchKernelGGL(k_ip_neigh, dim3(xcd_grid(64L * nip, kBlock)), dim3(kBlock), 0, ctx->stream, ig, ctx->nodex,
                      ctx->spa_int, ctx->spa_site, ctx->erad, ctx->sbc, ctx->eneigh, ctx->enumb, box_of(ctx),
-                     ctx->cutneigh, ctx->maxnip, ctx->iptmp, ctx->ipnum, ctx->bitmap, ctx->dflags);
+                     ctx->cutneigh, ctx->maxnip, ctx->iptmp, ctx->ipnum, ctx->bitmap, ctx->dflags, coupled,
+                     atom_bins(ctx), ctx->etouch);
   timer_end(ctx, "ip_neigh", t0);
   CAC_HIP(ctx, hipMemcpyAsync(ctx->nodexhold, ctx->nodex, sizeof(double) * 24 * (size_t)ne,
                               hipMemcpyDeviceToDevice, ctx->stream));
@@ -713,8 +1074,54 @@ int launch_ip_neighbors(cac_ctx *ctx) {
   return CAC_OK;
 }
 
+// Coupled atom lists: flag the atoms some element can reach, compact them,
+// and collect their element sites (raw entries in atmp).  Needs the element
+// geometry and cells of this build and the site bitmap cleared by
+// launch_ip_neighbors.
+int launch_atom_sites(cac_ctx *ctx) {
+  const int n = ctx->n, ne = ctx->ne;
+  ctx->niface = 0;
+  if (!ctx->coupled || n == 0 || ne == 0) return CAC_OK;
+  int rc;
+  if ((rc = ensure(ctx, ctx->b_iflag, (size_t)n + 1, &ctx->iflag)) ||
+      (rc = ensure(ctx, ctx->b_ipos, (size_t)n + 1, &ctx->ipos)))
+    return rc;
+  hipEvent_t t0;
+  timer_begin(ctx, "atom_sites", &t0);
+  hipLaunchKernelGGL(k_atom_iface, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, n, ctx->x4, elem_bins(ctx),
+                     box_of(ctx), ctx->cutneigh, ctx->iflag);
+  CAC_HIP(ctx, hipMemsetAsync(ctx->iflag + n, 0, sizeof(int), ctx->stream));
+  size_t tmp_bytes = 0;
+  CAC_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, ctx->iflag, ctx->ipos, n + 1, ctx->stream));
+  char *tmp;
+  if ((rc = ensure(ctx, ctx->b_scantmp, tmp_bytes, &tmp))) return rc;
+  CAC_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, ctx->iflag, ctx->ipos, n + 1, ctx->stream));
+  int niface = 0;
+  CAC_HIP(ctx, hipMemcpyAsync(&niface, ctx->ipos + n, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->niface = niface;
+  if (niface == 0) {
+    timer_end(ctx, "atom_sites", t0);
+    return CAC_OK;
+  }
+  if ((rc = ensure(ctx, ctx->b_iface, (size_t)niface, &ctx->iface)) ||
+      (rc = ensure(ctx, ctx->b_asnum, (size_t)niface, &ctx->asnum)) ||
+      (rc = ensure(ctx, ctx->b_atmp, (size_t)niface * ctx->maxneigh, &ctx->atmp)))
+    return rc;
+  hipLaunchKernelGGL(k_iface_compact, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, n, ctx->iflag, ctx->ipos,
+                     ctx->iface);
+  hipLaunchKernelGGL(k_atom_sites, dim3(xcd_grid(64L * niface, kBlock)), dim3(kBlock), 0, ctx->stream, niface,
+                     ctx->iface, ctx->x4, ctx->anum, ip_geom(ctx), ctx->nodex, ctx->spa_site, ctx->sbc,
+                     elem_bins(ctx), box_of(ctx), ctx->cutneigh, ctx->maxneigh, ctx->atmp, ctx->asnum, ctx->bitmap,
+                     ctx->etouch, ctx->dflags);
+  timer_end(ctx, "atom_sites", t0);
+  CAC_HIP(ctx, hipGetLastError());
+  return CAC_OK;
+}
+
 // After a successful search: number the referenced sites, build the
-// active-site table and rewrite the lists as (active id | image << 27).
+// active-site table, place the site rows in x4 after the atoms and rewrite
+// the lists as particle entries.
 int launch_active_sites(cac_ctx *ctx) {
   const int ne = ctx->ne;
   if (ne == 0) return CAC_OK;
@@ -739,18 +1146,59 @@ int launch_active_sites(cac_ctx *ctx) {
   CAC_HIP(ctx, hipMemcpyAsync(&total, estart + ne, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
   ctx->nactive = total;
+  const int n = ctx->n;
+  if ((long)n + total >= (1L << 27)) return fail(ctx, CAC_ERR_OVERFLOW, "more than 2^27-1 atoms plus active sites");
   if ((rc = ensure(ctx, ctx->b_asite, (size_t)total, &ctx->asite)) ||
-      (rc = ensure(ctx, ctx->b_xs4, 4 * (size_t)total, &ctx->xs4)) ||
+      (rc = ensure_keep(ctx, ctx->b_x4, 4 * ((size_t)n + total), 4 * (size_t)n, &ctx->x4)) ||
       (rc = ensure(ctx, ctx->b_ipneigh, (size_t)ctx->maxnip * round_up(nip, kRowGroup), &ctx->ipneigh)))
     return rc;
+  ctx->xs4 = ctx->x4 + 4 * (size_t)n;
   hipLaunchKernelGGL(k_active_finalize, dim3(grid_for(ne)), dim3(kBlock), 0, ctx->stream, ne, nw, ctx->bitmap,
                      estart, ctx->wprefix, ctx->asite);
   const long tot = (long)nip * ctx->maxnip;
   hipLaunchKernelGGL(k_ip_translate, dim3(grid_for(tot)), dim3(kBlock), 0, ctx->stream, nip, ctx->maxnip, ctx->iptmp,
-                     ctx->ipnum, nw, ctx->bitmap, ctx->wprefix, ctx->ipneigh);
+                     ctx->ipnum, n, nw, ctx->bitmap, ctx->wprefix, ctx->ipneigh);
+  if (ctx->niface > 0) {
+    const long ta = (long)ctx->niface * ctx->maxneigh;
+    hipLaunchKernelGGL(k_atom_site_translate, dim3(grid_for(ta)), dim3(kBlock), 0, ctx->stream, ctx->niface,
+                       ctx->maxneigh, ctx->iface, ctx->atmp, ctx->asnum, ctx->anum, n, nw, ctx->bitmap, ctx->wprefix,
+                       ctx->aneigh);
+    hipLaunchKernelGGL(k_atom_site_count, dim3(grid_for(ctx->niface)), dim3(kBlock), 0, ctx->stream, ctx->niface,
+                       ctx->iface, ctx->asnum, ctx->anum);
+  }
   CAC_HIP(ctx, hipGetLastError());
+  return validate_lists(ctx);
+}
+
+// Check both lists after a build (rows of x4: n atoms + nactive sites).
+int validate_lists(cac_ctx *ctx) {
+  const uint32_t nrow = (uint32_t)(ctx->n + ctx->nactive);
+  CAC_HIP(ctx, hipMemsetAsync(ctx->dflags + 7, 0, 4 * sizeof(int), ctx->stream));
+  const int nip = ctx->ne * ctx->nint;
+  if (ctx->n)
+    hipLaunchKernelGGL(k_validate_list, dim3(grid_for((long)ctx->n * ctx->maxneigh)), dim3(kBlock), 0, ctx->stream,
+                       ctx->n, ctx->maxneigh, ctx->aneigh, ctx->anum, nrow, ctx->dflags);
+  int bad[4];
+  CAC_HIP(ctx, hipMemcpyAsync(bad, ctx->dflags + 7, sizeof(bad), hipMemcpyDeviceToHost, ctx->stream));
+  CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (bad[0])
+    return fail(ctx, CAC_ERR_DATA, "atom list entry out of range: row " + std::to_string(bad[1]) + " slot " +
+                                       std::to_string(bad[2]) + " value " + std::to_string((unsigned)bad[3]) +
+                                       " (n " + std::to_string(ctx->n) + ", active sites " +
+                                       std::to_string(ctx->nactive) + ")");
+  if (nip)
+    hipLaunchKernelGGL(k_validate_list, dim3(grid_for((long)nip * ctx->maxnip)), dim3(kBlock), 0, ctx->stream, nip,
+                       ctx->maxnip, ctx->ipneigh, ctx->ipnum, nrow, ctx->dflags);
+  CAC_HIP(ctx, hipMemcpyAsync(bad, ctx->dflags + 7, sizeof(bad), hipMemcpyDeviceToHost, ctx->stream));
+  CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (bad[0])
+    return fail(ctx, CAC_ERR_DATA, "integration-point list entry out of range: row " + std::to_string(bad[1]) +
+                                       " slot " + std::to_string(bad[2]) + " value " +
+                                       std::to_string((unsigned)bad[3]) + " (n " + std::to_string(ctx->n) +
+                                       ", active sites " + std::to_string(ctx->nactive) + ")");
   return CAC_OK;
 }
+
 
 int launch_check_displacement(cac_ctx *ctx) {
   long tot = (long)ctx->n + 8L * ctx->ne;

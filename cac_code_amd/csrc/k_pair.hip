@@ -15,11 +15,12 @@
 // combined with a fixed xor butterfly (every lane ends with the same bits),
 // so results are reproducible run to run.
 //
-// Element path: the sites that some integration-point list refers to (the
-// "active sites", numbered at rebuild) are interpolated once per step
-// (k_site_pos: x = sum_k N_k X_k; k_site_fp: rho = sum_k N_k erho_k and
-// F'(rho)), so the erho and force passes gather 32 bytes per entry, like the
-// atom passes.  The image of a neighbor site is x_site + s*prd.
+// Element path: the sites that some list refers to (the "active sites",
+// numbered at rebuild) are interpolated once per step (k_site_pos:
+// x = sum_k N_k X_k; k_site_fp: rho = sum_k N_k erho_k and F'(rho)) into the
+// rows of x4 after the atoms, so every pass gathers 32-byte (x, y, z, fp)
+// rows of one particle array whether the neighbor is an atom or a site (the
+// coupled lists mix them).  The image of a neighbor is x_j + s*prd.
 //
 // Tables on the device: per r-knot, the density pass reads the 4 value
 // coefficients of rhor (32 bytes, 16 in FP32) and the force pass the 4 value
@@ -440,7 +441,7 @@ __global__ __launch_bounds__(kBlock) void k_site_fp(ElemArgs a, Tabs<P> t, const
 // Node density, pair_eam.cpp:927-986: erho[i][k] from the list of the
 // integration point at node k, measured from the node itself.
 template <class P>
-__global__ __launch_bounds__(kBlock) void k_eam_erho(ElemArgs a, Prd prd, Tabs<P> t, const double *__restrict__ xs4,
+__global__ __launch_bounds__(kBlock) void k_eam_erho(ElemArgs a, Prd prd, Tabs<P> t, const double *__restrict__ px4,
                                                      double *__restrict__ erho) {
   const int gt = xcd_block() * kBlock + (int)threadIdx.x;
   const int o = gt >> 3, l = gt & 7;
@@ -460,7 +461,7 @@ __global__ __launch_bounds__(kBlock) void k_eam_erho(ElemArgs a, Prd prd, Tabs<P
       Pt xj[kU];
 #pragma unroll
       for (int u = 0; u < kU; ++u)
-        if (e[u] != kNoEntry) xj[u] = load_xyz(xs4, e[u], prd);
+        if (e[u] != kNoEntry) xj[u] = load_xyz(px4, e[u], prd);
 #pragma unroll
       for (int u = 0; u < kU; ++u) {
         if (e[u] == kNoEntry) continue;
@@ -479,7 +480,7 @@ __global__ __launch_bounds__(kBlock) void k_eam_erho(ElemArgs a, Prd prd, Tabs<P
 template <class P, bool EV>
 __global__ __launch_bounds__(kBlock) void k_eam_ip_force(ElemArgs a, Prd prd, Tabs<P> t,
                                                          const double *__restrict__ erho,
-                                                         const double *__restrict__ xs4, double *__restrict__ ipf,
+                                                         const double *__restrict__ px4, double *__restrict__ ipf,
                                                          double *__restrict__ eip, double *__restrict__ vip) {
   const int gt = xcd_block() * kBlock + (int)threadIdx.x;
   const int o = gt >> 3, l = gt & 7;
@@ -509,7 +510,7 @@ __global__ __launch_bounds__(kBlock) void k_eam_ip_force(ElemArgs a, Prd prd, Ta
       double fpjv[kU];
 #pragma unroll
       for (int u = 0; u < kU; ++u)
-        if (e[u] != kNoEntry) xjv[u] = load_xyzw(xs4, e[u], prd, &fpjv[u]);
+        if (e[u] != kNoEntry) xjv[u] = load_xyzw(px4, e[u], prd, &fpjv[u]);
 #pragma unroll
       for (int u = 0; u < kU; ++u) {
       if (e[u] == kNoEntry) continue;
@@ -560,7 +561,7 @@ __global__ __launch_bounds__(kBlock) void k_eam_ip_force(ElemArgs a, Prd prd, Ta
 // LJ integration-point force, pair_lj_cut.cpp:355-422.
 template <class P, bool EV>
 __global__ __launch_bounds__(kBlock) void k_lj_ip_force(ElemArgs a, Prd prd, LjParams lj,
-                                                        const double *__restrict__ xs4, double *__restrict__ ipf,
+                                                        const double *__restrict__ px4, double *__restrict__ ipf,
                                                         double *__restrict__ eip, double *__restrict__ vip) {
   const int gt = xcd_block() * kBlock + (int)threadIdx.x;
   const int o = gt >> 3, l = gt & 7;
@@ -583,7 +584,7 @@ __global__ __launch_bounds__(kBlock) void k_lj_ip_force(ElemArgs a, Prd prd, LjP
     const uint32_t *row = list_row(a.ipneigh, p, a.M);
     for (int kk = l; kk < jn; kk += kLanes) {
       const uint32_t e = row[(size_t)kk * kRowGroup];
-      const Pt xj = load_xyz(xs4, e, prd);
+      const Pt xj = load_xyz(px4, e, prd);
       const double delx = ix - xj.x, dely = iy - xj.y, delz = iz - xj.z;
       const double rsq = delx * delx + dely * dely + delz * delz;
       if (rsq < lj.cutsq) {
@@ -703,35 +704,8 @@ ElemArgs elem_args(const cac_ctx *ctx) {
   return a;
 }
 
-template <class P, bool EV>
-int eam_atoms_t(cac_ctx *ctx) {
-  const int n = ctx->n;
-  Tabs<P> t = tabs_of<P>(ctx);
-  const unsigned grid = xcd_grid((long)n * kLanes, kBlock);
-  hipEvent_t t0;
-  timer_begin(ctx, "atom_rho", &t0);
-  hipLaunchKernelGGL((k_eam_atom_rho<P, EV>), dim3(grid), dim3(kBlock), 0, ctx->stream, n, ctx->x4, prd_of(ctx),
-                     ctx->aneigh, ctx->maxneigh, ctx->anum, t, ctx->rho, ctx->eatom);
-  timer_end(ctx, "atom_rho", t0);
-  timer_begin(ctx, "atom_force", &t0);
-  hipLaunchKernelGGL((k_eam_atom_force<P, EV>), dim3(grid), dim3(kBlock), 0, ctx->stream, n, ctx->x4, prd_of(ctx),
-                     ctx->aneigh, ctx->maxneigh, ctx->anum, t, ctx->f, ctx->eatom, ctx->vatom);
-  timer_end(ctx, "atom_force", t0);
-  CAC_HIP(ctx, hipGetLastError());
-  return CAC_OK;
-}
-
 long host_nsite(const ElemArgs &a) { return a.slist ? a.nslist : a.nactive; }
 long host_nelem(const ElemArgs &a) { return a.elist ? a.nelist : a.ne; }
-
-int site_positions(cac_ctx *ctx, const ElemArgs &a) {
-  if (host_nsite(a) == 0) return CAC_OK;
-  hipEvent_t t0;
-  timer_begin(ctx, "site_pos", &t0);
-  hipLaunchKernelGGL(k_site_pos, dim3(xcd_grid(host_nsite(a), kBlock)), dim3(kBlock), 0, ctx->stream, a, ctx->xs4);
-  timer_end(ctx, "site_pos", t0);
-  return CAC_OK;
-}
 
 ElemArgs with_elems(ElemArgs a, const int *el, int n) {
   a.elist = el;
@@ -744,114 +718,129 @@ ElemArgs with_sites(ElemArgs a, const int *sl, int n) {
   return a;
 }
 
-// Element EAM passes over optional subsets: site positions over `spos`,
-// node densities over `erho_el`, site F' over `sfp`, forces and scatter over
-// `force_el` (all elements when a subset is null).
-template <class P, bool EV>
-int eam_elements_sub(cac_ctx *ctx, const ElemArgs &base, const ElemArgs &spos, const ElemArgs &erho_el,
-                     const ElemArgs &sfp, const ElemArgs &force_el) {
-  Tabs<P> t = tabs_of<P>(ctx);
-  site_positions(ctx, spos);
+int site_positions(cac_ctx *ctx, const ElemArgs &a) {
+  if (host_nsite(a) == 0) return CAC_OK;
   hipEvent_t t0;
-  timer_begin(ctx, "ip_erho", &t0);
-  if (host_nelem(erho_el))
+  timer_begin(ctx, "site_pos", &t0);
+  hipLaunchKernelGGL(k_site_pos, dim3(xcd_grid(host_nsite(a), kBlock)), dim3(kBlock), 0, ctx->stream, a, ctx->xs4);
+  timer_end(ctx, "site_pos", t0);
+  return CAC_OK;
+}
+
+int scatter(cac_ctx *ctx, const ElemArgs &a) {
+  const long nf = host_nelem(a);
+  if (nf == 0) return CAC_OK;
+  hipEvent_t t0;
+  timer_begin(ctx, "scatter", &t0);
+  hipLaunchKernelGGL(k_scatter, dim3(grid_for(8L * nf)), dim3(kBlock), 0, ctx->stream, a, ctx->ipf, ctx->nodef);
+  timer_end(ctx, "scatter", t0);
+  return CAC_OK;
+}
+
+// One EAM force call (PairEAM::compute + ecompute, pair_eam.cpp:584-746,
+// 927-1137) in the order the coupled lists need: site positions, atom
+// density + F', node densities, site F', atom forces, integration-point
+// forces + scatter.  Without coupling the atom and element halves are
+// independent and the order changes nothing.  The element passes run over
+// optional subsets (surrogate mode): site positions over `spos`, node
+// densities over `erho_el`, site F' over `sfp`, forces and scatter over
+// `force_el`.
+template <class P, bool EV>
+int eam_all(cac_ctx *ctx, const ElemArgs &base, const ElemArgs &spos, const ElemArgs &erho_el, const ElemArgs &sfp,
+            const ElemArgs &force_el) {
+  Tabs<P> t = tabs_of<P>(ctx);
+  const int n = ctx->n;
+  const bool el = ctx->ne > 0;
+  const unsigned agrid = xcd_grid((long)n * kLanes, kBlock);
+  hipEvent_t t0;
+  if (el) site_positions(ctx, spos);
+  if (n) {
+    timer_begin(ctx, "atom_rho", &t0);
+    hipLaunchKernelGGL((k_eam_atom_rho<P, EV>), dim3(agrid), dim3(kBlock), 0, ctx->stream, n, ctx->x4, prd_of(ctx),
+                       ctx->aneigh, ctx->maxneigh, ctx->anum, t, ctx->rho, ctx->eatom);
+    timer_end(ctx, "atom_rho", t0);
+  }
+  if (el && host_nelem(erho_el)) {
+    timer_begin(ctx, "ip_erho", &t0);
     hipLaunchKernelGGL((k_eam_erho<P>), dim3(xcd_grid(8L * host_nelem(erho_el) * kLanes, kBlock)), dim3(kBlock), 0,
-                       ctx->stream, erho_el, prd_of(ctx), t, ctx->xs4, ctx->erho);
-  timer_end(ctx, "ip_erho", t0);
-  if (host_nsite(sfp)) {
+                       ctx->stream, erho_el, prd_of(ctx), t, ctx->x4, ctx->erho);
+    timer_end(ctx, "ip_erho", t0);
+  }
+  if (el && host_nsite(sfp)) {
     timer_begin(ctx, "site_fp", &t0);
     hipLaunchKernelGGL((k_site_fp<P>), dim3(xcd_grid(host_nsite(sfp), kBlock)), dim3(kBlock), 0, ctx->stream, sfp, t,
                        ctx->erho, ctx->xs4);
     timer_end(ctx, "site_fp", t0);
   }
-  const long nf = host_nelem(force_el);
+  if (n) {
+    timer_begin(ctx, "atom_force", &t0);
+    hipLaunchKernelGGL((k_eam_atom_force<P, EV>), dim3(agrid), dim3(kBlock), 0, ctx->stream, n, ctx->x4, prd_of(ctx),
+                       ctx->aneigh, ctx->maxneigh, ctx->anum, t, ctx->f, ctx->eatom, ctx->vatom);
+    timer_end(ctx, "atom_force", t0);
+  }
+  const long nf = el ? host_nelem(force_el) : 0;
   if (nf) {
     timer_begin(ctx, "ip_force", &t0);
     hipLaunchKernelGGL((k_eam_ip_force<P, EV>), dim3(xcd_grid(nf * base.nint * kLanes, kBlock)), dim3(kBlock), 0,
-                       ctx->stream, force_el, prd_of(ctx), t, ctx->erho, ctx->xs4, ctx->ipf, ctx->eip, ctx->vip);
+                       ctx->stream, force_el, prd_of(ctx), t, ctx->erho, ctx->x4, ctx->ipf, ctx->eip, ctx->vip);
     timer_end(ctx, "ip_force", t0);
-    timer_begin(ctx, "scatter", &t0);
-    hipLaunchKernelGGL(k_scatter, dim3(grid_for(8L * nf)), dim3(kBlock), 0, ctx->stream, force_el, ctx->ipf,
-                       ctx->nodef);
-    timer_end(ctx, "scatter", t0);
+    scatter(ctx, force_el);
   }
   CAC_HIP(ctx, hipGetLastError());
   return CAC_OK;
 }
 
 template <class P, bool EV>
-int eam_elements_t(cac_ctx *ctx) {
+int eam_t(cac_ctx *ctx) {
   ElemArgs a = elem_args(ctx);
   const SurrState &S = ctx->surr;
-  if (S.enabled && S.ready && !EV) {
-    int rc = eam_elements_sub<P, false>(ctx, a, with_sites(a, S.siteR2, S.nsR2), with_elems(a, S.elemR, S.nR),
-                                        with_sites(a, S.siteR, S.nsR), with_elems(a, S.elemF, S.nF));
+  if (ctx->ne && S.enabled && S.ready && !EV) {
+    int rc = eam_all<P, false>(ctx, a, with_sites(a, S.siteR2, S.nsR2), with_elems(a, S.elemR, S.nR),
+                               with_sites(a, S.siteR, S.nsR), with_elems(a, S.elemF, S.nF));
     if (rc) return rc;
     return surrogate_forces(ctx);
   }
-  return eam_elements_sub<P, EV>(ctx, a, a, a, a, a);
+  return eam_all<P, EV>(ctx, a, a, a, a, a);
 }
 
+// One LJ force call (PairLJCut::compute + ecompute, pair_lj_cut.cpp:227-288,
+// 295-442): site positions, atom forces, integration-point forces, scatter.
 template <class P, bool EV>
-int lj_atoms_t(cac_ctx *ctx) {
+int lj_t(cac_ctx *ctx) {
+  ElemArgs a = elem_args(ctx);
   const int n = ctx->n;
   hipEvent_t t0;
-  timer_begin(ctx, "atom_force", &t0);
-  hipLaunchKernelGGL((k_lj_atom_force<P, EV>), dim3(xcd_grid((long)n * kLanes, kBlock)), dim3(kBlock), 0,
-                     ctx->stream, n, ctx->x4, prd_of(ctx), ctx->aneigh, ctx->maxneigh, ctx->anum, ctx->lj, ctx->f,
-                     ctx->eatom, ctx->vatom);
-  timer_end(ctx, "atom_force", t0);
+  if (ctx->ne) site_positions(ctx, a);
+  if (n) {
+    timer_begin(ctx, "atom_force", &t0);
+    hipLaunchKernelGGL((k_lj_atom_force<P, EV>), dim3(xcd_grid((long)n * kLanes, kBlock)), dim3(kBlock), 0,
+                       ctx->stream, n, ctx->x4, prd_of(ctx), ctx->aneigh, ctx->maxneigh, ctx->anum, ctx->lj, ctx->f,
+                       ctx->eatom, ctx->vatom);
+    timer_end(ctx, "atom_force", t0);
+  }
+  if (ctx->ne) {
+    timer_begin(ctx, "ip_force", &t0);
+    hipLaunchKernelGGL((k_lj_ip_force<P, EV>), dim3(xcd_grid((long)a.nip * kLanes, kBlock)), dim3(kBlock), 0,
+                       ctx->stream, a, prd_of(ctx), ctx->lj, ctx->x4, ctx->ipf, ctx->eip, ctx->vip);
+    timer_end(ctx, "ip_force", t0);
+    scatter(ctx, a);
+  }
   CAC_HIP(ctx, hipGetLastError());
   return CAC_OK;
-}
-
-template <class P, bool EV>
-int lj_elements_t(cac_ctx *ctx) {
-  ElemArgs a = elem_args(ctx);
-  site_positions(ctx, a);
-  hipEvent_t t0;
-  timer_begin(ctx, "ip_force", &t0);
-  hipLaunchKernelGGL((k_lj_ip_force<P, EV>), dim3(xcd_grid((long)a.nip * kLanes, kBlock)), dim3(kBlock), 0,
-                     ctx->stream, a, prd_of(ctx), ctx->lj, ctx->xs4, ctx->ipf, ctx->eip, ctx->vip);
-  timer_end(ctx, "ip_force", t0);
-  CAC_HIP(ctx, hipGetLastError());
-  return launch_scatter(ctx);
 }
 
 }  // namespace
-
-int launch_scatter(cac_ctx *ctx) {
-  ElemArgs a = elem_args(ctx);
-  hipEvent_t t0;
-  timer_begin(ctx, "scatter", &t0);
-  hipLaunchKernelGGL(k_scatter, dim3(grid_for(8L * a.ne)), dim3(kBlock), 0, ctx->stream, a, ctx->ipf, ctx->nodef);
-  timer_end(ctx, "scatter", t0);
-  CAC_HIP(ctx, hipGetLastError());
-  return CAC_OK;
-}
 
 #define CAC_DISPATCH(fn, ctx, ev, precision)                                      \
   ((precision) == CAC_PREC_MIXED ? ((ev) ? fn<float, true>(ctx) : fn<float, false>(ctx)) \
                                  : ((ev) ? fn<double, true>(ctx) : fn<double, false>(ctx)))
 
-int launch_eam_atoms(cac_ctx *ctx, int eflag, int vflag, int precision) {
-  if (ctx->n == 0) return CAC_OK;
-  return CAC_DISPATCH(eam_atoms_t, ctx, eflag || vflag, precision);
+int launch_eam(cac_ctx *ctx, int eflag, int vflag, int precision) {
+  return CAC_DISPATCH(eam_t, ctx, eflag || vflag, precision);
 }
 
-int launch_eam_elements(cac_ctx *ctx, int eflag, int vflag, int precision) {
-  if (ctx->ne == 0) return CAC_OK;
-  return CAC_DISPATCH(eam_elements_t, ctx, eflag || vflag, precision);
-}
-
-int launch_lj_atoms(cac_ctx *ctx, int eflag, int vflag, int precision) {
-  if (ctx->n == 0) return CAC_OK;
-  return CAC_DISPATCH(lj_atoms_t, ctx, eflag || vflag, precision);
-}
-
-int launch_lj_elements(cac_ctx *ctx, int eflag, int vflag, int precision) {
-  if (ctx->ne == 0) return CAC_OK;
-  return CAC_DISPATCH(lj_elements_t, ctx, eflag || vflag, precision);
+int launch_lj(cac_ctx *ctx, int eflag, int vflag, int precision) {
+  return CAC_DISPATCH(lj_t, ctx, eflag || vflag, precision);
 }
 
 }  // namespace cac

@@ -281,14 +281,16 @@ __global__ void __launch_bounds__(512) k_surr_predict(int ns, const int *__restr
   out[2] = ((double)a2 + (double)yr[2]) * (double)out_scale[3 * k + 2];
 }
 
-// fallback class: 1 interface, 2 strained, 0 surrogate
+// fallback class: 1 interface (a mesh neighbor is not an element, or, with
+// atom-element coupling, an atom is in one of the element's lists or one of
+// its sites in an atom's list), 2 strained, 0 surrogate
 __global__ void k_surr_classify(int ne, const double *__restrict__ nodex, const double *__restrict__ ref,
-                                const int *__restrict__ rcls, const int *__restrict__ nbr, double strain_limit,
-                                int *__restrict__ cls) {
+                                const int *__restrict__ rcls, const int *__restrict__ nbr,
+                                const int *__restrict__ touch, double strain_limit, int *__restrict__ cls) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= ne) return;
   const double *r0 = ref + kRefLen * (size_t)rcls[e];
-  int c = 0;
+  int c = (touch && touch[e]) ? 1 : 0;
   for (int m = 0; m < 26; ++m)
     if (nbr[26 * (size_t)e + m] < 0) c = 1;
   if (!c) {
@@ -423,7 +425,7 @@ int surrogate_classify(cac_ctx *ctx) {
   int *cls, *mark;
   if ((rc = ensure(ctx, S.b_cls, (size_t)ne, &cls)) || (rc = ensure(ctx, S.b_mark, (size_t)ne, &mark))) return rc;
   hipLaunchKernelGGL(k_surr_classify, dim3(grid_for(ne)), dim3(kBlock), 0, ctx->stream, ne, ctx->nodex, S.ref, S.rcls,
-                     S.nbr, S.strain_limit, cls);
+                     S.nbr, ctx->coupled ? ctx->etouch : nullptr, S.strain_limit, cls);
   std::vector<int> hcls(ne);
   CAC_HIP(ctx, hipMemcpyAsync(hcls.data(), cls, ne * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -675,7 +677,7 @@ int cac_surrogate_features(cac_ctx *ctx, float *feats, int *cls) {
   if ((rc = frames(ctx, &fa))) return rc;
   hipLaunchKernelGGL(k_surr_features_all, dim3(grid_for(ne, kBlock / 64)), dim3(kBlock), 0, ctx->stream, ne, fa, f);
   hipLaunchKernelGGL(k_surr_classify, dim3(grid_for(ne)), dim3(kBlock), 0, ctx->stream, ne, ctx->nodex, S.ref, S.rcls,
-                     S.nbr, S.strain_limit, c);
+                     S.nbr, (ctx->coupled && ctx->built) ? ctx->etouch : nullptr, S.strain_limit, c);
   CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (feats) CAC_HIP(ctx, hipMemcpy(feats, f, kNFeat * (size_t)ne * sizeof(float), hipMemcpyDeviceToHost));
   if (cls) CAC_HIP(ctx, hipMemcpy(cls, c, (size_t)ne * sizeof(int), hipMemcpyDeviceToHost));

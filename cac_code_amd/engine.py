@@ -324,6 +324,12 @@ class Engine(_MlpMixin):
         _check(self.ctx, self.L.cac_set_neighbor(self.ctx, cutneigh, triggersq, every, delay, 1 if check else 0,
                                                  maxneigh, maxnip))
 
+    def set_coupling(self, on: bool = True):
+        """Atom-element coupling (build-defined, off by default = the
+        reference): atom lists also hold element sites, integration-point
+        lists also atoms.  Takes effect at the next build."""
+        _check(self.ctx, self.L.cac_set_coupling(self.ctx, 1 if on else 0))
+
     # ---- actions
     def build(self):
         _check(self.ctx, self.L.cac_build_neighbors(self.ctx))
@@ -373,10 +379,12 @@ class Engine(_MlpMixin):
         return a.value, b.value
 
     def atom_neighbors(self):
+        """Atom lists as uint64 entries j | img << 27 | site << 32 (site ==
+        ATOM_SITE for an atom j, else site `site` of element j)."""
         maxn, _ = self.capacities()
         num = np.zeros(self.n, dtype=np.int32)
-        nb = np.zeros((max(self.n, 1), maxn), dtype=np.uint32)
-        _check(self.ctx, self.L.cac_get_atom_neighbors(self.ctx, maxn, _p(num, _lib._i), _p(nb, _lib._u32)))
+        nb = np.zeros((max(self.n, 1), maxn), dtype=np.uint64)
+        _check(self.ctx, self.L.cac_get_atom_neighbors(self.ctx, maxn, _p(num, _lib._i), _p(nb, _lib._u64)))
         return num, nb[:self.n]
 
     def ip_neighbors(self):

@@ -12,11 +12,13 @@
  * Each declaration names the reference interface it replaces.  Reference
  * paths are relative to the fangyuanxiao/CAC_Code tree.
  *
- * Neighbor entry encodings (cac_get_atom_neighbors / cac_get_ip_neighbors):
- *   atom entry (uint32): j | (img << 27)
- *   IP entry   (uint64): j | (img << 27) | ((uint64)site << 32)
+ * Neighbor entry encoding (cac_get_atom_neighbors / cac_get_ip_neighbors),
+ * uint64: j | (img << 27) | ((uint64)site << 32), where site ==
+ * CAC_ATOM_SITE names atom j and any other value site `site` of element j.
  * img = (sx+1) + 3*(sy+1) + 9*(sz+1), s in {-1,0,1}^3 the periodic image of
- * j (position x_j + s*prd); img 13 is the unshifted copy.
+ * j (position x_j + s*prd); img 13 is the unshifted copy.  Without coupling
+ * (the reference) atom lists hold only atoms and integration-point lists
+ * only element sites.
  */
 #ifndef CAC_HIP_H
 #define CAC_HIP_H
@@ -39,6 +41,7 @@ enum {
 };
 
 enum { CAC_PREC_FP64 = 0, CAC_PREC_MIXED = 1 };
+#define CAC_ATOM_SITE 0xFFFFFFFFu
 enum { CAC_PAIR_NONE = 0, CAC_PAIR_EAM = 1, CAC_PAIR_LJ = 2 };
 
 /* ---- context ------------------------------------------------------------
@@ -101,6 +104,15 @@ int cac_set_lj(cac_ctx *ctx, double lj1, double lj2, double lj3, double lj4, dou
 int cac_set_neighbor(cac_ctx *ctx, double cutneigh, double triggersq, int every, int delay,
                      int check, int maxneigh, int maxnip);
 
+/* Atom-element coupling (no counterpart in the reference, where atoms see
+ * only atoms, pair_eam.cpp:584-746, and integration points only element
+ * sites, pair_eam.cpp:927-1137).  With on != 0 the next build adds to every
+ * atom list the interpolated element sites within cutneigh and to every
+ * integration-point list the atoms within cutneigh; the pair terms stay the
+ * reference's (atom pass formula for atoms, element pass formula for
+ * integration points).  Off by default. */
+int cac_set_coupling(cac_ctx *ctx, int on);
+
 /* Neighbor rebuild: Domain::pbc/elepbc + Neighbor::build/ebuild/nbuild
  * (verlet.cpp:173-193; neighbor.cpp:546-625, element_neighbor.cpp:11-75,
  * full_node_neighbor.cpp:19-239).  Regrows capacities on overflow. */
@@ -140,11 +152,11 @@ int cac_download_elements(cac_ctx *ctx, double *nodex, double *nodev, double *no
 int cac_get_energy_virial(cac_ctx *ctx, double eng[2], double virial[6]);
 
 /* Parity hooks: neighbor lists (row-major [n][maxneigh] / [ne*nint][maxnip],
- * device order), per-atom rho/fp, per-node erho ([ne][8]), per-integration-
- * point force ([ne*nint][3]).  Return CAC_ERR_ARG if the capacity given is
- * below the device list length (see cac_get_capacities). */
+ * device order, entries as above), per-atom rho/fp, per-node erho ([ne][8]),
+ * per-integration-point force ([ne*nint][3]).  Return CAC_ERR_ARG if the
+ * capacity given is below the device list length (see cac_get_capacities). */
 int cac_get_capacities(cac_ctx *ctx, int *maxneigh, int *maxnip);
-int cac_get_atom_neighbors(cac_ctx *ctx, int maxneigh, int *numneigh, uint32_t *neigh);
+int cac_get_atom_neighbors(cac_ctx *ctx, int maxneigh, int *numneigh, uint64_t *neigh);
 int cac_get_ip_neighbors(cac_ctx *ctx, int maxnip, int *numneigh, uint64_t *neigh);
 int cac_get_rho(cac_ctx *ctx, double *rho, double *fp);
 int cac_get_erho(cac_ctx *ctx, double *erho);

@@ -1024,6 +1024,454 @@ void orc_lj_elements(const orc_elems *el, const orc_box *box,
 }
 
 /* ------------------------------------------------------------------------
+ * Atom-element coupling (not in the reference, where atoms see only atoms
+ * and integration points only element sites: pair_eam.cpp:584-746 vs
+ * 927-1137).  Build-defined extension in the same force-based
+ * scheme: an atom's list also holds the interpolated element sites within
+ * cutneigh, an integration point's list also holds the atoms within
+ * cutneigh, and every pair term is the reference's own (atom pass formula
+ * for atoms, element pass formula for integration points).  With nae = 2
+ * and the 8 corners as integration points of weight 1 the coupled system is
+ * molecular dynamics of all sites (the invariant stated at
+ * pair_eam.cpp:996), which is how tests/test_oracle_coupled.py pins it.
+ *
+ * Entry encoding (uint64): (j | img << 27) | ((uint64)site << 32); site ==
+ * ORC_ATOM_SITE marks an atom j, otherwise j is an element and site one of
+ * its nae^3 sites.
+ * ---------------------------------------------------------------------- */
+static inline void entry_pos(const double *x, const orc_elems *el, uint64_t e, const double prd[3], double out[3]) {
+  int j = (int)(e & 0x7FFFFFF), img = (int)((e >> 27) & 31), s[3];
+  uint32_t site = (uint32_t)(e >> 32);
+  if (site == ORC_ATOM_SITE) {
+    img_shift(img, s);
+    for (int d = 0; d < 3; d++) out[d] = shifted(x[3 * j + d], s[d], prd[d]);
+  } else {
+    double X[8][3];
+    shifted_nodes(el->nodex, j, img, prd, X);
+    interp3(el->spa_site + 8 * site, X, out);
+  }
+}
+
+/* F'(rho) of an entry: the atom's fp, or F' of the interpolated site
+ * density sum_k N_k erho_k (pair_eam.cpp:1077, 1089-1096). */
+static inline double entry_fp(const orc_elems *el, const orc_eam *t, const double *fp, const double *erho,
+                              uint64_t e) {
+  int j = (int)(e & 0x7FFFFFF);
+  uint32_t site = (uint32_t)(e >> 32);
+  if (site == ORC_ATOM_SITE) return fp[j];
+  const double *Ns = el->spa_site + 8 * site;
+  double r = 0.0;
+  for (int kk = 0; kk < 8; kk++) r += Ns[kk] * erho[8 * j + kk];
+  return embed_fp(t, r);
+}
+
+int orc_coupled_neighbors(int n, const double *x, const orc_elems *el, const orc_box *box, double cutneigh,
+                          int nsplit, int brute, int maxneigh, int *anum, uint64_t *anb, int maxnip, int *inum,
+                          uint64_t *inb) {
+  double prd[3] = {box->hi[0] - box->lo[0], box->hi[1] - box->lo[1], box->hi[2] - box->lo[2]};
+  double cutsq = cutneigh * cutneigh;
+  int ne = el->ne, nint = el->nint, nip = ne * nint, rc = 0;
+  /* atom-atom and site-site parts: the reference lists */
+  uint32_t *tmp = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1) * maxneigh);
+  if (n > 0) {
+    rc = brute ? orc_atom_neighbors_brute(n, x, box, cutneigh, maxneigh, anum, tmp)
+               : orc_atom_neighbors(n, x, box, cutneigh, maxneigh, anum, tmp);
+    if (rc) { free(tmp); return rc; }
+  }
+  for (int i = 0; i < n; i++)
+    for (int k = 0; k < anum[i]; k++)
+      anb[(size_t)i * maxneigh + k] = (uint64_t)tmp[(size_t)i * maxneigh + k] | ((uint64_t)ORC_ATOM_SITE << 32);
+  free(tmp);
+  if (nip > 0) {
+    rc = brute ? orc_ip_neighbors_brute(el, box, cutneigh, maxnip, inum, inb)
+               : orc_ip_neighbors(el, box, cutneigh, nsplit, maxnip, inum, inb);
+    if (rc) return rc;
+  }
+  if (n == 0 || ne == 0) return 0;
+
+  /* element geometry: centers and corner radii (as orc_ip_neighbors) */
+  subele_t sb;
+  if (subele_build(&sb, el->nae, nsplit)) { subele_free(&sb); return -2; }
+  double *xc = (double *)malloc(sizeof(double) * 3 * ne);
+  double *R = (double *)malloc(sizeof(double) * ne);
+  orc_element_centers(ne, el->nodex, xc);
+  double Rmax = 0.0;
+  for (int i = 0; i < ne; i++) {
+    double X[8][3];
+    shifted_nodes(el->nodex, i, ORC_IMG_SELF, prd, X);
+    R[i] = node_radius(X, xc + 3 * i);
+    Rmax = MAXD(Rmax, R[i]);
+  }
+  cells_t ec, ac;
+  cells_build(&ec, ne, xc, box, cutneigh + 2.0 * Rmax);
+  cells_build(&ac, n, x, box, cutneigh);
+  int overflow = 0;
+
+  /* sites for every atom: element images whose center is within
+   * cutneigh + R_j, then their sub-elements and sites (scan_element) */
+#pragma omp parallel for schedule(dynamic, 64) reduction(| : overflow)
+  for (int i = 0; i < n; i++) {
+    uint64_t *out = anb + (size_t)i * maxneigh;
+    int cnt = anum[i];
+    const double *xi = x + 3 * i;
+    if (brute) {
+      for (int img = 0; img < 27; img++)
+        for (int j = 0; j < ne; j++) {
+          double X[8][3];
+          shifted_nodes(el->nodex, j, img, prd, X);
+          for (int site = 0; site < el->nsite; site++) {
+            double sx[3];
+            interp3(el->spa_site + 8 * site, X, sx);
+            double delx = sx[0] - xi[0], dely = sx[1] - xi[1], delz = sx[2] - xi[2];
+            double rsq = delx * delx + dely * dely + delz * delz;
+            if (rsq <= cutsq) {
+              if (cnt < maxneigh) out[cnt] = (uint64_t)j | ((uint64_t)img << 27) | ((uint64_t)site << 32);
+              cnt++;
+            }
+          }
+        }
+    } else {
+#define SCAN_ATOM(j, img)                                                                             \
+  do {                                                                                                \
+    int s_[3];                                                                                        \
+    img_shift(img, s_);                                                                               \
+    double cj[3] = {shifted(xc[3 * (j)], s_[0], prd[0]), shifted(xc[3 * (j) + 1], s_[1], prd[1]),    \
+                    shifted(xc[3 * (j) + 2], s_[2], prd[2])};                                         \
+    cnt = scan_element(el, &sb, (j), (img), 0, xi, prd, cutneigh, R[(j)], cj, cnt, maxneigh, out);    \
+  } while (0)
+      if (ec.ok) {
+        int ci[3];
+        cell_of(&ec, xi, box, ci);
+        for (int dz = -1; dz <= 1; dz++)
+          for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+              int cc[3] = {ci[0] + dx, ci[1] + dy, ci[2] + dz}, s[3];
+              for (int d = 0; d < 3; d++) {
+                s[d] = 0;
+                if (cc[d] < 0) { cc[d] += ec.n[d]; s[d] = -1; }
+                else if (cc[d] >= ec.n[d]) { cc[d] -= ec.n[d]; s[d] = 1; }
+              }
+              int img = (s[0] + 1) + 3 * (s[1] + 1) + 9 * (s[2] + 1);
+              for (int j = ec.head[(cc[2] * ec.n[1] + cc[1]) * ec.n[0] + cc[0]]; j >= 0; j = ec.next[j])
+                SCAN_ATOM(j, img);
+            }
+      } else {
+        for (int img = 0; img < 27; img++)
+          for (int j = 0; j < ne; j++) SCAN_ATOM(j, img);
+      }
+#undef SCAN_ATOM
+    }
+    if (cnt > maxneigh) overflow = 1;
+    anum[i] = cnt;
+    if (cnt <= maxneigh) qsort(out, cnt, sizeof(uint64_t), cmp_u64);
+  }
+
+  /* atoms for every integration point */
+#pragma omp parallel for schedule(dynamic, 16) reduction(| : overflow)
+  for (int p = 0; p < nip; p++) {
+    uint64_t *out = inb + (size_t)p * maxnip;
+    int cnt = inum[p];
+    double ip[3], q[3];
+    int t[3];
+    ip_position(el, p / nint, p % nint, ip);
+    /* an integration point can lie outside the box (element nodes follow
+     * the element center, domain.cpp:647-745): search the cells around its
+     * wrapped position q = ip - t*prd; an atom found with cell image s is
+     * the image s + t of the integration point */
+    for (int d = 0; d < 3; d++) {
+      t[d] = ip[d] < box->lo[d] ? -1 : (ip[d] >= box->hi[d] ? 1 : 0);
+      q[d] = ip[d] - (double)t[d] * prd[d];
+    }
+#define TEST_ATOM(j, img)                                                                          \
+  do {                                                                                             \
+    int s_[3];                                                                                     \
+    img_shift(img, s_);                                                                            \
+    double delx = shifted(x[3 * (j)], s_[0], prd[0]) - ip[0];                                      \
+    double dely = shifted(x[3 * (j) + 1], s_[1], prd[1]) - ip[1];                                  \
+    double delz = shifted(x[3 * (j) + 2], s_[2], prd[2]) - ip[2];                                  \
+    double rsq = delx * delx + dely * dely + delz * delz;                                          \
+    if (rsq <= cutsq) {                                                                            \
+      if (cnt < maxnip) out[cnt] = (uint64_t)(j) | ((uint64_t)(img) << 27) | ((uint64_t)ORC_ATOM_SITE << 32); \
+      cnt++;                                                                                       \
+    }                                                                                              \
+  } while (0)
+    if (ac.ok && !brute) {
+      int ci[3];
+      cell_of(&ac, q, box, ci);
+      for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+          for (int dx = -1; dx <= 1; dx++) {
+            int cc[3] = {ci[0] + dx, ci[1] + dy, ci[2] + dz}, s[3], far = 0;
+            for (int d = 0; d < 3; d++) {
+              s[d] = t[d];
+              if (cc[d] < 0) { cc[d] += ac.n[d]; s[d] -= 1; }
+              else if (cc[d] >= ac.n[d]) { cc[d] -= ac.n[d]; s[d] += 1; }
+              far |= s[d] < -1 || s[d] > 1;
+            }
+            if (far) continue; /* beyond one period: out of reach when prd > 2*(cutneigh + R) */
+            int img = (s[0] + 1) + 3 * (s[1] + 1) + 9 * (s[2] + 1);
+            for (int j = ac.head[(cc[2] * ac.n[1] + cc[1]) * ac.n[0] + cc[0]]; j >= 0; j = ac.next[j])
+              TEST_ATOM(j, img);
+          }
+    } else {
+      for (int img = 0; img < 27; img++)
+        for (int j = 0; j < n; j++) TEST_ATOM(j, img);
+    }
+#undef TEST_ATOM
+    if (cnt > maxnip) overflow = 1;
+    inum[p] = cnt;
+    if (cnt <= maxnip) qsort(out, cnt, sizeof(uint64_t), cmp_u64);
+  }
+  cells_free(&ec);
+  cells_free(&ac);
+  free(xc);
+  free(R);
+  subele_free(&sb);
+  return overflow ? -1 : 0;
+}
+
+/* EAM over atoms and elements together (the passes of pair_eam.cpp:584-746
+ * and 927-1137 on the coupled lists):
+ *   1. erho[e][k]: the list of the integration point at node k, from the
+ *      node coordinate (sites interpolated, atoms as they are);
+ *   2. rho_i, fp_i of atoms over their lists (atoms and sites);
+ *   3. atom force: psip = fp_i*rho' + fp_j*rho' + phi', fp_j of a site
+ *      F'(sum_k N_k erho_k);
+ *   4. integration-point force and node scatter as orc_eam_elements, fp of an
+ *      atom neighbor its own fp.
+ * Energy / virial tallies as in orc_eam_atoms / orc_eam_elements. */
+void orc_eam_coupled(int n, const double *x, const orc_elems *el, const orc_box *box, const int *anum,
+                     const uint64_t *anb, int maxneigh, const int *inum, const uint64_t *inb, int maxnip,
+                     const orc_eam *t, double *rho, double *fp, double *f, double *eatom, double *vatom,
+                     double *erho, double *nodef, double *ipf, double *eip, double *vip) {
+  int ne = el->ne, nint = el->nint;
+  double prd[3] = {box->hi[0] - box->lo[0], box->hi[1] - box->lo[1], box->hi[2] - box->lo[2]};
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < ne; i++)
+    for (int k = 0; k < 8; k++) {
+      const double *nk = el->nodex + 24 * i + 3 * k;
+      size_t nt = (size_t)i * nint + el->node_list[k];
+      double acc = 0.0;
+      for (int ll = 0; ll < inum[nt]; ll++) {
+        double s[3];
+        entry_pos(x, el, inb[nt * maxnip + ll], prd, s);
+        double delx = nk[0] - s[0], dely = nk[1] - s[1], delz = nk[2] - s[2];
+        double rsq = delx * delx + dely * dely + delz * delz;
+        if (rsq < t->cutforcesq) {
+          double p;
+          const double *c = rtab(t->rhor_spline, sqrt(rsq), t->rdr, t->nr, &p);
+          acc += ((c[3] * p + c[4]) * p + c[5]) * p + c[6];
+        }
+      }
+      erho[8 * i + k] = acc;
+    }
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; i++) {
+    const double *xi = x + 3 * i;
+    double r_i = 0.0;
+    for (int jj = 0; jj < anum[i]; jj++) {
+      double s[3];
+      entry_pos(x, el, anb[(size_t)i * maxneigh + jj], prd, s);
+      double delx = xi[0] - s[0], dely = xi[1] - s[1], delz = xi[2] - s[2];
+      double rsq = delx * delx + dely * dely + delz * delz;
+      if (rsq < t->cutforcesq) {
+        double p;
+        const double *c = rtab(t->rhor_spline, sqrt(rsq), t->rdr, t->nr, &p);
+        r_i += ((c[3] * p + c[4]) * p + c[5]) * p + c[6];
+      }
+    }
+    rho[i] = r_i;
+    fp[i] = embed_fp(t, r_i);
+  }
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; i++) {
+    const double *xi = x + 3 * i;
+    double fx = 0.0, fy = 0.0, fz = 0.0, epair = 0.0;
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    for (int jj = 0; jj < anum[i]; jj++) {
+      uint64_t e = anb[(size_t)i * maxneigh + jj];
+      double s[3];
+      entry_pos(x, el, e, prd, s);
+      double delx = xi[0] - s[0], dely = xi[1] - s[1], delz = xi[2] - s[2];
+      double rsq = delx * delx + dely * dely + delz * delz;
+      if (rsq < t->cutforcesq) {
+        double fpj = entry_fp(el, t, fp, erho, e);
+        double r = sqrt(rsq), p;
+        const double *c = rtab(t->rhor_spline, r, t->rdr, t->nr, &p);
+        double rhoip = (c[0] * p + c[1]) * p + c[2];
+        const double *z = t->z2r_spline + (c - t->rhor_spline);
+        double z2p = (z[0] * p + z[1]) * p + z[2];
+        double z2 = ((z[3] * p + z[4]) * p + z[5]) * p + z[6];
+        double recip = 1.0 / r;
+        double phi = z2 * recip;
+        double phip = z2p * recip - phi * recip;
+        double psip = fp[i] * rhoip + fpj * rhoip + phip;
+        double fpair = -psip * recip;
+        fx += delx * fpair;
+        fy += dely * fpair;
+        fz += delz * fpair;
+        epair += 0.5 * phi;
+        v[0] += 0.5 * delx * delx * fpair;
+        v[1] += 0.5 * dely * dely * fpair;
+        v[2] += 0.5 * delz * delz * fpair;
+        v[3] += 0.5 * delx * dely * fpair;
+        v[4] += 0.5 * delx * delz * fpair;
+        v[5] += 0.5 * dely * delz * fpair;
+      }
+    }
+    f[3 * i] = fx;
+    f[3 * i + 1] = fy;
+    f[3 * i + 2] = fz;
+    if (eatom) eatom[i] = embed_energy(t, rho[i], fp[i]) + epair;
+    if (vatom)
+      for (int a = 0; a < 6; a++) vatom[6 * i + a] = v[a];
+  }
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < ne; i++) {
+    double nf[8][3];
+    for (int k = 0; k < 8; k++) nf[k][0] = nf[k][1] = nf[k][2] = 0.0;
+    for (int g = 0; g < nint; g++) {
+      const double *Ng = el->spa_int + 8 * g;
+      double ixtmp = 0, iytmp = 0, iztmp = 0, rhoi = 0;
+      for (int kk = 0; kk < 8; kk++) {
+        ixtmp += Ng[kk] * el->nodex[24 * i + 3 * kk];
+        iytmp += Ng[kk] * el->nodex[24 * i + 3 * kk + 1];
+        iztmp += Ng[kk] * el->nodex[24 * i + 3 * kk + 2];
+        rhoi += Ng[kk] * erho[8 * i + kk];
+      }
+      double fpi = embed_fp(t, rhoi);
+      double fx = 0, fy = 0, fz = 0, epair = 0;
+      double v[6] = {0, 0, 0, 0, 0, 0};
+      size_t p = (size_t)i * nint + g;
+      for (int ll = 0; ll < inum[p]; ll++) {
+        uint64_t e = inb[p * maxnip + ll];
+        double s[3];
+        entry_pos(x, el, e, prd, s);
+        double delx = ixtmp - s[0], dely = iytmp - s[1], delz = iztmp - s[2];
+        double rsq = delx * delx + dely * dely + delz * delz;
+        if (rsq < t->cutforcesq) {
+          double fpj = entry_fp(el, t, fp, erho, e);
+          double r = sqrt(rsq), pp;
+          const double *c = rtab(t->rhor_spline, r, t->rdr, t->nr, &pp);
+          double rhoip = (c[0] * pp + c[1]) * pp + c[2];
+          const double *z = t->z2r_spline + (c - t->rhor_spline);
+          double z2p = (z[0] * pp + z[1]) * pp + z[2];
+          double z2 = ((z[3] * pp + z[4]) * pp + z[5]) * pp + z[6];
+          double recip = 1.0 / r;
+          double phi = z2 * recip;
+          double phip = z2p * recip - phi * recip;
+          double psip = (fpi + fpj) * rhoip + phip;
+          double fpair = -psip * recip;
+          fx += delx * fpair;
+          fy += dely * fpair;
+          fz += delz * fpair;
+          epair += 0.5 * phi;
+          v[0] += 0.5 * delx * delx * fpair;
+          v[1] += 0.5 * dely * dely * fpair;
+          v[2] += 0.5 * delz * delz * fpair;
+          v[3] += 0.5 * delx * dely * fpair;
+          v[4] += 0.5 * delx * delz * fpair;
+          v[5] += 0.5 * dely * delz * fpair;
+        }
+      }
+      double w = el->weight[g];
+      for (int kk = 0; kk < 8; kk++) {
+        nf[kk][0] += w * Ng[kk] * fx;
+        nf[kk][1] += w * Ng[kk] * fy;
+        nf[kk][2] += w * Ng[kk] * fz;
+      }
+      if (ipf) { ipf[3 * p] = fx; ipf[3 * p + 1] = fy; ipf[3 * p + 2] = fz; }
+      if (eip) eip[p] = w * (embed_energy(t, rhoi, fpi) + epair);
+      if (vip)
+        for (int a = 0; a < 6; a++) vip[6 * p + a] = w * v[a];
+    }
+    for (int k = 0; k < 8; k++)
+      for (int d = 0; d < 3; d++) nodef[24 * i + 3 * k + d] = nf[k][d];
+  }
+}
+
+/* LJ over the coupled lists: PairLJCut::compute / ::ecompute pair terms
+ * (pair_lj_cut.cpp:255-287, 376-428) with every neighbor position from
+ * entry_pos. */
+void orc_lj_coupled(int n, const double *x, const orc_elems *el, const orc_box *box, const int *anum,
+                    const uint64_t *anb, int maxneigh, const int *inum, const uint64_t *inb, int maxnip,
+                    const orc_lj *pp, double *f, double *eatom, double *vatom, double *nodef, double *ipf,
+                    double *eip, double *vip) {
+  int ne = el->ne, nint = el->nint;
+  double prd[3] = {box->hi[0] - box->lo[0], box->hi[1] - box->lo[1], box->hi[2] - box->lo[2]};
+#pragma omp parallel for schedule(static)
+  for (int q = 0; q < n + ne * nint; q++) {
+    int is_atom = q < n;
+    double c[3];
+    const int *num = is_atom ? anum : inum;
+    const uint64_t *nb = is_atom ? anb : inb;
+    int row = is_atom ? q : q - n, cap = is_atom ? maxneigh : maxnip;
+    if (is_atom) {
+      for (int d = 0; d < 3; d++) c[d] = x[3 * q + d];
+    } else {
+      ip_position(el, row / nint, row % nint, c);
+    }
+    double fx = 0, fy = 0, fz = 0, e = 0;
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    for (int jj = 0; jj < num[row]; jj++) {
+      double s[3];
+      entry_pos(x, el, nb[(size_t)row * cap + jj], prd, s);
+      double delx = c[0] - s[0], dely = c[1] - s[1], delz = c[2] - s[2];
+      double rsq = delx * delx + dely * dely + delz * delz;
+      if (rsq < pp->cutsq) {
+        double r2inv = 1.0 / rsq;
+        double r6inv = r2inv * r2inv * r2inv;
+        double forcelj = r6inv * (pp->lj1 * r6inv - pp->lj2);
+        double fpair = forcelj * r2inv;
+        fx += delx * fpair;
+        fy += dely * fpair;
+        fz += delz * fpair;
+        e += 0.5 * (r6inv * (pp->lj3 * r6inv - pp->lj4));
+        v[0] += 0.5 * delx * delx * fpair;
+        v[1] += 0.5 * dely * dely * fpair;
+        v[2] += 0.5 * delz * delz * fpair;
+        v[3] += 0.5 * delx * dely * fpair;
+        v[4] += 0.5 * delx * delz * fpair;
+        v[5] += 0.5 * dely * delz * fpair;
+      }
+    }
+    if (is_atom) {
+      f[3 * q] = fx;
+      f[3 * q + 1] = fy;
+      f[3 * q + 2] = fz;
+      if (eatom) eatom[q] = e;
+      if (vatom)
+        for (int a = 0; a < 6; a++) vatom[6 * q + a] = v[a];
+    } else {
+      double w = el->weight[row % nint];
+      if (ipf) { ipf[3 * row] = fx; ipf[3 * row + 1] = fy; ipf[3 * row + 2] = fz; }
+      if (eip) eip[row] = w * e;
+      if (vip)
+        for (int a = 0; a < 6; a++) vip[6 * row + a] = w * v[a];
+    }
+  }
+  /* node scatter in integration point order, pair_lj_cut.cpp:424-428
+   * (ipf is required here: it carries f_g to the scatter) */
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < ne; i++) {
+    double nf[8][3];
+    for (int k = 0; k < 8; k++) nf[k][0] = nf[k][1] = nf[k][2] = 0.0;
+    for (int g = 0; g < nint; g++) {
+      const double *Ng = el->spa_int + 8 * g;
+      size_t p = (size_t)i * nint + g;
+      double w = el->weight[g];
+      for (int kk = 0; kk < 8; kk++) {
+        nf[kk][0] += w * Ng[kk] * ipf[3 * p];
+        nf[kk][1] += w * Ng[kk] * ipf[3 * p + 1];
+        nf[kk][2] += w * Ng[kk] * ipf[3 * p + 2];
+      }
+    }
+    for (int k = 0; k < 8; k++)
+      for (int d = 0; d < 3; d++) nodef[24 * i + 3 * k + d] = nf[k][d];
+  }
+}
+
+/* ------------------------------------------------------------------------
  * FixNVE, fix_nve.cpp:40-242: dtf = 0.5*dt*ftm2v (:43); atoms use
  * dtfm = dtf/mass (:81); nodes use dtfm = dtf/(inv_npe*mass_e) (:124,:233).
  * ---------------------------------------------------------------------- */

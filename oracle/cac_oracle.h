@@ -109,6 +109,25 @@ void orc_lj_elements(const orc_elems *el, const orc_box *box,
                      const int *numneigh, const uint64_t *neigh, int maxnip,
                      const orc_lj *p, double *nodef, double *ipf, double *eip, double *vip);
 
+/* ---- atom-element coupling (build-defined; not in the reference) ----
+ * Lists hold uint64 entries (j | img << 27) | ((uint64)site << 32) with
+ * site == ORC_ATOM_SITE for an atom j.  Atom lists: atoms and element
+ * sites within cutneigh; integration-point lists: sites (reference rules)
+ * and atoms within cutneigh.  brute != 0 searches every image of every
+ * atom and site.  Rows are sorted. */
+#define ORC_ATOM_SITE 0xFFFFFFFFu
+int orc_coupled_neighbors(int n, const double *x, const orc_elems *el, const orc_box *box, double cutneigh,
+                          int nsplit, int brute, int maxneigh, int *anum, uint64_t *anb, int maxnip, int *inum,
+                          uint64_t *inb);
+void orc_eam_coupled(int n, const double *x, const orc_elems *el, const orc_box *box, const int *anum,
+                     const uint64_t *anb, int maxneigh, const int *inum, const uint64_t *inb, int maxnip,
+                     const orc_eam *t, double *rho, double *fp, double *f, double *eatom, double *vatom,
+                     double *erho, double *nodef, double *ipf, double *eip, double *vip);
+void orc_lj_coupled(int n, const double *x, const orc_elems *el, const orc_box *box, const int *anum,
+                    const uint64_t *anb, int maxneigh, const int *inum, const uint64_t *inb, int maxnip,
+                    const orc_lj *pp, double *f, double *eatom, double *vatom, double *nodef, double *ipf,
+                    double *eip, double *vip);
+
 /* ---- integration and periodic wrap ---- */
 void orc_nve_atoms_initial(int n, double *x, double *v, const double *f,
                            double dtv, double dtf, double mass);

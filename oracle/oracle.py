@@ -310,6 +310,69 @@ def displacement_exceeds(x, xhold, triggersq):
                                                   _p(np.ascontiguousarray(xhold)), C.c_double(triggersq)))
 
 
+# ------------------------------------------------------ atom-element coupling
+ATOM_SITE = 0xFFFFFFFF
+
+
+def coupled_neighbors(x, es: ElementSet, lo, hi, cutneigh, nsplit=2, maxneigh=160, maxnip=160, brute=False):
+    """Coupled lists (uint64 entries, site field ATOM_SITE for atoms): atom
+    rows hold atoms and element sites, integration-point rows sites and
+    atoms, each row sorted."""
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+    n = x.shape[0]
+    nip = es.nodex.shape[0] * es.spa_int.shape[0]
+    anum = np.zeros(max(n, 1), dtype=np.int32)
+    anb = np.zeros((max(n, 1), maxneigh), dtype=np.uint64)
+    inum = np.zeros(max(nip, 1), dtype=np.int32)
+    inb = np.zeros((max(nip, 1), maxnip), dtype=np.uint64)
+    b = _box(lo, hi)
+    es.refresh()
+    rc = lib().orc_coupled_neighbors(C.c_int(n), _p(x), C.byref(es.c), C.byref(b), C.c_double(cutneigh),
+                                     C.c_int(nsplit), C.c_int(1 if brute else 0), C.c_int(maxneigh), _p(anum, _i),
+                                     _p(anb, _u64), C.c_int(maxnip), _p(inum, _i), _p(inb, _u64))
+    if rc:
+        raise OverflowError(f"coupled neighbor build failed ({rc})")
+    return anum[:n], anb[:n], inum[:nip], inb[:nip]
+
+
+def eam_coupled(x, es: ElementSet, lo, hi, anum, anb, inum, inb, tables):
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+    n = x.shape[0]
+    ne, nint = es.nodex.shape[0], es.spa_int.shape[0]
+    out = dict(rho=np.zeros(n), fp=np.zeros(n), f=np.zeros((n, 3)), eatom=np.zeros(n), vatom=np.zeros((n, 6)),
+               erho=np.zeros((ne, 8)), nodef=np.zeros((ne, 8, 3)), ipf=np.zeros((ne * nint, 3)),
+               eip=np.zeros(ne * nint), vip=np.zeros((ne * nint, 6)))
+    t = _eam_struct(tables)
+    b = _box(lo, hi)
+    es.refresh()
+    anb = np.ascontiguousarray(anb)
+    inb = np.ascontiguousarray(inb)
+    lib().orc_eam_coupled(C.c_int(n), _p(x), C.byref(es.c), C.byref(b), _p(anum, _i), _p(anb, _u64),
+                          C.c_int(anb.shape[1] if anb.ndim == 2 else 0), _p(inum, _i), _p(inb, _u64),
+                          C.c_int(inb.shape[1] if inb.ndim == 2 else 0), C.byref(t),
+                          *[_p(out[k]) for k in ("rho", "fp", "f", "eatom", "vatom", "erho", "nodef", "ipf",
+                                                  "eip", "vip")])
+    return out
+
+
+def lj_coupled(x, es: ElementSet, lo, hi, anum, anb, inum, inb, lj):
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+    n = x.shape[0]
+    ne, nint = es.nodex.shape[0], es.spa_int.shape[0]
+    out = dict(f=np.zeros((n, 3)), eatom=np.zeros(n), vatom=np.zeros((n, 6)), nodef=np.zeros((ne, 8, 3)),
+               ipf=np.zeros((ne * nint, 3)), eip=np.zeros(ne * nint), vip=np.zeros((ne * nint, 6)))
+    p = LJ(*lj)
+    b = _box(lo, hi)
+    es.refresh()
+    anb = np.ascontiguousarray(anb)
+    inb = np.ascontiguousarray(inb)
+    lib().orc_lj_coupled(C.c_int(n), _p(x), C.byref(es.c), C.byref(b), _p(anum, _i), _p(anb, _u64),
+                         C.c_int(anb.shape[1] if anb.ndim == 2 else 0), _p(inum, _i), _p(inb, _u64),
+                         C.c_int(inb.shape[1] if inb.ndim == 2 else 0), C.byref(p),
+                         *[_p(out[k]) for k in ("f", "eatom", "vatom", "nodef", "ipf", "eip", "vip")])
+    return out
+
+
 def num_threads():
     return int(lib().orc_num_threads())
 

@@ -52,23 +52,45 @@ def rms_rel(a, b):
     return float(np.sqrt(np.mean((a - b) ** 2))) / nb if nb > 0 else 0.0
 
 
-def oracle_state(O, s: CACSystem, cutneigh, tables=None, lj=None, nsplit=2, maxneigh=200, maxnip=200):
-    """Oracle neighbor lists and forces for system s (positions as given)."""
+def atoms_u64(nb):
+    """Oracle atom entries (uint32 j | img << 27) in the library's uint64
+    form (site field CAC_ATOM_SITE)."""
+    return nb.astype(np.uint64) | (np.uint64(0xFFFFFFFF) << np.uint64(32))
+
+
+def oracle_state(O, s: CACSystem, cutneigh, tables=None, lj=None, nsplit=2, maxneigh=200, maxnip=200,
+                 coupled=False):
+    """Oracle neighbor lists and forces for system s (positions as given).
+    Atom entries are returned in the library's uint64 form."""
     es = O.ElementSet(s.nodex, s.nae, s.intp_ijk, s.intp_w)
-    out = {}
+    out = {"es": es}
+    if coupled:
+        out["anum"], out["anb"], out["inum"], out["inb"] = O.coupled_neighbors(
+            s.x, es, s.box_lo, s.box_hi, cutneigh, nsplit, maxneigh, maxnip)
+        args = (s.x, es, s.box_lo, s.box_hi, out["anum"], out["anb"], out["inum"], out["inb"])
+        r = None
+        if tables is not None:
+            r = O.eam_coupled(*args, tables)
+        if lj is not None:
+            r = O.lj_coupled(*args, lj)
+        if r is not None:
+            out["atoms"] = {k: r[k] for k in ("rho", "fp", "f", "eatom", "vatom") if k in r}
+            out["elems"] = {k: r[k] for k in ("erho", "nodef", "ipf", "eip", "vip") if k in r}
+        return out
     if s.n_atoms:
-        out["anum"], out["anb"] = O.atom_neighbors(s.x, s.box_lo, s.box_hi, cutneigh, maxneigh)
+        out["anum"], anb = O.atom_neighbors(s.x, s.box_lo, s.box_hi, cutneigh, maxneigh)
+        out["anb"] = atoms_u64(anb)
+        out["anb32"] = anb
     if s.n_elements:
         out["inum"], out["inb"] = O.ip_neighbors(es, s.box_lo, s.box_hi, cutneigh, nsplit, maxnip)
     if tables is not None:
         if s.n_atoms:
-            out["atoms"] = O.eam_atoms(s.x, s.box_lo, s.box_hi, out["anum"], out["anb"], tables)
+            out["atoms"] = O.eam_atoms(s.x, s.box_lo, s.box_hi, out["anum"], out["anb32"], tables)
         if s.n_elements:
             out["elems"] = O.eam_elements(es, s.box_lo, s.box_hi, out["inum"], out["inb"], tables)
     if lj is not None:
         if s.n_atoms:
-            out["atoms"] = O.lj_atoms(s.x, s.box_lo, s.box_hi, out["anum"], out["anb"], lj)
+            out["atoms"] = O.lj_atoms(s.x, s.box_lo, s.box_hi, out["anum"], out["anb32"], lj)
         if s.n_elements:
             out["elems"] = O.lj_elements(es, s.box_lo, s.box_hi, out["inum"], out["inb"], lj)
-    out["es"] = es
     return out

@@ -17,11 +17,12 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-10
 
 
-def make_engine(s, pair, cutneigh, maxneigh=0, maxnip=0):
+def make_engine(s, pair, cutneigh, maxneigh=0, maxnip=0, coupled=False):
     eng = Engine(0)
     eng.load_system(s)
     eng.set_pair(pair)
     eng.set_neighbor(cutneigh, every=1, delay=0, check=True, maxneigh=maxneigh, maxnip=maxnip)
+    eng.set_coupling(coupled)
     eng.build()
     return eng
 
@@ -45,12 +46,13 @@ SYSTEMS = {
 }
 
 
+@pytest.mark.parametrize("coupled", [False, True], ids=["plain", "coupled"])
 @pytest.mark.parametrize("name", list(SYSTEMS))
-def test_neighbor_sets_bitexact(name, orc, cu_funcfl):
+def test_neighbor_sets_bitexact(name, coupled, orc, cu_funcfl):
     s = SYSTEMS[name]()
-    print(f"[neighbors] {name}: atoms={s.n_atoms} elements={s.n_elements}", flush=True)
-    ref = oracle_state(orc, s, CUTNEIGH_EAM)
-    eng = make_engine(s, eam_pair(cu_funcfl), CUTNEIGH_EAM)
+    print(f"[neighbors] {name} coupled={coupled}: atoms={s.n_atoms} elements={s.n_elements}", flush=True)
+    ref = oracle_state(orc, s, CUTNEIGH_EAM, coupled=coupled)
+    eng = make_engine(s, eam_pair(cu_funcfl), CUTNEIGH_EAM, coupled=coupled)
     try:
         if s.n_atoms:
             num, nb = eng.atom_neighbors()
@@ -62,12 +64,13 @@ def test_neighbor_sets_bitexact(name, orc, cu_funcfl):
         eng.close()
 
 
+@pytest.mark.parametrize("coupled", [False, True], ids=["plain", "coupled"])
 @pytest.mark.parametrize("name", list(SYSTEMS))
-def test_eam_fp64_parity(name, orc, cu_funcfl, cu_tables):
+def test_eam_fp64_parity(name, coupled, orc, cu_funcfl, cu_tables):
     s = SYSTEMS[name]()
-    print(f"[eam fp64] {name}: atoms={s.n_atoms} elements={s.n_elements}", flush=True)
-    ref = oracle_state(orc, s, CUTNEIGH_EAM, tables=cu_tables)
-    eng = make_engine(s, eam_pair(cu_funcfl), CUTNEIGH_EAM)
+    print(f"[eam fp64] {name} coupled={coupled}: atoms={s.n_atoms} elements={s.n_elements}", flush=True)
+    ref = oracle_state(orc, s, CUTNEIGH_EAM, tables=cu_tables, coupled=coupled)
+    eng = make_engine(s, eam_pair(cu_funcfl), CUTNEIGH_EAM, coupled=coupled)
     try:
         eng.force(eflag=True, vflag=True)
         e, w = eng.energy_virial()
@@ -96,16 +99,17 @@ def test_eam_fp64_parity(name, orc, cu_funcfl, cu_tables):
         eng.close()
 
 
+@pytest.mark.parametrize("coupled", [False, True], ids=["plain", "coupled"])
 @pytest.mark.parametrize("name", ["small", "medium"])
-def test_lj_fp64_parity(name, orc):
+def test_lj_fp64_parity(name, coupled, orc):
     s = SYSTEMS[name]()
-    print(f"[lj fp64] {name}", flush=True)
+    print(f"[lj fp64] {name} coupled={coupled}", flush=True)
     lj = lj_coeffs(0.4093, 2.338, 5.0)
-    ref = oracle_state(orc, s, 6.0, lj=lj)
+    ref = oracle_state(orc, s, 6.0, lj=lj, coupled=coupled)
     p = PairLJCut()
     p.settings(["5.0"])
     p.coeff(["1", "1", "0.4093", "2.338"])
-    eng = make_engine(s, p, 6.0)
+    eng = make_engine(s, p, 6.0, coupled=coupled)
     try:
         eng.force(eflag=True, vflag=True)
         e, w = eng.energy_virial()
