@@ -325,6 +325,11 @@ int cac_ctx_create(int device, cac_ctx **out) {
   if (hipSetDevice(device) != hipSuccess) return CAC_ERR_HIP;
   cac_ctx *ctx = new cac_ctx();
   ctx->device = device;
+  {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && v > 0)
+      ctx->lds_bytes = (size_t)v;
+  }
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
     delete ctx;
     return CAC_ERR_HIP;
@@ -568,6 +573,13 @@ int cac_set_neighbor(cac_ctx *ctx, double cutneigh, double triggersq, int every,
   return CAC_OK;
 }
 
+int cac_set_list_order(cac_ctx *ctx, double spacing) {
+  if (!ctx || !(spacing >= 0.0)) return CAC_ERR_ARG;
+  if (spacing != ctx->list_spacing) ctx->built = false;
+  ctx->list_spacing = spacing;
+  return CAC_OK;
+}
+
 int cac_set_coupling(cac_ctx *ctx, int on) {
   if (!ctx) return CAC_ERR_ARG;
   if (ctx->coupled != (on != 0)) ctx->built = false;
@@ -687,9 +699,9 @@ int cac_get_capacities(cac_ctx *ctx, int *maxneigh, int *maxnip) {
 namespace {
 // Blocked device rows of particle entries -> row-major host rows of the
 // ABI's uint64 entries (atom j, or element/site of an active site).
-int download_rows(cac_ctx *ctx, const uint32_t *dev, const int *dnum, size_t rows, int M, int cap, int *numneigh,
-                  uint64_t *neigh) {
-  std::vector<uint32_t> cm((size_t)M * round_up((int)rows, kRowGroup));
+int download_rows(cac_ctx *ctx, const uint32_t *dev, const int *dnum, size_t rows, int M, int group, int cap,
+                  int *numneigh, uint64_t *neigh) {
+  std::vector<uint32_t> cm((size_t)M * round_up((int)rows, group));
   std::vector<int2> as((size_t)ctx->nactive);
   CAC_HIP(ctx, hipMemcpy(numneigh, dnum, rows * sizeof(int), hipMemcpyDeviceToHost));
   CAC_HIP(ctx, hipMemcpy(cm.data(), dev, cm.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
@@ -698,7 +710,7 @@ int download_rows(cac_ctx *ctx, const uint32_t *dev, const int *dnum, size_t row
   const uint32_t n = (uint32_t)ctx->n;
   for (size_t p = 0; p < rows; ++p)
     for (int k = 0; k < numneigh[p] && k < M; ++k) {
-      const uint32_t e = cm[((p >> 3) * M + k) * kRowGroup + (p & 7)];
+      const uint32_t e = cm[((p / group) * M + k) * group + (p % group)];
       const uint32_t idx = e & 0x7FFFFFFu;
       const uint64_t img = (uint64_t)(e >> 27) << 27;
       if (idx < n) {
@@ -723,7 +735,8 @@ int cac_get_atom_neighbors(cac_ctx *ctx, int maxneigh, int *numneigh, uint64_t *
   hipSetDevice(ctx->device);
   if (ctx->n == 0) return CAC_OK;
   CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  return download_rows(ctx, ctx->aneigh, ctx->anum, (size_t)ctx->n, ctx->maxneigh, maxneigh, numneigh, neigh);
+  return download_rows(ctx, ctx->aneigh, ctx->anum, (size_t)ctx->n, ctx->maxneigh, kAtomGroup, maxneigh, numneigh,
+                       neigh);
 }
 
 int cac_get_ip_neighbors(cac_ctx *ctx, int maxnip, int *numneigh, uint64_t *neigh) {
@@ -734,7 +747,7 @@ int cac_get_ip_neighbors(cac_ctx *ctx, int maxnip, int *numneigh, uint64_t *neig
   const size_t nip = (size_t)ctx->ne * ctx->nint;
   if (nip == 0) return CAC_OK;
   CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  return download_rows(ctx, ctx->ipneigh, ctx->ipnum, nip, ctx->maxnip, maxnip, numneigh, neigh);
+  return download_rows(ctx, ctx->ipneigh, ctx->ipnum, nip, ctx->maxnip, kRowGroup, maxnip, numneigh, neigh);
 }
 
 int cac_get_rho(cac_ctx *ctx, double *rho, double *fp) {

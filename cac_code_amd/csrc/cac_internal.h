@@ -33,6 +33,13 @@ constexpr uint32_t kAtomSite = 0xFFFFFFFFu;
 // 8 lanes each reads 64 consecutive entries per step.
 constexpr int kLanes = 8;
 constexpr int kRowGroup = 8;
+// Atom lists use groups of 64 rows: entry k of atom i lives at
+// ((i/64)*M + k)*64 + i%64, so the 64 threads of a wavefront (one atom each)
+// read 64 consecutive entries per step.
+constexpr int kAtomGroup = 64;
+__host__ __device__ __forceinline__ size_t atom_slot(int i, int k, int M) {
+  return ((size_t)(i >> 6) * M + k) * kAtomGroup + (i & 63);
+}
 
 // Workgroups are handed to the 8 XCDs of an MI355X round-robin.  Kernels
 // launched with xcd_grid() call xcd_block() to remap: XCD x then processes
@@ -99,6 +106,7 @@ struct LjParams {
 struct cac_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  size_t lds_bytes = 65536;  // shared memory per workgroup on this device
   std::string err;
 
   // box (fully periodic)
@@ -114,8 +122,12 @@ struct cac_ctx {
   double *x4 = nullptr, *v = nullptr, *f = nullptr, *xhold = nullptr, *rho = nullptr;
   double *eatom = nullptr, *vatom = nullptr;  // [n], [6][n]
   cac::DevBuf b_x4, b_v, b_f, b_xhold, b_rho, b_eatom, b_vatom;
-  // atom neighbor list, blocked: entry k of atom i at ((i/8)*maxneigh + k)*8 + i%8
+  // atom neighbor list, blocked: entry k of atom i at atom_slot(i, k, maxneigh).
+  // Each row is ordered by its displacement vectors rounded to list_spacing
+  // (lattice order; by entry value when 0), so on a lattice every atom lists
+  // its neighbors in the same relative order.
   int maxneigh = 128;
+  double list_spacing = 0.0;
   uint32_t *aneigh = nullptr;
   int *anum = nullptr;
   cac::DevBuf b_aneigh, b_anum;

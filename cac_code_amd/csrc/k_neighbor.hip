@@ -121,19 +121,55 @@ __global__ void k_cell_bounds(int n, const uint32_t *__restrict__ key, int *__re
 
 // ---------------------------------------------------------------- atoms
 // Full list (Neighbor::half_bin_no_newton, neighbor.cpp:546-625): every
-// (j, image) != (i, self) with rsq <= cutneighsq.  Candidates come from the
-// 27 cells around i's cell (cell edge >= cutneigh, >= 3 cells per dim), each
-// cell's atoms in ascending index, so the list order is deterministic.
-// Entry k of atom i is stored at ((i/8)*maxneigh + k)*8 + i%8.
-__global__ void k_atom_neigh(int n, const double *__restrict__ x4, Box3 b, Grid3 g,
-                             const uint32_t *__restrict__ sorted, const int *__restrict__ cs,
-                             const int *__restrict__ ce, double cutsq, int maxneigh,
-                             uint32_t *__restrict__ neigh, int *__restrict__ num,
-                             double *__restrict__ xhold, int *__restrict__ flags) {
-  const int i = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
-  if (i >= n) return;
-  uint32_t *row = neigh + (size_t)(i >> 3) * maxneigh * kRowGroup + (i & 7);
-  const double xi = x4[4 * i], yi = x4[4 * i + 1], zi = x4[4 * i + 2];
+// (j, image) != (i, self) with rsq <= cutneighsq, from the 27 cells around
+// i's cell (cell edge >= cutneigh, >= 3 cells per dim).
+//
+// One wavefront (one workgroup) per atom: lanes test the atoms of one cell
+// at a time and compact the accepted entries into LDS in lane order; the row
+// is then sorted in LDS (bitonic) by a key that puts the lattice-rounded
+// displacement x_j' - x_i (components rounded to multiples of `spacing`)
+// before the entry value, and written to the atom_slot layout.  On a lattice
+// of that spacing every atom lists its neighbors in the same relative order,
+// so at step k the 64 atoms of a force-pass wavefront gather neighboring rows
+// of x4 instead of 64 unrelated ones.  spacing = 0 sorts by entry value.
+// Either way the order is deterministic.
+__device__ __forceinline__ uint64_t list_key(double dx, double dy, double dz, double qinv, uint32_t entry) {
+  if (qinv <= 0.0) return entry;
+  auto q = [&](double d) {
+    int k = (int)rint(d * qinv);
+    k = k < -512 ? -512 : (k > 511 ? 511 : k);
+    return (uint64_t)(k + 512);
+  };
+  return (((q(dz) << 20) | (q(dy) << 10) | q(dx)) << 32) | entry;
+}
+
+__device__ void bitonic_sort_lds(uint64_t *a, int n2, int lane) {
+  for (int size = 2; size <= n2; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int t = lane; t < (n2 >> 1); t += 64) {
+        const int lo = 2 * t - (t & (stride - 1));
+        const int hi = lo + stride;
+        const bool up = (lo & size) == 0;
+        const uint64_t x = a[lo], y = a[hi];
+        if ((x > y) == up) {
+          a[lo] = y;
+          a[hi] = x;
+        }
+      }
+      __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(64) void k_atom_neigh(int n, const double *__restrict__ x4, Box3 b, Grid3 g,
+                                                   const uint32_t *__restrict__ sorted, const int *__restrict__ cs,
+                                                   const int *__restrict__ ce, double cutsq, int maxneigh, int cap2,
+                                                   double qinv, uint32_t *__restrict__ neigh, int *__restrict__ num,
+                                                   double *__restrict__ xhold, int *__restrict__ flags) {
+  extern __shared__ uint64_t keys[];  // [cap2], cap2 = power of two >= maxneigh
+  const int i = xcd_block();
+  if (i >= n) return;  // whole workgroup
+  const int lane = (int)threadIdx.x;
+  const double xi = x4[4 * (size_t)i], yi = x4[4 * (size_t)i + 1], zi = x4[4 * (size_t)i + 2];
   const int ci[3] = {cell_coord(xi, b.lo[0], g.inv[0], g.n[0]), cell_coord(yi, b.lo[1], g.inv[1], g.n[1]),
                      cell_coord(zi, b.lo[2], g.inv[2], g.n[2])};
   int cnt = 0;
@@ -148,28 +184,49 @@ __global__ void k_atom_neigh(int n, const double *__restrict__ x4, Box3 b, Grid3
         }
         const int img = (s[0] + 1) + 3 * (s[1] + 1) + 9 * (s[2] + 1);
         const int c = (cc[2] * g.n[1] + cc[1]) * g.n[0] + cc[0];
-        const int pe = ce[c];
-        for (int q = cs[c]; q < pe; ++q) {
-          const int j = (int)sorted[q];
-          if (j == i && img == kImgSelf) continue;
-          double delx = xi - shifted(x4[4 * j], s[0], b.prd[0]);
-          double dely = yi - shifted(x4[4 * j + 1], s[1], b.prd[1]);
-          double delz = zi - shifted(x4[4 * j + 2], s[2], b.prd[2]);
-          double rsq = delx * delx + dely * dely + delz * delz;
-          if (rsq <= cutsq) {
-            if (cnt < maxneigh) row[(size_t)cnt * kRowGroup] = (uint32_t)j | ((uint32_t)img << 27);
-            ++cnt;
+        const int qs = cs[c], qe = ce[c];
+        for (int q0 = qs; q0 < qe; q0 += 64) {
+          const int q = q0 + lane;
+          bool acc = false;
+          uint64_t key = 0;
+          if (q < qe) {
+            const int j = (int)sorted[q];
+            if (!(j == i && img == kImgSelf)) {
+              const double xj = shifted(x4[4 * (size_t)j], s[0], b.prd[0]);
+              const double yj = shifted(x4[4 * (size_t)j + 1], s[1], b.prd[1]);
+              const double zj = shifted(x4[4 * (size_t)j + 2], s[2], b.prd[2]);
+              const double delx = xi - xj, dely = yi - yj, delz = zi - zj;
+              const double rsq = delx * delx + dely * dely + delz * delz;
+              acc = rsq <= cutsq;
+              key = list_key(xj - xi, yj - yi, zj - zi, qinv, (uint32_t)j | ((uint32_t)img << 27));
+            }
           }
+          const unsigned long long am = __ballot(acc);
+          if (acc) {
+            const int pos = cnt + __popcll(am & ((1ull << lane) - 1ull));
+            if (pos < cap2) keys[pos] = key;
+          }
+          cnt += __popcll(am);
         }
       }
-  num[i] = cnt;
-  if (cnt > maxneigh) {
-    atomicOr(&flags[0], 1);
-    atomicMax(&flags[1], cnt);
+  const int kept = cnt < cap2 ? cnt : cap2;
+  int n2 = 1;
+  while (n2 < kept) n2 <<= 1;
+  for (int p = kept + lane; p < n2; p += 64) keys[p] = ~0ull;
+  __syncthreads();
+  bitonic_sort_lds(keys, n2, lane);
+  const int out = cnt < maxneigh ? cnt : maxneigh;
+  for (int k = lane; k < out; k += 64) neigh[atom_slot(i, k, maxneigh)] = (uint32_t)keys[k];
+  if (lane == 0) {
+    num[i] = cnt;
+    if (cnt > maxneigh) {
+      atomicOr(&flags[0], 1);
+      atomicMax(&flags[1], cnt);
+    }
+    xhold[i] = xi;
+    xhold[n + i] = yi;
+    xhold[2 * (size_t)n + i] = zi;
   }
-  xhold[i] = xi;
-  xhold[n + i] = yi;
-  xhold[2 * (size_t)n + i] = zi;
 }
 
 // ---------------------------------------------------------------- elements
@@ -775,8 +832,7 @@ __global__ void k_atom_site_translate(int niface, int maxneigh, const int *__res
   if (k >= asnum[r]) return;
   const int i = iface[r];
   const int pos = anum[i] + k;
-  aneigh[((size_t)(i >> 3) * maxneigh + pos) * kRowGroup + (i & 7)] = particle_entry(atmp[t], n, nwords, bitmap,
-                                                                                       wprefix);
+  aneigh[atom_slot(i, pos, maxneigh)] = particle_entry(atmp[t], n, nwords, bitmap, wprefix);
 }
 
 __global__ void k_atom_site_count(int niface, const int *__restrict__ iface, const int *__restrict__ asnum,
@@ -788,8 +844,8 @@ __global__ void k_atom_site_count(int niface, const int *__restrict__ iface, con
 // Every entry of a blocked list must name a row of x4 (< nrow) with an image
 // code < 27: the pair passes gather with it.  The first bad entry found is
 // reported as (row, slot, entry) in flags[8..10] with flags[7] set.
-__global__ void k_validate_list(int rows, int M, const uint32_t *__restrict__ list, const int *__restrict__ num,
-                                uint32_t nrow, int *__restrict__ flags) {
+__global__ void k_validate_list(int rows, int M, int group, const uint32_t *__restrict__ list,
+                                const int *__restrict__ num, uint32_t nrow, int *__restrict__ flags) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)rows * M) return;
   const int p = (int)(t / M), k = (int)(t % M);
@@ -803,7 +859,7 @@ __global__ void k_validate_list(int rows, int M, const uint32_t *__restrict__ li
     return;
   }
   if (k >= c) return;
-  const uint32_t e = list[((size_t)(p >> 3) * M + k) * kRowGroup + (p & 7)];
+  const uint32_t e = list[((size_t)(p / group) * M + k) * group + (p % group)];
   if ((e & 0x7FFFFFFu) >= nrow || (e >> 27) >= 27u) {
     if (atomicCAS(&flags[7], 0, 1) == 0) {
       flags[8] = p;
@@ -952,7 +1008,7 @@ int launch_atom_neighbors(cac_ctx *ctx) {
                        ctx->b_ace, &sorted, &cs, &ce)))
     return rc;
   ctx->maxneigh = round_up(ctx->maxneigh, kLanes);
-  const size_t rows = (size_t)round_up(n, kRowGroup);
+  const size_t rows = (size_t)round_up(n, kAtomGroup);
   if ((rc = ensure(ctx, ctx->b_aneigh, (size_t)ctx->maxneigh * rows, &ctx->aneigh)) ||
       (rc = ensure(ctx, ctx->b_anum, rows, &ctx->anum)) || (rc = ensure(ctx, ctx->b_xhold, 3 * (size_t)n, &ctx->xhold)))
     return rc;
@@ -961,9 +1017,12 @@ int launch_atom_neighbors(cac_ctx *ctx) {
   ctx->abin_e = ce;
   hipEvent_t t0;
   timer_begin(ctx, "atom_neigh", &t0);
-  hipLaunchKernelGGL(k_atom_neigh, dim3(xcd_grid(n, kBlock)), dim3(kBlock), 0, ctx->stream, n, ctx->x4, box_of(ctx),
-                     g, sorted, cs, ce, ctx->cutneigh * ctx->cutneigh, ctx->maxneigh, ctx->aneigh, ctx->anum,
-                     ctx->xhold, ctx->dflags);
+  int cap2 = 64;
+  while (cap2 < ctx->maxneigh) cap2 <<= 1;
+  const double qinv = ctx->list_spacing > 0.0 ? 1.0 / ctx->list_spacing : 0.0;
+  hipLaunchKernelGGL(k_atom_neigh, dim3(xcd_grid(n, 1)), dim3(64), cap2 * sizeof(uint64_t), ctx->stream, n, ctx->x4,
+                     box_of(ctx), g, sorted, cs, ce, ctx->cutneigh * ctx->cutneigh, ctx->maxneigh, cap2, qinv,
+                     ctx->aneigh, ctx->anum, ctx->xhold, ctx->dflags);
   timer_end(ctx, "atom_neigh", t0);
   CAC_HIP(ctx, hipGetLastError());
   return CAC_OK;
@@ -1177,7 +1236,7 @@ int validate_lists(cac_ctx *ctx) {
   const int nip = ctx->ne * ctx->nint;
   if (ctx->n)
     hipLaunchKernelGGL(k_validate_list, dim3(grid_for((long)ctx->n * ctx->maxneigh)), dim3(kBlock), 0, ctx->stream,
-                       ctx->n, ctx->maxneigh, ctx->aneigh, ctx->anum, nrow, ctx->dflags);
+                       ctx->n, ctx->maxneigh, kAtomGroup, ctx->aneigh, ctx->anum, nrow, ctx->dflags);
   int bad[4];
   CAC_HIP(ctx, hipMemcpyAsync(bad, ctx->dflags + 7, sizeof(bad), hipMemcpyDeviceToHost, ctx->stream));
   CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -1188,7 +1247,7 @@ int validate_lists(cac_ctx *ctx) {
                                        std::to_string(ctx->nactive) + ")");
   if (nip)
     hipLaunchKernelGGL(k_validate_list, dim3(grid_for((long)nip * ctx->maxnip)), dim3(kBlock), 0, ctx->stream, nip,
-                       ctx->maxnip, ctx->ipneigh, ctx->ipnum, nrow, ctx->dflags);
+                       ctx->maxnip, kRowGroup, ctx->ipneigh, ctx->ipnum, nrow, ctx->dflags);
   CAC_HIP(ctx, hipMemcpyAsync(bad, ctx->dflags + 7, sizeof(bad), hipMemcpyDeviceToHost, ctx->stream));
   CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (bad[0])

@@ -9,11 +9,12 @@
 // force are evaluated in FP32; positions, site interpolation, rho, force,
 // energy and virial sums stay FP64.
 //
-// Work mapping: 8 lanes per atom / node / integration point.  Lane l takes
-// list entries l, l+8, ...; the blocked list layout makes the 64 lanes of a
-// wavefront read 64 consecutive entries per iteration.  Partial sums are
-// combined with a fixed xor butterfly (every lane ends with the same bits),
-// so results are reproducible run to run.
+// Work mapping: one thread per atom (atom lists in lattice order, see the
+// atom section); 8 lanes per node / integration point, lane l taking list
+// entries l, l+8, ... so the 64 lanes of a wavefront read 64 consecutive
+// entries per iteration, partial sums combined with a fixed xor butterfly
+// (every lane ends with the same bits).  Results are reproducible run to
+// run.
 //
 // Element path: the sites that some list refers to (the "active sites",
 // numbered at rebuild) are interpolated once per step (k_site_pos:
@@ -100,6 +101,77 @@ __device__ __forceinline__ const P *rhorow(const Tabs<P> &t, P rho, P *pout) {
   p = p < P(1.0) ? p : P(1.0);
   *pout = p;
   return t.fr + 8 * (size_t)m;
+}
+
+// Table window in LDS: rows m0 .. nr-1 of a per-knot table (stride P
+// values per row) copied into shared memory at kernel start; rows below m0
+// (short pair distances, rarely reached) are read from the global table.
+// Gathering coefficient rows through LDS instead of the vector L1 keeps the
+// force passes off the texture-address path, which 64 distinct rows per
+// wavefront instruction otherwise saturate.
+template <class P>
+using lds_cptr = const __attribute__((address_space(3))) P *;
+
+template <class P>
+struct Win {
+  lds_cptr<P> lds;
+  int m0;
+};
+
+template <class P>
+__device__ __forceinline__ Win<P> make_win(const unsigned char *smem, int m0) {
+  return Win<P>{(lds_cptr<P>)(const __attribute__((address_space(3))) unsigned char *)smem, m0};
+}
+
+template <class P>
+__device__ void load_window(const P *__restrict__ table, int stride, int m0, int nr, P *lds) {
+  const int units = (nr - m0) * stride * (int)sizeof(P) / 16;
+  const uint4 *src = reinterpret_cast<const uint4 *>(table + (size_t)m0 * stride);
+  uint4 *dst = reinterpret_cast<uint4 *>(lds);
+  for (int q = (int)threadIdx.x; q < units; q += (int)blockDim.x) dst[q] = src[q];
+}
+
+// Row copies, one per address space: the two sources are read in separate
+// branches so the LDS side compiles to ds_read and the global side to
+// global_load (a pointer select between them would force flat loads, which
+// go through the texture-address path like any global gather).
+template <class P, int K>
+struct Row {
+  P c[K];
+};
+template <class P, int K>
+__device__ __forceinline__ Row<P, K> fetch_row(const Tabs<P> &t, const P *__restrict__ gtab, const Win<P> &w, int m) {
+  Row<P, K> out;
+  if (m >= w.m0) {
+    lds_cptr<P> q = w.lds + K * (m - w.m0);
+#pragma unroll
+    for (int k = 0; k < K; ++k) out.c[k] = q[k];
+  } else {
+    const P *q = gtab + K * (size_t)m;
+#pragma unroll
+    for (int k = 0; k < K; ++k) out.c[k] = q[k];
+  }
+  return out;
+}
+
+template <class P>
+__device__ __forceinline__ P rho_of_r_w(const Tabs<P> &t, const Win<P> &w, P r) {
+  P p;
+  const int m = rknot<P>(t, r, &p);
+  const Row<P, 4> R = fetch_row<P, 4>(t, t.rv, w, m);
+  const P *c = R.c;
+  return ((c[0] * p + c[1]) * p + c[2]) * p + c[3];
+}
+
+template <class P>
+__device__ __forceinline__ void rz_of_r_w(const Tabs<P> &t, const Win<P> &w, P r, P *rhoip, P *z2p, P *z2) {
+  P p;
+  const int m = rknot<P>(t, r, &p);
+  const Row<P, 8> R = fetch_row<P, 8>(t, t.rz, w, m);
+  const P *c = R.c;
+  *rhoip = ((P(3.0) * c[0] * p + P(2.0) * c[1]) * p + c[2]) * t.rdr;
+  *z2p = ((P(3.0) * c[4] * p + P(2.0) * c[5]) * p + c[6]) * t.rdr;
+  *z2 = ((c[4] * p + c[5]) * p + c[6]) * p + c[7];
 }
 
 template <class P>
@@ -209,9 +281,10 @@ struct Virial {
 // EAM pair terms shared by the atom and integration-point force passes,
 // pair_eam.cpp:706-738 / 1083-1120.
 template <class P>
-__device__ __forceinline__ double eam_fpair(const Tabs<P> &t, double rsq, P fpi, P fpj, P *phi_out) {
+__device__ __forceinline__ double eam_fpair(const Tabs<P> &t, const Win<P> &w, double rsq, P fpi, P fpj,
+                                            P *phi_out) {
   P r = sqrt_p<P>(rsq), rhoip, z2p, z2;
-  rz_of_r<P>(t, r, &rhoip, &z2p, &z2);
+  rz_of_r_w<P>(t, w, r, &rhoip, &z2p, &z2);
   P recip = P(1.0) / r;
   P phi = z2 * recip;
   P phip = z2p * recip - phi * recip;
@@ -220,81 +293,97 @@ __device__ __forceinline__ double eam_fpair(const Tabs<P> &t, double rsq, P fpi,
   return (double)(-psip * recip);
 }
 
-// ------------------------------------------------------------- atoms, EAM
+// ------------------------------------------------------------- atoms
+// One thread per atom.  Atom lists are in the atom_slot layout and every row
+// is in lattice order, so the 64 atoms of a wavefront (consecutive atoms,
+// spatially close) read their k-th entries from one 256-byte run and mostly
+// gather neighboring x4 rows.  Each atom sums its entries in list order, so
+// results are reproducible run to run.  kUA entries are fetched (list words,
+// then neighbor rows) before any arithmetic.
+constexpr int kUA = 8;
+
+template <int U>
+__device__ __forceinline__ void fetch_atom_entries(const uint32_t *__restrict__ row, int k0, int jn, uint32_t e[U]) {
+#pragma unroll
+  for (int u = 0; u < U; ++u) e[u] = k0 + u < jn ? row[(size_t)(k0 + u) * kAtomGroup] : kNoEntry;
+}
+
 // rho pass + fp pass fused (pair_eam.cpp:629-680); fp stored in x4.w, which
 // the force pass reads for ghosts too (forward_comm_pair, :683).
+constexpr int kAtomWG = 1024;
+
 template <class P, bool EV>
-__global__ __launch_bounds__(kBlock) void k_eam_atom_rho(int n, double *__restrict__ x4, Prd prd,
-                                                         const uint32_t *__restrict__ neigh, int M,
-                                                         const int *__restrict__ num, Tabs<P> t,
-                                                         double *__restrict__ rho_out, double *__restrict__ eatom) {
-  const int gt = xcd_block() * kBlock + (int)threadIdx.x;
-  const int i = gt >> 3, l = gt & 7;
+__global__ __launch_bounds__(kAtomWG) void k_eam_atom_rho(int n, double *__restrict__ x4, Prd prd,
+                                                          const uint32_t *__restrict__ neigh, int M,
+                                                          const int *__restrict__ num, Tabs<P> t, int m0,
+                                                          double *__restrict__ rho_out, double *__restrict__ eatom) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const Win<P> w = make_win<P>(smem, m0);
+  load_window<P>(t.rv, 4, m0, t.nr, reinterpret_cast<P *>(smem));
+  __syncthreads();
+  const int i = xcd_block() * kAtomWG + (int)threadIdx.x;
+  if (i >= n) return;
+  const double xi = x4[4 * (size_t)i], yi = x4[4 * (size_t)i + 1], zi = x4[4 * (size_t)i + 2];
+  const int jn = num[i];
+  const uint32_t *row = neigh + atom_slot(i, 0, M);
   double rho = 0.0;
-  if (i < n) {
-    const double xi = x4[4 * (size_t)i], yi = x4[4 * (size_t)i + 1], zi = x4[4 * (size_t)i + 2];
-    const int jn = num[i];
-    const uint32_t *row = list_row(neigh, i, M);
-    for (int k0 = l; k0 < jn; k0 += kU * kLanes) {
-      uint32_t e[kU];
-      fetch_entries<kU>(row, k0, jn, e);
-      Pt xj[kU];
+  for (int k0 = 0; k0 < jn; k0 += kUA) {
+    uint32_t e[kUA];
+    fetch_atom_entries<kUA>(row, k0, jn, e);
+    Pt xj[kUA];
 #pragma unroll
-      for (int u = 0; u < kU; ++u)
-        if (e[u] != kNoEntry) xj[u] = load_xyz(x4, e[u], prd);
+    for (int u = 0; u < kUA; ++u)
+      if (e[u] != kNoEntry) xj[u] = load_xyz(x4, e[u], prd);
 #pragma unroll
-      for (int u = 0; u < kU; ++u) {
-        if (e[u] == kNoEntry) continue;
-        const double delx = xi - xj[u].x, dely = yi - xj[u].y, delz = zi - xj[u].z;
-        const double rsq = delx * delx + dely * dely + delz * delz;
-        if (rsq < t.cutforcesq) rho += (double)rho_of_r<P>(t, sqrt_p<P>(rsq));
-      }
+    for (int u = 0; u < kUA; ++u) {
+      if (e[u] == kNoEntry) continue;
+      const double delx = xi - xj[u].x, dely = yi - xj[u].y, delz = zi - xj[u].z;
+      const double rsq = delx * delx + dely * dely + delz * delz;
+      if (rsq < t.cutforcesq) rho += (double)rho_of_r_w<P>(t, w, sqrt_p<P>(rsq));
     }
   }
-  rho = group_sum(rho);
-  if (i < n && l == 0) {
-    rho_out[i] = rho;
-    const double fp = (double)embed_fp<P>(t, rho);
-    x4[4 * (size_t)i + 3] = fp;
-    if (EV) eatom[i] = embed_energy<P>(t, rho, fp);
-  }
+  rho_out[i] = rho;
+  const double fp = (double)embed_fp<P>(t, rho);
+  x4[4 * (size_t)i + 3] = fp;
+  if (EV) eatom[i] = embed_energy<P>(t, rho, fp);
 }
 
 // force pass without Newton's third law, pair_eam.cpp:688-745
 template <class P, bool EV>
-__global__ __launch_bounds__(kBlock) void k_eam_atom_force(int n, const double *__restrict__ x4, Prd prd,
-                                                           const uint32_t *__restrict__ neigh, int M,
-                                                           const int *__restrict__ num, Tabs<P> t,
-                                                           double *__restrict__ f, double *__restrict__ eatom,
-                                                           double *__restrict__ vatom) {
-  const int gt = xcd_block() * kBlock + (int)threadIdx.x;
-  const int i = gt >> 3, l = gt & 7;
+__global__ __launch_bounds__(kAtomWG) void k_eam_atom_force(int n, const double *__restrict__ x4, Prd prd,
+                                                            const uint32_t *__restrict__ neigh, int M,
+                                                            const int *__restrict__ num, Tabs<P> t, int m0,
+                                                            double *__restrict__ f, double *__restrict__ eatom,
+                                                            double *__restrict__ vatom) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const Win<P> w = make_win<P>(smem, m0);
+  load_window<P>(t.rz, 8, m0, t.nr, reinterpret_cast<P *>(smem));
+  __syncthreads();
+  const int i = xcd_block() * kAtomWG + (int)threadIdx.x;
+  if (i >= n) return;
   double fx = 0.0, fy = 0.0, fz = 0.0, ep = 0.0;
   Virial vir;
   vir.zero();
-  if (i < n) {
-    const double xi = x4[4 * (size_t)i], yi = x4[4 * (size_t)i + 1], zi = x4[4 * (size_t)i + 2];
-    const P fpi = (P)x4[4 * (size_t)i + 3];
-    const int jn = num[i];
-    const uint32_t *row = list_row(neigh, i, M);
-    for (int k0 = l; k0 < jn; k0 += kU * kLanes) {
-      uint32_t e[kU];
-      fetch_entries<kU>(row, k0, jn, e);
-      Pt xjv[kU];
-      double fpjv[kU];
+  const double xi = x4[4 * (size_t)i], yi = x4[4 * (size_t)i + 1], zi = x4[4 * (size_t)i + 2];
+  const P fpi = (P)x4[4 * (size_t)i + 3];
+  const int jn = num[i];
+  const uint32_t *row = neigh + atom_slot(i, 0, M);
+  for (int k0 = 0; k0 < jn; k0 += kUA) {
+    uint32_t e[kUA];
+    fetch_atom_entries<kUA>(row, k0, jn, e);
+    Pt xjv[kUA];
+    double fpjv[kUA];
 #pragma unroll
-      for (int u = 0; u < kU; ++u)
-        if (e[u] != kNoEntry) xjv[u] = load_xyzw(x4, e[u], prd, &fpjv[u]);
+    for (int u = 0; u < kUA; ++u)
+      if (e[u] != kNoEntry) xjv[u] = load_xyzw(x4, e[u], prd, &fpjv[u]);
 #pragma unroll
-      for (int u = 0; u < kU; ++u) {
+    for (int u = 0; u < kUA; ++u) {
       if (e[u] == kNoEntry) continue;
-      const Pt xj = xjv[u];
-      const double fpj = fpjv[u];
-      const double delx = xi - xj.x, dely = yi - xj.y, delz = zi - xj.z;
+      const double delx = xi - xjv[u].x, dely = yi - xjv[u].y, delz = zi - xjv[u].z;
       const double rsq = delx * delx + dely * dely + delz * delz;
       if (rsq < t.cutforcesq) {
         P phi;
-        const double fpair = eam_fpair<P>(t, rsq, fpi, (P)fpj, &phi);
+        const double fpair = eam_fpair<P>(t, w, rsq, fpi, (P)fpjv[u], &phi);
         fx += delx * fpair;
         fy += dely * fpair;
         fz += delz * fpair;
@@ -303,24 +392,14 @@ __global__ __launch_bounds__(kBlock) void k_eam_atom_force(int n, const double *
           vir.add(delx, dely, delz, fpair);
         }
       }
-      }
     }
   }
-  fx = group_sum(fx);
-  fy = group_sum(fy);
-  fz = group_sum(fz);
+  f[i] = fx;
+  f[(size_t)n + i] = fy;
+  f[2 * (size_t)n + i] = fz;
   if (EV) {
-    ep = group_sum(ep);
-    vir.reduce();
-  }
-  if (i < n && l == 0) {
-    f[i] = fx;
-    f[(size_t)n + i] = fy;
-    f[2 * (size_t)n + i] = fz;
-    if (EV) {
-      eatom[i] += ep;
-      vir.store(vatom, (size_t)n, (size_t)i, 1.0);
-    }
+    eatom[i] += ep;
+    vir.store(vatom, (size_t)n, (size_t)i, 1.0);
   }
 }
 
@@ -332,20 +411,26 @@ __global__ __launch_bounds__(kBlock) void k_lj_atom_force(int n, const double *_
                                                           const int *__restrict__ num, LjParams lj,
                                                           double *__restrict__ f, double *__restrict__ eatom,
                                                           double *__restrict__ vatom) {
-  const int gt = xcd_block() * kBlock + (int)threadIdx.x;
-  const int i = gt >> 3, l = gt & 7;
+  const int i = xcd_block() * kBlock + (int)threadIdx.x;
+  if (i >= n) return;
   const P lj1 = (P)lj.lj1, lj2 = (P)lj.lj2, lj3 = (P)lj.lj3, lj4 = (P)lj.lj4;
   double fx = 0.0, fy = 0.0, fz = 0.0, ep = 0.0;
   Virial vir;
   vir.zero();
-  if (i < n) {
-    const double xi = x4[4 * (size_t)i], yi = x4[4 * (size_t)i + 1], zi = x4[4 * (size_t)i + 2];
-    const int jn = num[i];
-    const uint32_t *row = list_row(neigh, i, M);
-    for (int k = l; k < jn; k += kLanes) {
-      const uint32_t e = row[(size_t)k * kRowGroup];
-      const Pt xj = load_xyz(x4, e, prd);
-      const double delx = xi - xj.x, dely = yi - xj.y, delz = zi - xj.z;
+  const double xi = x4[4 * (size_t)i], yi = x4[4 * (size_t)i + 1], zi = x4[4 * (size_t)i + 2];
+  const int jn = num[i];
+  const uint32_t *row = neigh + atom_slot(i, 0, M);
+  for (int k0 = 0; k0 < jn; k0 += kUA) {
+    uint32_t e[kUA];
+    fetch_atom_entries<kUA>(row, k0, jn, e);
+    Pt xjv[kUA];
+#pragma unroll
+    for (int u = 0; u < kUA; ++u)
+      if (e[u] != kNoEntry) xjv[u] = load_xyz(x4, e[u], prd);
+#pragma unroll
+    for (int u = 0; u < kUA; ++u) {
+      if (e[u] == kNoEntry) continue;
+      const double delx = xi - xjv[u].x, dely = yi - xjv[u].y, delz = zi - xjv[u].z;
       const double rsq = delx * delx + dely * dely + delz * delz;
       if (rsq < lj.cutsq) {
         P r2inv = P(1.0) / (P)rsq;
@@ -362,21 +447,12 @@ __global__ __launch_bounds__(kBlock) void k_lj_atom_force(int n, const double *_
       }
     }
   }
-  fx = group_sum(fx);
-  fy = group_sum(fy);
-  fz = group_sum(fz);
+  f[i] = fx;
+  f[(size_t)n + i] = fy;
+  f[2 * (size_t)n + i] = fz;
   if (EV) {
-    ep = group_sum(ep);
-    vir.reduce();
-  }
-  if (i < n && l == 0) {
-    f[i] = fx;
-    f[(size_t)n + i] = fy;
-    f[2 * (size_t)n + i] = fz;
-    if (EV) {
-      eatom[i] = ep;
-      vir.store(vatom, (size_t)n, (size_t)i, 1.0);
-    }
+    eatom[i] = ep;
+    vir.store(vatom, (size_t)n, (size_t)i, 1.0);
   }
 }
 
@@ -745,19 +821,41 @@ int scatter(cac_ctx *ctx, const ElemArgs &a) {
 // optional subsets (surrogate mode): site positions over `spos`, node
 // densities over `erho_el`, site F' over `sfp`, forces and scatter over
 // `force_el`.
+// LDS window of a per-knot table with rows of `rowbytes`: the last rows up to
+// the device's shared-memory limit per workgroup (rows m0 .. nr-1).
+struct TabWindow {
+  int m0;
+  size_t bytes;
+};
+TabWindow table_window(const cac_ctx *ctx, size_t rowbytes) {
+  const int nr = ctx->eam.nr;
+  long rows = (long)(ctx->lds_bytes / rowbytes);
+  if (rows > nr - 1) rows = nr - 1;
+  if (rows < 1) rows = 1;
+  TabWindow w;
+  w.m0 = nr - (int)rows;
+  w.bytes = (size_t)rows * rowbytes;
+  return w;
+}
+
 template <class P, bool EV>
 int eam_all(cac_ctx *ctx, const ElemArgs &base, const ElemArgs &spos, const ElemArgs &erho_el, const ElemArgs &sfp,
             const ElemArgs &force_el) {
   Tabs<P> t = tabs_of<P>(ctx);
   const int n = ctx->n;
   const bool el = ctx->ne > 0;
-  const unsigned agrid = xcd_grid((long)n * kLanes, kBlock);
+  const unsigned agrid = xcd_grid((long)n, kAtomWG);
+  const TabWindow wr = table_window(ctx, 4 * sizeof(P)), wz = table_window(ctx, 8 * sizeof(P));
   hipEvent_t t0;
   if (el) site_positions(ctx, spos);
   if (n) {
+    CAC_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_eam_atom_rho<P, EV>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)wr.bytes));
+    CAC_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_eam_atom_force<P, EV>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)wz.bytes));
     timer_begin(ctx, "atom_rho", &t0);
-    hipLaunchKernelGGL((k_eam_atom_rho<P, EV>), dim3(agrid), dim3(kBlock), 0, ctx->stream, n, ctx->x4, prd_of(ctx),
-                       ctx->aneigh, ctx->maxneigh, ctx->anum, t, ctx->rho, ctx->eatom);
+    hipLaunchKernelGGL((k_eam_atom_rho<P, EV>), dim3(agrid), dim3(kAtomWG), wr.bytes, ctx->stream, n, ctx->x4,
+                       prd_of(ctx), ctx->aneigh, ctx->maxneigh, ctx->anum, t, wr.m0, ctx->rho, ctx->eatom);
     timer_end(ctx, "atom_rho", t0);
   }
   if (el && host_nelem(erho_el)) {
@@ -774,8 +872,8 @@ int eam_all(cac_ctx *ctx, const ElemArgs &base, const ElemArgs &spos, const Elem
   }
   if (n) {
     timer_begin(ctx, "atom_force", &t0);
-    hipLaunchKernelGGL((k_eam_atom_force<P, EV>), dim3(agrid), dim3(kBlock), 0, ctx->stream, n, ctx->x4, prd_of(ctx),
-                       ctx->aneigh, ctx->maxneigh, ctx->anum, t, ctx->f, ctx->eatom, ctx->vatom);
+    hipLaunchKernelGGL((k_eam_atom_force<P, EV>), dim3(agrid), dim3(kAtomWG), wz.bytes, ctx->stream, n, ctx->x4,
+                       prd_of(ctx), ctx->aneigh, ctx->maxneigh, ctx->anum, t, wz.m0, ctx->f, ctx->eatom, ctx->vatom);
     timer_end(ctx, "atom_force", t0);
   }
   const long nf = el ? host_nelem(force_el) : 0;
@@ -813,7 +911,7 @@ int lj_t(cac_ctx *ctx) {
   if (ctx->ne) site_positions(ctx, a);
   if (n) {
     timer_begin(ctx, "atom_force", &t0);
-    hipLaunchKernelGGL((k_lj_atom_force<P, EV>), dim3(xcd_grid((long)n * kLanes, kBlock)), dim3(kBlock), 0,
+    hipLaunchKernelGGL((k_lj_atom_force<P, EV>), dim3(xcd_grid((long)n, kBlock)), dim3(kBlock), 0,
                        ctx->stream, n, ctx->x4, prd_of(ctx), ctx->aneigh, ctx->maxneigh, ctx->anum, ctx->lj, ctx->f,
                        ctx->eatom, ctx->vatom);
     timer_end(ctx, "atom_force", t0);

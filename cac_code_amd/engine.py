@@ -300,6 +300,8 @@ class Engine(_MlpMixin):
         mass_e = float(s.nae[0] * s.nae[1] * s.nae[2]) * m
         self.mass_e = mass_e
         self.upload_elements(s.nodex, s.nodev, s.nae, mass_e, s.intp_ijk, s.intp_w)
+        if getattr(s, "a0", None):
+            self.set_list_order(0.5 * s.a0)                 # fcc neighbor vectors: multiples of a/2
 
     def set_pair(self, pair):
         self.pair = pair
@@ -323,6 +325,11 @@ class Engine(_MlpMixin):
         triggersq = 0.25 * skin * skin
         _check(self.ctx, self.L.cac_set_neighbor(self.ctx, cutneigh, triggersq, every, delay, 1 if check else 0,
                                                  maxneigh, maxnip))
+
+    def set_list_order(self, spacing: float):
+        """Sort each atom list by displacement rounded to `spacing` (a/2 for
+        fcc): a gather-locality hint, the neighbor sets do not change."""
+        _check(self.ctx, self.L.cac_set_list_order(self.ctx, float(spacing)))
 
     def set_coupling(self, on: bool = True):
         """Atom-element coupling (build-defined, off by default = the

@@ -104,6 +104,15 @@ int cac_set_lj(cac_ctx *ctx, double lj1, double lj2, double lj3, double lj4, dou
 int cac_set_neighbor(cac_ctx *ctx, double cutneigh, double triggersq, int every, int delay,
                      int check, int maxneigh, int maxnip);
 
+/* Order of the entries within each atom list (no counterpart in the
+ * reference, whose lists are in bin order, neighbor.cpp:583-613): entries are
+ * sorted by their displacement x_j' - x_i rounded to multiples of `spacing`
+ * (Angstrom), then by value.  On a lattice whose neighbor vectors are
+ * multiples of `spacing` (a/2 for fcc) every atom then lists its neighbors in
+ * the same relative order, which makes the force passes' gathers coalesce.
+ * 0 (the default) sorts by value.  The neighbor sets do not change. */
+int cac_set_list_order(cac_ctx *ctx, double spacing);
+
 /* Atom-element coupling (no counterpart in the reference, where atoms see
  * only atoms, pair_eam.cpp:584-746, and integration points only element
  * sites, pair_eam.cpp:927-1137).  With on != 0 the next build adds to every
