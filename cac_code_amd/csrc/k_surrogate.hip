@@ -165,7 +165,9 @@ __device__ __forceinline__ void feature_group(int e, int g, const FeatArgs &a, c
 // FP32 into xf for the least-squares part; one wavefront per element
 __global__ void k_surr_features(int ns, const int *__restrict__ meta, FeatArgs a, const float *__restrict__ inv_in,
                                 uint16_t *__restrict__ x, int ldx, float *__restrict__ xf) {
-  const int r = blockIdx.x * (kBlock / 64) + (int)(threadIdx.x >> 6);
+  // wave-uniform row: the element's meta row, nodes, centroid and reference
+  // class are then read with scalar loads
+  const int r = __builtin_amdgcn_readfirstlane(blockIdx.x * (kBlock / 64) + (int)(threadIdx.x >> 6));
   if (r >= ns) return;
   const int *mr = meta + kMeta * (size_t)r;
   const int e = mr[0];
@@ -200,10 +202,10 @@ __device__ __forceinline__ int local_feature(int k, int i) {
 }
 
 // nodal forces of the surrogate elements.  Block: 64 elements (one per
-// lane) x 8 nodes (one per wavefront).  The elements' features and all
-// least-squares coefficients are staged in LDS; the node index is
-// wave-uniform, so the coefficient reads are same-address (broadcast) LDS
-// reads, 4 floats at a time from rows padded to 16 bytes.
+// lane) x 8 nodes (one per wavefront).  The elements' features are staged in
+// LDS; the node index is wave-uniform, so the least-squares coefficients are
+// read with wave-uniform addresses from global memory (scalar loads: the
+// multiply-adds take them straight from scalar registers).
 constexpr int kLdAq = 280, kLdQq = 564;  // padded coefficient rows (floats)
 
 __global__ void __launch_bounds__(512) k_surr_predict(int ns, const int *__restrict__ elemS,
@@ -213,33 +215,25 @@ __global__ void __launch_bounds__(512) k_surr_predict(int ns, const int *__restr
                                                       const float *__restrict__ out_scale,
                                                       double *__restrict__ nodef) {
   __shared__ __align__(16) float xs[kPredElems * kNFeat];
-  __shared__ __align__(16) float as[kNOut * kLdAq];
-  __shared__ __align__(16) float qs[kNOut * kLdQq];
   const int r0 = blockIdx.x * kPredElems;
   const int nrow = min(kPredElems, ns - r0);
-  // stage with independent 16-byte loads, all issued before the LDS stores
-  // (64 rows of 279 floats start on a 16-byte boundary; the last block's
-  // tail is copied by scalars); A and Q arrive padded to kLdAq / kLdQq
   {
-    constexpr int kX4 = kPredElems * kNFeat / 4, kA4 = kNOut * kLdAq / 4, kQ4 = kNOut * kLdQq / 4;
-    constexpr int kPer = (kX4 + kA4 + kQ4 + 511) / 512;
+    // 64 rows of 279 floats start on a 16-byte boundary; the last block's
+    // tail is copied by scalars
+    constexpr int kX4 = kPredElems * kNFeat / 4;
+    constexpr int kPer = (kX4 + 511) / 512;
     const int nx4 = nrow * kNFeat / 4;
     const float4 *sx = reinterpret_cast<const float4 *>(xf + (size_t)r0 * kNFeat);
-    const float4 *sa = reinterpret_cast<const float4 *>(A), *sq = reinterpret_cast<const float4 *>(Q);
     float4 v[kPer];
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
       const int i = threadIdx.x + 512 * u;
       if (i < nx4) v[u] = sx[i];
-      else if (i >= kX4 && i < kX4 + kA4) v[u] = sa[i - kX4];
-      else if (i >= kX4 + kA4 && i < kX4 + kA4 + kQ4) v[u] = sq[i - kX4 - kA4];
     }
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
       const int i = threadIdx.x + 512 * u;
       if (i < nx4) reinterpret_cast<float4 *>(xs)[i] = v[u];
-      else if (i >= kX4 && i < kX4 + kA4) reinterpret_cast<float4 *>(as)[i - kX4] = v[u];
-      else if (i >= kX4 + kA4 && i < kX4 + kA4 + kQ4) reinterpret_cast<float4 *>(qs)[i - kX4 - kA4] = v[u];
     }
     for (int i = 4 * nx4 + (int)threadIdx.x; i < nrow * kNFeat; i += 512) xs[i] = xf[(size_t)r0 * kNFeat + i];
   }
@@ -248,21 +242,19 @@ __global__ void __launch_bounds__(512) k_surr_predict(int ns, const int *__restr
   const int k = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   if (lane >= nrow) return;
   const float *x = xs + lane * kNFeat;
-  const float4 *A0 = reinterpret_cast<const float4 *>(as + 3 * k * kLdAq);
-  const float4 *A1 = A0 + kLdAq / 4, *A2 = A1 + kLdAq / 4;
+  const float *A0 = A + 3 * k * kLdAq, *A1 = A0 + kLdAq, *A2 = A1 + kLdAq;
   float a0 = b[3 * k], a1 = b[3 * k + 1], a2 = b[3 * k + 2];
-  for (int f4 = 0; f4 < kLdAq / 4; ++f4) {
-    const float4 c0 = A0[f4], c1 = A1[f4], c2 = A2[f4];
-    const int f = 4 * f4;
-    const float x0 = x[f], x1 = x[f + 1], x2 = x[f + 2], x3 = f + 3 < kNFeat ? x[f + 3] : 0.0f;
-    a0 = fmaf(c0.x, x0, a0); a0 = fmaf(c0.y, x1, a0); a0 = fmaf(c0.z, x2, a0); a0 = fmaf(c0.w, x3, a0);
-    a1 = fmaf(c1.x, x0, a1); a1 = fmaf(c1.y, x1, a1); a1 = fmaf(c1.z, x2, a1); a1 = fmaf(c1.w, x3, a1);
-    a2 = fmaf(c2.x, x0, a2); a2 = fmaf(c2.y, x1, a2); a2 = fmaf(c2.z, x2, a2); a2 = fmaf(c2.w, x3, a2);
+#pragma unroll 8
+  for (int f = 0; f < kNFeat; ++f) {
+    const float xv = x[f];
+    a0 = fmaf(A0[f], xv, a0);
+    a1 = fmaf(A1[f], xv, a1);
+    a2 = fmaf(A2[f], xv, a2);
   }
   float z[kNLoc];
 #pragma unroll
   for (int i = 0; i < kNLoc; ++i) z[i] = x[local_feature(k, i)];
-  const float *Q0 = qs + 3 * k * kLdQq, *Q1 = Q0 + kLdQq, *Q2 = Q1 + kLdQq;
+  const float *Q0 = Q + 3 * k * kLdQq, *Q1 = Q0 + kLdQq, *Q2 = Q1 + kLdQq;
   int t = 0;
 #pragma unroll
   for (int i = 0; i < kNLoc; ++i)

@@ -188,6 +188,70 @@ def generate_dataset(pair: PairEAM, n_samples: int, L: int = 33, nae: int = 11, 
                               temperature=temperature, k_node=knode, sigma=sigma, configurations=ncfg))
 
 
+def generate_dataset_trajectory(pair: PairEAM, system_fn, n_samples: int, seed: int = 1234,
+                                strain_max: float = 0.02, first: int = 100, every: int = 100,
+                                cutneigh: float = 5.95, coupled: bool = True, device: int = 0,
+                                log=None) -> Dataset:
+    """Plain-path (FP64, GPU) nodal forces of the deployment system itself:
+    snapshots of its NVE trajectory (velocities drawn with `seed`, so the
+    thermal state differs from any run evaluated with another seed), taken
+    after `first` steps and then every `every` steps, each labelled under a
+    random normal strain of magnitude uniform in [0, strain_max] (box, atoms
+    and nodes mapped affinely).  Only elements the surrogate would cover
+    (class 0: no atom in their lists, all mesh neighbors present, strain
+    within the envelope) become samples; features are taken against the
+    unstrained reference nodes."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    s = system_fn(seed)
+    X0 = s.nodex.copy()
+    nbr = mesh_neighbors(s)
+    lo = s.box_lo.copy()
+    prd = s.box_hi - s.box_lo
+    ne = s.n_elements
+    xs, ts = [], []
+    total = 0
+    nsnap = 0
+    t0 = time.perf_counter()
+    with Engine(device) as run, Engine(device) as lab:
+        for e in (run, lab):
+            e.load_system(s)
+            e.set_pair(pair)
+            e.set_neighbor(cutneigh, skin=1.0, every=1, delay=10)
+            e.set_coupling(coupled)
+        run.build()
+        run.force()
+        ones = np.ones(NIN, dtype=np.float32)
+        lab.surrogate_setup(X0, nbr, ones, np.ones(NOUT, dtype=np.float32), strain_limit=10.0)
+        steps = 0
+        while total < n_samples:
+            k = first if steps == 0 else every
+            run.run(k)
+            steps += k
+            x, _, _ = run.atoms()
+            nodex, _, _ = run.elements()
+            eps = rng.uniform(-1, 1, 3)
+            eps *= strain_max * rng.uniform(0, 1) / max(np.abs(eps).max(), 1e-12)
+            F = 1.0 + eps
+            lab.set_box(lo, lo + prd * F)
+            lab.upload_atoms(lo + (x - lo) * F, None, s.mass)
+            lab.upload_elements(lo + (nodex - lo) * F, None, s.nae, s.mass_e, s.intp_ijk, s.intp_w)
+            lab.build()
+            lab.force()
+            f, cls = lab.surrogate_features()
+            keep = cls == 0
+            xs.append(f[keep])
+            ts.append(lab.elements()[2].reshape(ne, 24)[keep].astype(np.float32))
+            total += int(keep.sum())
+            nsnap += 1
+    x = np.concatenate(xs)[:n_samples]
+    t = np.concatenate(ts)[:n_samples]
+    if log:
+        log(f"[surrogate] trajectory dataset: {x.shape[0]} samples from {nsnap} snapshots (steps {first}.."
+            f"{steps}, seed {seed}) in {time.perf_counter() - t0:.1f}s")
+    return Dataset(x, t, dict(kind="trajectory", seed=seed, strain_max=strain_max, first=first, every=every,
+                              snapshots=nsnap, steps=steps, coupled=coupled))
+
+
 @dataclass
 class Model:
     weights: list           # MLP layers [(W [out][in], b [out])], FP32 masters
@@ -270,10 +334,16 @@ def train(ds: Dataset, epochs: int = 60, batch: int = 1024, lr: float = 1e-3, se
             if log:
                 log(f"[surrogate] epoch {done}/{epochs} lr {rate:.2e} loss {loss:.3e}")
         w = eng.mlp_get_weights()
-    W, bl = w[-1]
-    w[-1] = ((W * r_scale[:, None]).astype(np.float32), (bl * r_scale).astype(np.float32))
+        W, bl = w[-1]
+        w[-1] = ((W * r_scale[:, None]).astype(np.float32), (bl * r_scale).astype(np.float32))
+        # a perfect lattice (every deviation zero) carries no nodal force:
+        # shift the constant term so the whole prediction vanishes at x = 0
+        eng.mlp_set_weights(w)
+        y0 = eng.mlp_forward(np.zeros((1, NIN), dtype=np.float32))[0].astype(np.float64)
+    b = (b.astype(np.float64) - (b.astype(np.float64) + y0)).astype(np.float32)
     if log:
-        log(f"[surrogate] MLP trained in {time.perf_counter() - t0:.1f}s")
+        log(f"[surrogate] MLP trained in {time.perf_counter() - t0:.1f}s (constant term set for zero force at "
+            f"zero deviation, shift {float(np.sqrt(np.mean(y0 ** 2))):.2e})")
     return Model(w, in_scale, out_scale, A, Q, b, hist)
 
 
