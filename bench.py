@@ -42,9 +42,10 @@ ALG_BYTES = {
 }
 B_ATOM = 616
 B_ELEM = 7640
-# HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by the
-# copy-kernel calibration), for the kernels profiled on this workload
-TRAFFIC = {}
+# HBM bytes per launch of the FP64 kernels on the eam_c1 workload, from
+# rocprofv3 FETCH_SIZE and WRITE_SIZE runs scaled by a plain 1 GiB copy's
+# counts (fetch x2.000, write x1.000); the summary is profiles/eam_c1_kernels.md
+TRAFFIC = {"atom_force": 187.6e6, "atom_rho": 194.8e6, "ip_force": 315.6e6, "ip_erho": 417.7e6}
 
 
 def log(msg):
@@ -358,7 +359,8 @@ def main():
         ms = prof[dom][0]
         ach = units * per / (ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": TRAFFIC.get(dom),
+                "frac": round(ach / HBM_PEAK_GBS, 4),
+                "traffic": TRAFFIC.get(dom) if (args.workload == "eam_c1" and args.precision == "fp64") else None,
                 "alg_bytes_per_launch": units * per, "avg_launch_us": round(ms * 1e3, 2)}
     b_step = B_ATOM * s.n_atoms + B_ELEM * s.n_elements
     step_roof = {"alg_bytes_per_step": b_step,
