@@ -91,6 +91,69 @@ __global__ void k_nve_nodes_final(long nq, double *__restrict__ nodev, const dou
   if (q < nq) nodev[q] += dtfm * nodef[q];
 }
 
+// Step boundary of the device-resident loop: FixNVE::final_integrate of the
+// previous step (when fin != 0) and initial_integrate of this one
+// (fix_nve.cpp:181-212, then :54-98) in one pass over v, f and x, with the
+// roundings of the separate kernels (v += dtfm*f twice, then x += dtv*v).
+// When flag != null the same pass runs Neighbor::check_distance
+// (neighbor.cpp:1176-1217) on the new positions and stores 1 to *flag (host
+// memory the caller waits on) if anything moved more than sqrt(triggersq).
+__device__ __forceinline__ void raise_flag(int *flag) {
+  __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+__global__ void k_nve_atoms_step(int n, double *__restrict__ x4, double *__restrict__ v,
+                                 const double *__restrict__ f, double dtv, double dtfm, int fin,
+                                 const double *__restrict__ xhold, double triggersq, int *flag) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double dsq = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    const double fd = f[(size_t)d * n + i];
+    double vv = v[(size_t)d * n + i];
+    if (fin) vv += dtfm * fd;
+    vv = vv + dtfm * fd;
+    v[(size_t)d * n + i] = vv;
+    const double xx = x4[4 * (size_t)i + d] + dtv * vv;
+    x4[4 * (size_t)i + d] = xx;
+    const double dd = xx - xhold[(size_t)d * n + i];
+    dsq = d == 0 ? dd * dd : dsq + dd * dd;
+  }
+  if (flag && dsq > triggersq) raise_flag(flag);
+}
+
+// Nodes: 8 lanes per element (one per node, 24 contiguous bytes each); the
+// center is the node mean summed in node order as in k_nve_nodes_initial.
+__global__ void k_nve_nodes_step(int ne, double *__restrict__ nodex, double *__restrict__ nodev,
+                                 const double *__restrict__ nodef, double *__restrict__ xc, double dtv, double dtfm,
+                                 int fin, const double *__restrict__ nodexhold, double triggersq, int *flag) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = (int)(t >> 3), k = (int)(t & 7);
+  const bool live = i < ne;
+  const size_t b = 24 * (size_t)(live ? i : 0) + 3 * k;
+  double xn[3] = {0.0, 0.0, 0.0};
+  double dsq = 0.0;
+  if (live) {
+    for (int d = 0; d < 3; ++d) {
+      const double fd = nodef[b + d];
+      double vv = nodev[b + d];
+      if (fin) vv += dtfm * fd;
+      vv = vv + dtfm * fd;
+      nodev[b + d] = vv;
+      xn[d] = nodex[b + d] + dtv * vv;
+      nodex[b + d] = xn[d];
+      const double dd = xn[d] - nodexhold[b + d];
+      dsq = d == 0 ? dd * dd : dsq + dd * dd;
+    }
+  }
+  for (int d = 0; d < 3; ++d) {
+    double c = 0.0;
+    for (int kk = 0; kk < 8; ++kk) c += __shfl(xn[d], kk, 8);
+    if (live && k == 0) xc[3 * (size_t)i + d] = c / 8;
+  }
+  if (live && flag && dsq > triggersq) raise_flag(flag);
+}
+
 // ---------------------------------------------------------------- reductions
 // Deterministic sums: block b of the grid reduces array b with a fixed
 // stride pattern and a fixed LDS tree.
@@ -135,7 +198,35 @@ int upload_tables(cac_ctx *ctx, int nrho, double drho, const double *frho_spline
       fr[8 * m + c] = frho_spline[7 * m + c];
       frf[8 * m + c] = (float)frho_spline[7 * m + c];
     }
+  // Knot form of rhor and z2r for the FP64 force passes: per knot m the
+  // value and 5-point slope (c6, c5) of each, 32 bytes, from which row m's
+  // c4 and c3 are recomputed with array2spline's own operations
+  // (pair_eam.cpp:527-548).  Used only when that recomputation reproduces
+  // every stored c3, c4 bit for bit (it does for tables built per the
+  // reference, as cac_eam_tables_from_funcfl builds them).
+  std::vector<double> rk(nrow * 4);
+  bool knots = nr >= 3;
+  for (size_t m = 0; m < nrow; ++m) {
+    rk[4 * m] = rhor_spline[7 * m + 6];
+    rk[4 * m + 1] = rhor_spline[7 * m + 5];
+    rk[4 * m + 2] = z2r_spline[7 * m + 6];
+    rk[4 * m + 3] = z2r_spline[7 * m + 5];
+  }
+  for (size_t m = 1; knots && m + 1 < nrow; ++m) {
+    const double *sp[2] = {rhor_spline, z2r_spline};
+    for (int q = 0; q < 2; ++q) {
+      const double f0 = sp[q][7 * m + 6], f1 = sp[q][7 * (m + 1) + 6];
+      const double s0 = sp[q][7 * m + 5], s1 = sp[q][7 * (m + 1) + 5];
+      const double c4 = 3.0 * (f1 - f0) - 2.0 * s0 - s1;
+      const double c3 = s0 + s1 - 2.0 * (f1 - f0);
+      if (std::memcmp(&c4, &sp[q][7 * m + 4], sizeof(double)) || std::memcmp(&c3, &sp[q][7 * m + 3], sizeof(double)))
+        knots = false;
+    }
+  }
   int rc;
+  if ((rc = ensure(ctx, t.b_rk, rk.size(), &t.rk))) return rc;
+  CAC_HIP(ctx, hipMemcpy(t.rk, rk.data(), rk.size() * sizeof(double), hipMemcpyHostToDevice));
+  t.knots = knots;
   if ((rc = ensure(ctx, t.b_rv, rv.size(), &t.rv)) || (rc = ensure(ctx, t.b_rz, rz.size(), &t.rz)) ||
       (rc = ensure(ctx, t.b_fr, fr.size(), &t.fr)) || (rc = ensure(ctx, t.b_rv_f, rvf.size(), &t.rv_f)) ||
       (rc = ensure(ctx, t.b_rz_f, rzf.size(), &t.rz_f)) || (rc = ensure(ctx, t.b_fr_f, frf.size(), &t.fr_f)))
@@ -261,6 +352,37 @@ int nve_initial_impl(cac_ctx *ctx, double dtv, double dtf) {
   return CAC_OK;
 }
 
+// Host-visible check flag (fine-grained pinned memory the check kernel
+// stores to) and the event that orders it, created on the first cac_run.
+int ensure_step_sync(cac_ctx *ctx) {
+  if (ctx->hcheck) return CAC_OK;
+  void *p = nullptr;
+  CAC_HIP(ctx, hipHostMalloc(&p, 64, hipHostMallocMapped | hipHostMallocCoherent));
+  ctx->hcheck = static_cast<volatile int *>(p);
+  *ctx->hcheck = 0;
+  void *dp = nullptr;
+  CAC_HIP(ctx, hipHostGetDevicePointer(&dp, p, 0));
+  ctx->dcheck = static_cast<int *>(dp);
+  CAC_HIP(ctx, hipEventCreateWithFlags(&ctx->ev_check, hipEventDisableTiming));
+  return CAC_OK;
+}
+
+// One step boundary of cac_run (see k_nve_atoms_step); fin: apply the
+// previous step's final half-step first.
+int nve_step_impl(cac_ctx *ctx, double dtv, double dtf, bool fin, int *flag) {
+  if (ctx->n)
+    hipLaunchKernelGGL(k_nve_atoms_step, dim3(grid_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->n, ctx->x4,
+                       ctx->v, ctx->f, dtv, dtf / ctx->mass, fin ? 1 : 0, ctx->xhold, ctx->triggersq, flag);
+  if (ctx->ne) {
+    double inv_npe = 1 / (double)8;
+    hipLaunchKernelGGL(k_nve_nodes_step, dim3(grid_for(8L * ctx->ne)), dim3(kBlock), 0, ctx->stream, ctx->ne,
+                       ctx->nodex, ctx->nodev, ctx->nodef, ctx->xc, dtv, dtf / (inv_npe * ctx->mass_e), fin ? 1 : 0,
+                       ctx->nodexhold, ctx->triggersq, flag);
+  }
+  CAC_HIP(ctx, hipGetLastError());
+  return CAC_OK;
+}
+
 int nve_final_impl(cac_ctx *ctx, double dtf) {
   if (ctx->n)
     hipLaunchKernelGGL(k_nve_atoms_final, dim3(grid_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->n, ctx->v,
@@ -357,12 +479,14 @@ int cac_ctx_destroy(cac_ctx *ctx) {
                     &ctx->b_eval2, &ctx->b_ecs, &ctx->b_ece, &ctx->b_ipneigh, &ctx->b_ipnum, &ctx->b_erho,
                     &ctx->b_iptmp, &ctx->b_bitmap, &ctx->b_wprefix, &ctx->b_ecount, &ctx->b_estart, &ctx->b_asite,
                     &ctx->b_scantmp, &ctx->b_iflag, &ctx->b_ipos, &ctx->b_iface, &ctx->b_asnum, &ctx->b_atmp,
-                    &ctx->b_etouch,
+                    &ctx->b_etouch, &ctx->b_amask, &ctx->b_ipmask,
                     &ctx->b_ipf, &ctx->b_eip, &ctx->b_vip, &ctx->b_red, &ctx->b_dflags, &ctx->eam.b_rz,
-                    &ctx->eam.b_fr, &ctx->eam.b_rz_f, &ctx->eam.b_fr_f, &ctx->eam.b_rv, &ctx->eam.b_rv_f};
+                    &ctx->eam.b_fr, &ctx->eam.b_rz_f, &ctx->eam.b_fr_f, &ctx->eam.b_rv, &ctx->eam.b_rv_f, &ctx->eam.b_rk};
   for (DevBuf *b : bufs)
     if (b->p) hipFree(b->p);
   if (ctx->hflags) hipHostFree(ctx->hflags);
+  if (ctx->hcheck) hipHostFree(const_cast<int *>(ctx->hcheck));
+  if (ctx->ev_check) hipEventDestroy(ctx->ev_check);
   for (auto &kv : ctx->timers)
     for (auto &pr : kv.second.pending) {
       hipEventDestroy(pr.first);
@@ -517,6 +641,21 @@ int cac_set_quadrature(cac_ctx *ctx, int nint, const int *ijk, const double *w) 
     if (g == nint) return fail(ctx, CAC_ERR_DATA, "didn't contain all node integration point");
     ctx->node_list[k] = g;
   }
+  // Every integration point is a node point with shape values exactly 0
+  // and 1: the density pass then visits every integration point's list from
+  // the same position as the force pass, so it can compact the lists.
+  {
+    bool on_nodes = nint == 8;
+    bool seen[8] = {false, false, false, false, false, false, false, false};
+    for (int k = 0; on_nodes && k < 8; ++k) {
+      const int g = ctx->node_list[k];
+      if (seen[g]) on_nodes = false;
+      seen[g] = true;
+      for (int kk = 0; kk < 8; ++kk)
+        if (spa[8 * (size_t)g + kk] != (kk == k ? 1.0 : 0.0)) on_nodes = false;
+    }
+    ctx->ip_on_nodes = on_nodes;
+  }
   int rc;
   if ((rc = ensure(ctx, ctx->b_spa_int, spa.size(), &ctx->spa_int)) ||
       (rc = ensure(ctx, ctx->b_weight, (size_t)nint, &ctx->weight)))
@@ -625,19 +764,42 @@ int cac_run(cac_ctx *ctx, int nsteps, double dtv, double dtf, int precision, int
   hipSetDevice(ctx->device);
   int rc = ready_for_force(ctx);
   if (rc) return rc;
+  if ((rc = ensure_step_sync(ctx))) return rc;
   int builds = 0;
-  // Verlet::run, verlet.cpp:149-219
+  // Verlet::run, verlet.cpp:149-219.  Same sequence as nve_initial, decide,
+  // [build], force, nve_final per step, arranged so the device never waits
+  // for the host:
+  //  - the final half-step of step s-1 and the initial half-step of step s
+  //    are one kernel (the force does not change between them), which also
+  //    runs the displacement check when Neighbor::decide would;
+  //  - the check's flag goes to host memory, and the host waits for it only
+  //    after queueing the step's force on the current lists; when the flag
+  //    asks for a rebuild that force is discarded: the lists are rebuilt
+  //    and the force recomputed before the final half-step, as in the
+  //    reference order.
   for (int step = 0; step < nsteps; ++step) {
-    if ((rc = nve_initial_impl(ctx, dtv, dtf))) return rc;
-    int rebuild = 0;
-    if ((rc = decide_impl(ctx, &rebuild))) return rc;
-    if (rebuild) {
+    // Neighbor::decide, neighbor.cpp:1150-1163
+    ctx->ago++;
+    const bool due = ctx->ago >= ctx->delay && ctx->ago % ctx->every == 0;
+    const bool probe = due && ctx->check;
+    if (probe) *ctx->hcheck = 0;
+    if ((rc = nve_step_impl(ctx, dtv, dtf, step > 0, probe ? ctx->dcheck : nullptr))) return rc;
+    if (probe) CAC_HIP(ctx, hipEventRecord(ctx->ev_check, ctx->stream));
+    if (due && !ctx->check) {
       if ((rc = build_impl(ctx))) return rc;
       ++builds;
     }
     if ((rc = force_impl(ctx, 0, 0, precision))) return rc;
-    if ((rc = nve_final_impl(ctx, dtf))) return rc;
+    if (probe) {
+      CAC_HIP(ctx, hipEventSynchronize(ctx->ev_check));
+      if (*ctx->hcheck) {
+        if ((rc = build_impl(ctx))) return rc;
+        ++builds;
+        if ((rc = force_impl(ctx, 0, 0, precision))) return rc;
+      }
+    }
   }
+  if (nsteps > 0 && (rc = nve_final_impl(ctx, dtf))) return rc;
   if (nbuilds) *nbuilds = builds;
   return CAC_OK;
 }

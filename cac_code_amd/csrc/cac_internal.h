@@ -72,6 +72,11 @@ struct EamTables {
   double *rv = nullptr, *rz = nullptr, *fr = nullptr;
   float *rv_f = nullptr, *rz_f = nullptr, *fr_f = nullptr;
   DevBuf b_rv, b_rz, b_fr, b_rv_f, b_rz_f, b_fr_f;
+  // knot form [(nr+1)][4] = rhor c6, c5, z2r c6, c5 (FP64 force passes);
+  // knots: row c3, c4 recompute bit-exactly from it
+  double *rk = nullptr;
+  bool knots = false;
+  DevBuf b_rk;
 };
 
 struct MlpState;
@@ -131,6 +136,10 @@ struct cac_ctx {
   uint32_t *aneigh = nullptr;
   int *anum = nullptr;
   cac::DevBuf b_aneigh, b_anum;
+  // in-range bits of each atom row (r^2 < cutforcesq), written by the
+  // density pass of every EAM force call for its force pass
+  uint32_t *amask = nullptr;
+  cac::DevBuf b_amask;
   // atom cells (kept from the last build: the coupled integration-point
   // search reads them)
   int acell[3] = {0, 0, 0};
@@ -182,6 +191,12 @@ struct cac_ctx {
   uint32_t *ipneigh = nullptr;
   int *ipnum = nullptr;
   cac::DevBuf b_iptmp, b_ipneigh, b_ipnum;
+  // in-range bits of the integration-point rows from the node-density pass
+  // (only when every integration point is a node point, ip_on_nodes), per
+  // lane of the 8 serving a row
+  bool ip_on_nodes = false;
+  uint32_t *ipmask = nullptr;
+  cac::DevBuf b_ipmask;
   // active sites: the (element, site) pairs some list refers to; per step
   // xs4 = (x, y, z, fp) of each, interpolated once
   int nactive = 0, nwords = 0;
@@ -218,6 +233,11 @@ struct cac_ctx {
   // [7..10] first out-of-range list entry (set, row, slot, entry)
   int *dflags = nullptr;
   int *hflags = nullptr;  // pinned mirror
+  // cac_run's rebuild check: pinned host flag (host and device addresses)
+  // and the event recorded after the kernel that writes it
+  volatile int *hcheck = nullptr;
+  int *dcheck = nullptr;
+  hipEvent_t ev_check = nullptr;
   cac::DevBuf b_dflags;
 
   // ---- learned element surrogate (k_mlp.hip, k_surrogate.hip)

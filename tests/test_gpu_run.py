@@ -1,0 +1,113 @@
+"""The device-resident Verlet loop (cac_run) against the same step sequence
+issued call by call through the C-ABI, and against the oracle's NVE.
+
+cac_run fuses the final half-step of one step with the initial half-step of
+the next, writes the rebuild check to host memory and queues the force before
+reading it (recomputing after a rebuild).  None of that may change a bit:
+the call-by-call sequence (nve_initial, decide, [build], force, nve_final per
+step; verlet.cpp:149-219) must give identical positions, velocities and
+rebuild counts, with rebuilds forced often by a small skin.
+"""
+import numpy as np
+import pytest
+
+from cac_code_amd import Engine, PairEAM, _lib
+from cac_code_amd.lattice import FTM2V_METAL
+
+from gpu_common import CUTNEIGH_EAM, rel_err, small_system
+
+pytestmark = pytest.mark.gpu
+
+DT = 0.001
+DTF = 0.5 * DT * FTM2V_METAL
+
+
+def eam_pair(cu_funcfl):
+    p = PairEAM()
+    p.settings([])
+    p.coeff(["1", "1", "Cu_johnson.eam"], funcfl=cu_funcfl)
+    return p
+
+
+def engine(s, pair, skin, every, delay, check, coupled):
+    eng = Engine(0)
+    eng.load_system(s)
+    eng.set_pair(pair)
+    eng.set_neighbor(CUTNEIGH_EAM, skin=skin, every=every, delay=delay, check=check)
+    eng.set_coupling(coupled)
+    eng.build()
+    eng.force()
+    return eng
+
+
+def stepwise(eng, nsteps, precision):
+    builds = 0
+    for _ in range(nsteps):
+        eng.nve_initial(DT, DTF)
+        if eng.decide():
+            eng.build()
+            builds += 1
+        eng.force(precision=precision)
+        eng.nve_final(DTF)
+    return builds
+
+
+def state(eng):
+    x, v, f = eng.atoms()
+    nx, nv, nf = eng.elements()
+    return [x, v, f, nx, nv, nf]
+
+
+@pytest.mark.parametrize("precision", [_lib.PREC_FP64, _lib.PREC_MIXED], ids=["fp64", "mixed"])
+@pytest.mark.parametrize("every,delay,check", [(1, 0, True), (2, 4, True), (5, 0, False)],
+                         ids=["every1", "every2_delay4", "nocheck5"])
+def test_run_equals_stepwise_api(cu_funcfl, precision, every, delay, check):
+    s = small_system(seed=21, L=10, nae=5, core=4.0, sigma=0.02, temperature=600.0)
+    pair = eam_pair(cu_funcfl)
+    nsteps = 30
+    print(f"[run] prec={precision} every={every} delay={delay} check={check}: atoms={s.n_atoms} "
+          f"elements={s.n_elements}", flush=True)
+    # trigger 0.04 A: rebuilds every few steps at 600 K
+    a = engine(s, pair, 0.08, every, delay, check, True)
+    b = engine(s, pair, 0.08, every, delay, check, True)
+    try:
+        na = a.run(nsteps, dt=DT, precision=precision)
+        nb = stepwise(b, nsteps, precision)
+        assert na == nb and na > 0, (na, nb)
+        for qa, qb, name in zip(state(a), state(b), ["x", "v", "f", "nodex", "nodev", "nodef"]):
+            assert np.array_equal(qa, qb), name
+    finally:
+        a.close()
+        b.close()
+
+
+def test_run_matches_oracle_nve(orc, cu_funcfl, cu_tables):
+    """Five coupled EAM steps without a rebuild against the oracle's FixNVE
+    and forces (1e-10 norm-relative)."""
+    s = small_system(seed=23, L=10, nae=5, core=4.0, sigma=0.03, temperature=300.0)
+    pair = eam_pair(cu_funcfl)
+    eng = engine(s, pair, 1.0, 1, 0, True, True)
+    try:
+        nb = eng.run(5, dt=DT)
+        assert nb == 0
+        got = state(eng)
+    finally:
+        eng.close()
+    es = orc.ElementSet(s.nodex.copy(), s.nae, s.intp_ijk, s.intp_w)
+    x, v, nodev = s.x.copy(), s.v.copy(), s.nodev.copy()
+    anum, anb, inum, inb = orc.coupled_neighbors(x, es, s.box_lo, s.box_hi, CUTNEIGH_EAM, 2, 200, 200)
+
+    def force():
+        r = orc.eam_coupled(x, es, s.box_lo, s.box_hi, anum, anb, inum, inb, cu_tables)
+        return r["f"], r["nodef"]
+
+    f, nodef = force()
+    xc = np.zeros((s.n_elements, 3))
+    for _ in range(5):
+        orc.nve_atoms_initial(x, v, f, DT, DTF, s.mass)
+        orc.nve_nodes_initial(es.nodex, nodev, nodef, xc, DT, DTF, s.mass_e)
+        f, nodef = force()
+        orc.nve_atoms_final(v, f, DTF, s.mass)
+        orc.nve_nodes_final(nodev, nodef, DTF, s.mass_e)
+    for g, r, name in zip(got, [x, v, f, es.nodex, nodev, nodef], ["x", "v", "f", "nodex", "nodev", "nodef"]):
+        assert rel_err(g, r) < 1e-10, name
