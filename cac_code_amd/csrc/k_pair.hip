@@ -231,12 +231,39 @@ __device__ __forceinline__ double embed_energy(const Tabs<P> &t, double rho, dou
   return e;
 }
 
+// r and 1/r of a pair from r^2: the hardware reciprocal square root refined
+// by Newton-Raphson steps for 1/sqrt(x), y <- y + y*(1 - x*y^2)/2 (two in
+// FP64, one in FP32; each step squares the relative error), then r = x*y.
+// Within a few ulps of sqrt and 1/sqrt, for a fraction of the instructions
+// of a correctly rounded square root and division.  Pair distances here are
+// far from zero, overflow and denormals.
 template <class P>
-__device__ __forceinline__ P sqrt_p(double rsq);
+struct RInvP {
+  P r, rinv;
+};
+template <class P>
+__device__ __forceinline__ RInvP<P> r_and_inv(double rsq);
 template <>
-__device__ __forceinline__ double sqrt_p<double>(double rsq) { return sqrt(rsq); }
+__device__ __forceinline__ RInvP<double> r_and_inv<double>(double rsq) {
+  double y = __builtin_amdgcn_rsq(rsq);
+  double e = __builtin_fma(-(rsq * y), y, 1.0);
+  y = __builtin_fma(0.5 * y, e, y);
+  e = __builtin_fma(-(rsq * y), y, 1.0);
+  y = __builtin_fma(0.5 * y, e, y);
+  return RInvP<double>{rsq * y, y};
+}
 template <>
-__device__ __forceinline__ float sqrt_p<float>(double rsq) { return sqrtf((float)rsq); }
+__device__ __forceinline__ RInvP<float> r_and_inv<float>(double rsq) {
+  const float x = (float)rsq;
+  float y = __builtin_amdgcn_rsqf(x);
+  const float e = __builtin_fmaf(-(x * y), y, 1.0f);
+  y = __builtin_fmaf(0.5f * y, e, y);
+  return RInvP<float>{x * y, y};
+}
+template <class P>
+__device__ __forceinline__ P sqrt_p(double rsq) {
+  return r_and_inv<P>(rsq).r;
+}
 
 // Sum over the 8 lanes of a group; the xor butterfly pairs the same partial
 // sums on every lane, so all lanes hold identical bits.
@@ -326,9 +353,10 @@ struct Virial {
 template <class P>
 __device__ __forceinline__ double eam_fpair(const Tabs<P> &t, const Win<P> &w, double rsq, P fpi, P fpj,
                                             P *phi_out) {
-  P r = sqrt_p<P>(rsq), rhoip, z2p, z2;
+  const RInvP<P> rr = r_and_inv<P>(rsq);
+  P r = rr.r, rhoip, z2p, z2;
   rz_of_r_w<P>(t, w, r, &rhoip, &z2p, &z2);
-  P recip = P(1.0) / r;
+  P recip = rr.rinv;
   P phi = z2 * recip;
   P phip = z2p * recip - phi * recip;
   P psip = fpi * rhoip + fpj * rhoip + phip;
@@ -482,6 +510,7 @@ __global__ __launch_bounds__(kAtomWG) void k_eam_atom_force(int n, const double 
 }
 
 // ------------------------------------------------------------- atoms, LJ
+
 // PairLJCut::compute, pair_lj_cut.cpp:255-287
 template <class P, bool EV>
 __global__ __launch_bounds__(kBlock) void k_lj_atom_force(int n, const double *__restrict__ x4, Prd prd,
@@ -707,9 +736,10 @@ __global__ __launch_bounds__(kBlock) void k_eam_ip_force(ElemArgs a, Prd prd, Ta
       if (rsq < t.cutforcesq) {
         P phi;
         // psip = (fpi + fpj)*rhoip + phip, pair_eam.cpp:1118
-        P r = sqrt_p<P>(rsq), rhoip, z2p, z2;
+        const RInvP<P> rr = r_and_inv<P>(rsq);
+        P r = rr.r, rhoip, z2p, z2;
         rz_of_r<P>(t, r, &rhoip, &z2p, &z2);
-        P recip = P(1.0) / r;
+        P recip = rr.rinv;
         phi = z2 * recip;
         P phip = z2p * recip - phi * recip;
         P psip = (fpi + (P)fpj) * rhoip + phip;
