@@ -80,22 +80,36 @@ def make_pair(workload):
 COUPLED = True  # atom-element coupling (set from --no-coupling in main)
 
 
-def gpu_run(s, pair, cutneigh, skin, steps, warmup, precision, every, delay):
-    eng = new_engine(s, pair, cutneigh, skin, every, delay)
-    nb_warm = eng.run(warmup, precision=precision) if warmup else 0
+KERNELS = ("atom_rho", "atom_force", "site_pos", "ip_erho", "site_fp", "ip_force", "scatter", "surr_feat", "surr_mlp",
+           "surr_fit", "atom_neigh", "ip_neigh", "elem_neigh", "atom_sites")
+
+
+def kernel_profile(eng, steps, precision):
+    """Per-kernel average durations (HIP events on the context stream) over
+    `steps` further steps; kept out of the timed region, whose clock must not
+    include the event bookkeeping."""
     eng.synchronize()
     eng.profile(True)
-    t0 = time.perf_counter()
-    nb = eng.run(steps, precision=precision)
+    eng.run(steps, precision=precision)
     eng.synchronize()
-    t1 = time.perf_counter()
     prof = {}
-    for name in ("atom_rho", "atom_force", "site_pos", "ip_erho", "site_fp", "ip_force", "scatter", "atom_neigh",
-                 "ip_neigh", "elem_neigh", "atom_sites"):
+    for name in KERNELS:
         ms, cnt = eng.profile_avg_ms(name)
         if cnt:
             prof[name] = (ms, cnt)
     eng.profile(False)
+    return prof
+
+
+def gpu_run(s, pair, cutneigh, skin, steps, warmup, precision, every, delay, prof_steps=100):
+    eng = new_engine(s, pair, cutneigh, skin, every, delay)
+    nb_warm = eng.run(warmup, precision=precision) if warmup else 0
+    eng.synchronize()
+    t0 = time.perf_counter()
+    nb = eng.run(steps, precision=precision)
+    eng.synchronize()
+    t1 = time.perf_counter()
+    prof = kernel_profile(eng, min(steps, prof_steps), precision)
     return eng, t1 - t0, nb, nb_warm, prof
 
 
@@ -468,18 +482,11 @@ def surrogate_variant(s, pair, cutneigh, skin, args, n_int, secs_fp64):
     eng.force(precision=_lib.PREC_MIXED)
     nb_w = eng.run(args.warmup, precision=_lib.PREC_MIXED) if args.warmup else 0
     eng.synchronize()
-    eng.profile(True)
     t1 = time.perf_counter()
     nb = eng.run(args.steps, precision=_lib.PREC_MIXED)
     eng.synchronize()
     secs = time.perf_counter() - t1
-    prof = {}
-    for name in ("atom_rho", "atom_force", "site_pos", "ip_erho", "site_fp", "ip_force", "scatter", "surr_feat",
-                 "surr_mlp", "surr_fit", "atom_neigh", "ip_neigh", "elem_neigh", "atom_sites"):
-        ms, cnt = eng.profile_avg_ms(name)
-        if cnt:
-            prof[name] = round(ms * 1e3, 1)
-    eng.profile(False)
+    prof = {k: round(v[0] * 1e3, 1) for k, v in kernel_profile(eng, min(args.steps, 100), _lib.PREC_MIXED).items()}
     info = eng.surrogate_info()
     eng.close()
     ne = s.n_elements

@@ -315,6 +315,7 @@ int build_impl(cac_ctx *ctx) {
     if (ctx->hflags[4]) { ctx->maxnip = ctx->hflags[5] + 16; again = true; }
     if (again) continue;
     if ((rc = launch_active_sites(ctx))) return rc;
+    if ((rc = launch_clusters(ctx))) return rc;
     if (ctx->surr.enabled && (rc = surrogate_classify(ctx))) return rc;
     ctx->built = true;
     ctx->ago = 0;
@@ -479,7 +480,8 @@ int cac_ctx_destroy(cac_ctx *ctx) {
                     &ctx->b_eval2, &ctx->b_ecs, &ctx->b_ece, &ctx->b_ipneigh, &ctx->b_ipnum, &ctx->b_erho,
                     &ctx->b_iptmp, &ctx->b_bitmap, &ctx->b_wprefix, &ctx->b_ecount, &ctx->b_estart, &ctx->b_asite,
                     &ctx->b_scantmp, &ctx->b_iflag, &ctx->b_ipos, &ctx->b_iface, &ctx->b_asnum, &ctx->b_atmp,
-                    &ctx->b_etouch, &ctx->b_amask, &ctx->b_ipmask,
+                    &ctx->b_etouch, &ctx->b_amask, &ctx->b_ipmask, &ctx->b_cl_start, &ctx->b_cl_slot,
+                    &ctx->b_cl_union, &ctx->b_cl_nu, &ctx->b_iplocal, &ctx->b_aref, &ctx->b_amark, &ctx->b_cl_order,
                     &ctx->b_ipf, &ctx->b_eip, &ctx->b_vip, &ctx->b_red, &ctx->b_dflags, &ctx->eam.b_rz,
                     &ctx->eam.b_fr, &ctx->eam.b_rz_f, &ctx->eam.b_fr_f, &ctx->eam.b_rv, &ctx->eam.b_rv_f, &ctx->eam.b_rk};
   for (DevBuf *b : bufs)
@@ -723,6 +725,44 @@ int cac_set_coupling(cac_ctx *ctx, int on) {
   if (!ctx) return CAC_ERR_ARG;
   if (ctx->coupled != (on != 0)) ctx->built = false;
   ctx->coupled = on != 0;
+  return CAC_OK;
+}
+
+int cac_set_ip_clusters(cac_ctx *ctx, int ncl, const int *start, const int *slots) {
+  if (!ctx || ncl < 0 || (ncl > 0 && (!start || !slots))) return CAC_ERR_ARG;
+  hipSetDevice(ctx->device);
+  ctx->built = false;
+  ctx->cl_ok = false;
+  ctx->ncl = 0;
+  if (ncl == 0) return CAC_OK;
+  const long nslot = 8L * ctx->ne;
+  if (start[0] != 0 || start[ncl] != nslot)
+    return fail(ctx, CAC_ERR_ARG, "clusters must cover the 8 node slots of every element");
+  std::vector<char> seen((size_t)nslot, 0);
+  for (int c = 0; c < ncl; ++c) {
+    const int sz = start[c + 1] - start[c];
+    if (sz < 1 || sz > kClMax) return fail(ctx, CAC_ERR_ARG, "a cluster holds 1 to 8 node slots");
+    for (int q = start[c]; q < start[c + 1]; ++q) {
+      if (slots[q] < 0 || slots[q] >= nslot || seen[slots[q]])
+        return fail(ctx, CAC_ERR_ARG, "every node slot must appear in exactly one cluster");
+      seen[slots[q]] = 1;
+    }
+  }
+  int rc;
+  if ((rc = ensure(ctx, ctx->b_cl_start, (size_t)ncl + 1, &ctx->cl_start)) ||
+      (rc = ensure(ctx, ctx->b_cl_slot, (size_t)nslot, &ctx->cl_slot)))
+    return rc;
+  CAC_HIP(ctx, hipMemcpy(ctx->cl_start, start, ((size_t)ncl + 1) * sizeof(int), hipMemcpyHostToDevice));
+  CAC_HIP(ctx, hipMemcpy(ctx->cl_slot, slots, (size_t)nslot * sizeof(int), hipMemcpyHostToDevice));
+  ctx->ncl = ncl;
+  return CAC_OK;
+}
+
+int cac_cluster_info(cac_ctx *ctx, int *ncl, int *active, int *max_union) {
+  if (!ctx) return CAC_ERR_ARG;
+  if (ncl) *ncl = ctx->ncl;
+  if (active) *active = ctx->cl_ok ? 1 : 0;
+  if (max_union) *max_union = ctx->cl_umax;
   return CAC_OK;
 }
 

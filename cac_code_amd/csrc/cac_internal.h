@@ -37,6 +37,9 @@ constexpr int kRowGroup = 8;
 // ((i/64)*M + k)*64 + i%64, so the 64 threads of a wavefront (one atom each)
 // read 64 consecutive entries per step.
 constexpr int kAtomGroup = 64;
+// Vertex clusters of integration points: at most 8 node slots per cluster.
+constexpr int kClMax = 8;
+constexpr int kUSmall = 256;
 __host__ __device__ __forceinline__ size_t atom_slot(int i, int k, int M) {
   return ((size_t)(i >> 6) * M + k) * kAtomGroup + (i & 63);
 }
@@ -197,6 +200,27 @@ struct cac_ctx {
   bool ip_on_nodes = false;
   uint32_t *ipmask = nullptr;
   cac::DevBuf b_ipmask;
+  // vertex clusters (cac_set_ip_clusters): node slots 8e+k meeting at a mesh
+  // vertex, CSR.  Per build: each cluster's union of list entries
+  // (cl_union [ncl][cl_stride], count cl_nu, largest cl_umax) and every
+  // integration-point list rewritten as indices into its cluster's union
+  // (iplocal, the ipneigh layout); cl_ok when built and the largest union
+  // fits the LDS budget of the cluster passes.  aref: the active sites atom
+  // lists refer to (coupling), the only ones still interpolated into x4.
+  int ncl = 0;
+  int *cl_start = nullptr, *cl_slot = nullptr;
+  uint32_t *cl_union = nullptr;
+  int *cl_nu = nullptr;
+  uint16_t *iplocal = nullptr;
+  int cl_stride = 0, cl_umax = 0;
+  // launch order: clusters with unions of at most kUSmall entries first
+  // (cl_nsmall of them, largest cl_umax_small), then the rest
+  int *cl_order = nullptr;
+  int cl_nsmall = 0, cl_umax_small = 0;
+  bool cl_ok = false;
+  int naref = 0;
+  int *aref = nullptr, *amark = nullptr;
+  cac::DevBuf b_cl_start, b_cl_slot, b_cl_union, b_cl_nu, b_iplocal, b_aref, b_amark, b_cl_order;
   // active sites: the (element, site) pairs some list refers to; per step
   // xs4 = (x, y, z, fp) of each, interpolated once
   int nactive = 0, nwords = 0;
@@ -317,6 +341,7 @@ int launch_element_neighbors(cac_ctx *ctx);
 int launch_ip_neighbors(cac_ctx *ctx);
 int launch_atom_sites(cac_ctx *ctx);
 int launch_active_sites(cac_ctx *ctx);
+int launch_clusters(cac_ctx *ctx);
 int validate_lists(cac_ctx *ctx);
 int launch_check_displacement(cac_ctx *ctx);
 

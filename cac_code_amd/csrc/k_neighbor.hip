@@ -1229,6 +1229,102 @@ int launch_active_sites(cac_ctx *ctx) {
   return validate_lists(ctx);
 }
 
+// ------------------------------------------------------------ clusters
+// Per vertex cluster (one wavefront): the entries of its integration points'
+// lists (<= 8 rows of the blocked ipneigh layout) are sorted in LDS, the
+// distinct values become the cluster's union (ascending: atoms first, then
+// active sites element by element), and every list entry is rewritten as
+// its index in the union (uint16, same layout).  The largest union size
+// goes to flags[12]: it sets the LDS rows of the cluster passes.
+struct NodeList8 {
+  int k[8];
+};
+
+__global__ __launch_bounds__(64) void k_cl_build(int ncl, const int *__restrict__ cl_start,
+                                                 const int *__restrict__ cl_slot, int nint, NodeList8 nl,
+                                                 const uint32_t *__restrict__ ipneigh, const int *__restrict__ ipnum,
+                                                 int M, int cap2, uint32_t *__restrict__ cl_union,
+                                                 int *__restrict__ cl_nu, uint16_t *__restrict__ iplocal,
+                                                 int *__restrict__ flags) {
+  extern __shared__ uint64_t keys[];  // [cap2] sort keys, then [cap2] uint32 union
+  uint32_t *uni = reinterpret_cast<uint32_t *>(keys + cap2);
+  const int c = blockIdx.x;
+  if (c >= ncl) return;
+  const int lane = (int)threadIdx.x;
+  const int s0 = cl_start[c], ns = cl_start[c + 1] - s0;
+  int pid[kClMax], off[kClMax + 1];
+  off[0] = 0;
+#pragma unroll
+  for (int i = 0; i < kClMax; ++i) {
+    pid[i] = 0;
+    int cnt = 0;
+    if (i < ns) {
+      const int slot = cl_slot[s0 + i];
+      pid[i] = (slot >> 3) * nint + nl.k[slot & 7];
+      cnt = min(ipnum[pid[i]], M);
+    }
+    off[i + 1] = off[i] + cnt;
+  }
+  const int m = off[kClMax];
+  auto entry_ptr = [&](int p, int k) -> size_t { return ((size_t)(p >> 3) * M + k) * kRowGroup + (p & 7); };
+  for (int t = lane; t < cap2; t += 64) {
+    uint64_t v = ~0ull;
+    if (t < m) {
+      int i = 0;
+      while (t >= off[i + 1]) ++i;
+      v = ipneigh[entry_ptr(pid[i], t - off[i])];
+    }
+    keys[t] = v;
+  }
+  __syncthreads();
+  bitonic_sort_lds(keys, cap2, lane);
+  int u = 0;
+  for (int t0 = 0; t0 < m; t0 += 64) {
+    const int t = t0 + lane;
+    const bool first = t < m && (t == 0 || keys[t] != keys[t - 1]);
+    const unsigned long long bm = __ballot(first);
+    if (first) uni[u + __popcll(bm & ((1ull << lane) - 1ull))] = (uint32_t)keys[t];
+    u += __popcll(bm);
+  }
+  __syncthreads();
+  if (lane == 0) {
+    cl_nu[c] = u;
+    atomicMax(&flags[12], u);
+  }
+  for (int t = lane; t < u; t += 64) cl_union[(size_t)c * cap2 + t] = uni[t];
+  for (int t = lane; t < m; t += 64) {
+    int i = 0;
+    while (t >= off[i + 1]) ++i;
+    const size_t q = entry_ptr(pid[i], t - off[i]);
+    const uint32_t v = ipneigh[q];
+    int lo = 0, hi = u;  // first union index with value >= v
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (uni[mid] < v) lo = mid + 1; else hi = mid;
+    }
+    iplocal[q] = (uint16_t)lo;
+  }
+}
+
+// Active sites that atom lists refer to (atom-element coupling): marks, then
+// the ascending list of marked ids.
+__global__ void k_mark_aref(int niface, int maxneigh, const int *__restrict__ iface, const int *__restrict__ anum,
+                            const uint32_t *__restrict__ aneigh, int n, int *__restrict__ mark) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)niface * maxneigh) return;
+  const int r = (int)(t / maxneigh), k = (int)(t % maxneigh);
+  const int i = iface[r];
+  if (k >= anum[i]) return;
+  const uint32_t j = aneigh[atom_slot(i, k, maxneigh)] & 0x7FFFFFFu;
+  if ((int)j >= n) mark[j - n] = 1;
+}
+
+__global__ void k_compact_marks(int na, const int *__restrict__ mark, const int *__restrict__ pos,
+                                int *__restrict__ out) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < na && mark[q]) out[pos[q]] = q;
+}
+
 // Check both lists after a build (rows of x4: n atoms + nactive sites).
 int validate_lists(cac_ctx *ctx) {
   const uint32_t nrow = (uint32_t)(ctx->n + ctx->nactive);
@@ -1258,6 +1354,81 @@ int validate_lists(cac_ctx *ctx) {
   return CAC_OK;
 }
 
+
+// Cluster unions and union-index lists after a build (see k_cl_build), and
+// the active sites atom lists refer to.
+int launch_clusters(cac_ctx *ctx) {
+  ctx->cl_ok = false;
+  ctx->naref = 0;
+  const int nip = ctx->ne * ctx->nint;
+  if (ctx->ncl == 0 || !ctx->ip_on_nodes || nip == 0) return CAC_OK;
+  int rc;
+  const int M = ctx->maxnip;
+  int cap2 = 64;
+  while (cap2 < kClMax * M) cap2 <<= 1;
+  const size_t lds = (size_t)cap2 * (sizeof(uint64_t) + sizeof(uint32_t));
+  if (lds > ctx->lds_bytes) return CAC_OK;  // lists too long for the cluster passes
+  if ((rc = ensure(ctx, ctx->b_cl_union, (size_t)ctx->ncl * cap2, &ctx->cl_union)) ||
+      (rc = ensure(ctx, ctx->b_cl_nu, (size_t)ctx->ncl, &ctx->cl_nu)) ||
+      (rc = ensure(ctx, ctx->b_iplocal, (size_t)M * round_up(nip, kRowGroup), &ctx->iplocal)))
+    return rc;
+  ctx->cl_stride = cap2;
+  NodeList8 nl;
+  for (int k = 0; k < 8; ++k) nl.k[k] = ctx->node_list[k];
+  CAC_HIP(ctx, hipMemsetAsync(ctx->dflags + 11, 0, 2 * sizeof(int), ctx->stream));
+  CAC_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_cl_build),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(k_cl_build, dim3(ctx->ncl), dim3(64), lds, ctx->stream, ctx->ncl, ctx->cl_start, ctx->cl_slot,
+                     ctx->nint, nl, ctx->ipneigh, ctx->ipnum, M, cap2, ctx->cl_union, ctx->cl_nu, ctx->iplocal,
+                     ctx->dflags);
+  CAC_HIP(ctx, hipGetLastError());
+  if (ctx->coupled && ctx->niface > 0 && ctx->nactive > 0) {
+    const int na = ctx->nactive;
+    int *pos;
+    // marks [0, na] (the last one stays 0) and their exclusive sums [na+1, 2na+2)
+    if ((rc = ensure(ctx, ctx->b_amark, 2 * (size_t)na + 2, &ctx->amark)) ||
+        (rc = ensure(ctx, ctx->b_aref, (size_t)na, &ctx->aref)))
+      return rc;
+    pos = ctx->amark + na + 1;
+    CAC_HIP(ctx, hipMemsetAsync(ctx->amark, 0, ((size_t)na + 1) * sizeof(int), ctx->stream));
+    const long tot = (long)ctx->niface * ctx->maxneigh;
+    hipLaunchKernelGGL(k_mark_aref, dim3(grid_for(tot)), dim3(kBlock), 0, ctx->stream, ctx->niface, ctx->maxneigh,
+                       ctx->iface, ctx->anum, ctx->aneigh, ctx->n, ctx->amark);
+    size_t tmp_bytes = 0;
+    CAC_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, ctx->amark, pos, na + 1, ctx->stream));
+    char *tmp;
+    if ((rc = ensure(ctx, ctx->b_scantmp, tmp_bytes, &tmp))) return rc;
+    CAC_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, ctx->amark, pos, na + 1, ctx->stream));
+    hipLaunchKernelGGL(k_compact_marks, dim3(grid_for(na)), dim3(kBlock), 0, ctx->stream, na, ctx->amark, pos,
+                       ctx->aref);
+    CAC_HIP(ctx, hipMemcpyAsync(&ctx->naref, pos + na, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  }
+  int fl[2] = {0, 0};
+  CAC_HIP(ctx, hipMemcpyAsync(fl, ctx->dflags + 11, sizeof(fl), hipMemcpyDeviceToHost, ctx->stream));
+  CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->cl_umax = fl[1];
+  // LDS rows of 32 bytes per union entry, one wavefront per cluster: past
+  // 48 KB a CU would hold too few wavefronts to hide the gathers
+  if ((size_t)fl[1] * 32 > 48 * 1024) return CAC_OK;
+  // launch order: small unions first (see kUSmall)
+  std::vector<int> nu(ctx->ncl), order;
+  order.reserve(ctx->ncl);
+  CAC_HIP(ctx, hipMemcpy(nu.data(), ctx->cl_nu, (size_t)ctx->ncl * sizeof(int), hipMemcpyDeviceToHost));
+  int small_max = 0;
+  for (int c = 0; c < ctx->ncl; ++c)
+    if (nu[c] <= kUSmall) {
+      order.push_back(c);
+      small_max = std::max(small_max, nu[c]);
+    }
+  ctx->cl_nsmall = (int)order.size();
+  for (int c = 0; c < ctx->ncl; ++c)
+    if (nu[c] > kUSmall) order.push_back(c);
+  ctx->cl_umax_small = small_max;
+  if ((rc = ensure(ctx, ctx->b_cl_order, (size_t)ctx->ncl, &ctx->cl_order))) return rc;
+  CAC_HIP(ctx, hipMemcpy(ctx->cl_order, order.data(), (size_t)ctx->ncl * sizeof(int), hipMemcpyHostToDevice));
+  ctx->cl_ok = true;
+  return CAC_OK;
+}
 
 int launch_check_displacement(cac_ctx *ctx) {
   long tot = (long)ctx->n + 8L * ctx->ne;

@@ -574,6 +574,8 @@ struct ElemArgs {
   const int *ipnum;          // [nip]
   const int2 *asite;         // [nactive] (element, site)
   int node_list[8];
+  int nae[3];
+  double pstep[3];           // parent-coordinate step 2/(nae-1) per direction
   // optional subsets (surrogate mode): elements for the erho / force /
   // scatter passes, active sites for the site passes; null = all
   const int *elist;
@@ -774,6 +776,260 @@ __global__ __launch_bounds__(kBlock) void k_eam_ip_force(ElemArgs a, Prd prd, Ta
   }
 }
 
+// ------------------------------------------------------------- clusters
+// Element passes by vertex cluster (one wavefront per cluster of <= 8
+// integration points at one mesh vertex, whose lists overlap almost
+// entirely).  The cluster's union of list entries (built at rebuild) is evaluated once into LDS rows (x, y, z, fp): an atom row is read
+// from x4, a site is interpolated from its element's nodes (and, for the
+// force pass, its density from the element's node densities, then F'),
+// with exactly k_site_pos's and k_site_fp's operations, so the rows equal
+// the x4 rows those kernels would write.  The 8 lanes of each integration
+// point then walk their list entries l, l+8, ... as in k_eam_erho /
+// k_eam_ip_force, reading union indices (uint16) and LDS rows instead of
+// gathering x4 rows from HBM, and sum in the same order: the same bits.
+struct ClArgs {
+  int ncl, n, stride;
+  const int *start, *slot;
+  const uint32_t *uni;     // [ncl][stride]
+  const int *nu;
+  const uint16_t *local;   // ipneigh layout
+  const int *order;        // clusters of this launch
+  int count;
+  int rows;                // LDS rows of this launch (its largest union)
+};
+
+// Shape functions of site s, computed as the host computes the site table
+// (Element::inter_ele + shape_array_inter_ele, element.cpp:1361-1423: the
+// same operations in the same order, so the same bits as spa_site).
+__device__ __forceinline__ void site_shape(const ElemArgs &a, int s, double N[8]) {
+  const int n12 = a.nae[1] * a.nae[2];
+  const int ii = s / n12, rem = s - ii * n12, jj = rem / a.nae[2], kk = rem - jj * a.nae[2];
+  const double px = a.pstep[0] * (double)ii - 1, py = a.pstep[1] * (double)jj - 1,
+               pz = a.pstep[2] * (double)kk - 1;
+  N[0] = 0.125 * (1 - px) * (1 - py) * (1 - pz);
+  N[1] = 0.125 * (1 + px) * (1 - py) * (1 - pz);
+  N[2] = 0.125 * (1 + px) * (1 + py) * (1 - pz);
+  N[3] = 0.125 * (1 - px) * (1 + py) * (1 - pz);
+  N[4] = 0.125 * (1 - px) * (1 - py) * (1 + pz);
+  N[5] = 0.125 * (1 + px) * (1 - py) * (1 + pz);
+  N[6] = 0.125 * (1 + px) * (1 + py) * (1 + pz);
+  N[7] = 0.125 * (1 - px) * (1 + py) * (1 + pz);
+}
+
+// LDS of a cluster wavefront: rows [c.rows][4] (x, y, z, fp of every union
+// entry), then the nodes [8][24] and node densities [8][8] of the cluster's
+// own elements (the elements whose nodes meet at the vertex: nearly every
+// union site lies in one of them).
+template <class P, bool FP>
+__device__ __forceinline__ void cl_fill(const ClArgs &c, int cl, int s0, int ns, const ElemArgs &a, const Prd &prd,
+                                        const Tabs<P> &t, const double *__restrict__ x4,
+                                        const double *__restrict__ erho, double *rows, int lane) {
+  double *nodes = rows + 4 * c.rows, *enode = nodes + kClMax * 24;
+  int el[kClMax];
+#pragma unroll
+  for (int i = 0; i < kClMax; ++i) el[i] = i < ns ? (c.slot[s0 + i] >> 3) : -1;
+  for (int q = lane; q < ns * 24; q += 64) nodes[q] = a.nodex[24 * (size_t)el[q / 24] + q % 24];
+  if (FP)
+    for (int q = lane; q < ns * 8; q += 64) enode[q] = erho[8 * (size_t)el[q / 8] + q % 8];
+  __syncthreads();
+  const int nu = c.nu[cl];
+  for (int u = lane; u < nu; u += 64) {
+    const uint32_t e = c.uni[(size_t)cl * c.stride + u];
+    const int j = (int)(e & 0x7FFFFFFu);
+    double x, y, z, w = 0.0;
+    if (j < c.n) {
+      const double2 xy = reinterpret_cast<const double2 *>(x4)[2 * (size_t)j];
+      const double2 zw = reinterpret_cast<const double2 *>(x4)[2 * (size_t)j + 1];
+      x = xy.x;
+      y = xy.y;
+      z = zw.x;
+      w = zw.y;
+    } else {
+      const int2 js = a.asite[j - c.n];
+      double N[8];
+      site_shape(a, js.y, N);
+      int li = -1;
+#pragma unroll
+      for (int i = 0; i < kClMax; ++i)
+        if (el[i] == js.x) li = i;
+      const double *X = li >= 0 ? nodes + 24 * li : a.nodex + 24 * (size_t)js.x;
+      x = 0.0;
+      y = 0.0;
+      z = 0.0;
+      for (int k = 0; k < 8; ++k) {
+        x += N[k] * X[3 * k];
+        y += N[k] * X[3 * k + 1];
+        z += N[k] * X[3 * k + 2];
+      }
+      if (FP) {
+        const double *E = li >= 0 ? enode + 8 * li : erho + 8 * (size_t)js.x;
+        double rho = 0.0;
+        for (int k = 0; k < 8; ++k) rho += N[k] * E[k];
+        w = (double)embed_fp<P>(t, rho);
+      }
+    }
+    const int img = (int)(e >> 27);
+    if (img != kImgSelf) {
+      int s[3];
+      img_shift(img, s);
+      x = shifted(x, s[0], prd.v[0]);
+      y = shifted(y, s[1], prd.v[1]);
+      z = shifted(z, s[2], prd.v[2]);
+    }
+    double *r = rows + 4 * u;
+    r[0] = x;
+    r[1] = y;
+    r[2] = z;
+    r[3] = w;
+  }
+}
+
+template <class P>
+__global__ __launch_bounds__(64) void k_cl_erho(ClArgs c, ElemArgs a, Prd prd, Tabs<P> t,
+                                                const double *__restrict__ px4, double *__restrict__ erho,
+                                                uint32_t *__restrict__ mask) {
+  extern __shared__ __align__(16) double rows[];
+  const int o = xcd_block();
+  if (o >= c.count) return;
+  const int cl = c.order[o];
+  const int lane = (int)threadIdx.x;
+  const int s0 = c.start[cl], ns = c.start[cl + 1] - s0;
+  cl_fill<P, false>(c, cl, s0, ns, a, prd, t, px4, nullptr, rows, lane);
+  __syncthreads();
+  const int gi = lane >> 3, l = lane & 7;
+  double acc = 0.0;
+  int slot = 0;
+  if (gi < ns) {
+    slot = c.slot[s0 + gi];
+    const int i = slot >> 3, k = slot & 7;
+    const double ix = a.nodex[24 * (size_t)i + 3 * k], iy = a.nodex[24 * (size_t)i + 3 * k + 1],
+                 iz = a.nodex[24 * (size_t)i + 3 * k + 2];
+    const int p = i * a.nint + a.node_list[k];
+    const int jn = a.ipnum[p];
+    const uint16_t *row = c.local + (size_t)(p >> 3) * a.M * kRowGroup + (p & 7);
+    const int Wl = (a.M + 8 * 32 - 1) / (8 * 32);
+    uint32_t *mrow = mask + ((size_t)p * kLanes + l) * Wl;
+    uint32_t bits = 0;
+    for (int k0 = l, j0 = 0; k0 < jn; k0 += kU * kLanes, j0 += kU) {
+      int q[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int kk = k0 + u * kLanes;
+        q[u] = kk < jn ? (int)row[(size_t)kk * kRowGroup] : -1;
+      }
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        if (q[u] < 0) continue;
+        const double *r = rows + 4 * q[u];
+        const double delx = ix - r[0], dely = iy - r[1], delz = iz - r[2];
+        const double rsq = delx * delx + dely * dely + delz * delz;
+        if (rsq < t.cutforcesq) {
+          acc += (double)rho_of_r<P>(t, sqrt_p<P>(rsq));
+          bits |= 1u << ((j0 + u) & 31);
+        }
+      }
+      if (((j0 + kU) & 31) == 0 || k0 + kU * kLanes >= jn) {
+        mrow[j0 >> 5] = bits;
+        bits = 0;
+      }
+    }
+  }
+  acc = group_sum(acc);
+  if (gi < ns && l == 0) erho[slot] = acc;
+}
+
+template <class P, bool EV>
+__global__ __launch_bounds__(64) void k_cl_force(ClArgs c, ElemArgs a, Prd prd, Tabs<P> t,
+                                                 const double *__restrict__ erho, const double *__restrict__ px4,
+                                                 double *__restrict__ ipf, double *__restrict__ eip,
+                                                 double *__restrict__ vip, const uint32_t *__restrict__ mask) {
+  extern __shared__ __align__(16) double rows[];
+  const int o = xcd_block();
+  if (o >= c.count) return;
+  const int cl = c.order[o];
+  const int lane = (int)threadIdx.x;
+  const int s0 = c.start[cl], ns = c.start[cl + 1] - s0;
+  cl_fill<P, true>(c, cl, s0, ns, a, prd, t, px4, erho, rows, lane);
+  __syncthreads();
+  const int gi = lane >> 3, l = lane & 7;
+  double fx = 0.0, fy = 0.0, fz = 0.0, ep = 0.0, rhoi = 0.0;
+  P fpi = P(0);
+  Virial vir;
+  vir.zero();
+  int p = 0;
+  if (gi < ns) {
+    const int slot = c.slot[s0 + gi];
+    const int i = slot >> 3, g = a.node_list[slot & 7];
+    p = i * a.nint + g;
+    const double *Ng = a.spa_int + 8 * g;
+    double ix = 0.0, iy = 0.0, iz = 0.0;
+    for (int kk = 0; kk < 8; ++kk) {
+      ix += Ng[kk] * a.nodex[24 * (size_t)i + 3 * kk];
+      iy += Ng[kk] * a.nodex[24 * (size_t)i + 3 * kk + 1];
+      iz += Ng[kk] * a.nodex[24 * (size_t)i + 3 * kk + 2];
+      rhoi += Ng[kk] * erho[8 * (size_t)i + kk];
+    }
+    fpi = embed_fp<P>(t, rhoi);
+    const int jn = a.ipnum[p];
+    const uint16_t *row = c.local + (size_t)(p >> 3) * a.M * kRowGroup + (p & 7);
+    const int Wl = (a.M + 8 * 32 - 1) / (8 * 32);
+    const uint32_t *mrow = mask + ((size_t)p * kLanes + l) * Wl;
+    uint32_t bits = 0;
+    for (int k0 = l, j0 = 0; k0 < jn; k0 += kU * kLanes, j0 += kU) {
+      if ((j0 & 31) == 0) bits = mrow[j0 >> 5];
+      const uint32_t sel = bits >> (j0 & 31);
+      int q[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int kk = k0 + u * kLanes;
+        q[u] = (kk < jn && ((sel >> u) & 1u)) ? (int)row[(size_t)kk * kRowGroup] : -1;
+      }
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        if (q[u] < 0) continue;
+        const double *r = rows + 4 * q[u];
+        const double delx = ix - r[0], dely = iy - r[1], delz = iz - r[2];
+        const double rsq = delx * delx + dely * dely + delz * delz;
+        if (rsq < t.cutforcesq) {
+          P phi;
+          const RInvP<P> rr = r_and_inv<P>(rsq);
+          P rhoip, z2p, z2;
+          rz_of_r<P>(t, rr.r, &rhoip, &z2p, &z2);
+          P recip = rr.rinv;
+          phi = z2 * recip;
+          P phip = z2p * recip - phi * recip;
+          P psip = (fpi + (P)r[3]) * rhoip + phip;
+          const double fpair = (double)(-psip * recip);
+          fx += delx * fpair;
+          fy += dely * fpair;
+          fz += delz * fpair;
+          if (EV) {
+            ep += 0.5 * (double)phi;
+            vir.add(delx, dely, delz, fpair);
+          }
+        }
+      }
+    }
+  }
+  fx = group_sum(fx);
+  fy = group_sum(fy);
+  fz = group_sum(fz);
+  if (EV) {
+    ep = group_sum(ep);
+    vir.reduce();
+  }
+  if (gi < ns && l == 0) {
+    ipf[p] = fx;
+    ipf[(size_t)a.nip + p] = fy;
+    ipf[2 * (size_t)a.nip + p] = fz;
+    if (EV) {
+      const double w = a.weight[p % a.nint];
+      eip[p] = w * (embed_energy<P>(t, rhoi, (double)fpi) + ep);
+      vir.store(vip, (size_t)a.nip, (size_t)p, w);
+    }
+  }
+}
+
 // LJ integration-point force, pair_lj_cut.cpp:355-422.
 template <class P, bool EV>
 __global__ __launch_bounds__(kBlock) void k_lj_ip_force(ElemArgs a, Prd prd, LjParams lj,
@@ -915,6 +1171,10 @@ ElemArgs elem_args(const cac_ctx *ctx) {
   a.ipnum = ctx->ipnum;
   a.asite = ctx->asite;
   for (int k = 0; k < 8; ++k) a.node_list[k] = ctx->node_list[k];
+  for (int d = 0; d < 3; ++d) {
+    a.nae[d] = ctx->nae[d];
+    a.pstep[d] = 2 / ((double)ctx->nae[d] - 1);
+  }
   a.elist = nullptr;
   a.nelist = 0;
   a.slist = nullptr;
@@ -1008,6 +1268,94 @@ int ensure_masks(cac_ctx *ctx) {
   return CAC_OK;
 }
 
+ClArgs cl_args(const cac_ctx *ctx) {
+  ClArgs c;
+  c.ncl = ctx->ncl;
+  c.n = ctx->n;
+  c.stride = ctx->cl_stride;
+  c.start = ctx->cl_start;
+  c.slot = ctx->cl_slot;
+  c.uni = ctx->cl_union;
+  c.nu = ctx->cl_nu;
+  c.local = ctx->iplocal;
+  return c;
+}
+
+// The two launches of a cluster pass: clusters with small unions (most of
+// them) and the rest, each with the LDS rows its largest union needs, so
+// the few large unions at interfaces do not lower the occupancy of all.
+ClArgs cl_part(const cac_ctx *ctx, ClArgs c, int part) {
+  c.order = ctx->cl_order + (part ? ctx->cl_nsmall : 0);
+  c.count = part ? ctx->ncl - ctx->cl_nsmall : ctx->cl_nsmall;
+  c.rows = part ? ctx->cl_umax : ctx->cl_umax_small;
+  if (c.rows < 1) c.rows = 1;
+  return c;
+}
+size_t cl_lds(const ClArgs &c) { return ((size_t)4 * c.rows + kClMax * (24 + 8)) * sizeof(double); }
+
+// One EAM force call with the element passes by vertex cluster (every
+// element in the passes, the cluster structures built): only the active
+// sites that atom lists refer to are still interpolated into x4.
+template <class P, bool EV>
+int eam_clusters(cac_ctx *ctx, const ElemArgs &base) {
+  Tabs<P> t = tabs_of<P>(ctx);
+  const int n = ctx->n;
+  const unsigned agrid = xcd_grid((long)n, kAtomWG);
+  const TabWindow wr = table_window(ctx, 4 * sizeof(P));
+  const TabWindow wz = t.rk ? knot_window(ctx) : table_window(ctx, 8 * sizeof(P));
+  hipEvent_t t0;
+  int rc;
+  if ((rc = ensure_masks(ctx))) return rc;
+  const ElemArgs aref = with_sites(base, ctx->aref, ctx->naref);
+  const ClArgs c = cl_args(ctx);
+
+  if (ctx->naref) site_positions(ctx, aref);
+  if (n) {
+    CAC_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_eam_atom_rho<P, EV>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)wr.bytes));
+    CAC_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_eam_atom_force<P, EV>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)wz.bytes));
+    timer_begin(ctx, "atom_rho", &t0);
+    hipLaunchKernelGGL((k_eam_atom_rho<P, EV>), dim3(agrid), dim3(kAtomWG), wr.bytes, ctx->stream, n, ctx->x4,
+                       prd_of(ctx), ctx->aneigh, ctx->maxneigh, ctx->anum, t, wr.m0, ctx->rho, ctx->eatom,
+                       ctx->amask);
+    timer_end(ctx, "atom_rho", t0);
+  }
+  timer_begin(ctx, "ip_erho", &t0);
+  for (int part = 0; part < 2; ++part) {
+    const ClArgs cp = cl_part(ctx, c, part);
+    if (cp.count)
+      hipLaunchKernelGGL((k_cl_erho<P>), dim3(xcd_grid((long)cp.count * 64, 64)), dim3(64), cl_lds(cp), ctx->stream,
+                         cp, base, prd_of(ctx), t, ctx->x4, ctx->erho, ctx->ipmask);
+  }
+  timer_end(ctx, "ip_erho", t0);
+  if (ctx->naref) {
+    timer_begin(ctx, "site_fp", &t0);
+    hipLaunchKernelGGL((k_site_fp<P>), dim3(xcd_grid(ctx->naref, kBlock)), dim3(kBlock), 0, ctx->stream, aref, t,
+                       ctx->erho, ctx->xs4);
+    timer_end(ctx, "site_fp", t0);
+  }
+  if (n) {
+    timer_begin(ctx, "atom_force", &t0);
+    hipLaunchKernelGGL((k_eam_atom_force<P, EV>), dim3(agrid), dim3(kAtomWG), wz.bytes, ctx->stream, n, ctx->x4,
+                       prd_of(ctx), ctx->aneigh, ctx->maxneigh, ctx->anum, t, wz.m0, ctx->f, ctx->eatom,
+                       ctx->vatom, ctx->amask);
+    timer_end(ctx, "atom_force", t0);
+  }
+  timer_begin(ctx, "ip_force", &t0);
+  for (int part = 0; part < 2; ++part) {
+    const ClArgs cp = cl_part(ctx, c, part);
+    if (cp.count)
+      hipLaunchKernelGGL((k_cl_force<P, EV>), dim3(xcd_grid((long)cp.count * 64, 64)), dim3(64), cl_lds(cp),
+                         ctx->stream, cp, base, prd_of(ctx), t, ctx->erho, ctx->x4, ctx->ipf, ctx->eip, ctx->vip,
+                         ctx->ipmask);
+  }
+  timer_end(ctx, "ip_force", t0);
+  scatter(ctx, base);
+  CAC_HIP(ctx, hipGetLastError());
+  return CAC_OK;
+}
+
 template <class P, bool EV>
 int eam_all(cac_ctx *ctx, const ElemArgs &base, const ElemArgs &spos, const ElemArgs &erho_el, const ElemArgs &sfp,
             const ElemArgs &force_el) {
@@ -1074,6 +1422,7 @@ int eam_t(cac_ctx *ctx) {
     if (rc) return rc;
     return surrogate_forces(ctx);
   }
+  if (ctx->ne && ctx->cl_ok) return eam_clusters<P, EV>(ctx, a);
   return eam_all<P, EV>(ctx, a, a, a, a, a);
 }
 

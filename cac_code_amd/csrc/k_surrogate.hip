@@ -63,8 +63,11 @@ struct Prd3 {
   double v[3];
 };
 
-__device__ __forceinline__ double min_image(double d, double prd) {
-  return d - prd * rint(d / prd);
+// nearest periodic image of a separation d along an axis of period prd
+// (1/prd precomputed): only the choice of image matters, and d*(1/prd)
+// picks the same one as d/prd except at exactly half a period
+__device__ __forceinline__ double min_image(double d, double prd, double rprd) {
+  return d - prd * rint(d * rprd);
 }
 
 __device__ __forceinline__ uint16_t to_bf16(float f) {
@@ -96,7 +99,7 @@ struct FeatArgs {
   const double *nodex, *xbar, *ref;
   const int *nbr, *rcls;
   CornerTab ct;
-  Prd3 prd;
+  Prd3 prd, rprd;
 };
 
 // Per surrogate row, built at classification: element, reference class and
@@ -143,14 +146,14 @@ __device__ __forceinline__ void feature_group(int e, int g, const FeatArgs &a, c
     const int m = g - 8, j = nb[m];
     have = j >= 0;
     if (have)
-      for (int d = 0; d < 3; ++d) v[d] = min_image(a.xbar[3 * (size_t)j + d] - xb[d], a.prd.v[d]);
+      for (int d = 0; d < 3; ++d) v[d] = min_image(a.xbar[3 * (size_t)j + d] - xb[d], a.prd.v[d], a.rprd.v[d]);
     r = ref + kRefD0 + 3 * m;
   } else if (g < 90) {
     const int q = g - 34, j = nb[a.ct.m[q]];
     have = j >= 0;
     if (have)
       for (int d = 0; d < 3; ++d)
-        v[d] = min_image(a.nodex[24 * (size_t)j + 3 * a.ct.node[q] + d] - xb[d], a.prd.v[d]);
+        v[d] = min_image(a.nodex[24 * (size_t)j + 3 * a.ct.node[q] + d] - xb[d], a.prd.v[d], a.rprd.v[d]);
     r = ref + kRefC0 + 3 * q;
   } else {
     const int i = g - 90;
@@ -205,7 +208,9 @@ __device__ __forceinline__ int local_feature(int k, int i) {
 // lane) x 8 nodes (one per wavefront).  The elements' features are staged in
 // LDS; the node index is wave-uniform, so the least-squares coefficients are
 // read with wave-uniform addresses from global memory (scalar loads: the
-// multiply-adds take them straight from scalar registers).
+// multiply-adds take them straight from scalar registers).  (Staging the
+// coefficients in LDS as well measured slower: the 149 KB then allow one
+// block per CU.)
 constexpr int kLdAq = 280, kLdQq = 564;  // padded coefficient rows (floats)
 
 __global__ void __launch_bounds__(512) k_surr_predict(int ns, const int *__restrict__ elemS,
@@ -394,7 +399,9 @@ int frames(cac_ctx *ctx, FeatArgs *a) {
   int rc;
   if ((rc = ensure(ctx, S.b_ubar, 3 * (size_t)ne, &S.ubar))) return rc;
   hipLaunchKernelGGL(k_surr_centroid, dim3(grid_for(3L * ne)), dim3(kBlock), 0, ctx->stream, ne, ctx->nodex, S.ubar);
-  *a = FeatArgs{ctx->nodex, S.ubar, S.ref, S.nbr, S.rcls, corner_table(), prd_of(ctx)};
+  Prd3 rp;
+  for (int d = 0; d < 3; ++d) rp.v[d] = 1.0 / ctx->prd[d];
+  *a = FeatArgs{ctx->nodex, S.ubar, S.ref, S.nbr, S.rcls, corner_table(), prd_of(ctx), rp};
   return CAC_OK;
 }
 

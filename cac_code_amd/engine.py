@@ -21,7 +21,7 @@ from typing import Optional
 import numpy as np
 
 from . import _lib
-from .lattice import CACSystem, FTM2V_METAL
+from .lattice import CACSystem, FTM2V_METAL, vertex_clusters
 from .potential import Funcfl, lj_coeffs, read_funcfl
 
 SURR_NFEAT, SURR_NOUT, SURR_NQUAD = 279, 24, 561   # CAC_SURR_* in cac_hip.h
@@ -302,6 +302,25 @@ class Engine(_MlpMixin):
         self.upload_elements(s.nodex, s.nodev, s.nae, mass_e, s.intp_ijk, s.intp_w)
         if getattr(s, "a0", None):
             self.set_list_order(0.5 * s.a0)                 # fcc neighbor vectors: multiples of a/2
+        cl = vertex_clusters(s)
+        if cl is not None:
+            self.set_ip_clusters(*cl)
+
+    def set_ip_clusters(self, start, slots):
+        """Group the integration points by mesh vertex (a work grouping for
+        the element passes; the forces do not change).  start[ncl+1], slots:
+        node slots 8*e + k per cluster."""
+        start = np.ascontiguousarray(start, dtype=np.int32)
+        slots = np.ascontiguousarray(slots, dtype=np.int32)
+        _check(self.ctx, self.L.cac_set_ip_clusters(self.ctx, int(start.size - 1), _p(start, _lib._i),
+                                                    _p(slots, _lib._i)))
+
+    def cluster_info(self):
+        """(clusters, cluster passes active after the last build, largest
+        union of a cluster's lists)."""
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        _check(self.ctx, self.L.cac_cluster_info(self.ctx, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, bool(b.value), c.value
 
     def set_pair(self, pair):
         self.pair = pair

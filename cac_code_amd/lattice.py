@@ -158,6 +158,32 @@ def build_cac_system(L: int, nae: int, core_cells: float, a: float = 3.615, mass
     return sysm
 
 
+def vertex_clusters(s: CACSystem):
+    """Node slots 8*e + k grouped by mesh vertex, as (start[ncl+1], slots).
+    Node k of the element of block B (corner flags c_k) sits next to the
+    block-grid vertex V = B + c_k: the 8 elements meeting at V put their
+    nodes on the lattice points nae*V - c' around it.  Vertices are matched
+    by the lattice point nae*V wrapped into the periodic box (integer
+    arithmetic in units of a/2); clusters are ordered by their first slot."""
+    if s.blocks is None or s.n_elements == 0:
+        return None
+    nae = int(s.nae[0])
+    V = s.blocks[:, None, :].astype(np.int64) + NODE_CORNERS[None, :, :]          # [ne, 8, 3]
+    n = nae * V.reshape(-1, 3)
+    c2 = np.stack([n[:, 0] + n[:, 1], n[:, 0] + n[:, 2], n[:, 1] + n[:, 2]], axis=1)  # a/2 units
+    c2 = np.mod(c2, 2 * s.cells)
+    key = (c2[:, 2] * (2 * s.cells) + c2[:, 1]) * (2 * s.cells) + c2[:, 0]
+    order = np.argsort(key, kind="stable")
+    ks = key[order]
+    cut = np.flatnonzero(np.diff(ks)) + 1
+    groups = np.split(order, cut)
+    groups.sort(key=lambda g: int(g.min()))
+    start = np.zeros(len(groups) + 1, dtype=np.int32)
+    start[1:] = np.cumsum([len(g) for g in groups])
+    slots = np.concatenate([np.sort(g) for g in groups]).astype(np.int32)
+    return start, slots
+
+
 def assign_velocities(s: CACSystem, temperature: float, seed: int) -> None:
     """Gaussian velocities at `temperature` over atoms and nodes, zero total
     momentum, exact temperature after rescale (build-defined)."""

@@ -122,6 +122,23 @@ int cac_set_list_order(cac_ctx *ctx, double spacing);
  * integration points).  Off by default. */
 int cac_set_coupling(cac_ctx *ctx, int on);
 
+/* Vertex clusters of the element integration points (no counterpart in the
+ * reference: a grouping of work, the forces do not change).  Cluster c holds
+ * the node slots slots[start[c] .. start[c+1]), slot 8*e + k naming node k
+ * of element e, of the elements meeting at one mesh vertex: at most 8 per
+ * cluster, every slot of every element in exactly one cluster.  When set
+ * and every integration point is a node point, the node-density and
+ * integration-point force passes run one cluster per wavefront: the union of
+ * the cluster's neighbor lists is interpolated once into LDS and the
+ * integration-point sums read it there.  ncl = 0 clears it.  Takes effect at
+ * the next build. */
+int cac_set_ip_clusters(cac_ctx *ctx, int ncl, const int *start, const int *slots);
+
+/* State of the vertex-cluster element passes after the last build: *active
+ * = 1 when the force calls use them, *max_union = the largest union of a
+ * cluster's lists (its LDS rows).  No counterpart in the reference. */
+int cac_cluster_info(cac_ctx *ctx, int *ncl, int *active, int *max_union);
+
 /* Neighbor rebuild: Domain::pbc/elepbc + Neighbor::build/ebuild/nbuild
  * (verlet.cpp:173-193; neighbor.cpp:546-625, element_neighbor.cpp:11-75,
  * full_node_neighbor.cpp:19-239).  Regrows capacities on overflow. */
