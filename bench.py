@@ -44,8 +44,10 @@ B_ATOM = 616
 B_ELEM = 7640
 # HBM bytes per launch of the FP64 kernels on the eam_c1 workload, from
 # rocprofv3 FETCH_SIZE and WRITE_SIZE runs scaled by a plain 1 GiB copy's
-# counts (fetch x2.000, write x1.000); the summary is profiles/eam_c1_kernels.md
-TRAFFIC = {"atom_force": 187.6e6, "atom_rho": 194.8e6, "ip_force": 315.6e6, "ip_erho": 417.7e6}
+# counts (fetch x2.000, write x1.000); the summary is profiles/eam_c1_kernels.md.
+# The cluster passes (ip_erho, ip_force) are two launches per call: the sum.
+TRAFFIC = {"atom_force": 197.9e6, "atom_rho": 206.9e6, "ip_force": 2 * (71.3e6 + 4.5e6),
+           "ip_erho": 2 * (63.1e6 + 4.8e6)}
 
 
 def log(msg):
@@ -318,6 +320,8 @@ def main():
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the mixed-precision and surrogate lines, accuracy and drift")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--surrogate-only", action="store_true",
+                    help="only train the surrogate and time mixed precision + surrogate (profiling runs)")
     ap.add_argument("--surrogate-samples", type=int, default=200000)
     ap.add_argument("--surrogate-epochs", type=int, default=400)
     ap.add_argument("--surrogate-data", choices=["trajectory", "box"], default="box",
@@ -353,6 +357,11 @@ def main():
     log(f"[bench] workload {args.workload}: atoms={s.n_atoms} elements={s.n_elements} "
         f"(built in {time.perf_counter() - t0:.1f}s)")
     prec = _lib.PREC_FP64 if args.precision == "fp64" else _lib.PREC_MIXED
+    if args.surrogate_only:
+        n_int = s.n_atoms + 8 * s.n_elements
+        _, rec = surrogate_variant(s, pair, cutneigh, skin, args, n_int, float("nan"))
+        print(json.dumps({"surrogate": rec}), flush=True)
+        return
     eng, secs, nbuilds, nbw, prof = gpu_run(s, pair, cutneigh, skin, args.steps, args.warmup, prec, args.every,
                                             args.delay)
     eng.close()

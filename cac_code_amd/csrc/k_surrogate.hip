@@ -165,26 +165,104 @@ __device__ __forceinline__ void feature_group(int e, int g, const FeatArgs &a, c
 }
 
 // normalized features of the surrogate elements: BF16 into the MLP input,
-// FP32 into xf for the least-squares part; one wavefront per element
-__global__ void k_surr_features(int ns, const int *__restrict__ meta, FeatArgs a, const float *__restrict__ inv_in,
-                                uint16_t *__restrict__ x, int ldx, float *__restrict__ xf) {
-  // wave-uniform row: the element's meta row, nodes, centroid and reference
-  // class are then read with scalar loads
-  const int r = __builtin_amdgcn_readfirstlane(blockIdx.x * (kBlock / 64) + (int)(threadIdx.x >> 6));
-  if (r >= ns) return;
-  const int *mr = meta + kMeta * (size_t)r;
+// FP32 into xf for the least-squares part; one wavefront per element.  The
+// nodes of the element and of its 26 mesh neighbors (27 contiguous 192-byte
+// rows) are first copied to LDS with coalesced loads and their centroids
+// formed there (the node mean in node order, as k_surr_centroid forms it), so
+// every feature group then reads LDS: one round of global loads per element
+// instead of chains of dependent gathers.  Same values as feature_group.
+constexpr int kFeatWaves = kBlock / 64;
+
+__global__ __launch_bounds__(kBlock) void k_surr_features(int ns, const int *__restrict__ meta, FeatArgs a,
+                                                          const float *__restrict__ inv_in, uint16_t *__restrict__ x,
+                                                          int ldx, float *__restrict__ xf) {
+  __shared__ double sn[kFeatWaves][27 * 24];
+  __shared__ double sc[kFeatWaves][27 * 3];
+  __shared__ signed char ctm[56], ctn[56];
+  const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+  if (threadIdx.x < 56) {
+    ctm[threadIdx.x] = a.ct.m[threadIdx.x];
+    ctn[threadIdx.x] = a.ct.node[threadIdx.x];
+  }
+  // wave-uniform row (the meta row and reference class are read with scalar
+  // loads); rows past ns stage nothing and store nothing
+  const int r = __builtin_amdgcn_readfirstlane(blockIdx.x * kFeatWaves + w);
+  const bool live = r < ns;
+  const int *mr = meta + kMeta * (size_t)(live ? r : 0);
   const int e = mr[0];
+  const int *nb = mr + 2;
+  double *N = sn[w], *C = sc[w];
+  if (live)
+    for (int q = lane; q < 27 * 24; q += 64) {
+      const int sl = q / 24;
+      const int j = sl == 0 ? e : nb[sl - 1];
+      N[q] = j >= 0 ? a.nodex[24 * (size_t)j + (q - 24 * sl)] : 0.0;
+    }
+  __syncthreads();
+  if (live)
+    for (int q = lane; q < 27 * 3; q += 64) {
+      const int sl = q / 3, d = q - 3 * sl;
+      double acc = 0.0;
+      for (int k = 0; k < 8; ++k) acc += N[24 * sl + 3 * k + d];
+      C[q] = acc / 8;
+    }
+  __syncthreads();
+  if (!live) return;
   const double *ref = a.ref + kRefLen * (size_t)mr[1];
+  // affine map of the element (element_frame with the staged nodes)
   double F[9];
-  element_frame(e, a, ref, F);
+  {
+    double M[9] = {};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const double u0 = N[3 * k] - C[0], u1 = N[3 * k + 1] - C[1], u2 = N[3 * k + 2] - C[2];
+      const double s0 = ref[3 * k], s1 = ref[3 * k + 1], s2 = ref[3 * k + 2];
+      M[0] += u0 * s0; M[1] += u0 * s1; M[2] += u0 * s2;
+      M[3] += u1 * s0; M[4] += u1 * s1; M[5] += u1 * s2;
+      M[6] += u2 * s0; M[7] += u2 * s1; M[8] += u2 * s2;
+    }
+    const double *G = ref + kRefG;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        F[3 * i + j] = M[3 * i] * G[j] + M[3 * i + 1] * G[3 + j] + M[3 * i + 2] * G[6 + j];
+  }
   uint16_t *row = x + (size_t)r * ldx;
   float *rowf = xf + (size_t)r * kNFeat;
-  for (int g = threadIdx.x & 63; g < kGroups; g += 64)
-    feature_group(e, g, a, mr + 2, ref, F, [&](int c, float v) {
-      v *= inv_in[c];
-      row[c] = to_bf16(v);
-      rowf[c] = v;
-    });
+  auto emit = [&](int c, float v) {
+    v *= inv_in[c];
+    row[c] = to_bf16(v);
+    rowf[c] = v;
+  };
+  for (int g = lane; g < kGroups; g += 64) {
+    double v[3];
+    const double *rr = nullptr;
+    bool have = true;
+    if (g < 8) {
+      for (int d = 0; d < 3; ++d) v[d] = N[3 * g + d] - C[d];
+      rr = ref + 3 * g;
+    } else if (g < 34) {
+      const int m = g - 8;
+      have = nb[m] >= 0;
+      if (have)
+        for (int d = 0; d < 3; ++d) v[d] = min_image(C[3 * (1 + m) + d] - C[d], a.prd.v[d], a.rprd.v[d]);
+      rr = ref + kRefD0 + 3 * m;
+    } else if (g < 90) {
+      const int q = g - 34, m = ctm[q];
+      have = nb[m] >= 0;
+      if (have)
+        for (int d = 0; d < 3; ++d)
+          v[d] = min_image(N[24 * (1 + m) + 3 * ctn[q] + d] - C[d], a.prd.v[d], a.rprd.v[d]);
+      rr = ref + kRefC0 + 3 * q;
+    } else {
+      const int i = g - 90;
+      for (int d = 0; d < 3; ++d) emit(3 * g + d, (float)(F[3 * i + d] - (i == d ? 1.0 : 0.0)));
+      continue;
+    }
+    for (int d = 0; d < 3; ++d)
+      emit(3 * g + d, have ? (float)(v[d] - (F[3 * d] * rr[0] + F[3 * d + 1] * rr[1] + F[3 * d + 2] * rr[2])) : 0.0f);
+  }
 }
 
 // raw FP32 features of every element (dataset generation, tests)
@@ -393,12 +471,14 @@ Prd3 prd_of(const cac_ctx *ctx) {
 }
 
 // current centroids of all elements, then the feature arguments
-int frames(cac_ctx *ctx, FeatArgs *a) {
+int frames(cac_ctx *ctx, FeatArgs *a, bool centroids = true) {
   SurrState &S = ctx->surr;
   const int ne = ctx->ne;
   int rc;
   if ((rc = ensure(ctx, S.b_ubar, 3 * (size_t)ne, &S.ubar))) return rc;
-  hipLaunchKernelGGL(k_surr_centroid, dim3(grid_for(3L * ne)), dim3(kBlock), 0, ctx->stream, ne, ctx->nodex, S.ubar);
+  if (centroids)
+    hipLaunchKernelGGL(k_surr_centroid, dim3(grid_for(3L * ne)), dim3(kBlock), 0, ctx->stream, ne, ctx->nodex,
+                       S.ubar);
   Prd3 rp;
   for (int d = 0; d < 3; ++d) rp.v[d] = 1.0 / ctx->prd[d];
   *a = FeatArgs{ctx->nodex, S.ubar, S.ref, S.nbr, S.rcls, corner_table(), prd_of(ctx), rp};
@@ -495,13 +575,13 @@ int surrogate_forces(cac_ctx *ctx) {
   hipEvent_t t0;
   timer_begin(ctx, "surr_feat", &t0);
   FeatArgs fa;
-  if ((rc = frames(ctx, &fa))) return rc;
+  if ((rc = frames(ctx, &fa, false))) return rc;
   int ldx = 0;
   uint16_t *x = mlp_input_buffer(ctx, S.nS, &ldx);
   if (!x) return fail(ctx, CAC_ERR_HIP, "surrogate input buffer");
   float *xf;
   if ((rc = ensure(ctx, S.b_xf, (size_t)S.nS * kNFeat, &xf))) return rc;
-  hipLaunchKernelGGL(k_surr_features, dim3(grid_for(S.nS, kBlock / 64)), dim3(kBlock), 0, ctx->stream, S.nS,
+  hipLaunchKernelGGL(k_surr_features, dim3(grid_for(S.nS, kFeatWaves)), dim3(kBlock), 0, ctx->stream, S.nS,
                      S.meta, fa, S.inv_in, x, ldx, xf);
   timer_end(ctx, "surr_feat", t0);
   timer_begin(ctx, "surr_mlp", &t0);
