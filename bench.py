@@ -55,9 +55,11 @@ def log(msg):
 
 
 def make_workload(name, seed):
-    from cac_code_amd.lattice import build_cac_system, eam_c1_system, lj_c0_system
+    from cac_code_amd.lattice import build_cac_system, eam_c1_system, eam_c3_system, lj_c0_system
     if name == "eam_c1":
         return eam_c1_system(seed=seed)
+    if name == "eam_c3":
+        return eam_c3_system(seed=seed)
     if name == "lj_c0":
         return lj_c0_system(seed=seed)
     if name == "small":
@@ -103,8 +105,13 @@ def kernel_profile(eng, steps, precision):
     return prof
 
 
+STRAIN_RATE_PS = 0.0  # uniaxial strain rate along x per ps (set from --strain-rate in main)
+
+
 def gpu_run(s, pair, cutneigh, skin, steps, warmup, precision, every, delay, prof_steps=100):
     eng = new_engine(s, pair, cutneigh, skin, every, delay)
+    if STRAIN_RATE_PS:
+        eng.set_deform_rate([STRAIN_RATE_PS, 0.0, 0.0])
     nb_warm = eng.run(warmup, precision=precision) if warmup else 0
     eng.synchronize()
     t0 = time.perf_counter()
@@ -307,7 +314,9 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
-    ap.add_argument("--workload", choices=["eam_c1", "lj_c0", "small"], default="eam_c1")
+    ap.add_argument("--workload", choices=["eam_c1", "eam_c3", "lj_c0", "small"], default="eam_c1")
+    ap.add_argument("--strain-rate", type=float, default=None,
+                    help="uniaxial engineering strain rate along x, per second (eam_c3 default 1e8)")
     ap.add_argument("--seed", type=int, default=4928459)
     ap.add_argument("--every", type=int, default=1)
     ap.add_argument("--delay", type=int, default=10)
@@ -349,8 +358,10 @@ def main():
         sys.exit(2)
 
     from cac_code_amd import _lib
-    global COUPLED
+    global COUPLED, STRAIN_RATE_PS
     COUPLED = not args.no_coupling
+    rate = args.strain_rate if args.strain_rate is not None else (1e8 if args.workload == "eam_c3" else 0.0)
+    STRAIN_RATE_PS = rate * 1e-12                       # metal units: time in ps
     t0 = time.perf_counter()
     s = make_workload(args.workload, args.seed)
     pair, cutneigh, skin, ff = make_pair(args.workload)
@@ -453,7 +464,7 @@ def main():
                    "nae": list(s.nae), "integration_points": int(s.intp_w.size), "box_cells": s.cells,
                    "represented_atoms": n_rep, "rebuilds": nbuilds, "neighbor": [cutneigh, skin],
                    "neigh_modify": {"every": args.every, "delay": args.delay, "check": "yes"},
-                   "coupling": COUPLED,
+                   "coupling": COUPLED, "strain_rate_per_s": rate,
                    "seed": args.seed, "parallelism": "single GPU"},
         "represented_Matom_steps_per_s": round(n_rep * args.steps / secs / 1e6, 1),
         "roofline": roof,

@@ -43,6 +43,8 @@ SIGNATURES = {
     "cac_set_list_order": (C.c_int, [_ctx, C.c_double]),
     "cac_set_ip_clusters": (C.c_int, [_ctx, C.c_int, _i, _i]),
     "cac_cluster_info": (C.c_int, [_ctx, _i, _i, _i]),
+    "cac_deform": (C.c_int, [_ctx, _d]),
+    "cac_set_deform_rate": (C.c_int, [_ctx, _d]),
     "cac_build_neighbors": (C.c_int, [_ctx]),
     "cac_decide": (C.c_int, [_ctx, _i]),
     "cac_force": (C.c_int, [_ctx, C.c_int, C.c_int, C.c_int]),

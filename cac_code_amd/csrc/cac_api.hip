@@ -154,6 +154,33 @@ __global__ void k_nve_nodes_step(int ne, double *__restrict__ nodex, double *__r
   if (live && flag && dsq > triggersq) raise_flag(flag);
 }
 
+// Affine remap x -> lo + (x - lo)*scale of atoms, their held positions,
+// nodes, their held positions and element centers (one thread per atom or
+// node).
+struct Scale3 {
+  double lo[3], s[3];
+};
+
+__global__ void k_remap(int n, double *__restrict__ x4, double *__restrict__ xhold, int ne,
+                        double *__restrict__ nodex, double *__restrict__ nodexhold, double *__restrict__ xc,
+                        Scale3 m) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) {
+    for (int d = 0; d < 3; ++d) {
+      x4[4 * (size_t)t + d] = m.lo[d] + (x4[4 * (size_t)t + d] - m.lo[d]) * m.s[d];
+      if (xhold) xhold[(size_t)d * n + t] = m.lo[d] + (xhold[(size_t)d * n + t] - m.lo[d]) * m.s[d];
+    }
+  } else if (t < (long)n + 8L * ne) {
+    const size_t q = (size_t)(t - n);
+    for (int d = 0; d < 3; ++d) {
+      nodex[3 * q + d] = m.lo[d] + (nodex[3 * q + d] - m.lo[d]) * m.s[d];
+      if (nodexhold) nodexhold[3 * q + d] = m.lo[d] + (nodexhold[3 * q + d] - m.lo[d]) * m.s[d];
+    }
+    if ((q & 7) == 0)
+      for (int d = 0; d < 3; ++d) xc[3 * (q >> 3) + d] = m.lo[d] + (xc[3 * (q >> 3) + d] - m.lo[d]) * m.s[d];
+  }
+}
+
 // ---------------------------------------------------------------- reductions
 // Deterministic sums: block b of the grid reduces array b with a fixed
 // stride pattern and a fixed LDS tree.
@@ -382,6 +409,35 @@ int nve_step_impl(cac_ctx *ctx, double dtv, double dtf, bool fin, int *flag) {
   }
   CAC_HIP(ctx, hipGetLastError());
   return CAC_OK;
+}
+
+int deform_impl(cac_ctx *ctx, const double scale[3]) {
+  Scale3 m;
+  for (int d = 0; d < 3; ++d) {
+    if (!(scale[d] > 0)) return fail(ctx, CAC_ERR_ARG, "deformation scale must be positive");
+    m.lo[d] = ctx->lo[d];
+    m.s[d] = scale[d];
+  }
+  const long tot = (long)ctx->n + 8L * ctx->ne;
+  if (tot > 0)
+    hipLaunchKernelGGL(k_remap, dim3(grid_for(tot)), dim3(kBlock), 0, ctx->stream, ctx->n, ctx->x4, ctx->xhold,
+                       ctx->ne, ctx->nodex, ctx->ne ? ctx->nodexhold : nullptr, ctx->xc, m);
+  CAC_HIP(ctx, hipGetLastError());
+  for (int d = 0; d < 3; ++d) {
+    ctx->prd[d] *= scale[d];
+    ctx->hi[d] = ctx->lo[d] + ctx->prd[d];
+  }
+  return CAC_OK;
+}
+
+// The remap of one cac_run step under a constant strain rate (engineering:
+// the edge at elapsed time t is L0*(1 + rate*t)).
+int deform_step(cac_ctx *ctx, double dtv) {
+  ctx->deform_t += dtv;
+  double scale[3];
+  for (int d = 0; d < 3; ++d)
+    scale[d] = ctx->deform_l0[d] * (1.0 + ctx->deform_rate[d] * ctx->deform_t) / ctx->prd[d];
+  return deform_impl(ctx, scale);
 }
 
 int nve_final_impl(cac_ctx *ctx, double dtf) {
@@ -758,6 +814,26 @@ int cac_set_ip_clusters(cac_ctx *ctx, int ncl, const int *start, const int *slot
   return CAC_OK;
 }
 
+int cac_deform(cac_ctx *ctx, const double scale[3]) {
+  if (!ctx || !scale) return CAC_ERR_ARG;
+  if (!ctx->box_set) return fail(ctx, CAC_ERR_ARG, "cac_set_box not called");
+  hipSetDevice(ctx->device);
+  return deform_impl(ctx, scale);
+}
+
+int cac_set_deform_rate(cac_ctx *ctx, const double rate[3]) {
+  if (!ctx || !rate) return CAC_ERR_ARG;
+  if (!ctx->box_set) return fail(ctx, CAC_ERR_ARG, "cac_set_box not called");
+  ctx->deform_on = false;
+  for (int d = 0; d < 3; ++d) {
+    ctx->deform_rate[d] = rate[d];
+    ctx->deform_l0[d] = ctx->prd[d];
+    if (rate[d] != 0.0) ctx->deform_on = true;
+  }
+  ctx->deform_t = 0.0;
+  return CAC_OK;
+}
+
 int cac_cluster_info(cac_ctx *ctx, int *ncl, int *active, int *max_union) {
   if (!ctx) return CAC_ERR_ARG;
   if (ncl) *ncl = ctx->ncl;
@@ -822,6 +898,7 @@ int cac_run(cac_ctx *ctx, int nsteps, double dtv, double dtf, int precision, int
     ctx->ago++;
     const bool due = ctx->ago >= ctx->delay && ctx->ago % ctx->every == 0;
     const bool probe = due && ctx->check;
+    if (ctx->deform_on && (rc = deform_step(ctx, dtv))) return rc;
     if (probe) *ctx->hcheck = 0;
     if ((rc = nve_step_impl(ctx, dtv, dtf, step > 0, probe ? ctx->dcheck : nullptr))) return rc;
     if (probe) CAC_HIP(ctx, hipEventRecord(ctx->ev_check, ctx->stream));

@@ -239,6 +239,11 @@ struct cac_ctx {
   cac::EamTables eam;
   cac::LjParams lj;
 
+  // ---- deformation (cac_set_deform_rate): rate per time unit, edge L0 at
+  // the time the rate was set, time elapsed since then
+  bool deform_on = false;
+  double deform_rate[3] = {0, 0, 0}, deform_l0[3] = {0, 0, 0}, deform_t = 0.0;
+
   // ---- neighbor settings / state
   double cutneigh = 0.0, triggersq = 0.0;
   int every = 1, delay = 0, check = 1;

@@ -315,6 +315,17 @@ class Engine(_MlpMixin):
         _check(self.ctx, self.L.cac_set_ip_clusters(self.ctx, int(start.size - 1), _p(start, _lib._i),
                                                     _p(slots, _lib._i)))
 
+    def deform(self, scale):
+        """Affine remap of the box and its contents by scale[3] about box lo."""
+        sc = np.ascontiguousarray(scale, dtype=np.float64)
+        _check(self.ctx, self.L.cac_deform(self.ctx, _p(sc)))
+
+    def set_deform_rate(self, rate):
+        """Engineering strain rate per time unit (ps for metal units) applied
+        by run() before every step; zeros stop it."""
+        r = np.ascontiguousarray(rate, dtype=np.float64)
+        _check(self.ctx, self.L.cac_set_deform_rate(self.ctx, _p(r)))
+
     def cluster_info(self):
         """(clusters, cluster passes active after the last build, largest
         union of a cluster's lists)."""

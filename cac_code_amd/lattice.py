@@ -27,6 +27,7 @@ m_e/8, fix_nve.cpp:124).  The reference has no velocity create
 from __future__ import annotations
 
 from dataclasses import dataclass, field
+from typing import Optional
 
 import numpy as np
 
@@ -126,15 +127,32 @@ def build_cac_system(L: int, nae: int, core_cells: float, a: float = 3.615, mass
                    axis=-1).reshape(-1, 3)                     # site order ii, jj, kk
     center = box_hi / 2.0 if core_center is None else np.asarray(core_center, dtype=np.float64)
     half = 0.5 * core_cells * a
-    # site positions block by block (chunked to bound memory)
+    # A block is dropped iff one of its sites (wrapped into the box) lies in
+    # the core cube.  Only blocks whose corner-spanned box, taken over the
+    # periodic images of their centroid, meets the core can qualify (a block
+    # is a parallelepiped, the hull of its corner sites), so the sites are
+    # generated for those candidates alone.
+    corners0 = NODE_CORNERS * (nae - 1)
+    cpos = origin + (nae * B[:, None, :] + corners0[None, :, :]) @ P          # [nb, 8, 3] unwrapped
+    cen = cpos.mean(axis=1)
+    shift = np.mod(cen - box_lo, box_hi - box_lo) + box_lo - cen               # wrap by the centroid
+    cmin = cpos.min(axis=1) + shift
+    cmax = cpos.max(axis=1) + shift
+    L3 = box_hi - box_lo
+    cand = np.zeros(B.shape[0], dtype=bool)
+    for img in range(27):
+        sft = (np.array([img % 3, (img // 3) % 3, img // 9]) - 1) * L3
+        cand |= np.all((cmin + sft < center + half) & (cmax + sft > center - half), axis=1)
     drop = np.zeros(B.shape[0], dtype=bool)
+    idx = np.flatnonzero(cand)
     chunk = max(1, 2_000_000 // (nae ** 3))
-    for s in range(0, B.shape[0], chunk):
-        n = nae * B[s:s + chunk, None, :] + loc[None, :, :]
+    for s in range(0, idx.size, chunk):
+        ii = idx[s:s + chunk]
+        n = nae * B[ii, None, :] + loc[None, :, :]
         pos = origin + n @ P
         pos = np.mod(pos - box_lo, box_hi - box_lo) + box_lo
         inside = np.all(np.abs(pos - center) < half, axis=-1)
-        drop[s:s + chunk] = inside.any(axis=1)
+        drop[ii] = inside.any(axis=1)
     keepB = B[~drop]
     dropB = B[drop]
     # atoms: all sites of dropped blocks, wrapped into the box
@@ -219,6 +237,68 @@ def temperature_of(s: CACSystem, ke: float) -> float:
 def lj_c0_system(seed: int = 4928459) -> CACSystem:
     """LJ Cu: 40^3-cell periodic box, central 10^3-cell core, nae=5, 10 K."""
     return build_cac_system(40, 5, 10, temperature=10.0, seed=seed)
+
+
+def edge_dipole(s: CACSystem, nu: float = 0.34, separation: Optional[float] = None, center=None) -> None:
+    """Insert an edge-dislocation dipole (isotropic Volterra field) into the
+    atom and node positions, in place.  Burgers vector b = (a/2)[110]
+    (|b| = a/sqrt(2)), line along [001], the two cores `separation` apart
+    along the glide-plane normal [-110]/sqrt(2), centered on `center` (default:
+    the box center, separation half the atomistic core's width so both cores
+    lie among the atoms).  Each dislocation displaces a point at (x', y')
+    from its core, in the frame x' = b/|b|, y' = normal, by
+        u_x' =  b/2pi [theta + x'y' / (2(1-nu) r^2)]
+        u_y' = -b/2pi [(1-2nu)/(4(1-nu)) ln r^2 + (x'^2 - y'^2) / (4(1-nu) r^2)]
+    (theta = atan2(y', x')); the second core carries -b.  Between the two
+    branch cuts the material is displaced by the lattice vector b, so the
+    periodic box stays consistent.  The positions are wrapped back into the
+    box."""
+    a = s.a0
+    bmag = a / np.sqrt(2.0)
+    ex = np.array([1.0, 1.0, 0.0]) / np.sqrt(2.0)
+    ey = np.array([-1.0, 1.0, 0.0]) / np.sqrt(2.0)
+    c = (s.box_lo + s.box_hi) / 2.0 if center is None else np.asarray(center, dtype=np.float64)
+    if separation is None:
+        span = np.ptp(s.x, axis=0).min() if s.n_atoms else 0.25 * (s.box_hi - s.box_lo).min()
+        separation = 0.5 * span
+    cores = [(c - 0.5 * separation * ey, 1.0), (c + 0.5 * separation * ey, -1.0)]
+
+    def disp(p):
+        u = np.zeros_like(p)
+        for c0, sign in cores:
+            d = p - c0
+            xp, yp = d @ ex, d @ ey
+            r2 = np.maximum(xp * xp + yp * yp, 1e-12)
+            ux = bmag / (2 * np.pi) * (np.arctan2(yp, xp) + xp * yp / (2 * (1 - nu) * r2))
+            uy = -bmag / (2 * np.pi) * ((1 - 2 * nu) / (4 * (1 - nu)) * np.log(r2)
+                                        + (xp * xp - yp * yp) / (4 * (1 - nu) * r2))
+            u += sign * (ux[:, None] * ex[None, :] + uy[:, None] * ey[None, :])
+        return u
+
+    L3 = s.box_hi - s.box_lo
+    if s.n_atoms:
+        x = s.x + disp(s.x)
+        s.x = np.ascontiguousarray(np.mod(x - s.box_lo, L3) + s.box_lo)
+    if s.n_elements:
+        X = s.nodex.reshape(-1, 3)
+        X = (X + disp(X)).reshape(s.nodex.shape)
+        # keep each element's nodes together: wrap by the element center
+        cen = X.mean(axis=1)
+        X = X + (np.mod(cen - s.box_lo, L3) + s.box_lo - cen)[:, None, :]
+        s.nodex = np.ascontiguousarray(X)
+
+
+def eam_c3_system(seed: int = 4928459, L: int = 777, nae: int = 21, core_cells: float = 106.0,
+                  temperature: float = 300.0) -> CACSystem:
+    """Large EAM CAC: periodic box of L^3 fcc cells (L a multiple of nae),
+    elements of nae sites per edge with 8 corner integration points, an
+    atomistic core (~8M atoms at the defaults, 202,612 blocks of 21^3 sites),
+    an edge-dislocation dipole in the core, seeded velocities."""
+    s = build_cac_system(L, nae, core_cells, temperature=0.0, seed=seed)
+    edge_dipole(s)
+    if temperature > 0.0:
+        assign_velocities(s, temperature, seed)
+    return s
 
 
 def eam_c1_system(seed: int = 4928459, L: int = 198, core_cells: float = 42.0) -> CACSystem:

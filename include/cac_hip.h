@@ -122,6 +122,21 @@ int cac_set_list_order(cac_ctx *ctx, double spacing);
  * integration points).  Off by default. */
 int cac_set_coupling(cac_ctx *ctx, int on);
 
+/* Affine remap of the box and everything in it (no counterpart in the
+ * reference, whose input has no deformation command; it is LAMMPS' fix
+ * deform with "remap x").  Every coordinate maps to lo + (x - lo)*scale[d]:
+ * atoms, element nodes and centers, and the positions held at the last
+ * build (so the rebuild check measures motion, not the remap); box hi
+ * follows, velocities do not change.  The neighbor lists stay in use until
+ * the usual check asks for a rebuild. */
+int cac_deform(cac_ctx *ctx, const double scale[3]);
+
+/* Constant engineering strain rate (per time unit of dtv) that cac_run
+ * applies before every step: the box edge at elapsed time t is
+ * L0*(1 + rate*t), L0 the edge when the rate was set.  A zero rate stops
+ * it. */
+int cac_set_deform_rate(cac_ctx *ctx, const double rate[3]);
+
 /* Vertex clusters of the element integration points (no counterpart in the
  * reference: a grouping of work, the forces do not change).  Cluster c holds
  * the node slots slots[start[c] .. start[c+1]), slot 8*e + k naming node k

@@ -111,3 +111,50 @@ def test_run_matches_oracle_nve(orc, cu_funcfl, cu_tables):
         orc.nve_nodes_final(nodev, nodef, DTF, s.mass_e)
     for g, r, name in zip(got, [x, v, f, es.nodex, nodev, nodef], ["x", "v", "f", "nodex", "nodev", "nodef"]):
         assert rel_err(g, r) < 1e-10, name
+
+
+def test_deform_matches_rescaled_system(cu_funcfl):
+    """cac_deform maps atoms, nodes and the box affinely: the forces equal
+    those of the system built directly at the deformed coordinates."""
+    s = small_system(seed=25, L=10, nae=5, core=4.0, sigma=0.03, temperature=300.0)
+    scale = np.array([1.002, 0.999, 1.001])
+    pair = eam_pair(cu_funcfl)
+    a = engine(s, pair, 1.0, 1, 0, True, True)
+    try:
+        a.deform(scale)
+        a.force()
+        fa = state(a)
+    finally:
+        a.close()
+    t = small_system(seed=25, L=10, nae=5, core=4.0, sigma=0.03, temperature=300.0)
+    t.x = np.ascontiguousarray(t.box_lo + (t.x - t.box_lo) * scale)
+    t.nodex = np.ascontiguousarray(t.box_lo + (t.nodex - t.box_lo) * scale)
+    t.box_hi = t.box_lo + (t.box_hi - t.box_lo) * scale
+    b = engine(t, pair, 1.0, 1, 0, True, True)
+    try:
+        fb = state(b)
+    finally:
+        b.close()
+    for qa, qb, name in zip(fa, fb, ["x", "v", "f", "nodex", "nodev", "nodef"]):
+        assert rel_err(qa, qb) < 1e-10, name
+
+
+def test_strain_rate_ramp_in_run(cu_funcfl):
+    """A constant engineering strain rate in cac_run: a perfect, motionless
+    crystal (zero forces) is stretched exactly to L0*(1 + rate*t)."""
+    from cac_code_amd.lattice import build_cac_system
+    s = build_cac_system(10, 5, 4.0, temperature=0.0)
+    pair = eam_pair(cu_funcfl)
+    rate = np.array([1e-2, 0.0, 0.0])               # per ps
+    nsteps = 10
+    eng = engine(s, pair, 1.0, 1, 0, True, True)
+    try:
+        eng.set_deform_rate(rate)
+        eng.run(nsteps, dt=DT)
+        x, v, _ = eng.atoms()
+        nx, _, _ = eng.elements()
+    finally:
+        eng.close()
+    stretch = 1.0 + rate * nsteps * DT
+    assert np.allclose(x, s.box_lo + (s.x - s.box_lo) * stretch, rtol=0, atol=1e-9)
+    assert np.allclose(nx, s.box_lo + (s.nodex - s.box_lo) * stretch, rtol=0, atol=1e-9)
