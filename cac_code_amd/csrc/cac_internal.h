@@ -149,6 +149,7 @@ struct cac_ctx {
   uint32_t *abin = nullptr;
   int *abin_s = nullptr, *abin_e = nullptr;
   cac::DevBuf b_akey, b_aval, b_akey2, b_aval2, b_acs, b_ace, b_sorttmp;
+  cac::DevBuf b_ahead, b_acells;  // atom list build: cell-start flags and scan, non-empty cells
   // atom-element coupling (build-defined, off by default = the reference):
   // atom lists also hold element sites, integration-point lists also atoms.
   // Interface atoms are those within cutneigh + R_j of some element center.
@@ -174,8 +175,8 @@ struct cac_ctx {
   bool quad_set = false;
   // sub-blocks: per element per sub-block (cx, cy, cz, radius); element (cx, cy, cz, R)
   int nsb[3] = {0, 0, 0}, nsb_total = 0;
-  double *sbc = nullptr, *erad = nullptr;
-  cac::DevBuf b_sbc, b_erad;
+  double *sbc = nullptr, *erad = nullptr, *ehull = nullptr;  // ehull: [ne][16] parallelepiped hull
+  cac::DevBuf b_sbc, b_erad, b_ehull;
   // element neighbor list, column-major [maxelist][ne]
   int maxelist = 192;
   uint32_t *eneigh = nullptr;

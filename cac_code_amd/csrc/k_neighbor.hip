@@ -124,15 +124,20 @@ __global__ void k_cell_bounds(int n, const uint32_t *__restrict__ key, int *__re
 // (j, image) != (i, self) with rsq <= cutneighsq, from the 27 cells around
 // i's cell (cell edge >= cutneigh, >= 3 cells per dim).
 //
-// One wavefront (one workgroup) per atom: lanes test the atoms of one cell
-// at a time and compact the accepted entries into LDS in lane order; the row
-// is then sorted in LDS (bitonic) by a key that puts the lattice-rounded
-// displacement x_j' - x_i (components rounded to multiples of `spacing`)
-// before the entry value, and written to the atom_slot layout.  On a lattice
-// of that spacing every atom lists its neighbors in the same relative order,
-// so at step k the 64 atoms of a force-pass wavefront gather neighboring rows
-// of x4 instead of 64 unrelated ones.  spacing = 0 sorts by entry value.
-// Either way the order is deterministic.
+// One workgroup of four wavefronts per non-empty cell.  The candidates (the
+// atoms of the 27 surrounding cells, image shift applied) are staged in LDS
+// once and shared by the cell's atoms; each wavefront then takes the cell's
+// atoms in turn, tests the staged candidates 64 at a time and compacts the
+// accepted entries into LDS.  Each entry is written straight to its rank in
+// the row (rank = number of smaller keys), i.e. the row sorted by a key that
+// puts the lattice-rounded displacement x_j' - x_i (components rounded to
+// multiples of `spacing`) before the entry value, in the atom_slot layout.
+// On a lattice of that spacing every atom lists its neighbors in the same
+// relative order, so at step k the 64 atoms of a force-pass wavefront gather
+// neighboring rows of x4 instead of 64 unrelated ones.  spacing = 0 sorts by
+// entry value.  Keys are unique (they end in the entry), so the row is
+// independent of the candidate order.  A cell with more candidates than the
+// LDS holds reads them from global memory instead.
 __device__ __forceinline__ uint64_t list_key(double dx, double dy, double dz, double qinv, uint32_t entry) {
   if (qinv <= 0.0) return entry;
   auto q = [&](double d) {
@@ -160,72 +165,170 @@ __device__ void bitonic_sort_lds(uint64_t *a, int n2, int lane) {
     }
 }
 
-__global__ __launch_bounds__(64) void k_atom_neigh(int n, const double *__restrict__ x4, Box3 b, Grid3 g,
-                                                   const uint32_t *__restrict__ sorted, const int *__restrict__ cs,
-                                                   const int *__restrict__ ce, double cutsq, int maxneigh, int cap2,
-                                                   double qinv, uint32_t *__restrict__ neigh, int *__restrict__ num,
-                                                   double *__restrict__ xhold, int *__restrict__ flags) {
-  extern __shared__ uint64_t keys[];  // [cap2], cap2 = power of two >= maxneigh
-  const int i = xcd_block();
-  if (i >= n) return;  // whole workgroup
-  const int lane = (int)threadIdx.x;
-  const double xi = x4[4 * (size_t)i], yi = x4[4 * (size_t)i + 1], zi = x4[4 * (size_t)i + 2];
-  const int ci[3] = {cell_coord(xi, b.lo[0], g.inv[0], g.n[0]), cell_coord(yi, b.lo[1], g.inv[1], g.n[1]),
-                     cell_coord(zi, b.lo[2], g.inv[2], g.n[2])};
-  int cnt = 0;
-  for (int dz = -1; dz <= 1; ++dz)
-    for (int dy = -1; dy <= 1; ++dy)
-      for (int dx = -1; dx <= 1; ++dx) {
-        int cc[3] = {ci[0] + dx, ci[1] + dy, ci[2] + dz}, s[3];
-        for (int d = 0; d < 3; ++d) {
-          s[d] = 0;
-          if (cc[d] < 0) { cc[d] += g.n[d]; s[d] = -1; }
-          else if (cc[d] >= g.n[d]) { cc[d] -= g.n[d]; s[d] = 1; }
+// LDS written by some lanes of a wavefront is read by others after this.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ void cell_split(int c, const Grid3 &g, int cc[3]) {
+  cc[0] = c % g.n[0];
+  cc[1] = (c / g.n[0]) % g.n[1];
+  cc[2] = c / (g.n[0] * g.n[1]);
+}
+
+// Neighbor cell t (0..26, x fastest) of cell ci: its id and image.
+__device__ __forceinline__ int cell_around(const int ci[3], int t, const Grid3 &g, int *img) {
+  int cc[3] = {ci[0] + t % 3 - 1, ci[1] + (t / 3) % 3 - 1, ci[2] + t / 9 - 1}, s[3];
+  for (int d = 0; d < 3; ++d) {
+    s[d] = 0;
+    if (cc[d] < 0) { cc[d] += g.n[d]; s[d] = -1; }
+    else if (cc[d] >= g.n[d]) { cc[d] -= g.n[d]; s[d] = 1; }
+  }
+  *img = (s[0] + 1) + 3 * (s[1] + 1) + 9 * (s[2] + 1);
+  return (cc[2] * g.n[1] + cc[1]) * g.n[0] + cc[0];
+}
+
+// Sorted position p starts a cell: hflag[p] = 1 (hflag[n] = 0 for the scan).
+__global__ void k_cell_heads(int n, const uint32_t *__restrict__ key, int *__restrict__ hflag) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) hflag[p] = (p == 0 || key[p - 1] != key[p]) ? 1 : 0;
+  if (p == n) hflag[p] = 0;
+}
+
+// Non-empty cells in cell order, and the largest candidate count of any.
+__global__ void k_cell_list(int n, const uint32_t *__restrict__ key, const int *__restrict__ hflag,
+                            const int *__restrict__ hpos, Grid3 g, const int *__restrict__ cs,
+                            const int *__restrict__ ce, int *__restrict__ cells, int *__restrict__ maxcand) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n || !hflag[p]) return;
+  const int c = (int)key[p];
+  cells[hpos[p]] = c;
+  int ci[3], tot = 0, img;
+  cell_split(c, g, ci);
+  for (int t = 0; t < 27; ++t) {
+    const int cn = cell_around(ci, t, g, &img);
+    tot += ce[cn] - cs[cn];
+  }
+  atomicMax(maxcand, tot);
+}
+
+constexpr int kNeighWaves = 4;
+
+__global__ __launch_bounds__(64 * kNeighWaves) void k_atom_neigh(
+    int ncells, const int *__restrict__ cells, const double *__restrict__ x4, Box3 b, Grid3 g,
+    const uint32_t *__restrict__ sorted, const int *__restrict__ cs, const int *__restrict__ ce, double cutsq,
+    int maxneigh, int cap2, int ccap, double qinv, int n, uint32_t *__restrict__ neigh, int *__restrict__ num,
+    double *__restrict__ xhold, int *__restrict__ flags) {
+  // LDS: keys[kNeighWaves][cap2] (u64), then sx, sy, sz [ccap] (f64), se [ccap] (u32)
+  extern __shared__ uint64_t smem[];
+  __shared__ int pre[28], rstart[27], rimg[27];
+  const int w = xcd_block();
+  if (w >= ncells) return;  // whole workgroup
+  const int tid = (int)threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int c = cells[w];
+  if (tid < 27) {
+    int ci[3], img;
+    cell_split(c, g, ci);
+    const int cn = cell_around(ci, tid, g, &img);
+    rstart[tid] = cs[cn];
+    rimg[tid] = img;
+    pre[tid + 1] = ce[cn] - cs[cn];
+  }
+  __syncthreads();
+  if (tid == 0) {
+    pre[0] = 0;
+    for (int t = 0; t < 27; ++t) pre[t + 1] += pre[t];
+  }
+  __syncthreads();
+  const int T = pre[27];
+  // candidate f (0 <= f < T): range r = max{r : pre[r] <= f}
+  auto fetch = [&](int f, double &xj, double &yj, double &zj, uint32_t &e) {
+    int lo = 0, hi = 27;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (pre[mid] <= f) lo = mid;
+      else hi = mid;
+    }
+    const int j = (int)sorted[rstart[lo] + f - pre[lo]];
+    const int img = rimg[lo];
+    int s[3];
+    img_shift(img, s);
+    xj = shifted(x4[4 * (size_t)j], s[0], b.prd[0]);
+    yj = shifted(x4[4 * (size_t)j + 1], s[1], b.prd[1]);
+    zj = shifted(x4[4 * (size_t)j + 2], s[2], b.prd[2]);
+    e = (uint32_t)j | ((uint32_t)img << 27);
+  };
+  double *sx = reinterpret_cast<double *>(smem + kNeighWaves * cap2);
+  double *sy = sx + ccap, *sz = sy + ccap;
+  uint32_t *se = reinterpret_cast<uint32_t *>(sz + ccap);
+  const bool staged = T <= ccap;
+  if (staged)
+    for (int f = tid; f < T; f += 64 * kNeighWaves) fetch(f, sx[f], sy[f], sz[f], se[f]);
+  __syncthreads();
+  uint64_t *keys = smem + wave * cap2;
+  for (int h = cs[c] + wave; h < ce[c]; h += kNeighWaves) {
+    const int i = (int)sorted[h];
+    const double xi = x4[4 * (size_t)i], yi = x4[4 * (size_t)i + 1], zi = x4[4 * (size_t)i + 2];
+    const uint32_t self = (uint32_t)i | ((uint32_t)kImgSelf << 27);
+    int cnt = 0;
+    for (int f0 = 0; f0 < T; f0 += 64) {
+      const int f = f0 + lane;
+      bool acc = false;
+      uint64_t key = 0;
+      if (f < T) {
+        double xj, yj, zj;
+        uint32_t e;
+        if (staged) {
+          xj = sx[f];
+          yj = sy[f];
+          zj = sz[f];
+          e = se[f];
+        } else {
+          fetch(f, xj, yj, zj, e);
         }
-        const int img = (s[0] + 1) + 3 * (s[1] + 1) + 9 * (s[2] + 1);
-        const int c = (cc[2] * g.n[1] + cc[1]) * g.n[0] + cc[0];
-        const int qs = cs[c], qe = ce[c];
-        for (int q0 = qs; q0 < qe; q0 += 64) {
-          const int q = q0 + lane;
-          bool acc = false;
-          uint64_t key = 0;
-          if (q < qe) {
-            const int j = (int)sorted[q];
-            if (!(j == i && img == kImgSelf)) {
-              const double xj = shifted(x4[4 * (size_t)j], s[0], b.prd[0]);
-              const double yj = shifted(x4[4 * (size_t)j + 1], s[1], b.prd[1]);
-              const double zj = shifted(x4[4 * (size_t)j + 2], s[2], b.prd[2]);
-              const double delx = xi - xj, dely = yi - yj, delz = zi - zj;
-              const double rsq = delx * delx + dely * dely + delz * delz;
-              acc = rsq <= cutsq;
-              key = list_key(xj - xi, yj - yi, zj - zi, qinv, (uint32_t)j | ((uint32_t)img << 27));
-            }
-          }
-          const unsigned long long am = __ballot(acc);
-          if (acc) {
-            const int pos = cnt + __popcll(am & ((1ull << lane) - 1ull));
-            if (pos < cap2) keys[pos] = key;
-          }
-          cnt += __popcll(am);
+        if (e != self) {
+          const double delx = xi - xj, dely = yi - yj, delz = zi - zj;
+          const double rsq = delx * delx + dely * dely + delz * delz;
+          acc = rsq <= cutsq;
+          key = list_key(xj - xi, yj - yi, zj - zi, qinv, e);
         }
       }
-  const int kept = cnt < cap2 ? cnt : cap2;
-  int n2 = 1;
-  while (n2 < kept) n2 <<= 1;
-  for (int p = kept + lane; p < n2; p += 64) keys[p] = ~0ull;
-  __syncthreads();
-  bitonic_sort_lds(keys, n2, lane);
-  const int out = cnt < maxneigh ? cnt : maxneigh;
-  for (int k = lane; k < out; k += 64) neigh[atom_slot(i, k, maxneigh)] = (uint32_t)keys[k];
-  if (lane == 0) {
-    num[i] = cnt;
-    if (cnt > maxneigh) {
-      atomicOr(&flags[0], 1);
-      atomicMax(&flags[1], cnt);
+      const unsigned long long am = __ballot(acc);
+      if (acc) {
+        const int pos = cnt + __popcll(am & ((1ull << lane) - 1ull));
+        if (pos < cap2) keys[pos] = key;
+      }
+      cnt += __popcll(am);
     }
-    xhold[i] = xi;
-    xhold[n + i] = yi;
-    xhold[2 * (size_t)n + i] = zi;
+    wave_sync();
+    const int kept = cnt < cap2 ? cnt : cap2;
+    // rank of keys a and a + 64 in one sweep over the row (broadcast reads,
+    // unrolled so that several are in flight)
+    for (int a = lane; a < ((kept + 127) & ~127); a += 128) {
+      const uint64_t ka = a < kept ? keys[a] : 0, kb = a + 64 < kept ? keys[a + 64] : 0;
+      int ra = 0, rb = 0;
+#pragma unroll 8
+      for (int q = 0; q < kept; ++q) {
+        const uint64_t t = keys[q];
+        ra += t < ka ? 1 : 0;
+        rb += t < kb ? 1 : 0;
+      }
+      if (a < kept && ra < maxneigh) neigh[atom_slot(i, ra, maxneigh)] = (uint32_t)ka;
+      if (a + 64 < kept && rb < maxneigh) neigh[atom_slot(i, rb, maxneigh)] = (uint32_t)kb;
+    }
+    if (lane == 0) {
+      num[i] = cnt;
+      if (cnt > maxneigh) {
+        atomicOr(&flags[0], 1);
+        atomicMax(&flags[1], cnt);
+      }
+      xhold[i] = xi;
+      xhold[n + i] = yi;
+      xhold[2 * (size_t)n + i] = zi;
+    }
+    wave_sync();  // every lane has read keys before the next atom refills them
   }
 }
 
@@ -235,8 +338,28 @@ __global__ __launch_bounds__(64) void k_atom_neigh(int n, const double *__restri
 // per 4x4x4 sub-block, the center and radius of its 8 corner sites (the
 // trilinear map restricted to the sub-block is trilinear in the block's own
 // coordinates, so its sites are convex combinations of those corners).
+//
+// Also a parallelepiped hull: with corner signs s_k (node k at parent
+// corner (s_k + 1)/2), J = (1/8) sum_k (X_k - c) s_k^T and residuals
+// r_k = X_k - c - J s_k, |r_k| <= delta.  A site is sum_k N_k X_k with
+// convex weights N_k, i.e. c + J xi + sum_k N_k r_k with xi = sum_k N_k s_k in
+// [-1,1]^3, so it lies within delta of P = c + J [-1,1]^3.  A point q within
+// cutneigh of a site is within rho = cutneigh + delta of P, and then
+// |(J^-1 (q - c))_d| <= 1 + rho |row_d(J^-1)| for every d: hull[16 i ..]
+// holds J^-1 (row-major) and these three bounds (slack for rounding added;
+// a degenerate J gets J^-1 = 0 so the test always passes).
+__device__ __forceinline__ bool hull_hit(const double *__restrict__ hull, int j, double dx, double dy, double dz) {
+  const double *h = hull + 16 * (size_t)j;
+  for (int d = 0; d < 3; ++d) {
+    const double u = h[3 * d] * dx + h[3 * d + 1] * dy + h[3 * d + 2] * dz;
+    if (!(fabs(u) <= h[9 + d])) return false;
+  }
+  return true;
+}
+
 __global__ void k_elem_radius(int ne, const double *__restrict__ nodex, const double *__restrict__ xc,
-                              double *__restrict__ erad, unsigned long long *__restrict__ rmax_bits) {
+                              double cutneigh, double *__restrict__ erad, double *__restrict__ hull,
+                              unsigned long long *__restrict__ rmax_bits) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ne) return;
   const double *X = nodex + 24 * (size_t)i;
@@ -256,6 +379,52 @@ __global__ void k_elem_radius(int ne, const double *__restrict__ nodex, const do
   erad[4 * i + 2] = c[2];
   erad[4 * i + 3] = R;
   atomicMax(rmax_bits, (unsigned long long)__double_as_longlong(R));
+  // corner signs of nodes 0..7: (0,0,0) (1,0,0) (1,1,0) (0,1,0) (0,0,1) (1,0,1) (1,1,1) (0,1,1)
+  constexpr int kSx = 0x66, kSy = 0xCC, kSz = 0xF0;
+  double J[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+  for (int k = 0; k < 8; ++k) {
+    const double sg[3] = {(kSx >> k) & 1 ? 1.0 : -1.0, (kSy >> k) & 1 ? 1.0 : -1.0, (kSz >> k) & 1 ? 1.0 : -1.0};
+    for (int d = 0; d < 3; ++d)
+      for (int e = 0; e < 3; ++e) J[d][e] += sg[e] * (X[3 * k + d] - c[d]);
+  }
+  for (int d = 0; d < 3; ++d)
+    for (int e = 0; e < 3; ++e) J[d][e] *= 0.125;
+  double del2 = 0.0;
+  for (int k = 0; k < 8; ++k) {
+    const double sg[3] = {(kSx >> k) & 1 ? 1.0 : -1.0, (kSy >> k) & 1 ? 1.0 : -1.0, (kSz >> k) & 1 ? 1.0 : -1.0};
+    double q = 0.0;
+    for (int d = 0; d < 3; ++d) {
+      const double r = X[3 * k + d] - c[d] - (J[d][0] * sg[0] + J[d][1] * sg[1] + J[d][2] * sg[2]);
+      q += r * r;
+    }
+    del2 = q > del2 ? q : del2;
+  }
+  double A[3][3];  // adjugate: J^-1 = A / det
+  A[0][0] = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+  A[0][1] = J[0][2] * J[2][1] - J[0][1] * J[2][2];
+  A[0][2] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+  A[1][0] = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+  A[1][1] = J[0][0] * J[2][2] - J[0][2] * J[2][0];
+  A[1][2] = J[0][2] * J[1][0] - J[0][0] * J[1][2];
+  A[2][0] = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+  A[2][1] = J[0][1] * J[2][0] - J[0][0] * J[2][1];
+  A[2][2] = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+  const double det = J[0][0] * A[0][0] + J[0][1] * A[1][0] + J[0][2] * A[2][0];
+  double cn[3];
+  for (int e = 0; e < 3; ++e) cn[e] = sqrt(J[0][e] * J[0][e] + J[1][e] * J[1][e] + J[2][e] * J[2][e]);
+  const double rho = (cutneigh + sqrt(del2)) * (1.0 + 1e-9) + 1e-6;
+  double *h = hull + 16 * (size_t)i;
+  const bool ok = isfinite(det) && fabs(det) > 1e-6 * cn[0] * cn[1] * cn[2];
+  for (int d = 0; d < 3; ++d) {
+    double rn = 0.0;
+    for (int e = 0; e < 3; ++e) {
+      const double v = ok ? A[d][e] / det : 0.0;
+      h[3 * d + e] = v;
+      rn += v * v;
+    }
+    h[9 + d] = ok ? (1.0 + rho * sqrt(rn)) * (1.0 + 1e-9) + 1e-9 : 1e300;
+  }
+  for (int d = 12; d < 16; ++d) h[d] = 0.0;
 }
 
 struct SubBlockGeom {
@@ -418,69 +587,6 @@ __device__ __forceinline__ void put_raw(uint64_t *__restrict__ out, int pos, uin
   reinterpret_cast<uint2 *>(out)[pos] = make_uint2(lo, hi);
 }
 
-// Sites of element image (cj, cimg) within cutneigh of point q: the
-// element's sub-blocks are tested lane-parallel against their radii, then the
-// (up to) 64 sites of each surviving 4x4x4 sub-block are taken one per lane.
-// Accepted sites are compacted in lane order into out[cnt..] (raw entries
-// cj | cimg << 27 | site << 32, capacity cap) and marked in cj's
-// referenced-site bitmap.  `self` applies the rsq > 1e-4 exclusion of the
-// own element (full_node_neighbor.cpp:216).  Wave-uniform; returns the new
-// count.
-__device__ int scan_sites(const IpGeom &ig, const double *__restrict__ nodex, const double *__restrict__ spa_site,
-                          const double *__restrict__ sbc, const Box3 &b, int cj, int cimg, bool self,
-                          const double q[3], double cutneigh, int cnt, int cap, uint64_t *__restrict__ out,
-                          uint32_t *__restrict__ bitmap, int lane) {
-  const double cutsq = cutneigh * cutneigh;
-  int s[3];
-  img_shift(cimg, s);
-  double X[24];
-  for (int k = 0; k < 8; ++k)
-    for (int d = 0; d < 3; ++d) X[3 * k + d] = shifted(nodex[24 * (size_t)cj + 3 * k + d], s[d], b.prd[d]);
-  for (int b0 = 0; b0 < ig.nsbt; b0 += 64) {
-    const int sbi = b0 + lane;
-    bool sk = false;
-    if (sbi < ig.nsbt) {
-      const double *sb = sbc + 4 * ((size_t)cj * ig.nsbt + sbi);
-      const double dx = q[0] - shifted(sb[0], s[0], b.prd[0]);
-      const double dy = q[1] - shifted(sb[1], s[1], b.prd[1]);
-      const double dz = q[2] - shifted(sb[2], s[2], b.prd[2]);
-      const double reach = cutneigh + sb[3];
-      sk = dx * dx + dy * dy + dz * dz <= reach * reach;
-    }
-    unsigned long long sm = __ballot(sk);
-    while (sm) {
-      const int sq = b0 + __ffsll((long long)sm) - 1;
-      sm &= sm - 1;
-      const int ba = sq / (ig.nsb[1] * ig.nsb[2]);
-      const int bb = (sq / ig.nsb[2]) % ig.nsb[1];
-      const int bc = sq % ig.nsb[2];
-      int lo[3], hi[3];
-      sb_range(ba, ig.nae[0], &lo[0], &hi[0]);
-      sb_range(bb, ig.nae[1], &lo[1], &hi[1]);
-      sb_range(bc, ig.nae[2], &lo[2], &hi[2]);
-      const int a = lo[0] + (lane >> 4), bq = lo[1] + ((lane >> 2) & 3), cq = lo[2] + (lane & 3);
-      bool acc = false;
-      int site = 0;
-      if (a <= hi[0] && bq <= hi[1] && cq <= hi[2]) {
-        site = (a * ig.nae[1] + bq) * ig.nae[2] + cq;
-        double xs[3];
-        interp_site(spa_site + 8 * site, X, xs);
-        const double delx = xs[0] - q[0], dely = xs[1] - q[1], delz = xs[2] - q[2];
-        const double rsq = delx * delx + dely * dely + delz * delz;
-        acc = rsq <= cutsq && (!self || rsq > 1.0e-4);
-      }
-      const unsigned long long am = __ballot(acc);
-      if (acc) {
-        const int pos = cnt + __popcll(am & ((1ull << lane) - 1ull));
-        if (pos < cap) put_raw(out, pos, (uint32_t)cj | ((uint32_t)cimg << 27), (uint32_t)site);
-        atomicOr(&bitmap[(size_t)cj * ig.nwords + (site >> 5)], 1u << (site & 31));
-      }
-      cnt += __popcll(am);
-    }
-  }
-  return cnt;
-}
-
 // Atom cells for the coupled searches.
 struct AtomBins {
   int n;
@@ -490,11 +596,37 @@ struct AtomBins {
   const int *bs, *be;
 };
 
+// The 27 cell ranges around a point, one per lane 0..26 (length 0 on the
+// other lanes), walked as one sequence: lane_prefix32 gives each lane the
+// inclusive prefix of the lengths, range_of the range holding position f
+// (all lanes must call it; f beyond the total gives a valid lane index).
+__device__ __forceinline__ int lane_prefix32(int len, int lane) {
+  int incl = len;
+  for (int o = 1; o < 32; o <<= 1) {
+    const int v = __shfl_up(incl, o);
+    if (lane >= o) incl += v;
+  }
+  return incl;
+}
+
+__device__ __forceinline__ int range_of(int incl, int f) {
+  int r = 0;  // number of ranges ending at or before f
+  for (int step = 16; step > 0; step >>= 1) {
+    const int idx = r + step - 1;
+    const int v = __shfl(incl, idx < 26 ? idx : 26);
+    if (idx < 27 && v <= f) r += step;
+  }
+  return r < 26 ? r : 26;
+}
+
 // Atoms within cutneigh of point p (coupled integration-point lists).  p may
 // lie outside the box (element nodes follow the element center): the cells
 // around its wrapped position q = p - t*prd are searched, and an atom found
-// with cell image s is image s + t of p.  Lanes take the atoms of one cell at
-// a time; accepted atoms are compacted in lane order as raw entries
+// with cell image s is image s + t of p.  Lanes 0..26 read the ranges of the
+// 27 cells at once (x fastest, cells beyond one period skipped); the ranges
+// are then walked as one sequence, 64 atoms at a time, each lane locating
+// its range by binary search over the prefix sums.  Accepted atoms are
+// compacted in lane order (cell order, then bin order) as raw entries
 // j | img << 27 | kAtomSite << 32.  Wave-uniform; returns the new count.
 __device__ int scan_atoms(const AtomBins &ab, const Box3 &b, const double p[3], double cutneigh, int cnt, int cap,
                           uint64_t *__restrict__ out, int lane) {
@@ -505,41 +637,130 @@ __device__ int scan_atoms(const AtomBins &ab, const Box3 &b, const double p[3], 
     const double q = p[d] - (double)t[d] * b.prd[d];
     ci[d] = cell_coord(q, b.lo[d], ab.g.inv[d], ab.g.n[d]);
   }
-  for (int dz = -1; dz <= 1; ++dz)
-    for (int dy = -1; dy <= 1; ++dy)
-      for (int dx = -1; dx <= 1; ++dx) {
-        int cc[3] = {ci[0] + dx, ci[1] + dy, ci[2] + dz}, s[3];
-        bool far = false;
-        for (int d = 0; d < 3; ++d) {
-          s[d] = t[d];
-          if (cc[d] < 0) { cc[d] += ab.g.n[d]; s[d] -= 1; }
-          else if (cc[d] >= ab.g.n[d]) { cc[d] -= ab.g.n[d]; s[d] += 1; }
-          far |= s[d] < -1 || s[d] > 1;
+  int qs = 0, len = 0, img = 0;
+  if (lane < 27) {
+    int cc[3] = {ci[0] + lane % 3 - 1, ci[1] + (lane / 3) % 3 - 1, ci[2] + lane / 9 - 1}, s[3];
+    bool far = false;
+    for (int d = 0; d < 3; ++d) {
+      s[d] = t[d];
+      if (cc[d] < 0) { cc[d] += ab.g.n[d]; s[d] -= 1; }
+      else if (cc[d] >= ab.g.n[d]) { cc[d] -= ab.g.n[d]; s[d] += 1; }
+      far |= s[d] < -1 || s[d] > 1;
+    }
+    if (!far) {  // beyond one period: out of reach when prd > 2*(cutneigh + R)
+      img = (s[0] + 1) + 3 * (s[1] + 1) + 9 * (s[2] + 1);
+      const int c = (cc[2] * ab.g.n[1] + cc[1]) * ab.g.n[0] + cc[0];
+      qs = ab.bs[c];
+      len = ab.be[c] - qs;
+    }
+  }
+  const int incl = lane_prefix32(len, lane), excl = incl - len;
+  const int total = __shfl(incl, 26);
+  for (int f0 = 0; f0 < total; f0 += 64) {
+    const int f = f0 + lane;
+    const int r = range_of(incl, f);
+    const int rs = __shfl(qs, r), rx = __shfl(excl, r), rimg = __shfl(img, r);
+    bool acc = false;
+    int j = 0;
+    if (f < total) {
+      int s[3];
+      img_shift(rimg, s);
+      j = (int)ab.bin[rs + f - rx];
+      const double delx = shifted(ab.x4[4 * (size_t)j], s[0], b.prd[0]) - p[0];
+      const double dely = shifted(ab.x4[4 * (size_t)j + 1], s[1], b.prd[1]) - p[1];
+      const double delz = shifted(ab.x4[4 * (size_t)j + 2], s[2], b.prd[2]) - p[2];
+      acc = delx * delx + dely * dely + delz * delz <= cutsq;
+    }
+    const unsigned long long am = __ballot(acc);
+    if (acc) {
+      const int pos = cnt + __popcll(am & ((1ull << lane) - 1ull));
+      if (pos < cap) put_raw(out, pos, (uint32_t)j | ((uint32_t)rimg << 27), kAtomSite);
+    }
+    cnt += __popcll(am);
+  }
+  return cnt;
+}
+
+// Sites of element image (cj, cimg) within cutneigh of each point q[g]
+// (g < kIpGroup) whose bit is set in gmask -- a group of integration points of
+// one element, or of nearby atoms.  The element's sub-blocks are tested
+// lane-parallel against their radii for every point; the (up to) 64 sites of
+// each sub-block that some point reaches are then taken one per lane,
+// interpolated once and tested against each such point.  Point g's accepted
+// sites are compacted in lane order into out + g*cap from cnt[g] on (raw
+// entries cj | cimg << 27 | site << 32, capacity cap); a site accepted by any
+// point is marked in cj's referenced-site bitmap.  `self` applies the
+// rsq > 1e-4 exclusion of the own element (full_node_neighbor.cpp:216).
+// Wave-uniform.
+constexpr int kIpGroup = 8;
+
+__device__ void scan_sites_group(const IpGeom &ig, const double *__restrict__ nodex,
+                                 const double *__restrict__ spa_site, const double *__restrict__ sbc, const Box3 &b,
+                                 int cj, int cimg, bool self, const double (*q)[3], unsigned gmask,
+                                 double cutneigh, int (&cnt)[kIpGroup], int cap, uint64_t *__restrict__ out,
+                                 uint32_t *__restrict__ bitmap, int lane) {
+  const double cutsq = cutneigh * cutneigh;
+  int s[3];
+  img_shift(cimg, s);
+  double X[24];
+  for (int k = 0; k < 8; ++k)
+    for (int d = 0; d < 3; ++d) X[3 * k + d] = shifted(nodex[24 * (size_t)cj + 3 * k + d], s[d], b.prd[d]);
+  const uint32_t lo32 = (uint32_t)cj | ((uint32_t)cimg << 27);
+  for (int b0 = 0; b0 < ig.nsbt; b0 += 64) {
+    const int sbi = b0 + lane;
+    unsigned sk = 0;
+    if (sbi < ig.nsbt) {
+      const double *sb = sbc + 4 * ((size_t)cj * ig.nsbt + sbi);
+      const double cx = shifted(sb[0], s[0], b.prd[0]);
+      const double cy = shifted(sb[1], s[1], b.prd[1]);
+      const double cz = shifted(sb[2], s[2], b.prd[2]);
+      const double reach = cutneigh + sb[3];
+#pragma unroll
+      for (int g = 0; g < kIpGroup; ++g)
+        if ((gmask >> g) & 1u) {
+          const double dx = q[g][0] - cx, dy = q[g][1] - cy, dz = q[g][2] - cz;
+          if (dx * dx + dy * dy + dz * dz <= reach * reach) sk |= 1u << g;
         }
-        if (far) continue;  // beyond one period: out of reach when prd > 2*(cutneigh + R)
-        const int img = (s[0] + 1) + 3 * (s[1] + 1) + 9 * (s[2] + 1);
-        const int c = (cc[2] * ab.g.n[1] + cc[1]) * ab.g.n[0] + cc[0];
-        const int qs = ab.bs[c], qe = ab.be[c];
-        for (int q0 = qs; q0 < qe; q0 += 64) {
-          const int qq = q0 + lane;
-          bool acc = false;
-          int j = 0;
-          if (qq < qe) {
-            j = (int)ab.bin[qq];
-            const double delx = shifted(ab.x4[4 * (size_t)j], s[0], b.prd[0]) - p[0];
-            const double dely = shifted(ab.x4[4 * (size_t)j + 1], s[1], b.prd[1]) - p[1];
-            const double delz = shifted(ab.x4[4 * (size_t)j + 2], s[2], b.prd[2]) - p[2];
-            acc = delx * delx + dely * dely + delz * delz <= cutsq;
-          }
+    }
+    unsigned long long sm = __ballot(sk != 0);
+    while (sm) {
+      const int src = __ffsll((long long)sm) - 1;
+      sm &= sm - 1;
+      const int sq = b0 + src;
+      const unsigned gm = (unsigned)__shfl((int)sk, src);
+      const int ba = sq / (ig.nsb[1] * ig.nsb[2]);
+      const int bb = (sq / ig.nsb[2]) % ig.nsb[1];
+      const int bc = sq % ig.nsb[2];
+      int lo[3], hi[3];
+      sb_range(ba, ig.nae[0], &lo[0], &hi[0]);
+      sb_range(bb, ig.nae[1], &lo[1], &hi[1]);
+      sb_range(bc, ig.nae[2], &lo[2], &hi[2]);
+      const int a = lo[0] + (lane >> 4), bq = lo[1] + ((lane >> 2) & 3), cq = lo[2] + (lane & 3);
+      const bool valid = a <= hi[0] && bq <= hi[1] && cq <= hi[2];
+      int site = 0;
+      double xs[3] = {0.0, 0.0, 0.0};
+      if (valid) {
+        site = (a * ig.nae[1] + bq) * ig.nae[2] + cq;
+        interp_site(spa_site + 8 * site, X, xs);
+      }
+      bool any = false;
+#pragma unroll
+      for (int g = 0; g < kIpGroup; ++g)
+        if ((gm >> g) & 1u) {
+          const double delx = xs[0] - q[g][0], dely = xs[1] - q[g][1], delz = xs[2] - q[g][2];
+          const double rsq = delx * delx + dely * dely + delz * delz;
+          const bool acc = valid && rsq <= cutsq && (!self || rsq > 1.0e-4);
           const unsigned long long am = __ballot(acc);
           if (acc) {
-            const int pos = cnt + __popcll(am & ((1ull << lane) - 1ull));
-            if (pos < cap) put_raw(out, pos, (uint32_t)j | ((uint32_t)img << 27), kAtomSite);
+            const int pos = cnt[g] + __popcll(am & ((1ull << lane) - 1ull));
+            if (pos < cap) put_raw(out + (size_t)g * cap, pos, lo32, (uint32_t)site);
           }
-          cnt += __popcll(am);
+          cnt[g] += __popcll(am);
+          any |= acc;
         }
-      }
-  return cnt;
+      if (any) atomicOr(&bitmap[(size_t)cj * ig.nwords + (site >> 5)], 1u << (site & 31));
+    }
+  }
 }
 
 // Integration-point lists (Neighbor::node_neighbor,
@@ -549,36 +770,51 @@ __device__ int scan_atoms(const AtomBins &ab, const Box3 &b, const double p[3], 
 // the integration point's own site; with coupling, then the atoms within
 // cutneigh.
 //
-// One wavefront per integration point.  Lanes test 64 candidate elements at
-// a time against the element radius; each survivor is scanned by scan_sites.
-// The list order is deterministic.
+// One wavefront per group of up to kIpGroup integration points of one
+// element (they share the element's candidate list).  Lanes test 64
+// candidate elements at a time against the element radius, for every point
+// of the group; each survivor is scanned once for the points that reach it
+// (scan_sites_group).  Per point, the list order is candidate order, then
+// sub-block order, then site order: deterministic.
 __global__ __launch_bounds__(256) void k_ip_neigh(IpGeom ig, const double *__restrict__ nodex,
                                                   const double *__restrict__ spa_int,
                                                   const double *__restrict__ spa_site,
-                                                  const double *__restrict__ erad, const double *__restrict__ sbc,
+                                                  const double *__restrict__ erad, const double *__restrict__ hull,
+                                                  const double *__restrict__ sbc,
                                                   const uint32_t *__restrict__ eneigh, const int *__restrict__ enumb,
                                                   Box3 b, double cutneigh, int maxnip, uint64_t *__restrict__ tmp,
                                                   int *__restrict__ ipnum, uint32_t *__restrict__ bitmap,
                                                   int *__restrict__ flags, bool coupled, AtomBins ab,
                                                   int *__restrict__ etouch) {
-  const int nip = ig.ne * ig.nint;
+  const int ngrp = (ig.nint + kIpGroup - 1) / kIpGroup;
   const int lane = threadIdx.x & 63;
-  const int p = (xcd_block() * (int)blockDim.x + (int)threadIdx.x) >> 6;
-  if (p >= nip) return;  // whole wavefront
-  const int i = p / ig.nint, g = p % ig.nint;
-  double ip[3];
-  {
+  const int wv = (xcd_block() * (int)blockDim.x + (int)threadIdx.x) >> 6;
+  if (wv >= ig.ne * ngrp) return;  // whole wavefront
+  const int i = wv / ngrp, g0 = (wv % ngrp) * kIpGroup;
+  const int ng = ig.nint - g0 < kIpGroup ? ig.nint - g0 : kIpGroup;
+  // the group's points in LDS (lane g interpolates point g)
+  __shared__ double sip[4][kIpGroup][3];
+  double (*ip)[3] = sip[threadIdx.x >> 6];
+  if (lane < kIpGroup) {
     double Xi[24];
     for (int q = 0; q < 24; ++q) Xi[q] = nodex[24 * (size_t)i + q];
-    interp_site(spa_int + 8 * g, Xi, ip);
+    double t[3] = {0.0, 0.0, 0.0};
+    if (lane < ng) interp_site(spa_int + 8 * (g0 + lane), Xi, t);
+    ip[lane][0] = t[0];
+    ip[lane][1] = t[1];
+    ip[lane][2] = t[2];
   }
-  uint64_t *out = tmp + (size_t)p * maxnip;
-  int cnt = 0;
+  wave_sync();
+  const unsigned all = (1u << ng) - 1u;
+  uint64_t *out = tmp + ((size_t)i * ig.nint + g0) * maxnip;
+  int cnt[kIpGroup];
+#pragma unroll
+  for (int g = 0; g < kIpGroup; ++g) cnt[g] = 0;
   const int nel = enumb[i] + 1;  // candidates, then the element itself
   for (int l0 = 0; l0 < nel; l0 += 64) {
     const int l = l0 + lane;
     int j = i, img = kImgSelf;
-    bool keep = false;
+    unsigned keep = 0;
     if (l < nel) {
       if (l < nel - 1) {
         const uint32_t e = eneigh[(size_t)l * ig.ne + i];
@@ -587,105 +823,67 @@ __global__ __launch_bounds__(256) void k_ip_neigh(IpGeom ig, const double *__res
       }
       int s[3];
       img_shift(img, s);
-      const double dx = ip[0] - shifted(erad[4 * j], s[0], b.prd[0]);
-      const double dy = ip[1] - shifted(erad[4 * j + 1], s[1], b.prd[1]);
-      const double dz = ip[2] - shifted(erad[4 * j + 2], s[2], b.prd[2]);
+      const double cx = shifted(erad[4 * j], s[0], b.prd[0]);
+      const double cy = shifted(erad[4 * j + 1], s[1], b.prd[1]);
+      const double cz = shifted(erad[4 * j + 2], s[2], b.prd[2]);
       const double reach = cutneigh + erad[4 * j + 3];
-      keep = dx * dx + dy * dy + dz * dz <= reach * reach;
+#pragma unroll
+      for (int g = 0; g < kIpGroup; ++g) {
+        const double dx = ip[g][0] - cx, dy = ip[g][1] - cy, dz = ip[g][2] - cz;
+        if (dx * dx + dy * dy + dz * dz <= reach * reach && hull_hit(hull, j, dx, dy, dz)) keep |= 1u << g;
+      }
+      keep &= all;
     }
-    unsigned long long cm = __ballot(keep);
+    unsigned long long cm = __ballot(keep != 0);
     while (cm) {
       const int src = __ffsll((long long)cm) - 1;
       cm &= cm - 1;
       const int cj = __shfl(j, src);
       const int cimg = __shfl(img, src);
-      cnt = scan_sites(ig, nodex, spa_site, sbc, b, cj, cimg, cj == i && cimg == kImgSelf, ip, cutneigh, cnt, maxnip,
-                       out, bitmap, lane);
+      const unsigned gm = (unsigned)__shfl((int)keep, src);
+      scan_sites_group(ig, nodex, spa_site, sbc, b, cj, cimg, cj == i && cimg == kImgSelf, ip, gm, cutneigh, cnt,
+                       maxnip, out, bitmap, lane);
     }
   }
-  if (coupled) {
-    const int c0 = cnt;
-    cnt = scan_atoms(ab, b, ip, cutneigh, cnt, maxnip, out, lane);
-    if (lane == 0 && cnt > c0) etouch[i] = 1;
-  }
-  if (lane == 0) {
-    ipnum[p] = cnt;
-    if (cnt > maxnip) {
-      atomicOr(&flags[4], 1);
-      atomicMax(&flags[5], cnt);
+  bool touched = false;
+  for (int g = 0; g < ng; ++g) {
+    int c = 0;  // cnt[g] without dynamic register indexing
+#pragma unroll
+    for (int h = 0; h < kIpGroup; ++h)
+      if (h == g) c = cnt[h];
+    const double p[3] = {ip[g][0], ip[g][1], ip[g][2]};
+    if (coupled) {
+      const int c0 = c;
+      c = scan_atoms(ab, b, p, cutneigh, c, maxnip, out + (size_t)g * maxnip, lane);
+      touched |= c > c0;
+    }
+    if (lane == 0) {
+      ipnum[(size_t)i * ig.nint + g0 + g] = c;
+      if (c > maxnip) {
+        atomicOr(&flags[4], 1);
+        atomicMax(&flags[5], c);
+      }
     }
   }
+  if (lane == 0 && touched) etouch[i] = 1;
 }
 
-// Element images whose center is within cutneigh + R_j of point q, found
-// through the element cells (or, for boxes under 3 element cells per
-// dimension, all elements and images); fn(j, img) is called wave-uniformly
-// for each, in cell order then ascending index.  Lanes test 64 candidates at
-// a time.
+// Element cells for the coupled atom searches (grid false: fewer than 3
+// element cells per dimension, every element and image is a candidate).
 struct ElemBins {
   int ne;
-  const double *erad;
+  const double *erad, *hull;
   bool grid;
   Grid3 g;
   const uint32_t *bin;
   const int *bs, *be;
 };
 
-template <class Fn>
-__device__ void for_elements_near(const ElemBins &eb, const Box3 &b, const double q[3], double cutneigh, int lane,
-                                  Fn fn) {
-  auto test = [&](int j, int img) {
-    int s[3];
-    img_shift(img, s);
-    const double dx = q[0] - shifted(eb.erad[4 * j], s[0], b.prd[0]);
-    const double dy = q[1] - shifted(eb.erad[4 * j + 1], s[1], b.prd[1]);
-    const double dz = q[2] - shifted(eb.erad[4 * j + 2], s[2], b.prd[2]);
-    const double reach = cutneigh + eb.erad[4 * j + 3];
-    return dx * dx + dy * dy + dz * dz <= reach * reach;
-  };
-  auto run = [&](bool keep, int j, int img) {
-    unsigned long long cm = __ballot(keep);
-    while (cm) {
-      const int src = __ffsll((long long)cm) - 1;
-      cm &= cm - 1;
-      fn(__shfl(j, src), __shfl(img, src));
-    }
-  };
-  if (eb.grid) {
-    const int ci[3] = {cell_coord(q[0], b.lo[0], eb.g.inv[0], eb.g.n[0]),
-                       cell_coord(q[1], b.lo[1], eb.g.inv[1], eb.g.n[1]),
-                       cell_coord(q[2], b.lo[2], eb.g.inv[2], eb.g.n[2])};
-    for (int dz = -1; dz <= 1; ++dz)
-      for (int dy = -1; dy <= 1; ++dy)
-        for (int dx = -1; dx <= 1; ++dx) {
-          int cc[3] = {ci[0] + dx, ci[1] + dy, ci[2] + dz}, s[3];
-          for (int d = 0; d < 3; ++d) {
-            s[d] = 0;
-            if (cc[d] < 0) { cc[d] += eb.g.n[d]; s[d] = -1; }
-            else if (cc[d] >= eb.g.n[d]) { cc[d] -= eb.g.n[d]; s[d] = 1; }
-          }
-          const int img = (s[0] + 1) + 3 * (s[1] + 1) + 9 * (s[2] + 1);
-          const int c = (cc[2] * eb.g.n[1] + cc[1]) * eb.g.n[0] + cc[0];
-          const int qs = eb.bs[c], qe = eb.be[c];
-          for (int q0 = qs; q0 < qe; q0 += 64) {
-            const int qq = q0 + lane;
-            const int j = qq < qe ? (int)eb.bin[qq] : 0;
-            run(qq < qe && test(j, img), j, img);
-          }
-        }
-  } else {
-    for (int img = 0; img < 27; ++img)
-      for (int j0 = 0; j0 < eb.ne; j0 += 64) {
-        const int j = j0 + lane;
-        run(j < eb.ne && test(j, img), j, img);
-      }
-  }
-}
-
 // Coupled atom lists, step 1: flag the atoms that some element image can
-// reach (|x - c_j'| <= cutneigh + R_j).  One thread per atom.
-__global__ void k_atom_iface(int n, const double *__restrict__ x4, ElemBins eb, Box3 b, double cutneigh,
-                             int *__restrict__ iflag) {
+// reach (|x - c_j'| <= cutneigh + R_j and inside the expanded hull).  One
+// thread per atom.
+__global__ __launch_bounds__(256) void k_atom_iface(int n, const double *__restrict__ x4, ElemBins eb, Box3 b,
+                                                    double cutneigh, int *__restrict__ iflag) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double q[3] = {x4[4 * (size_t)i], x4[4 * (size_t)i + 1], x4[4 * (size_t)i + 2]};
@@ -696,26 +894,24 @@ __global__ void k_atom_iface(int n, const double *__restrict__ x4, ElemBins eb, 
     const double dy = q[1] - shifted(eb.erad[4 * j + 1], s[1], b.prd[1]);
     const double dz = q[2] - shifted(eb.erad[4 * j + 2], s[2], b.prd[2]);
     const double reach = cutneigh + eb.erad[4 * j + 3];
-    return dx * dx + dy * dy + dz * dz <= reach * reach;
+    return dx * dx + dy * dy + dz * dz <= reach * reach && hull_hit(eb.hull, j, dx, dy, dz);
   };
   int found = 0;
   if (eb.grid) {
     const int ci[3] = {cell_coord(q[0], b.lo[0], eb.g.inv[0], eb.g.n[0]),
                        cell_coord(q[1], b.lo[1], eb.g.inv[1], eb.g.n[1]),
                        cell_coord(q[2], b.lo[2], eb.g.inv[2], eb.g.n[2])};
-    for (int dz = -1; dz <= 1 && !found; ++dz)
-      for (int dy = -1; dy <= 1 && !found; ++dy)
-        for (int dx = -1; dx <= 1 && !found; ++dx) {
-          int cc[3] = {ci[0] + dx, ci[1] + dy, ci[2] + dz}, s[3];
-          for (int d = 0; d < 3; ++d) {
-            s[d] = 0;
-            if (cc[d] < 0) { cc[d] += eb.g.n[d]; s[d] = -1; }
-            else if (cc[d] >= eb.g.n[d]) { cc[d] -= eb.g.n[d]; s[d] = 1; }
-          }
-          const int img = (s[0] + 1) + 3 * (s[1] + 1) + 9 * (s[2] + 1);
-          const int c = (cc[2] * eb.g.n[1] + cc[1]) * eb.g.n[0] + cc[0];
-          for (int qq = eb.bs[c]; qq < eb.be[c] && !found; ++qq) found = hit((int)eb.bin[qq], img);
-        }
+    // the 27 ranges are read up front (independent loads), then scanned
+    int rs[27], re[27], im[27];
+#pragma unroll
+    for (int t = 0; t < 27; ++t) {
+      const int c = cell_around(ci, t, eb.g, &im[t]);
+      rs[t] = eb.bs[c];
+      re[t] = eb.be[c];
+    }
+#pragma unroll
+    for (int t = 0; t < 27; ++t)
+      for (int qq = rs[t]; qq < re[t] && !found; ++qq) found = hit((int)eb.bin[qq], im[t]);
   } else {
     for (int img = 0; img < 27 && !found; ++img)
       for (int j = 0; j < eb.ne && !found; ++j) found = hit(j, img);
@@ -729,9 +925,16 @@ __global__ void k_iface_compact(int n, const int *__restrict__ iflag, const int 
   if (i < n && iflag[i]) iface[ipos[i]] = i;
 }
 
-// Coupled atom lists, step 2: one wavefront per flagged atom collects the
-// element sites within cutneigh (raw entries, after the atom's own atom
-// entries, capacity maxneigh in total).
+// Coupled atom lists, step 2: the element sites within cutneigh of each
+// flagged atom (raw entries, after the atom's own atom entries, capacity
+// maxneigh in total).  One wavefront per kIpGroup consecutive flagged atoms
+// (consecutive in index, so close in space).  The atoms of the group that
+// share an element cell share its candidate walk: each candidate element
+// image (the 27 cells around that cell, in cell then bin order; every image
+// of every element for a box under 3 element cells wide) is tested against
+// each of those atoms (radius, then hull), and each survivor is scanned once
+// for the atoms that reach it (scan_sites_group).  Per atom the entries come
+// in candidate order, then sub-block order, then site order.
 __global__ __launch_bounds__(256) void k_atom_sites(int niface, const int *__restrict__ iface,
                                                     const double *__restrict__ x4, const int *__restrict__ anum,
                                                     IpGeom ig, const double *__restrict__ nodex,
@@ -741,23 +944,102 @@ __global__ __launch_bounds__(256) void k_atom_sites(int niface, const int *__res
                                                     int *__restrict__ asnum, uint32_t *__restrict__ bitmap,
                                                     int *__restrict__ etouch, int *__restrict__ flags) {
   const int lane = threadIdx.x & 63;
-  const int r = (xcd_block() * (int)blockDim.x + (int)threadIdx.x) >> 6;
-  if (r >= niface) return;  // whole wavefront
-  const int i = iface[r];
-  const double q[3] = {x4[4 * (size_t)i], x4[4 * (size_t)i + 1], x4[4 * (size_t)i + 2]};
-  uint64_t *out = atmp + (size_t)r * maxneigh;
-  int cnt = 0;
-  for_elements_near(eb, b, q, cutneigh, lane, [&](int cj, int cimg) {
-    const int c0 = cnt;
-    cnt = scan_sites(ig, nodex, spa_site, sbc, b, cj, cimg, false, q, cutneigh, cnt, maxneigh, out, bitmap, lane);
-    if (lane == 0 && cnt > c0) etouch[cj] = 1;
-  });
-  if (lane == 0) {
-    asnum[r] = cnt;
-    const int tot = anum[i] + cnt;
-    if (tot > maxneigh) {
-      atomicOr(&flags[0], 1);
-      atomicMax(&flags[1], tot);
+  const int r0 = ((xcd_block() * (int)blockDim.x + (int)threadIdx.x) >> 6) * kIpGroup;
+  if (r0 >= niface) return;  // whole wavefront
+  const int ng = niface - r0 < kIpGroup ? niface - r0 : kIpGroup;
+  __shared__ double sq[4][kIpGroup][3];
+  double (*q)[3] = sq[threadIdx.x >> 6];
+  int cell = 0;
+  if (lane < ng) {
+    const int i = iface[r0 + lane];
+    const double p[3] = {x4[4 * (size_t)i], x4[4 * (size_t)i + 1], x4[4 * (size_t)i + 2]};
+    q[lane][0] = p[0];
+    q[lane][1] = p[1];
+    q[lane][2] = p[2];
+    if (eb.grid)
+      cell = (cell_coord(p[2], b.lo[2], eb.g.inv[2], eb.g.n[2]) * eb.g.n[1] +
+              cell_coord(p[1], b.lo[1], eb.g.inv[1], eb.g.n[1])) * eb.g.n[0] +
+             cell_coord(p[0], b.lo[0], eb.g.inv[0], eb.g.n[0]);
+  }
+  wave_sync();
+  uint64_t *out = atmp + (size_t)r0 * maxneigh;
+  int cnt[kIpGroup];
+#pragma unroll
+  for (int g = 0; g < kIpGroup; ++g) cnt[g] = 0;
+  unsigned todo = (1u << ng) - 1u;
+  while (todo) {
+    const int lc = __shfl(cell, __ffs((int)todo) - 1);
+    const unsigned gmask = (unsigned)__ballot(lane < ng && cell == lc) & todo;
+    todo &= ~gmask;
+    auto visit = [&](bool valid, int j, int img) {
+      unsigned keep = 0;
+      if (valid) {
+        int s[3];
+        img_shift(img, s);
+        const double cx = shifted(eb.erad[4 * j], s[0], b.prd[0]);
+        const double cy = shifted(eb.erad[4 * j + 1], s[1], b.prd[1]);
+        const double cz = shifted(eb.erad[4 * j + 2], s[2], b.prd[2]);
+        const double reach = cutneigh + eb.erad[4 * j + 3];
+#pragma unroll
+        for (int g = 0; g < kIpGroup; ++g)
+          if ((gmask >> g) & 1u) {
+            const double dx = q[g][0] - cx, dy = q[g][1] - cy, dz = q[g][2] - cz;
+            if (dx * dx + dy * dy + dz * dz <= reach * reach && hull_hit(eb.hull, j, dx, dy, dz)) keep |= 1u << g;
+          }
+      }
+      unsigned long long cm = __ballot(keep != 0);
+      while (cm) {
+        const int src = __ffsll((long long)cm) - 1;
+        cm &= cm - 1;
+        const int cj = __shfl(j, src), cimg = __shfl(img, src);
+        const unsigned gm = (unsigned)__shfl((int)keep, src);
+        int before = 0, after = 0;
+#pragma unroll
+        for (int g = 0; g < kIpGroup; ++g) before += cnt[g];
+        scan_sites_group(ig, nodex, spa_site, sbc, b, cj, cimg, false, q, gm, cutneigh, cnt, maxneigh, out, bitmap,
+                         lane);
+#pragma unroll
+        for (int g = 0; g < kIpGroup; ++g) after += cnt[g];
+        if (lane == 0 && after > before) etouch[cj] = 1;
+      }
+    };
+    if (eb.grid) {
+      int ci[3], qs = 0, len = 0, img = 0;
+      cell_split(lc, eb.g, ci);
+      if (lane < 27) {
+        const int c = cell_around(ci, lane, eb.g, &img);
+        qs = eb.bs[c];
+        len = eb.be[c] - qs;
+      }
+      const int incl = lane_prefix32(len, lane), excl = incl - len;
+      const int total = __shfl(incl, 26);
+      for (int f0 = 0; f0 < total; f0 += 64) {
+        const int f = f0 + lane;
+        const int r = range_of(incl, f);
+        const int rs = __shfl(qs, r), rx = __shfl(excl, r), rimg = __shfl(img, r);
+        visit(f < total, f < total ? (int)eb.bin[rs + f - rx] : 0, rimg);
+      }
+    } else {
+      const int tot = 27 * eb.ne;  // image-major, then element index
+      for (int f0 = 0; f0 < tot; f0 += 64) {
+        const int f = f0 + lane;
+        const bool v = f < tot;
+        visit(v, v ? f % eb.ne : 0, v ? f / eb.ne : 0);
+      }
+    }
+  }
+  for (int g = 0; g < ng; ++g) {
+    int c = 0;  // cnt[g] without dynamic register indexing
+#pragma unroll
+    for (int h = 0; h < kIpGroup; ++h)
+      if (h == g) c = cnt[h];
+    if (lane == 0) {
+      asnum[r0 + g] = c;
+      const int tot = anum[iface[r0 + g]] + c;
+      if (tot > maxneigh) {
+        atomicOr(&flags[0], 1);
+        atomicMax(&flags[1], tot);
+      }
     }
   }
 }
@@ -958,6 +1240,7 @@ ElemBins elem_bins(const cac_ctx *ctx) {
   ElemBins eb;
   eb.ne = ctx->ne;
   eb.erad = ctx->erad;
+  eb.hull = ctx->ehull;
   eb.grid = ctx->ecell[0] > 0;
   for (int d = 0; d < 3; ++d) {
     eb.g.n[d] = ctx->ecell[d];
@@ -1017,12 +1300,40 @@ int launch_atom_neighbors(cac_ctx *ctx) {
   ctx->abin_e = ce;
   hipEvent_t t0;
   timer_begin(ctx, "atom_neigh", &t0);
+  int *hflag, *cells;
+  if ((rc = ensure(ctx, ctx->b_ahead, 2 * ((size_t)n + 1), &hflag)) || (rc = ensure(ctx, ctx->b_acells, n, &cells)))
+    return rc;
+  int *hpos = hflag + n + 1;
+  const uint32_t *key2 = static_cast<const uint32_t *>(ctx->b_akey2.p);
+  hipLaunchKernelGGL(k_cell_heads, dim3(grid_for(n + 1)), dim3(kBlock), 0, ctx->stream, n, key2, hflag);
+  size_t tmp_bytes = 0;
+  CAC_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, hflag, hpos, n + 1, ctx->stream));
+  char *tmp;
+  if ((rc = ensure(ctx, ctx->b_scantmp, tmp_bytes, &tmp))) return rc;
+  CAC_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, hflag, hpos, n + 1, ctx->stream));
+  CAC_HIP(ctx, hipMemsetAsync(ctx->dflags + 13, 0, sizeof(int), ctx->stream));
+  hipLaunchKernelGGL(k_cell_list, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, n, key2, hflag, hpos, g, cs, ce,
+                     cells, ctx->dflags + 13);
+  int hv[2] = {0, 0};
+  CAC_HIP(ctx, hipMemcpyAsync(&hv[0], hpos + n, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  CAC_HIP(ctx, hipMemcpyAsync(&hv[1], ctx->dflags + 13, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  const int ncells = hv[0], maxcand = hv[1];
   int cap2 = 64;
   while (cap2 < ctx->maxneigh) cap2 <<= 1;
+  // LDS: the four wavefronts' key rows, then up to ccap staged candidates
+  // (28 B each); cells with more candidates read them from global memory.
+  const size_t kLdsMax = 64 * 1024, key_bytes = (size_t)kNeighWaves * cap2 * sizeof(uint64_t);
+  if (key_bytes + 64 * 28 > kLdsMax)
+    return fail(ctx, CAC_ERR_OVERFLOW, "atom list capacity above 1024 entries per atom");
+  int ccap = round_up(maxcand, 64);
+  const int ccap_max = (int)((kLdsMax - key_bytes) / 28) / 64 * 64;
+  ccap = ccap < ccap_max ? ccap : ccap_max;
+  const size_t lds = key_bytes + (size_t)ccap * 28;
   const double qinv = ctx->list_spacing > 0.0 ? 1.0 / ctx->list_spacing : 0.0;
-  hipLaunchKernelGGL(k_atom_neigh, dim3(xcd_grid(n, 1)), dim3(64), cap2 * sizeof(uint64_t), ctx->stream, n, ctx->x4,
-                     box_of(ctx), g, sorted, cs, ce, ctx->cutneigh * ctx->cutneigh, ctx->maxneigh, cap2, qinv,
-                     ctx->aneigh, ctx->anum, ctx->xhold, ctx->dflags);
+  hipLaunchKernelGGL(k_atom_neigh, dim3(xcd_grid(ncells, 1)), dim3(64 * kNeighWaves), lds, ctx->stream, ncells, cells,
+                     ctx->x4, box_of(ctx), g, sorted, cs, ce, ctx->cutneigh * ctx->cutneigh, ctx->maxneigh, cap2, ccap,
+                     qinv, n, ctx->aneigh, ctx->anum, ctx->xhold, ctx->dflags);
   timer_end(ctx, "atom_neigh", t0);
   CAC_HIP(ctx, hipGetLastError());
   return CAC_OK;
@@ -1033,13 +1344,15 @@ int launch_element_neighbors(cac_ctx *ctx) {
   if (ne == 0) return CAC_OK;
   int rc;
   unsigned long long *rmax_bits;
-  if ((rc = ensure(ctx, ctx->b_erad, 4 * (size_t)ne, &ctx->erad))) return rc;
+  if ((rc = ensure(ctx, ctx->b_erad, 4 * (size_t)ne, &ctx->erad)) ||
+      (rc = ensure(ctx, ctx->b_ehull, 16 * (size_t)ne, &ctx->ehull)))
+    return rc;
   // scratch for the radius max lives in the reduction buffer
   if ((rc = ensure(ctx, ctx->b_red, 64 * 16, &ctx->red))) return rc;
   rmax_bits = reinterpret_cast<unsigned long long *>(ctx->red);
   CAC_HIP(ctx, hipMemsetAsync(rmax_bits, 0, sizeof(unsigned long long), ctx->stream));
   hipLaunchKernelGGL(k_elem_radius, dim3(grid_for(ne)), dim3(kBlock), 0, ctx->stream, ne, ctx->nodex, ctx->xc,
-                     ctx->erad, rmax_bits);
+                     ctx->cutneigh, ctx->erad, ctx->ehull, rmax_bits);
   // sub-block geometry
   for (int d = 0; d < 3; ++d) ctx->nsb[d] = (ctx->nae[d] + kSubBlock - 1) / kSubBlock;
   ctx->nsb_total = ctx->nsb[0] * ctx->nsb[1] * ctx->nsb[2];
@@ -1122,8 +1435,9 @@ int launch_ip_neighbors(cac_ctx *ctx) {
   ig.nwords = ctx->nwords;
   hipEvent_t t0;
   timer_begin(ctx, "ip_neigh", &t0);
-  hipLaunchKernelGGL(k_ip_neigh, dim3(xcd_grid(64L * nip, kBlock)), dim3(kBlock), 0, ctx->stream, ig, ctx->nodex,
-                     ctx->spa_int, ctx->spa_site, ctx->erad, ctx->sbc, ctx->eneigh, ctx->enumb, box_of(ctx),
+  const long waves = (long)ne * ((ctx->nint + kIpGroup - 1) / kIpGroup);
+  hipLaunchKernelGGL(k_ip_neigh, dim3(xcd_grid(64L * waves, kBlock)), dim3(kBlock), 0, ctx->stream, ig, ctx->nodex,
+                     ctx->spa_int, ctx->spa_site, ctx->erad, ctx->ehull, ctx->sbc, ctx->eneigh, ctx->enumb, box_of(ctx),
                      ctx->cutneigh, ctx->maxnip, ctx->iptmp, ctx->ipnum, ctx->bitmap, ctx->dflags, coupled,
                      atom_bins(ctx), ctx->etouch);
   timer_end(ctx, "ip_neigh", t0);
@@ -1169,7 +1483,8 @@ int launch_atom_sites(cac_ctx *ctx) {
     return rc;
   hipLaunchKernelGGL(k_iface_compact, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, n, ctx->iflag, ctx->ipos,
                      ctx->iface);
-  hipLaunchKernelGGL(k_atom_sites, dim3(xcd_grid(64L * niface, kBlock)), dim3(kBlock), 0, ctx->stream, niface,
+  const long groups = (niface + kIpGroup - 1) / kIpGroup;
+  hipLaunchKernelGGL(k_atom_sites, dim3(xcd_grid(64L * groups, kBlock)), dim3(kBlock), 0, ctx->stream, niface,
                      ctx->iface, ctx->x4, ctx->anum, ip_geom(ctx), ctx->nodex, ctx->spa_site, ctx->sbc,
                      elem_bins(ctx), box_of(ctx), ctx->cutneigh, ctx->maxneigh, ctx->atmp, ctx->asnum, ctx->bitmap,
                      ctx->etouch, ctx->dflags);
