@@ -576,6 +576,7 @@ __global__ void k_elem_neigh_brute(int ne, const double *__restrict__ erad, Box3
 // is also marked in the element's referenced-site bitmap.
 struct IpGeom {
   int ne, nint, nae[3], nsb[3], nsbt, nwords;
+  double pstep[3];  // parent-coordinate step 2/(nae-1), as the host computes it
 };
 
 // Raw entry store as two 32-bit halves (lo = j | img << 27, hi = site).
@@ -691,25 +692,71 @@ __device__ int scan_atoms(const AtomBins &ab, const Box3 &b, const double p[3], 
 // entries cj | cimg << 27 | site << 32, capacity cap); a site accepted by any
 // point is marked in cj's referenced-site bitmap.  `self` applies the
 // rsq > 1e-4 exclusion of the own element (full_node_neighbor.cpp:216).
-// Wave-uniform.
+// kBox: enumerate only the sub-blocks inside the points' parent-coordinate
+// reach (below), else all of them.  Wave-uniform.
 constexpr int kIpGroup = 8;
 
+template <bool kBox>
 __device__ void scan_sites_group(const IpGeom &ig, const double *__restrict__ nodex,
-                                 const double *__restrict__ spa_site, const double *__restrict__ sbc, const Box3 &b,
+                                 const double *__restrict__ spa_site, const double *__restrict__ sbc,
+                                 const double *__restrict__ erad, const double *__restrict__ hull, const Box3 &b,
                                  int cj, int cimg, bool self, const double (*q)[3], unsigned gmask,
                                  double cutneigh, int (&cnt)[kIpGroup], int cap, uint64_t *__restrict__ out,
-                                 uint32_t *__restrict__ bitmap, int lane) {
+                                 uint32_t *__restrict__ bitmap, double *X, int lane) {
   const double cutsq = cutneigh * cutneigh;
   int s[3];
   img_shift(cimg, s);
-  double X[24];
-  for (int k = 0; k < 8; ++k)
-    for (int d = 0; d < 3; ++d) X[3 * k + d] = shifted(nodex[24 * (size_t)cj + 3 * k + d], s[d], b.prd[d]);
+  // the image's node coordinates, in the wavefront's LDS slot X[24]
+  wave_sync();  // earlier readers of X are done
+  if (lane < 24) X[lane] = shifted(nodex[24 * (size_t)cj + lane], s[lane % 3], b.prd[lane % 3]);
+  wave_sync();
   const uint32_t lo32 = (uint32_t)cj | ((uint32_t)cimg << 27);
-  for (int b0 = 0; b0 < ig.nsbt; b0 += 64) {
-    const int sbi = b0 + lane;
+  // Sub-blocks within parent-coordinate reach of some point: a site with
+  // parent coordinates xi within cutneigh of q has |xi_d - u_d| <=
+  // rho |row_d(J^-1)| < hull bound_d - 1, u = J^-1 (q - c') (hull_hit), and
+  // site index k along d has xi_d = k * 2/(nae_d - 1) - 1.  The sub-blocks of
+  // the union of the points' index boxes are enumerated in index order.
+  int ulo[3] = {ig.nsb[0], ig.nsb[1], ig.nsb[2]}, uhi[3] = {-1, -1, -1};
+  if (!kBox) {
+    for (int d = 0; d < 3; ++d) {
+      ulo[d] = 0;
+      uhi[d] = ig.nsb[d] - 1;
+    }
+  } else {
+    const double *h = hull + 16 * (size_t)cj;
+    const double c0 = shifted(erad[4 * cj], s[0], b.prd[0]);
+    const double c1 = shifted(erad[4 * cj + 1], s[1], b.prd[1]);
+    const double c2 = shifted(erad[4 * cj + 2], s[2], b.prd[2]);
+#pragma unroll
+    for (int g = 0; g < kIpGroup; ++g)
+      if ((gmask >> g) & 1u) {
+        const double dx = q[g][0] - c0, dy = q[g][1] - c1, dz = q[g][2] - c2;
+        int lo[3], hi[3];
+        bool reach = true;
+        for (int d = 0; d < 3; ++d) {
+          const double u = h[3 * d] * dx + h[3 * d + 1] * dy + h[3 * d + 2] * dz;
+          const double w = h[9 + d] - 1.0, half = 0.5 * (double)(ig.nae[d] - 1);
+          const double flo = fmax((u - w + 1.0) * half - 1e-6, 0.0);
+          const double fhi = fmin((u + w + 1.0) * half + 1e-6, (double)(ig.nae[d] - 1));
+          lo[d] = (int)ceil(flo);
+          hi[d] = (int)floor(fhi);
+          reach = reach && lo[d] <= hi[d];
+        }
+        if (reach)
+          for (int d = 0; d < 3; ++d) {
+            ulo[d] = min(ulo[d], lo[d] / kSubBlock);
+            uhi[d] = max(uhi[d], hi[d] / kSubBlock);
+          }
+      }
+  }
+  const int nbx = uhi[0] - ulo[0] + 1, nby = uhi[1] - ulo[1] + 1, nbz = uhi[2] - ulo[2] + 1;
+  const int nbt = nbx > 0 && nby > 0 && nbz > 0 ? nbx * nby * nbz : 0;
+  for (int b0 = 0; b0 < nbt; b0 += 64) {
+    const int t = b0 + lane;
+    int sbi = 0;
     unsigned sk = 0;
-    if (sbi < ig.nsbt) {
+    if (t < nbt) {
+      sbi = ((ulo[0] + t / (nby * nbz)) * ig.nsb[1] + ulo[1] + (t / nbz) % nby) * ig.nsb[2] + ulo[2] + t % nbz;
       const double *sb = sbc + 4 * ((size_t)cj * ig.nsbt + sbi);
       const double cx = shifted(sb[0], s[0], b.prd[0]);
       const double cy = shifted(sb[1], s[1], b.prd[1]);
@@ -726,8 +773,9 @@ __device__ void scan_sites_group(const IpGeom &ig, const double *__restrict__ no
     while (sm) {
       const int src = __ffsll((long long)sm) - 1;
       sm &= sm - 1;
-      const int sq = b0 + src;
-      const unsigned gm = (unsigned)__shfl((int)sk, src);
+      // wave-uniform: decoded in scalar registers
+      const int sq = __builtin_amdgcn_readfirstlane(__shfl(sbi, src));
+      const unsigned gm = (unsigned)__builtin_amdgcn_readfirstlane(__shfl((int)sk, src));
       const int ba = sq / (ig.nsb[1] * ig.nsb[2]);
       const int bb = (sq / ig.nsb[2]) % ig.nsb[1];
       const int bc = sq % ig.nsb[2];
@@ -741,7 +789,20 @@ __device__ void scan_sites_group(const IpGeom &ig, const double *__restrict__ no
       double xs[3] = {0.0, 0.0, 0.0};
       if (valid) {
         site = (a * ig.nae[1] + bq) * ig.nae[2] + cq;
-        interp_site(spa_site + 8 * site, X, xs);
+        // shape functions of the site, with the operations (and so the bits)
+        // of the host's site table (cac_upload_elements)
+        const double px = ig.pstep[0] * (double)a - 1, py = ig.pstep[1] * (double)bq - 1,
+                     pz = ig.pstep[2] * (double)cq - 1;
+        double N[8];
+        N[0] = 0.125 * (1 - px) * (1 - py) * (1 - pz);
+        N[1] = 0.125 * (1 + px) * (1 - py) * (1 - pz);
+        N[2] = 0.125 * (1 + px) * (1 + py) * (1 - pz);
+        N[3] = 0.125 * (1 - px) * (1 + py) * (1 - pz);
+        N[4] = 0.125 * (1 - px) * (1 - py) * (1 + pz);
+        N[5] = 0.125 * (1 + px) * (1 - py) * (1 + pz);
+        N[6] = 0.125 * (1 + px) * (1 + py) * (1 + pz);
+        N[7] = 0.125 * (1 - px) * (1 + py) * (1 + pz);
+        interp_site(N, X, xs);
       }
       bool any = false;
 #pragma unroll
@@ -793,7 +854,7 @@ __global__ __launch_bounds__(256) void k_ip_neigh(IpGeom ig, const double *__res
   const int i = wv / ngrp, g0 = (wv % ngrp) * kIpGroup;
   const int ng = ig.nint - g0 < kIpGroup ? ig.nint - g0 : kIpGroup;
   // the group's points in LDS (lane g interpolates point g)
-  __shared__ double sip[4][kIpGroup][3];
+  __shared__ double sip[4][kIpGroup][3], snode[4][24];
   double (*ip)[3] = sip[threadIdx.x >> 6];
   if (lane < kIpGroup) {
     double Xi[24];
@@ -841,8 +902,8 @@ __global__ __launch_bounds__(256) void k_ip_neigh(IpGeom ig, const double *__res
       const int cj = __shfl(j, src);
       const int cimg = __shfl(img, src);
       const unsigned gm = (unsigned)__shfl((int)keep, src);
-      scan_sites_group(ig, nodex, spa_site, sbc, b, cj, cimg, cj == i && cimg == kImgSelf, ip, gm, cutneigh, cnt,
-                       maxnip, out, bitmap, lane);
+      scan_sites_group<false>(ig, nodex, spa_site, sbc, erad, hull, b, cj, cimg, cj == i && cimg == kImgSelf, ip, gm,
+                       cutneigh, cnt, maxnip, out, bitmap, snode[threadIdx.x >> 6], lane);
     }
   }
   bool touched = false;
@@ -947,7 +1008,7 @@ __global__ __launch_bounds__(256) void k_atom_sites(int niface, const int *__res
   const int r0 = ((xcd_block() * (int)blockDim.x + (int)threadIdx.x) >> 6) * kIpGroup;
   if (r0 >= niface) return;  // whole wavefront
   const int ng = niface - r0 < kIpGroup ? niface - r0 : kIpGroup;
-  __shared__ double sq[4][kIpGroup][3];
+  __shared__ double sq[4][kIpGroup][3], snode[4][24];
   double (*q)[3] = sq[threadIdx.x >> 6];
   int cell = 0;
   if (lane < ng) {
@@ -996,8 +1057,8 @@ __global__ __launch_bounds__(256) void k_atom_sites(int niface, const int *__res
         int before = 0, after = 0;
 #pragma unroll
         for (int g = 0; g < kIpGroup; ++g) before += cnt[g];
-        scan_sites_group(ig, nodex, spa_site, sbc, b, cj, cimg, false, q, gm, cutneigh, cnt, maxneigh, out, bitmap,
-                         lane);
+        scan_sites_group<true>(ig, nodex, spa_site, sbc, eb.erad, eb.hull, b, cj, cimg, false, q, gm, cutneigh, cnt,
+                         maxneigh, out, bitmap, snode[threadIdx.x >> 6], lane);
 #pragma unroll
         for (int g = 0; g < kIpGroup; ++g) after += cnt[g];
         if (lane == 0 && after > before) etouch[cj] = 1;
@@ -1262,6 +1323,7 @@ IpGeom ip_geom(const cac_ctx *ctx) {
   }
   ig.nsbt = ctx->nsb_total;
   ig.nwords = ctx->nwords;
+  for (int d = 0; d < 3; ++d) ig.pstep[d] = 2 / ((double)ctx->nae[d] - 1);
   return ig;
 }
 
@@ -1433,6 +1495,7 @@ int launch_ip_neighbors(cac_ctx *ctx) {
   }
   ig.nsbt = ctx->nsb_total;
   ig.nwords = ctx->nwords;
+  for (int d = 0; d < 3; ++d) ig.pstep[d] = 2 / ((double)ctx->nae[d] - 1);
   hipEvent_t t0;
   timer_begin(ctx, "ip_neigh", &t0);
   const long waves = (long)ne * ((ctx->nint + kIpGroup - 1) / kIpGroup);
