@@ -531,7 +531,7 @@ int cac_ctx_destroy(cac_ctx *ctx) {
   DevBuf *bufs[] = {&ctx->b_x4, &ctx->b_v, &ctx->b_f, &ctx->b_xhold, &ctx->b_rho, &ctx->b_eatom, &ctx->b_vatom,
                     &ctx->b_aneigh, &ctx->b_anum, &ctx->b_akey, &ctx->b_aval, &ctx->b_akey2, &ctx->b_aval2,
                     &ctx->b_ahead, &ctx->b_acells, &ctx->b_acs, &ctx->b_ace, &ctx->b_sorttmp, &ctx->b_nodex, &ctx->b_nodev, &ctx->b_nodef,
-                    &ctx->b_nodexhold, &ctx->b_xc, &ctx->b_spa_site, &ctx->b_spa_int, &ctx->b_weight, &ctx->b_sbc,
+                    &ctx->b_nodexhold, &ctx->b_xc, &ctx->b_spa_site, &ctx->b_spa_int, &ctx->b_weight,
                     &ctx->b_erad, &ctx->b_ehull, &ctx->b_eneigh, &ctx->b_enumb, &ctx->b_ekey, &ctx->b_eval, &ctx->b_ekey2,
                     &ctx->b_eval2, &ctx->b_ecs, &ctx->b_ece, &ctx->b_ipneigh, &ctx->b_ipnum, &ctx->b_erho,
                     &ctx->b_iptmp, &ctx->b_bitmap, &ctx->b_wprefix, &ctx->b_ecount, &ctx->b_estart, &ctx->b_asite,
@@ -610,14 +610,9 @@ int cac_upload_elements(cac_ctx *ctx, int ne, const int nae[3], const double *no
   if (!ctx || ne < 0 || !nae || (ne > 0 && !nodex) || !(mass_e > 0)) return CAC_ERR_ARG;
   if (ne >= (1 << 27)) return fail(ctx, CAC_ERR_ARG, "at most 2^27-1 elements");
   for (int d = 0; d < 3; ++d)
-    if (nae[d] < 2 || nae[d] > kSubBlock * 8 * 4)
+    if (nae[d] < 2 || nae[d] > 128)
       return fail(ctx, CAC_ERR_ARG, "nae must be in [2, 128] per edge");  // element.cpp:1104-1107
   if ((long)nae[0] * nae[1] * nae[2] >= 65536) return fail(ctx, CAC_ERR_ARG, "at most 65535 sites per element");
-  {
-    int nsb = 1;
-    for (int d = 0; d < 3; ++d) nsb *= (nae[d] + kSubBlock - 1) / kSubBlock;
-    if (nsb > kMaxSubBlocks) return fail(ctx, CAC_ERR_ARG, "element too large (sub-block limit)");
-  }
   hipSetDevice(ctx->device);
   const bool new_type = (nae[0] != ctx->nae[0] || nae[1] != ctx->nae[1] || nae[2] != ctx->nae[2]);
   ctx->ne = ne;

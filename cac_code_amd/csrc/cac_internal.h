@@ -19,10 +19,6 @@ struct DevBuf {
   size_t bytes = 0;
 };
 
-// Sub-block edge (sites) used to prune the integration-point site search: a
-// 4x4x4 block holds 64 sites, one per lane of a wavefront.
-constexpr int kSubBlock = 4;
-constexpr int kMaxSubBlocks = 512;   // nae <= 32 per edge
 constexpr int kMaxNint = 64;         // integration points per element
 constexpr int kImgSelf = 13;
 // site field of a raw list entry that names an atom (CAC_ATOM_SITE)
@@ -173,10 +169,9 @@ struct cac_ctx {
   std::vector<double> h_spa_int, h_weight;
   int node_list[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   bool quad_set = false;
-  // sub-blocks: per element per sub-block (cx, cy, cz, radius); element (cx, cy, cz, R)
-  int nsb[3] = {0, 0, 0}, nsb_total = 0;
-  double *sbc = nullptr, *erad = nullptr, *ehull = nullptr;  // ehull: [ne][16] parallelepiped hull
-  cac::DevBuf b_sbc, b_erad, b_ehull;
+  // element geometry at build
+  double *erad = nullptr, *ehull = nullptr;  // erad: [ne] center, radius; ehull: [ne][16] parallelepiped hull
+  cac::DevBuf b_erad, b_ehull;
   // element neighbor list, column-major [maxelist][ne]
   int maxelist = 192;
   uint32_t *eneigh = nullptr;

@@ -8,7 +8,7 @@
 // element.cpp:933-953); site positions are sum_k N_k*X'_k in node order
 // (full_node_neighbor.cpp:141-159); rsq is summed x, y, z left to right and
 // compared with <= (neighbor.cpp:609, full_node_neighbor.cpp:167,216).
-// Search pruning (cells, element radii, sub-block radii) is conservative by
+// Search pruning (cells, element radii, element hulls) is conservative by
 // construction, so the sets equal the brute-force sets.
 #include <hipcub/hipcub.hpp>
 
@@ -334,10 +334,7 @@ __global__ __launch_bounds__(64 * kNeighWaves) void k_atom_neigh(
 
 // ---------------------------------------------------------------- elements
 // Geometry at rebuild: element radius R = max_k |X_k - c| (every site is a
-// convex combination of the nodes, so it lies within R of the center) and,
-// per 4x4x4 sub-block, the center and radius of its 8 corner sites (the
-// trilinear map restricted to the sub-block is trilinear in the block's own
-// coordinates, so its sites are convex combinations of those corners).
+// convex combination of the nodes, so it lies within R of the center).
 //
 // Also a parallelepiped hull: with corner signs s_k (node k at parent
 // corner (s_k + 1)/2), J = (1/8) sum_k (X_k - c) s_k^T and residuals
@@ -427,17 +424,8 @@ __global__ void k_elem_radius(int ne, const double *__restrict__ nodex, const do
   for (int d = 12; d < 16; ++d) h[d] = 0.0;
 }
 
-struct SubBlockGeom {
-  int nae[3];
-  int nsb[3];
-};
-
-__device__ __forceinline__ void sb_range(int b, int nae, int *lo, int *hi) {
-  *lo = b * kSubBlock;
-  int h = *lo + kSubBlock - 1;
-  *hi = h < nae - 1 ? h : nae - 1;
-}
-
+// Position sum_k N_k X_k of a site with shape functions N, summed in node
+// order (full_node_neighbor.cpp:141-159).
 __device__ __forceinline__ void interp_site(const double *__restrict__ N, const double X[24], double out[3]) {
   double a = 0.0, b = 0.0, c = 0.0;
   for (int k = 0; k < 8; ++k) {
@@ -448,45 +436,6 @@ __device__ __forceinline__ void interp_site(const double *__restrict__ N, const 
   out[0] = a;
   out[1] = b;
   out[2] = c;
-}
-
-__global__ void k_subblocks(int ne, SubBlockGeom sg, const double *__restrict__ nodex,
-                            const double *__restrict__ spa_site, double *__restrict__ sbc) {
-  const int nsbt = sg.nsb[0] * sg.nsb[1] * sg.nsb[2];
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)ne * nsbt) return;
-  const int i = (int)(t / nsbt), bidx = (int)(t % nsbt);
-  const int ba = bidx / (sg.nsb[1] * sg.nsb[2]);
-  const int bb = (bidx / sg.nsb[2]) % sg.nsb[1];
-  const int bc = bidx % sg.nsb[2];
-  int lo[3], hi[3];
-  sb_range(ba, sg.nae[0], &lo[0], &hi[0]);
-  sb_range(bb, sg.nae[1], &lo[1], &hi[1]);
-  sb_range(bc, sg.nae[2], &lo[2], &hi[2]);
-  double X[24];
-  for (int q = 0; q < 24; ++q) X[q] = nodex[24 * (size_t)i + q];
-  double P[8][3];
-  double c[3] = {0.0, 0.0, 0.0};
-  for (int k = 0; k < 8; ++k) {
-    int a = (k & 1) ? hi[0] : lo[0];
-    int bq = (k & 2) ? hi[1] : lo[1];
-    int cq = (k & 4) ? hi[2] : lo[2];
-    int site = (a * sg.nae[1] + bq) * sg.nae[2] + cq;
-    interp_site(spa_site + 8 * site, X, P[k]);
-    for (int d = 0; d < 3; ++d) c[d] += P[k][d];
-  }
-  for (int d = 0; d < 3; ++d) c[d] = c[d] / 8;
-  double r2 = 0.0;
-  for (int k = 0; k < 8; ++k) {
-    double dx = P[k][0] - c[0], dy = P[k][1] - c[1], dz = P[k][2] - c[2];
-    double q = dx * dx + dy * dy + dz * dz;
-    r2 = q > r2 ? q : r2;
-  }
-  double* o = sbc + 4 * (size_t)t;
-  o[0] = c[0];
-  o[1] = c[1];
-  o[2] = c[2];
-  o[3] = sqrt(r2) * (1.0 + 1e-12) + 1e-9;
 }
 
 // Candidate element images of element i: every (j, image) != (i, self)
@@ -562,20 +511,9 @@ __global__ void k_elem_neigh_brute(int ne, const double *__restrict__ erad, Box3
   }
 }
 
-// Integration-point site lists (Neighbor::node_neighbor,
-// full_node_neighbor.cpp:19-239): for integration point g of element i, every
-// site of every candidate element image with rsq <= cutneighsq, then the
-// sites of i itself with rsq > 1e-4 (full_node_neighbor.cpp:216), which drops
-// the integration point's own site.
-//
-// One wavefront per integration point.  Lanes test 64 candidate elements at
-// a time against the element radius; for each survivor (wave-uniform loop)
-// lanes test its sub-blocks, and for each surviving 4x4x4 sub-block every
-// lane takes one site.  Accepted sites are compacted in lane order with a
-// ballot, so the list order is deterministic.  Each accepted (element, site)
-// is also marked in the element's referenced-site bitmap.
+// Element geometry for the site searches.
 struct IpGeom {
-  int ne, nint, nae[3], nsb[3], nsbt, nwords;
+  int ne, nint, nae[3], nwords;
   double pstep[3];  // parent-coordinate step 2/(nae-1), as the host computes it
 };
 
@@ -684,107 +622,93 @@ __device__ int scan_atoms(const AtomBins &ab, const Box3 &b, const double p[3], 
 
 // Sites of element image (cj, cimg) within cutneigh of each point q[g]
 // (g < kIpGroup) whose bit is set in gmask -- a group of integration points of
-// one element, or of nearby atoms.  The element's sub-blocks are tested
-// lane-parallel against their radii for every point; the (up to) 64 sites of
-// each sub-block that some point reaches are then taken one per lane,
-// interpolated once and tested against each such point.  Point g's accepted
+// one element, or of nearby atoms.  Candidate sites come from the hull: a site
+// with parent coordinates xi within cutneigh of q has |xi_d - u_d| <=
+// rho |row_d(J^-1)| < bound_d - 1, u = J^-1 (q - c') (hull_hit), and site
+// index k along d has xi_d = k * 2/(nae_d - 1) - 1, so each point reaches a
+// box of site indices.  Boxes are walked densely, 64 sites at a time in index
+// order: one box per point, or, when the points are close together (their
+// union box is no larger than the boxes' total), the union once, each site
+// interpolated once and tested against every point.  Point g's accepted
 // sites are compacted in lane order into out + g*cap from cnt[g] on (raw
 // entries cj | cimg << 27 | site << 32, capacity cap); a site accepted by any
 // point is marked in cj's referenced-site bitmap.  `self` applies the
-// rsq > 1e-4 exclusion of the own element (full_node_neighbor.cpp:216).
-// kBox: enumerate only the sub-blocks inside the points' parent-coordinate
-// reach (below), else all of them.  Wave-uniform.
+// rsq > 1e-4 exclusion of the own element (full_node_neighbor.cpp:216).  X
+// (24 doubles) and box (6 ints per point) are the wavefront's LDS slots.
+// Wave-uniform.
 constexpr int kIpGroup = 8;
 
-template <bool kBox>
 __device__ void scan_sites_group(const IpGeom &ig, const double *__restrict__ nodex,
-                                 const double *__restrict__ spa_site, const double *__restrict__ sbc,
                                  const double *__restrict__ erad, const double *__restrict__ hull, const Box3 &b,
                                  int cj, int cimg, bool self, const double (*q)[3], unsigned gmask,
                                  double cutneigh, int (&cnt)[kIpGroup], int cap, uint64_t *__restrict__ out,
-                                 uint32_t *__restrict__ bitmap, double *X, int lane) {
+                                 uint32_t *__restrict__ bitmap, double *X, int *box, int lane) {
   const double cutsq = cutneigh * cutneigh;
   int s[3];
   img_shift(cimg, s);
-  // the image's node coordinates, in the wavefront's LDS slot X[24]
-  wave_sync();  // earlier readers of X are done
+  wave_sync();  // earlier readers of X and box are done
   if (lane < 24) X[lane] = shifted(nodex[24 * (size_t)cj + lane], s[lane % 3], b.prd[lane % 3]);
-  wave_sync();
-  const uint32_t lo32 = (uint32_t)cj | ((uint32_t)cimg << 27);
-  // Sub-blocks within parent-coordinate reach of some point: a site with
-  // parent coordinates xi within cutneigh of q has |xi_d - u_d| <=
-  // rho |row_d(J^-1)| < hull bound_d - 1, u = J^-1 (q - c') (hull_hit), and
-  // site index k along d has xi_d = k * 2/(nae_d - 1) - 1.  The sub-blocks of
-  // the union of the points' index boxes are enumerated in index order.
-  int ulo[3] = {ig.nsb[0], ig.nsb[1], ig.nsb[2]}, uhi[3] = {-1, -1, -1};
-  if (!kBox) {
-    for (int d = 0; d < 3; ++d) {
-      ulo[d] = 0;
-      uhi[d] = ig.nsb[d] - 1;
-    }
-  } else {
-    const double *h = hull + 16 * (size_t)cj;
-    const double c0 = shifted(erad[4 * cj], s[0], b.prd[0]);
-    const double c1 = shifted(erad[4 * cj + 1], s[1], b.prd[1]);
-    const double c2 = shifted(erad[4 * cj + 2], s[2], b.prd[2]);
-#pragma unroll
-    for (int g = 0; g < kIpGroup; ++g)
-      if ((gmask >> g) & 1u) {
-        const double dx = q[g][0] - c0, dy = q[g][1] - c1, dz = q[g][2] - c2;
-        int lo[3], hi[3];
-        bool reach = true;
-        for (int d = 0; d < 3; ++d) {
-          const double u = h[3 * d] * dx + h[3 * d + 1] * dy + h[3 * d + 2] * dz;
-          const double w = h[9 + d] - 1.0, half = 0.5 * (double)(ig.nae[d] - 1);
-          const double flo = fmax((u - w + 1.0) * half - 1e-6, 0.0);
-          const double fhi = fmin((u + w + 1.0) * half + 1e-6, (double)(ig.nae[d] - 1));
-          lo[d] = (int)ceil(flo);
-          hi[d] = (int)floor(fhi);
-          reach = reach && lo[d] <= hi[d];
-        }
-        if (reach)
-          for (int d = 0; d < 3; ++d) {
-            ulo[d] = min(ulo[d], lo[d] / kSubBlock);
-            uhi[d] = max(uhi[d], hi[d] / kSubBlock);
-          }
+  if (lane < kIpGroup) {  // lane g: the index box of point g (lo > hi: none)
+    int lo[3] = {1, 1, 1}, hi[3] = {0, 0, 0};
+    if ((gmask >> lane) & 1u) {
+      const double *h = hull + 16 * (size_t)cj;
+      const double dx = q[lane][0] - shifted(erad[4 * cj], s[0], b.prd[0]);
+      const double dy = q[lane][1] - shifted(erad[4 * cj + 1], s[1], b.prd[1]);
+      const double dz = q[lane][2] - shifted(erad[4 * cj + 2], s[2], b.prd[2]);
+      bool reach = true;
+      int l[3], u[3];
+      for (int d = 0; d < 3; ++d) {
+        const double c = h[3 * d] * dx + h[3 * d + 1] * dy + h[3 * d + 2] * dz;
+        const double w = h[9 + d] - 1.0, half = 0.5 * (double)(ig.nae[d] - 1);
+        const double flo = fmax((c - w + 1.0) * half - 1e-6, 0.0);
+        const double fhi = fmin((c + w + 1.0) * half + 1e-6, (double)(ig.nae[d] - 1));
+        l[d] = (int)ceil(flo);
+        u[d] = (int)floor(fhi);
+        reach = reach && l[d] <= u[d];
       }
-  }
-  const int nbx = uhi[0] - ulo[0] + 1, nby = uhi[1] - ulo[1] + 1, nbz = uhi[2] - ulo[2] + 1;
-  const int nbt = nbx > 0 && nby > 0 && nbz > 0 ? nbx * nby * nbz : 0;
-  for (int b0 = 0; b0 < nbt; b0 += 64) {
-    const int t = b0 + lane;
-    int sbi = 0;
-    unsigned sk = 0;
-    if (t < nbt) {
-      sbi = ((ulo[0] + t / (nby * nbz)) * ig.nsb[1] + ulo[1] + (t / nbz) % nby) * ig.nsb[2] + ulo[2] + t % nbz;
-      const double *sb = sbc + 4 * ((size_t)cj * ig.nsbt + sbi);
-      const double cx = shifted(sb[0], s[0], b.prd[0]);
-      const double cy = shifted(sb[1], s[1], b.prd[1]);
-      const double cz = shifted(sb[2], s[2], b.prd[2]);
-      const double reach = cutneigh + sb[3];
-#pragma unroll
-      for (int g = 0; g < kIpGroup; ++g)
-        if ((gmask >> g) & 1u) {
-          const double dx = q[g][0] - cx, dy = q[g][1] - cy, dz = q[g][2] - cz;
-          if (dx * dx + dy * dy + dz * dz <= reach * reach) sk |= 1u << g;
+      if (reach)
+        for (int d = 0; d < 3; ++d) {
+          lo[d] = l[d];
+          hi[d] = u[d];
         }
     }
-    unsigned long long sm = __ballot(sk != 0);
-    while (sm) {
-      const int src = __ffsll((long long)sm) - 1;
-      sm &= sm - 1;
-      // wave-uniform: decoded in scalar registers
-      const int sq = __builtin_amdgcn_readfirstlane(__shfl(sbi, src));
-      const unsigned gm = (unsigned)__builtin_amdgcn_readfirstlane(__shfl((int)sk, src));
-      const int ba = sq / (ig.nsb[1] * ig.nsb[2]);
-      const int bb = (sq / ig.nsb[2]) % ig.nsb[1];
-      const int bc = sq % ig.nsb[2];
-      int lo[3], hi[3];
-      sb_range(ba, ig.nae[0], &lo[0], &hi[0]);
-      sb_range(bb, ig.nae[1], &lo[1], &hi[1]);
-      sb_range(bc, ig.nae[2], &lo[2], &hi[2]);
-      const int a = lo[0] + (lane >> 4), bq = lo[1] + ((lane >> 2) & 3), cq = lo[2] + (lane & 3);
-      const bool valid = a <= hi[0] && bq <= hi[1] && cq <= hi[2];
+    for (int d = 0; d < 3; ++d) {
+      box[6 * lane + d] = lo[d];
+      box[6 * lane + 3 + d] = hi[d];
+    }
+  }
+  wave_sync();
+  unsigned live = 0;
+  int ulo[3] = {1 << 30, 1 << 30, 1 << 30}, uhi[3] = {-1, -1, -1};
+  long vsum = 0;
+  for (int g = 0; g < kIpGroup; ++g) {
+    const int *bg = box + 6 * g;
+    if (!((gmask >> g) & 1u) || bg[0] > bg[3]) continue;
+    live |= 1u << g;
+    vsum += (long)(bg[3] - bg[0] + 1) * (bg[4] - bg[1] + 1) * (bg[5] - bg[2] + 1);
+    for (int d = 0; d < 3; ++d) {
+      ulo[d] = min(ulo[d], bg[d]);
+      uhi[d] = max(uhi[d], bg[3 + d]);
+    }
+  }
+  if (!live) return;
+  const uint32_t lo32 = (uint32_t)cj | ((uint32_t)cimg << 27);
+  // sites of box [lo, hi] (index order) against the points in gm
+  auto walk = [&](int lo0, int lo1, int lo2, int hi0, int hi1, int hi2, unsigned gm) {
+    const int nz = hi2 - lo2 + 1, nyz = (hi1 - lo1 + 1) * nz, tot = (hi0 - lo0 + 1) * nyz;
+    const float rnyz = 1.0f / (float)nyz, rnz = 1.0f / (float)nz;
+    for (int t0 = 0; t0 < tot; t0 += 64) {
+      const int t = t0 + lane;
+      // t -> (ia, ib, ic), ic fastest: float quotient, then exact correction
+      int ia = (int)((float)t * rnyz);
+      ia -= ia * nyz > t ? 1 : 0;
+      ia += (ia + 1) * nyz <= t ? 1 : 0;
+      const int rem = t - ia * nyz;
+      int ib = (int)((float)rem * rnz);
+      ib -= ib * nz > rem ? 1 : 0;
+      ib += (ib + 1) * nz <= rem ? 1 : 0;
+      const int a = lo0 + ia, bq = lo1 + ib, cq = lo2 + rem - ib * nz;
+      const bool valid = t < tot;
       int site = 0;
       double xs[3] = {0.0, 0.0, 0.0};
       if (valid) {
@@ -821,6 +745,18 @@ __device__ void scan_sites_group(const IpGeom &ig, const double *__restrict__ no
         }
       if (any) atomicOr(&bitmap[(size_t)cj * ig.nwords + (site >> 5)], 1u << (site & 31));
     }
+  };
+  const long vu = (long)(uhi[0] - ulo[0] + 1) * (uhi[1] - ulo[1] + 1) * (uhi[2] - ulo[2] + 1);
+  if (vu <= vsum) {
+    walk(ulo[0], ulo[1], ulo[2], uhi[0], uhi[1], uhi[2], live);
+  } else {
+    for (int g = 0; g < kIpGroup; ++g)
+      if ((live >> g) & 1u) {
+        const int *bg = box + 6 * g;
+        walk(__builtin_amdgcn_readfirstlane(bg[0]), __builtin_amdgcn_readfirstlane(bg[1]),
+             __builtin_amdgcn_readfirstlane(bg[2]), __builtin_amdgcn_readfirstlane(bg[3]),
+             __builtin_amdgcn_readfirstlane(bg[4]), __builtin_amdgcn_readfirstlane(bg[5]), 1u << g);
+      }
   }
 }
 
@@ -836,12 +772,10 @@ __device__ void scan_sites_group(const IpGeom &ig, const double *__restrict__ no
 // candidate elements at a time against the element radius, for every point
 // of the group; each survivor is scanned once for the points that reach it
 // (scan_sites_group).  Per point, the list order is candidate order, then
-// sub-block order, then site order: deterministic.
-__global__ __launch_bounds__(256) void k_ip_neigh(IpGeom ig, const double *__restrict__ nodex,
+// site index order: deterministic.
+__global__ __launch_bounds__(256, 4) void k_ip_neigh(IpGeom ig, const double *__restrict__ nodex,
                                                   const double *__restrict__ spa_int,
-                                                  const double *__restrict__ spa_site,
                                                   const double *__restrict__ erad, const double *__restrict__ hull,
-                                                  const double *__restrict__ sbc,
                                                   const uint32_t *__restrict__ eneigh, const int *__restrict__ enumb,
                                                   Box3 b, double cutneigh, int maxnip, uint64_t *__restrict__ tmp,
                                                   int *__restrict__ ipnum, uint32_t *__restrict__ bitmap,
@@ -855,6 +789,7 @@ __global__ __launch_bounds__(256) void k_ip_neigh(IpGeom ig, const double *__res
   const int ng = ig.nint - g0 < kIpGroup ? ig.nint - g0 : kIpGroup;
   // the group's points in LDS (lane g interpolates point g)
   __shared__ double sip[4][kIpGroup][3], snode[4][24];
+  __shared__ int sbox[4][6 * kIpGroup];
   double (*ip)[3] = sip[threadIdx.x >> 6];
   if (lane < kIpGroup) {
     double Xi[24];
@@ -902,8 +837,8 @@ __global__ __launch_bounds__(256) void k_ip_neigh(IpGeom ig, const double *__res
       const int cj = __shfl(j, src);
       const int cimg = __shfl(img, src);
       const unsigned gm = (unsigned)__shfl((int)keep, src);
-      scan_sites_group<false>(ig, nodex, spa_site, sbc, erad, hull, b, cj, cimg, cj == i && cimg == kImgSelf, ip, gm,
-                       cutneigh, cnt, maxnip, out, bitmap, snode[threadIdx.x >> 6], lane);
+      scan_sites_group(ig, nodex, erad, hull, b, cj, cimg, cj == i && cimg == kImgSelf, ip, gm, cutneigh, cnt, maxnip,
+                       out, bitmap, snode[threadIdx.x >> 6], sbox[threadIdx.x >> 6], lane);
     }
   }
   bool touched = false;
@@ -995,12 +930,11 @@ __global__ void k_iface_compact(int n, const int *__restrict__ iflag, const int 
 // of every element for a box under 3 element cells wide) is tested against
 // each of those atoms (radius, then hull), and each survivor is scanned once
 // for the atoms that reach it (scan_sites_group).  Per atom the entries come
-// in candidate order, then sub-block order, then site order.
+// in candidate order, then site index order.
 __global__ __launch_bounds__(256) void k_atom_sites(int niface, const int *__restrict__ iface,
                                                     const double *__restrict__ x4, const int *__restrict__ anum,
                                                     IpGeom ig, const double *__restrict__ nodex,
-                                                    const double *__restrict__ spa_site,
-                                                    const double *__restrict__ sbc, ElemBins eb, Box3 b,
+                                                    ElemBins eb, Box3 b,
                                                     double cutneigh, int maxneigh, uint64_t *__restrict__ atmp,
                                                     int *__restrict__ asnum, uint32_t *__restrict__ bitmap,
                                                     int *__restrict__ etouch, int *__restrict__ flags) {
@@ -1009,6 +943,7 @@ __global__ __launch_bounds__(256) void k_atom_sites(int niface, const int *__res
   if (r0 >= niface) return;  // whole wavefront
   const int ng = niface - r0 < kIpGroup ? niface - r0 : kIpGroup;
   __shared__ double sq[4][kIpGroup][3], snode[4][24];
+  __shared__ int sbox[4][6 * kIpGroup];
   double (*q)[3] = sq[threadIdx.x >> 6];
   int cell = 0;
   if (lane < ng) {
@@ -1057,8 +992,8 @@ __global__ __launch_bounds__(256) void k_atom_sites(int niface, const int *__res
         int before = 0, after = 0;
 #pragma unroll
         for (int g = 0; g < kIpGroup; ++g) before += cnt[g];
-        scan_sites_group<true>(ig, nodex, spa_site, sbc, eb.erad, eb.hull, b, cj, cimg, false, q, gm, cutneigh, cnt,
-                         maxneigh, out, bitmap, snode[threadIdx.x >> 6], lane);
+        scan_sites_group(ig, nodex, eb.erad, eb.hull, b, cj, cimg, false, q, gm, cutneigh, cnt, maxneigh, out,
+                         bitmap, snode[threadIdx.x >> 6], sbox[threadIdx.x >> 6], lane);
 #pragma unroll
         for (int g = 0; g < kIpGroup; ++g) after += cnt[g];
         if (lane == 0 && after > before) etouch[cj] = 1;
@@ -1319,11 +1254,9 @@ IpGeom ip_geom(const cac_ctx *ctx) {
   ig.nint = ctx->nint;
   for (int d = 0; d < 3; ++d) {
     ig.nae[d] = ctx->nae[d];
-    ig.nsb[d] = ctx->nsb[d];
+    ig.pstep[d] = 2 / ((double)ctx->nae[d] - 1);
   }
-  ig.nsbt = ctx->nsb_total;
   ig.nwords = ctx->nwords;
-  for (int d = 0; d < 3; ++d) ig.pstep[d] = 2 / ((double)ctx->nae[d] - 1);
   return ig;
 }
 
@@ -1415,17 +1348,6 @@ int launch_element_neighbors(cac_ctx *ctx) {
   CAC_HIP(ctx, hipMemsetAsync(rmax_bits, 0, sizeof(unsigned long long), ctx->stream));
   hipLaunchKernelGGL(k_elem_radius, dim3(grid_for(ne)), dim3(kBlock), 0, ctx->stream, ne, ctx->nodex, ctx->xc,
                      ctx->cutneigh, ctx->erad, ctx->ehull, rmax_bits);
-  // sub-block geometry
-  for (int d = 0; d < 3; ++d) ctx->nsb[d] = (ctx->nae[d] + kSubBlock - 1) / kSubBlock;
-  ctx->nsb_total = ctx->nsb[0] * ctx->nsb[1] * ctx->nsb[2];
-  if ((rc = ensure(ctx, ctx->b_sbc, 4 * (size_t)ne * ctx->nsb_total, &ctx->sbc))) return rc;
-  SubBlockGeom sg;
-  for (int d = 0; d < 3; ++d) {
-    sg.nae[d] = ctx->nae[d];
-    sg.nsb[d] = ctx->nsb[d];
-  }
-  hipLaunchKernelGGL(k_subblocks, dim3(grid_for((long)ne * ctx->nsb_total)), dim3(kBlock), 0, ctx->stream, ne, sg,
-                     ctx->nodex, ctx->spa_site, ctx->sbc);
   unsigned long long hbits = 0;
   CAC_HIP(ctx, hipMemcpyAsync(&hbits, rmax_bits, sizeof(hbits), hipMemcpyDeviceToHost, ctx->stream));
   CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -1486,21 +1408,12 @@ int launch_ip_neighbors(cac_ctx *ctx) {
                      reinterpret_cast<int *>(ctx->bitmap), 0);
   hipLaunchKernelGGL(k_fill_int, dim3(grid_for(ne)), dim3(kBlock), 0, ctx->stream, ne, ctx->etouch, 0);
   const bool coupled = ctx->coupled && ctx->n > 0;
-  IpGeom ig;
-  ig.ne = ne;
-  ig.nint = ctx->nint;
-  for (int d = 0; d < 3; ++d) {
-    ig.nae[d] = ctx->nae[d];
-    ig.nsb[d] = ctx->nsb[d];
-  }
-  ig.nsbt = ctx->nsb_total;
-  ig.nwords = ctx->nwords;
-  for (int d = 0; d < 3; ++d) ig.pstep[d] = 2 / ((double)ctx->nae[d] - 1);
+  const IpGeom ig = ip_geom(ctx);
   hipEvent_t t0;
   timer_begin(ctx, "ip_neigh", &t0);
   const long waves = (long)ne * ((ctx->nint + kIpGroup - 1) / kIpGroup);
   hipLaunchKernelGGL(k_ip_neigh, dim3(xcd_grid(64L * waves, kBlock)), dim3(kBlock), 0, ctx->stream, ig, ctx->nodex,
-                     ctx->spa_int, ctx->spa_site, ctx->erad, ctx->ehull, ctx->sbc, ctx->eneigh, ctx->enumb, box_of(ctx),
+                     ctx->spa_int, ctx->erad, ctx->ehull, ctx->eneigh, ctx->enumb, box_of(ctx),
                      ctx->cutneigh, ctx->maxnip, ctx->iptmp, ctx->ipnum, ctx->bitmap, ctx->dflags, coupled,
                      atom_bins(ctx), ctx->etouch);
   timer_end(ctx, "ip_neigh", t0);
@@ -1548,7 +1461,7 @@ int launch_atom_sites(cac_ctx *ctx) {
                      ctx->iface);
   const long groups = (niface + kIpGroup - 1) / kIpGroup;
   hipLaunchKernelGGL(k_atom_sites, dim3(xcd_grid(64L * groups, kBlock)), dim3(kBlock), 0, ctx->stream, niface,
-                     ctx->iface, ctx->x4, ctx->anum, ip_geom(ctx), ctx->nodex, ctx->spa_site, ctx->sbc,
+                     ctx->iface, ctx->x4, ctx->anum, ip_geom(ctx), ctx->nodex,
                      elem_bins(ctx), box_of(ctx), ctx->cutneigh, ctx->maxneigh, ctx->atmp, ctx->asnum, ctx->bitmap,
                      ctx->etouch, ctx->dflags);
   timer_end(ctx, "atom_sites", t0);
