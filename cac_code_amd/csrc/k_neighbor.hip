@@ -1041,29 +1041,43 @@ __global__ __launch_bounds__(256) void k_atom_sites(int niface, const int *__res
 }
 
 // Active-site numbering: per element, the referenced sites in site order;
-// elements in index order.  wprefix[j][w] = first id of word w.
+// elements in index order.  wprefix[j][w] = first id of word w.  One
+// wavefront per element, a lane per bitmap word, 64 words at a time with a
+// wavefront prefix sum of the word counts.
+__device__ __forceinline__ int wave_incl_sum(int v, int lane) {
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(v, o);
+    if (lane >= o) v += u;
+  }
+  return v;
+}
+
 __global__ void k_active_count(int ne, int nwords, const uint32_t *__restrict__ bitmap, int *__restrict__ wprefix,
                                int *__restrict__ ecount) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= ne) return;
+  const int j = (int)((blockIdx.x * (long)blockDim.x + threadIdx.x) >> 6), lane = threadIdx.x & 63;
+  if (j >= ne) return;  // whole wavefront
   int run = 0;
-  for (int w = 0; w < nwords; ++w) {
-    wprefix[(size_t)j * nwords + w] = run;
-    run += __popc(bitmap[(size_t)j * nwords + w]);
+  for (int w0 = 0; w0 < nwords; w0 += 64) {
+    const int w = w0 + lane;
+    const int c = w < nwords ? __popc(bitmap[(size_t)j * nwords + w]) : 0;
+    const int incl = wave_incl_sum(c, lane);
+    if (w < nwords) wprefix[(size_t)j * nwords + w] = run + incl - c;
+    run += __shfl(incl, 63);
   }
-  ecount[j] = run;
+  if (lane == 0) ecount[j] = run;
 }
 
 __global__ void k_active_finalize(int ne, int nwords, const uint32_t *__restrict__ bitmap,
                                   const int *__restrict__ estart, int *__restrict__ wprefix,
                                   int2 *__restrict__ asite) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= ne) return;
+  const int j = (int)((blockIdx.x * (long)blockDim.x + threadIdx.x) >> 6), lane = threadIdx.x & 63;
+  if (j >= ne) return;  // whole wavefront
   const int base = estart[j];
-  int id = base;
-  for (int w = 0; w < nwords; ++w) {
-    wprefix[(size_t)j * nwords + w] += base;
-    uint32_t bits = bitmap[(size_t)j * nwords + w];
+  for (int w = lane; w < nwords; w += 64) {
+    const size_t q = (size_t)j * nwords + w;
+    int id = wprefix[q] + base;
+    wprefix[q] = id;
+    uint32_t bits = bitmap[q];
     while (bits) {
       const int bit = __ffs(bits) - 1;
       bits &= bits - 1;
@@ -1086,15 +1100,16 @@ __device__ __forceinline__ uint32_t particle_entry(uint64_t e, int n, int nwords
 }
 
 // Integration-point lists: raw entries -> particle entries in the blocked
-// layout.
+// layout.  One thread per row.
 __global__ void k_ip_translate(int nip, int maxnip, const uint64_t *__restrict__ tmp, const int *__restrict__ ipnum,
                                int n, int nwords, const uint32_t *__restrict__ bitmap,
                                const int *__restrict__ wprefix, uint32_t *__restrict__ out) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)nip * maxnip) return;
-  const int p = (int)(t / maxnip), k = (int)(t % maxnip);
-  if (k >= ipnum[p]) return;
-  out[((size_t)(p >> 3) * maxnip + k) * kRowGroup + (p & 7)] = particle_entry(tmp[t], n, nwords, bitmap, wprefix);
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nip) return;
+  const int c = ipnum[p] < maxnip ? ipnum[p] : maxnip;
+  const uint64_t *row = tmp + (size_t)p * maxnip;
+  for (int k = 0; k < c; ++k)
+    out[((size_t)(p >> 3) * maxnip + k) * kRowGroup + (p & 7)] = particle_entry(row[k], n, nwords, bitmap, wprefix);
 }
 
 // Coupled atom lists: the raw site entries of flagged atom r go after the
@@ -1104,13 +1119,12 @@ __global__ void k_atom_site_translate(int niface, int maxneigh, const int *__res
                                       const int *__restrict__ anum, int n, int nwords,
                                       const uint32_t *__restrict__ bitmap, const int *__restrict__ wprefix,
                                       uint32_t *__restrict__ aneigh) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)niface * maxneigh) return;
-  const int r = (int)(t / maxneigh), k = (int)(t % maxneigh);
-  if (k >= asnum[r]) return;
-  const int i = iface[r];
-  const int pos = anum[i] + k;
-  aneigh[atom_slot(i, pos, maxneigh)] = particle_entry(atmp[t], n, nwords, bitmap, wprefix);
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= niface) return;
+  const int i = iface[r], a = anum[i];
+  const uint64_t *row = atmp + (size_t)r * maxneigh;
+  for (int k = 0; k < asnum[r] && a + k < maxneigh; ++k)
+    aneigh[atom_slot(i, a + k, maxneigh)] = particle_entry(row[k], n, nwords, bitmap, wprefix);
 }
 
 __global__ void k_atom_site_count(int niface, const int *__restrict__ iface, const int *__restrict__ asnum,
@@ -1121,28 +1135,32 @@ __global__ void k_atom_site_count(int niface, const int *__restrict__ iface, con
 
 // Every entry of a blocked list must name a row of x4 (< nrow) with an image
 // code < 27: the pair passes gather with it.  The first bad entry found is
-// reported as (row, slot, entry) in flags[8..10] with flags[7] set.
-__global__ void k_validate_list(int rows, int M, int group, const uint32_t *__restrict__ list,
-                                const int *__restrict__ num, uint32_t nrow, int *__restrict__ flags) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)rows * M) return;
-  const int p = (int)(t / M), k = (int)(t % M);
+// reported as (row, slot, entry) in flags[8..10] with flags[7] set.  One
+// thread per row; rows come in groups of G.
+template <int G>
+__global__ void k_validate_list(int rows, int M, const uint32_t *__restrict__ list, const int *__restrict__ num,
+                                uint32_t nrow, int *__restrict__ flags) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= rows) return;
   const int c = num[p];
   if (c < 0 || c > M) {
-    if (k == 0 && atomicCAS(&flags[7], 0, 1) == 0) {
+    if (atomicCAS(&flags[7], 0, 1) == 0) {
       flags[8] = p;
       flags[9] = -1;
       flags[10] = c;
     }
     return;
   }
-  if (k >= c) return;
-  const uint32_t e = list[((size_t)(p / group) * M + k) * group + (p % group)];
-  if ((e & 0x7FFFFFFu) >= nrow || (e >> 27) >= 27u) {
-    if (atomicCAS(&flags[7], 0, 1) == 0) {
-      flags[8] = p;
-      flags[9] = k;
-      flags[10] = (int)e;
+  const uint32_t *row = list + (size_t)(p / G) * M * G + (p % G);
+  for (int k = 0; k < c; ++k) {
+    const uint32_t e = row[(size_t)k * G];
+    if ((e & 0x7FFFFFFu) >= nrow || (e >> 27) >= 27u) {
+      if (atomicCAS(&flags[7], 0, 1) == 0) {
+        flags[8] = p;
+        flags[9] = k;
+        flags[10] = (int)e;
+      }
+      return;
     }
   }
 }
@@ -1483,7 +1501,7 @@ int launch_active_sites(cac_ctx *ctx) {
       (rc = ensure(ctx, ctx->b_estart, (size_t)ne + 1, &ctx->estart)))
     return rc;
   int *estart = ctx->estart;
-  hipLaunchKernelGGL(k_active_count, dim3(grid_for(ne)), dim3(kBlock), 0, ctx->stream, ne, nw, ctx->bitmap,
+  hipLaunchKernelGGL(k_active_count, dim3(grid_for(64L * ne)), dim3(kBlock), 0, ctx->stream, ne, nw, ctx->bitmap,
                      ctx->wprefix, ctx->ecount);
   size_t tmp_bytes = 0;
   CAC_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, ctx->ecount, estart, ne + 1, ctx->stream));
@@ -1503,14 +1521,12 @@ int launch_active_sites(cac_ctx *ctx) {
       (rc = ensure(ctx, ctx->b_ipneigh, (size_t)ctx->maxnip * round_up(nip, kRowGroup), &ctx->ipneigh)))
     return rc;
   ctx->xs4 = ctx->x4 + 4 * (size_t)n;
-  hipLaunchKernelGGL(k_active_finalize, dim3(grid_for(ne)), dim3(kBlock), 0, ctx->stream, ne, nw, ctx->bitmap,
+  hipLaunchKernelGGL(k_active_finalize, dim3(grid_for(64L * ne)), dim3(kBlock), 0, ctx->stream, ne, nw, ctx->bitmap,
                      estart, ctx->wprefix, ctx->asite);
-  const long tot = (long)nip * ctx->maxnip;
-  hipLaunchKernelGGL(k_ip_translate, dim3(grid_for(tot)), dim3(kBlock), 0, ctx->stream, nip, ctx->maxnip, ctx->iptmp,
+  hipLaunchKernelGGL(k_ip_translate, dim3(grid_for(nip)), dim3(kBlock), 0, ctx->stream, nip, ctx->maxnip, ctx->iptmp,
                      ctx->ipnum, n, nw, ctx->bitmap, ctx->wprefix, ctx->ipneigh);
   if (ctx->niface > 0) {
-    const long ta = (long)ctx->niface * ctx->maxneigh;
-    hipLaunchKernelGGL(k_atom_site_translate, dim3(grid_for(ta)), dim3(kBlock), 0, ctx->stream, ctx->niface,
+    hipLaunchKernelGGL(k_atom_site_translate, dim3(grid_for(ctx->niface)), dim3(kBlock), 0, ctx->stream, ctx->niface,
                        ctx->maxneigh, ctx->iface, ctx->atmp, ctx->asnum, ctx->anum, n, nw, ctx->bitmap, ctx->wprefix,
                        ctx->aneigh);
     hipLaunchKernelGGL(k_atom_site_count, dim3(grid_for(ctx->niface)), dim3(kBlock), 0, ctx->stream, ctx->niface,
@@ -1622,8 +1638,8 @@ int validate_lists(cac_ctx *ctx) {
   CAC_HIP(ctx, hipMemsetAsync(ctx->dflags + 7, 0, 4 * sizeof(int), ctx->stream));
   const int nip = ctx->ne * ctx->nint;
   if (ctx->n)
-    hipLaunchKernelGGL(k_validate_list, dim3(grid_for((long)ctx->n * ctx->maxneigh)), dim3(kBlock), 0, ctx->stream,
-                       ctx->n, ctx->maxneigh, kAtomGroup, ctx->aneigh, ctx->anum, nrow, ctx->dflags);
+    hipLaunchKernelGGL(k_validate_list<kAtomGroup>, dim3(grid_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->n,
+                       ctx->maxneigh, ctx->aneigh, ctx->anum, nrow, ctx->dflags);
   int bad[4];
   CAC_HIP(ctx, hipMemcpyAsync(bad, ctx->dflags + 7, sizeof(bad), hipMemcpyDeviceToHost, ctx->stream));
   CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -1633,8 +1649,8 @@ int validate_lists(cac_ctx *ctx) {
                                        " (n " + std::to_string(ctx->n) + ", active sites " +
                                        std::to_string(ctx->nactive) + ")");
   if (nip)
-    hipLaunchKernelGGL(k_validate_list, dim3(grid_for((long)nip * ctx->maxnip)), dim3(kBlock), 0, ctx->stream, nip,
-                       ctx->maxnip, kRowGroup, ctx->ipneigh, ctx->ipnum, nrow, ctx->dflags);
+    hipLaunchKernelGGL(k_validate_list<kRowGroup>, dim3(grid_for(nip)), dim3(kBlock), 0, ctx->stream, nip,
+                       ctx->maxnip, ctx->ipneigh, ctx->ipnum, nrow, ctx->dflags);
   CAC_HIP(ctx, hipMemcpyAsync(bad, ctx->dflags + 7, sizeof(bad), hipMemcpyDeviceToHost, ctx->stream));
   CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (bad[0])
