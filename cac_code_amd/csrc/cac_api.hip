@@ -98,13 +98,21 @@ __global__ void k_nve_nodes_final(long nq, double *__restrict__ nodev, const dou
 // When flag != null the same pass runs Neighbor::check_distance
 // (neighbor.cpp:1176-1217) on the new positions and stores 1 to *flag (host
 // memory the caller waits on) if anything moved more than sqrt(triggersq).
+//
+// With remap set, the step starts with the affine remap of k_remap (same
+// operations) on the positions and the held positions; the velocities do not
+// depend on it, so this equals k_remap followed by the step.
+struct Scale3 {
+  double lo[3], s[3];
+};
+
 __device__ __forceinline__ void raise_flag(int *flag) {
   __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 __global__ void k_nve_atoms_step(int n, double *__restrict__ x4, double *__restrict__ v,
                                  const double *__restrict__ f, double dtv, double dtfm, int fin,
-                                 const double *__restrict__ xhold, double triggersq, int *flag) {
+                                 double *__restrict__ xhold, double triggersq, int *flag, int remap, Scale3 m) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double dsq = 0.0;
@@ -114,9 +122,15 @@ __global__ void k_nve_atoms_step(int n, double *__restrict__ x4, double *__restr
     if (fin) vv += dtfm * fd;
     vv = vv + dtfm * fd;
     v[(size_t)d * n + i] = vv;
-    const double xx = x4[4 * (size_t)i + d] + dtv * vv;
+    double x0 = x4[4 * (size_t)i + d], xh = xhold[(size_t)d * n + i];
+    if (remap) {
+      x0 = m.lo[d] + (x0 - m.lo[d]) * m.s[d];
+      xh = m.lo[d] + (xh - m.lo[d]) * m.s[d];
+      xhold[(size_t)d * n + i] = xh;
+    }
+    const double xx = x0 + dtv * vv;
     x4[4 * (size_t)i + d] = xx;
-    const double dd = xx - xhold[(size_t)d * n + i];
+    const double dd = xx - xh;
     dsq = d == 0 ? dd * dd : dsq + dd * dd;
   }
   if (flag && dsq > triggersq) raise_flag(flag);
@@ -126,7 +140,8 @@ __global__ void k_nve_atoms_step(int n, double *__restrict__ x4, double *__restr
 // center is the node mean summed in node order as in k_nve_nodes_initial.
 __global__ void k_nve_nodes_step(int ne, double *__restrict__ nodex, double *__restrict__ nodev,
                                  const double *__restrict__ nodef, double *__restrict__ xc, double dtv, double dtfm,
-                                 int fin, const double *__restrict__ nodexhold, double triggersq, int *flag) {
+                                 int fin, double *__restrict__ nodexhold, double triggersq, int *flag, int remap,
+                                 Scale3 m) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const int i = (int)(t >> 3), k = (int)(t & 7);
   const bool live = i < ne;
@@ -140,9 +155,15 @@ __global__ void k_nve_nodes_step(int ne, double *__restrict__ nodex, double *__r
       if (fin) vv += dtfm * fd;
       vv = vv + dtfm * fd;
       nodev[b + d] = vv;
-      xn[d] = nodex[b + d] + dtv * vv;
+      double x0 = nodex[b + d], xh = nodexhold[b + d];
+      if (remap) {  // the center is recomputed below from the remapped nodes
+        x0 = m.lo[d] + (x0 - m.lo[d]) * m.s[d];
+        xh = m.lo[d] + (xh - m.lo[d]) * m.s[d];
+        nodexhold[b + d] = xh;
+      }
+      xn[d] = x0 + dtv * vv;
       nodex[b + d] = xn[d];
-      const double dd = xn[d] - nodexhold[b + d];
+      const double dd = xn[d] - xh;
       dsq = d == 0 ? dd * dd : dsq + dd * dd;
     }
   }
@@ -157,10 +178,6 @@ __global__ void k_nve_nodes_step(int ne, double *__restrict__ nodex, double *__r
 // Affine remap x -> lo + (x - lo)*scale of atoms, their held positions,
 // nodes, their held positions and element centers (one thread per atom or
 // node).
-struct Scale3 {
-  double lo[3], s[3];
-};
-
 __global__ void k_remap(int n, double *__restrict__ x4, double *__restrict__ xhold, int ne,
                         double *__restrict__ nodex, double *__restrict__ nodexhold, double *__restrict__ xc,
                         Scale3 m) {
@@ -396,16 +413,19 @@ int ensure_step_sync(cac_ctx *ctx) {
 }
 
 // One step boundary of cac_run (see k_nve_atoms_step); fin: apply the
-// previous step's final half-step first.
-int nve_step_impl(cac_ctx *ctx, double dtv, double dtf, bool fin, int *flag) {
+// previous step's final half-step first; remap: the step's affine remap.
+int nve_step_impl(cac_ctx *ctx, double dtv, double dtf, bool fin, int *flag, const Scale3 *remap = nullptr) {
+  Scale3 m = {};
+  if (remap) m = *remap;
   if (ctx->n)
     hipLaunchKernelGGL(k_nve_atoms_step, dim3(grid_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->n, ctx->x4,
-                       ctx->v, ctx->f, dtv, dtf / ctx->mass, fin ? 1 : 0, ctx->xhold, ctx->triggersq, flag);
+                       ctx->v, ctx->f, dtv, dtf / ctx->mass, fin ? 1 : 0, ctx->xhold, ctx->triggersq, flag,
+                       remap ? 1 : 0, m);
   if (ctx->ne) {
     double inv_npe = 1 / (double)8;
     hipLaunchKernelGGL(k_nve_nodes_step, dim3(grid_for(8L * ctx->ne)), dim3(kBlock), 0, ctx->stream, ctx->ne,
                        ctx->nodex, ctx->nodev, ctx->nodef, ctx->xc, dtv, dtf / (inv_npe * ctx->mass_e), fin ? 1 : 0,
-                       ctx->nodexhold, ctx->triggersq, flag);
+                       ctx->nodexhold, ctx->triggersq, flag, remap ? 1 : 0, m);
   }
   CAC_HIP(ctx, hipGetLastError());
   return CAC_OK;
@@ -431,13 +451,23 @@ int deform_impl(cac_ctx *ctx, const double scale[3]) {
 }
 
 // The remap of one cac_run step under a constant strain rate (engineering:
-// the edge at elapsed time t is L0*(1 + rate*t)).
-int deform_step(cac_ctx *ctx, double dtv) {
+// the edge at elapsed time t is L0*(1 + rate*t)): the box changes here, the
+// coordinates in the step kernels (Scale3 m).
+int deform_step(cac_ctx *ctx, double dtv, Scale3 *m) {
   ctx->deform_t += dtv;
-  double scale[3];
-  for (int d = 0; d < 3; ++d)
-    scale[d] = ctx->deform_l0[d] * (1.0 + ctx->deform_rate[d] * ctx->deform_t) / ctx->prd[d];
-  return deform_impl(ctx, scale);
+  double sc[3];
+  for (int d = 0; d < 3; ++d) {
+    sc[d] = ctx->deform_l0[d] * (1.0 + ctx->deform_rate[d] * ctx->deform_t) / ctx->prd[d];
+    if (!(sc[d] > 0)) return fail(ctx, CAC_ERR_ARG, "deformation scale must be positive");
+  }
+  for (int d = 0; d < 3; ++d) {
+    const double scale = sc[d];
+    m->lo[d] = ctx->lo[d];
+    m->s[d] = scale;
+    ctx->prd[d] *= scale;
+    ctx->hi[d] = ctx->lo[d] + ctx->prd[d];
+  }
+  return CAC_OK;
 }
 
 int nve_final_impl(cac_ctx *ctx, double dtf) {
@@ -893,9 +923,11 @@ int cac_run(cac_ctx *ctx, int nsteps, double dtv, double dtf, int precision, int
     ctx->ago++;
     const bool due = ctx->ago >= ctx->delay && ctx->ago % ctx->every == 0;
     const bool probe = due && ctx->check;
-    if (ctx->deform_on && (rc = deform_step(ctx, dtv))) return rc;
+    Scale3 m;
+    if (ctx->deform_on && (rc = deform_step(ctx, dtv, &m))) return rc;
     if (probe) *ctx->hcheck = 0;
-    if ((rc = nve_step_impl(ctx, dtv, dtf, step > 0, probe ? ctx->dcheck : nullptr))) return rc;
+    if ((rc = nve_step_impl(ctx, dtv, dtf, step > 0, probe ? ctx->dcheck : nullptr, ctx->deform_on ? &m : nullptr)))
+      return rc;
     if (probe) CAC_HIP(ctx, hipEventRecord(ctx->ev_check, ctx->stream));
     if (due && !ctx->check) {
       if ((rc = build_impl(ctx))) return rc;

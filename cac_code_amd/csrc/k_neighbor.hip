@@ -273,34 +273,52 @@ __global__ __launch_bounds__(64 * kNeighWaves) void k_atom_neigh(
     const double xi = x4[4 * (size_t)i], yi = x4[4 * (size_t)i + 1], zi = x4[4 * (size_t)i + 2];
     const uint32_t self = (uint32_t)i | ((uint32_t)kImgSelf << 27);
     int cnt = 0;
-    for (int f0 = 0; f0 < T; f0 += 64) {
-      const int f = f0 + lane;
-      bool acc = false;
-      uint64_t key = 0;
-      if (f < T) {
-        double xj, yj, zj;
-        uint32_t e;
-        if (staged) {
-          xj = sx[f];
-          yj = sy[f];
-          zj = sz[f];
-          e = se[f];
-        } else {
+    if (staged) {
+      // accepted candidates are compacted as their index f; keys are formed
+      // afterwards for those only
+      for (int f0 = 0; f0 < T; f0 += 64) {
+        const int f = f0 + lane;
+        bool acc = false;
+        if (f < T && se[f] != self) {
+          const double delx = xi - sx[f], dely = yi - sy[f], delz = zi - sz[f];
+          acc = delx * delx + dely * dely + delz * delz <= cutsq;
+        }
+        const unsigned long long am = __ballot(acc);
+        if (acc) {
+          const int pos = cnt + __popcll(am & ((1ull << lane) - 1ull));
+          if (pos < cap2) keys[pos] = (uint64_t)f;
+        }
+        cnt += __popcll(am);
+      }
+      wave_sync();
+      const int kept = cnt < cap2 ? cnt : cap2;
+      for (int a = lane; a < kept; a += 64) {
+        const int f = (int)keys[a];
+        keys[a] = list_key(sx[f] - xi, sy[f] - yi, sz[f] - zi, qinv, se[f]);
+      }
+    } else {
+      for (int f0 = 0; f0 < T; f0 += 64) {
+        const int f = f0 + lane;
+        bool acc = false;
+        uint64_t key = 0;
+        if (f < T) {
+          double xj, yj, zj;
+          uint32_t e;
           fetch(f, xj, yj, zj, e);
+          if (e != self) {
+            const double delx = xi - xj, dely = yi - yj, delz = zi - zj;
+            const double rsq = delx * delx + dely * dely + delz * delz;
+            acc = rsq <= cutsq;
+            key = list_key(xj - xi, yj - yi, zj - zi, qinv, e);
+          }
         }
-        if (e != self) {
-          const double delx = xi - xj, dely = yi - yj, delz = zi - zj;
-          const double rsq = delx * delx + dely * dely + delz * delz;
-          acc = rsq <= cutsq;
-          key = list_key(xj - xi, yj - yi, zj - zi, qinv, e);
+        const unsigned long long am = __ballot(acc);
+        if (acc) {
+          const int pos = cnt + __popcll(am & ((1ull << lane) - 1ull));
+          if (pos < cap2) keys[pos] = key;
         }
+        cnt += __popcll(am);
       }
-      const unsigned long long am = __ballot(acc);
-      if (acc) {
-        const int pos = cnt + __popcll(am & ((1ull << lane) - 1ull));
-        if (pos < cap2) keys[pos] = key;
-      }
-      cnt += __popcll(am);
     }
     wave_sync();
     const int kept = cnt < cap2 ? cnt : cap2;

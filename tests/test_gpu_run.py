@@ -158,3 +158,39 @@ def test_strain_rate_ramp_in_run(cu_funcfl):
     stretch = 1.0 + rate * nsteps * DT
     assert np.allclose(x, s.box_lo + (s.x - s.box_lo) * stretch, rtol=0, atol=1e-9)
     assert np.allclose(nx, s.box_lo + (s.nodex - s.box_lo) * stretch, rtol=0, atol=1e-9)
+
+
+def test_run_strain_rate_equals_stepwise(cu_funcfl):
+    """cac_run under a strain rate (the remap fused into the step kernels)
+    equals cac_deform at the start of each step of the call-by-call
+    sequence, bit for bit, with rebuilds."""
+    s = small_system(seed=27, L=10, nae=5, core=4.0, sigma=0.02, temperature=600.0)
+    pair = eam_pair(cu_funcfl)
+    rate = np.array([2e-2, -1e-2, 0.5e-2])   # per ps
+    nsteps = 20
+    a = engine(s, pair, 0.08, 1, 0, True, True)
+    b = engine(s, pair, 0.08, 1, 0, True, True)
+    try:
+        a.set_deform_rate(rate)
+        na = a.run(nsteps, dt=DT)
+        l0 = s.box_hi - s.box_lo
+        prd = l0.copy()
+        t = 0.0
+        nb = 0
+        for _ in range(nsteps):
+            t += DT
+            scale = np.array([l0[d] * (1.0 + rate[d] * t) / prd[d] for d in range(3)])
+            prd = np.array([prd[d] * scale[d] for d in range(3)])
+            b.deform(scale)
+            b.nve_initial(DT, DTF)
+            if b.decide():
+                b.build()
+                nb += 1
+            b.force()
+            b.nve_final(DTF)
+        assert na == nb and na > 0, (na, nb)
+        for qa, qb, name in zip(state(a), state(b), ["x", "v", "f", "nodex", "nodev", "nodef"]):
+            assert np.array_equal(qa, qb), name
+    finally:
+        a.close()
+        b.close()
