@@ -146,6 +146,8 @@ struct cac_ctx {
   int *abin_s = nullptr, *abin_e = nullptr;
   cac::DevBuf b_akey, b_aval, b_akey2, b_aval2, b_acs, b_ace, b_sorttmp;
   cac::DevBuf b_ahead, b_acells;  // atom list build: cell-start flags and scan, non-empty cells
+  int *acells = nullptr, nacells = 0;  // non-empty atom cells of the last build
+  double rmax = 0.0;                   // largest element radius of the last build
   // atom-element coupling (build-defined, off by default = the reference):
   // atom lists also hold element sites, integration-point lists also atoms.
   // Interface atoms are those within cutneigh + R_j of some element center.

@@ -148,23 +148,6 @@ __device__ __forceinline__ uint64_t list_key(double dx, double dy, double dz, do
   return (((q(dz) << 20) | (q(dy) << 10) | q(dx)) << 32) | entry;
 }
 
-__device__ void bitonic_sort_lds(uint64_t *a, int n2, int lane) {
-  for (int size = 2; size <= n2; size <<= 1)
-    for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      for (int t = lane; t < (n2 >> 1); t += 64) {
-        const int lo = 2 * t - (t & (stride - 1));
-        const int hi = lo + stride;
-        const bool up = (lo & size) == 0;
-        const uint64_t x = a[lo], y = a[hi];
-        if ((x > y) == up) {
-          a[lo] = y;
-          a[hi] = x;
-        }
-      }
-      __syncthreads();
-    }
-}
-
 // LDS written by some lanes of a wavefront is read by others after this.
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -361,8 +344,9 @@ __global__ __launch_bounds__(64 * kNeighWaves) void k_atom_neigh(
 // [-1,1]^3, so it lies within delta of P = c + J [-1,1]^3.  A point q within
 // cutneigh of a site is within rho = cutneigh + delta of P, and then
 // |(J^-1 (q - c))_d| <= 1 + rho |row_d(J^-1)| for every d: hull[16 i ..]
-// holds J^-1 (row-major) and these three bounds (slack for rounding added;
-// a degenerate J gets J^-1 = 0 so the test always passes).
+// holds J^-1 (row-major), these three bounds and the row norms |row_d(J^-1)|
+// (slack for rounding added; a degenerate J gets J^-1 = 0 so the test always
+// passes).
 __device__ __forceinline__ bool hull_hit(const double *__restrict__ hull, int j, double dx, double dy, double dz) {
   const double *h = hull + 16 * (size_t)j;
   for (int d = 0; d < 3; ++d) {
@@ -438,8 +422,9 @@ __global__ void k_elem_radius(int ne, const double *__restrict__ nodex, const do
       rn += v * v;
     }
     h[9 + d] = ok ? (1.0 + rho * sqrt(rn)) * (1.0 + 1e-9) + 1e-9 : 1e300;
+    h[12 + d] = sqrt(rn) * (1.0 + 1e-9);
   }
-  for (int d = 12; d < 16; ++d) h[d] = 0.0;
+  h[15] = 0.0;
 }
 
 // Position sum_k N_k X_k of a site with shape functions N, summed in node
@@ -933,6 +918,99 @@ __global__ __launch_bounds__(256) void k_atom_iface(int n, const double *__restr
   iflag[i] = found;
 }
 
+// The same flags, one wavefront per non-empty atom cell (ag, abin, abs/abe:
+// the atom cells of this build).  The cell's atoms have a bounding sphere
+// (center m, radius h); an element image that reaches one of them has
+// |c_j' - m| <= cutneigh + R_j + h and |(J^-1 (m - c_j'))_d| <= bound_d +
+// h |row_d|, and its center lies in the 27 element cells around m's (the
+// host uses this kernel only when the element cells are at least
+// cutneigh + R_max + h wide).  Lanes test 64 candidate images at a time
+// against the sphere; each survivor is then tested exactly (as in
+// k_atom_iface) by every atom of the cell, a lane per atom.  Cells deep in
+// the atomistic region see no candidate at all.
+__global__ __launch_bounds__(256) void k_atom_iface_cells(int ncells, const int *__restrict__ cells,
+                                                          const uint32_t *__restrict__ abin,
+                                                          const int *__restrict__ abs_, const int *__restrict__ abe,
+                                                          const double *__restrict__ x4, ElemBins eb, Box3 b,
+                                                          double cutneigh, int *__restrict__ iflag) {
+  const int lane = threadIdx.x & 63;
+  const int w = (xcd_block() * (int)blockDim.x + (int)threadIdx.x) >> 6;
+  if (w >= ncells) return;  // whole wavefront
+  const int c = cells[w], hs = abs_[c], na = abe[c] - hs;
+  // bounding box of the cell's atoms -> sphere (m, h)
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int a = lane; a < na; a += 64) {
+    const int i = (int)abin[hs + a];
+    for (int d = 0; d < 3; ++d) {
+      const double x = x4[4 * (size_t)i + d];
+      lo[d] = fmin(lo[d], x);
+      hi[d] = fmax(hi[d], x);
+    }
+  }
+  double m[3], h2 = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    for (int o = 32; o > 0; o >>= 1) {
+      lo[d] = fmin(lo[d], __shfl_xor(lo[d], o));
+      hi[d] = fmax(hi[d], __shfl_xor(hi[d], o));
+    }
+    m[d] = 0.5 * (lo[d] + hi[d]);
+    const double e = 0.5 * (hi[d] - lo[d]);
+    h2 += e * e;
+  }
+  const double h = sqrt(h2) * (1.0 + 1e-9) + 1e-6;
+  int ci[3], qs = 0, len = 0, img = 0;
+  for (int d = 0; d < 3; ++d) ci[d] = cell_coord(m[d], b.lo[d], eb.g.inv[d], eb.g.n[d]);
+  if (lane < 27) {
+    const int cc = cell_around(ci, lane, eb.g, &img);
+    qs = eb.bs[cc];
+    len = eb.be[cc] - qs;
+  }
+  const int incl = lane_prefix32(len, lane), excl = incl - len;
+  const int total = __shfl(incl, 26);
+  for (int a0 = 0; a0 < na; a0 += 64) {
+    const bool valid = a0 + lane < na;
+    const int i = valid ? (int)abin[hs + a0 + lane] : 0;
+    const double q[3] = {x4[4 * (size_t)i], x4[4 * (size_t)i + 1], x4[4 * (size_t)i + 2]};
+    int found = 0;
+    for (int f0 = 0; f0 < total; f0 += 64) {
+      const int f = f0 + lane;
+      const int r = range_of(incl, f);
+      const int rs = __shfl(qs, r), rx = __shfl(excl, r), rimg = __shfl(img, r);
+      const int j = f < total ? (int)eb.bin[rs + f - rx] : 0;
+      bool pass = false;
+      if (f < total) {
+        int s[3];
+        img_shift(rimg, s);
+        const double dx = m[0] - shifted(eb.erad[4 * j], s[0], b.prd[0]);
+        const double dy = m[1] - shifted(eb.erad[4 * j + 1], s[1], b.prd[1]);
+        const double dz = m[2] - shifted(eb.erad[4 * j + 2], s[2], b.prd[2]);
+        const double reach = cutneigh + eb.erad[4 * j + 3] + h;
+        pass = dx * dx + dy * dy + dz * dz <= reach * reach;
+        const double *hh = eb.hull + 16 * (size_t)j;
+        for (int d = 0; d < 3 && pass; ++d) {
+          const double u = hh[3 * d] * dx + hh[3 * d + 1] * dy + hh[3 * d + 2] * dz;
+          pass = fabs(u) <= hh[9 + d] + h * hh[12 + d];
+        }
+      }
+      unsigned long long sm = __ballot(pass);
+      while (sm) {
+        const int src = __ffsll((long long)sm) - 1;
+        sm &= sm - 1;
+        const int cj = __builtin_amdgcn_readfirstlane(__shfl(j, src));
+        const int cimg = __builtin_amdgcn_readfirstlane(__shfl(rimg, src));
+        int s[3];
+        img_shift(cimg, s);
+        const double dx = q[0] - shifted(eb.erad[4 * cj], s[0], b.prd[0]);
+        const double dy = q[1] - shifted(eb.erad[4 * cj + 1], s[1], b.prd[1]);
+        const double dz = q[2] - shifted(eb.erad[4 * cj + 2], s[2], b.prd[2]);
+        const double reach = cutneigh + eb.erad[4 * cj + 3];
+        if (dx * dx + dy * dy + dz * dz <= reach * reach && hull_hit(eb.hull, cj, dx, dy, dz)) found = 1;
+      }
+    }
+    if (valid) iflag[i] = found;
+  }
+}
+
 __global__ void k_iface_compact(int n, const int *__restrict__ iflag, const int *__restrict__ ipos,
                                 int *__restrict__ iface) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1350,6 +1428,8 @@ int launch_atom_neighbors(cac_ctx *ctx) {
   CAC_HIP(ctx, hipMemcpyAsync(&hv[1], ctx->dflags + 13, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
   const int ncells = hv[0], maxcand = hv[1];
+  ctx->acells = cells;
+  ctx->nacells = ncells;
   int cap2 = 64;
   while (cap2 < ctx->maxneigh) cap2 <<= 1;
   // LDS: the four wavefronts' key rows, then up to ccap staged candidates
@@ -1389,6 +1469,7 @@ int launch_element_neighbors(cac_ctx *ctx) {
   CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
   double rmax;
   memcpy(&rmax, &hbits, sizeof(rmax));
+  ctx->rmax = rmax;
   if ((rc = ensure(ctx, ctx->b_eneigh, (size_t)ctx->maxelist * ne, &ctx->eneigh)) ||
       (rc = ensure(ctx, ctx->b_enumb, ne, &ctx->enumb)))
     return rc;
@@ -1473,8 +1554,19 @@ int launch_atom_sites(cac_ctx *ctx) {
     return rc;
   hipEvent_t t0;
   timer_begin(ctx, "atom_sites", &t0);
-  hipLaunchKernelGGL(k_atom_iface, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, n, ctx->x4, elem_bins(ctx),
-                     box_of(ctx), ctx->cutneigh, ctx->iflag);
+  // per atom cell when the element cells are wide enough (see k_atom_iface_cells)
+  double hcell2 = 0.0;
+  for (int d = 0; d < 3; ++d) hcell2 += 0.25 * (ctx->prd[d] / ctx->acell[d]) * (ctx->prd[d] / ctx->acell[d]);
+  bool by_cell = ctx->ecell[0] > 0 && ctx->nacells > 0;
+  for (int d = 0; d < 3 && by_cell; ++d)
+    by_cell = ctx->prd[d] / ctx->ecell[d] >= (ctx->cutneigh + ctx->rmax + sqrt(hcell2)) * 1.001 + 1e-6;
+  if (by_cell)
+    hipLaunchKernelGGL(k_atom_iface_cells, dim3(xcd_grid(64L * ctx->nacells, kBlock)), dim3(kBlock), 0, ctx->stream,
+                       ctx->nacells, ctx->acells, ctx->abin, ctx->abin_s, ctx->abin_e, ctx->x4, elem_bins(ctx),
+                       box_of(ctx), ctx->cutneigh, ctx->iflag);
+  else
+    hipLaunchKernelGGL(k_atom_iface, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, n, ctx->x4, elem_bins(ctx),
+                       box_of(ctx), ctx->cutneigh, ctx->iflag);
   CAC_HIP(ctx, hipMemsetAsync(ctx->iflag + n, 0, sizeof(int), ctx->stream));
   size_t tmp_bytes = 0;
   CAC_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, ctx->iflag, ctx->ipos, n + 1, ctx->stream));
@@ -1555,27 +1647,47 @@ int launch_active_sites(cac_ctx *ctx) {
 }
 
 // ------------------------------------------------------------ clusters
-// Per vertex cluster (one wavefront): the entries of its integration points'
-// lists (<= 8 rows of the blocked ipneigh layout) are sorted in LDS, the
-// distinct values become the cluster's union (ascending: atoms first, then
-// active sites element by element), and every list entry is rewritten as
-// its index in the union (uint16, same layout).  The largest union size
-// goes to flags[12]: it sets the LDS rows of the cluster passes.
+// Per vertex cluster (one workgroup of 256 threads): the entries of its
+// integration points' lists (<= 8 rows of the blocked ipneigh layout) are
+// sorted in LDS (bitonic, 32-bit), the distinct values become the cluster's
+// union (ascending: atoms first, then active sites element by element), and
+// every list entry is rewritten as its index in the union (uint16, same
+// layout).  The largest union size goes to flags[12]: it sets the LDS rows of
+// the cluster passes.
 struct NodeList8 {
   int k[8];
 };
 
-__global__ __launch_bounds__(64) void k_cl_build(int ncl, const int *__restrict__ cl_start,
-                                                 const int *__restrict__ cl_slot, int nint, NodeList8 nl,
-                                                 const uint32_t *__restrict__ ipneigh, const int *__restrict__ ipnum,
-                                                 int M, int cap2, uint32_t *__restrict__ cl_union,
-                                                 int *__restrict__ cl_nu, uint16_t *__restrict__ iplocal,
-                                                 int *__restrict__ flags) {
-  extern __shared__ uint64_t keys[];  // [cap2] sort keys, then [cap2] uint32 union
-  uint32_t *uni = reinterpret_cast<uint32_t *>(keys + cap2);
+constexpr int kClBuildThreads = 256;
+
+__device__ void bitonic_sort_lds32(uint32_t *a, int n2, int tid) {
+  for (int size = 2; size <= n2; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int t = tid; t < (n2 >> 1); t += kClBuildThreads) {
+        const int lo = 2 * t - (t & (stride - 1));
+        const int hi = lo + stride;
+        const bool up = (lo & size) == 0;
+        const uint32_t x = a[lo], y = a[hi];
+        if ((x > y) == up) {
+          a[lo] = y;
+          a[hi] = x;
+        }
+      }
+      __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kClBuildThreads) void k_cl_build(
+    int ncl, const int *__restrict__ cl_start, const int *__restrict__ cl_slot, int nint, NodeList8 nl,
+    const uint32_t *__restrict__ ipneigh, const int *__restrict__ ipnum, int M, int cap2,
+    uint32_t *__restrict__ cl_union, int *__restrict__ cl_nu, uint16_t *__restrict__ iplocal,
+    int *__restrict__ flags) {
+  extern __shared__ uint32_t cl_lds[];  // [cap2] sorted entries, then [cap2] union
+  uint32_t *keys = cl_lds, *uni = cl_lds + cap2;
+  __shared__ int s_u;
   const int c = blockIdx.x;
-  if (c >= ncl) return;
-  const int lane = (int)threadIdx.x;
+  if (c >= ncl) return;  // whole workgroup
+  const int tid = (int)threadIdx.x, lane = tid & 63;
   const int s0 = cl_start[c], ns = cl_start[c + 1] - s0;
   int pid[kClMax], off[kClMax + 1];
   off[0] = 0;
@@ -1591,9 +1703,11 @@ __global__ __launch_bounds__(64) void k_cl_build(int ncl, const int *__restrict_
     off[i + 1] = off[i] + cnt;
   }
   const int m = off[kClMax];
+  int n2 = 64;  // <= cap2: m <= kClMax * M
+  while (n2 < m) n2 <<= 1;
   auto entry_ptr = [&](int p, int k) -> size_t { return ((size_t)(p >> 3) * M + k) * kRowGroup + (p & 7); };
-  for (int t = lane; t < cap2; t += 64) {
-    uint64_t v = ~0ull;
+  for (int t = tid; t < n2; t += kClBuildThreads) {
+    uint32_t v = 0xFFFFFFFFu;  // above every entry (image codes are < 27)
     if (t < m) {
       int i = 0;
       while (t >= off[i + 1]) ++i;
@@ -1602,22 +1716,26 @@ __global__ __launch_bounds__(64) void k_cl_build(int ncl, const int *__restrict_
     keys[t] = v;
   }
   __syncthreads();
-  bitonic_sort_lds(keys, cap2, lane);
-  int u = 0;
-  for (int t0 = 0; t0 < m; t0 += 64) {
-    const int t = t0 + lane;
-    const bool first = t < m && (t == 0 || keys[t] != keys[t - 1]);
-    const unsigned long long bm = __ballot(first);
-    if (first) uni[u + __popcll(bm & ((1ull << lane) - 1ull))] = (uint32_t)keys[t];
-    u += __popcll(bm);
+  bitonic_sort_lds32(keys, n2, tid);
+  if (tid < 64) {  // the first wavefront compacts the distinct values
+    int u = 0;
+    for (int t0 = 0; t0 < m; t0 += 64) {
+      const int t = t0 + lane;
+      const bool first = t < m && (t == 0 || keys[t] != keys[t - 1]);
+      const unsigned long long bm = __ballot(first);
+      if (first) uni[u + __popcll(bm & ((1ull << lane) - 1ull))] = keys[t];
+      u += __popcll(bm);
+    }
+    if (lane == 0) {
+      s_u = u;
+      cl_nu[c] = u;
+      atomicMax(&flags[12], u);
+    }
   }
   __syncthreads();
-  if (lane == 0) {
-    cl_nu[c] = u;
-    atomicMax(&flags[12], u);
-  }
-  for (int t = lane; t < u; t += 64) cl_union[(size_t)c * cap2 + t] = uni[t];
-  for (int t = lane; t < m; t += 64) {
+  const int u = s_u;
+  for (int t = tid; t < u; t += kClBuildThreads) cl_union[(size_t)c * cap2 + t] = uni[t];
+  for (int t = tid; t < m; t += kClBuildThreads) {
     int i = 0;
     while (t >= off[i + 1]) ++i;
     const size_t q = entry_ptr(pid[i], t - off[i]);
@@ -1691,7 +1809,7 @@ int launch_clusters(cac_ctx *ctx) {
   const int M = ctx->maxnip;
   int cap2 = 64;
   while (cap2 < kClMax * M) cap2 <<= 1;
-  const size_t lds = (size_t)cap2 * (sizeof(uint64_t) + sizeof(uint32_t));
+  const size_t lds = (size_t)cap2 * 2 * sizeof(uint32_t);
   if (lds > ctx->lds_bytes) return CAC_OK;  // lists too long for the cluster passes
   if ((rc = ensure(ctx, ctx->b_cl_union, (size_t)ctx->ncl * cap2, &ctx->cl_union)) ||
       (rc = ensure(ctx, ctx->b_cl_nu, (size_t)ctx->ncl, &ctx->cl_nu)) ||
@@ -1703,7 +1821,8 @@ int launch_clusters(cac_ctx *ctx) {
   CAC_HIP(ctx, hipMemsetAsync(ctx->dflags + 11, 0, 2 * sizeof(int), ctx->stream));
   CAC_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_cl_build),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL(k_cl_build, dim3(ctx->ncl), dim3(64), lds, ctx->stream, ctx->ncl, ctx->cl_start, ctx->cl_slot,
+  hipLaunchKernelGGL(k_cl_build, dim3(ctx->ncl), dim3(kClBuildThreads), lds, ctx->stream, ctx->ncl, ctx->cl_start,
+                     ctx->cl_slot,
                      ctx->nint, nl, ctx->ipneigh, ctx->ipnum, M, cap2, ctx->cl_union, ctx->cl_nu, ctx->iplocal,
                      ctx->dflags);
   CAC_HIP(ctx, hipGetLastError());
