@@ -112,10 +112,11 @@ __device__ __forceinline__ void raise_flag(int *flag) {
 
 __global__ void k_nve_atoms_step(int n, double *__restrict__ x4, double *__restrict__ v,
                                  const double *__restrict__ f, double dtv, double dtfm, int fin,
-                                 double *__restrict__ xhold, double triggersq, int *flag, int remap, Scale3 m) {
+                                 double *__restrict__ xhold, double triggersq, int *flag, int remap, Scale3 m,
+                                 double *__restrict__ xhold_in, double thr_in, int *__restrict__ ibad) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double dsq = 0.0;
+  double dsq = 0.0, din = 0.0;
   for (int d = 0; d < 3; ++d) {
     const double fd = f[(size_t)d * n + i];
     double vv = v[(size_t)d * n + i];
@@ -123,17 +124,26 @@ __global__ void k_nve_atoms_step(int n, double *__restrict__ x4, double *__restr
     vv = vv + dtfm * fd;
     v[(size_t)d * n + i] = vv;
     double x0 = x4[4 * (size_t)i + d], xh = xhold[(size_t)d * n + i];
+    double xq = xhold_in ? xhold_in[(size_t)d * n + i] : 0.0;
     if (remap) {
       x0 = m.lo[d] + (x0 - m.lo[d]) * m.s[d];
       xh = m.lo[d] + (xh - m.lo[d]) * m.s[d];
       xhold[(size_t)d * n + i] = xh;
+      if (xhold_in) {
+        xq = m.lo[d] + (xq - m.lo[d]) * m.s[d];
+        xhold_in[(size_t)d * n + i] = xq;
+      }
     }
     const double xx = x0 + dtv * vv;
     x4[4 * (size_t)i + d] = xx;
     const double dd = xx - xh;
     dsq = d == 0 ? dd * dd : dsq + dd * dd;
+    const double dq = xx - xq;
+    din += dq * dq;
   }
   if (flag && dsq > triggersq) raise_flag(flag);
+  // the density pass's inner mask (see k_eam_atom_rho) needs motion below thr_in
+  if (xhold_in && din > thr_in) __hip_atomic_store(ibad, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Nodes: 8 lanes per element (one per node, 24 contiguous bytes each); the
@@ -290,6 +300,15 @@ int upload_tables(cac_ctx *ctx, int nrho, double drho, const double *frho_spline
   return CAC_OK;
 }
 
+// The density pass's inner mask no longer matches the positions (atoms moved
+// by a path without its motion check, a new build or a remap): the next
+// density pass is a full one.
+int invalidate_inner(cac_ctx *ctx) {
+  for (int d = 0; d < 3; ++d) ctx->inner_prod[d] = 1.0;
+  CAC_HIP(ctx, hipMemsetAsync(ctx->dflags + 14, 0xFF, sizeof(int), ctx->stream));
+  return CAC_OK;
+}
+
 int read_flags(cac_ctx *ctx) {
   CAC_HIP(ctx, hipMemcpyAsync(ctx->hflags, ctx->dflags, 8 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -364,7 +383,7 @@ int build_impl(cac_ctx *ctx) {
     ctx->built = true;
     ctx->ago = 0;
     ctx->ncalls++;
-    return CAC_OK;
+    return invalidate_inner(ctx);
   }
   return fail(ctx, CAC_ERR_OVERFLOW, "neighbor capacity still exceeded after regrowth");
 }
@@ -385,6 +404,7 @@ int decide_impl(cac_ctx *ctx, int *rebuild) {
 }
 
 int nve_initial_impl(cac_ctx *ctx, double dtv, double dtf) {
+  if (ctx->n) invalidate_inner(ctx);
   if (ctx->n)
     hipLaunchKernelGGL(k_nve_atoms_initial, dim3(grid_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->n, ctx->x4,
                        ctx->v, ctx->f, dtv, dtf / ctx->mass);
@@ -417,10 +437,25 @@ int ensure_step_sync(cac_ctx *ctx) {
 int nve_step_impl(cac_ctx *ctx, double dtv, double dtf, bool fin, int *flag, const Scale3 *remap = nullptr) {
   Scale3 m = {};
   if (remap) m = *remap;
+  // inner-mask motion check (k_eam_atom_rho) with the held positions of
+  // this atom set.  Remaps since the last invalidation scale pair distances
+  // by at most eps = max |prod(s) - 1|: an atom pair outside the inner set
+  // (r >= cin) is still beyond cutforce while 2*motion < margin - eps*cin.
+  // Otherwise the next density pass is a full one.
+  double thr = 0.0;
+  if (remap)
+    for (int d = 0; d < 3; ++d) ctx->inner_prod[d] *= remap->s[d];
+  {
+    double eps = 0.0;
+    for (int d = 0; d < 3; ++d) eps = fmax(eps, fabs(ctx->inner_prod[d] - 1.0));
+    thr = 0.5 * (ctx->inner_margin - eps * ctx->inner_cut);
+  }
+  const bool inner = thr > 0.0 && ctx->xhold_in && ctx->b_xhold_in.bytes >= 3 * (size_t)ctx->n * sizeof(double);
+  if (ctx->n && !inner) invalidate_inner(ctx);
   if (ctx->n)
     hipLaunchKernelGGL(k_nve_atoms_step, dim3(grid_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->n, ctx->x4,
                        ctx->v, ctx->f, dtv, dtf / ctx->mass, fin ? 1 : 0, ctx->xhold, ctx->triggersq, flag,
-                       remap ? 1 : 0, m);
+                       remap ? 1 : 0, m, inner ? ctx->xhold_in : nullptr, thr * thr, ctx->dflags + 14);
   if (ctx->ne) {
     double inv_npe = 1 / (double)8;
     hipLaunchKernelGGL(k_nve_nodes_step, dim3(grid_for(8L * ctx->ne)), dim3(kBlock), 0, ctx->stream, ctx->ne,
@@ -439,6 +474,7 @@ int deform_impl(cac_ctx *ctx, const double scale[3]) {
     m.s[d] = scale[d];
   }
   const long tot = (long)ctx->n + 8L * ctx->ne;
+  if (ctx->n) invalidate_inner(ctx);
   if (tot > 0)
     hipLaunchKernelGGL(k_remap, dim3(grid_for(tot)), dim3(kBlock), 0, ctx->stream, ctx->n, ctx->x4, ctx->xhold,
                        ctx->ne, ctx->nodex, ctx->ne ? ctx->nodexhold : nullptr, ctx->xc, m);

@@ -139,6 +139,16 @@ struct cac_ctx {
   // density pass of every EAM force call for its force pass
   uint32_t *amask = nullptr;
   cac::DevBuf b_amask;
+  // inner mask: bits of the entries with r^2 < inner_cutsq, (cutforce +
+  // margin)^2, at the positions xhold_in of the last full density pass; the
+  // density pass gathers only those while no atom has moved more than
+  // margin/2 since (device flag *ibad == 0, raised by cac_run's step kernel,
+  // set by the host on any other change)
+  uint32_t *amask_in = nullptr;
+  double *xhold_in = nullptr;
+  cac::DevBuf b_amask_in, b_xhold_in;
+  double inner_cutsq = -1.0, inner_margin = 0.0, inner_cut = 0.0;
+  double inner_prod[3] = {1.0, 1.0, 1.0};  // remap scales applied since the last invalidation
   // atom cells (kept from the last build: the coupled integration-point
   // search reads them)
   int acell[3] = {0, 0, 0};

@@ -394,6 +394,14 @@ __device__ __forceinline__ void fetch_atom_entries(const uint32_t *__restrict__ 
 // exactly the pairs the full list would give it, in list order (same
 // bits), while skipping the row gathers of the skin shell (about 46% of the
 // entries at 5.95 A vs 4.95 A).
+//
+// Inner mask (a second, shorter Verlet list inside the neighbor list): a
+// full density pass (*ibad != 0) gathers every entry and also records the
+// entries with r^2 < cutinsq = (cutforce + margin)^2, and the positions
+// (xhold_in).  While no atom has moved more than margin/2 since, no atom pair
+// outside that set can be within cutforce, so the next density passes
+// (*ibad == 0) gather only the inner entries (and every element site, which
+// moves with its nodes): the same pairs in the same order, so the same bits.
 constexpr int kAtomWG = 1024;
 
 template <class P, bool EV>
@@ -401,22 +409,32 @@ __global__ __launch_bounds__(kAtomWG) void k_eam_atom_rho(int n, double *__restr
                                                           const uint32_t *__restrict__ neigh, int M,
                                                           const int *__restrict__ num, Tabs<P> t, int m0,
                                                           double *__restrict__ rho_out, double *__restrict__ eatom,
-                                                          uint32_t *__restrict__ mask) {
+                                                          uint32_t *__restrict__ mask, const int *__restrict__ ibad,
+                                                          uint32_t *__restrict__ mask_in,
+                                                          double *__restrict__ xhold_in, double cutinsq) {
   extern __shared__ __align__(16) unsigned char smem[];
   const Win<P> w = make_win<P>(smem, m0);
   load_window<P>(t.rv, 4, m0, t.nr, reinterpret_cast<P *>(smem));
   __syncthreads();
   const int i = xcd_block() * kAtomWG + (int)threadIdx.x;
   if (i >= n) return;
+  const bool inner = *ibad == 0;
   const double xi = x4[4 * (size_t)i], yi = x4[4 * (size_t)i + 1], zi = x4[4 * (size_t)i + 2];
   const int jn = num[i];
   const uint32_t *row = neigh + atom_slot(i, 0, M);
   const int W = (M + 31) >> 5;
-  uint32_t bits = 0;
+  uint32_t bits = 0, bits_in = 0;
   double rho = 0.0;
   for (int k0 = 0; k0 < jn; k0 += kUA) {
+    if (inner && (k0 & 31) == 0) bits_in = mask_in[mask_slot(i, k0 >> 5, W)];
     uint32_t e[kUA];
     fetch_atom_entries<kUA>(row, k0, jn, e);
+    if (inner) {  // element sites (rows >= n, moved by their nodes) are always gathered
+      const uint32_t sel = bits_in >> (k0 & 31);
+#pragma unroll
+      for (int u = 0; u < kUA; ++u)
+        if (!((sel >> u) & 1u) && (e[u] & 0x7FFFFFFu) < (uint32_t)n) e[u] = kNoEntry;
+    }
     Pt xj[kUA];
 #pragma unroll
     for (int u = 0; u < kUA; ++u)
@@ -430,11 +448,19 @@ __global__ __launch_bounds__(kAtomWG) void k_eam_atom_rho(int n, double *__restr
         rho += (double)rho_of_r_w<P>(t, w, sqrt_p<P>(rsq));
         bits |= 1u << ((k0 + u) & 31);
       }
+      if (!inner && rsq < cutinsq) bits_in |= 1u << ((k0 + u) & 31);
     }
     if (((k0 + kUA) & 31) == 0 || k0 + kUA >= jn) {
       mask[mask_slot(i, k0 >> 5, W)] = bits;
+      if (!inner) mask_in[mask_slot(i, k0 >> 5, W)] = bits_in;
       bits = 0;
+      bits_in = 0;
     }
+  }
+  if (!inner) {
+    xhold_in[i] = xi;
+    xhold_in[n + i] = yi;
+    xhold_in[2 * (size_t)n + i] = zi;
   }
   rho_out[i] = rho;
   const double fp = (double)embed_fp<P>(t, rho);
@@ -1251,6 +1277,28 @@ TabWindow knot_window(const cac_ctx *ctx) {
   return w;
 }
 
+// The atom density pass (see k_eam_atom_rho): full or inner, as the device
+// flag says; the inner mask is then valid for the current positions.
+template <class P, bool EV>
+int atom_rho(cac_ctx *ctx, const Tabs<P> &t, unsigned agrid, const TabWindow &wr) {
+  const double skin = ctx->cutneigh - sqrt(ctx->eam.cutforcesq);
+  const double margin = 0.4 * (skin > 0.0 ? skin : 0.0);
+  const double cin = sqrt(ctx->eam.cutforcesq) + margin;
+  const double cutinsq = cin * cin;
+  int *ibad = ctx->dflags + 14;
+  if (cutinsq != ctx->inner_cutsq) {  // new cutoff or skin: the inner mask is stale
+    ctx->inner_cutsq = cutinsq;
+    CAC_HIP(ctx, hipMemsetAsync(ibad, 0xFF, sizeof(int), ctx->stream));
+  }
+  ctx->inner_margin = margin;
+  ctx->inner_cut = cin;
+  hipLaunchKernelGGL((k_eam_atom_rho<P, EV>), dim3(agrid), dim3(kAtomWG), wr.bytes, ctx->stream, ctx->n, ctx->x4,
+                     prd_of(ctx), ctx->aneigh, ctx->maxneigh, ctx->anum, t, wr.m0, ctx->rho, ctx->eatom, ctx->amask,
+                     ibad, ctx->amask_in, ctx->xhold_in, cutinsq);
+  CAC_HIP(ctx, hipMemsetAsync(ibad, 0, sizeof(int), ctx->stream));
+  return CAC_OK;
+}
+
 // In-range masks written by the density passes: atoms one bit per list
 // entry (mask_slot layout), integration points one word group per lane
 // (only when every integration point is a node point).
@@ -1258,7 +1306,11 @@ int ensure_masks(cac_ctx *ctx) {
   int rc;
   if (ctx->n) {
     const size_t rows = (size_t)round_up(ctx->n, kAtomGroup);
-    if ((rc = ensure(ctx, ctx->b_amask, rows * ((ctx->maxneigh + 31) / 32), &ctx->amask))) return rc;
+    const size_t words = rows * ((ctx->maxneigh + 31) / 32);
+    if ((rc = ensure(ctx, ctx->b_amask, words, &ctx->amask)) ||
+        (rc = ensure(ctx, ctx->b_amask_in, words, &ctx->amask_in)) ||
+        (rc = ensure(ctx, ctx->b_xhold_in, 3 * (size_t)ctx->n, &ctx->xhold_in)))
+      return rc;
   }
   const long nip = (long)ctx->ne * ctx->nint;
   if (nip && ctx->ip_on_nodes) {
@@ -1316,9 +1368,7 @@ int eam_clusters(cac_ctx *ctx, const ElemArgs &base) {
     CAC_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_eam_atom_force<P, EV>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)wz.bytes));
     timer_begin(ctx, "atom_rho", &t0);
-    hipLaunchKernelGGL((k_eam_atom_rho<P, EV>), dim3(agrid), dim3(kAtomWG), wr.bytes, ctx->stream, n, ctx->x4,
-                       prd_of(ctx), ctx->aneigh, ctx->maxneigh, ctx->anum, t, wr.m0, ctx->rho, ctx->eatom,
-                       ctx->amask);
+    if ((rc = atom_rho<P, EV>(ctx, t, agrid, wr))) return rc;
     timer_end(ctx, "atom_rho", t0);
   }
   timer_begin(ctx, "ip_erho", &t0);
@@ -1376,9 +1426,7 @@ int eam_all(cac_ctx *ctx, const ElemArgs &base, const ElemArgs &spos, const Elem
     CAC_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_eam_atom_force<P, EV>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)wz.bytes));
     timer_begin(ctx, "atom_rho", &t0);
-    hipLaunchKernelGGL((k_eam_atom_rho<P, EV>), dim3(agrid), dim3(kAtomWG), wr.bytes, ctx->stream, n, ctx->x4,
-                       prd_of(ctx), ctx->aneigh, ctx->maxneigh, ctx->anum, t, wr.m0, ctx->rho, ctx->eatom,
-                       ctx->amask);
+    if ((rc = atom_rho<P, EV>(ctx, t, agrid, wr))) return rc;
     timer_end(ctx, "atom_rho", t0);
   }
   if (el && host_nelem(erho_el)) {

@@ -194,3 +194,25 @@ def test_run_strain_rate_equals_stepwise(cu_funcfl):
     finally:
         a.close()
         b.close()
+
+
+@pytest.mark.parametrize("precision", [_lib.PREC_FP64, _lib.PREC_MIXED], ids=["fp64", "mixed"])
+def test_run_inner_mask_equals_stepwise(cu_funcfl, precision):
+    """Hot run with a wide rebuild trigger: atoms cross the density pass's
+    inner-mask margin between rebuilds, so cac_run's density passes switch
+    between inner and full gathers; the call-by-call path (full gathers
+    every step) must give the same bits."""
+    s = small_system(seed=29, L=10, nae=5, core=4.0, sigma=0.02, temperature=1500.0)
+    pair = eam_pair(cu_funcfl)
+    nsteps = 40
+    a = engine(s, pair, 0.9, 1, 0, True, True)
+    b = engine(s, pair, 0.9, 1, 0, True, True)
+    try:
+        na = a.run(nsteps, dt=DT, precision=precision)
+        nb = stepwise(b, nsteps, precision)
+        assert na == nb, (na, nb)
+        for qa, qb, name in zip(state(a), state(b), ["x", "v", "f", "nodex", "nodev", "nodef"]):
+            assert np.array_equal(qa, qb), name
+    finally:
+        a.close()
+        b.close()
