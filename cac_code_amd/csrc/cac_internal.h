@@ -111,6 +111,7 @@ struct cac_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   size_t lds_bytes = 65536;  // shared memory per workgroup on this device
+  int ncu = 256;             // compute units on this device
   std::string err;
 
   // box (fully periodic)

@@ -402,7 +402,30 @@ __device__ __forceinline__ void fetch_atom_entries(const uint32_t *__restrict__ 
 // outside that set can be within cutforce, so the next density passes
 // (*ibad == 0) gather only the inner entries (and every element site, which
 // moves with its nodes): the same pairs in the same order, so the same bits.
+//
+// The table window fills a CU's LDS, so one workgroup runs per CU at a time.
+// A large system runs one thread per atom (many rounds of workgroups; the
+// block order keeps each XCD's workgroups on neighboring atoms).  When the
+// atoms fill fewer than two rounds, the last round would leave CUs idle:
+// then one workgroup per CU takes an equal contiguous range of `per` atoms
+// (a multiple of 64) in strides of kAtomWG.
 constexpr int kAtomWG = 1024;
+
+struct AtomGrid {
+  unsigned blocks;
+  int per;
+};
+
+AtomGrid atom_grid(const cac_ctx *ctx, int n) {
+  AtomGrid g = {0, 0};
+  if (n <= 0) return g;  // no atom launches
+  long nb = ((long)n + kAtomWG - 1) / kAtomWG;
+  if (nb <= 2L * ctx->ncu && nb > ctx->ncu) nb = ctx->ncu;
+  nb = ((nb + 7) / 8) * 8;  // xcd_block() needs a multiple of 8
+  g.blocks = (unsigned)nb;
+  g.per = (int)((((long)n + nb - 1) / nb + 63) / 64 * 64);
+  return g;
+}
 
 template <class P, bool EV>
 __global__ __launch_bounds__(kAtomWG) void k_eam_atom_rho(int n, double *__restrict__ x4, Prd prd,
@@ -411,61 +434,62 @@ __global__ __launch_bounds__(kAtomWG) void k_eam_atom_rho(int n, double *__restr
                                                           double *__restrict__ rho_out, double *__restrict__ eatom,
                                                           uint32_t *__restrict__ mask, const int *__restrict__ ibad,
                                                           uint32_t *__restrict__ mask_in,
-                                                          double *__restrict__ xhold_in, double cutinsq) {
+                                                          double *__restrict__ xhold_in, double cutinsq, int per) {
   extern __shared__ __align__(16) unsigned char smem[];
   const Win<P> w = make_win<P>(smem, m0);
   load_window<P>(t.rv, 4, m0, t.nr, reinterpret_cast<P *>(smem));
   __syncthreads();
-  const int i = xcd_block() * kAtomWG + (int)threadIdx.x;
-  if (i >= n) return;
   const bool inner = *ibad == 0;
-  const double xi = x4[4 * (size_t)i], yi = x4[4 * (size_t)i + 1], zi = x4[4 * (size_t)i + 2];
-  const int jn = num[i];
-  const uint32_t *row = neigh + atom_slot(i, 0, M);
-  const int W = (M + 31) >> 5;
-  uint32_t bits = 0, bits_in = 0;
-  double rho = 0.0;
-  for (int k0 = 0; k0 < jn; k0 += kUA) {
-    if (inner && (k0 & 31) == 0) bits_in = mask_in[mask_slot(i, k0 >> 5, W)];
-    uint32_t e[kUA];
-    fetch_atom_entries<kUA>(row, k0, jn, e);
-    if (inner) {  // element sites (rows >= n, moved by their nodes) are always gathered
-      const uint32_t sel = bits_in >> (k0 & 31);
-#pragma unroll
-      for (int u = 0; u < kUA; ++u)
-        if (!((sel >> u) & 1u) && (e[u] & 0x7FFFFFFu) < (uint32_t)n) e[u] = kNoEntry;
-    }
-    Pt xj[kUA];
-#pragma unroll
-    for (int u = 0; u < kUA; ++u)
-      if (e[u] != kNoEntry) xj[u] = load_xyz(x4, e[u], prd);
-#pragma unroll
-    for (int u = 0; u < kUA; ++u) {
-      if (e[u] == kNoEntry) continue;
-      const double delx = xi - xj[u].x, dely = yi - xj[u].y, delz = zi - xj[u].z;
-      const double rsq = delx * delx + dely * dely + delz * delz;
-      if (rsq < t.cutforcesq) {
-        rho += (double)rho_of_r_w<P>(t, w, sqrt_p<P>(rsq));
-        bits |= 1u << ((k0 + u) & 31);
+  const int i0 = xcd_block() * per, i1 = min(n, i0 + per);
+  for (int i = i0 + (int)threadIdx.x; i < i1; i += kAtomWG) {
+    const double xi = x4[4 * (size_t)i], yi = x4[4 * (size_t)i + 1], zi = x4[4 * (size_t)i + 2];
+    const int jn = num[i];
+    const uint32_t *row = neigh + atom_slot(i, 0, M);
+    const int W = (M + 31) >> 5;
+    uint32_t bits = 0, bits_in = 0;
+    double rho = 0.0;
+    for (int k0 = 0; k0 < jn; k0 += kUA) {
+      if (inner && (k0 & 31) == 0) bits_in = mask_in[mask_slot(i, k0 >> 5, W)];
+      uint32_t e[kUA];
+      fetch_atom_entries<kUA>(row, k0, jn, e);
+      if (inner) {  // element sites (rows >= n, moved by their nodes) are always gathered
+        const uint32_t sel = bits_in >> (k0 & 31);
+  #pragma unroll
+        for (int u = 0; u < kUA; ++u)
+          if (!((sel >> u) & 1u) && (e[u] & 0x7FFFFFFu) < (uint32_t)n) e[u] = kNoEntry;
       }
-      if (!inner && rsq < cutinsq) bits_in |= 1u << ((k0 + u) & 31);
+      Pt xj[kUA];
+  #pragma unroll
+      for (int u = 0; u < kUA; ++u)
+        if (e[u] != kNoEntry) xj[u] = load_xyz(x4, e[u], prd);
+  #pragma unroll
+      for (int u = 0; u < kUA; ++u) {
+        if (e[u] == kNoEntry) continue;
+        const double delx = xi - xj[u].x, dely = yi - xj[u].y, delz = zi - xj[u].z;
+        const double rsq = delx * delx + dely * dely + delz * delz;
+        if (rsq < t.cutforcesq) {
+          rho += (double)rho_of_r_w<P>(t, w, sqrt_p<P>(rsq));
+          bits |= 1u << ((k0 + u) & 31);
+        }
+        if (!inner && rsq < cutinsq) bits_in |= 1u << ((k0 + u) & 31);
+      }
+      if (((k0 + kUA) & 31) == 0 || k0 + kUA >= jn) {
+        mask[mask_slot(i, k0 >> 5, W)] = bits;
+        if (!inner) mask_in[mask_slot(i, k0 >> 5, W)] = bits_in;
+        bits = 0;
+        bits_in = 0;
+      }
     }
-    if (((k0 + kUA) & 31) == 0 || k0 + kUA >= jn) {
-      mask[mask_slot(i, k0 >> 5, W)] = bits;
-      if (!inner) mask_in[mask_slot(i, k0 >> 5, W)] = bits_in;
-      bits = 0;
-      bits_in = 0;
+    if (!inner) {
+      xhold_in[i] = xi;
+      xhold_in[n + i] = yi;
+      xhold_in[2 * (size_t)n + i] = zi;
     }
+    rho_out[i] = rho;
+    const double fp = (double)embed_fp<P>(t, rho);
+    x4[4 * (size_t)i + 3] = fp;
+    if (EV) eatom[i] = embed_energy<P>(t, rho, fp);
   }
-  if (!inner) {
-    xhold_in[i] = xi;
-    xhold_in[n + i] = yi;
-    xhold_in[2 * (size_t)n + i] = zi;
-  }
-  rho_out[i] = rho;
-  const double fp = (double)embed_fp<P>(t, rho);
-  x4[4 * (size_t)i + 3] = fp;
-  if (EV) eatom[i] = embed_energy<P>(t, rho, fp);
 }
 
 // force pass without Newton's third law, pair_eam.cpp:688-745, gathering
@@ -476,7 +500,7 @@ __global__ __launch_bounds__(kAtomWG) void k_eam_atom_force(int n, const double 
                                                             const int *__restrict__ num, Tabs<P> t, int m0,
                                                             double *__restrict__ f, double *__restrict__ eatom,
                                                             double *__restrict__ vatom,
-                                                            const uint32_t *__restrict__ mask) {
+                                                            const uint32_t *__restrict__ mask, int per) {
   extern __shared__ __align__(16) unsigned char smem[];
   const Win<P> w = make_win<P>(smem, m0);
   if (t.rk)
@@ -484,54 +508,55 @@ __global__ __launch_bounds__(kAtomWG) void k_eam_atom_force(int n, const double 
   else
     load_window<P>(t.rz, 8, m0, t.nr, reinterpret_cast<P *>(smem));
   __syncthreads();
-  const int i = xcd_block() * kAtomWG + (int)threadIdx.x;
-  if (i >= n) return;
-  double fx = 0.0, fy = 0.0, fz = 0.0, ep = 0.0;
-  Virial vir;
-  vir.zero();
-  const double xi = x4[4 * (size_t)i], yi = x4[4 * (size_t)i + 1], zi = x4[4 * (size_t)i + 2];
-  const P fpi = (P)x4[4 * (size_t)i + 3];
-  const int jn = num[i];
-  const uint32_t *row = neigh + atom_slot(i, 0, M);
-  const int W = (M + 31) >> 5;
-  uint32_t bits = 0;
-  for (int k0 = 0; k0 < jn; k0 += kUA) {
-    if ((k0 & 31) == 0) bits = mask[mask_slot(i, k0 >> 5, W)];
-    const uint32_t sel = bits >> (k0 & 31);
-    uint32_t e[kUA];
-    fetch_atom_entries<kUA>(row, k0, jn, e);
-#pragma unroll
-    for (int u = 0; u < kUA; ++u)
-      if (!((sel >> u) & 1u)) e[u] = kNoEntry;
-    Pt xjv[kUA];
-    double fpjv[kUA];
-#pragma unroll
-    for (int u = 0; u < kUA; ++u)
-      if (e[u] != kNoEntry) xjv[u] = load_xyzw(x4, e[u], prd, &fpjv[u]);
-#pragma unroll
-    for (int u = 0; u < kUA; ++u) {
-      if (e[u] == kNoEntry) continue;
-      const double delx = xi - xjv[u].x, dely = yi - xjv[u].y, delz = zi - xjv[u].z;
-      const double rsq = delx * delx + dely * dely + delz * delz;
-      if (rsq < t.cutforcesq) {
-        P phi;
-        const double fpair = eam_fpair<P>(t, w, rsq, fpi, (P)fpjv[u], &phi);
-        fx += delx * fpair;
-        fy += dely * fpair;
-        fz += delz * fpair;
-        if (EV) {
-          ep += 0.5 * (double)phi;
-          vir.add(delx, dely, delz, fpair);
+  const int i0 = xcd_block() * per, i1 = min(n, i0 + per);
+  for (int i = i0 + (int)threadIdx.x; i < i1; i += kAtomWG) {
+    double fx = 0.0, fy = 0.0, fz = 0.0, ep = 0.0;
+    Virial vir;
+    vir.zero();
+    const double xi = x4[4 * (size_t)i], yi = x4[4 * (size_t)i + 1], zi = x4[4 * (size_t)i + 2];
+    const P fpi = (P)x4[4 * (size_t)i + 3];
+    const int jn = num[i];
+    const uint32_t *row = neigh + atom_slot(i, 0, M);
+    const int W = (M + 31) >> 5;
+    uint32_t bits = 0;
+    for (int k0 = 0; k0 < jn; k0 += kUA) {
+      if ((k0 & 31) == 0) bits = mask[mask_slot(i, k0 >> 5, W)];
+      const uint32_t sel = bits >> (k0 & 31);
+      uint32_t e[kUA];
+      fetch_atom_entries<kUA>(row, k0, jn, e);
+  #pragma unroll
+      for (int u = 0; u < kUA; ++u)
+        if (!((sel >> u) & 1u)) e[u] = kNoEntry;
+      Pt xjv[kUA];
+      double fpjv[kUA];
+  #pragma unroll
+      for (int u = 0; u < kUA; ++u)
+        if (e[u] != kNoEntry) xjv[u] = load_xyzw(x4, e[u], prd, &fpjv[u]);
+  #pragma unroll
+      for (int u = 0; u < kUA; ++u) {
+        if (e[u] == kNoEntry) continue;
+        const double delx = xi - xjv[u].x, dely = yi - xjv[u].y, delz = zi - xjv[u].z;
+        const double rsq = delx * delx + dely * dely + delz * delz;
+        if (rsq < t.cutforcesq) {
+          P phi;
+          const double fpair = eam_fpair<P>(t, w, rsq, fpi, (P)fpjv[u], &phi);
+          fx += delx * fpair;
+          fy += dely * fpair;
+          fz += delz * fpair;
+          if (EV) {
+            ep += 0.5 * (double)phi;
+            vir.add(delx, dely, delz, fpair);
+          }
         }
       }
     }
-  }
-  f[i] = fx;
-  f[(size_t)n + i] = fy;
-  f[2 * (size_t)n + i] = fz;
-  if (EV) {
-    eatom[i] += ep;
-    vir.store(vatom, (size_t)n, (size_t)i, 1.0);
+    f[i] = fx;
+    f[(size_t)n + i] = fy;
+    f[2 * (size_t)n + i] = fz;
+    if (EV) {
+      eatom[i] += ep;
+      vir.store(vatom, (size_t)n, (size_t)i, 1.0);
+    }
   }
 }
 
@@ -1280,7 +1305,7 @@ TabWindow knot_window(const cac_ctx *ctx) {
 // The atom density pass (see k_eam_atom_rho): full or inner, as the device
 // flag says; the inner mask is then valid for the current positions.
 template <class P, bool EV>
-int atom_rho(cac_ctx *ctx, const Tabs<P> &t, unsigned agrid, const TabWindow &wr) {
+int atom_rho(cac_ctx *ctx, const Tabs<P> &t, AtomGrid agrid, const TabWindow &wr) {
   const double skin = ctx->cutneigh - sqrt(ctx->eam.cutforcesq);
   const double margin = 0.4 * (skin > 0.0 ? skin : 0.0);
   const double cin = sqrt(ctx->eam.cutforcesq) + margin;
@@ -1292,9 +1317,9 @@ int atom_rho(cac_ctx *ctx, const Tabs<P> &t, unsigned agrid, const TabWindow &wr
   }
   ctx->inner_margin = margin;
   ctx->inner_cut = cin;
-  hipLaunchKernelGGL((k_eam_atom_rho<P, EV>), dim3(agrid), dim3(kAtomWG), wr.bytes, ctx->stream, ctx->n, ctx->x4,
-                     prd_of(ctx), ctx->aneigh, ctx->maxneigh, ctx->anum, t, wr.m0, ctx->rho, ctx->eatom, ctx->amask,
-                     ibad, ctx->amask_in, ctx->xhold_in, cutinsq);
+  hipLaunchKernelGGL((k_eam_atom_rho<P, EV>), dim3(agrid.blocks), dim3(kAtomWG), wr.bytes, ctx->stream, ctx->n,
+                     ctx->x4, prd_of(ctx), ctx->aneigh, ctx->maxneigh, ctx->anum, t, wr.m0, ctx->rho, ctx->eatom,
+                     ctx->amask, ibad, ctx->amask_in, ctx->xhold_in, cutinsq, agrid.per);
   CAC_HIP(ctx, hipMemsetAsync(ibad, 0, sizeof(int), ctx->stream));
   return CAC_OK;
 }
@@ -1352,7 +1377,7 @@ template <class P, bool EV>
 int eam_clusters(cac_ctx *ctx, const ElemArgs &base) {
   Tabs<P> t = tabs_of<P>(ctx);
   const int n = ctx->n;
-  const unsigned agrid = xcd_grid((long)n, kAtomWG);
+  const AtomGrid agrid = atom_grid(ctx, n);
   const TabWindow wr = table_window(ctx, 4 * sizeof(P));
   const TabWindow wz = t.rk ? knot_window(ctx) : table_window(ctx, 8 * sizeof(P));
   hipEvent_t t0;
@@ -1387,9 +1412,9 @@ int eam_clusters(cac_ctx *ctx, const ElemArgs &base) {
   }
   if (n) {
     timer_begin(ctx, "atom_force", &t0);
-    hipLaunchKernelGGL((k_eam_atom_force<P, EV>), dim3(agrid), dim3(kAtomWG), wz.bytes, ctx->stream, n, ctx->x4,
-                       prd_of(ctx), ctx->aneigh, ctx->maxneigh, ctx->anum, t, wz.m0, ctx->f, ctx->eatom,
-                       ctx->vatom, ctx->amask);
+    hipLaunchKernelGGL((k_eam_atom_force<P, EV>), dim3(agrid.blocks), dim3(kAtomWG), wz.bytes, ctx->stream, n,
+                       ctx->x4, prd_of(ctx), ctx->aneigh, ctx->maxneigh, ctx->anum, t, wz.m0, ctx->f, ctx->eatom,
+                       ctx->vatom, ctx->amask, agrid.per);
     timer_end(ctx, "atom_force", t0);
   }
   timer_begin(ctx, "ip_force", &t0);
@@ -1412,7 +1437,7 @@ int eam_all(cac_ctx *ctx, const ElemArgs &base, const ElemArgs &spos, const Elem
   Tabs<P> t = tabs_of<P>(ctx);
   const int n = ctx->n;
   const bool el = ctx->ne > 0;
-  const unsigned agrid = xcd_grid((long)n, kAtomWG);
+  const AtomGrid agrid = atom_grid(ctx, n);
   const TabWindow wr = table_window(ctx, 4 * sizeof(P));
   const TabWindow wz = t.rk ? knot_window(ctx) : table_window(ctx, 8 * sizeof(P));
   hipEvent_t t0;
@@ -1443,9 +1468,9 @@ int eam_all(cac_ctx *ctx, const ElemArgs &base, const ElemArgs &spos, const Elem
   }
   if (n) {
     timer_begin(ctx, "atom_force", &t0);
-    hipLaunchKernelGGL((k_eam_atom_force<P, EV>), dim3(agrid), dim3(kAtomWG), wz.bytes, ctx->stream, n, ctx->x4,
-                       prd_of(ctx), ctx->aneigh, ctx->maxneigh, ctx->anum, t, wz.m0, ctx->f, ctx->eatom,
-                       ctx->vatom, ctx->amask);
+    hipLaunchKernelGGL((k_eam_atom_force<P, EV>), dim3(agrid.blocks), dim3(kAtomWG), wz.bytes, ctx->stream, n,
+                       ctx->x4, prd_of(ctx), ctx->aneigh, ctx->maxneigh, ctx->anum, t, wz.m0, ctx->f, ctx->eatom,
+                       ctx->vatom, ctx->amask, agrid.per);
     timer_end(ctx, "atom_force", t0);
   }
   const long nf = el ? host_nelem(force_el) : 0;
