@@ -110,11 +110,11 @@ __device__ __forceinline__ void raise_flag(int *flag) {
   __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-__global__ void k_nve_atoms_step(int n, double *__restrict__ x4, double *__restrict__ v,
-                                 const double *__restrict__ f, double dtv, double dtfm, int fin,
-                                 double *__restrict__ xhold, double triggersq, int *flag, int remap, Scale3 m,
-                                 double *__restrict__ xhold_in, double thr_in, int *__restrict__ ibad) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void nve_atom_step(int i, int n, double *__restrict__ x4, double *__restrict__ v,
+                                              const double *__restrict__ f, double dtv, double dtfm, int fin,
+                                              double *__restrict__ xhold, double triggersq, int *flag, int remap,
+                                              const Scale3 &m, double *__restrict__ xhold_in, double thr_in,
+                                              int *__restrict__ ibad) {
   if (i >= n) return;
   double dsq = 0.0, din = 0.0;
   for (int d = 0; d < 3; ++d) {
@@ -148,11 +148,11 @@ __global__ void k_nve_atoms_step(int n, double *__restrict__ x4, double *__restr
 
 // Nodes: 8 lanes per element (one per node, 24 contiguous bytes each); the
 // center is the node mean summed in node order as in k_nve_nodes_initial.
-__global__ void k_nve_nodes_step(int ne, double *__restrict__ nodex, double *__restrict__ nodev,
-                                 const double *__restrict__ nodef, double *__restrict__ xc, double dtv, double dtfm,
-                                 int fin, double *__restrict__ nodexhold, double triggersq, int *flag, int remap,
-                                 Scale3 m) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+// Wave-uniform (every lane of the wavefront calls it).
+__device__ __forceinline__ void nve_node_step(long t, int ne, double *__restrict__ nodex, double *__restrict__ nodev,
+                                              const double *__restrict__ nodef, double *__restrict__ xc, double dtv,
+                                              double dtfm, int fin, double *__restrict__ nodexhold, double triggersq,
+                                              int *flag, int remap, const Scale3 &m) {
   const int i = (int)(t >> 3), k = (int)(t & 7);
   const bool live = i < ne;
   const size_t b = 24 * (size_t)(live ? i : 0) + 3 * k;
@@ -183,6 +183,26 @@ __global__ void k_nve_nodes_step(int ne, double *__restrict__ nodex, double *__r
     if (live && k == 0) xc[3 * (size_t)i + d] = c / 8;
   }
   if (live && flag && dsq > triggersq) raise_flag(flag);
+}
+
+// One launch for both: blocks [0, ablocks) take atoms, the rest take nodes.
+struct NveStepArgs {
+  int n, ne, ablocks, fin, remap;
+  double *x4, *v, *xhold, *xhold_in, *nodex, *nodev, *xc, *nodexhold;
+  const double *f, *nodef;
+  double dtv, dtfm, dtfm_node, triggersq, thr_in;
+  int *flag, *ibad;
+  Scale3 m;
+};
+
+__global__ void k_nve_step(NveStepArgs a) {
+  if ((int)blockIdx.x < a.ablocks) {
+    nve_atom_step((int)(blockIdx.x * blockDim.x + threadIdx.x), a.n, a.x4, a.v, a.f, a.dtv, a.dtfm, a.fin, a.xhold,
+                  a.triggersq, a.flag, a.remap, a.m, a.xhold_in, a.thr_in, a.ibad);
+  } else {
+    nve_node_step((long)(blockIdx.x - a.ablocks) * blockDim.x + threadIdx.x, a.ne, a.nodex, a.nodev, a.nodef, a.xc,
+                  a.dtv, a.dtfm_node, a.fin, a.nodexhold, a.triggersq, a.flag, a.remap, a.m);
+  }
 }
 
 // Affine remap x -> lo + (x - lo)*scale of atoms, their held positions,
@@ -432,7 +452,7 @@ int ensure_step_sync(cac_ctx *ctx) {
   return CAC_OK;
 }
 
-// One step boundary of cac_run (see k_nve_atoms_step); fin: apply the
+// One step boundary of cac_run (k_nve_step); fin: apply the
 // previous step's final half-step first; remap: the step's affine remap.
 int nve_step_impl(cac_ctx *ctx, double dtv, double dtf, bool fin, int *flag, const Scale3 *remap = nullptr) {
   Scale3 m = {};
@@ -452,16 +472,33 @@ int nve_step_impl(cac_ctx *ctx, double dtv, double dtf, bool fin, int *flag, con
   }
   const bool inner = thr > 0.0 && ctx->xhold_in && ctx->b_xhold_in.bytes >= 3 * (size_t)ctx->n * sizeof(double);
   if (ctx->n && !inner) invalidate_inner(ctx);
-  if (ctx->n)
-    hipLaunchKernelGGL(k_nve_atoms_step, dim3(grid_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->n, ctx->x4,
-                       ctx->v, ctx->f, dtv, dtf / ctx->mass, fin ? 1 : 0, ctx->xhold, ctx->triggersq, flag,
-                       remap ? 1 : 0, m, inner ? ctx->xhold_in : nullptr, thr * thr, ctx->dflags + 14);
-  if (ctx->ne) {
-    double inv_npe = 1 / (double)8;
-    hipLaunchKernelGGL(k_nve_nodes_step, dim3(grid_for(8L * ctx->ne)), dim3(kBlock), 0, ctx->stream, ctx->ne,
-                       ctx->nodex, ctx->nodev, ctx->nodef, ctx->xc, dtv, dtf / (inv_npe * ctx->mass_e), fin ? 1 : 0,
-                       ctx->nodexhold, ctx->triggersq, flag, remap ? 1 : 0, m);
-  }
+  NveStepArgs a;
+  a.n = ctx->n;
+  a.ne = ctx->ne;
+  a.ablocks = (int)grid_for(ctx->n);
+  a.fin = fin ? 1 : 0;
+  a.remap = remap ? 1 : 0;
+  a.m = m;
+  a.x4 = ctx->x4;
+  a.v = ctx->v;
+  a.f = ctx->f;
+  a.xhold = ctx->xhold;
+  a.xhold_in = inner ? ctx->xhold_in : nullptr;
+  a.nodex = ctx->nodex;
+  a.nodev = ctx->nodev;
+  a.nodef = ctx->nodef;
+  a.xc = ctx->xc;
+  a.nodexhold = ctx->nodexhold;
+  a.dtv = dtv;
+  a.dtfm = ctx->n ? dtf / ctx->mass : 0.0;
+  double inv_npe = 1 / (double)8;
+  a.dtfm_node = ctx->ne ? dtf / (inv_npe * ctx->mass_e) : 0.0;
+  a.triggersq = ctx->triggersq;
+  a.thr_in = thr * thr;
+  a.flag = flag;
+  a.ibad = ctx->dflags + 14;
+  const unsigned blocks = (unsigned)a.ablocks + grid_for(8L * ctx->ne);
+  if (blocks) hipLaunchKernelGGL(k_nve_step, dim3(blocks), dim3(kBlock), 0, ctx->stream, a);
   CAC_HIP(ctx, hipGetLastError());
   return CAC_OK;
 }
@@ -598,7 +635,8 @@ int cac_ctx_destroy(cac_ctx *ctx) {
   surrogate_destroy(ctx);
   DevBuf *bufs[] = {&ctx->b_x4, &ctx->b_v, &ctx->b_f, &ctx->b_xhold, &ctx->b_rho, &ctx->b_eatom, &ctx->b_vatom,
                     &ctx->b_aneigh, &ctx->b_anum, &ctx->b_akey, &ctx->b_aval, &ctx->b_akey2, &ctx->b_aval2,
-                    &ctx->b_ahead, &ctx->b_acells, &ctx->b_acs, &ctx->b_ace, &ctx->b_sorttmp, &ctx->b_nodex, &ctx->b_nodev, &ctx->b_nodef,
+                    &ctx->b_ahead, &ctx->b_acells, &ctx->b_acs, &ctx->b_ace, &ctx->b_sorttmp, &ctx->b_nodex,
+                    &ctx->b_nodev, &ctx->b_nodef, &ctx->b_amask_in, &ctx->b_xhold_in,
                     &ctx->b_nodexhold, &ctx->b_xc, &ctx->b_spa_site, &ctx->b_spa_int, &ctx->b_weight,
                     &ctx->b_erad, &ctx->b_ehull, &ctx->b_eneigh, &ctx->b_enumb, &ctx->b_ekey, &ctx->b_eval, &ctx->b_ekey2,
                     &ctx->b_eval2, &ctx->b_ecs, &ctx->b_ece, &ctx->b_ipneigh, &ctx->b_ipnum, &ctx->b_erho,

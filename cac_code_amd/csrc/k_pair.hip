@@ -500,8 +500,12 @@ __global__ __launch_bounds__(kAtomWG) void k_eam_atom_force(int n, const double 
                                                             const int *__restrict__ num, Tabs<P> t, int m0,
                                                             double *__restrict__ f, double *__restrict__ eatom,
                                                             double *__restrict__ vatom,
-                                                            const uint32_t *__restrict__ mask, int per) {
+                                                            const uint32_t *__restrict__ mask, int per,
+                                                            int *__restrict__ ibad) {
   extern __shared__ __align__(16) unsigned char smem[];
+  // the density pass has read the inner-mask flag: its inner mask now holds
+  // for these positions (see k_eam_atom_rho)
+  if (blockIdx.x == 0 && threadIdx.x == 0) *ibad = 0;
   const Win<P> w = make_win<P>(smem, m0);
   if (t.rk)
     load_window<double>(t.rk, 4, m0, t.nr + 1, reinterpret_cast<double *>(smem));
@@ -1303,7 +1307,8 @@ TabWindow knot_window(const cac_ctx *ctx) {
 }
 
 // The atom density pass (see k_eam_atom_rho): full or inner, as the device
-// flag says; the inner mask is then valid for the current positions.
+// flag says; the inner mask is then valid for the current positions, and
+// the atom force pass that follows clears the flag.
 template <class P, bool EV>
 int atom_rho(cac_ctx *ctx, const Tabs<P> &t, AtomGrid agrid, const TabWindow &wr) {
   const double skin = ctx->cutneigh - sqrt(ctx->eam.cutforcesq);
@@ -1320,7 +1325,6 @@ int atom_rho(cac_ctx *ctx, const Tabs<P> &t, AtomGrid agrid, const TabWindow &wr
   hipLaunchKernelGGL((k_eam_atom_rho<P, EV>), dim3(agrid.blocks), dim3(kAtomWG), wr.bytes, ctx->stream, ctx->n,
                      ctx->x4, prd_of(ctx), ctx->aneigh, ctx->maxneigh, ctx->anum, t, wr.m0, ctx->rho, ctx->eatom,
                      ctx->amask, ibad, ctx->amask_in, ctx->xhold_in, cutinsq, agrid.per);
-  CAC_HIP(ctx, hipMemsetAsync(ibad, 0, sizeof(int), ctx->stream));
   return CAC_OK;
 }
 
@@ -1414,7 +1418,7 @@ int eam_clusters(cac_ctx *ctx, const ElemArgs &base) {
     timer_begin(ctx, "atom_force", &t0);
     hipLaunchKernelGGL((k_eam_atom_force<P, EV>), dim3(agrid.blocks), dim3(kAtomWG), wz.bytes, ctx->stream, n,
                        ctx->x4, prd_of(ctx), ctx->aneigh, ctx->maxneigh, ctx->anum, t, wz.m0, ctx->f, ctx->eatom,
-                       ctx->vatom, ctx->amask, agrid.per);
+                       ctx->vatom, ctx->amask, agrid.per, ctx->dflags + 14);
     timer_end(ctx, "atom_force", t0);
   }
   timer_begin(ctx, "ip_force", &t0);
@@ -1470,7 +1474,7 @@ int eam_all(cac_ctx *ctx, const ElemArgs &base, const ElemArgs &spos, const Elem
     timer_begin(ctx, "atom_force", &t0);
     hipLaunchKernelGGL((k_eam_atom_force<P, EV>), dim3(agrid.blocks), dim3(kAtomWG), wz.bytes, ctx->stream, n,
                        ctx->x4, prd_of(ctx), ctx->aneigh, ctx->maxneigh, ctx->anum, t, wz.m0, ctx->f, ctx->eatom,
-                       ctx->vatom, ctx->amask, agrid.per);
+                       ctx->vatom, ctx->amask, agrid.per, ctx->dflags + 14);
     timer_end(ctx, "atom_force", t0);
   }
   const long nf = el ? host_nelem(force_el) : 0;
