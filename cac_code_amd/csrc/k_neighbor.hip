@@ -1196,16 +1196,25 @@ __device__ __forceinline__ uint32_t particle_entry(uint64_t e, int n, int nwords
 }
 
 // Integration-point lists: raw entries -> particle entries in the blocked
-// layout.  One thread per row.
+// layout.  One wavefront per group of 8 rows: lane q of a pass takes entry
+// q/8 of row q%8, so the writes are contiguous and the reads 8 runs.
 __global__ void k_ip_translate(int nip, int maxnip, const uint64_t *__restrict__ tmp, const int *__restrict__ ipnum,
                                int n, int nwords, const uint32_t *__restrict__ bitmap,
                                const int *__restrict__ wprefix, uint32_t *__restrict__ out) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= nip) return;
-  const int c = ipnum[p] < maxnip ? ipnum[p] : maxnip;
-  const uint64_t *row = tmp + (size_t)p * maxnip;
-  for (int k = 0; k < c; ++k)
-    out[((size_t)(p >> 3) * maxnip + k) * kRowGroup + (p & 7)] = particle_entry(row[k], n, nwords, bitmap, wprefix);
+  const int g = (int)((blockIdx.x * (long)blockDim.x + threadIdx.x) >> 6), lane = threadIdx.x & 63;
+  if (g * kRowGroup >= nip) return;  // whole wavefront
+  const int pr = g * kRowGroup + (lane & 7);
+  int c = lane < kRowGroup && pr < nip ? min(ipnum[pr], maxnip) : 0;
+  int kmax = c;
+  for (int o = 1; o < kRowGroup; o <<= 1) kmax = max(kmax, __shfl_xor(kmax, o));
+  kmax = __shfl(kmax, 0);
+  c = __shfl(c, lane & 7);  // the count of this lane's row
+  for (int q = lane; q < kRowGroup * kmax; q += 64) {
+    const int k = q >> 3;
+    if (k < c)
+      out[((size_t)g * maxnip + k) * kRowGroup + (q & 7)] =
+          particle_entry(tmp[(size_t)pr * maxnip + k], n, nwords, bitmap, wprefix);
+  }
 }
 
 // Coupled atom lists: the raw site entries of flagged atom r go after the
@@ -1633,7 +1642,8 @@ int launch_active_sites(cac_ctx *ctx) {
   ctx->xs4 = ctx->x4 + 4 * (size_t)n;
   hipLaunchKernelGGL(k_active_finalize, dim3(grid_for(64L * ne)), dim3(kBlock), 0, ctx->stream, ne, nw, ctx->bitmap,
                      estart, ctx->wprefix, ctx->asite);
-  hipLaunchKernelGGL(k_ip_translate, dim3(grid_for(nip)), dim3(kBlock), 0, ctx->stream, nip, ctx->maxnip, ctx->iptmp,
+  hipLaunchKernelGGL(k_ip_translate, dim3(grid_for(64L * ((nip + kRowGroup - 1) / kRowGroup))), dim3(kBlock), 0,
+                     ctx->stream, nip, ctx->maxnip, ctx->iptmp,
                      ctx->ipnum, n, nw, ctx->bitmap, ctx->wprefix, ctx->ipneigh);
   if (ctx->niface > 0) {
     hipLaunchKernelGGL(k_atom_site_translate, dim3(grid_for(ctx->niface)), dim3(kBlock), 0, ctx->stream, ctx->niface,
