@@ -2,9 +2,12 @@
 // points, and the integration-point -> node scatter.
 //
 // FP64 path: the reference's arithmetic term by term (pair_eam.cpp:584-746,
-// 927-1137; pair_lj_cut.cpp:227-288, 295-442), compiled with
-// -ffp-contract=off.  Results differ from the CPU oracle only through the
-// order of the neighbor sums.
+// 927-1137; pair_lj_cut.cpp:227-288, 295-442).  Unlike the neighbor build,
+// this file lets the compiler fuse multiplies and adds (fp contract fast;
+// the force pass's VALU work falls by about a tenth), except where bits must
+// match a stored table (the knot-form spline coefficients).  Results differ
+// from the CPU oracle through the order of the neighbor sums and those
+// fused roundings, both far below the 1e-10 parity bar.
 // MIXED path (P = float): r, the spline polynomials, F'(rho) and the pair
 // force are evaluated in FP32; positions, site interpolation, rho, force,
 // energy and virial sums stay FP64.
@@ -32,6 +35,8 @@
 #include <type_traits>
 
 #include "cac_internal.h"
+
+#pragma clang fp contract(fast)
 
 namespace cac {
 namespace {
@@ -186,9 +191,15 @@ __device__ __forceinline__ void rz_of_r_knots(const Tabs<double> &t, const Win<d
 #pragma unroll
     for (int c = 0; c < 8; ++c) k[c] = q[c];
   }
-  const double dr = k[4] - k[0], dz = k[6] - k[2];
-  const double r4 = 3.0 * dr - 2.0 * k[1] - k[5], r3 = k[1] + k[5] - 2.0 * dr;
-  const double z4 = 3.0 * dz - 2.0 * k[3] - k[7], z3 = k[3] + k[7] - 2.0 * dz;
+  double r4, r3, z4, z3;
+  {  // array2spline's operations, unfused: the stored c3, c4 bit for bit
+#pragma clang fp contract(off)
+    const double dr = k[4] - k[0], dz = k[6] - k[2];
+    r4 = 3.0 * dr - 2.0 * k[1] - k[5];
+    r3 = k[1] + k[5] - 2.0 * dr;
+    z4 = 3.0 * dz - 2.0 * k[3] - k[7];
+    z3 = k[3] + k[7] - 2.0 * dz;
+  }
   *rhoip = ((3.0 * r3 * p + 2.0 * r4) * p + k[1]) * t.rdr;
   *z2p = ((3.0 * z3 * p + 2.0 * z4) * p + k[3]) * t.rdr;
   *z2 = ((z3 * p + z4) * p + k[3]) * p + k[2];
