@@ -222,10 +222,13 @@ struct cac_ctx {
   int *cl_nu = nullptr;
   uint16_t *iplocal = nullptr;
   int cl_stride = 0, cl_umax = 0;
-  // launch order: clusters with unions of at most kUSmall entries first
-  // (cl_nsmall of them, largest cl_umax_small), then the rest
-  int *cl_order = nullptr;
-  int cl_nsmall = 0, cl_umax_small = 0;
+  // launch order of the cluster passes: the clusters with unions of more
+  // than kUSmall entries (cl_nbig, rows in cl_scratch slot cl_big[cl]) at the
+  // start of every XCD's share, the rest (largest union cl_umax_small) after
+  int *cl_order = nullptr, *cl_big = nullptr;
+  int cl_nbig = 0, cl_umax_small = 0;
+  double *cl_scratch = nullptr;
+  cac::DevBuf b_cl_big, b_cl_scratch;
   bool cl_ok = false;
   int naref = 0;
   int *aref = nullptr, *amark = nullptr;

@@ -1864,22 +1864,40 @@ int launch_clusters(cac_ctx *ctx) {
   // LDS rows of 32 bytes per union entry, one wavefront per cluster: past
   // 48 KB a CU would hold too few wavefronts to hide the gathers
   if ((size_t)fl[1] * 32 > 48 * 1024) return CAC_OK;
-  // launch order: small unions first (see kUSmall)
-  std::vector<int> nu(ctx->ncl), order;
-  order.reserve(ctx->ncl);
+  // launch order (one launch per pass, LDS rows for unions of at most
+  // kUSmall entries): the larger unions, whose rows live in global scratch,
+  // go to the start of the 8 XCDs' shares of the grid (xcd_block() gives
+  // XCD x the logical blocks [x*q, (x+1)*q) in order), so each starts early
+  // and runs beside the small ones; the small unions fill the rest in order.
+  std::vector<int> nu(ctx->ncl), big(ctx->ncl, -1), order(ctx->ncl, -1);
   CAC_HIP(ctx, hipMemcpy(nu.data(), ctx->cl_nu, (size_t)ctx->ncl * sizeof(int), hipMemcpyDeviceToHost));
-  int small_max = 0;
-  for (int c = 0; c < ctx->ncl; ++c)
+  int small_max = 0, nbig = 0;
+  const int q = (ctx->ncl + 7) / 8;
+  for (int c = 0; c < ctx->ncl; ++c) {
     if (nu[c] <= kUSmall) {
-      order.push_back(c);
       small_max = std::max(small_max, nu[c]);
+      continue;
     }
-  ctx->cl_nsmall = (int)order.size();
-  for (int c = 0; c < ctx->ncl; ++c)
-    if (nu[c] > kUSmall) order.push_back(c);
+    int pos = (nbig % 8) * q + nbig / 8;
+    if (pos >= ctx->ncl || order[pos] >= 0) pos = 0;
+    while (order[pos] >= 0) ++pos;  // a free slot exists: fewer placed than ncl
+    order[pos] = c;
+    big[c] = nbig++;
+  }
+  for (int c = 0, pos = 0; c < ctx->ncl; ++c) {
+    if (nu[c] > kUSmall) continue;
+    while (order[pos] >= 0) ++pos;
+    order[pos] = c;
+  }
+  ctx->cl_nbig = nbig;
   ctx->cl_umax_small = small_max;
-  if ((rc = ensure(ctx, ctx->b_cl_order, (size_t)ctx->ncl, &ctx->cl_order))) return rc;
+  const size_t sstride = (size_t)4 * ctx->cl_umax + kClMax * (24 + 8);
+  if ((rc = ensure(ctx, ctx->b_cl_order, (size_t)ctx->ncl, &ctx->cl_order)) ||
+      (rc = ensure(ctx, ctx->b_cl_big, (size_t)ctx->ncl, &ctx->cl_big)) ||
+      (rc = ensure(ctx, ctx->b_cl_scratch, sstride * (nbig > 0 ? nbig : 1), &ctx->cl_scratch)))
+    return rc;
   CAC_HIP(ctx, hipMemcpy(ctx->cl_order, order.data(), (size_t)ctx->ncl * sizeof(int), hipMemcpyHostToDevice));
+  CAC_HIP(ctx, hipMemcpy(ctx->cl_big, big.data(), (size_t)ctx->ncl * sizeof(int), hipMemcpyHostToDevice));
   ctx->cl_ok = true;
   return CAC_OK;
 }

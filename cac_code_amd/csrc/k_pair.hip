@@ -861,7 +861,14 @@ struct ClArgs {
   const uint16_t *local;   // ipneigh layout
   const int *order;        // clusters of this launch
   int count;
-  int rows;                // LDS rows of this launch (its largest union)
+  int rows;                // LDS rows of this launch (its largest small union)
+  // clusters whose union exceeds the LDS rows keep their rows in global
+  // memory instead: big[cl] = slot in scratch (-1: LDS), slot stride sstride
+  // doubles, srows rows
+  const int *big;
+  double *scratch;
+  size_t sstride;
+  int srows;
 };
 
 // Shape functions of site s, computed as the host computes the site table
@@ -889,8 +896,8 @@ __device__ __forceinline__ void site_shape(const ElemArgs &a, int s, double N[8]
 template <class P, bool FP>
 __device__ __forceinline__ void cl_fill(const ClArgs &c, int cl, int s0, int ns, const ElemArgs &a, const Prd &prd,
                                         const Tabs<P> &t, const double *__restrict__ x4,
-                                        const double *__restrict__ erho, double *rows, int lane) {
-  double *nodes = rows + 4 * c.rows, *enode = nodes + kClMax * 24;
+                                        const double *__restrict__ erho, double *rows, int cap, int lane) {
+  double *nodes = rows + 4 * cap, *enode = nodes + kClMax * 24;
   int el[kClMax];
 #pragma unroll
   for (int i = 0; i < kClMax; ++i) el[i] = i < ns ? (c.slot[s0 + i] >> 3) : -1;
@@ -950,17 +957,14 @@ __device__ __forceinline__ void cl_fill(const ClArgs &c, int cl, int s0, int ns,
   }
 }
 
+// rows: the cluster's union rows, in LDS or (a large union) in global
+// memory; cap: their capacity.  Inlined once per memory space.
 template <class P>
-__global__ __launch_bounds__(64) void k_cl_erho(ClArgs c, ElemArgs a, Prd prd, Tabs<P> t,
-                                                const double *__restrict__ px4, double *__restrict__ erho,
-                                                uint32_t *__restrict__ mask) {
-  extern __shared__ __align__(16) double rows[];
-  const int o = xcd_block();
-  if (o >= c.count) return;
-  const int cl = c.order[o];
-  const int lane = (int)threadIdx.x;
+__device__ __forceinline__ void cl_erho_body(const ClArgs &c, const ElemArgs &a, const Prd &prd, const Tabs<P> &t,
+                                             const double *__restrict__ px4, double *__restrict__ erho,
+                                             uint32_t *__restrict__ mask, double *rows, int cap, int cl, int lane) {
   const int s0 = c.start[cl], ns = c.start[cl + 1] - s0;
-  cl_fill<P, false>(c, cl, s0, ns, a, prd, t, px4, nullptr, rows, lane);
+  cl_fill<P, false>(c, cl, s0, ns, a, prd, t, px4, nullptr, rows, cap, lane);
   __syncthreads();
   const int gi = lane >> 3, l = lane & 7;
   double acc = 0.0;
@@ -1004,18 +1008,30 @@ __global__ __launch_bounds__(64) void k_cl_erho(ClArgs c, ElemArgs a, Prd prd, T
   if (gi < ns && l == 0) erho[slot] = acc;
 }
 
-template <class P, bool EV>
-__global__ __launch_bounds__(64) void k_cl_force(ClArgs c, ElemArgs a, Prd prd, Tabs<P> t,
-                                                 const double *__restrict__ erho, const double *__restrict__ px4,
-                                                 double *__restrict__ ipf, double *__restrict__ eip,
-                                                 double *__restrict__ vip, const uint32_t *__restrict__ mask) {
-  extern __shared__ __align__(16) double rows[];
+template <class P>
+__global__ __launch_bounds__(64) void k_cl_erho(ClArgs c, ElemArgs a, Prd prd, Tabs<P> t,
+                                                const double *__restrict__ px4, double *__restrict__ erho,
+                                                uint32_t *__restrict__ mask) {
+  extern __shared__ __align__(16) double lds_rows[];
   const int o = xcd_block();
-  if (o >= c.count) return;
+  if (o >= c.count) return;  // whole workgroup
   const int cl = c.order[o];
-  const int lane = (int)threadIdx.x;
+  const int big = c.big[cl];
+  if (big >= 0)
+    cl_erho_body<P>(c, a, prd, t, px4, erho, mask, c.scratch + (size_t)big * c.sstride, c.srows, cl,
+                    (int)threadIdx.x);
+  else
+    cl_erho_body<P>(c, a, prd, t, px4, erho, mask, lds_rows, c.rows, cl, (int)threadIdx.x);
+}
+
+template <class P, bool EV>
+__device__ __forceinline__ void cl_force_body(const ClArgs &c, const ElemArgs &a, const Prd &prd, const Tabs<P> &t,
+                                              const double *__restrict__ erho, const double *__restrict__ px4,
+                                              double *__restrict__ ipf, double *__restrict__ eip,
+                                              double *__restrict__ vip, const uint32_t *__restrict__ mask,
+                                              double *rows, int cap, int cl, int lane) {
   const int s0 = c.start[cl], ns = c.start[cl + 1] - s0;
-  cl_fill<P, true>(c, cl, s0, ns, a, prd, t, px4, erho, rows, lane);
+  cl_fill<P, true>(c, cl, s0, ns, a, prd, t, px4, erho, rows, cap, lane);
   __syncthreads();
   const int gi = lane >> 3, l = lane & 7;
   double fx = 0.0, fy = 0.0, fz = 0.0, ep = 0.0, rhoi = 0.0;
@@ -1094,6 +1110,23 @@ __global__ __launch_bounds__(64) void k_cl_force(ClArgs c, ElemArgs a, Prd prd, 
       vir.store(vip, (size_t)a.nip, (size_t)p, w);
     }
   }
+}
+
+template <class P, bool EV>
+__global__ __launch_bounds__(64) void k_cl_force(ClArgs c, ElemArgs a, Prd prd, Tabs<P> t,
+                                                 const double *__restrict__ erho, const double *__restrict__ px4,
+                                                 double *__restrict__ ipf, double *__restrict__ eip,
+                                                 double *__restrict__ vip, const uint32_t *__restrict__ mask) {
+  extern __shared__ __align__(16) double lds_rows[];
+  const int o = xcd_block();
+  if (o >= c.count) return;  // whole workgroup
+  const int cl = c.order[o];
+  const int big = c.big[cl];
+  if (big >= 0)
+    cl_force_body<P, EV>(c, a, prd, t, erho, px4, ipf, eip, vip, mask, c.scratch + (size_t)big * c.sstride,
+                         c.srows, cl, (int)threadIdx.x);
+  else
+    cl_force_body<P, EV>(c, a, prd, t, erho, px4, ipf, eip, vip, mask, lds_rows, c.rows, cl, (int)threadIdx.x);
 }
 
 // LJ integration-point force, pair_lj_cut.cpp:355-422.
@@ -1370,17 +1403,17 @@ ClArgs cl_args(const cac_ctx *ctx) {
   c.uni = ctx->cl_union;
   c.nu = ctx->cl_nu;
   c.local = ctx->iplocal;
-  return c;
-}
-
-// The two launches of a cluster pass: clusters with small unions (most of
-// them) and the rest, each with the LDS rows its largest union needs, so
-// the few large unions at interfaces do not lower the occupancy of all.
-ClArgs cl_part(const cac_ctx *ctx, ClArgs c, int part) {
-  c.order = ctx->cl_order + (part ? ctx->cl_nsmall : 0);
-  c.count = part ? ctx->ncl - ctx->cl_nsmall : ctx->cl_nsmall;
-  c.rows = part ? ctx->cl_umax : ctx->cl_umax_small;
-  if (c.rows < 1) c.rows = 1;
+  // one launch: LDS rows for the small unions (most of them), so the few
+  // large unions at interfaces do not lower the occupancy of all; those keep
+  // their rows in global memory and start first on every XCD (see
+  // launch_clusters), running beside the rest
+  c.order = ctx->cl_order;
+  c.count = ctx->ncl;
+  c.rows = ctx->cl_umax_small > 0 ? ctx->cl_umax_small : 1;
+  c.big = ctx->cl_big;
+  c.scratch = ctx->cl_scratch;
+  c.srows = ctx->cl_umax;
+  c.sstride = (size_t)4 * ctx->cl_umax + kClMax * (24 + 8);
   return c;
 }
 size_t cl_lds(const ClArgs &c) { return ((size_t)4 * c.rows + kClMax * (24 + 8)) * sizeof(double); }
@@ -1412,12 +1445,9 @@ int eam_clusters(cac_ctx *ctx, const ElemArgs &base) {
     timer_end(ctx, "atom_rho", t0);
   }
   timer_begin(ctx, "ip_erho", &t0);
-  for (int part = 0; part < 2; ++part) {
-    const ClArgs cp = cl_part(ctx, c, part);
-    if (cp.count)
-      hipLaunchKernelGGL((k_cl_erho<P>), dim3(xcd_grid((long)cp.count * 64, 64)), dim3(64), cl_lds(cp), ctx->stream,
-                         cp, base, prd_of(ctx), t, ctx->x4, ctx->erho, ctx->ipmask);
-  }
+  if (c.count)
+    hipLaunchKernelGGL((k_cl_erho<P>), dim3(xcd_grid((long)c.count * 64, 64)), dim3(64), cl_lds(c), ctx->stream, c,
+                       base, prd_of(ctx), t, ctx->x4, ctx->erho, ctx->ipmask);
   timer_end(ctx, "ip_erho", t0);
   if (ctx->naref) {
     timer_begin(ctx, "site_fp", &t0);
@@ -1433,13 +1463,9 @@ int eam_clusters(cac_ctx *ctx, const ElemArgs &base) {
     timer_end(ctx, "atom_force", t0);
   }
   timer_begin(ctx, "ip_force", &t0);
-  for (int part = 0; part < 2; ++part) {
-    const ClArgs cp = cl_part(ctx, c, part);
-    if (cp.count)
-      hipLaunchKernelGGL((k_cl_force<P, EV>), dim3(xcd_grid((long)cp.count * 64, 64)), dim3(64), cl_lds(cp),
-                         ctx->stream, cp, base, prd_of(ctx), t, ctx->erho, ctx->x4, ctx->ipf, ctx->eip, ctx->vip,
-                         ctx->ipmask);
-  }
+  if (c.count)
+    hipLaunchKernelGGL((k_cl_force<P, EV>), dim3(xcd_grid((long)c.count * 64, 64)), dim3(64), cl_lds(c), ctx->stream,
+                       c, base, prd_of(ctx), t, ctx->erho, ctx->x4, ctx->ipf, ctx->eip, ctx->vip, ctx->ipmask);
   timer_end(ctx, "ip_force", t0);
   scatter(ctx, base);
   CAC_HIP(ctx, hipGetLastError());
