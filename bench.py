@@ -45,9 +45,8 @@ B_ELEM = 7640
 # HBM bytes per launch of the FP64 kernels on the eam_c1 workload, from
 # rocprofv3 FETCH_SIZE and WRITE_SIZE runs scaled by a plain 1 GiB copy's
 # counts (fetch x2.000, write x1.000); the summary is profiles/eam_c1_kernels.md.
-# The cluster passes (ip_erho, ip_force) are two launches per call: the sum.
-TRAFFIC = {"atom_force": 187.8e6 + 11.9e6, "atom_rho": 190.1e6 + 26.0e6, "ip_force": 2 * (71.3e6 + 4.5e6),
-           "ip_erho": 2 * (63.1e6 + 4.7e6)}
+TRAFFIC = {"atom_force": 187.8e6 + 11.9e6, "atom_rho": 190.1e6 + 26.0e6, "ip_force": 147.2e6 + 11.8e6,
+           "ip_erho": 131.1e6 + 12.2e6}
 
 
 def log(msg):
