@@ -1029,7 +1029,8 @@ __device__ __forceinline__ void cl_force_body(const ClArgs &c, const ElemArgs &a
                                               const double *__restrict__ erho, const double *__restrict__ px4,
                                               double *__restrict__ ipf, double *__restrict__ eip,
                                               double *__restrict__ vip, const uint32_t *__restrict__ mask,
-                                              double *rows, int cap, int cl, int lane) {
+                                              double *__restrict__ nodef, double *rows, int cap, int cl,
+                                              int lane) {
   const int s0 = c.start[cl], ns = c.start[cl + 1] - s0;
   cl_fill<P, true>(c, cl, s0, ns, a, prd, t, px4, erho, rows, cap, lane);
   __syncthreads();
@@ -1038,9 +1039,9 @@ __device__ __forceinline__ void cl_force_body(const ClArgs &c, const ElemArgs &a
   P fpi = P(0);
   Virial vir;
   vir.zero();
-  int p = 0;
+  int p = 0, slot = 0;
   if (gi < ns) {
-    const int slot = c.slot[s0 + gi];
+    slot = c.slot[s0 + gi];
     const int i = slot >> 3, g = a.node_list[slot & 7];
     p = i * a.nint + g;
     const double *Ng = a.spa_int + 8 * g;
@@ -1104,6 +1105,13 @@ __device__ __forceinline__ void cl_force_body(const ClArgs &c, const ElemArgs &a
     ipf[p] = fx;
     ipf[(size_t)a.nip + p] = fy;
     ipf[2 * (size_t)a.nip + p] = fz;
+    // the scatter of this node (every integration point on a node, shape
+    // rows exact unit vectors): k_scatter's sum has this one nonzero term
+    const int g = p % a.nint, k = slot & 7;
+    const double wn = a.weight[g] * a.spa_int[8 * g + k];
+    nodef[3 * (size_t)slot] = wn * fx;
+    nodef[3 * (size_t)slot + 1] = wn * fy;
+    nodef[3 * (size_t)slot + 2] = wn * fz;
     if (EV) {
       const double w = a.weight[p % a.nint];
       eip[p] = w * (embed_energy<P>(t, rhoi, (double)fpi) + ep);
@@ -1116,17 +1124,19 @@ template <class P, bool EV>
 __global__ __launch_bounds__(64) void k_cl_force(ClArgs c, ElemArgs a, Prd prd, Tabs<P> t,
                                                  const double *__restrict__ erho, const double *__restrict__ px4,
                                                  double *__restrict__ ipf, double *__restrict__ eip,
-                                                 double *__restrict__ vip, const uint32_t *__restrict__ mask) {
+                                                 double *__restrict__ vip, const uint32_t *__restrict__ mask,
+                                                 double *__restrict__ nodef) {
   extern __shared__ __align__(16) double lds_rows[];
   const int o = xcd_block();
   if (o >= c.count) return;  // whole workgroup
   const int cl = c.order[o];
   const int big = c.big[cl];
   if (big >= 0)
-    cl_force_body<P, EV>(c, a, prd, t, erho, px4, ipf, eip, vip, mask, c.scratch + (size_t)big * c.sstride,
+    cl_force_body<P, EV>(c, a, prd, t, erho, px4, ipf, eip, vip, mask, nodef, c.scratch + (size_t)big * c.sstride,
                          c.srows, cl, (int)threadIdx.x);
   else
-    cl_force_body<P, EV>(c, a, prd, t, erho, px4, ipf, eip, vip, mask, lds_rows, c.rows, cl, (int)threadIdx.x);
+    cl_force_body<P, EV>(c, a, prd, t, erho, px4, ipf, eip, vip, mask, nodef, lds_rows, c.rows, cl,
+                         (int)threadIdx.x);
 }
 
 // LJ integration-point force, pair_lj_cut.cpp:355-422.
@@ -1463,11 +1473,13 @@ int eam_clusters(cac_ctx *ctx, const ElemArgs &base) {
     timer_end(ctx, "atom_force", t0);
   }
   timer_begin(ctx, "ip_force", &t0);
+  // also the node forces: every integration point is on a node (the clusters
+  // need that), so each writes its node's scatter term (no k_scatter launch)
   if (c.count)
     hipLaunchKernelGGL((k_cl_force<P, EV>), dim3(xcd_grid((long)c.count * 64, 64)), dim3(64), cl_lds(c), ctx->stream,
-                       c, base, prd_of(ctx), t, ctx->erho, ctx->x4, ctx->ipf, ctx->eip, ctx->vip, ctx->ipmask);
+                       c, base, prd_of(ctx), t, ctx->erho, ctx->x4, ctx->ipf, ctx->eip, ctx->vip, ctx->ipmask,
+                       ctx->nodef);
   timer_end(ctx, "ip_force", t0);
-  scatter(ctx, base);
   CAC_HIP(ctx, hipGetLastError());
   return CAC_OK;
 }
