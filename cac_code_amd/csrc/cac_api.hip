@@ -106,15 +106,19 @@ struct Scale3 {
   double lo[3], s[3];
 };
 
-__device__ __forceinline__ void raise_flag(int *flag) {
+// The flag goes to host memory; `abort` (device memory) tells the force
+// passes the host has already queued for this step that their results will
+// be discarded (a rebuild comes first), so they return at once.
+__device__ __forceinline__ void raise_flag(int *flag, int *abort) {
   __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ void nve_atom_step(int i, int n, double *__restrict__ x4, double *__restrict__ v,
                                               const double *__restrict__ f, double dtv, double dtfm, int fin,
-                                              double *__restrict__ xhold, double triggersq, int *flag, int remap,
-                                              const Scale3 &m, double *__restrict__ xhold_in, double thr_in,
-                                              int *__restrict__ ibad) {
+                                              double *__restrict__ xhold, double triggersq, int *flag, int *abort,
+                                              int remap, const Scale3 &m, double *__restrict__ xhold_in,
+                                              double thr_in, int *__restrict__ ibad) {
   if (i >= n) return;
   double dsq = 0.0, din = 0.0;
   for (int d = 0; d < 3; ++d) {
@@ -141,7 +145,7 @@ __device__ __forceinline__ void nve_atom_step(int i, int n, double *__restrict__
     const double dq = xx - xq;
     din += dq * dq;
   }
-  if (flag && dsq > triggersq) raise_flag(flag);
+  if (flag && dsq > triggersq) raise_flag(flag, abort);
   // the density pass's inner mask (see k_eam_atom_rho) needs motion below thr_in
   if (xhold_in && din > thr_in) __hip_atomic_store(ibad, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -152,7 +156,7 @@ __device__ __forceinline__ void nve_atom_step(int i, int n, double *__restrict__
 __device__ __forceinline__ void nve_node_step(long t, int ne, double *__restrict__ nodex, double *__restrict__ nodev,
                                               const double *__restrict__ nodef, double *__restrict__ xc, double dtv,
                                               double dtfm, int fin, double *__restrict__ nodexhold, double triggersq,
-                                              int *flag, int remap, const Scale3 &m) {
+                                              int *flag, int *abort, int remap, const Scale3 &m) {
   const int i = (int)(t >> 3), k = (int)(t & 7);
   const bool live = i < ne;
   const size_t b = 24 * (size_t)(live ? i : 0) + 3 * k;
@@ -182,7 +186,7 @@ __device__ __forceinline__ void nve_node_step(long t, int ne, double *__restrict
     for (int kk = 0; kk < 8; ++kk) c += __shfl(xn[d], kk, 8);
     if (live && k == 0) xc[3 * (size_t)i + d] = c / 8;
   }
-  if (live && flag && dsq > triggersq) raise_flag(flag);
+  if (live && flag && dsq > triggersq) raise_flag(flag, abort);
 }
 
 // One launch for both: blocks [0, ablocks) take atoms, the rest take nodes.
@@ -191,17 +195,17 @@ struct NveStepArgs {
   double *x4, *v, *xhold, *xhold_in, *nodex, *nodev, *xc, *nodexhold;
   const double *f, *nodef;
   double dtv, dtfm, dtfm_node, triggersq, thr_in;
-  int *flag, *ibad;
+  int *flag, *abort, *ibad;
   Scale3 m;
 };
 
 __global__ void k_nve_step(NveStepArgs a) {
   if ((int)blockIdx.x < a.ablocks) {
     nve_atom_step((int)(blockIdx.x * blockDim.x + threadIdx.x), a.n, a.x4, a.v, a.f, a.dtv, a.dtfm, a.fin, a.xhold,
-                  a.triggersq, a.flag, a.remap, a.m, a.xhold_in, a.thr_in, a.ibad);
+                  a.triggersq, a.flag, a.abort, a.remap, a.m, a.xhold_in, a.thr_in, a.ibad);
   } else {
     nve_node_step((long)(blockIdx.x - a.ablocks) * blockDim.x + threadIdx.x, a.ne, a.nodex, a.nodev, a.nodef, a.xc,
-                  a.dtv, a.dtfm_node, a.fin, a.nodexhold, a.triggersq, a.flag, a.remap, a.m);
+                  a.dtv, a.dtfm_node, a.fin, a.nodexhold, a.triggersq, a.flag, a.abort, a.remap, a.m);
   }
 }
 
@@ -403,6 +407,8 @@ int build_impl(cac_ctx *ctx) {
     ctx->built = true;
     ctx->ago = 0;
     ctx->ncalls++;
+    // force passes queued after this run in full (see raise_flag)
+    CAC_HIP(ctx, hipMemsetAsync(ctx->dflags + 15, 0, sizeof(int), ctx->stream));
     return invalidate_inner(ctx);
   }
   return fail(ctx, CAC_ERR_OVERFLOW, "neighbor capacity still exceeded after regrowth");
@@ -496,6 +502,7 @@ int nve_step_impl(cac_ctx *ctx, double dtv, double dtf, bool fin, int *flag, con
   a.triggersq = ctx->triggersq;
   a.thr_in = thr * thr;
   a.flag = flag;
+  a.abort = ctx->dflags + 15;
   a.ibad = ctx->dflags + 14;
   const unsigned blocks = (unsigned)a.ablocks + grid_for(8L * ctx->ne);
   if (blocks) hipLaunchKernelGGL(k_nve_step, dim3(blocks), dim3(kBlock), 0, ctx->stream, a);
@@ -618,7 +625,7 @@ int cac_ctx_create(int device, cac_ctx **out) {
     delete ctx;
     return CAC_ERR_HIP;
   }
-  if (ensure(ctx, ctx->b_dflags, 16, &ctx->dflags) ||
+  if (ensure(ctx, ctx->b_dflags, 16, &ctx->dflags) || hipMemset(ctx->dflags, 0, 16 * sizeof(int)) != hipSuccess ||
       hipHostMalloc((void **)&ctx->hflags, 8 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
     cac_ctx_destroy(ctx);
     return CAC_ERR_HIP;
@@ -961,6 +968,9 @@ int cac_force(cac_ctx *ctx, int eflag, int vflag, int precision) {
   hipSetDevice(ctx->device);
   int rc = ready_for_force(ctx);
   if (rc) return rc;
+  // a cac_run that stopped on an error between a raised check flag and
+  // its rebuild leaves the skip flag set
+  CAC_HIP(ctx, hipMemsetAsync(ctx->dflags + 15, 0, sizeof(int), ctx->stream));
   return force_impl(ctx, eflag, vflag, precision);
 }
 
@@ -982,6 +992,9 @@ int cac_run(cac_ctx *ctx, int nsteps, double dtv, double dtf, int precision, int
   int rc = ready_for_force(ctx);
   if (rc) return rc;
   if ((rc = ensure_step_sync(ctx))) return rc;
+  // The skip flag (dflags[15]) is zero here and after every rebuild; it is
+  // raised with the host check flag and always followed by a rebuild.
+  CAC_HIP(ctx, hipMemsetAsync(ctx->dflags + 15, 0, sizeof(int), ctx->stream));
   int builds = 0;
   // Verlet::run, verlet.cpp:149-219.  Same sequence as nve_initial, decide,
   // [build], force, nve_final per step, arranged so the device never waits

@@ -445,7 +445,9 @@ __global__ __launch_bounds__(kAtomWG) void k_eam_atom_rho(int n, double *__restr
                                                           double *__restrict__ rho_out, double *__restrict__ eatom,
                                                           uint32_t *__restrict__ mask, const int *__restrict__ ibad,
                                                           uint32_t *__restrict__ mask_in,
-                                                          double *__restrict__ xhold_in, double cutinsq, int per) {
+                                                          double *__restrict__ xhold_in, double cutinsq, int per,
+                                                          const int *__restrict__ skip) {
+  if (*skip) return;  // this step rebuilds first: the caller discards these results
   extern __shared__ __align__(16) unsigned char smem[];
   const Win<P> w = make_win<P>(smem, m0);
   load_window<P>(t.rv, 4, m0, t.nr, reinterpret_cast<P *>(smem));
@@ -512,7 +514,8 @@ __global__ __launch_bounds__(kAtomWG) void k_eam_atom_force(int n, const double 
                                                             double *__restrict__ f, double *__restrict__ eatom,
                                                             double *__restrict__ vatom,
                                                             const uint32_t *__restrict__ mask, int per,
-                                                            int *__restrict__ ibad) {
+                                                            int *__restrict__ ibad, const int *__restrict__ skip) {
+  if (*skip) return;  // this step rebuilds first: the caller discards these results
   extern __shared__ __align__(16) unsigned char smem[];
   // the density pass has read the inner-mask flag: its inner mask now holds
   // for these positions (see k_eam_atom_rho)
@@ -869,6 +872,7 @@ struct ClArgs {
   double *scratch;
   size_t sstride;
   int srows;
+  const int *skip;  // nonzero: this step rebuilds first, return at once
 };
 
 // Shape functions of site s, computed as the host computes the site table
@@ -1014,7 +1018,7 @@ __global__ __launch_bounds__(64) void k_cl_erho(ClArgs c, ElemArgs a, Prd prd, T
                                                 uint32_t *__restrict__ mask) {
   extern __shared__ __align__(16) double lds_rows[];
   const int o = xcd_block();
-  if (o >= c.count) return;  // whole workgroup
+  if (o >= c.count || *c.skip) return;  // whole workgroup
   const int cl = c.order[o];
   const int big = c.big[cl];
   if (big >= 0)
@@ -1128,7 +1132,7 @@ __global__ __launch_bounds__(64) void k_cl_force(ClArgs c, ElemArgs a, Prd prd, 
                                                  double *__restrict__ nodef) {
   extern __shared__ __align__(16) double lds_rows[];
   const int o = xcd_block();
-  if (o >= c.count) return;  // whole workgroup
+  if (o >= c.count || *c.skip) return;  // whole workgroup
   const int cl = c.order[o];
   const int big = c.big[cl];
   if (big >= 0)
@@ -1378,7 +1382,7 @@ int atom_rho(cac_ctx *ctx, const Tabs<P> &t, AtomGrid agrid, const TabWindow &wr
   ctx->inner_cut = cin;
   hipLaunchKernelGGL((k_eam_atom_rho<P, EV>), dim3(agrid.blocks), dim3(kAtomWG), wr.bytes, ctx->stream, ctx->n,
                      ctx->x4, prd_of(ctx), ctx->aneigh, ctx->maxneigh, ctx->anum, t, wr.m0, ctx->rho, ctx->eatom,
-                     ctx->amask, ibad, ctx->amask_in, ctx->xhold_in, cutinsq, agrid.per);
+                     ctx->amask, ibad, ctx->amask_in, ctx->xhold_in, cutinsq, agrid.per, ctx->dflags + 15);
   return CAC_OK;
 }
 
@@ -1424,6 +1428,7 @@ ClArgs cl_args(const cac_ctx *ctx) {
   c.scratch = ctx->cl_scratch;
   c.srows = ctx->cl_umax;
   c.sstride = (size_t)4 * ctx->cl_umax + kClMax * (24 + 8);
+  c.skip = ctx->dflags + 15;
   return c;
 }
 size_t cl_lds(const ClArgs &c) { return ((size_t)4 * c.rows + kClMax * (24 + 8)) * sizeof(double); }
@@ -1469,7 +1474,7 @@ int eam_clusters(cac_ctx *ctx, const ElemArgs &base) {
     timer_begin(ctx, "atom_force", &t0);
     hipLaunchKernelGGL((k_eam_atom_force<P, EV>), dim3(agrid.blocks), dim3(kAtomWG), wz.bytes, ctx->stream, n,
                        ctx->x4, prd_of(ctx), ctx->aneigh, ctx->maxneigh, ctx->anum, t, wz.m0, ctx->f, ctx->eatom,
-                       ctx->vatom, ctx->amask, agrid.per, ctx->dflags + 14);
+                       ctx->vatom, ctx->amask, agrid.per, ctx->dflags + 14, ctx->dflags + 15);
     timer_end(ctx, "atom_force", t0);
   }
   timer_begin(ctx, "ip_force", &t0);
@@ -1523,7 +1528,7 @@ int eam_all(cac_ctx *ctx, const ElemArgs &base, const ElemArgs &spos, const Elem
     timer_begin(ctx, "atom_force", &t0);
     hipLaunchKernelGGL((k_eam_atom_force<P, EV>), dim3(agrid.blocks), dim3(kAtomWG), wz.bytes, ctx->stream, n,
                        ctx->x4, prd_of(ctx), ctx->aneigh, ctx->maxneigh, ctx->anum, t, wz.m0, ctx->f, ctx->eatom,
-                       ctx->vatom, ctx->amask, agrid.per, ctx->dflags + 14);
+                       ctx->vatom, ctx->amask, agrid.per, ctx->dflags + 14, ctx->dflags + 15);
     timer_end(ctx, "atom_force", t0);
   }
   const long nf = el ? host_nelem(force_el) : 0;
