@@ -45,8 +45,8 @@ B_ELEM = 7640
 # HBM bytes per launch of the FP64 kernels on the eam_c1 workload, from
 # rocprofv3 FETCH_SIZE and WRITE_SIZE runs scaled by a plain 1 GiB copy's
 # counts (fetch x2.000, write x1.000); the summary is profiles/eam_c1_kernels.md.
-TRAFFIC = {"atom_force": 187.8e6 + 11.9e6, "atom_rho": 190.1e6 + 26.0e6, "ip_force": 147.2e6 + 11.8e6,
-           "ip_erho": 131.1e6 + 12.2e6}
+TRAFFIC = {"atom_force": 187.7e6 + 11.9e6, "atom_rho": 190.1e6 + 26.0e6, "ip_force": 148.6e6 + 19.0e6,
+           "ip_erho": 131.0e6 + 12.2e6}
 
 
 def log(msg):
@@ -419,8 +419,11 @@ def main():
                                       _lib.PREC_FP64)
             acc["drift_mixed"] = drift(s, pair, cutneigh, skin, args.every, args.delay, args.drift_steps,
                                        _lib.PREC_MIXED)
-            log(f"[bench] drift fp64 {acc['drift_fp64']['drift_eV_per_particle']:.3e}, mixed "
-                f"{acc['drift_mixed']['drift_eV_per_particle']:.3e} eV/particle over {args.drift_steps} steps")
+            for k in ("drift_fp64", "drift_mixed"):
+                d = acc[k]
+                end = (f"diverged (|dE| > {d['limit_eV_per_particle']} eV/particle) at step {d['diverged_at']}"
+                       if d["diverged_at"] else f"after {d['steps']} steps")
+                log(f"[bench] {k}: {d['drift_eV_per_particle']:+.3e} eV/particle, {end}")
 
     cpu = None
     if not args.no_cpu:
