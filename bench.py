@@ -121,6 +121,22 @@ def gpu_run(s, pair, cutneigh, skin, steps, warmup, precision, every, delay, pro
     return eng, t1 - t0, nb, nb_warm, prof
 
 
+def centro_leg(eng, lo=1.0, hi=10.0, nnn=12):
+    """Centro-symmetry of every atom after the timed run, the reference's
+    defect output (Output::centro_atom, output.cpp:352-441; the xyz dump keeps
+    lo < c < hi, dump.cpp:355-365), over the current lists; outside the timed
+    region.  Its kernel time comes from the HIP-event timer."""
+    import numpy as np
+    eng.centro(nnn)                 # first launch loads the kernel's code object
+    eng.synchronize()
+    eng.profile(True)
+    c = eng.centro(nnn)
+    ms, cnt = eng.profile_avg_ms("centro")
+    eng.profile(False)
+    return {"nnn": nnn, "kernel_ms": round(ms, 3) if cnt else None, "window": [lo, hi],
+            "defect_atoms": int(np.count_nonzero((c > lo) & (c < hi))), "atoms": int(c.size)}
+
+
 def cpu_baseline(s, pair, ff, cutneigh, threads, warm, timed, rebuild_rate):
     """The CPU FP64 oracle on this host with `threads` OpenMP threads: one
     neighbor build, `warm` untimed and `timed` timed full steps (force on
@@ -374,6 +390,7 @@ def main():
         return
     eng, secs, nbuilds, nbw, prof = gpu_run(s, pair, cutneigh, skin, args.steps, args.warmup, prec, args.every,
                                             args.delay)
+    centro = centro_leg(eng) if s.n_atoms else None
     eng.close()
     n_int = s.n_atoms + 8 * s.n_elements
     n_rep = s.n_atoms + s.nae[0] * s.nae[1] * s.nae[2] * s.n_elements
@@ -382,6 +399,9 @@ def main():
         f"{nbuilds} rebuilds in {args.steps} steps")
     for k, (ms, c) in prof.items():
         log(f"[bench]   {k:12s} {ms * 1e3:9.1f} us x {c}")
+    if centro:
+        log(f"[bench] centro-symmetry: {centro['kernel_ms']} ms for {centro['atoms']} atoms, "
+            f"{centro['defect_atoms']} in the dump window {centro['window']}")
 
     # dominant kernel by total time among those with an algorithmic byte count
     dom = max((k for k in prof if k in ALG_BYTES), key=lambda k: prof[k][0] * prof[k][1], default=None)
@@ -471,6 +491,7 @@ def main():
         "represented_Matom_steps_per_s": round(n_rep * args.steps / secs / 1e6, 1),
         "roofline": roof,
         "step_roofline": step_roof,
+        "centro": centro,
         "variants": variants,
         "accuracy": acc,
         "cpu_baseline": cpu,

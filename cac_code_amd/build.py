@@ -22,7 +22,8 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcac_hip.so")
 
-SOURCES = ["cac_api.hip", "k_neighbor.hip", "k_pair.hip", "k_mlp.hip", "k_surrogate.hip", "tables.cpp"]
+SOURCES = ["cac_api.hip", "k_neighbor.hip", "k_pair.hip", "k_centro.hip", "k_mlp.hip", "k_surrogate.hip",
+           "tables.cpp"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CAC_OFFLOAD_ARCH", "gfx950")

@@ -642,7 +642,7 @@ int cac_ctx_destroy(cac_ctx *ctx) {
   surrogate_destroy(ctx);
   DevBuf *bufs[] = {&ctx->b_x4, &ctx->b_v, &ctx->b_f, &ctx->b_xhold, &ctx->b_rho, &ctx->b_eatom, &ctx->b_vatom,
                     &ctx->b_aneigh, &ctx->b_anum, &ctx->b_akey, &ctx->b_aval, &ctx->b_akey2, &ctx->b_aval2,
-                    &ctx->b_ahead, &ctx->b_acells, &ctx->b_cl_big, &ctx->b_cl_scratch, &ctx->b_acs, &ctx->b_ace, &ctx->b_sorttmp, &ctx->b_nodex,
+                    &ctx->b_ahead, &ctx->b_acells, &ctx->b_cl_big, &ctx->b_cl_scratch, &ctx->b_centro, &ctx->b_acs, &ctx->b_ace, &ctx->b_sorttmp, &ctx->b_nodex,
                     &ctx->b_nodev, &ctx->b_nodef, &ctx->b_amask_in, &ctx->b_xhold_in,
                     &ctx->b_nodexhold, &ctx->b_xc, &ctx->b_spa_site, &ctx->b_spa_int, &ctx->b_weight,
                     &ctx->b_erad, &ctx->b_ehull, &ctx->b_eneigh, &ctx->b_enumb, &ctx->b_ekey, &ctx->b_eval, &ctx->b_ekey2,
@@ -1079,6 +1079,26 @@ int cac_get_energy_virial(cac_ctx *ctx, double eng[2], double virial[6]) {
   if (eng) { eng[0] = ctx->eng[0]; eng[1] = ctx->eng[1]; }
   if (virial)
     for (int c = 0; c < 6; ++c) virial[c] = ctx->virial[c];
+  return CAC_OK;
+}
+
+int cac_centro_atoms(cac_ctx *ctx, int nnn, double *centro) {
+  if (!ctx) return CAC_ERR_ARG;
+  hipSetDevice(ctx->device);
+  if (nnn != 8 && nnn != 12) return fail(ctx, CAC_ERR_ARG, "centro-symmetry: nnn must be 8 or 12");
+  if (!ctx->box_set) return fail(ctx, CAC_ERR_ARG, "cac_set_box not called");
+  if (!ctx->built) return fail(ctx, CAC_ERR_ARG, "neighbor lists not built");
+  const int n = ctx->n;
+  if (n == 0) return CAC_OK;
+  if (!centro) return fail(ctx, CAC_ERR_ARG, "centro-symmetry: null output");
+  int rc;
+  if ((rc = ensure(ctx, ctx->b_centro, (size_t)n, &ctx->centro))) return rc;
+  // site rows of coupled lists (the lists of the last build, whatever the
+  // coupling switch says now)
+  if ((rc = atom_list_site_rows(ctx))) return rc;
+  if ((rc = launch_centro(ctx, nnn, ctx->centro))) return rc;
+  CAC_HIP(ctx, hipMemcpyAsync(centro, ctx->centro, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return CAC_OK;
 }
 

@@ -229,6 +229,9 @@ struct cac_ctx {
   int cl_nbig = 0, cl_umax_small = 0;
   double *cl_scratch = nullptr;
   cac::DevBuf b_cl_big, b_cl_scratch;
+  // centro-symmetry values of the last cac_centro_atoms call
+  double *centro = nullptr;
+  cac::DevBuf b_centro;
   bool cl_ok = false;
   int naref = 0;
   int *aref = nullptr, *amark = nullptr;
@@ -366,6 +369,8 @@ int launch_check_displacement(cac_ctx *ctx);
 // elements interleaved in the order the coupled lists need
 int launch_eam(cac_ctx *ctx, int eflag, int vflag, int precision);
 int launch_lj(cac_ctx *ctx, int eflag, int vflag, int precision);
+int atom_list_site_rows(cac_ctx *ctx);
+int launch_centro(cac_ctx *ctx, int nnn, double *dout);
 
 // ---- launchers (cac_api.hip)
 int launch_reduce_ev(cac_ctx *ctx, int eflag, int vflag);

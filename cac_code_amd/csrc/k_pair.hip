@@ -1597,4 +1597,13 @@ int launch_lj(cac_ctx *ctx, int eflag, int vflag, int precision) {
   return CAC_DISPATCH(lj_t, ctx, eflag || vflag, precision);
 }
 
+// Interpolated positions of every active site, the rows j >= n that coupled
+// atom lists refer to (the force passes refresh only those they read).
+int atom_list_site_rows(cac_ctx *ctx) {
+  if (ctx->ne == 0 || ctx->nactive == 0) return CAC_OK;
+  site_positions(ctx, elem_args(ctx));
+  CAC_HIP(ctx, hipGetLastError());
+  return CAC_OK;
+}
+
 }  // namespace cac

@@ -410,6 +410,28 @@ class Engine(_MlpMixin):
         _check(self.ctx, self.L.cac_get_energy_virial(self.ctx, _p(e), _p(w)))
         return e, w
 
+    # ---- centro-symmetry and the xyz dump (Output::centro_atom,
+    # output.cpp:352-441; DumpXyz, dump_xyz.cpp:36-76, 127-141)
+    def centro(self, nnn: int = 12) -> np.ndarray:
+        """Centro-symmetry parameter of every atom over its current list (the
+        reference computes it right after a build, verlet.cpp:93-94)."""
+        out = np.zeros(self.n)
+        _check(self.ctx, self.L.cac_centro_atoms(self.ctx, int(nnn), _p(out)))
+        return out
+
+    def write_dump_xyz(self, path: str, nnn: int = 12, lo: float = 1.0, hi: float = 10.0) -> int:
+        """Write the reference's xyz dump: a count line, "XYZ", then
+        "Cu x y z c" (%g) for every atom with lo < c < hi (Dump::count and
+        DumpXyz::pack_noscale_noimage select 1 < c < 10).  Returns the count."""
+        c = self.centro(nnn)
+        x, _, _ = self.atoms()
+        sel = np.nonzero((c > lo) & (c < hi))[0]
+        with open(path, "w") as fh:
+            fh.write(f"{len(sel)}\nXYZ\n")
+            for i in sel:
+                fh.write("Cu %g %g %g %g\n" % (x[i, 0], x[i, 1], x[i, 2], c[i]))
+        return len(sel)
+
     def capacities(self):
         a, b = C.c_int(), C.c_int()
         _check(self.ctx, self.L.cac_get_capacities(self.ctx, C.byref(a), C.byref(b)))

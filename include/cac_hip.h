@@ -192,6 +192,19 @@ int cac_download_elements(cac_ctx *ctx, double *nodex, double *nodev, double *no
  * reference: integrate.cpp:97-114 forces eflag = vflag = 0). */
 int cac_get_energy_virial(cac_ctx *ctx, double eng[2], double virial[6]);
 
+/* Centro-symmetry parameter of every atom (replaces Output::centro_atom,
+ * output.cpp:352-441, which verlet.cpp:93-94 calls after Neighbor::build):
+ * over the list entries with r^2 < cutneigh^2 (cutneighmaxsq), the nnn
+ * nearest (nnn = 12 in the reference, output.cpp:84; 8 for bcc), and
+ * centro[i] = the sum of the nnn/2 smallest |x_j + x_k - 2 x_i|^2 over their
+ * pairs, or 0 with fewer than nnn entries.  centro: [n] on the host.  Uses
+ * the current lists; called right after cac_build_neighbors it is the
+ * reference's call.  With coupling on, element sites in the atom lists
+ * count as neighbors.  Ties at the nnn-th distance go to the smaller entry
+ * word (atom index | image << 27); the reference leaves them to its
+ * quickselect. */
+int cac_centro_atoms(cac_ctx *ctx, int nnn, double *centro);
+
 /* Parity hooks: neighbor lists (row-major [n][maxneigh] / [ne*nint][maxnip],
  * device order, entries as above), per-atom rho/fp, per-node erho ([ne][8]),
  * per-integration-point force ([ne*nint][3]).  Return CAC_ERR_ARG if the

@@ -1052,6 +1052,84 @@ static inline void entry_pos(const double *x, const orc_elems *el, uint64_t e, c
   }
 }
 
+/* ------------------------------------------------------------------------
+ * Centro-symmetry parameter, Output::centro_atom (output.cpp:352-441): the
+ * list entries with rsq < cutneighmaxsq (:374, :398-410), 0 with fewer than
+ * nnn (:414-417), the nnn nearest (select2, :421), the nnn(nnn-1)/2 values
+ * |x_jj + x_kk - 2 x_i|^2 (:426-434) and the sum of the nnn/2 smallest
+ * (select, :438-442).  The reference's quickselect leaves the order of the
+ * selected values, and the choice among entries tied at the nnn-th distance,
+ * to its partitioning; here both are fixed (full sorts, ties by key), so the
+ * value depends on the neighbor set alone.
+ * ---------------------------------------------------------------------- */
+typedef struct {
+  double rsq;
+  uint64_t key;
+  uint64_t e;
+} csp_nb_t;
+
+static int cmp_csp_nb(const void *a, const void *b) {
+  const csp_nb_t *p = (const csp_nb_t *)a, *q = (const csp_nb_t *)b;
+  if (p->rsq != q->rsq) return p->rsq < q->rsq ? -1 : 1;
+  return p->key < q->key ? -1 : p->key > q->key;
+}
+
+static int cmp_f64(const void *a, const void *b) {
+  double x = *(const double *)a, y = *(const double *)b;
+  return x < y ? -1 : x > y;
+}
+
+void orc_centro_atoms(int n, const double *x, const orc_elems *el, const orc_box *box, const int *anum,
+                      const uint64_t *anb, int maxneigh, double cutneigh, int nnn, double *centro) {
+  double prd[3] = {box->hi[0] - box->lo[0], box->hi[1] - box->lo[1], box->hi[2] - box->lo[2]};
+  double cutsq = cutneigh * cutneigh;
+  int nhalf = nnn / 2, npairs = nnn * (nnn - 1) / 2;
+#pragma omp parallel
+  {
+    csp_nb_t *nb = (csp_nb_t *)malloc(sizeof(csp_nb_t) * (size_t)(maxneigh > 0 ? maxneigh : 1));
+    double *pairs = (double *)malloc(sizeof(double) * (size_t)(npairs > 0 ? npairs : 1));
+#pragma omp for schedule(static)
+    for (int i = 0; i < n; i++) {
+      double xtmp = x[3 * i], ytmp = x[3 * i + 1], ztmp = x[3 * i + 2];
+      int cnt = 0;
+      for (int jj = 0; jj < anum[i]; jj++) {
+        uint64_t e = anb[(size_t)i * maxneigh + jj];
+        double s[3];
+        entry_pos(x, el, e, prd, s);
+        double delx = xtmp - s[0], dely = ytmp - s[1], delz = ztmp - s[2];
+        double rsq = delx * delx + dely * dely + delz * delz;
+        if (rsq < cutsq) {
+          nb[cnt].rsq = rsq;
+          nb[cnt].key = (uint32_t)(e >> 32) == ORC_ATOM_SITE ? (e & 0xFFFFFFFFu) : ((1ull << 63) | e);
+          nb[cnt].e = e;
+          cnt++;
+        }
+      }
+      if (cnt < nnn) {
+        centro[i] = 0.0;
+        continue;
+      }
+      qsort(nb, cnt, sizeof(csp_nb_t), cmp_csp_nb);
+      double p[16][3];
+      for (int a = 0; a < nnn; a++) entry_pos(x, el, nb[a].e, prd, p[a]);
+      int m = 0;
+      for (int j = 0; j < nnn; j++)
+        for (int k = j + 1; k < nnn; k++) {
+          double delx = p[j][0] + p[k][0] - 2.0 * xtmp;
+          double dely = p[j][1] + p[k][1] - 2.0 * ytmp;
+          double delz = p[j][2] + p[k][2] - 2.0 * ztmp;
+          pairs[m++] = delx * delx + dely * dely + delz * delz;
+        }
+      qsort(pairs, npairs, sizeof(double), cmp_f64);
+      double value = 0.0;
+      for (int h = 0; h < nhalf; h++) value += pairs[h];
+      centro[i] = value;
+    }
+    free(nb);
+    free(pairs);
+  }
+}
+
 /* F'(rho) of an entry: the atom's fp, or F' of the interpolated site
  * density sum_k N_k erho_k (pair_eam.cpp:1077, 1089-1096). */
 static inline double entry_fp(const orc_elems *el, const orc_eam *t, const double *fp, const double *erho,

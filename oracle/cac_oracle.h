@@ -128,6 +128,16 @@ void orc_lj_coupled(int n, const double *x, const orc_elems *el, const orc_box *
                     const orc_lj *pp, double *f, double *eatom, double *vatom, double *nodef, double *ipf,
                     double *eip, double *vip);
 
+/* ---- centro-symmetry (Output::centro_atom, output.cpp:352-441) ----
+ * Over the entries of each atom's list (uint64 entries as above; el may be
+ * NULL when no entry is a site) with r^2 < cutneigh^2: fewer than nnn give
+ * 0; else the nnn nearest by (r^2, tie key), and the sum, in ascending
+ * order, of the nnn/2 smallest |x_j + x_k - 2 x_i|^2 over their pairs.  The
+ * tie key orders atoms by their 32-bit entry (j | img << 27) and puts sites
+ * after atoms.  nnn even, 2 <= nnn <= 16. */
+void orc_centro_atoms(int n, const double *x, const orc_elems *el, const orc_box *box, const int *anum,
+                      const uint64_t *anb, int maxneigh, double cutneigh, int nnn, double *centro);
+
 /* ---- integration and periodic wrap ---- */
 void orc_nve_atoms_initial(int n, double *x, double *v, const double *f,
                            double dtv, double dtf, double mass);

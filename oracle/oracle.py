@@ -355,6 +355,31 @@ def eam_coupled(x, es: ElementSet, lo, hi, anum, anb, inum, inb, tables):
     return out
 
 
+def centro_atoms(x, lo, hi, num, nb, cutneigh, nnn=12, es: ElementSet | None = None):
+    """Centro-symmetry parameter of every atom from its list (uint32 atom
+    entries, or the uint64 coupled entries with es for the sites)."""
+    if not (nnn % 2 == 0 and 2 <= nnn <= 16):
+        raise ValueError("nnn must be even, 2..16")
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+    n = x.shape[0]
+    nb = np.asarray(nb)
+    if nb.dtype == np.uint32:
+        nb = nb.astype(np.uint64) | (np.uint64(ATOM_SITE) << np.uint64(32))
+    out = np.zeros(n)
+    if n == 0:
+        return out
+    nb = np.ascontiguousarray(nb, dtype=np.uint64).reshape(n, -1)
+    b = _box(lo, hi)
+    ec = None
+    if es is not None:
+        es.refresh()
+        ec = C.byref(es.c)
+    num = np.ascontiguousarray(num, dtype=np.int32)
+    lib().orc_centro_atoms(C.c_int(n), _p(x), ec, C.byref(b), _p(num, _i), _p(nb, _u64), C.c_int(nb.shape[1]),
+                           C.c_double(cutneigh), C.c_int(nnn), _p(out))
+    return out
+
+
 def lj_coupled(x, es: ElementSet, lo, hi, anum, anb, inum, inb, lj):
     x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
     n = x.shape[0]
