@@ -55,6 +55,7 @@ SIGNATURES = {
     "cac_download_elements": (C.c_int, [_ctx, _d, _d, _d]),
     "cac_get_energy_virial": (C.c_int, [_ctx, _d, _d]),
     "cac_centro_atoms": (C.c_int, [_ctx, C.c_int, _d]),
+    "cac_interpolate_sites": (C.c_int, [_ctx, _d]),
     "cac_get_capacities": (C.c_int, [_ctx, _i, _i]),
     "cac_get_atom_neighbors": (C.c_int, [_ctx, C.c_int, _i, _u64]),
     "cac_get_ip_neighbors": (C.c_int, [_ctx, C.c_int, _i, _u64]),

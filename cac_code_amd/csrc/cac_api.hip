@@ -642,7 +642,7 @@ int cac_ctx_destroy(cac_ctx *ctx) {
   surrogate_destroy(ctx);
   DevBuf *bufs[] = {&ctx->b_x4, &ctx->b_v, &ctx->b_f, &ctx->b_xhold, &ctx->b_rho, &ctx->b_eatom, &ctx->b_vatom,
                     &ctx->b_aneigh, &ctx->b_anum, &ctx->b_akey, &ctx->b_aval, &ctx->b_akey2, &ctx->b_aval2,
-                    &ctx->b_ahead, &ctx->b_acells, &ctx->b_cl_big, &ctx->b_cl_scratch, &ctx->b_centro, &ctx->b_acs, &ctx->b_ace, &ctx->b_sorttmp, &ctx->b_nodex,
+                    &ctx->b_ahead, &ctx->b_acells, &ctx->b_cl_big, &ctx->b_cl_scratch, &ctx->b_centro, &ctx->b_sitex, &ctx->b_acs, &ctx->b_ace, &ctx->b_sorttmp, &ctx->b_nodex,
                     &ctx->b_nodev, &ctx->b_nodef, &ctx->b_amask_in, &ctx->b_xhold_in,
                     &ctx->b_nodexhold, &ctx->b_xc, &ctx->b_spa_site, &ctx->b_spa_int, &ctx->b_weight,
                     &ctx->b_erad, &ctx->b_ehull, &ctx->b_eneigh, &ctx->b_enumb, &ctx->b_ekey, &ctx->b_eval, &ctx->b_ekey2,
@@ -1098,6 +1098,20 @@ int cac_centro_atoms(cac_ctx *ctx, int nnn, double *centro) {
   if ((rc = atom_list_site_rows(ctx))) return rc;
   if ((rc = launch_centro(ctx, nnn, ctx->centro))) return rc;
   CAC_HIP(ctx, hipMemcpyAsync(centro, ctx->centro, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return CAC_OK;
+}
+
+int cac_interpolate_sites(cac_ctx *ctx, double *x) {
+  if (!ctx) return CAC_ERR_ARG;
+  hipSetDevice(ctx->device);
+  const size_t total = (size_t)ctx->ne * ctx->nsite;
+  if (total == 0) return CAC_OK;
+  if (!x) return fail(ctx, CAC_ERR_ARG, "interpolate sites: null output");
+  int rc;
+  if ((rc = ensure(ctx, ctx->b_sitex, 3 * total, &ctx->sitex))) return rc;
+  if ((rc = launch_interp_sites(ctx, ctx->sitex))) return rc;
+  CAC_HIP(ctx, hipMemcpyAsync(x, ctx->sitex, 3 * total * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return CAC_OK;
 }

@@ -232,6 +232,9 @@ struct cac_ctx {
   // centro-symmetry values of the last cac_centro_atoms call
   double *centro = nullptr;
   cac::DevBuf b_centro;
+  // every interpolated site of every element (cac_interpolate_sites)
+  double *sitex = nullptr;
+  cac::DevBuf b_sitex;
   bool cl_ok = false;
   int naref = 0;
   int *aref = nullptr, *amark = nullptr;
@@ -371,6 +374,7 @@ int launch_eam(cac_ctx *ctx, int eflag, int vflag, int precision);
 int launch_lj(cac_ctx *ctx, int eflag, int vflag, int precision);
 int atom_list_site_rows(cac_ctx *ctx);
 int launch_centro(cac_ctx *ctx, int nnn, double *dout);
+int launch_interp_sites(cac_ctx *ctx, double *dout);
 
 // ---- launchers (cac_api.hip)
 int launch_reduce_ev(cac_ctx *ctx, int eflag, int vflag);

@@ -139,7 +139,40 @@ __global__ __launch_bounds__(kCentroBlock) void k_centro(int n, const double *__
   centro[i] = value;
 }
 
+// Every site of every element, Element::interpolate_atom_vec
+// (element.cpp:1241-1265): x = sum_k N_k(site) X_k in node order from 0.0,
+// elements in order, sites in the inter_ele order (x outermost); one thread
+// per site, rows [ne*nsite][3].
+__global__ __launch_bounds__(kCentroBlock) void k_interp_sites(int ne, int nsite, const double *__restrict__ nodex,
+                                                               const double *__restrict__ spa_site,
+                                                               double *__restrict__ out) {
+  const long q = (long)blockIdx.x * kCentroBlock + threadIdx.x;
+  if (q >= (long)ne * nsite) return;
+  const long e = q / nsite;
+  const int s = (int)(q - e * nsite);
+  const double *X = nodex + 24 * e;
+  const double *N = spa_site + 8 * (size_t)s;
+  double x = 0.0, y = 0.0, z = 0.0;
+  for (int k = 0; k < 8; ++k) {
+    x += N[k] * X[3 * k];
+    y += N[k] * X[3 * k + 1];
+    z += N[k] * X[3 * k + 2];
+  }
+  out[3 * q] = x;
+  out[3 * q + 1] = y;
+  out[3 * q + 2] = z;
+}
+
 }  // namespace
+
+int launch_interp_sites(cac_ctx *ctx, double *dout) {
+  const long total = (long)ctx->ne * ctx->nsite;
+  if (total == 0) return CAC_OK;
+  hipLaunchKernelGGL(k_interp_sites, dim3((unsigned)((total + kCentroBlock - 1) / kCentroBlock)), dim3(kCentroBlock),
+                     0, ctx->stream, ctx->ne, ctx->nsite, ctx->nodex, ctx->spa_site, dout);
+  CAC_HIP(ctx, hipGetLastError());
+  return CAC_OK;
+}
 
 // Site rows of the coupled atom lists must hold the current interpolated
 // positions; the caller has brought them up to date (atom_list_site_rows).

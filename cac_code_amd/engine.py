@@ -353,6 +353,7 @@ class Engine(_MlpMixin):
             if skin <= 0.0:
                 raise PairError("neighbor cutoff must exceed the force cutoff")
         triggersq = 0.25 * skin * skin
+        self.neighbor_args = (cutneigh, skin, every, delay, check)
         _check(self.ctx, self.L.cac_set_neighbor(self.ctx, cutneigh, triggersq, every, delay, 1 if check else 0,
                                                  maxneigh, maxnip))
 
@@ -418,6 +419,32 @@ class Engine(_MlpMixin):
         out = np.zeros(self.n)
         _check(self.ctx, self.L.cac_centro_atoms(self.ctx, int(nnn), _p(out)))
         return out
+
+    def interpolated_sites(self) -> np.ndarray:
+        """Every site of every element, [ne * nsite][3] in element and
+        inter_ele order (Element::interpolate_atom_vec, element.cpp:1241-1265)."""
+        nsite = int(np.prod(self.sys.nae)) if self.sys is not None else 0
+        x = np.zeros((self.ne * nsite, 3))
+        if x.size:
+            _check(self.ctx, self.L.cac_interpolate_sites(self.ctx, _p(x)))
+        return x
+
+    def centro_of_sites(self, nnn: int = 12, device: int = 0):
+        """The reference's executed path (verlet.cpp:91-94): the atoms are
+        replaced by the interpolated element sites (element.cpp:1226-1233),
+        the atom list is built over them and their centro-symmetry computed.
+        Runs in a second context with this one's box, pair style and neighbor
+        settings; returns (site positions, values)."""
+        xs = self.interpolated_sites()
+        with Engine(device) as e2:
+            e2.set_box(self.sys.box_lo, self.sys.box_hi)
+            e2.upload_atoms(xs, None, self.mass)
+            e2.set_pair(self.pair)
+            cutneigh, skin, every, delay, check = self.neighbor_args
+            e2.set_neighbor(cutneigh, skin, every, delay, check)
+            e2.build()
+            c = e2.centro(nnn)
+        return xs, c
 
     def write_dump_xyz(self, path: str, nnn: int = 12, lo: float = 1.0, hi: float = 10.0) -> int:
         """Write the reference's xyz dump: a count line, "XYZ", then

@@ -205,6 +205,16 @@ int cac_get_energy_virial(cac_ctx *ctx, double eng[2], double virial[6]);
  * quickselect. */
 int cac_centro_atoms(cac_ctx *ctx, int nnn, double *centro);
 
+/* Every interpolated site of every element (replaces
+ * Element::interpolate_atom_vec, element.cpp:1226-1265): x[(e*nsite + s)*3 + d]
+ * = sum_k N_k(site s) nodex[e][k][d], summed in node order, sites in the
+ * inter_ele order (element.cpp:1394-1423), nsite = nae[0]*nae[1]*nae[2].
+ * The reference's only executed path (verlet.cpp:91-95) replaces its atoms
+ * by these sites, builds the atom list over them and computes their
+ * centro-symmetry: upload them as the atoms of a context and call
+ * cac_build_neighbors and cac_centro_atoms there. */
+int cac_interpolate_sites(cac_ctx *ctx, double *x);
+
 /* Parity hooks: neighbor lists (row-major [n][maxneigh] / [ne*nint][maxnip],
  * device order, entries as above), per-atom rho/fp, per-node erho ([ne][8]),
  * per-integration-point force ([ne*nint][3]).  Return CAC_ERR_ARG if the

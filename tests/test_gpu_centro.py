@@ -180,3 +180,29 @@ def test_centro_argument_errors(cu_funcfl):
             eng.centro(nnn=10)
     finally:
         eng.close()
+
+
+def test_centro_of_interpolated_sites(orc, cu_funcfl):
+    # the reference's executed path (verlet.cpp:91-94): atoms replaced by the
+    # interpolated element sites, list built over them, their centro-symmetry
+    s = small_system(seed=13, L=10, nae=5, core=4.0, sigma=0.03)
+    print(f"[centro] interpolated sites: elements={s.n_elements} nae={list(s.nae)}", flush=True)
+    eng = engine_for(s, cu_funcfl)
+    try:
+        xs = eng.interpolated_sites()
+        xs2, c = eng.centro_of_sites()
+    finally:
+        eng.close()
+    spa = orc.site_shapes(s.nae)                        # [nsite][8], inter_ele order
+    ref = np.zeros((s.n_elements, spa.shape[0], 3))
+    for k in range(8):                                  # node order from 0.0, unfused
+        ref = ref + spa[None, :, k, None] * s.nodex[:, None, k, :]
+    np.testing.assert_allclose(xs, ref.reshape(-1, 3), rtol=1e-14, atol=1e-12)
+    assert np.array_equal(xs, xs2)
+    w = np.ascontiguousarray(xs.copy())
+    orc.pbc_atoms(w, s.box_lo, s.box_hi)
+    num, nb = orc.atom_neighbors(w, s.box_lo, s.box_hi, CUTNEIGH_EAM, 200)
+    cref = orc.centro_atoms(w, s.box_lo, s.box_hi, num, nb, CUTNEIGH_EAM, 12)
+    np.testing.assert_allclose(c, cref, rtol=RTOL, atol=ATOL)
+    # the atomistic core is gone: its surface is in the dump window
+    assert np.count_nonzero((cref > 1) & (cref < 10)) > 0
