@@ -56,6 +56,7 @@ class CACSystem:
     blocks: np.ndarray = field(default=None)  # [ne,3] block indices (mesh adjacency)
     a0: float = 3.615
     cells: int = 0
+    node_tags: np.ndarray = field(default=None)  # [ne,8] mesh vertex ids (data file), when blocks are unknown
 
     @property
     def n_atoms(self) -> int:
@@ -182,15 +183,21 @@ def vertex_clusters(s: CACSystem):
     block-grid vertex V = B + c_k: the 8 elements meeting at V put their
     nodes on the lattice points nae*V - c' around it.  Vertices are matched
     by the lattice point nae*V wrapped into the periodic box (integer
-    arithmetic in units of a/2); clusters are ordered by their first slot."""
-    if s.blocks is None or s.n_elements == 0:
+    arithmetic in units of a/2); clusters are ordered by their first slot.
+    Without block indices the node tags of a data file name the vertices."""
+    if s.n_elements == 0:
         return None
-    nae = int(s.nae[0])
-    V = s.blocks[:, None, :].astype(np.int64) + NODE_CORNERS[None, :, :]          # [ne, 8, 3]
-    n = nae * V.reshape(-1, 3)
-    c2 = np.stack([n[:, 0] + n[:, 1], n[:, 0] + n[:, 2], n[:, 1] + n[:, 2]], axis=1)  # a/2 units
-    c2 = np.mod(c2, 2 * s.cells)
-    key = (c2[:, 2] * (2 * s.cells) + c2[:, 1]) * (2 * s.cells) + c2[:, 0]
+    if s.blocks is None:
+        if s.node_tags is None:
+            return None
+        key = np.asarray(s.node_tags, dtype=np.int64).reshape(-1)
+    else:
+        nae = int(s.nae[0])
+        V = s.blocks[:, None, :].astype(np.int64) + NODE_CORNERS[None, :, :]          # [ne, 8, 3]
+        n = nae * V.reshape(-1, 3)
+        c2 = np.stack([n[:, 0] + n[:, 1], n[:, 0] + n[:, 2], n[:, 1] + n[:, 2]], axis=1)  # a/2 units
+        c2 = np.mod(c2, 2 * s.cells)
+        key = (c2[:, 2] * (2 * s.cells) + c2[:, 1]) * (2 * s.cells) + c2[:, 0]
     order = np.argsort(key, kind="stable")
     ks = key[order]
     cut = np.flatnonzero(np.diff(ks)) + 1
