@@ -248,6 +248,10 @@ class Engine(_MlpMixin):
         self.nint = 0
         self.pair = None
         self.sys: Optional[CACSystem] = None
+        self.box = None
+        self.mass = 0.0
+        self.nsite = 0
+        self.neighbor_args = None
 
     def close(self):
         if self.ctx:
@@ -272,18 +276,21 @@ class Engine(_MlpMixin):
         hi = np.ascontiguousarray(hi, dtype=np.float64)
         per = np.ones(3, dtype=np.int32)
         _check(self.ctx, self.L.cac_set_box(self.ctx, _p(lo), _p(hi), _p(per, _lib._i)))
+        self.box = (lo.copy(), hi.copy())
 
     def upload_atoms(self, x, v, mass):
         x = np.ascontiguousarray(x, dtype=np.float64)
         vv = None if v is None else np.ascontiguousarray(v, dtype=np.float64)
         self.n = x.shape[0]
         _check(self.ctx, self.L.cac_upload_atoms(self.ctx, self.n, _p(x), _p(vv) if vv is not None else None, mass))
+        self.mass = mass
 
     def upload_elements(self, nodex, nodev, nae, mass_e, intp_ijk, intp_w):
         nodex = np.ascontiguousarray(nodex, dtype=np.float64)
         nv = None if nodev is None else np.ascontiguousarray(nodev, dtype=np.float64)
         nae_c = np.ascontiguousarray(nae, dtype=np.int32)
         self.ne = nodex.shape[0]
+        self.nsite = int(np.prod(nae_c))
         _check(self.ctx, self.L.cac_upload_elements(self.ctx, self.ne, _p(nae_c, _lib._i), _p(nodex),
                                                     _p(nv) if nv is not None else None, mass_e))
         ijk = np.ascontiguousarray(intp_ijk, dtype=np.int32)
@@ -423,8 +430,7 @@ class Engine(_MlpMixin):
     def interpolated_sites(self) -> np.ndarray:
         """Every site of every element, [ne * nsite][3] in element and
         inter_ele order (Element::interpolate_atom_vec, element.cpp:1241-1265)."""
-        nsite = int(np.prod(self.sys.nae)) if self.sys is not None else 0
-        x = np.zeros((self.ne * nsite, 3))
+        x = np.zeros((self.ne * self.nsite, 3))
         if x.size:
             _check(self.ctx, self.L.cac_interpolate_sites(self.ctx, _p(x)))
         return x
@@ -435,9 +441,11 @@ class Engine(_MlpMixin):
         the atom list is built over them and their centro-symmetry computed.
         Runs in a second context with this one's box, pair style and neighbor
         settings; returns (site positions, values)."""
+        if self.box is None or self.pair is None or self.neighbor_args is None:
+            raise PairError("centro_of_sites needs the box, the pair style and the neighbor settings")
         xs = self.interpolated_sites()
         with Engine(device) as e2:
-            e2.set_box(self.sys.box_lo, self.sys.box_hi)
+            e2.set_box(*self.box)
             e2.upload_atoms(xs, None, self.mass)
             e2.set_pair(self.pair)
             cutneigh, skin, every, delay, check = self.neighbor_args
