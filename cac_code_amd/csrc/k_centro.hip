@@ -27,6 +27,7 @@ namespace cac {
 namespace {
 
 constexpr int kCentroBlock = 256;
+constexpr int kCU = 4;  // list entries fetched together
 
 __device__ __forceinline__ void img_shift(int img, int s[3]) {
   s[0] = img % 3 - 1;
@@ -81,29 +82,40 @@ __global__ __launch_bounds__(kCentroBlock) void k_centro(int n, const double *__
   const int jn = num[i];
   const uint32_t *row = neigh + atom_slot(i, 0, M);
   int cnt = 0;
-  for (int k = 0; k < jn; ++k) {
-    const uint32_t e = row[(size_t)k * kAtomGroup];
-    double xj, yj, zj;
-    entry_xyz(x4, e, prd, &xj, &yj, &zj);
-    const double delx = xtmp - xj, dely = ytmp - yj, delz = ztmp - zj;
-    const double rsq = delx * delx + dely * dely + delz * delz;
-    if (!(rsq < cutsq)) continue;
-    ++cnt;
-    if (!key_less(rsq, e, d[NNN - 1], ek[NNN - 1])) continue;
-    // insert, keeping d ascending by (r^2, entry)
+  // kCU entries at a time: their list words, then their position gathers,
+  // are issued before any arithmetic, so several loads are in flight
+  for (int k0 = 0; k0 < jn; k0 += kCU) {
+    uint32_t eb[kCU];
+    double xb[kCU], yb[kCU], zb[kCU];
 #pragma unroll
-    for (int s = NNN - 1; s > 0; --s) {
-      if (key_less(rsq, e, d[s - 1], ek[s - 1])) {
-        d[s] = d[s - 1];
-        ek[s] = ek[s - 1];
-      } else if (key_less(rsq, e, d[s], ek[s])) {
-        d[s] = rsq;
-        ek[s] = e;
+    for (int u = 0; u < kCU; ++u) eb[u] = k0 + u < jn ? row[(size_t)(k0 + u) * kAtomGroup] : 0xFFFFFFFFu;
+#pragma unroll
+    for (int u = 0; u < kCU; ++u)
+      if (eb[u] != 0xFFFFFFFFu) entry_xyz(x4, eb[u], prd, &xb[u], &yb[u], &zb[u]);
+#pragma unroll
+    for (int u = 0; u < kCU; ++u) {
+      if (eb[u] == 0xFFFFFFFFu) continue;
+      const uint32_t e = eb[u];
+      const double delx = xtmp - xb[u], dely = ytmp - yb[u], delz = ztmp - zb[u];
+      const double rsq = delx * delx + dely * dely + delz * delz;
+      if (!(rsq < cutsq)) continue;
+      ++cnt;
+      if (!key_less(rsq, e, d[NNN - 1], ek[NNN - 1])) continue;
+      // insert, keeping d ascending by (r^2, entry)
+#pragma unroll
+      for (int s = NNN - 1; s > 0; --s) {
+        if (key_less(rsq, e, d[s - 1], ek[s - 1])) {
+          d[s] = d[s - 1];
+          ek[s] = ek[s - 1];
+        } else if (key_less(rsq, e, d[s], ek[s])) {
+          d[s] = rsq;
+          ek[s] = e;
+        }
       }
-    }
-    if (key_less(rsq, e, d[0], ek[0])) {
-      d[0] = rsq;
-      ek[0] = e;
+      if (key_less(rsq, e, d[0], ek[0])) {
+        d[0] = rsq;
+        ek[0] = e;
+      }
     }
   }
   if (cnt < NNN) {
