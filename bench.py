@@ -45,7 +45,7 @@ B_ELEM = 7640
 # HBM bytes per launch of the FP64 kernels on the eam_c1 workload, from
 # rocprofv3 FETCH_SIZE and WRITE_SIZE runs scaled by a plain 1 GiB copy's
 # counts (fetch x2.000, write x1.000); the summary is profiles/eam_c1_kernels.md.
-TRAFFIC = {"atom_force": 187.7e6 + 11.9e6, "atom_rho": 190.1e6 + 26.0e6, "ip_force": 148.6e6 + 19.0e6,
+TRAFFIC = {"atom_force": 187.7e6 + 11.9e6, "atom_rho": 190.1e6 + 26.0e6, "ip_force": 148.6e6 + 19.1e6,
            "ip_erho": 131.0e6 + 12.2e6}
 
 
