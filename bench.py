@@ -33,7 +33,10 @@ if ROOT not in sys.path:
 HBM_PEAK_GBS = 8000.0
 K_RC = 42  # fcc Cu neighbours inside rc = 4.95 A (12 + 6 + 24)
 
-# algorithmic bytes per unit for each kernel family
+# Algorithmic bytes per unit for each kernel family: compulsory FP64 state
+# (positions 24 B, F' 8 B, forces 24 B per atom; 8 nodes x 24 B, erho 8 x 8 B
+# per element) plus 4-byte atom-list entries, or 8-byte integration-point
+# entries for 8 points, for the K_RC neighbors inside the force cutoff.
 ALG_BYTES = {
     "atom_rho": ("atoms", 24 + 8 + 4 * K_RC),
     "atom_force": ("atoms", 24 + 8 + 24 + 4 * K_RC),
@@ -368,7 +371,7 @@ def main():
         return
 
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU path.",
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the library has no multi-GPU path.",
               file=sys.stderr)
         sys.exit(2)
 

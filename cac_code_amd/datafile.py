@@ -335,7 +335,7 @@ def to_system(d: DataFile, center_tol: float = 0.1, a0: float = 0.0) -> CACSyste
     (Element::element_size, element.cpp:608-622).  a0 (the fcc lattice
     constant, if known) only sets the atom-list order hint."""
     if d.ntypes != 1 or d.entypes != 1:
-        raise DataFileError("this build runs one atom type and one element type")
+        raise DataFileError("the device path runs one atom type and one element type")
     M = d.xc.shape[0]
     mass = float(d.masses.get(1, 0.0))
     if M:
