@@ -180,21 +180,40 @@ __global__ void k_cell_heads(int n, const uint32_t *__restrict__ key, int *__res
   if (p == n) hflag[p] = 0;
 }
 
-// Non-empty cells in cell order, and the largest candidate count of any.
+// Non-empty cells in cell order.
 __global__ void k_cell_list(int n, const uint32_t *__restrict__ key, const int *__restrict__ hflag,
-                            const int *__restrict__ hpos, Grid3 g, const int *__restrict__ cs,
-                            const int *__restrict__ ce, int *__restrict__ cells, int *__restrict__ maxcand) {
+                            const int *__restrict__ hpos, int *__restrict__ cells) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n || !hflag[p]) return;
-  const int c = (int)key[p];
-  cells[hpos[p]] = c;
-  int ci[3], tot = 0, img;
-  cell_split(c, g, ci);
-  for (int t = 0; t < 27; ++t) {
-    const int cn = cell_around(ci, t, g, &img);
-    tot += ce[cn] - cs[cn];
+  if (p < n && hflag[p]) cells[hpos[p]] = (int)key[p];
+}
+
+// The largest candidate count (atoms in the 27 cells around) of any
+// non-empty cell: one thread per cell of the list, grid-stride over the
+// count the scan left on the device, then a block maximum and one atomic
+// per block.  (Over all atoms, only the one thread in ~30 that starts a cell
+// had work, so each wavefront carried two lanes of scattered reads.)
+constexpr int kMaxcandBlocks = 2048;
+__global__ __launch_bounds__(kBlock) void k_cell_maxcand(const int *__restrict__ ncells_p,
+                                                         const int *__restrict__ cells, Grid3 g,
+                                                         const int *__restrict__ cs, const int *__restrict__ ce,
+                                                         int *__restrict__ maxcand) {
+  __shared__ int smax;
+  if (threadIdx.x == 0) smax = 0;
+  __syncthreads();
+  const int ncells = *ncells_p;
+  int best = 0;
+  for (int t = (int)(blockIdx.x * blockDim.x + threadIdx.x); t < ncells; t += (int)(gridDim.x * blockDim.x)) {
+    int ci[3], tot = 0, img;
+    cell_split(cells[t], g, ci);
+    for (int u = 0; u < 27; ++u) {
+      const int cn = cell_around(ci, u, g, &img);
+      tot += ce[cn] - cs[cn];
+    }
+    best = tot > best ? tot : best;
   }
-  atomicMax(maxcand, tot);
+  atomicMax(&smax, best);
+  __syncthreads();
+  if (threadIdx.x == 0 && smax > 0) atomicMax(maxcand, smax);
 }
 
 constexpr int kNeighWaves = 4;
@@ -1430,8 +1449,9 @@ int launch_atom_neighbors(cac_ctx *ctx) {
   if ((rc = ensure(ctx, ctx->b_scantmp, tmp_bytes, &tmp))) return rc;
   CAC_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, hflag, hpos, n + 1, ctx->stream));
   CAC_HIP(ctx, hipMemsetAsync(ctx->dflags + 13, 0, sizeof(int), ctx->stream));
-  hipLaunchKernelGGL(k_cell_list, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, n, key2, hflag, hpos, g, cs, ce,
-                     cells, ctx->dflags + 13);
+  hipLaunchKernelGGL(k_cell_list, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, n, key2, hflag, hpos, cells);
+  hipLaunchKernelGGL(k_cell_maxcand, dim3(kMaxcandBlocks), dim3(kBlock), 0, ctx->stream, hpos + n, cells, g, cs, ce,
+                     ctx->dflags + 13);
   int hv[2] = {0, 0};
   CAC_HIP(ctx, hipMemcpyAsync(&hv[0], hpos + n, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   CAC_HIP(ctx, hipMemcpyAsync(&hv[1], ctx->dflags + 13, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
