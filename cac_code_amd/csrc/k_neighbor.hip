@@ -1785,7 +1785,9 @@ __global__ void k_mark_aref(int niface, int maxneigh, const int *__restrict__ if
                             const uint32_t *__restrict__ aneigh, int n, int *__restrict__ mark) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)niface * maxneigh) return;
-  const int r = (int)(t / maxneigh), k = (int)(t % maxneigh);
+  // consecutive threads take consecutive interface atoms at one entry k: the
+  // atoms of a 64-row group share the line that holds their k-th entries
+  const int k = (int)(t / niface), r = (int)(t % niface);
   const int i = iface[r];
   if (k >= anum[i]) return;
   const uint32_t j = aneigh[atom_slot(i, k, maxneigh)] & 0x7FFFFFFu;
