@@ -380,6 +380,8 @@ int force_impl(cac_ctx *ctx, int eflag, int vflag, int precision) {
 
 int build_impl(cac_ctx *ctx) {
   int rc;
+  // a failed build leaves no usable lists (capacities may have grown)
+  ctx->built = false;
   if (!ctx->neigh_set) return fail(ctx, CAC_ERR_ARG, "cac_set_neighbor not called");
   if (!ctx->box_set) return fail(ctx, CAC_ERR_ARG, "cac_set_box not called");
   if (ctx->ne > 0 && !ctx->quad_set) return fail(ctx, CAC_ERR_ARG, "cac_set_quadrature not called");
@@ -403,10 +405,12 @@ int build_impl(cac_ctx *ctx) {
     if (again) continue;
     if ((rc = launch_active_sites(ctx))) return rc;
     if ((rc = launch_clusters(ctx))) return rc;
+    if ((rc = launch_reaction_lists(ctx))) return rc;
     if (ctx->surr.enabled && (rc = surrogate_classify(ctx))) return rc;
     ctx->built = true;
     ctx->ago = 0;
     ctx->ncalls++;
+    for (int d = 0; d < 3; ++d) ctx->build_prod[d] = 1.0;
     // force passes queued after this run in full (see raise_flag)
     CAC_HIP(ctx, hipMemsetAsync(ctx->dflags + 15, 0, sizeof(int), ctx->stream));
     return invalidate_inner(ctx);
@@ -414,15 +418,32 @@ int build_impl(cac_ctx *ctx) {
   return fail(ctx, CAC_ERR_OVERFLOW, "neighbor capacity still exceeded after regrowth");
 }
 
+// Rebuild trigger under remaps.  The held positions are remapped with the
+// positions, so the check measures motion alone; a compressive remap since
+// the build (scales prod_d < 1) brings a pair that was just outside cutneigh
+// to cutneigh * min_d prod_d, so each point may move only
+// sqrt(triggersq) - eps*cutneigh/2 with eps = max_d (1 - prod_d) before a
+// pair could enter the force cutoff unlisted.  Returns that squared
+// allowance, or -1 when the remap alone has used it up (rebuild now).
+double effective_triggersq(const cac_ctx *ctx) {
+  double eps = 0.0;
+  for (int d = 0; d < 3; ++d) eps = fmax(eps, 1.0 - ctx->build_prod[d]);
+  if (eps <= 0.0) return ctx->triggersq;
+  const double trig = sqrt(ctx->triggersq) - 0.5 * eps * ctx->cutneigh;
+  return trig > 0.0 ? trig * trig : -1.0;
+}
+
 int decide_impl(cac_ctx *ctx, int *rebuild) {
   // Neighbor::decide, neighbor.cpp:1150-1163
   ctx->ago++;
   *rebuild = 0;
+  const double trigsq = effective_triggersq(ctx);
+  if (trigsq < 0.0) { *rebuild = 1; return CAC_OK; }
   if (ctx->ago >= ctx->delay && ctx->ago % ctx->every == 0) {
     if (!ctx->check) { *rebuild = 1; return CAC_OK; }
     int rc;
     if ((rc = clear_flags(ctx))) return rc;
-    if ((rc = launch_check_displacement(ctx))) return rc;
+    if ((rc = launch_check_displacement(ctx, trigsq))) return rc;
     if ((rc = read_flags(ctx))) return rc;
     *rebuild = ctx->hflags[6] ? 1 : 0;
   }
@@ -460,7 +481,8 @@ int ensure_step_sync(cac_ctx *ctx) {
 
 // One step boundary of cac_run (k_nve_step); fin: apply the
 // previous step's final half-step first; remap: the step's affine remap.
-int nve_step_impl(cac_ctx *ctx, double dtv, double dtf, bool fin, int *flag, const Scale3 *remap = nullptr) {
+int nve_step_impl(cac_ctx *ctx, double dtv, double dtf, bool fin, int *flag, double triggersq,
+                  const Scale3 *remap = nullptr) {
   Scale3 m = {};
   if (remap) m = *remap;
   // inner-mask motion check (k_eam_atom_rho) with the held positions of
@@ -499,7 +521,7 @@ int nve_step_impl(cac_ctx *ctx, double dtv, double dtf, bool fin, int *flag, con
   a.dtfm = ctx->n ? dtf / ctx->mass : 0.0;
   double inv_npe = 1 / (double)8;
   a.dtfm_node = ctx->ne ? dtf / (inv_npe * ctx->mass_e) : 0.0;
-  a.triggersq = ctx->triggersq;
+  a.triggersq = triggersq;
   a.thr_in = thr * thr;
   a.flag = flag;
   a.abort = ctx->dflags + 15;
@@ -526,6 +548,7 @@ int deform_impl(cac_ctx *ctx, const double scale[3]) {
   for (int d = 0; d < 3; ++d) {
     ctx->prd[d] *= scale[d];
     ctx->hi[d] = ctx->lo[d] + ctx->prd[d];
+    ctx->build_prod[d] *= scale[d];
   }
   return CAC_OK;
 }
@@ -546,6 +569,7 @@ int deform_step(cac_ctx *ctx, double dtv, Scale3 *m) {
     m->s[d] = scale;
     ctx->prd[d] *= scale;
     ctx->hi[d] = ctx->lo[d] + ctx->prd[d];
+    ctx->build_prod[d] *= scale;
   }
   return CAC_OK;
 }
@@ -649,7 +673,8 @@ int cac_ctx_destroy(cac_ctx *ctx) {
                     &ctx->b_eval2, &ctx->b_ecs, &ctx->b_ece, &ctx->b_ipneigh, &ctx->b_ipnum, &ctx->b_erho,
                     &ctx->b_iptmp, &ctx->b_bitmap, &ctx->b_wprefix, &ctx->b_ecount, &ctx->b_estart, &ctx->b_asite,
                     &ctx->b_scantmp, &ctx->b_iflag, &ctx->b_ipos, &ctx->b_iface, &ctx->b_asnum, &ctx->b_atmp,
-                    &ctx->b_etouch, &ctx->b_amask, &ctx->b_ipmask, &ctx->b_cl_start, &ctx->b_cl_slot,
+                    &ctx->b_etouch, &ctx->b_rcount, &ctx->b_rstart, &ctx->b_rkey, &ctx->b_rsite,
+                    &ctx->b_relem, &ctx->b_rel_start, &ctx->b_rforce, &ctx->b_rtmp, &ctx->b_amask, &ctx->b_ipmask, &ctx->b_cl_start, &ctx->b_cl_slot,
                     &ctx->b_cl_union, &ctx->b_cl_nu, &ctx->b_iplocal, &ctx->b_aref, &ctx->b_amark, &ctx->b_cl_order,
                     &ctx->b_ipf, &ctx->b_eip, &ctx->b_vip, &ctx->b_red, &ctx->b_dflags, &ctx->eam.b_rz,
                     &ctx->eam.b_fr, &ctx->eam.b_rz_f, &ctx->eam.b_fr_f, &ctx->eam.b_rv, &ctx->eam.b_rv_f, &ctx->eam.b_rk};
@@ -728,6 +753,12 @@ int cac_upload_elements(cac_ctx *ctx, int ne, const int nae[3], const double *no
   if ((long)nae[0] * nae[1] * nae[2] >= 65536) return fail(ctx, CAC_ERR_ARG, "at most 65535 sites per element");
   hipSetDevice(ctx->device);
   const bool new_type = (nae[0] != ctx->nae[0] || nae[1] != ctx->nae[1] || nae[2] != ctx->nae[2]);
+  if (ne != ctx->ne) {
+    // vertex clusters and the surrogate's tables describe the old mesh
+    ctx->ncl = 0;
+    ctx->cl_ok = false;
+    ctx->surr.setup = ctx->surr.enabled = ctx->surr.ready = false;
+  }
   ctx->ne = ne;
   for (int d = 0; d < 3; ++d) ctx->nae[d] = nae[d];
   ctx->nsite = nae[0] * nae[1] * nae[2];
@@ -746,6 +777,8 @@ int cac_upload_elements(cac_ctx *ctx, int ne, const int nae[3], const double *no
       CAC_HIP(ctx, hipMemset(ctx->nodev, 0, nq * sizeof(double)));
     CAC_HIP(ctx, hipMemset(ctx->nodef, 0, nq * sizeof(double)));
   }
+  if (ctx->quad_set && ctx->nint > 0 && (rc = ensure(ctx, ctx->b_ipf, 3 * (size_t)ne * ctx->nint, &ctx->ipf)))
+    return rc;
   // site shape functions, Element::inter_ele + shape_array_inter_ele
   // (element.cpp:1361-1423): parent coordinate 2/(n-1)*i - 1, x outermost.
   {
@@ -1011,14 +1044,19 @@ int cac_run(cac_ctx *ctx, int nsteps, double dtv, double dtf, int precision, int
     // Neighbor::decide, neighbor.cpp:1150-1163
     ctx->ago++;
     const bool due = ctx->ago >= ctx->delay && ctx->ago % ctx->every == 0;
-    const bool probe = due && ctx->check;
     Scale3 m;
     if (ctx->deform_on && (rc = deform_step(ctx, dtv, &m))) return rc;
+    // a compressive remap can use up the whole skin: then rebuild whatever
+    // every/delay say (see effective_triggersq)
+    const double trigsq = effective_triggersq(ctx);
+    const bool forced = trigsq < 0.0;
+    const bool probe = due && ctx->check && !forced;
     if (probe) *ctx->hcheck = 0;
-    if ((rc = nve_step_impl(ctx, dtv, dtf, step > 0, probe ? ctx->dcheck : nullptr, ctx->deform_on ? &m : nullptr)))
+    if ((rc = nve_step_impl(ctx, dtv, dtf, step > 0, probe ? ctx->dcheck : nullptr, trigsq,
+                            ctx->deform_on ? &m : nullptr)))
       return rc;
     if (probe) CAC_HIP(ctx, hipEventRecord(ctx->ev_check, ctx->stream));
-    if (due && !ctx->check) {
+    if ((due && !ctx->check) || forced) {
       if ((rc = build_impl(ctx))) return rc;
       ++builds;
     }

@@ -168,6 +168,18 @@ struct cac_ctx {
   uint64_t *atmp = nullptr;  // [niface][maxneigh] raw (element | img<<27 | site<<32)
   int *etouch = nullptr;     // [ne] element has an atom in a list or a site in an atom list
   cac::DevBuf b_iflag, b_ipos, b_iface, b_asnum, b_atmp, b_etouch;
+  // Reactions of the atom-site pair forces (coupling): per active site the
+  // atoms whose lists hold it, as sorted entries (atom | img << 27) in
+  // rkey[rstart[a] .. rstart[a+1]); the sites with at least one, ascending
+  // (rsite[nrs]); the elements owning them (relem[nre]) with their range
+  // rel_start[r] .. rel_start[r+1] in rsite; per step the summed reaction of
+  // each such site (rforce [3][nrs]).  Built at every coupled rebuild.
+  int nrs = 0, nre = 0;
+  long nrev = 0;
+  int *rcount = nullptr, *rstart = nullptr, *rsite = nullptr, *relem = nullptr, *rel_start = nullptr;
+  uint32_t *rkey = nullptr;
+  double *rforce = nullptr;
+  cac::DevBuf b_rcount, b_rstart, b_rkey, b_rsite, b_relem, b_rel_start, b_rforce, b_rtmp;
 
   // ---- elements (one type)
   int ne = 0, nae[3] = {0, 0, 0}, nsite = 0;
@@ -267,6 +279,7 @@ struct cac_ctx {
   int every = 1, delay = 0, check = 1;
   bool neigh_set = false, built = false;
   int ago = 0;
+  double build_prod[3] = {1.0, 1.0, 1.0};  // remap scales applied since the last build
   long ncalls = 0;
 
   // ---- energy / virial of the last force call
@@ -365,8 +378,9 @@ int launch_ip_neighbors(cac_ctx *ctx);
 int launch_atom_sites(cac_ctx *ctx);
 int launch_active_sites(cac_ctx *ctx);
 int launch_clusters(cac_ctx *ctx);
+int launch_reaction_lists(cac_ctx *ctx);
 int validate_lists(cac_ctx *ctx);
-int launch_check_displacement(cac_ctx *ctx);
+int launch_check_displacement(cac_ctx *ctx, double triggersq);
 
 // ---- launchers (k_pair.hip): all passes of one force call, atoms and
 // elements interleaved in the order the coupled lists need

@@ -1924,11 +1924,150 @@ int launch_clusters(cac_ctx *ctx) {
   return CAC_OK;
 }
 
-int launch_check_displacement(cac_ctx *ctx) {
+// ------------------------------------------------------------ reactions
+// Reverse lists of the atom-site pairs (coupling): every atom list entry
+// that names an active site (particle row >= n) is counted for that site,
+// the counts are scanned into ranges, the ranges filled with the entries
+// (atom | img << 27, the image the atom's list gives the site) and each range
+// sorted, so the per-step reaction sum has a fixed order.  The traversal is
+// k_mark_aref's: consecutive threads take consecutive interface atoms at one
+// entry k.
+__global__ void k_rev_count(int niface, int maxneigh, const int *__restrict__ iface, const int *__restrict__ anum,
+                            const uint32_t *__restrict__ aneigh, int n, int *__restrict__ rcount) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)niface * maxneigh) return;
+  const int k = (int)(t / niface), r = (int)(t % niface);
+  const int i = iface[r];
+  if (k >= anum[i]) return;
+  const uint32_t j = aneigh[atom_slot(i, k, maxneigh)] & 0x7FFFFFFu;
+  if ((int)j >= n) atomicAdd(&rcount[j - n], 1);
+}
+
+__global__ void k_rev_fill(int niface, int maxneigh, const int *__restrict__ iface, const int *__restrict__ anum,
+                           const uint32_t *__restrict__ aneigh, int n, const int *__restrict__ rstart,
+                           int *__restrict__ rfill, uint32_t *__restrict__ rkey) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)niface * maxneigh) return;
+  const int k = (int)(t / niface), r = (int)(t % niface);
+  const int i = iface[r];
+  if (k >= anum[i]) return;
+  const uint32_t e = aneigh[atom_slot(i, k, maxneigh)];
+  const uint32_t j = e & 0x7FFFFFFu;
+  if ((int)j < n) return;
+  const int slot = rstart[j - n] + atomicAdd(&rfill[j - n], 1);
+  rkey[slot] = (uint32_t)i | (e & ~0x7FFFFFFu);
+}
+
+// One thread per active site: insertion sort of its range (a few tens of
+// entries), then flag[a] = 1 if the site has any.
+__global__ void k_rev_sort(int na, const int *__restrict__ rstart, uint32_t *__restrict__ rkey,
+                           int *__restrict__ flag) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= na) return;
+  const int b = rstart[a], e = rstart[a + 1];
+  for (int q = b + 1; q < e; ++q) {
+    const uint32_t v = rkey[q];
+    int p = q - 1;
+    while (p >= b && rkey[p] > v) {
+      rkey[p + 1] = rkey[p];
+      --p;
+    }
+    rkey[p + 1] = v;
+  }
+  flag[a] = e > b ? 1 : 0;
+}
+
+// Elements of the reaction sites (ascending site ids are grouped by
+// element): head flags, then the element list and its ranges in rsite.
+__global__ void k_rel_heads(int nrs, const int *__restrict__ rsite, const int2 *__restrict__ asite,
+                            int *__restrict__ head) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q > nrs) return;
+  if (q == nrs) { head[q] = 0; return; }
+  head[q] = (q == 0 || asite[rsite[q]].x != asite[rsite[q - 1]].x) ? 1 : 0;
+}
+
+__global__ void k_rel_compact(int nrs, const int *__restrict__ rsite, const int2 *__restrict__ asite,
+                              const int *__restrict__ head, const int *__restrict__ pos, int *__restrict__ relem,
+                              int *__restrict__ rel_start) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nrs) return;
+  if (head[q]) {
+    relem[pos[q]] = asite[rsite[q]].x;
+    rel_start[pos[q]] = q;
+  }
+  if (q == nrs - 1) rel_start[pos[nrs]] = nrs;
+}
+
+int launch_reaction_lists(cac_ctx *ctx) {
+  ctx->nrs = ctx->nre = 0;
+  ctx->nrev = 0;
+  const int na = ctx->nactive, n = ctx->n;
+  if (!ctx->coupled || ctx->niface == 0 || na == 0) return CAC_OK;
+  int rc;
+  if ((rc = ensure(ctx, ctx->b_rcount, (size_t)na + 1, &ctx->rcount)) ||
+      (rc = ensure(ctx, ctx->b_rstart, (size_t)na + 1, &ctx->rstart)) ||
+      (rc = ensure(ctx, ctx->b_rsite, (size_t)na, &ctx->rsite)))
+    return rc;
+  int *tmpi;
+  if ((rc = ensure(ctx, ctx->b_rtmp, 2 * ((size_t)na + 1), &tmpi))) return rc;
+  const long tot = (long)ctx->niface * ctx->maxneigh;
+  CAC_HIP(ctx, hipMemsetAsync(ctx->rcount, 0, ((size_t)na + 1) * sizeof(int), ctx->stream));
+  hipLaunchKernelGGL(k_rev_count, dim3(grid_for(tot)), dim3(kBlock), 0, ctx->stream, ctx->niface, ctx->maxneigh,
+                     ctx->iface, ctx->anum, ctx->aneigh, n, ctx->rcount);
+  size_t tmp_bytes = 0;
+  CAC_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, ctx->rcount, ctx->rstart, na + 1, ctx->stream));
+  char *tmp;
+  if ((rc = ensure(ctx, ctx->b_scantmp, tmp_bytes, &tmp))) return rc;
+  CAC_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, ctx->rcount, ctx->rstart, na + 1, ctx->stream));
+  int nrev = 0;
+  CAC_HIP(ctx, hipMemcpyAsync(&nrev, ctx->rstart + na, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->nrev = nrev;
+  if (nrev == 0) return CAC_OK;
+  if ((rc = ensure(ctx, ctx->b_rkey, (size_t)nrev, &ctx->rkey))) return rc;
+  CAC_HIP(ctx, hipMemsetAsync(ctx->rcount, 0, ((size_t)na + 1) * sizeof(int), ctx->stream));
+  hipLaunchKernelGGL(k_rev_fill, dim3(grid_for(tot)), dim3(kBlock), 0, ctx->stream, ctx->niface, ctx->maxneigh,
+                     ctx->iface, ctx->anum, ctx->aneigh, n, ctx->rstart, ctx->rcount, ctx->rkey);
+  // rcount[a] becomes the "has entries" flag, rcount[na] stays 0
+  hipLaunchKernelGGL(k_rev_sort, dim3(grid_for(na)), dim3(kBlock), 0, ctx->stream, na, ctx->rstart, ctx->rkey,
+                     ctx->rcount);
+  int *pos = tmpi;
+  CAC_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, ctx->rcount, pos, na + 1, ctx->stream));
+  hipLaunchKernelGGL(k_compact_marks, dim3(grid_for(na)), dim3(kBlock), 0, ctx->stream, na, ctx->rcount, pos,
+                     ctx->rsite);
+  int nrs = 0;
+  CAC_HIP(ctx, hipMemcpyAsync(&nrs, pos + na, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (nrs == 0) return CAC_OK;
+  if ((rc = ensure(ctx, ctx->b_relem, (size_t)nrs, &ctx->relem)) ||
+      (rc = ensure(ctx, ctx->b_rel_start, (size_t)nrs + 1, &ctx->rel_start)) ||
+      (rc = ensure(ctx, ctx->b_rforce, 3 * (size_t)nrs, &ctx->rforce)))
+    return rc;
+  // head flags [nrs + 1] in tmpi[0 ..], their exclusive sums after them
+  int *head = tmpi, *hpos = tmpi + (nrs + 1);
+  hipLaunchKernelGGL(k_rel_heads, dim3(grid_for((long)nrs + 1)), dim3(kBlock), 0, ctx->stream, nrs, ctx->rsite,
+                     ctx->asite, head);
+  tmp_bytes = 0;
+  CAC_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, head, hpos, nrs + 1, ctx->stream));
+  if ((rc = ensure(ctx, ctx->b_scantmp, tmp_bytes, &tmp))) return rc;
+  CAC_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, head, hpos, nrs + 1, ctx->stream));
+  hipLaunchKernelGGL(k_rel_compact, dim3(grid_for(nrs)), dim3(kBlock), 0, ctx->stream, nrs, ctx->rsite, ctx->asite,
+                     head, hpos, ctx->relem, ctx->rel_start);
+  int nre = 0;
+  CAC_HIP(ctx, hipMemcpyAsync(&nre, hpos + nrs, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  CAC_HIP(ctx, hipGetLastError());
+  ctx->nrs = nrs;
+  ctx->nre = nre;
+  return CAC_OK;
+}
+
+int launch_check_displacement(cac_ctx *ctx, double triggersq) {
   long tot = (long)ctx->n + 8L * ctx->ne;
   if (tot == 0) return CAC_OK;
   hipLaunchKernelGGL(k_check_disp, dim3(grid_for(tot)), dim3(kBlock), 0, ctx->stream, ctx->n, ctx->x4, ctx->xhold,
-                     ctx->ne, ctx->nodex, ctx->nodexhold, ctx->triggersq, ctx->dflags);
+                     ctx->ne, ctx->nodex, ctx->nodexhold, triggersq, ctx->dflags);
   CAC_HIP(ctx, hipGetLastError());
   return CAC_OK;
 }

@@ -343,13 +343,15 @@ struct Virial {
   __device__ void zero() {
     for (int c = 0; c < 6; ++c) v[c] = 0.0;
   }
-  __device__ void add(double dx, double dy, double dz, double fpair) {
-    v[0] += 0.5 * dx * dx * fpair;
-    v[1] += 0.5 * dy * dy * fpair;
-    v[2] += 0.5 * dz * dz * fpair;
-    v[3] += 0.5 * dx * dy * fpair;
-    v[4] += 0.5 * dx * dz * fpair;
-    v[5] += 0.5 * dy * dz * fpair;
+  // share: the part of the pair's virial this side tallies (1/2, or all of
+  // it for an atom-site pair, whose site side tallies nothing)
+  __device__ void add(double dx, double dy, double dz, double fpair, double share = 0.5) {
+    v[0] += share * dx * dx * fpair;
+    v[1] += share * dy * dy * fpair;
+    v[2] += share * dz * dz * fpair;
+    v[3] += share * dx * dy * fpair;
+    v[4] += share * dx * dz * fpair;
+    v[5] += share * dy * dz * fpair;
   }
   __device__ void reduce() {
     for (int c = 0; c < 6; ++c) v[c] = group_sum(v[c]);
@@ -561,9 +563,10 @@ __global__ __launch_bounds__(kAtomWG) void k_eam_atom_force(int n, const double 
           fx += delx * fpair;
           fy += dely * fpair;
           fz += delz * fpair;
-          if (EV) {
-            ep += 0.5 * (double)phi;
-            vir.add(delx, dely, delz, fpair);
+          if (EV) {  // an atom-site pair's whole energy and virial go to the atom
+            const double share = (e[u] & 0x7FFFFFFu) < (uint32_t)n ? 0.5 : 1.0;
+            ep += share * (double)phi;
+            vir.add(delx, dely, delz, fpair, share);
           }
         }
       }
@@ -616,9 +619,10 @@ __global__ __launch_bounds__(kBlock) void k_lj_atom_force(int n, const double *_
         fx += delx * fpair;
         fy += dely * fpair;
         fz += delz * fpair;
-        if (EV) {
-          ep += 0.5 * (double)(r6inv * (lj3 * r6inv - lj4));
-          vir.add(delx, dely, delz, fpair);
+        if (EV) {  // an atom-site pair's whole energy and virial go to the atom
+          const double share = (e[u] & 0x7FFFFFFu) < (uint32_t)n ? 0.5 : 1.0;
+          ep += share * (double)(r6inv * (lj3 * r6inv - lj4));
+          vir.add(delx, dely, delz, fpair, share);
         }
       }
     }
@@ -635,6 +639,7 @@ __global__ __launch_bounds__(kBlock) void k_lj_atom_force(int n, const double *_
 // ------------------------------------------------------------- elements
 struct ElemArgs {
   int ne, nint, nip, nactive, M;
+  int natoms;                // particle rows below natoms are atoms
   const double *nodex;       // [ne][8][3]
   const double *spa_site;    // [nsite][8]
   const double *spa_int;     // [nint][8]
@@ -789,9 +794,11 @@ __global__ __launch_bounds__(kBlock) void k_eam_ip_force(ElemArgs a, Prd prd, Ta
       const uint32_t sel = bits >> (j0 & 31);
       uint32_t e[kU];
       fetch_entries<kU>(row, k0, jn, e);
+      // atoms count in the node densities only: their pair forces reach the
+      // element as the reactions of the atom pass (k_site_react_*)
 #pragma unroll
       for (int u = 0; u < kU; ++u)
-        if (!((sel >> u) & 1u)) e[u] = kNoEntry;
+        if (!((sel >> u) & 1u) || (e[u] & 0x7FFFFFFu) < (uint32_t)a.natoms) e[u] = kNoEntry;
       Pt xjv[kU];
       double fpjv[kU];
 #pragma unroll
@@ -920,7 +927,8 @@ __device__ __forceinline__ void cl_fill(const ClArgs &c, int cl, int s0, int ns,
       x = xy.x;
       y = xy.y;
       z = zw.x;
-      w = zw.y;
+      // the force pass reads the atom's F'; the density pass marks atom rows
+      w = FP ? zw.y : 1.0;
     } else {
       const int2 js = a.asite[j - c.n];
       double N[8];
@@ -999,7 +1007,8 @@ __device__ __forceinline__ void cl_erho_body(const ClArgs &c, const ElemArgs &a,
         const double rsq = delx * delx + dely * dely + delz * delz;
         if (rsq < t.cutforcesq) {
           acc += (double)rho_of_r<P>(t, sqrt_p<P>(rsq));
-          bits |= 1u << ((j0 + u) & 31);
+          // the force pass takes element sites only (atom rows carry 1.0)
+          if (r[3] == 0.0) bits |= 1u << ((j0 + u) & 31);
         }
       }
       if (((j0 + kU) & 31) == 0 || k0 + kU * kLanes >= jn) {
@@ -1169,6 +1178,7 @@ __global__ __launch_bounds__(kBlock) void k_lj_ip_force(ElemArgs a, Prd prd, LjP
     const uint32_t *row = list_row(a.ipneigh, p, a.M);
     for (int kk = l; kk < jn; kk += kLanes) {
       const uint32_t e = row[(size_t)kk * kRowGroup];
+      if ((e & 0x7FFFFFFu) < (uint32_t)a.natoms) continue;  // atoms: reactions of the atom pass
       const Pt xj = load_xyz(px4, e, prd);
       const double delx = ix - xj.x, dely = iy - xj.y, delz = iz - xj.z;
       const double rsq = delx * delx + dely * dely + delz * delz;
@@ -1228,6 +1238,125 @@ __global__ void k_scatter(ElemArgs a, const double *__restrict__ ipf, double *__
   nodef[3 * (size_t)q + 2] = fz;
 }
 
+// Reactions of the atom-site pair forces (atom-element coupling).  The atom
+// pass gives atom i the force f_is of every element site s in its list; the
+// site takes -f_is, and site s passes it to its element's nodes through its
+// shape functions, nodef[e][k] += N_k(s) * R_s, so atoms and nodes receive
+// equal and opposite totals (sum_k N_k = 1).  k_site_react_* recompute f_is
+// for the atoms of each site's reverse list (rkey, ascending) with the atom
+// pass's own operations: del = x_i - (x_s + img*prd), the same pair
+// function and arguments in the same order.
+struct ReactArgs {
+  int nrs, n;
+  const int *rsite, *rstart;
+  const uint32_t *rkey;
+  double *R;  // [3][nrs]
+  const int *skip;
+};
+
+template <class P>
+__global__ __launch_bounds__(kBlock) void k_site_react_eam(ReactArgs r, const double *__restrict__ x4, Prd prd,
+                                                           Tabs<P> t) {
+  const int q = xcd_block() * kBlock + (int)threadIdx.x;
+  if (q >= r.nrs || *r.skip) return;
+  const Win<P> w = Win<P>{nullptr, t.nr + 2};  // no LDS window: every row from the tables
+  const int a = r.rsite[q];
+  const size_t row = (size_t)r.n + a;
+  const double2 sxy = reinterpret_cast<const double2 *>(x4)[2 * row];
+  const double2 szw = reinterpret_cast<const double2 *>(x4)[2 * row + 1];
+  const P fps = (P)szw.y;
+  double rx = 0.0, ry = 0.0, rz = 0.0;
+  for (int k = r.rstart[a]; k < r.rstart[a + 1]; ++k) {
+    const uint32_t key = r.rkey[k];
+    const size_t i = key & 0x7FFFFFFu;
+    const int img = (int)(key >> 27);
+    double sx = sxy.x, sy = sxy.y, sz = szw.x;
+    if (img != kImgSelf) {
+      int sh[3];
+      img_shift(img, sh);
+      sx = shifted(sx, sh[0], prd.v[0]);
+      sy = shifted(sy, sh[1], prd.v[1]);
+      sz = shifted(sz, sh[2], prd.v[2]);
+    }
+    const double2 ixy = reinterpret_cast<const double2 *>(x4)[2 * i];
+    const double2 izw = reinterpret_cast<const double2 *>(x4)[2 * i + 1];
+    const double delx = ixy.x - sx, dely = ixy.y - sy, delz = izw.x - sz;
+    const double rsq = delx * delx + dely * dely + delz * delz;
+    if (rsq < t.cutforcesq) {
+      P phi;
+      const double fpair = eam_fpair<P>(t, w, rsq, (P)izw.y, fps, &phi);
+      rx -= delx * fpair;
+      ry -= dely * fpair;
+      rz -= delz * fpair;
+    }
+  }
+  r.R[q] = rx;
+  r.R[(size_t)r.nrs + q] = ry;
+  r.R[2 * (size_t)r.nrs + q] = rz;
+}
+
+template <class P>
+__global__ __launch_bounds__(kBlock) void k_site_react_lj(ReactArgs r, const double *__restrict__ x4, Prd prd,
+                                                          LjParams lj) {
+  const int q = xcd_block() * kBlock + (int)threadIdx.x;
+  if (q >= r.nrs) return;
+  const P lj1 = (P)lj.lj1, lj2 = (P)lj.lj2;
+  const int a = r.rsite[q];
+  const size_t row = (size_t)r.n + a;
+  const double sx0 = x4[4 * row], sy0 = x4[4 * row + 1], sz0 = x4[4 * row + 2];
+  double rx = 0.0, ry = 0.0, rz = 0.0;
+  for (int k = r.rstart[a]; k < r.rstart[a + 1]; ++k) {
+    const uint32_t key = r.rkey[k];
+    const size_t i = key & 0x7FFFFFFu;
+    const int img = (int)(key >> 27);
+    double sx = sx0, sy = sy0, sz = sz0;
+    if (img != kImgSelf) {
+      int sh[3];
+      img_shift(img, sh);
+      sx = shifted(sx, sh[0], prd.v[0]);
+      sy = shifted(sy, sh[1], prd.v[1]);
+      sz = shifted(sz, sh[2], prd.v[2]);
+    }
+    const double delx = x4[4 * i] - sx, dely = x4[4 * i + 1] - sy, delz = x4[4 * i + 2] - sz;
+    const double rsq = delx * delx + dely * dely + delz * delz;
+    if (rsq < lj.cutsq) {
+      P r2inv = P(1.0) / (P)rsq;
+      P r6inv = r2inv * r2inv * r2inv;
+      P forcelj = r6inv * (lj1 * r6inv - lj2);
+      const double fpair = (double)(P(1.0) * forcelj * r2inv);
+      rx -= delx * fpair;
+      ry -= dely * fpair;
+      rz -= delz * fpair;
+    }
+  }
+  r.R[q] = rx;
+  r.R[(size_t)r.nrs + q] = ry;
+  r.R[2 * (size_t)r.nrs + q] = rz;
+}
+
+// 8 threads per element with reaction sites (one per node): the node adds
+// N_k(s) * R_s over the element's reaction sites in ascending order, after
+// its integration-point term.
+__global__ __launch_bounds__(kBlock) void k_site_react_nodes(ReactArgs r, int nre, const int *__restrict__ relem,
+                                                             const int *__restrict__ rel_start, ElemArgs a,
+                                                             double *__restrict__ nodef) {
+  const int t = blockIdx.x * blockDim.x + (int)threadIdx.x;
+  if ((t >> 3) >= nre || *r.skip) return;
+  const int ri = t >> 3, k = t & 7;
+  const size_t b = 24 * (size_t)relem[ri] + 3 * k;
+  double fx = nodef[b], fy = nodef[b + 1], fz = nodef[b + 2];
+  for (int q = rel_start[ri]; q < rel_start[ri + 1]; ++q) {
+    double N[8];
+    site_shape(a, a.asite[r.rsite[q]].y, N);
+    fx += N[k] * r.R[q];
+    fy += N[k] * r.R[(size_t)r.nrs + q];
+    fz += N[k] * r.R[2 * (size_t)r.nrs + q];
+  }
+  nodef[b] = fx;
+  nodef[b + 1] = fy;
+  nodef[b + 2] = fz;
+}
+
 inline unsigned grid_for(long n, int block = kBlock) { return (unsigned)((n + block - 1) / block); }
 
 Prd prd_of(const cac_ctx *ctx) {
@@ -1275,6 +1404,7 @@ ElemArgs elem_args(const cac_ctx *ctx) {
   a.nint = ctx->nint;
   a.nip = ctx->ne * ctx->nint;
   a.nactive = ctx->nactive;
+  a.natoms = ctx->n;
   a.M = ctx->maxnip;
   a.nodex = ctx->nodex;
   a.spa_site = ctx->spa_site;
@@ -1325,6 +1455,33 @@ int scatter(cac_ctx *ctx, const ElemArgs &a) {
   timer_begin(ctx, "scatter", &t0);
   hipLaunchKernelGGL(k_scatter, dim3(grid_for(8L * nf)), dim3(kBlock), 0, ctx->stream, a, ctx->ipf, ctx->nodef);
   timer_end(ctx, "scatter", t0);
+  return CAC_OK;
+}
+
+// The reactions of one force call, after every node force is written.
+template <class P>
+int site_reactions(cac_ctx *ctx, bool eam, const ElemArgs &a) {
+  if (ctx->nrs == 0 || ctx->nre == 0) return CAC_OK;
+  ReactArgs r;
+  r.nrs = ctx->nrs;
+  r.n = ctx->n;
+  r.rsite = ctx->rsite;
+  r.rstart = ctx->rstart;
+  r.rkey = ctx->rkey;
+  r.R = ctx->rforce;
+  r.skip = ctx->dflags + 15;
+  hipEvent_t t0;
+  timer_begin(ctx, "site_react", &t0);
+  if (eam)
+    hipLaunchKernelGGL((k_site_react_eam<P>), dim3(xcd_grid(ctx->nrs, kBlock)), dim3(kBlock), 0, ctx->stream, r,
+                       ctx->x4, prd_of(ctx), tabs_of<P>(ctx));
+  else
+    hipLaunchKernelGGL((k_site_react_lj<P>), dim3(xcd_grid(ctx->nrs, kBlock)), dim3(kBlock), 0, ctx->stream, r,
+                       ctx->x4, prd_of(ctx), ctx->lj);
+  hipLaunchKernelGGL(k_site_react_nodes, dim3(grid_for(8L * ctx->nre)), dim3(kBlock), 0, ctx->stream, r, ctx->nre,
+                     ctx->relem, ctx->rel_start, a, ctx->nodef);
+  timer_end(ctx, "site_react", t0);
+  CAC_HIP(ctx, hipGetLastError());
   return CAC_OK;
 }
 
@@ -1486,7 +1643,7 @@ int eam_clusters(cac_ctx *ctx, const ElemArgs &base) {
                        ctx->nodef);
   timer_end(ctx, "ip_force", t0);
   CAC_HIP(ctx, hipGetLastError());
-  return CAC_OK;
+  return site_reactions<P>(ctx, true, base);
 }
 
 template <class P, bool EV>
@@ -1551,10 +1708,13 @@ int eam_t(cac_ctx *ctx) {
     int rc = eam_all<P, false>(ctx, a, with_sites(a, S.siteR2, S.nsR2), with_elems(a, S.elemR, S.nR),
                                with_sites(a, S.siteR, S.nsR), with_elems(a, S.elemF, S.nF));
     if (rc) return rc;
-    return surrogate_forces(ctx);
+    if ((rc = surrogate_forces(ctx))) return rc;
+    return site_reactions<P>(ctx, true, a);
   }
   if (ctx->ne && ctx->cl_ok) return eam_clusters<P, EV>(ctx, a);
-  return eam_all<P, EV>(ctx, a, a, a, a, a);
+  int rc = eam_all<P, EV>(ctx, a, a, a, a, a);
+  if (rc) return rc;
+  return site_reactions<P>(ctx, true, a);
 }
 
 // One LJ force call (PairLJCut::compute + ecompute, pair_lj_cut.cpp:227-288,
@@ -1580,7 +1740,7 @@ int lj_t(cac_ctx *ctx) {
     scatter(ctx, a);
   }
   CAC_HIP(ctx, hipGetLastError());
-  return CAC_OK;
+  return site_reactions<P>(ctx, false, a);
 }
 
 }  // namespace

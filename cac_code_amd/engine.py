@@ -310,8 +310,9 @@ class Engine(_MlpMixin):
         if getattr(s, "a0", None):
             self.set_list_order(0.5 * s.a0)                 # fcc neighbor vectors: multiples of a/2
         cl = vertex_clusters(s)
-        if cl is not None:
-            self.set_ip_clusters(*cl)
+        if cl is None:                                       # no mesh vertices known: no clusters
+            cl = (np.zeros(1, np.int32), np.zeros(0, np.int32))
+        self.set_ip_clusters(*cl)
 
     def set_ip_clusters(self, start, slots):
         """Group the integration points by mesh vertex (a work grouping for

@@ -117,9 +117,14 @@ int cac_set_list_order(cac_ctx *ctx, double spacing);
  * only atoms, pair_eam.cpp:584-746, and integration points only element
  * sites, pair_eam.cpp:927-1137).  With on != 0 the next build adds to every
  * atom list the interpolated element sites within cutneigh and to every
- * integration-point list the atoms within cutneigh; the pair terms stay the
- * reference's (atom pass formula for atoms, element pass formula for
- * integration points).  Off by default. */
+ * integration-point list the atoms within cutneigh.  Atoms take the atom
+ * pass's pair force from each site (the site's F' interpolated from its
+ * element's node densities); integration points count atoms in their
+ * densities but not in their forces; instead every site passes the
+ * reaction of its atom pairs to its element's nodes through its shape
+ * functions, so atom-element forces are equal and opposite (Newton's third
+ * law across the interface).  An atom-site pair's energy and virial are
+ * tallied on the atom.  Off by default. */
 int cac_set_coupling(cac_ctx *ctx, int on);
 
 /* Affine remap of the box and everything in it (no counterpart in the

@@ -1308,6 +1308,61 @@ int orc_coupled_neighbors(int n, const double *x, const orc_elems *el, const orc
   return overflow ? -1 : 0;
 }
 
+/* Reactions of the atom-site pair forces (build-defined coupling).  Every
+ * (atom i, site s of element j at image img) pair that exerts a force f_is
+ * on the atom puts -f_is on the site, and the site passes it to its nodes
+ * through its shape functions: nodef[j][k] += N_k(xi_s) * R_s, with
+ * R_s = sum of -f_is over the atoms in ascending entry order (atom | img<<27),
+ * sites of an element in ascending index.  Since sum_k N_k = 1 the atoms
+ * and the nodes receive equal and opposite totals for every such pair.
+ * fsite holds f_is of atom list slot (i, jj) where has[i*maxneigh + jj]. */
+typedef struct {
+  uint64_t skey;  /* element << 32 | site */
+  uint32_t akey;  /* atom | img << 27 */
+  size_t slot;    /* i * maxneigh + jj */
+} react_t;
+
+static int cmp_react(const void *a, const void *b) {
+  const react_t *p = (const react_t *)a, *q = (const react_t *)b;
+  if (p->skey != q->skey) return p->skey < q->skey ? -1 : 1;
+  return p->akey < q->akey ? -1 : p->akey > q->akey;
+}
+
+static void add_site_reactions(int n, const orc_elems *el, const int *anum, const uint64_t *anb, int maxneigh,
+                               const double *fsite, const unsigned char *has, double *nodef) {
+  size_t np = 0;
+  for (int i = 0; i < n; i++)
+    for (int jj = 0; jj < anum[i]; jj++) np += has[(size_t)i * maxneigh + jj];
+  if (np == 0) return;
+  react_t *r = (react_t *)malloc(sizeof(react_t) * np);
+  size_t q = 0;
+  for (int i = 0; i < n; i++)
+    for (int jj = 0; jj < anum[i]; jj++) {
+      size_t slot = (size_t)i * maxneigh + jj;
+      if (!has[slot]) continue;
+      uint64_t e = anb[slot];
+      r[q].skey = ((e & 0x7FFFFFFull) << 32) | (e >> 32);
+      r[q].akey = (uint32_t)i | (uint32_t)(((e >> 27) & 31) << 27);
+      r[q].slot = slot;
+      q++;
+    }
+  qsort(r, np, sizeof(react_t), cmp_react);
+  for (size_t a = 0; a < np;) {
+    size_t b = a;
+    double R[3] = {0.0, 0.0, 0.0};
+    while (b < np && r[b].skey == r[a].skey) {
+      for (int d = 0; d < 3; d++) R[d] -= fsite[3 * r[b].slot + d];
+      b++;
+    }
+    int j = (int)(r[a].skey >> 32), site = (int)(r[a].skey & 0xFFFFFFFFu);
+    const double *Ns = el->spa_site + 8 * site;
+    for (int k = 0; k < 8; k++)
+      for (int d = 0; d < 3; d++) nodef[24 * j + 3 * k + d] += Ns[k] * R[d];
+    a = b;
+  }
+  free(r);
+}
+
 /* EAM over atoms and elements together (the passes of pair_eam.cpp:584-746
  * and 927-1137 on the coupled lists):
  *   1. erho[e][k]: the list of the integration point at node k, from the
@@ -1315,15 +1370,25 @@ int orc_coupled_neighbors(int n, const double *x, const orc_elems *el, const orc
  *   2. rho_i, fp_i of atoms over their lists (atoms and sites);
  *   3. atom force: psip = fp_i*rho' + fp_j*rho' + phi', fp_j of a site
  *      F'(sum_k N_k erho_k);
- *   4. integration-point force and node scatter as orc_eam_elements, fp of an
- *      atom neighbor its own fp.
- * Energy / virial tallies as in orc_eam_atoms / orc_eam_elements. */
+ *   4. integration-point force and node scatter as orc_eam_elements over the
+ *      site entries of the lists (atoms count in the density of step 1 only);
+ *   5. the reactions of step 3's atom-site forces on the sites' nodes
+ *      (add_site_reactions).
+ * Energy / virial tallies as in orc_eam_atoms / orc_eam_elements, except that
+ * an atom-site pair puts its whole pair energy phi and virial on the atom
+ * (the site's half is not in the integration-point sums). */
 void orc_eam_coupled(int n, const double *x, const orc_elems *el, const orc_box *box, const int *anum,
                      const uint64_t *anb, int maxneigh, const int *inum, const uint64_t *inb, int maxnip,
                      const orc_eam *t, double *rho, double *fp, double *f, double *eatom, double *vatom,
                      double *erho, double *nodef, double *ipf, double *eip, double *vip) {
   int ne = el->ne, nint = el->nint;
   double prd[3] = {box->hi[0] - box->lo[0], box->hi[1] - box->lo[1], box->hi[2] - box->lo[2]};
+  double *fsite = NULL;
+  unsigned char *has = NULL;
+  if (n > 0 && maxneigh > 0) {
+    fsite = (double *)malloc(sizeof(double) * 3 * (size_t)n * maxneigh);
+    has = (unsigned char *)calloc((size_t)n * maxneigh, 1);
+  }
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < ne; i++)
     for (int k = 0; k < 8; k++) {
@@ -1373,6 +1438,8 @@ void orc_eam_coupled(int n, const double *x, const orc_elems *el, const orc_box 
       double delx = xi[0] - s[0], dely = xi[1] - s[1], delz = xi[2] - s[2];
       double rsq = delx * delx + dely * dely + delz * delz;
       if (rsq < t->cutforcesq) {
+        int site_e = (uint32_t)(e >> 32) != ORC_ATOM_SITE;
+        double share = site_e ? 1.0 : 0.5;
         double fpj = entry_fp(el, t, fp, erho, e);
         double r = sqrt(rsq), p;
         const double *c = rtab(t->rhor_spline, r, t->rdr, t->nr, &p);
@@ -1388,13 +1455,20 @@ void orc_eam_coupled(int n, const double *x, const orc_elems *el, const orc_box 
         fx += delx * fpair;
         fy += dely * fpair;
         fz += delz * fpair;
-        epair += 0.5 * phi;
-        v[0] += 0.5 * delx * delx * fpair;
-        v[1] += 0.5 * dely * dely * fpair;
-        v[2] += 0.5 * delz * delz * fpair;
-        v[3] += 0.5 * delx * dely * fpair;
-        v[4] += 0.5 * delx * delz * fpair;
-        v[5] += 0.5 * dely * delz * fpair;
+        if (fsite && site_e) {
+          size_t slot = (size_t)i * maxneigh + jj;
+          fsite[3 * slot] = delx * fpair;
+          fsite[3 * slot + 1] = dely * fpair;
+          fsite[3 * slot + 2] = delz * fpair;
+          has[slot] = 1;
+        }
+        epair += share * phi;
+        v[0] += share * delx * delx * fpair;
+        v[1] += share * dely * dely * fpair;
+        v[2] += share * delz * delz * fpair;
+        v[3] += share * delx * dely * fpair;
+        v[4] += share * delx * delz * fpair;
+        v[5] += share * dely * delz * fpair;
       }
     }
     f[3 * i] = fx;
@@ -1423,6 +1497,7 @@ void orc_eam_coupled(int n, const double *x, const orc_elems *el, const orc_box 
       size_t p = (size_t)i * nint + g;
       for (int ll = 0; ll < inum[p]; ll++) {
         uint64_t e = inb[p * maxnip + ll];
+        if ((uint32_t)(e >> 32) == ORC_ATOM_SITE) continue; /* density only */
         double s[3];
         entry_pos(x, el, e, prd, s);
         double delx = ixtmp - s[0], dely = iytmp - s[1], delz = iztmp - s[2];
@@ -1466,17 +1541,30 @@ void orc_eam_coupled(int n, const double *x, const orc_elems *el, const orc_box 
     for (int k = 0; k < 8; k++)
       for (int d = 0; d < 3; d++) nodef[24 * i + 3 * k + d] = nf[k][d];
   }
+  if (fsite) {
+    add_site_reactions(n, el, anum, anb, maxneigh, fsite, has, nodef);
+    free(fsite);
+    free(has);
+  }
 }
 
 /* LJ over the coupled lists: PairLJCut::compute / ::ecompute pair terms
  * (pair_lj_cut.cpp:255-287, 376-428) with every neighbor position from
- * entry_pos. */
+ * entry_pos.  Integration points sum their site entries only; the atom-site
+ * forces reach the sites' nodes as reactions (add_site_reactions), and an
+ * atom-site pair puts its whole energy and virial on the atom. */
 void orc_lj_coupled(int n, const double *x, const orc_elems *el, const orc_box *box, const int *anum,
                     const uint64_t *anb, int maxneigh, const int *inum, const uint64_t *inb, int maxnip,
                     const orc_lj *pp, double *f, double *eatom, double *vatom, double *nodef, double *ipf,
                     double *eip, double *vip) {
   int ne = el->ne, nint = el->nint;
   double prd[3] = {box->hi[0] - box->lo[0], box->hi[1] - box->lo[1], box->hi[2] - box->lo[2]};
+  double *fsite = NULL;
+  unsigned char *has = NULL;
+  if (n > 0 && maxneigh > 0) {
+    fsite = (double *)malloc(sizeof(double) * 3 * (size_t)n * maxneigh);
+    has = (unsigned char *)calloc((size_t)n * maxneigh, 1);
+  }
 #pragma omp parallel for schedule(static)
   for (int q = 0; q < n + ne * nint; q++) {
     int is_atom = q < n;
@@ -1492,11 +1580,15 @@ void orc_lj_coupled(int n, const double *x, const orc_elems *el, const orc_box *
     double fx = 0, fy = 0, fz = 0, e = 0;
     double v[6] = {0, 0, 0, 0, 0, 0};
     for (int jj = 0; jj < num[row]; jj++) {
+      uint64_t ent = nb[(size_t)row * cap + jj];
+      int site_e = (uint32_t)(ent >> 32) != ORC_ATOM_SITE;
+      if (!is_atom && !site_e) continue; /* atoms: reactions below */
       double s[3];
-      entry_pos(x, el, nb[(size_t)row * cap + jj], prd, s);
+      entry_pos(x, el, ent, prd, s);
       double delx = c[0] - s[0], dely = c[1] - s[1], delz = c[2] - s[2];
       double rsq = delx * delx + dely * dely + delz * delz;
       if (rsq < pp->cutsq) {
+        double share = (is_atom && site_e) ? 1.0 : 0.5;
         double r2inv = 1.0 / rsq;
         double r6inv = r2inv * r2inv * r2inv;
         double forcelj = r6inv * (pp->lj1 * r6inv - pp->lj2);
@@ -1504,13 +1596,20 @@ void orc_lj_coupled(int n, const double *x, const orc_elems *el, const orc_box *
         fx += delx * fpair;
         fy += dely * fpair;
         fz += delz * fpair;
-        e += 0.5 * (r6inv * (pp->lj3 * r6inv - pp->lj4));
-        v[0] += 0.5 * delx * delx * fpair;
-        v[1] += 0.5 * dely * dely * fpair;
-        v[2] += 0.5 * delz * delz * fpair;
-        v[3] += 0.5 * delx * dely * fpair;
-        v[4] += 0.5 * delx * delz * fpair;
-        v[5] += 0.5 * dely * delz * fpair;
+        if (fsite && is_atom && site_e) {
+          size_t slot = (size_t)row * maxneigh + jj;
+          fsite[3 * slot] = delx * fpair;
+          fsite[3 * slot + 1] = dely * fpair;
+          fsite[3 * slot + 2] = delz * fpair;
+          has[slot] = 1;
+        }
+        e += share * (r6inv * (pp->lj3 * r6inv - pp->lj4));
+        v[0] += share * delx * delx * fpair;
+        v[1] += share * dely * dely * fpair;
+        v[2] += share * delz * delz * fpair;
+        v[3] += share * delx * dely * fpair;
+        v[4] += share * delx * delz * fpair;
+        v[5] += share * dely * delz * fpair;
       }
     }
     if (is_atom) {
@@ -1546,6 +1645,11 @@ void orc_lj_coupled(int n, const double *x, const orc_elems *el, const orc_box *
     }
     for (int k = 0; k < 8; k++)
       for (int d = 0; d < 3; d++) nodef[24 * i + 3 * k + d] = nf[k][d];
+  }
+  if (fsite) {
+    add_site_reactions(n, el, anum, anb, maxneigh, fsite, has, nodef);
+    free(fsite);
+    free(has);
   }
 }
 

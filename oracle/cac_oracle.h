@@ -114,7 +114,15 @@ void orc_lj_elements(const orc_elems *el, const orc_box *box,
  * site == ORC_ATOM_SITE for an atom j.  Atom lists: atoms and element
  * sites within cutneigh; integration-point lists: sites (reference rules)
  * and atoms within cutneigh.  brute != 0 searches every image of every
- * atom and site.  Rows are sorted. */
+ * atom and site.  Rows are sorted.
+ * Forces (orc_eam_coupled / orc_lj_coupled): an atom takes the pair force
+ * f_is of every site s in its list; the integration points count atoms in
+ * their densities only, and each site s passes the reaction -f_is to its
+ * element's nodes through its shape functions (nodef[e][k] += N_k(s) R_s,
+ * R_s summed over the atoms in ascending (atom | img << 27) order, sites
+ * in ascending index, after the integration-point scatter), so atom-element
+ * forces are equal and opposite.  An atom-site pair tallies its whole
+ * energy and virial on the atom. */
 #define ORC_ATOM_SITE 0xFFFFFFFFu
 int orc_coupled_neighbors(int n, const double *x, const orc_elems *el, const orc_box *box, double cutneigh,
                           int nsplit, int brute, int maxneigh, int *anum, uint64_t *anb, int maxnip, int *inum,

@@ -7,8 +7,17 @@ integration points only element sites, pair_eam.cpp:927-1137).
   nae = 2 elements with the 8 corner integration points of weight 1 is
   molecular dynamics of all sites, so the coupled atom forces, nodal forces,
   densities, energy and virial equal the all-atom values.
-* Patch test: a perfect lattice of atoms and nae = 5 elements carries no
-  force anywhere, interface included.
+* Newton's third law across the interface: integration points count atoms
+  in their density only; each atom-site pair force f_is acts on the atom,
+  and -f_is on the site, which passes it to its element's nodes through its
+  shape functions N_k.  So the atom forces plus the nodal reactions (nodal
+  force minus the integration-point scatter) sum to zero (linear momentum),
+  and, the pair forces being central and sum_k N_k X_k = x_s, so does their
+  moment about the origin (angular momentum), at nae = 3 and 5, EAM and LJ.
+  A reaction with the wrong sign, weight or shape functions fails one of
+  the two.
+* A perfect lattice of atoms and nae = 5 elements: atoms and the nodes of
+  elements away from the interface carry no force.
 * Without atoms or without elements the coupled path is the reference path.
 """
 import numpy as np
@@ -82,10 +91,11 @@ def test_coupled_finest_mesh_equals_md(orc, cu_tables, pot):
                        atol=1e-11 * np.abs(vtot).max())
 
 
-def test_coupled_patch_test(orc, cu_tables):
+def test_coupled_perfect_lattice(orc, cu_tables):
     """Perfect lattice, atoms in a core and nae=5 elements around it: every
-    atom and integration point sees the perfect-lattice neighborhood."""
-    s = build_cac_system(10, 5, 4.0)
+    atom and every integration point away from the interface sees the
+    perfect-lattice neighborhood and carries no force."""
+    s = build_cac_system(20, 5, 8.0)
     es = orc.ElementSet(s.nodex, s.nae, s.intp_ijk, s.intp_w)
     anum, anb, inum, inb = orc.coupled_neighbors(s.x, es, s.box_lo, s.box_hi, CUT, 2, 220, 220)
     assert np.all(anum == 78) and np.all(inum == 78)      # fcc shells within 5.95 A
@@ -94,7 +104,62 @@ def test_coupled_patch_test(orc, cu_tables):
     assert np.allclose(c["rho"], rho_e, rtol=1e-9)
     assert np.allclose(c["erho"], rho_e, rtol=1e-9)
     assert np.max(np.abs(c["f"])) < 1e-9
-    assert np.max(np.abs(c["nodef"])) < 1e-8
+    touched = _interface_elements(orc, s, anum, anb, inum, inb)
+    assert touched.any() and not touched.all()
+    assert np.max(np.abs(c["nodef"][~touched])) < 1e-8
+
+
+def _interface_elements(orc, s, anum, anb, inum, inb):
+    """Elements with an atom in one of their lists or a site in an atom list."""
+    t = np.zeros(s.n_elements, dtype=bool)
+    for i in range(len(anum)):
+        e = anb[i, :anum[i]]
+        sites = e[(e >> np.uint64(32)) != orc.ATOM_SITE]
+        t[(sites & np.uint64(0x7FFFFFF)).astype(np.int64)] = True
+    nint = len(s.intp_w)
+    for p in range(len(inum)):
+        if ((inb[p, :inum[p]] >> np.uint64(32)) == orc.ATOM_SITE).any():
+            t[p // nint] = True
+    return t
+
+
+def _scatter(es, ipf):
+    """Integration-point part of the nodal forces, sum_g w_g N_k(g) f_g
+    (pair_eam.cpp:1130-1134)."""
+    ne, nint = es.nodex.shape[0], es.spa_int.shape[0]
+    f = ipf.reshape(ne, nint, 3)
+    wn = es.weight[:, None] * es.spa_int                    # [nint][8]
+    return np.einsum("gk,egd->ekd", wn, f)
+
+
+@pytest.mark.parametrize("pot", ["eam", "lj"])
+@pytest.mark.parametrize("nae,L,core", [(3, 15, 4.0), (5, 20, 6.0)])
+def test_coupled_newton_third_law(orc, cu_tables, pot, nae, L, core):
+    s = jitter(build_cac_system(L, nae, core, temperature=300.0, seed=7), 0.08, 53 + nae)
+    es = orc.ElementSet(s.nodex, s.nae, s.intp_ijk, s.intp_w)
+    anum, anb, inum, inb = orc.coupled_neighbors(s.x, es, s.box_lo, s.box_hi, CUT if pot == "eam" else 6.0, 2,
+                                                 220, 220)
+    if pot == "eam":
+        c = orc.eam_coupled(s.x, es, s.box_lo, s.box_hi, anum, anb, inum, inb, cu_tables)
+    else:
+        c = orc.lj_coupled(s.x, es, s.box_lo, s.box_hi, anum, anb, inum, inb, LJ)
+    react = c["nodef"] - _scatter(es, c["ipf"])             # nodal reactions of the atom-site pairs
+    touched = _interface_elements(orc, s, anum, anb, inum, inb)
+    assert np.max(np.abs(react[~touched])) == 0.0
+    scale = np.max(np.abs(c["f"]))
+    fsum = c["f"].sum(axis=0) + react.reshape(-1, 3).sum(axis=0)
+    # the atoms alone feel a net force from the elements: the balance is real
+    assert np.linalg.norm(c["f"].sum(axis=0)) > 1e-3 * scale
+    assert np.max(np.abs(fsum)) < 1e-11 * scale * len(s.x)
+    # no atom-site or atom-atom pair crosses a box face here (a central core
+    # well inside the box), so moments about the origin use the stored
+    # coordinates
+    img = (anb >> np.uint64(27)) & np.uint64(31)
+    assert np.all(img[np.arange(anb.shape[1])[None, :] < anum[:, None]] == 13)
+    X = es.nodex.reshape(-1, 3)
+    tau = np.cross(s.x, c["f"]).sum(axis=0) + np.cross(X, react.reshape(-1, 3)).sum(axis=0)
+    tau_scale = np.abs(np.cross(s.x, c["f"])).sum()
+    assert np.max(np.abs(tau)) < 1e-12 * tau_scale * 10
 
 
 def test_coupled_reduces_to_reference_without_interface(orc, cu_tables):
