@@ -10,13 +10,15 @@ Metric: Matom-steps/s = (N_atoms + 8*N_elements) * steps / s / 1e6.
 
 Default workload: EAM Cu CAC, periodic box of 198^3 fcc cells (715.8 A),
 elements of 11 sites per edge with 8 corner integration points, atomistic
-core of dropped blocks (484,484 atoms, 22,964 elements), Johnson-type
+core of dropped blocks (500,456 atoms, 22,952 elements), Johnson-type
 analytic EAM tabulated with nrho = nr = 10000, rc = 4.95 A, neighbor cutoff
 5.95 A, 300 K seeded velocities, dt = 1 fs.  Synthetic data, generated here.
 
 The JSON line carries the roofline of the dominant kernel (algorithmic
-bytes / measured average launch time, HIP events on the context stream) and
-the CPU oracle timed on this host on a bounded sample of the same workload.
+bytes / measured average launch time, HIP events on the context stream), the
+CPU oracle timed on this host on a bounded sample of the same workload, and
+the parity of the device's first force call with the oracle's on the full
+workload (neighbor sets compared sorted, FP64 forces norm-relative).
 """
 from __future__ import annotations
 
@@ -47,7 +49,9 @@ B_ATOM = 616
 B_ELEM = 7640
 # HBM bytes per launch of the FP64 kernels on the eam_c1 workload, from
 # rocprofv3 FETCH_SIZE and WRITE_SIZE runs scaled by a plain 1 GiB copy's
-# counts (fetch x2.000, write x1.000); the summary is profiles/eam_c1_kernels.md.
+# counts (fetch x2.000, write x1.000).  A profile-derived constant, not
+# measured by this script: TRAFFIC_SOURCE names the summary it comes from.
+TRAFFIC_SOURCE = "profiles/eam_c1_kernels.md (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)"
 TRAFFIC = {"atom_force": 187.7e6 + 11.9e6, "atom_rho": 190.1e6 + 26.0e6, "ip_force": 148.6e6 + 19.1e6,
            "ip_erho": 131.0e6 + 12.2e6}
 
@@ -86,7 +90,8 @@ def make_pair(workload):
 COUPLED = True  # atom-element coupling (set from --no-coupling in main)
 
 
-KERNELS = ("atom_rho", "atom_force", "site_pos", "ip_erho", "site_fp", "ip_force", "scatter", "surr_feat", "surr_mlp",
+KERNELS = ("atom_rho", "atom_force", "site_pos", "ip_erho", "site_fp", "ip_force", "site_react", "scatter", "surr_feat",
+           "surr_mlp",
            "surr_fit", "atom_neigh", "ip_neigh", "elem_neigh", "atom_sites")
 
 
@@ -181,8 +186,10 @@ def cpu_baseline(s, pair, ff, cutneigh, threads, warm, timed, rebuild_rate):
     m, me = s.mass, s.mass_e
     f = fa(); nodef = fe()
     xc = np.zeros((s.n_elements, 3))
+    state = None
     for i in range(warm + timed):
         if i == warm:
+            state = dict(x=x.copy(), nodex=es.nodex.copy())     # the configuration for the parity check
             t0 = time.perf_counter()
         O.nve_atoms_initial(x, v, f, dt, dtf, m)
         O.nve_nodes_initial(es.nodex, nodev, nodef, xc, dt, dtf, me)
@@ -192,7 +199,68 @@ def cpu_baseline(s, pair, ff, cutneigh, threads, warm, timed, rebuild_rate):
     t_step = (time.perf_counter() - t0) / timed
     per_step = t_step + rebuild_rate * t_build
     return dict(per_step=per_step, steps=timed, warm=warm, t_build=t_build, t_step=t_step,
-                cores=O.num_threads())
+                cores=O.num_threads(), state=state)
+
+
+def full_size_parity(s, pair, ff, cutneigh, skin, every, delay, state):
+    """The device's lists and FP64 forces against the oracle's on the full
+    workload, at the configuration the CPU baseline reached after its warm-up
+    steps (thermal forces, not the vanishing forces of the perfect start),
+    wrapped into the box (orc_pbc_*) first; outside the timed region.  Sets
+    are compared sorted, row by row; forces as max |gpu - cpu| / max |cpu|
+    over atoms and over nodes."""
+    import dataclasses
+    import numpy as np
+    from oracle import oracle as O
+    x = state["x"].copy()
+    nodex = state["nodex"].copy()
+    O.pbc_atoms(x, s.box_lo, s.box_hi)
+    O.pbc_elements(O.element_centers(nodex), nodex, s.box_lo, s.box_hi)
+    es = O.ElementSet(nodex, s.nae, s.intp_ijk, s.intp_w)
+    if COUPLED:
+        rnum_a, rnb_a, rnum_i, rnb_i = O.coupled_neighbors(x, es, s.box_lo, s.box_hi, cutneigh, 2, 128, 128)
+        r = (O.eam_coupled(x, es, s.box_lo, s.box_hi, rnum_a, rnb_a, rnum_i, rnb_i, O.eam_tables(ff)) if ff is not None
+             else O.lj_coupled(x, es, s.box_lo, s.box_hi, rnum_a, rnb_a, rnum_i, rnb_i, pair.init_one(1, 1)))
+        rf, rnf = r["f"], r["nodef"]
+    else:
+        rnum_a, anb32 = O.atom_neighbors(x, s.box_lo, s.box_hi, cutneigh, 128)
+        rnb_a = anb32.astype(np.uint64) | (np.uint64(O.ATOM_SITE) << np.uint64(32))
+        rnum_i, rnb_i = O.ip_neighbors(es, s.box_lo, s.box_hi, cutneigh, 2, 128)
+        if ff is not None:
+            T = O.eam_tables(ff)
+            rf = O.eam_atoms(x, s.box_lo, s.box_hi, rnum_a, anb32, T)["f"]
+            rnf = O.eam_elements(es, s.box_lo, s.box_hi, rnum_i, rnb_i, T)["nodef"]
+        else:
+            lj = pair.init_one(1, 1)
+            rf = O.lj_atoms(x, s.box_lo, s.box_hi, rnum_a, anb32, lj)["f"]
+            rnf = O.lj_elements(es, s.box_lo, s.box_hi, rnum_i, rnb_i, lj)["nodef"]
+    eng = new_engine(dataclasses.replace(s, x=x, nodex=nodex), pair, cutneigh, skin, every, delay)
+    try:
+        _, _, f = eng.atoms()
+        _, _, nodef = eng.elements()
+        sets = {}
+        for name, (num, nb), rnum, rnb in (("atoms", eng.atom_neighbors(), rnum_a, rnb_a),
+                                           ("integration_points", eng.ip_neighbors(), rnum_i, rnb_i)):
+            same_counts = bool(np.array_equal(num, rnum))
+            w = int(max(num.max(initial=0), rnum.max(initial=0)))
+            pad = np.uint64(0xFFFFFFFFFFFFFFFF)
+            a = np.where(np.arange(w)[None, :] < num[:, None], nb[:, :w], pad)
+            b = np.where(np.arange(w)[None, :] < rnum[:, None], rnb[:, :w], pad)
+            a.sort(axis=1)
+            b.sort(axis=1)
+            sets[name] = {"rows": int(num.size), "entries": int(num.sum()), "counts_equal": same_counts,
+                          "sets_bit_exact": bool(same_counts and np.array_equal(a, b))}
+    finally:
+        eng.close()
+
+    def rel(a, b):
+        sc = float(np.max(np.abs(b))) if b.size else 0.0
+        return float(np.max(np.abs(a - b))) / sc if sc > 0 else 0.0
+
+    return {"neighbor_sets": sets, "atom_force_rel_max": rel(f, rf), "node_force_rel_max": rel(nodef, rnf),
+            "atom_force_max_eV_A": float(np.max(np.abs(rf))) if rf.size else 0.0,
+            "node_force_max_eV_A": float(np.max(np.abs(rnf))) if rnf.size else 0.0,
+            "tolerance": 1e-10, "configuration": "after the CPU baseline's warm-up steps, FP64"}
 
 
 def host_info():
@@ -227,8 +295,8 @@ def new_engine(s, pair, cutneigh, skin, every, delay):
 
 def _err(test, ref):
     """Force difference: absolute RMS (eV/A) of |test - ref| per particle,
-    RMS |ref|, and relative RMS / max (divided by RMS |ref|; meaningless
-    where the reference forces vanish, e.g. the perfect lattice at step 0)."""
+    RMS |ref|, and relative RMS / max (divided by RMS |ref|), the relative
+    ones only where the reference forces do not vanish."""
     import numpy as np
     d = (test - ref).reshape(-1, 3)
     r = ref.reshape(-1, 3)
@@ -236,7 +304,7 @@ def _err(test, ref):
     drms = float(np.sqrt(np.mean(np.sum(d * d, axis=1))))
     dmax = float(np.sqrt(np.max(np.sum(d * d, axis=1))))
     out = {"abs_rms_eV_A": drms, "ref_rms_eV_A": rms}
-    if rms > 0.0:
+    if rms > 1e-6:          # relative figures only against forces that do not vanish
         out.update(rel_rms=drms / rms, rel_max=dmax / rms)
     return out
 
@@ -417,6 +485,7 @@ def main():
         roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4),
                 "traffic": TRAFFIC.get(dom) if (args.workload == "eam_c1" and args.precision == "fp64") else None,
+                "traffic_source": TRAFFIC_SOURCE if (args.workload == "eam_c1" and args.precision == "fp64") else None,
                 "alg_bytes_per_launch": units * per, "avg_launch_us": round(ms * 1e3, 2)}
     b_step = B_ATOM * s.n_atoms + B_ELEM * s.n_elements
     step_roof = {"alg_bytes_per_step": b_step,
@@ -459,7 +528,11 @@ def main():
         cn = cpu_baseline(s, pair, ff, cutneigh, nthr, args.cpu_warmup, args.cpu_steps, rate)
         log(f"[bench] cpu: {cn['per_step']:.3f} s/step on {cn['cores']} threads; 1+{args.cpu_steps_1t} steps on 1 "
             f"thread ...")
+        log("[bench] full-size parity of the device's first force call against the oracle's ...")
+        parity = full_size_parity(s, pair, ff, cutneigh, skin, args.every, args.delay, cn.pop("state"))
+        log(f"[bench] parity: {json.dumps(parity)}")
         c1 = cpu_baseline(s, pair, ff, cutneigh, 1, 1, args.cpu_steps_1t, rate)
+        c1.pop("state")
         cpu = {"value": round(n_int / cn["per_step"] / 1e6, 4), "unit": "Matom-steps/s", "cores": cn["cores"],
                "kind": "port",
                "sample": f"{args.cpu_warmup} warm-up + {cn['steps']} timed full FP64 steps of the same system on "
@@ -469,6 +542,7 @@ def main():
                                  "t_build": round(c1["t_build"], 2),
                                  "sample": f"1 warm-up + {c1['steps']} timed steps, one neighbor build"},
                "coupling": COUPLED,
+               "parity_full_size": parity,
                "host": host}
         log(f"[bench] cpu: {cn['per_step']:.3f} s/step on {cn['cores']} threads, {c1['per_step']:.3f} s/step on 1")
 

@@ -177,9 +177,10 @@ struct cac_ctx {
   int nrs = 0, nre = 0;
   long nrev = 0;
   int *rcount = nullptr, *rstart = nullptr, *rsite = nullptr, *relem = nullptr, *rel_start = nullptr;
+  int *rsl = nullptr;  // [nrs] site index of each reaction site in its element
   uint32_t *rkey = nullptr;
   double *rforce = nullptr;
-  cac::DevBuf b_rcount, b_rstart, b_rkey, b_rsite, b_relem, b_rel_start, b_rforce, b_rtmp;
+  cac::DevBuf b_rcount, b_rstart, b_rkey, b_rsite, b_relem, b_rel_start, b_rforce, b_rtmp, b_rsl;
 
   // ---- elements (one type)
   int ne = 0, nae[3] = {0, 0, 0}, nsite = 0;

@@ -999,18 +999,36 @@ __device__ __forceinline__ void cl_erho_body(const ClArgs &c, const ElemArgs &a,
         const int kk = k0 + u * kLanes;
         q[u] = kk < jn ? (int)row[(size_t)kk * kRowGroup] : -1;
       }
+      // distances and knots, then the rows (all in flight), then the sums
+      P pp[kU];
+      int mm[kU];
 #pragma unroll
       for (int u = 0; u < kU; ++u) {
+        mm[u] = -1;
         if (q[u] < 0) continue;
         const double *r = rows + 4 * q[u];
         const double delx = ix - r[0], dely = iy - r[1], delz = iz - r[2];
         const double rsq = delx * delx + dely * dely + delz * delz;
         if (rsq < t.cutforcesq) {
-          acc += (double)rho_of_r<P>(t, sqrt_p<P>(rsq));
+          mm[u] = rknot<P>(t, sqrt_p<P>(rsq), &pp[u]);
           // the force pass takes element sites only (atom rows carry 1.0)
           if (r[3] == 0.0) bits |= 1u << ((j0 + u) & 31);
         }
       }
+      P c[kU][4];
+#pragma unroll
+      for (int u = 0; u < kU; ++u)
+        if (mm[u] >= 0) {
+          const P *src = t.rv + 4 * (size_t)mm[u];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) c[u][k] = src[k];
+        }
+#pragma unroll
+      for (int u = 0; u < kU; ++u)
+        if (mm[u] >= 0) {
+          const P p = pp[u];  // rho_of_r's polynomial, pair_eam.cpp:655
+          acc += (double)(((c[u][0] * p + c[u][1]) * p + c[u][2]) * p + c[u][3]);
+        }
       if (((j0 + kU) & 31) == 0 || k0 + kU * kLanes >= jn) {
         mrow[j0 >> 5] = bits;
         bits = 0;
@@ -1254,107 +1272,153 @@ struct ReactArgs {
   const int *skip;
 };
 
+// 8 lanes per reaction site (the layout of the integration-point passes):
+// lane l takes the site's reverse entries l, l+8, ..., fetching kU of them
+// and their atom rows before the arithmetic; the lanes' partial sums are
+// combined by group_sum.
 template <class P>
 __global__ __launch_bounds__(kBlock) void k_site_react_eam(ReactArgs r, const double *__restrict__ x4, Prd prd,
                                                            Tabs<P> t) {
-  const int q = xcd_block() * kBlock + (int)threadIdx.x;
-  if (q >= r.nrs || *r.skip) return;
+  const int gt = xcd_block() * kBlock + (int)threadIdx.x;
+  const int q = gt >> 3, l = gt & 7;
+  if (*r.skip) return;
+  const bool live = q < r.nrs;
   const Win<P> w = Win<P>{nullptr, t.nr + 2};  // no LDS window: every row from the tables
-  const int a = r.rsite[q];
-  const size_t row = (size_t)r.n + a;
-  const double2 sxy = reinterpret_cast<const double2 *>(x4)[2 * row];
-  const double2 szw = reinterpret_cast<const double2 *>(x4)[2 * row + 1];
-  const P fps = (P)szw.y;
   double rx = 0.0, ry = 0.0, rz = 0.0;
-  for (int k = r.rstart[a]; k < r.rstart[a + 1]; ++k) {
-    const uint32_t key = r.rkey[k];
-    const size_t i = key & 0x7FFFFFFu;
-    const int img = (int)(key >> 27);
-    double sx = sxy.x, sy = sxy.y, sz = szw.x;
-    if (img != kImgSelf) {
-      int sh[3];
-      img_shift(img, sh);
-      sx = shifted(sx, sh[0], prd.v[0]);
-      sy = shifted(sy, sh[1], prd.v[1]);
-      sz = shifted(sz, sh[2], prd.v[2]);
-    }
-    const double2 ixy = reinterpret_cast<const double2 *>(x4)[2 * i];
-    const double2 izw = reinterpret_cast<const double2 *>(x4)[2 * i + 1];
-    const double delx = ixy.x - sx, dely = ixy.y - sy, delz = izw.x - sz;
-    const double rsq = delx * delx + dely * dely + delz * delz;
-    if (rsq < t.cutforcesq) {
-      P phi;
-      const double fpair = eam_fpair<P>(t, w, rsq, (P)izw.y, fps, &phi);
-      rx -= delx * fpair;
-      ry -= dely * fpair;
-      rz -= delz * fpair;
+  if (live) {
+    const int a = r.rsite[q];
+    const size_t row = (size_t)r.n + a;
+    const double2 sxy = reinterpret_cast<const double2 *>(x4)[2 * row];
+    const double2 szw = reinterpret_cast<const double2 *>(x4)[2 * row + 1];
+    const P fps = (P)szw.y;
+    const int k1 = r.rstart[a + 1];
+    for (int k0 = r.rstart[a] + l; k0 < k1; k0 += kU * kLanes) {
+      uint32_t key[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) key[u] = k0 + u * kLanes < k1 ? r.rkey[k0 + u * kLanes] : kNoEntry;
+      double2 ixy[kU], izw[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u)
+        if (key[u] != kNoEntry) {
+          const size_t i = key[u] & 0x7FFFFFFu;
+          ixy[u] = reinterpret_cast<const double2 *>(x4)[2 * i];
+          izw[u] = reinterpret_cast<const double2 *>(x4)[2 * i + 1];
+        }
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        if (key[u] == kNoEntry) continue;
+        const int img = (int)(key[u] >> 27);
+        double sx = sxy.x, sy = sxy.y, sz = szw.x;
+        if (img != kImgSelf) {
+          int sh[3];
+          img_shift(img, sh);
+          sx = shifted(sx, sh[0], prd.v[0]);
+          sy = shifted(sy, sh[1], prd.v[1]);
+          sz = shifted(sz, sh[2], prd.v[2]);
+        }
+        const double delx = ixy[u].x - sx, dely = ixy[u].y - sy, delz = izw[u].x - sz;
+        const double rsq = delx * delx + dely * dely + delz * delz;
+        if (rsq < t.cutforcesq) {
+          P phi;
+          const double fpair = eam_fpair<P>(t, w, rsq, (P)izw[u].y, fps, &phi);
+          rx -= delx * fpair;
+          ry -= dely * fpair;
+          rz -= delz * fpair;
+        }
+      }
     }
   }
-  r.R[q] = rx;
-  r.R[(size_t)r.nrs + q] = ry;
-  r.R[2 * (size_t)r.nrs + q] = rz;
+  rx = group_sum(rx);
+  ry = group_sum(ry);
+  rz = group_sum(rz);
+  if (live && l == 0) {
+    r.R[q] = rx;
+    r.R[(size_t)r.nrs + q] = ry;
+    r.R[2 * (size_t)r.nrs + q] = rz;
+  }
 }
 
 template <class P>
 __global__ __launch_bounds__(kBlock) void k_site_react_lj(ReactArgs r, const double *__restrict__ x4, Prd prd,
                                                           LjParams lj) {
-  const int q = xcd_block() * kBlock + (int)threadIdx.x;
-  if (q >= r.nrs) return;
+  const int gt = xcd_block() * kBlock + (int)threadIdx.x;
+  const int q = gt >> 3, l = gt & 7;
+  const bool live = q < r.nrs;
   const P lj1 = (P)lj.lj1, lj2 = (P)lj.lj2;
-  const int a = r.rsite[q];
-  const size_t row = (size_t)r.n + a;
-  const double sx0 = x4[4 * row], sy0 = x4[4 * row + 1], sz0 = x4[4 * row + 2];
   double rx = 0.0, ry = 0.0, rz = 0.0;
-  for (int k = r.rstart[a]; k < r.rstart[a + 1]; ++k) {
-    const uint32_t key = r.rkey[k];
-    const size_t i = key & 0x7FFFFFFu;
-    const int img = (int)(key >> 27);
-    double sx = sx0, sy = sy0, sz = sz0;
-    if (img != kImgSelf) {
-      int sh[3];
-      img_shift(img, sh);
-      sx = shifted(sx, sh[0], prd.v[0]);
-      sy = shifted(sy, sh[1], prd.v[1]);
-      sz = shifted(sz, sh[2], prd.v[2]);
-    }
-    const double delx = x4[4 * i] - sx, dely = x4[4 * i + 1] - sy, delz = x4[4 * i + 2] - sz;
-    const double rsq = delx * delx + dely * dely + delz * delz;
-    if (rsq < lj.cutsq) {
-      P r2inv = P(1.0) / (P)rsq;
-      P r6inv = r2inv * r2inv * r2inv;
-      P forcelj = r6inv * (lj1 * r6inv - lj2);
-      const double fpair = (double)(P(1.0) * forcelj * r2inv);
-      rx -= delx * fpair;
-      ry -= dely * fpair;
-      rz -= delz * fpair;
+  if (live) {
+    const int a = r.rsite[q];
+    const size_t row = (size_t)r.n + a;
+    const double sx0 = x4[4 * row], sy0 = x4[4 * row + 1], sz0 = x4[4 * row + 2];
+    for (int k = r.rstart[a] + l; k < r.rstart[a + 1]; k += kLanes) {
+      const uint32_t key = r.rkey[k];
+      const size_t i = key & 0x7FFFFFFu;
+      const int img = (int)(key >> 27);
+      double sx = sx0, sy = sy0, sz = sz0;
+      if (img != kImgSelf) {
+        int sh[3];
+        img_shift(img, sh);
+        sx = shifted(sx, sh[0], prd.v[0]);
+        sy = shifted(sy, sh[1], prd.v[1]);
+        sz = shifted(sz, sh[2], prd.v[2]);
+      }
+      const double delx = x4[4 * i] - sx, dely = x4[4 * i + 1] - sy, delz = x4[4 * i + 2] - sz;
+      const double rsq = delx * delx + dely * dely + delz * delz;
+      if (rsq < lj.cutsq) {
+        P r2inv = P(1.0) / (P)rsq;
+        P r6inv = r2inv * r2inv * r2inv;
+        P forcelj = r6inv * (lj1 * r6inv - lj2);
+        const double fpair = (double)(P(1.0) * forcelj * r2inv);
+        rx -= delx * fpair;
+        ry -= dely * fpair;
+        rz -= delz * fpair;
+      }
     }
   }
-  r.R[q] = rx;
-  r.R[(size_t)r.nrs + q] = ry;
-  r.R[2 * (size_t)r.nrs + q] = rz;
+  rx = group_sum(rx);
+  ry = group_sum(ry);
+  rz = group_sum(rz);
+  if (live && l == 0) {
+    r.R[q] = rx;
+    r.R[(size_t)r.nrs + q] = ry;
+    r.R[2 * (size_t)r.nrs + q] = rz;
+  }
 }
 
-// 8 threads per element with reaction sites (one per node): the node adds
-// N_k(s) * R_s over the element's reaction sites in ascending order, after
-// its integration-point term.
-__global__ __launch_bounds__(kBlock) void k_site_react_nodes(ReactArgs r, int nre, const int *__restrict__ relem,
-                                                             const int *__restrict__ rel_start, ElemArgs a,
-                                                             double *__restrict__ nodef) {
-  const int t = blockIdx.x * blockDim.x + (int)threadIdx.x;
-  if ((t >> 3) >= nre || *r.skip) return;
-  const int ri = t >> 3, k = t & 7;
-  const size_t b = 24 * (size_t)relem[ri] + 3 * k;
-  double fx = nodef[b], fy = nodef[b + 1], fz = nodef[b + 2];
-  for (int q = rel_start[ri]; q < rel_start[ri + 1]; ++q) {
+// One wavefront per element with reaction sites: lane l takes the sites
+// l, l+64, ... of the element's range and sums N_k(s) * R_s for its 8 nodes
+// (24 partial sums); a fixed xor butterfly combines the lanes (every lane ends
+// with the same bits), and lanes 0..23 add node k = l/3, component l%3 to the
+// node force written by the integration-point pass.  rsl[q]: the site index
+// of reaction site q in its element.
+__global__ __launch_bounds__(64) void k_site_react_nodes(ReactArgs r, int nre, const int *__restrict__ relem,
+                                                         const int *__restrict__ rel_start,
+                                                         const int *__restrict__ rsl, ElemArgs a,
+                                                         double *__restrict__ nodef) {
+  const int ri = xcd_block();
+  if (ri >= nre || *r.skip) return;  // whole wavefront
+  const int lane = (int)threadIdx.x;
+  double acc[24];
+#pragma unroll
+  for (int c = 0; c < 24; ++c) acc[c] = 0.0;
+  for (int q = rel_start[ri] + lane; q < rel_start[ri + 1]; q += 64) {
     double N[8];
-    site_shape(a, a.asite[r.rsite[q]].y, N);
-    fx += N[k] * r.R[q];
-    fy += N[k] * r.R[(size_t)r.nrs + q];
-    fz += N[k] * r.R[2 * (size_t)r.nrs + q];
+    site_shape(a, rsl[q], N);
+    const double R[3] = {r.R[q], r.R[(size_t)r.nrs + q], r.R[2 * (size_t)r.nrs + q]};
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) acc[3 * k + d] += N[k] * R[d];
   }
-  nodef[b] = fx;
-  nodef[b + 1] = fy;
-  nodef[b + 2] = fz;
+#pragma unroll
+  for (int c = 0; c < 24; ++c)
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) acc[c] += __shfl_xor(acc[c], m, 64);
+  double mine = 0.0;
+#pragma unroll
+  for (int c = 0; c < 24; ++c)
+    if (lane == c) mine = acc[c];
+  if (lane < 24) nodef[24 * (size_t)relem[ri] + lane] += mine;
 }
 
 inline unsigned grid_for(long n, int block = kBlock) { return (unsigned)((n + block - 1) / block); }
@@ -1472,14 +1536,15 @@ int site_reactions(cac_ctx *ctx, bool eam, const ElemArgs &a) {
   r.skip = ctx->dflags + 15;
   hipEvent_t t0;
   timer_begin(ctx, "site_react", &t0);
+  const unsigned g = xcd_grid((long)ctx->nrs * kLanes, kBlock);
   if (eam)
-    hipLaunchKernelGGL((k_site_react_eam<P>), dim3(xcd_grid(ctx->nrs, kBlock)), dim3(kBlock), 0, ctx->stream, r,
-                       ctx->x4, prd_of(ctx), tabs_of<P>(ctx));
+    hipLaunchKernelGGL((k_site_react_eam<P>), dim3(g), dim3(kBlock), 0, ctx->stream, r, ctx->x4, prd_of(ctx),
+                       tabs_of<P>(ctx));
   else
-    hipLaunchKernelGGL((k_site_react_lj<P>), dim3(xcd_grid(ctx->nrs, kBlock)), dim3(kBlock), 0, ctx->stream, r,
-                       ctx->x4, prd_of(ctx), ctx->lj);
-  hipLaunchKernelGGL(k_site_react_nodes, dim3(grid_for(8L * ctx->nre)), dim3(kBlock), 0, ctx->stream, r, ctx->nre,
-                     ctx->relem, ctx->rel_start, a, ctx->nodef);
+    hipLaunchKernelGGL((k_site_react_lj<P>), dim3(g), dim3(kBlock), 0, ctx->stream, r, ctx->x4, prd_of(ctx),
+                       ctx->lj);
+  hipLaunchKernelGGL(k_site_react_nodes, dim3(xcd_grid(64L * ctx->nre, 64)), dim3(64), 0, ctx->stream, r, ctx->nre,
+                     ctx->relem, ctx->rel_start, ctx->rsl, a, ctx->nodef);
   timer_end(ctx, "site_react", t0);
   CAC_HIP(ctx, hipGetLastError());
   return CAC_OK;

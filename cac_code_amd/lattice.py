@@ -116,9 +116,14 @@ def fcc_box_blocks(L: int, nae: int):
 
 
 def build_cac_system(L: int, nae: int, core_cells: float, a: float = 3.615, mass: float = 63.546,
-                     temperature: float = 0.0, seed: int = 4928459, core_center=None) -> CACSystem:
+                     temperature: float = 0.0, seed: int = 4928459, core_center=None,
+                     min_atoms: int = 0) -> CACSystem:
     """fcc box of L^3 cubic cells; blocks with any site inside the central
-    core cube of edge core_cells*a become atoms, the rest become elements."""
+    core cube of edge core_cells*a become atoms, the rest become elements.
+    With min_atoms, further blocks become atoms, nearest first (periodic
+    distance of the block centroid to the core center, then block order),
+    until the atoms number at least min_atoms: the cube's atom count grows
+    in steps of whole block layers."""
     P = _prim(a)
     origin = np.full(3, a / 4.0)
     box_lo = np.zeros(3)
@@ -154,6 +159,16 @@ def build_cac_system(L: int, nae: int, core_cells: float, a: float = 3.615, mass
         pos = np.mod(pos - box_lo, box_hi - box_lo) + box_lo
         inside = np.all(np.abs(pos - center) < half, axis=-1)
         drop[ii] = inside.any(axis=1)
+    nsite = nae ** 3
+    need = -(-(min_atoms - nsite * int(drop.sum())) // nsite)
+    if need > 0:
+        cw = np.mod(cen - box_lo, L3) + box_lo
+        d = cw - center
+        d -= L3 * np.round(d / L3)
+        dist = np.sqrt((d * d).sum(axis=1))
+        order = np.lexsort((np.arange(B.shape[0]), dist))
+        extra = order[~drop[order]][:need]
+        drop[extra] = True
     keepB = B[~drop]
     dropB = B[drop]
     # atoms: all sites of dropped blocks, wrapped into the box
@@ -309,6 +324,7 @@ def eam_c3_system(seed: int = 4928459, L: int = 777, nae: int = 21, core_cells: 
 
 
 def eam_c1_system(seed: int = 4928459, L: int = 198, core_cells: float = 42.0) -> CACSystem:
-    """EAM Cu: L=198 cells (715.8 A), nae=11 (~22.9k elements), core sized for
-    ~500k atoms, 300 K."""
-    return build_cac_system(L, 11, core_cells, temperature=300.0, seed=seed)
+    """EAM Cu: L=198 cells (715.8 A), nae=11, an atomistic core of at least
+    500,000 atoms (the 42-cell cube's 484,484 plus the 12 blocks nearest to
+    it: 500,456 atoms) and 22,952 elements, 300 K."""
+    return build_cac_system(L, 11, core_cells, temperature=300.0, seed=seed, min_atoms=500_000)

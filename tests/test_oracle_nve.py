@@ -102,3 +102,29 @@ def test_pbc_wraps_and_moves_nodes(orc):
     # domain.cpp:682; fixed)
     assert np.allclose(xc[0, 0], 0.535)
     assert np.allclose(nodex[0, :, 0], 0.5 + np.arange(8) * 0.01)
+
+
+def test_displacement_check_closed_form(orc):
+    """Neighbor::check_distance (neighbor.cpp:1176-1217): a rebuild when any
+    point has moved strictly more than sqrt(triggersq) since the build.
+    Displacements of exactly representable length: (3, 4, 0)/16 has squared
+    length 25/256 exactly, so a trigger of 25/256 does not fire (strict
+    comparison) and the next smaller double does.  Only the last point moves
+    (an off-by-one in the loop misses it), along x/y and then along z alone
+    (a dropped component misses it); many points each below the trigger do
+    not fire even when their squares sum above it (a maximum, not a sum)."""
+    n = 40
+    xh = np.random.default_rng(3).integers(0, 480, (n, 3)) / 16.0          # dyadic: x - xh is exact
+    t = 25.0 / 256.0
+    for d in (np.array([3.0, 4.0, 0.0]) / 16.0, np.array([0.0, 0.0, 5.0]) / 16.0):
+        x = xh.copy()
+        x[-1] += d
+        assert ((x[-1] - xh[-1]) ** 2).sum() == t                      # exact in binary
+        assert not orc.displacement_exceeds(x, xh, t)
+        assert orc.displacement_exceeds(x, xh, np.nextafter(t, 0.0))
+        assert not orc.displacement_exceeds(xh, xh, 0.0)              # no motion never fires
+    x = xh.copy()
+    x[:, 0] += 0.25                                                    # every point 0.25 A, trigger 0.3 A
+    assert not orc.displacement_exceeds(x, xh, 0.09)
+    assert orc.displacement_exceeds(x, xh, 0.0625 - 1e-12)
+    assert not orc.displacement_exceeds(np.zeros((0, 3)), np.zeros((0, 3)), 0.0)
