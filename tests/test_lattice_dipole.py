@@ -73,5 +73,10 @@ def test_candidate_dropping_matches_full_check():
 
 
 def test_c1_counts_unchanged():
+    """The 500k-atom + 20k-element workload: the 42-cell core cube (484,484
+    atoms) plus the 12 blocks nearest to it; every lattice site is either an
+    atom or an element site exactly once."""
     s = eam_c1_system()
-    assert (s.n_atoms, s.n_elements) == (484484, 22964)
+    assert (s.n_atoms, s.n_elements) == (500456, 22952)
+    assert s.n_atoms >= 500_000 and s.n_elements >= 20_000
+    assert s.n_atoms + 11 ** 3 * s.n_elements == 4 * 198 ** 3
