@@ -654,6 +654,10 @@ int cac_ctx_create(int device, cac_ctx **out) {
     cac_ctx_destroy(ctx);
     return CAC_ERR_HIP;
   }
+  if (noalloc_init(ctx) != CAC_OK) {
+    cac_ctx_destroy(ctx);
+    return CAC_ERR_HIP;
+  }
   *out = ctx;
   return CAC_OK;
 }

@@ -394,6 +394,10 @@ int launch_interp_sites(cac_ctx *ctx, double *dout);
 // ---- launchers (cac_api.hip)
 int launch_reduce_ev(cac_ctx *ctx, int eflag, int vflag);
 
+// ---- the passes that allocate nothing (k_noalloc.hip): this device's
+// address of their fixed scratch array, looked up at context creation
+int noalloc_init(cac_ctx *ctx);
+
 // ---- MLP and surrogate (k_mlp.hip, k_surrogate.hip)
 void mlp_destroy(cac_ctx *ctx);
 int surrogate_classify(cac_ctx *ctx);

@@ -231,6 +231,21 @@ int cac_get_rho(cac_ctx *ctx, double *rho, double *fp);
 int cac_get_erho(cac_ctx *ctx, double *erho);
 int cac_get_ip_forces(cac_ctx *ctx, double *ipf);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops with fixed memory):
+ * the EAM atom passes of cac_force (PairEAM::compute's rho, fp and force
+ * loops, pair_eam.cpp:620-745) over caller-owned DEVICE arrays, allocating
+ * nothing.  x: [n][3] positions; numneigh: [n]; neigh: [n][maxneigh] full
+ * lists of entries j | img << 27 (atom j < n at image img, as in
+ * cac_get_atom_neighbors' low 32 bits; other entries are skipped); f: [n][3]
+ * forces out; rho: [n] densities out, or NULL.  Tables, box and stream are
+ * the context's (cac_set_eam, cac_set_box).  F'(rho) lives in a fixed
+ * per-device scratch array of the library: n <= 1,048,576 (8 MB), larger n
+ * fails with CAC_ERR_ARG.  Queued on the context stream (cac_synchronize
+ * waits); calls that overlap on one device share the scratch and race. */
+int cac_eam_atoms_noalloc(cac_ctx *ctx, int n, const double *x, int maxneigh, const int *numneigh,
+                          const uint32_t *neigh, double *f, double *rho);
+
 /* Timing hook: average duration (ms) of the named kernel family over the
  * launches recorded since the last reset, measured with HIP events on the
  * context stream ("atom_force", "ip_force", ...); returns -1 if unknown. */
