@@ -1073,16 +1073,15 @@ __device__ __forceinline__ void cl_force_body(const ClArgs &c, const ElemArgs &a
   int p = 0, slot = 0;
   if (gi < ns) {
     slot = c.slot[s0 + gi];
-    const int i = slot >> 3, g = a.node_list[slot & 7];
-    p = i * a.nint + g;
-    const double *Ng = a.spa_int + 8 * g;
-    double ix = 0.0, iy = 0.0, iz = 0.0;
-    for (int kk = 0; kk < 8; ++kk) {
-      ix += Ng[kk] * a.nodex[24 * (size_t)i + 3 * kk];
-      iy += Ng[kk] * a.nodex[24 * (size_t)i + 3 * kk + 1];
-      iz += Ng[kk] * a.nodex[24 * (size_t)i + 3 * kk + 2];
-      rhoi += Ng[kk] * erho[8 * (size_t)i + kk];
-    }
+    const int i = slot >> 3, k = slot & 7;
+    p = i * a.nint + a.node_list[k];
+    // the integration point is node k of element i (the cluster's gi-th
+    // slot, whose nodes and node densities cl_fill put in LDS): the
+    // shape-function sums over its unit shape row give exactly the node's
+    // position and density
+    const double *nodes = rows + 4 * cap, *enode = nodes + kClMax * 24;
+    const double ix = nodes[24 * gi + 3 * k], iy = nodes[24 * gi + 3 * k + 1], iz = nodes[24 * gi + 3 * k + 2];
+    rhoi = enode[8 * gi + k];
     fpi = embed_fp<P>(t, rhoi);
     const int jn = a.ipnum[p];
     const uint16_t *row = c.local + (size_t)(p >> 3) * a.M * kRowGroup + (p & 7);

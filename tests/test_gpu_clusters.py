@@ -1,13 +1,18 @@
 """Element passes by vertex cluster (cac_set_ip_clusters) against the
-per-integration-point passes: the same forces bit for bit, FP64 and mixed,
-with and without atom-element coupling, and the clusters' own checks."""
+per-integration-point passes, FP64 and mixed, with and without atom-element
+coupling, and the clusters' own checks.  The two are separate kernels that
+the compiler fuses multiplies and adds in independently (e.g. which square
+of r^2 = dx*dx + dy*dy + dz*dz is fused), so they agree to rounding:
+1e-12 norm-relative, against the oracle's 1e-10 bar."""
 import numpy as np
 import pytest
 
 from cac_code_amd import Engine, PairEAM, _lib
 from cac_code_amd.lattice import vertex_clusters
 
-from gpu_common import CUTNEIGH_EAM, small_system
+from gpu_common import CUTNEIGH_EAM, rel_err, small_system
+
+TOL = 1e-12
 
 pytestmark = pytest.mark.gpu
 
@@ -64,8 +69,12 @@ def test_clusters_equal_per_point_passes(name, coupled, precision, cu_funcfl):
     ev = precision == _lib.PREC_FP64
     a = forces(s, pair, True, coupled, precision, ev)
     b = forces(s, pair, False, coupled, precision, ev)
-    for i, (qa, qb) in enumerate(zip(a, b)):
-        assert np.array_equal(qa, qb), f"quantity {i} differs"
+    names = ["f", "nodef", "erho", "ipf", "energy", "virial", "x", "v", "f3", "nodex", "nodev", "nodef3"]
+    names = names if len(a) == len(names) else names[:4] + names[6:]
+    diff = {n: rel_err(qa, qb) for n, qa, qb in zip(names, a, b)}
+    print("  max |cluster - per point| / max |per point|:", {k: f"{v:.1e}" for k, v in diff.items()}, flush=True)
+    bad = [k for k, v in diff.items() if not v <= TOL]
+    assert not bad, f"differ: {bad}"
 
 
 def test_cluster_argument_checks():

@@ -176,7 +176,7 @@ def test_trained_surrogate_gain(pair, small_model):
 
 def test_fallback_elements_keep_plain_forces(pair, small_model):
     """With the surrogate on, interface elements use the plain path on the
-    subsets F, R, R2 and get exactly the plain nodal forces; the others get
+    subsets F, R, R2 and get the plain nodal forces (to rounding); the others get
     the surrogate's (linear part on the device in FP32, MLP in BF16)."""
     _, model = small_model
     s = build_cac_system(33, 11, 8.0, temperature=300.0, seed=3)
@@ -200,7 +200,10 @@ def test_fallback_elements_keep_plain_forces(pair, small_model):
     fb = cls != 0
     assert fb.sum() == info["interface"] + info["strained"] and (~fb).sum() == info["surrogate"]
     assert fb.any() and (~fb).any()
-    assert np.array_equal(surr[fb], plain[fb])
-    assert np.array_equal(fsurr, fplain)
+    # the plain forces: per-point kernels for the fallback subsets against the
+    # cluster kernels of the full plain call, equal to rounding (independently
+    # fused multiply-adds)
+    assert np.max(np.abs(surr[fb] - plain[fb])) <= 1e-12 * np.max(np.abs(plain[fb]))
+    assert np.max(np.abs(fsurr - fplain)) <= 1e-12 * max(np.max(np.abs(fplain)), 1e-30)
     pred = S.predict(model, f[~fb])
     assert np.allclose(surr[~fb].reshape(-1, 24), pred, rtol=2e-2, atol=2e-3 * np.abs(pred).max())
