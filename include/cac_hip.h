@@ -12,6 +12,11 @@
  * Each declaration names the reference interface it replaces.  Reference
  * paths are relative to the fangyuanxiao/CAC_Code tree.
  *
+ * One atom type and one element type per context: the reference's element
+ * path hard-codes type 1 (pair_eam.cpp:980, 1047, 1091, 1109, 1111;
+ * pair_lj_cut.cpp:327-331), so the uploads take one atom mass and one
+ * element shape, and the host bindings refuse more than one type of each.
+ *
  * Neighbor entry encoding (cac_get_atom_neighbors / cac_get_ip_neighbors),
  * uint64: j | (img << 27) | ((uint64)site << 32), where site ==
  * CAC_ATOM_SITE names atom j and any other value site `site` of element j.
