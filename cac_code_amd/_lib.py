@@ -57,6 +57,7 @@ SIGNATURES = {
     "cac_centro_atoms": (C.c_int, [_ctx, C.c_int, _d]),
     "cac_interpolate_sites": (C.c_int, [_ctx, _d]),
     "cac_get_capacities": (C.c_int, [_ctx, _i, _i]),
+    "cac_thermo": (C.c_int, [_ctx, C.c_double, C.c_double, C.c_double, _d]),
     "cac_eam_atoms_noalloc": (C.c_int, [_ctx, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.c_void_p]),
     "cac_get_atom_neighbors": (C.c_int, [_ctx, C.c_int, _i, _u64]),

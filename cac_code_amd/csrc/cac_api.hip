@@ -256,6 +256,26 @@ __global__ void k_reduce(RedArgs a, double *__restrict__ out) {
   if (threadIdx.x == 0) out[q] = s[0];
 }
 
+// Sum of m*v^2 over the atoms (block 0) and over the nodes (block 1, node
+// mass m_e/8): a fixed stride pattern and a fixed LDS tree, so the same
+// bits every call.
+__global__ void k_mv2(int n, const double *__restrict__ v, double m, long nq, const double *__restrict__ nodev,
+                      double mn, double *__restrict__ out) {
+  __shared__ double s[1024];
+  const bool atoms = blockIdx.x == 0;
+  const long len = atoms ? 3L * n : nq;
+  const double *src = atoms ? v : nodev;
+  double acc = 0.0;
+  for (long i = threadIdx.x; i < len; i += blockDim.x) acc += src[i] * src[i];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = (atoms ? m : mn) * s[0];
+}
+
 int upload_tables(cac_ctx *ctx, int nrho, double drho, const double *frho_spline, int nr, double dr,
                   const double *rhor_spline, const double *z2r_spline, double cutforcesq) {
   EamTables &t = ctx->eam;
@@ -1121,6 +1141,40 @@ int cac_get_energy_virial(cac_ctx *ctx, double eng[2], double virial[6]) {
   if (eng) { eng[0] = ctx->eng[0]; eng[1] = ctx->eng[1]; }
   if (virial)
     for (int c = 0; c < 6; ++c) virial[c] = ctx->virial[c];
+  return CAC_OK;
+}
+
+int cac_thermo(cac_ctx *ctx, double mvv2e, double boltz, double nktv2p, double out[5]) {
+  if (!ctx || !out || !(boltz > 0)) return CAC_ERR_ARG;
+  hipSetDevice(ctx->device);
+  int rc = ready_for_force(ctx);
+  if (rc) return rc;
+  CAC_HIP(ctx, hipMemsetAsync(ctx->dflags + 15, 0, sizeof(int), ctx->stream));
+  if ((rc = force_impl(ctx, 1, 1, CAC_PREC_FP64))) return rc;
+  if ((rc = ensure(ctx, ctx->b_red, 64 * 16, &ctx->red))) return rc;
+  const double inv_npe = 1 / (double)8;
+  hipLaunchKernelGGL(k_mv2, dim3(2), dim3(1024), 0, ctx->stream, ctx->n, ctx->v, ctx->mass, 24L * ctx->ne,
+                     ctx->nodev, inv_npe * ctx->mass_e, ctx->red);
+  CAC_HIP(ctx, hipGetLastError());
+  double mv2[2];
+  CAC_HIP(ctx, hipMemcpyAsync(mv2, ctx->red, sizeof(mv2), hipMemcpyDeviceToHost, ctx->stream));
+  CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  // ComputeTemp / ComputePressure / thermo pe, ke, etotal as LAMMPS defines
+  // them (the reference's computes are empty, compute_pe.cpp:16-49,
+  // compute_temp.cpp:15-32, compute_pressure.cpp:18-138): dof = 3 N - 3 over
+  // the integrated particles (atoms and nodes)
+  const double pe = ctx->eng[0] + ctx->eng[1];
+  const double ke = 0.5 * mvv2e * (mv2[0] + mv2[1]);
+  const long np = (long)ctx->n + 8L * ctx->ne;
+  const double dof = 3.0 * (double)np - 3.0;
+  const double temp = dof > 0 ? 2.0 * ke / (dof * boltz) : 0.0;
+  const double vol = ctx->prd[0] * ctx->prd[1] * ctx->prd[2];
+  const double press = (dof * boltz * temp + ctx->virial[0] + ctx->virial[1] + ctx->virial[2]) / 3.0 / vol * nktv2p;
+  out[0] = pe;
+  out[1] = ke;
+  out[2] = temp;
+  out[3] = press;
+  out[4] = pe + ke;
   return CAC_OK;
 }
 

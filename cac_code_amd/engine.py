@@ -21,7 +21,7 @@ from typing import Optional
 import numpy as np
 
 from . import _lib
-from .lattice import CACSystem, FTM2V_METAL, vertex_clusters
+from .lattice import BOLTZ_METAL, CACSystem, FTM2V_METAL, MVV2E_METAL, NKTV2P_METAL, vertex_clusters
 from .potential import Funcfl, lj_coeffs, read_funcfl
 
 SURR_NFEAT, SURR_NOUT, SURR_NQUAD = 279, 24, 561   # CAC_SURR_* in cac_hip.h
@@ -418,6 +418,26 @@ class Engine(_MlpMixin):
         e = np.zeros(2); w = np.zeros(6)
         _check(self.ctx, self.L.cac_get_energy_virial(self.ctx, _p(e), _p(w)))
         return e, w
+
+    # ---- thermodynamic output (thermo_style pe ke temp press etotal; the
+    # reference prints only the step, thermo.cpp:195-209)
+    def thermo(self, mvv2e: float = MVV2E_METAL, boltz: float = BOLTZ_METAL, nktv2p: float = NKTV2P_METAL):
+        """pe, ke, temp, press, etotal of the current state (metal units by
+        default): one FP64 force call with the tallies on the device."""
+        out = np.zeros(5)
+        _check(self.ctx, self.L.cac_thermo(self.ctx, mvv2e, boltz, nktv2p, _p(out)))
+        return dict(zip(("pe", "ke", "temp", "press", "etotal"), (float(v) for v in out)))
+
+    @staticmethod
+    def thermo_header() -> str:
+        return f"{'Step':>10} {'PotEng':>18} {'KinEng':>16} {'Temp':>12} {'Press':>14} {'TotEng':>18}"
+
+    @staticmethod
+    def thermo_line(step: int, t: dict) -> str:
+        """One thermo line, the columns of thermo_header (LAMMPS' one-line
+        format: step as an integer, the rest %g-like)."""
+        return (f"{step:>10d} {t['pe']:>18.8f} {t['ke']:>16.8f} {t['temp']:>12.6f} {t['press']:>14.6f} "
+                f"{t['etotal']:>18.8f}")
 
     # ---- centro-symmetry and the xyz dump (Output::centro_atom,
     # output.cpp:352-441; DumpXyz, dump_xyz.cpp:36-76, 127-141)

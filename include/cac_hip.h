@@ -202,6 +202,21 @@ int cac_download_elements(cac_ctx *ctx, double *nodex, double *nodev, double *no
  * reference: integrate.cpp:97-114 forces eflag = vflag = 0). */
 int cac_get_energy_virial(cac_ctx *ctx, double eng[2], double virial[6]);
 
+/* Thermodynamic output of the current state (the reference's thermo prints
+ * only the step, thermo.cpp:195-209, and its computes are empty,
+ * compute_pe.cpp:16-49, compute_temp.cpp:15-32, compute_pressure.cpp:18-138;
+ * these are LAMMPS' definitions).  Runs one FP64 force call with the energy
+ * and virial tallies (forces are recomputed on the same positions, so they
+ * do not change) and returns out = {pe, ke, temp, press, etotal}:
+ *   pe    = atom + integration-point energies (cac_get_energy_virial),
+ *   ke    = 0.5 * mvv2e * (sum m v^2 over atoms + sum (m_e/8) v^2 over nodes),
+ *   temp  = 2 ke / (dof * boltz), dof = 3 (n + 8 ne) - 3,
+ *   press = (dof*boltz*temp + W_xx + W_yy + W_zz) / (3 V) * nktv2p,
+ *   etotal = pe + ke.
+ * Unit constants from the caller (metal: mvv2e = 1.0364269e-4,
+ * boltz = 8.617343e-5, nktv2p = 1.6021765e6, update.cpp:107-125). */
+int cac_thermo(cac_ctx *ctx, double mvv2e, double boltz, double nktv2p, double out[5]);
+
 /* Centro-symmetry parameter of every atom (replaces Output::centro_atom,
  * output.cpp:352-441, which verlet.cpp:93-94 calls after Neighbor::build):
  * over the list entries with r^2 < cutneigh^2 (cutneighmaxsq), the nnn
