@@ -463,10 +463,13 @@ __global__ __launch_bounds__(kAtomWG) void k_eam_atom_rho(int n, double *__restr
     const int W = (M + 31) >> 5;
     uint32_t bits = 0, bits_in = 0;
     double rho = 0.0;
+    // software pipeline: the list words (and inner-mask word) of chunk
+    // k0 + kUA are loaded while the neighbor rows of chunk k0 are in flight
+    uint32_t e[kUA];
+    fetch_atom_entries<kUA>(row, 0, jn, e);
+    uint32_t in_w = (inner && jn > 0) ? mask_in[mask_slot(i, 0, W)] : 0u;
     for (int k0 = 0; k0 < jn; k0 += kUA) {
-      if (inner && (k0 & 31) == 0) bits_in = mask_in[mask_slot(i, k0 >> 5, W)];
-      uint32_t e[kUA];
-      fetch_atom_entries<kUA>(row, k0, jn, e);
+      if (inner && (k0 & 31) == 0) bits_in = in_w;
       if (inner) {  // element sites (rows >= n, moved by their nodes) are always gathered
         const uint32_t sel = bits_in >> (k0 & 31);
   #pragma unroll
@@ -477,6 +480,11 @@ __global__ __launch_bounds__(kAtomWG) void k_eam_atom_rho(int n, double *__restr
   #pragma unroll
       for (int u = 0; u < kUA; ++u)
         if (e[u] != kNoEntry) xj[u] = load_xyz(x4, e[u], prd);
+      __asm__ volatile("" ::: "memory");  // keep the next chunk's loads behind this chunk's gathers
+      const int k1 = k0 + kUA;
+      uint32_t en[kUA];
+      fetch_atom_entries<kUA>(row, k1, jn, en);
+      if (inner && (k1 & 31) == 0 && k1 < jn) in_w = mask_in[mask_slot(i, k1 >> 5, W)];
   #pragma unroll
       for (int u = 0; u < kUA; ++u) {
         if (e[u] == kNoEntry) continue;
@@ -494,6 +502,8 @@ __global__ __launch_bounds__(kAtomWG) void k_eam_atom_rho(int n, double *__restr
         bits = 0;
         bits_in = 0;
       }
+  #pragma unroll
+      for (int u = 0; u < kUA; ++u) e[u] = en[u];
     }
     if (!inner) {
       xhold_in[i] = xi;
@@ -538,12 +548,13 @@ __global__ __launch_bounds__(kAtomWG) void k_eam_atom_force(int n, const double 
     const int jn = num[i];
     const uint32_t *row = neigh + atom_slot(i, 0, M);
     const int W = (M + 31) >> 5;
-    uint32_t bits = 0;
+    // software pipeline: the list words and mask word of chunk k0 + kUA are
+    // loaded while the neighbor rows of chunk k0 are in flight
+    uint32_t e[kUA];
+    fetch_atom_entries<kUA>(row, 0, jn, e);
+    uint32_t bits = jn > 0 ? mask[mask_slot(i, 0, W)] : 0u;
     for (int k0 = 0; k0 < jn; k0 += kUA) {
-      if ((k0 & 31) == 0) bits = mask[mask_slot(i, k0 >> 5, W)];
       const uint32_t sel = bits >> (k0 & 31);
-      uint32_t e[kUA];
-      fetch_atom_entries<kUA>(row, k0, jn, e);
   #pragma unroll
       for (int u = 0; u < kUA; ++u)
         if (!((sel >> u) & 1u)) e[u] = kNoEntry;
@@ -552,6 +563,11 @@ __global__ __launch_bounds__(kAtomWG) void k_eam_atom_force(int n, const double 
   #pragma unroll
       for (int u = 0; u < kUA; ++u)
         if (e[u] != kNoEntry) xjv[u] = load_xyzw(x4, e[u], prd, &fpjv[u]);
+      __asm__ volatile("" ::: "memory");  // keep the next chunk's loads behind this chunk's gathers
+      const int k1 = k0 + kUA;
+      uint32_t en[kUA];
+      fetch_atom_entries<kUA>(row, k1, jn, en);
+      const uint32_t bits_n = ((k1 & 31) == 0 && k1 < jn) ? mask[mask_slot(i, k1 >> 5, W)] : bits;
   #pragma unroll
       for (int u = 0; u < kUA; ++u) {
         if (e[u] == kNoEntry) continue;
@@ -570,6 +586,9 @@ __global__ __launch_bounds__(kAtomWG) void k_eam_atom_force(int n, const double 
           }
         }
       }
+  #pragma unroll
+      for (int u = 0; u < kUA; ++u) e[u] = en[u];
+      bits = bits_n;
     }
     f[i] = fx;
     f[(size_t)n + i] = fy;
@@ -900,6 +919,20 @@ __device__ __forceinline__ void site_shape(const ElemArgs &a, int s, double N[8]
   N[7] = 0.125 * (1 - px) * (1 + py) * (1 + pz);
 }
 
+// Union indices of a lane's next kU list entries (k, k+8, ...), -1 past
+// the end or where the selection bits are clear.
+template <int U>
+__device__ __forceinline__ void cl_indices(const uint16_t *__restrict__ row, int k, int jn, uint32_t sel, int q[U]) {
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int kk = k + u * kLanes;
+    q[u] = (kk < jn && ((sel >> u) & 1u)) ? (int)row[(size_t)kk * kRowGroup] : -1;
+  }
+}
+__device__ __forceinline__ uint32_t k0_mask(const uint32_t *__restrict__ mrow, int jn, int l) {
+  return l < jn ? mrow[0] : 0u;
+}
+
 // LDS of a cluster wavefront: rows [c.rows][4] (x, y, z, fp of every union
 // entry), then the nodes [8][24] and node densities [8][8] of the cluster's
 // own elements (the elements whose nodes meet at the vertex: nearly every
@@ -992,13 +1025,15 @@ __device__ __forceinline__ void cl_erho_body(const ClArgs &c, const ElemArgs &a,
     const int Wl = (a.M + 8 * 32 - 1) / (8 * 32);
     uint32_t *mrow = mask + ((size_t)p * kLanes + l) * Wl;
     uint32_t bits = 0;
+    // software pipeline: the next chunk's union indices are requested
+    // before this chunk's spline rows
+    int qn[kU];
+    cl_indices<kU>(row, l, jn, 0xFFFFFFFFu, qn);
     for (int k0 = l, j0 = 0; k0 < jn; k0 += kU * kLanes, j0 += kU) {
       int q[kU];
 #pragma unroll
-      for (int u = 0; u < kU; ++u) {
-        const int kk = k0 + u * kLanes;
-        q[u] = kk < jn ? (int)row[(size_t)kk * kRowGroup] : -1;
-      }
+      for (int u = 0; u < kU; ++u) q[u] = qn[u];
+      cl_indices<kU>(row, k0 + kU * kLanes, jn, 0xFFFFFFFFu, qn);
       // distances and knots, then the rows (all in flight), then the sums
       P pp[kU];
       int mm[kU];
@@ -1087,20 +1122,25 @@ __device__ __forceinline__ void cl_force_body(const ClArgs &c, const ElemArgs &a
     const uint16_t *row = c.local + (size_t)(p >> 3) * a.M * kRowGroup + (p & 7);
     const int Wl = (a.M + 8 * 32 - 1) / (8 * 32);
     const uint32_t *mrow = mask + ((size_t)p * kLanes + l) * Wl;
-    uint32_t bits = 0;
+    // software pipeline: the union indices and mask word of the next chunk
+    // are requested before this chunk's spline rows, so they arrive while
+    // this chunk computes
+    uint32_t bits = k0_mask(mrow, jn, l);
+    int q[kU];
+    cl_indices<kU>(row, l, jn, bits, q);
     for (int k0 = l, j0 = 0; k0 < jn; k0 += kU * kLanes, j0 += kU) {
-      if ((j0 & 31) == 0) bits = mrow[j0 >> 5];
-      const uint32_t sel = bits >> (j0 & 31);
-      int q[kU];
+      int qc[kU];
 #pragma unroll
-      for (int u = 0; u < kU; ++u) {
-        const int kk = k0 + u * kLanes;
-        q[u] = (kk < jn && ((sel >> u) & 1u)) ? (int)row[(size_t)kk * kRowGroup] : -1;
+      for (int u = 0; u < kU; ++u) qc[u] = q[u];
+      {
+        const int j1 = j0 + kU, k1 = k0 + kU * kLanes;
+        if ((j1 & 31) == 0 && k1 < jn) bits = mrow[j1 >> 5];
+        cl_indices<kU>(row, k1, jn, bits >> (j1 & 31), q);
       }
 #pragma unroll
       for (int u = 0; u < kU; ++u) {
-        if (q[u] < 0) continue;
-        const double *r = rows + 4 * q[u];
+        if (qc[u] < 0) continue;
+        const double *r = rows + 4 * qc[u];
         const double delx = ix - r[0], dely = iy - r[1], delz = iz - r[2];
         const double rsq = delx * delx + dely * dely + delz * delz;
         if (rsq < t.cutforcesq) {
