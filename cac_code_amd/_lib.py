@@ -1,4 +1,5 @@
-"""ctypes binding of libcac_hip.so (the C-ABI in include/cac_hip.h).
+"""ctypes binding of libcac_hip.so (the C-ABI in include/cac_hip.h and
+include/cac_io.h).
 
 Loading fails loudly when the library is missing: there is no CPU fallback on
 the product path.  Build it with ``python -m cac_code_amd.build``.
@@ -11,9 +12,11 @@ import re
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libcac_hip.so")
-HEADER = os.path.join(os.path.dirname(HERE), "include", "cac_hip.h")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+HEADER = os.path.join(INCLUDE, "cac_hip.h")
+HEADERS = (HEADER, os.path.join(INCLUDE, "cac_io.h"))
 
-CAC_OK, CAC_ERR_ARG, CAC_ERR_HIP, CAC_ERR_OVERFLOW, CAC_ERR_BOX, CAC_ERR_DATA = range(6)
+CAC_OK, CAC_ERR_ARG, CAC_ERR_HIP, CAC_ERR_OVERFLOW, CAC_ERR_BOX, CAC_ERR_DATA, CAC_ERR_IO = range(7)
 PREC_FP64, PREC_MIXED = 0, 1
 ATOM_SITE = 0xFFFFFFFF  # CAC_ATOM_SITE: site field of a list entry naming an atom
 
@@ -23,6 +26,19 @@ _u32 = C.POINTER(C.c_uint32)
 _u64 = C.POINTER(C.c_uint64)
 _f = C.POINTER(C.c_float)
 _ctx = C.c_void_p
+_i64 = C.POINTER(C.c_int64)
+
+
+class DataView(C.Structure):
+    """cac_data_view (include/cac_io.h)."""
+    _fields_ = [("comment", C.c_char_p), ("lo", C.c_double * 3), ("hi", C.c_double * 3),
+                ("ntypes", C.c_int), ("entypes", C.c_int), ("masses", _d), ("nae", _i), ("numint", _i),
+                ("nintp", C.c_int64), ("intp_ijk", _i), ("intp_w", _d),
+                ("natoms", C.c_int64), ("nelements", C.c_int64),
+                ("atom_tag", _i64), ("atom_type", _i), ("x", _d), ("atom_image", _i),
+                ("elem_tag", _i64), ("elem_type", _i), ("xc", _d), ("elem_image", _i),
+                ("nodex", _d), ("nodev", _d), ("nodetag", _i64), ("node_velocities", C.c_int)]
+
 
 # name -> (restype, argtypes)
 SIGNATURES = {
@@ -81,6 +97,16 @@ SIGNATURES = {
     "cac_surrogate_enable": (C.c_int, [_ctx, C.c_int]),
     "cac_surrogate_features": (C.c_int, [_ctx, _f, _i]),
     "cac_surrogate_info": (C.c_int, [_ctx, _i, _i, _i]),
+    # on-disk formats (include/cac_io.h)
+    "cac_io_last_error": (C.c_char_p, []),
+    "cac_data_read": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    "cac_data_get": (C.POINTER(DataView), [C.c_void_p]),
+    "cac_data_free": (None, [C.c_void_p]),
+    "cac_data_write": (C.c_int, [C.c_char_p, C.POINTER(DataView), C.c_int]),
+    "cac_dump_atom_write": (C.c_int, [C.c_char_p, C.c_int, C.c_int64, _d, _d, _i, C.c_int64, _i64, _i, _d, _d, _d,
+                                      C.c_int64, _i64, _d, _d, _d]),
+    "cac_dump_element_write": (C.c_int, [C.c_char_p, C.c_int, C.c_int64, C.c_int64, _d, _d, _d]),
+    "cac_eam_tables_from_setfl": (C.c_int, [C.c_int, C.c_double, C.c_int, C.c_double, _d, _d, _d, _d, _d, _d]),
 }
 
 
@@ -91,9 +117,12 @@ class CacError(RuntimeError):
 
 
 def header_symbols():
-    """Function names declared in include/cac_hip.h."""
-    text = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:int|double|const char \*)\s*\**\s*(cac_\w+)\s*\(", text, re.M)))
+    """Function names declared in include/*.h."""
+    names = set()
+    for h in HEADERS:
+        text = open(h).read()
+        names.update(re.findall(r"^\s*(?:const\s+)?\w+\s*\**\s*(cac_\w+)\s*\(", text, re.M))
+    return sorted(names)
 
 
 _lib = None

@@ -23,7 +23,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcac_hip.so")
 
 SOURCES = ["cac_api.hip", "k_neighbor.hip", "k_pair.hip", "k_centro.hip", "k_mlp.hip", "k_surrogate.hip",
-           "k_noalloc.hip", "tables.cpp"]
+           "k_noalloc.hip", "tables.cpp", "io.cpp"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CAC_OFFLOAD_ARCH", "gfx950")

@@ -38,6 +38,8 @@ void lagrange_resample(const std::vector<double> &src, int nsrc, double hsrc, in
   }
 }
 
+}  // namespace
+
 // Knot coefficients; out is [(n+1)][7], row 0 zero.
 void knot_coefficients(const std::vector<double> &f, int n, double delta, double *out) {
   auto c = [out](int m, int j) -> double & { return out[7 * m + j]; };
@@ -62,8 +64,6 @@ void knot_coefficients(const std::vector<double> &f, int n, double delta, double
     c(m, 0) = 3.0 * c(m, 3) / delta;
   }
 }
-
-}  // namespace
 
 int eam_regrid_spline(int nrho_f, double drho_f, int nr_f, double dr_f, const double *frho_f,
                       const double *rhor_f, const double *zr_f, int *nrho, double *drho, int *nr,
@@ -104,4 +104,24 @@ extern "C" int cac_eam_tables_from_funcfl(int nrho_f, double drho_f, int nr_f, d
   if ((frho_spline || rhor_spline || z2r_spline) && (!frho_f || !rhor_f || !zr_f)) return CAC_ERR_ARG;
   return cac::eam_regrid_spline(nrho_f, drho_f, nr_f, dr_f, frho_f, rhor_f, zr_f, nrho_out, drho_out,
                                 nr_out, dr_out, frho_spline, rhor_spline, z2r_spline);
+}
+
+// setfl (LAMMPS eam/alloy, one element): the file's arrays are already on
+// the common grid, so they become the knot values as they are (no resample)
+// and get the same knot coefficients as a funcfl's resampled arrays.
+extern "C" int cac_eam_tables_from_setfl(int nrho, double drho, int nr, double dr, const double *frho,
+                                         const double *rhor, const double *z2r, double *frho_spline,
+                                         double *rhor_spline, double *z2r_spline) {
+  if (nrho < 5 || nr < 5 || !(drho > 0) || !(dr > 0)) return CAC_ERR_ARG;
+  if (!frho || !rhor || !z2r || !frho_spline || !rhor_spline || !z2r_spline) return CAC_ERR_ARG;
+  std::vector<double> f;
+  auto one = [&f](const double *src, int n, double h, double *out) {
+    f.assign(n + 1, 0.0);
+    for (int m = 0; m < n; ++m) f[m + 1] = src[m];
+    cac::knot_coefficients(f, n, h, out);
+  };
+  one(frho, nrho, drho, frho_spline);
+  one(rhor, nr, dr, rhor_spline);
+  one(z2r, nr, dr, z2r_spline);
+  return CAC_OK;
 }

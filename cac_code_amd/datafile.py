@@ -26,14 +26,21 @@ Numbers are written with 17 significant digits, so a written and re-read
 system is the same bit for bit.  Atom velocities are not in the format (the
 Atoms lines carry none, and ReadData::command rejects a Velocities
 section); node velocities are, in 9-word Nodes lines.
+
+`read_data` and `write_data` run in the native library (include/cac_io.h:
+the file is read whole and its large sections parsed and formatted in
+parallel); `read_data_py` and `write_data_py` do the same in Python and
+produce the same arrays and the same bytes.
 """
 from __future__ import annotations
 
+import ctypes as C
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional, Tuple
 
 import numpy as np
 
+from . import _lib
 from .lattice import CACSystem, vertex_clusters
 
 # the sections ReadData::command handles (read_data.cpp:155-215); any other
@@ -81,8 +88,8 @@ def _word_count(line: str) -> int:
     return len(line.split())
 
 
-def read_data(path: str) -> DataFile:
-    """Parse a CAC data file (ReadData::command, read_data.cpp:70-240)."""
+def read_data_py(path: str) -> DataFile:
+    """Parse a CAC data file in Python (ReadData::command, read_data.cpp:70-240)."""
     with open(path) as fh:
         lines = fh.read().split("\n")
     if lines and lines[-1] == "":
@@ -245,9 +252,9 @@ def _g(v: float) -> str:
     return format(float(v), ".17g")
 
 
-def write_data(path: str, d: DataFile, node_velocities: bool = True) -> None:
-    """Write d in the reference's format (one blank line after each section
-    keyword, as parse_keyword expects)."""
+def write_data_py(path: str, d: DataFile, node_velocities: bool = True) -> None:
+    """Write d in the reference's format in Python (one blank line after each
+    section keyword, as parse_keyword expects)."""
     N, M = d.x.shape[0], d.xc.shape[0]
     out = [d.comment.replace("\n", " "), ""]
     out.append(f"{N} atoms")
@@ -296,6 +303,125 @@ def write_data(path: str, d: DataFile, node_velocities: bool = True) -> None:
             for i in range(N)])
     with open(path, "w") as fh:
         fh.write("\n".join(out))
+
+
+def _arr(ptr, shape, dtype):
+    n = int(np.prod(shape))
+    if n == 0:
+        return np.zeros(shape, dtype=dtype)
+    return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True).reshape(shape)
+
+
+def read_data(path: str) -> DataFile:
+    """Parse a CAC data file in the native library (cac_data_read; the
+    reference's ReadData::command, read_data.cpp:70-240).  Raises
+    DataFileError with the reference's message for a malformed file, OSError
+    for one that cannot be read."""
+    L = _lib.lib()
+    h = C.c_void_p()
+    rc = L.cac_data_read(str(path).encode(), C.byref(h))
+    if rc != _lib.CAC_OK:
+        msg = L.cac_io_last_error().decode()
+        raise (OSError if rc == _lib.CAC_ERR_IO else DataFileError)(msg)
+    try:
+        v = L.cac_data_get(h).contents
+        N, M = int(v.natoms), int(v.nelements)
+        d = DataFile(comment=v.comment.decode(), box_lo=np.array(v.lo[:]), box_hi=np.array(v.hi[:]),
+                     ntypes=int(v.ntypes), entypes=int(v.entypes))
+        if v.masses:
+            m = _arr(v.masses, (d.ntypes,), np.float64)
+            d.masses = {t + 1: float(m[t]) for t in range(d.ntypes) if not np.isnan(m[t])}
+        if v.nae:
+            a = _arr(v.nae, (d.entypes, 3), np.int64)
+            d.nae = {t + 1: tuple(int(c) for c in a[t]) for t in range(d.entypes) if a[t, 0] > 0}
+        if v.numint:
+            ni = _arr(v.numint, (d.entypes,), np.int64)
+            d.numint = {t + 1: int(ni[t]) for t in range(d.entypes) if ni[t] >= 0}
+            if v.intp_w or v.nintp:
+                ijk = _arr(v.intp_ijk, (int(v.nintp), 3), np.float64)
+                w = _arr(v.intp_w, (int(v.nintp),), np.float64)
+                row = 0
+                for t in range(1, d.entypes + 1):
+                    n = d.numint.get(t, -1)
+                    if n < 0:
+                        continue
+                    d.intp[t] = np.column_stack([ijk[row:row + n], w[row:row + n]]) if n else np.zeros((0, 4))
+                    row += n
+        d.atom_tag = _arr(v.atom_tag, (N,), np.int64)
+        d.atom_type = _arr(v.atom_type, (N,), np.int64)
+        d.x = _arr(v.x, (N, 3), np.float64)
+        d.atom_image = _arr(v.atom_image, (N, 3), np.int64) if v.atom_image else None
+        d.elem_tag = _arr(v.elem_tag, (M,), np.int64)
+        d.elem_type = _arr(v.elem_type, (M,), np.int64)
+        d.xc = _arr(v.xc, (M, 3), np.float64)
+        d.elem_image = _arr(v.elem_image, (M, 3), np.int64) if v.elem_image else None
+        d.nodex = _arr(v.nodex, (M, 8, 3), np.float64)
+        d.nodev = _arr(v.nodev, (M, 8, 3), np.float64)
+        d.nodetag = _arr(v.nodetag, (M, 8), np.int64)
+    finally:
+        L.cac_data_free(h)
+    return d
+
+
+def write_data(path: str, d: DataFile, node_velocities: bool = True) -> None:
+    """Write d in the reference's format in the native library
+    (cac_data_write): the bytes write_data_py writes."""
+    N, M = d.x.shape[0], d.xc.shape[0]
+    keep = []
+
+    def ptr(a, dtype, ctype):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        keep.append(a)
+        return a.ctypes.data_as(C.POINTER(ctype))
+
+    v = _lib.DataView()
+    v.comment = d.comment.encode()
+    v.lo[:] = [float(c) for c in d.box_lo]
+    v.hi[:] = [float(c) for c in d.box_hi]
+    v.ntypes, v.entypes = int(d.ntypes), int(d.entypes)
+    if d.masses:
+        m = np.full(d.ntypes, np.nan)
+        for t, val in d.masses.items():
+            m[t - 1] = val
+        v.masses = ptr(m, np.float64, C.c_double)
+    nae = np.zeros((d.entypes, 3), np.int32)
+    for t, val in d.nae.items():
+        nae[t - 1] = val
+    ni = np.full(d.entypes, -1, np.int32)
+    for t, val in d.numint.items():
+        ni[t - 1] = val
+    rows = []
+    for t in range(1, d.entypes + 1):
+        if t in d.intp:
+            if ni[t - 1] != d.intp[t].shape[0]:
+                raise DataFileError("Intp rows differ from Numint")
+            rows.append(np.asarray(d.intp[t], np.float64).reshape(-1, 4))
+    ip = np.concatenate(rows) if rows else np.zeros((0, 4))
+    v.nae = ptr(nae, np.int32, C.c_int)
+    v.numint = ptr(ni, np.int32, C.c_int)
+    v.nintp = ip.shape[0]
+    v.intp_ijk = ptr(ip[:, :3], np.int32, C.c_int)
+    v.intp_w = ptr(ip[:, 3], np.float64, C.c_double)
+    v.natoms, v.nelements = N, M
+    v.atom_tag = ptr(d.atom_tag, np.int64, C.c_int64)
+    v.atom_type = ptr(d.atom_type, np.int32, C.c_int)
+    v.x = ptr(d.x, np.float64, C.c_double)
+    if d.atom_image is not None:
+        v.atom_image = ptr(d.atom_image, np.int32, C.c_int)
+    v.elem_tag = ptr(d.elem_tag, np.int64, C.c_int64)
+    v.elem_type = ptr(d.elem_type, np.int32, C.c_int)
+    v.xc = ptr(d.xc, np.float64, C.c_double)
+    if d.elem_image is not None:
+        v.elem_image = ptr(d.elem_image, np.int32, C.c_int)
+    v.nodex = ptr(d.nodex.reshape(M, 8, 3) if M else np.zeros((0, 8, 3)), np.float64, C.c_double)
+    nodev = d.nodev if d.nodev is not None else np.zeros((M, 8, 3))
+    v.nodev = ptr(nodev.reshape(M, 8, 3) if M else np.zeros((0, 8, 3)), np.float64, C.c_double)
+    v.nodetag = ptr(d.nodetag, np.int64, C.c_int64)
+    L = _lib.lib()
+    rc = L.cac_data_write(str(path).encode(), C.byref(v), int(bool(node_velocities)))
+    if rc != _lib.CAC_OK:
+        msg = L.cac_io_last_error().decode()
+        raise (OSError if rc == _lib.CAC_ERR_IO else DataFileError)(msg)
 
 
 def from_system(s: CACSystem, comment: str = "CAC data file") -> DataFile:

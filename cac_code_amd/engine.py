@@ -22,7 +22,7 @@ import numpy as np
 
 from . import _lib
 from .lattice import BOLTZ_METAL, CACSystem, FTM2V_METAL, MVV2E_METAL, NKTV2P_METAL, vertex_clusters
-from .potential import Funcfl, lj_coeffs, read_funcfl
+from .potential import Funcfl, Setfl, lj_coeffs, read_funcfl, read_setfl
 
 SURR_NFEAT, SURR_NOUT, SURR_NQUAD = 279, 24, 561   # CAC_SURR_* in cac_hip.h
 
@@ -92,6 +92,34 @@ class PairEAM:
                                                t["dr"], _p(t["rhor_spline"]), _p(t["z2r_spline"]), cut * cut))
 
 
+class PairEAMAlloy(PairEAM):
+    """pair_style eam/alloy: one LAMMPS setfl file, `pair_coeff * * file el`
+    mapping atom type 1 to element `el` of the file (LAMMPS' PairEAMAlloy).
+    No counterpart in the reference, which reads funcfl only
+    (pair_eam.cpp:197-268); the tables come from cac_eam_tables_from_setfl
+    and run through the same device passes as eam."""
+    style = "eam/alloy"
+
+    def __init__(self, ntypes: int = 1):
+        super().__init__(ntypes)
+        self.setfl: Optional[Setfl] = None
+        self.element = None
+
+    def coeff(self, args, setfl: Optional[Setfl] = None):
+        if len(args) != 3 + self.ntypes or args[0] != "*" or args[1] != "*":
+            raise PairError("Incorrect args for pair coefficients")
+        sf = setfl if setfl is not None else read_setfl(args[2])
+        e = sf.element_index(args[3])
+        self.setfl, self.element = sf, e
+        self.mass = sf.mass[e]
+        self.tables = eam_tables_setfl(sf, e)
+
+    def init_one(self, i: int, j: int) -> float:
+        if self.setfl is None:
+            raise PairError("All pair coeffs are not set")
+        return self.setfl.cut
+
+
 class PairLJCut:
     """pair_style lj/cut cut (pair_lj_cut.cpp:74-198)."""
     style = "lj/cut"
@@ -142,6 +170,24 @@ def eam_tables(ff: Funcfl):
     rc = L.cac_eam_tables_from_funcfl(ff.nrho, ff.drho, ff.nr, ff.dr, _p(fr), _p(rr), _p(zr), C.byref(nrho),
                                       C.byref(drho), C.byref(nr), C.byref(dr), _p(out["frho_spline"]),
                                       _p(out["rhor_spline"]), _p(out["z2r_spline"]))
+    if rc:
+        raise PairError("bad EAM file")
+    return out
+
+
+def eam_tables_setfl(sf: Setfl, element: int = 0):
+    """Spline tables of one element of a setfl file: its arrays are the knot
+    values as they are (cac_eam_tables_from_setfl)."""
+    L = _lib.lib()
+    fr = np.ascontiguousarray(sf.frho[element], dtype=np.float64)
+    rr = np.ascontiguousarray(sf.rhor[element], dtype=np.float64)
+    zr = np.ascontiguousarray(sf.z2r[(element, element)], dtype=np.float64)
+    out = dict(nrho=sf.nrho, nr=sf.nr, drho=sf.drho, dr=sf.dr, cutforcesq=sf.cut * sf.cut)
+    out["frho_spline"] = np.zeros((sf.nrho + 1, 7))
+    out["rhor_spline"] = np.zeros((sf.nr + 1, 7))
+    out["z2r_spline"] = np.zeros((sf.nr + 1, 7))
+    rc = L.cac_eam_tables_from_setfl(sf.nrho, sf.drho, sf.nr, sf.dr, _p(fr), _p(rr), _p(zr),
+                                     _p(out["frho_spline"]), _p(out["rhor_spline"]), _p(out["z2r_spline"]))
     if rc:
         raise PairError("bad EAM file")
     return out
@@ -487,6 +533,22 @@ class Engine(_MlpMixin):
             for i in sel:
                 fh.write("Cu %g %g %g %g\n" % (x[i, 0], x[i, 1], x[i, 2], c[i]))
         return len(sel)
+
+    def write_dump_atom(self, path: str, step: int, append: bool = False, tag=None, nodetag=None) -> int:
+        """One atom-dump snapshot of the current state (DumpAtom,
+        dump_atom.cpp:35-157): atoms, then element nodes as type-1 rows."""
+        from .dump import write_dump_atom
+        x, v, f = self.atoms()
+        nx, nv, nf = self.elements()
+        return write_dump_atom(path, step, self.box[0], self.box[1], x, f, v, nx, nf, nv, tag=tag, nodetag=nodetag,
+                               append=append)
+
+    def write_dump_element(self, path: str, step: int, append: bool = False) -> int:
+        """One element-dump snapshot of the current state (DumpElement,
+        dump_element.cpp:72-263)."""
+        from .dump import write_dump_element
+        nx, nv, nf = self.elements()
+        return write_dump_element(path, step, nx, nv, nf, append=append)
 
     def capacities(self):
         a, b = C.c_int(), C.c_int()

@@ -7,6 +7,13 @@
 * ``read_funcfl`` / ``write_funcfl`` read and write that file
   (pair_eam.cpp:197-268: line 1 comment, line 2 ``Z mass``, line 3
   ``nrho drho nr dr cut``, then nrho + 2*nr values, any number per line).
+* ``read_setfl`` / ``write_setfl`` read and write LAMMPS setfl files
+  (pair_style eam/alloy): 3 comment lines, ``N el1 .. elN``,
+  ``nrho drho nr dr cut``, then per element ``Z mass a lattice``, F(rho)
+  [nrho] and rho(r) [nr], then r*phi(r) [nr] for every pair i >= j (i, j in
+  file order, j fastest), values in any number per line.  The reference
+  reads funcfl only (pair_eam.cpp:197-268); its tables come from
+  ``cac_eam_tables_from_setfl``.
 * ``lj_coeffs`` gives lj1..lj4 exactly as PairLJCut::init_one computes them
   (pair_lj_cut.cpp:140-198, libm ``pow``).
 
@@ -17,7 +24,8 @@ share one set of bits.
 from __future__ import annotations
 
 import math
-from dataclasses import dataclass
+from dataclasses import dataclass, field
+from typing import Dict, List, Tuple
 
 import numpy as np
 
@@ -152,6 +160,109 @@ def read_funcfl(path: str) -> Funcfl:
         raise ValueError(f"{path}: expected {need} table values, found {vals.size}")
     return Funcfl(comment, z, mass, nrho, drho, nr, dr, cut,
                   vals[:nrho].copy(), vals[nrho:nrho + nr].copy(), vals[nrho + nr:need].copy())
+
+
+@dataclass
+class Setfl:
+    """Contents of one LAMMPS setfl file."""
+    comments: List[str]
+    elements: List[str]
+    nrho: int
+    drho: float
+    nr: int
+    dr: float
+    cut: float
+    number: List[int]
+    mass: List[float]
+    lattice_constant: List[float]
+    lattice: List[str]
+    frho: List[np.ndarray]                       # per element, [nrho]
+    rhor: List[np.ndarray]                       # per element, [nr]
+    z2r: Dict[Tuple[int, int], np.ndarray] = field(default_factory=dict)  # (i, j), i >= j: r*phi [nr]
+
+    def element_index(self, name: str) -> int:
+        if name not in self.elements:
+            raise ValueError(f"No matching element {name} in EAM potential file")
+        return self.elements.index(name)
+
+
+def read_setfl(path: str) -> Setfl:
+    """A LAMMPS setfl file (values parsed as atof parses them)."""
+    with open(path) as fh:
+        lines = fh.read().split("\n")
+    if len(lines) < 5:
+        raise ValueError(f"{path}: not a setfl file")
+    comments = lines[:3]
+    w = lines[3].split()
+    nel = int(w[0])
+    if nel < 1 or len(w) < 1 + nel:
+        raise ValueError(f"{path}: incorrect element names in EAM potential file")
+    elements = w[1:1 + nel]
+    w = lines[4].split()
+    nrho, drho, nr, dr, cut = int(w[0]), float(w[1]), int(w[2]), float(w[3]), float(w[4])
+    pos = 5
+
+    def values(n):
+        nonlocal pos
+        out = []
+        while len(out) < n:
+            if pos >= len(lines):
+                raise ValueError(f"{path}: premature end of EAM potential file")
+            out.extend(lines[pos].split())
+            pos += 1
+        if len(out) != n:
+            raise ValueError(f"{path}: table values do not end at a line end")
+        return np.array(out, dtype=np.float64)
+
+    number, mass, alat, lat, frho, rhor = [], [], [], [], [], []
+    for _ in range(nel):
+        while pos < len(lines) and not lines[pos].split():
+            pos += 1
+        if pos >= len(lines):
+            raise ValueError(f"{path}: premature end of EAM potential file")
+        w = lines[pos].split()
+        pos += 1
+        number.append(int(w[0]))
+        mass.append(float(w[1]))
+        alat.append(float(w[2]) if len(w) > 2 else 0.0)
+        lat.append(w[3] if len(w) > 3 else "")
+        frho.append(values(nrho))
+        rhor.append(values(nr))
+    z2r = {}
+    for i in range(nel):
+        for j in range(i + 1):
+            z2r[(i, j)] = values(nr)
+    return Setfl(comments, elements, nrho, drho, nr, dr, cut, number, mass, alat, lat, frho, rhor, z2r)
+
+
+def write_setfl(path: str, sf: Setfl) -> None:
+    """Write sf (17 significant digits, 5 values per line)."""
+    def block(fh, arr):
+        vals = [format(float(v), ".17g") for v in arr]
+        for s in range(0, len(vals), 5):
+            fh.write(" ".join(vals[s:s + 5]) + "\n")
+
+    with open(path, "w") as fh:
+        for c in (list(sf.comments) + ["", "", ""])[:3]:
+            fh.write(c.replace("\n", " ") + "\n")
+        fh.write(f"{len(sf.elements)} " + " ".join(sf.elements) + "\n")
+        fh.write(f"{sf.nrho} {sf.drho!r} {sf.nr} {sf.dr!r} {sf.cut!r}\n")
+        for e in range(len(sf.elements)):
+            fh.write(f"{sf.number[e]} {sf.mass[e]!r} {sf.lattice_constant[e]!r} {sf.lattice[e]}\n")
+            block(fh, sf.frho[e])
+            block(fh, sf.rhor[e])
+        for i in range(len(sf.elements)):
+            for j in range(i + 1):
+                block(fh, sf.z2r[(i, j)])
+
+
+def setfl_from_funcfl(ff: Funcfl, element: str = "Cu", lattice_constant: float = CU_A0,
+                      lattice: str = "fcc") -> Setfl:
+    """A one-element setfl holding a funcfl's arrays on its own grid (the
+    funcfl's third block is r*phi(r) already, pair_eam.cpp:486-487)."""
+    return Setfl([ff.comment, "converted from funcfl", ""], [element], ff.nrho, ff.drho, ff.nr, ff.dr, ff.cut,
+                 [ff.z], [ff.mass], [lattice_constant], [lattice], [ff.frho.copy()], [ff.rhor.copy()],
+                 {(0, 0): ff.zr.copy()})
 
 
 def lj_coeffs(epsilon: float, sigma: float, cut: float):

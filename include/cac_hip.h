@@ -42,7 +42,8 @@ enum {
   CAC_ERR_HIP = 2,      /* HIP runtime error */
   CAC_ERR_OVERFLOW = 3, /* neighbor capacity exceeded (after regrow attempts) */
   CAC_ERR_BOX = 4,      /* box too small for the cutoffs */
-  CAC_ERR_DATA = 5      /* inconsistent element data */
+  CAC_ERR_DATA = 5,     /* inconsistent element data, or a malformed file */
+  CAC_ERR_IO = 6        /* a file could not be opened, read or written */
 };
 
 enum { CAC_PREC_FP64 = 0, CAC_PREC_MIXED = 1 };
@@ -88,6 +89,16 @@ int cac_eam_tables_from_funcfl(int nrho_f, double drho_f, int nr_f, double dr_f,
                                const double *frho_f, const double *rhor_f, const double *zr_f,
                                int *nrho_out, double *drho_out, int *nr_out, double *dr_out,
                                double *frho_spline, double *rhor_spline, double *z2r_spline);
+
+/* The same tables from a LAMMPS setfl file of one element (eam/alloy).  No
+ * counterpart in the reference, which reads funcfl only
+ * (pair_eam.cpp:197-268).  The setfl arrays F(rho) [nrho], rho(r) [nr] and
+ * r*phi(r) [nr] are already on one grid: they are the knot values as they
+ * are, with the knot coefficients of array2spline (pair_eam.cpp:527-580).
+ * Outputs [(nrho+1)][7] and [(nr+1)][7] as above. */
+int cac_eam_tables_from_setfl(int nrho, double drho, int nr, double dr, const double *frho,
+                              const double *rhor, const double *z2r, double *frho_spline,
+                              double *rhor_spline, double *z2r_spline);
 
 /* pair_style eam + pair_coeff (pair_eam.cpp:124-160, init_style :274-283,
  * init_one :288-304): upload the spline tables (rows 1..n of 7) and

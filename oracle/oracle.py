@@ -134,6 +134,23 @@ def eam_tables(ff):
     return out
 
 
+def eam_tables_setfl(sf, element=0):
+    """Tables of one element of a LAMMPS setfl file (PairEAMAlloy::file2array:
+    the file's arrays are the knot values as they are, no resample), through
+    array2spline (pair_eam.cpp:527-580)."""
+    out = dict(nrho=sf.nrho, nr=sf.nr, drho=sf.drho, dr=sf.dr, rdrho=1.0 / sf.drho, rdr=1.0 / sf.dr,
+               cutforcesq=sf.cut * sf.cut)
+    for name, arr, n, d in (("frho", sf.frho[element], sf.nrho, sf.drho), ("rhor", sf.rhor[element], sf.nr, sf.dr),
+                            ("z2r", sf.z2r[(element, element)], sf.nr, sf.dr)):
+        f = np.zeros(n + 1)
+        f[1:] = arr
+        out[name] = f
+        sp = np.zeros((n + 1, 7))
+        lib().orc_spline(C.c_int(n), C.c_double(d), _p(f), _p(sp))
+        out[name + "_spline"] = sp
+    return out
+
+
 def _eam_struct(t):
     e = EAM()
     e.nrho, e.nr = t["nrho"], t["nr"]
