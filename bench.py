@@ -309,17 +309,15 @@ def _err(test, ref):
     return out
 
 
-def accuracy(s, pair, cutneigh, skin, every, delay, at_steps, model=None):
+def accuracy(s, pair, cutneigh, skin, every, delay, at_steps, models=None):
     """Along an FP64 trajectory: at each step in at_steps, the mixed (and,
-    with a model, the mixed + surrogate) forces against the FP64 forces on
-    the same configuration."""
+    for each named model, the mixed + surrogate) forces against the FP64
+    forces on the same configuration."""
     import numpy as np
     from cac_code_amd import _lib
     from cac_code_amd import surrogate as S
     eng = new_engine(s, pair, cutneigh, skin, every, delay)
-    if model is not None:
-        S.attach(eng, model, s)
-        eng.surrogate_enable(False)
+    models = models or {}
     out = []
     done = 0
     for step in at_steps:
@@ -338,8 +336,8 @@ def accuracy(s, pair, cutneigh, skin, every, delay, at_steps, model=None):
         _, _, fnm = eng.elements()
         rec = {"step": step, "mixed": {"atoms": _err(fam, fa), "nodes": _err(fnm, fn),
                                        "energy_rel": abs(emx - e64) / abs(e64), "energy_fp64_eV": e64}}
-        if model is not None:
-            eng.surrogate_enable(True)
+        for name, model in models.items():
+            S.attach(eng, model, s)
             eng.force(precision=_lib.PREC_MIXED)
             _, _, fas = eng.atoms()
             _, _, fns = eng.elements()
@@ -347,9 +345,9 @@ def accuracy(s, pair, cutneigh, skin, every, delay, at_steps, model=None):
             info = eng.surrogate_info()
             eng.surrogate_enable(False)
             sm = cls == 0
-            rec["surrogate"] = {"atoms": _err(fas, fa), "nodes": _err(fns, fn),
-                                "surrogate_nodes": _err(fns[sm], fn[sm]) if sm.any() else None,
-                                "fallback_fraction": float(1.0 - sm.mean()), **info}
+            rec[name] = {"atoms": _err(fas, fa), "nodes": _err(fns, fn),
+                         "surrogate_nodes": _err(fns[sm], fn[sm]) if sm.any() else None,
+                         "fallback_fraction": float(1.0 - sm.mean()), **info}
         eng.force(precision=_lib.PREC_FP64)      # the trajectory continues on FP64 forces
         out.append(rec)
         log(f"[bench] accuracy at step {step}: {json.dumps(rec)}")
@@ -456,7 +454,7 @@ def main():
     prec = _lib.PREC_FP64 if args.precision == "fp64" else _lib.PREC_MIXED
     if args.surrogate_only:
         n_int = s.n_atoms + 8 * s.n_elements
-        _, rec = surrogate_variant(s, pair, cutneigh, skin, args, n_int, float("nan"))
+        _, rec = surrogate_variant(s, pair, cutneigh, skin, args, n_int, float("nan"), surrogate_data(s, pair, args))
         print(json.dumps({"surrogate": rec}), flush=True)
         return
     eng, secs, nbuilds, nbw, prof = gpu_run(s, pair, cutneigh, skin, args.steps, args.warmup, prec, args.every,
@@ -501,11 +499,15 @@ def main():
                              "ms_per_step": round(secs2 * 1e3 / args.steps, 4), "rebuilds": nb2,
                              "kernels_us": {k: round(v[0] * 1e3, 1) for k, v in prof2.items()}}
         log(f"[bench] mixed: {secs2 * 1e3 / args.steps:.3f} ms/step")
-        model = None
+        models = {}
         if args.workload == "eam_c1" and not args.no_surrogate:
-            model, variants["surrogate"] = surrogate_variant(s, pair, cutneigh, skin, args, n_int, secs)
+            data = surrogate_data(s, pair, args)
+            models["surrogate"], variants["surrogate"] = surrogate_variant(s, pair, cutneigh, skin, args, n_int,
+                                                                           secs, data)
+            models["surrogate_config"], variants["surrogate_config"] = surrogate_variant(
+                s, pair, cutneigh, skin, args, n_int, secs, data, config=True)
         steps = [int(x) for x in args.error_steps.split(",") if x.strip()]
-        acc["forces"] = accuracy(s, pair, cutneigh, skin, args.every, args.delay, steps, model)
+        acc["forces"] = accuracy(s, pair, cutneigh, skin, args.every, args.delay, steps, models)
         if args.drift_steps:
             acc["drift_fp64"] = drift(s, pair, cutneigh, skin, args.every, args.delay, args.drift_steps,
                                       _lib.PREC_FP64)
@@ -576,11 +578,9 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def surrogate_variant(s, pair, cutneigh, skin, args, n_int, secs_fp64):
-    """Train the element surrogate (dataset seed 1234, init seed 0, Adam
-    lr 1e-3) and time mixed precision + surrogate on the same workload."""
-    import numpy as np
-    from cac_code_amd import _lib
+def surrogate_data(s, pair, args):
+    """Training set (dataset seed 1234) and held-out set (seed 77) of the
+    element surrogate, from the plain integration-point path."""
     from cac_code_amd import surrogate as S
     t0 = time.perf_counter()
     if args.surrogate_data == "trajectory":
@@ -591,12 +591,30 @@ def surrogate_variant(s, pair, cutneigh, skin, args, n_int, secs_fp64):
     else:
         ds = S.generate_dataset(pair, args.surrogate_samples, seed=1234, log=log)
         held = S.generate_dataset(pair, 20000, seed=77)
-    model = S.train(ds, epochs=args.surrogate_epochs, batch=1024, lr=1e-3, seed=0, log=log)
+    return ds, held, time.perf_counter() - t0
+
+
+def surrogate_variant(s, pair, cutneigh, skin, args, n_int, secs_fp64, data, config=False):
+    """Train the element surrogate (init seed 0, Adam lr 1e-3) on the
+    dataset, the structured 279-input one or (config) the configured
+    102-input network, and time mixed precision + surrogate on the same
+    workload."""
+    import numpy as np
+    from cac_code_amd import _lib
+    from cac_code_amd import surrogate as S
+    ds, held, t_data = data
+    t0 = time.perf_counter()
+    if config:
+        model = S.train_config(ds, epochs=args.surrogate_epochs, batch=1024, lr=1e-3, seed=0, log=log)
+        held_fit = None
+    else:
+        model = S.train(ds, epochs=args.surrogate_epochs, batch=1024, lr=1e-3, seed=0, log=log)
+        held_fit = S.rel_rms(S.predict(model, held.x, structured_only=True), held.t)
     held_err = S.rel_rms(S.predict(model, held.x), held.t)
-    held_fit = S.rel_rms(S.predict(model, held.x, structured_only=True), held.t)
-    t_train = time.perf_counter() - t0
-    log(f"[bench] surrogate trained in {t_train:.0f}s: held-out rel RMS {held_err:.5f} "
-        f"(least-squares part alone {held_fit:.5f})")
+    t_train = time.perf_counter() - t0 + t_data
+    name = "configured 102-input network" if config else "structured 279-input surrogate"
+    log(f"[bench] {name} trained in {t_train:.0f}s: held-out rel RMS {held_err:.5f}"
+        + (f" (least-squares part alone {held_fit:.5f})" if held_fit is not None else ""))
     eng = new_engine(s, pair, cutneigh, skin, args.every, args.delay)
     info = S.attach(eng, model, s)
     eng.force(precision=_lib.PREC_MIXED)
@@ -612,15 +630,15 @@ def surrogate_variant(s, pair, cutneigh, skin, args, n_int, secs_fp64):
     ne = s.n_elements
     f_fb = 1.0 - info["surrogate"] / ne
     b_step = B_ATOM * s.n_atoms + ne * ((1.0 - f_fb) * 2568 + f_fb * B_ELEM)
-    mlp_flop = 2.0 * (S.NIN * S.NHID + (S.NLAYERS - 1) * S.NHID * S.NHID + S.NHID * S.NOUT) * info["surrogate"]
+    mlp_flop = 2.0 * (model.nin * S.NHID + (S.NLAYERS - 1) * S.NHID * S.NHID + S.NHID * S.NOUT) * info["surrogate"]
     mlp_roof = None
     if "surr_mlp" in prof:
         tf = mlp_flop / (prof["surr_mlp"] * 1e-6) / 1e12
         mlp_roof = {"bound": "mfma", "kernel": "surr_mlp (3 hidden + output GEMMs, BF16 MFMA)",
                     "achieved": round(tf, 1), "peak": 2500.0, "unit": "TFLOP/s", "frac": round(tf / 2500.0, 4),
                     "flop_per_launch": mlp_flop}
-    log(f"[bench] mixed+surrogate: {secs * 1e3 / args.steps:.3f} ms/step, fallback fraction {f_fb:.3f}, "
-        f"{prof}")
+    log(f"[bench] mixed+{'configured network' if config else 'surrogate'}: {secs * 1e3 / args.steps:.3f} ms/step, "
+        f"fallback fraction {f_fb:.3f}, {prof}")
     return model, {"value": round(n_int * args.steps / secs / 1e6, 2),
                    "ms_per_step": round(secs * 1e3 / args.steps, 4),
                    "speedup_vs_fp64": round(secs_fp64 / secs, 3), "rebuilds": nb,
@@ -630,7 +648,10 @@ def surrogate_variant(s, pair, cutneigh, skin, args, n_int, secs_fp64):
                                 "data": args.surrogate_data, "dataset": {k: v for k, v in ds.meta.items()
                                                                          if not isinstance(v, np.ndarray)},
                                 "dataset_seed": 1234, "init_seed": 0, "lr": 1e-3, "seconds": round(t_train, 1),
-                                "heldout_rel_rms": round(held_err, 6), "heldout_least_squares_only": round(held_fit, 6)}}
+                                "heldout_rel_rms": round(held_err, 6),
+                                "heldout_least_squares_only": None if held_fit is None else round(held_fit, 6)},
+                   "network": (f"MLP {model.nin}->256->256->256->24 SiLU BF16"
+                               + ("" if config else " + FP32 linear and corner-local quadratic least-squares part"))}
 
 
 if __name__ == "__main__":

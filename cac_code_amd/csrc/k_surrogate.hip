@@ -49,6 +49,8 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kNFeat = CAC_SURR_NFEAT, kNOut = CAC_SURR_NOUT, kNLoc = 33, kNQuad = CAC_SURR_NQUAD;
 constexpr int kGroups = kNFeat / 3;
+constexpr int kNFeatCfg = CAC_SURR_NFEAT_CONFIG, kGroupsCfg = kNFeatCfg / 3;
+static_assert(kGroupsCfg == 34, "configured inputs: 8 own nodes and 26 neighbor centroids");
 static_assert(kNQuad == kNLoc * (kNLoc + 1) / 2, "quadratic term count");
 constexpr int kPredElems = 64;  // elements per prediction block (one per lane)
 
@@ -173,6 +175,10 @@ __device__ __forceinline__ void feature_group(int e, int g, const FeatArgs &a, c
 // instead of chains of dependent gathers.  Same values as feature_group.
 constexpr int kFeatWaves = kBlock / 64;
 
+// NG = kGroups: all 279 features, BF16 into the MLP input and FP32 into xf;
+// NG = kGroupsCfg: the first 102 (own nodes and neighbor centroids), BF16
+// only (the configured network, which has no least-squares part).
+template <int NG>
 __global__ __launch_bounds__(kBlock) void k_surr_features(int ns, const int *__restrict__ meta, FeatArgs a,
                                                           const float *__restrict__ inv_in, uint16_t *__restrict__ x,
                                                           int ldx, float *__restrict__ xf) {
@@ -229,13 +235,13 @@ __global__ __launch_bounds__(kBlock) void k_surr_features(int ns, const int *__r
         F[3 * i + j] = M[3 * i] * G[j] + M[3 * i + 1] * G[3 + j] + M[3 * i + 2] * G[6 + j];
   }
   uint16_t *row = x + (size_t)r * ldx;
-  float *rowf = xf + (size_t)r * kNFeat;
+  float *rowf = NG == kGroups ? xf + (size_t)r * kNFeat : nullptr;
   auto emit = [&](int c, float v) {
     v *= inv_in[c];
     row[c] = to_bf16(v);
-    rowf[c] = v;
+    if (NG == kGroups) rowf[c] = v;
   };
-  for (int g = lane; g < kGroups; g += 64) {
+  for (int g = lane; g < NG; g += 64) {
     double v[3];
     const double *rr = nullptr;
     bool have = true;
@@ -354,6 +360,16 @@ __global__ void __launch_bounds__(512) k_surr_predict(int ns, const int *__restr
   out[0] = ((double)a0 + (double)yr[0]) * (double)out_scale[3 * k];
   out[1] = ((double)a1 + (double)yr[1]) * (double)out_scale[3 * k + 1];
   out[2] = ((double)a2 + (double)yr[2]) * (double)out_scale[3 * k + 2];
+}
+
+// nodal forces of the surrogate elements from the configured network
+// alone: force c of row r = MLP(x)_c * out_scale[c]
+__global__ void k_surr_output(int ns, const int *__restrict__ elemS, const float *__restrict__ y, int ldy,
+                              const float *__restrict__ out_scale, double *__restrict__ nodef) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= ns * kNOut) return;
+  const int r = q / kNOut, c = q - kNOut * r;
+  nodef[kNOut * (size_t)elemS[r] + c] = (double)y[(size_t)r * ldy + c] * (double)out_scale[c];
 }
 
 // fallback class: 1 interface (a mesh neighbor is not an element, or, with
@@ -579,17 +595,27 @@ int surrogate_forces(cac_ctx *ctx) {
   int ldx = 0;
   uint16_t *x = mlp_input_buffer(ctx, S.nS, &ldx);
   if (!x) return fail(ctx, CAC_ERR_HIP, "surrogate input buffer");
-  float *xf;
-  if ((rc = ensure(ctx, S.b_xf, (size_t)S.nS * kNFeat, &xf))) return rc;
-  hipLaunchKernelGGL(k_surr_features, dim3(grid_for(S.nS, kFeatWaves)), dim3(kBlock), 0, ctx->stream, S.nS,
-                     S.meta, fa, S.inv_in, x, ldx, xf);
+  const bool full = mlp_shape_ok(ctx, kNFeat, kNOut);  // else the configured 102-input network
+  float *xf = nullptr;
+  if (full) {
+    if ((rc = ensure(ctx, S.b_xf, (size_t)S.nS * kNFeat, &xf))) return rc;
+    hipLaunchKernelGGL(k_surr_features<kGroups>, dim3(grid_for(S.nS, kFeatWaves)), dim3(kBlock), 0, ctx->stream,
+                       S.nS, S.meta, fa, S.inv_in, x, ldx, xf);
+  } else {
+    hipLaunchKernelGGL(k_surr_features<kGroupsCfg>, dim3(grid_for(S.nS, kFeatWaves)), dim3(kBlock), 0,
+                       ctx->stream, S.nS, S.meta, fa, S.inv_in, x, ldx, xf);
+  }
   timer_end(ctx, "surr_feat", t0);
   timer_begin(ctx, "surr_mlp", &t0);
   const float *y = mlp_forward_staged(ctx, S.nS);
   timer_end(ctx, "surr_mlp", t0);
   timer_begin(ctx, "surr_fit", &t0);
-  hipLaunchKernelGGL(k_surr_predict, dim3(grid_for(S.nS, kPredElems)), dim3(8 * 64), 0, ctx->stream, S.nS,
-                     S.elemS, xf, y, mlp_out_ld(ctx), S.fit_A, S.fit_Q, S.fit_b, S.out_scale, ctx->nodef);
+  if (full)
+    hipLaunchKernelGGL(k_surr_predict, dim3(grid_for(S.nS, kPredElems)), dim3(8 * 64), 0, ctx->stream, S.nS,
+                       S.elemS, xf, y, mlp_out_ld(ctx), S.fit_A, S.fit_Q, S.fit_b, S.out_scale, ctx->nodef);
+  else
+    hipLaunchKernelGGL(k_surr_output, dim3(grid_for((long)S.nS * kNOut)), dim3(kBlock), 0, ctx->stream, S.nS,
+                       S.elemS, y, mlp_out_ld(ctx), S.out_scale, ctx->nodef);
   timer_end(ctx, "surr_fit", t0);
   CAC_HIP(ctx, hipGetLastError());
   return CAC_OK;
@@ -733,7 +759,8 @@ int cac_surrogate_enable(cac_ctx *ctx, int on) {
   if (on) {
     if (!S.setup) return fail(ctx, CAC_ERR_ARG, "cac_surrogate_setup not called");
     if (!ctx->mlp) return fail(ctx, CAC_ERR_ARG, "cac_mlp_create not called");
-    if (!mlp_shape_ok(ctx, kNFeat, kNOut)) return fail(ctx, CAC_ERR_ARG, "the MLP must map 279 features to 24 forces");
+    if (!mlp_shape_ok(ctx, kNFeat, kNOut) && !mlp_shape_ok(ctx, kNFeatCfg, kNOut))
+      return fail(ctx, CAC_ERR_ARG, "the MLP must map 279 (or the configured 102) features to 24 forces");
     if (ctx->pair != CAC_PAIR_EAM) return fail(ctx, CAC_ERR_ARG, "the surrogate is trained for pair_style eam");
   }
   S.enabled = on != 0;

@@ -25,6 +25,7 @@ from .lattice import BOLTZ_METAL, CACSystem, FTM2V_METAL, MVV2E_METAL, NKTV2P_ME
 from .potential import Funcfl, Setfl, lj_coeffs, read_funcfl, read_setfl
 
 SURR_NFEAT, SURR_NOUT, SURR_NQUAD = 279, 24, 561   # CAC_SURR_* in cac_hip.h
+SURR_NFEAT_CONFIG = 102                            # CAC_SURR_NFEAT_CONFIG
 
 
 def _p(a, t=_lib._d):

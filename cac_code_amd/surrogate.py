@@ -22,6 +22,13 @@ the corner nodes a linear map leaves ~8%, nearly all of it the anharmonic
 (quadratic) response, which the corner-local quadratic terms capture down
 to ~0.8%; the MLP takes the remainder to ~0.2% held-out.
 
+The configured network (``train_config``): the first 102 inputs only (own
+nodes and neighbor centroids), MLP 102 -> 256 -> 256 -> 256 -> 24 (SiLU,
+BF16, FP32 accumulation) trained with the same Adam on the forces
+themselves, no least-squares part.  It is deployed by the same device path
+(an MLP of 102 inputs switches the feature and output kernels) and reported
+beside the structured one.
+
 Training data: periodic all-element fcc boxes of the same element type under
 random normal strains up to `strain_max` (an orthogonal periodic box cannot
 carry shear), periodic long-wave displacement modes (local strain variation,
@@ -38,10 +45,11 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .engine import SURR_NFEAT, SURR_NOUT, SURR_NQUAD, Engine, PairEAM
+from .engine import SURR_NFEAT, SURR_NFEAT_CONFIG, SURR_NOUT, SURR_NQUAD, Engine, PairEAM
 from .lattice import CACSystem, build_cac_system
 
 NIN, NHID, NLAYERS, NOUT = SURR_NFEAT, 256, 3, SURR_NOUT
+NIN_CONFIG = SURR_NFEAT_CONFIG
 NLOC = 33
 _IU = np.triu_indices(NLOC)
 assert len(_IU[0]) == SURR_NQUAD
@@ -255,12 +263,13 @@ def generate_dataset_trajectory(pair: PairEAM, system_fn, n_samples: int, seed: 
 @dataclass
 class Model:
     weights: list           # MLP layers [(W [out][in], b [out])], FP32 masters
-    in_scale: np.ndarray    # [279]
+    in_scale: np.ndarray    # [nin]
     out_scale: np.ndarray   # [24]
-    fit_A: np.ndarray       # [24][279] least-squares linear part, normalized units
+    fit_A: np.ndarray       # [24][279] least-squares linear part, normalized units (None: MLP only)
     fit_Q: np.ndarray       # [24][561] least-squares corner-local quadratic part
     fit_b: np.ndarray       # [24]
     history: list = field(default_factory=list)
+    nin: int = NIN          # 279, or 102 for the configured network
 
 
 def fit_structured(xn: np.ndarray, tn: np.ndarray, ridge: float = 1e-10):
@@ -347,14 +356,53 @@ def train(ds: Dataset, epochs: int = 60, batch: int = 1024, lr: float = 1e-3, se
     return Model(w, in_scale, out_scale, A, Q, b, hist)
 
 
+def train_config(ds: Dataset, epochs: int = 60, batch: int = 1024, lr: float = 1e-3, seed: int = 0, device: int = 0,
+                 log=None, lr_decay_to: float = 0.05) -> Model:
+    """The configured network: MLP 102 -> 256 -> 256 -> 256 -> 24 on the
+    first 102 features, trained with the device Adam on the normalized forces
+    (same schedule as train); the output bias is then shifted so a perfect
+    lattice (zero deviation) gets zero force."""
+    x = np.ascontiguousarray(ds.x[:, :NIN_CONFIG])
+    in_scale = np.sqrt(np.mean(x.astype(np.float64) ** 2, axis=0)).astype(np.float32)
+    in_scale = np.maximum(in_scale, 1e-6 * max(float(in_scale.max()), 1e-12)).astype(np.float32)
+    out_scale = np.sqrt(np.mean(ds.t.astype(np.float64) ** 2, axis=0)).astype(np.float32)
+    out_scale = np.maximum(out_scale, 1e-6 * max(float(out_scale.max()), 1e-12)).astype(np.float32)
+    xn = (x * (1.0 / in_scale)).astype(np.float32)
+    tn = (ds.t / out_scale).astype(np.float32)
+    hist = []
+    t0 = time.perf_counter()
+    with Engine(device) as eng:
+        eng.mlp_create(NIN_CONFIG, NHID, NLAYERS, NOUT, seed)
+        eng.mlp_set_dataset(xn, tn)
+        chunk = max(1, epochs // 10)
+        done = 0
+        while done < epochs:
+            e = min(chunk, epochs - done)
+            rate = lr * (lr_decay_to ** (done / max(epochs - 1, 1)))
+            loss = eng.mlp_train(e, batch, rate, seed + done)
+            done += e
+            hist.append((done, rate, loss))
+            if log:
+                log(f"[surrogate/config] epoch {done}/{epochs} lr {rate:.2e} loss {loss:.3e}")
+        w = eng.mlp_get_weights()
+        y0 = eng.mlp_forward(np.zeros((1, NIN_CONFIG), dtype=np.float32))[0]
+        W, bl = w[-1]
+        w[-1] = (W, (bl - y0).astype(np.float32))
+    if log:
+        log(f"[surrogate/config] MLP trained in {time.perf_counter() - t0:.1f}s")
+    return Model(w, in_scale, out_scale, None, None, None, hist, nin=NIN_CONFIG)
+
+
 def predict(model: Model, x: np.ndarray, device: int = 0, structured_only: bool = False) -> np.ndarray:
     """Nodal forces [n][24] for raw features x [n][279]: the least-squares
-    part on the host (FP64), the MLP on the device."""
-    xn = (x * (1.0 / model.in_scale)).astype(np.float32)
-    y = structured_predict(model.fit_A, model.fit_Q, model.fit_b, xn)
+    part on the host (FP64), the MLP on the device (the configured network:
+    the MLP of the first 102 features alone)."""
+    xn = (x[:, :model.nin] * (1.0 / model.in_scale)).astype(np.float32)
+    y = (structured_predict(model.fit_A, model.fit_Q, model.fit_b, xn) if model.fit_A is not None
+         else np.zeros((len(x), NOUT)))
     if not structured_only:
         with Engine(device) as eng:
-            eng.mlp_create(NIN, NHID, NLAYERS, NOUT, 0)
+            eng.mlp_create(model.nin, NHID, NLAYERS, NOUT, 0)
             eng.mlp_set_weights(model.weights)
             y = y + eng.mlp_forward(xn)
     return y * model.out_scale
@@ -369,10 +417,13 @@ def rel_rms(pred, ref):
 def attach(eng: Engine, model: Model, s: CACSystem, strain_limit: float = 0.02):
     """Load the model into eng's context and switch its element forces to
     the surrogate (fallback elements keep the plain path)."""
-    eng.mlp_create(NIN, NHID, NLAYERS, NOUT, 0)
+    eng.mlp_create(model.nin, NHID, NLAYERS, NOUT, 0)
     eng.mlp_set_weights(model.weights)
-    eng.surrogate_setup(s.nodex if getattr(s, "X0", None) is None else s.X0, mesh_neighbors(s), model.in_scale,
+    in_scale = np.ones(NIN, dtype=np.float32)             # the device scales all 279 features
+    in_scale[:model.nin] = model.in_scale
+    eng.surrogate_setup(s.nodex if getattr(s, "X0", None) is None else s.X0, mesh_neighbors(s), in_scale,
                         model.out_scale, strain_limit)
-    eng.surrogate_set_fit(model.fit_A, model.fit_Q, model.fit_b)
+    if model.fit_A is not None:
+        eng.surrogate_set_fit(model.fit_A, model.fit_Q, model.fit_b)
     eng.surrogate_enable(True)
     return eng.surrogate_info()

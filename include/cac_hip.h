@@ -317,8 +317,12 @@ int cac_mlp_train(cac_ctx *ctx, int epochs, int batch, float lr, uint64_t seed, 
  * strain above strain_limit), which keep the plain integration-point path;
  * classification is refreshed at each rebuild.  cac_surrogate_features
  * writes the raw features [ne][CAC_SURR_NFEAT] and the class per element
- * (0 surrogate, 1 interface, 2 strained). */
+ * (0 surrogate, 1 interface, 2 strained).
+ * With an MLP of CAC_SURR_NFEAT_CONFIG inputs instead (the configured
+ * network: the first 102 features, own nodes and neighbor centroids), the
+ * prediction is MLP(x)[c] alone and cac_surrogate_set_fit is not used. */
 #define CAC_SURR_NFEAT 279
+#define CAC_SURR_NFEAT_CONFIG 102
 #define CAC_SURR_NOUT 24
 #define CAC_SURR_NQUAD 561
 int cac_surrogate_setup(cac_ctx *ctx, const double *X0, const int *nbr, const float *in_scale,

@@ -207,3 +207,45 @@ def test_fallback_elements_keep_plain_forces(pair, small_model):
     assert np.max(np.abs(fsurr - fplain)) <= 1e-12 * max(np.max(np.abs(fplain)), 1e-30)
     pred = S.predict(model, f[~fb])
     assert np.allclose(surr[~fb].reshape(-1, 24), pred, rtol=2e-2, atol=2e-3 * np.abs(pred).max())
+
+
+@pytest.fixture(scope="module")
+def config_model(small_model):
+    ds, _ = small_model
+    return S.train_config(ds, epochs=40, batch=512, lr=1e-3, log=print)
+
+
+def test_configured_network(pair, small_model, config_model):
+    """The configured 102-input network (own nodes and neighbor centroids,
+    MLP only) runs on the same device path: fallback elements keep the
+    plain forces, surrogate elements get the network's forces (BF16
+    inference on the device as in predict).  On held-out thermal data it
+    misses by far more than the structured surrogate: a node's force
+    depends on the nodes of the elements meeting at it, which the centroids
+    do not carry."""
+    _, model = small_model
+    held = S.generate_dataset(pair, 4000, L=33, seed=99)
+    err = S.rel_rms(S.predict(model, held.x), held.t)
+    err_cfg = S.rel_rms(S.predict(config_model, held.x), held.t)
+    print(f"[surrogate] held-out rel RMS: configured 102-input network {err_cfg:.5f}, structured {err:.5f}")
+    assert err < 0.5 * err_cfg
+    s = build_cac_system(33, 11, 8.0, temperature=300.0, seed=3)
+    X0 = s.nodex.copy()
+    s = jitter(s, 0.04, 4)
+    with Engine(0) as eng:
+        eng.load_system(s)
+        eng.set_pair(pair)
+        eng.set_neighbor(5.95, skin=1.0)
+        eng.build()
+        eng.force()
+        _, _, plain = eng.elements()
+        s.X0 = X0
+        info = S.attach(eng, config_model, s)
+        eng.force()
+        _, _, surr = eng.elements()
+        f, cls = eng.surrogate_features()
+    fb = cls != 0
+    assert fb.any() and (~fb).any() and (~fb).sum() == info["surrogate"]
+    assert np.max(np.abs(surr[fb] - plain[fb])) <= 1e-12 * np.max(np.abs(plain[fb]))
+    pred = S.predict(config_model, f[~fb])
+    assert np.allclose(surr[~fb].reshape(-1, 24), pred, rtol=2e-2, atol=2e-3 * np.abs(pred).max())
