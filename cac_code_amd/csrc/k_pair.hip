@@ -933,11 +933,32 @@ __device__ __forceinline__ uint32_t k0_mask(const uint32_t *__restrict__ mrow, i
   return l < jn ? mrow[0] : 0u;
 }
 
+// Site position x = sum_k N_k X_k and (FP) density rho = sum_k N_k E_k, in
+// node order, k_site_pos's and k_site_fp's operations.
+template <bool FP, class PX>
+__device__ __forceinline__ void site_interp(PX X, PX E, const double N[8], double *x, double *y, double *z,
+                                            double *rho) {
+  double xx = 0.0, yy = 0.0, zz = 0.0;
+  for (int k = 0; k < 8; ++k) {
+    xx += N[k] * X[3 * k];
+    yy += N[k] * X[3 * k + 1];
+    zz += N[k] * X[3 * k + 2];
+  }
+  *x = xx;
+  *y = yy;
+  *z = zz;
+  if (FP) {
+    double r = 0.0;
+    for (int k = 0; k < 8; ++k) r += N[k] * E[k];
+    *rho = r;
+  }
+}
+
 // LDS of a cluster wavefront: rows [c.rows][4] (x, y, z, fp of every union
 // entry), then the nodes [8][24] and node densities [8][8] of the cluster's
 // own elements (the elements whose nodes meet at the vertex: nearly every
 // union site lies in one of them).
-template <class P, bool FP>
+template <class P, bool FP, bool LDS>
 __device__ __forceinline__ void cl_fill(const ClArgs &c, int cl, int s0, int ns, const ElemArgs &a, const Prd &prd,
                                         const Tabs<P> &t, const double *__restrict__ x4,
                                         const double *__restrict__ erho, double *rows, int cap, int lane) {
@@ -970,21 +991,20 @@ __device__ __forceinline__ void cl_fill(const ClArgs &c, int cl, int s0, int ns,
 #pragma unroll
       for (int i = 0; i < kClMax; ++i)
         if (el[i] == js.x) li = i;
-      const double *X = li >= 0 ? nodes + 24 * li : a.nodex + 24 * (size_t)js.x;
-      x = 0.0;
-      y = 0.0;
-      z = 0.0;
-      for (int k = 0; k < 8; ++k) {
-        x += N[k] * X[3 * k];
-        y += N[k] * X[3 * k + 1];
-        z += N[k] * X[3 * k + 2];
-      }
-      if (FP) {
-        const double *E = li >= 0 ? enode + 8 * li : erho + 8 * (size_t)js.x;
-        double rho = 0.0;
-        for (int k = 0; k < 8; ++k) rho += N[k] * E[k];
-        w = (double)embed_fp<P>(t, rho);
-      }
+      // one branch per memory space (the cluster's own elements from LDS,
+      // any other from global memory): a pointer selected between the two
+      // would be read with flat loads, which take the texture-address path
+      // even for LDS data
+      double rho = 0.0;
+      using gptr = const __attribute__((address_space(1))) double *;
+      if (li < 0)
+        site_interp<FP>((gptr)(a.nodex + 24 * (size_t)js.x), (gptr)(FP ? erho + 8 * (size_t)js.x : nullptr), N, &x, &y,
+                        &z, &rho);
+      else if (LDS)
+        site_interp<FP>((lds_cptr<double>)(nodes + 24 * li), (lds_cptr<double>)(enode + 8 * li), N, &x, &y, &z, &rho);
+      else
+        site_interp<FP>((gptr)(nodes + 24 * li), (gptr)(enode + 8 * li), N, &x, &y, &z, &rho);
+      if (FP) w = (double)embed_fp<P>(t, rho);
     }
     const int img = (int)(e >> 27);
     if (img != kImgSelf) {
@@ -1004,12 +1024,12 @@ __device__ __forceinline__ void cl_fill(const ClArgs &c, int cl, int s0, int ns,
 
 // rows: the cluster's union rows, in LDS or (a large union) in global
 // memory; cap: their capacity.  Inlined once per memory space.
-template <class P>
+template <class P, bool LDS>
 __device__ __forceinline__ void cl_erho_body(const ClArgs &c, const ElemArgs &a, const Prd &prd, const Tabs<P> &t,
                                              const double *__restrict__ px4, double *__restrict__ erho,
                                              uint32_t *__restrict__ mask, double *rows, int cap, int cl, int lane) {
   const int s0 = c.start[cl], ns = c.start[cl + 1] - s0;
-  cl_fill<P, false>(c, cl, s0, ns, a, prd, t, px4, nullptr, rows, cap, lane);
+  cl_fill<P, false, LDS>(c, cl, s0, ns, a, prd, t, px4, nullptr, rows, cap, lane);
   __syncthreads();
   const int gi = lane >> 3, l = lane & 7;
   double acc = 0.0;
@@ -1084,13 +1104,13 @@ __global__ __launch_bounds__(64) void k_cl_erho(ClArgs c, ElemArgs a, Prd prd, T
   const int cl = c.order[o];
   const int big = c.big[cl];
   if (big >= 0)
-    cl_erho_body<P>(c, a, prd, t, px4, erho, mask, c.scratch + (size_t)big * c.sstride, c.srows, cl,
+    cl_erho_body<P, false>(c, a, prd, t, px4, erho, mask, c.scratch + (size_t)big * c.sstride, c.srows, cl,
                     (int)threadIdx.x);
   else
-    cl_erho_body<P>(c, a, prd, t, px4, erho, mask, lds_rows, c.rows, cl, (int)threadIdx.x);
+    cl_erho_body<P, true>(c, a, prd, t, px4, erho, mask, lds_rows, c.rows, cl, (int)threadIdx.x);
 }
 
-template <class P, bool EV>
+template <class P, bool EV, bool LDS>
 __device__ __forceinline__ void cl_force_body(const ClArgs &c, const ElemArgs &a, const Prd &prd, const Tabs<P> &t,
                                               const double *__restrict__ erho, const double *__restrict__ px4,
                                               double *__restrict__ ipf, double *__restrict__ eip,
@@ -1098,7 +1118,7 @@ __device__ __forceinline__ void cl_force_body(const ClArgs &c, const ElemArgs &a
                                               double *__restrict__ nodef, double *rows, int cap, int cl,
                                               int lane) {
   const int s0 = c.start[cl], ns = c.start[cl + 1] - s0;
-  cl_fill<P, true>(c, cl, s0, ns, a, prd, t, px4, erho, rows, cap, lane);
+  cl_fill<P, true, LDS>(c, cl, s0, ns, a, prd, t, px4, erho, rows, cap, lane);
   __syncthreads();
   const int gi = lane >> 3, l = lane & 7;
   double fx = 0.0, fy = 0.0, fz = 0.0, ep = 0.0, rhoi = 0.0;
@@ -1202,10 +1222,10 @@ __global__ __launch_bounds__(64) void k_cl_force(ClArgs c, ElemArgs a, Prd prd, 
   const int cl = c.order[o];
   const int big = c.big[cl];
   if (big >= 0)
-    cl_force_body<P, EV>(c, a, prd, t, erho, px4, ipf, eip, vip, mask, nodef, c.scratch + (size_t)big * c.sstride,
-                         c.srows, cl, (int)threadIdx.x);
+    cl_force_body<P, EV, false>(c, a, prd, t, erho, px4, ipf, eip, vip, mask, nodef,
+                                c.scratch + (size_t)big * c.sstride, c.srows, cl, (int)threadIdx.x);
   else
-    cl_force_body<P, EV>(c, a, prd, t, erho, px4, ipf, eip, vip, mask, nodef, lds_rows, c.rows, cl,
+    cl_force_body<P, EV, true>(c, a, prd, t, erho, px4, ipf, eip, vip, mask, nodef, lds_rows, c.rows, cl,
                          (int)threadIdx.x);
 }
 
