@@ -1235,7 +1235,8 @@ int download_rows(cac_ctx *ctx, const uint32_t *dev, const int *dnum, size_t row
   const uint32_t n = (uint32_t)ctx->n;
   for (size_t p = 0; p < rows; ++p)
     for (int k = 0; k < numneigh[p] && k < M; ++k) {
-      const uint32_t e = cm[((p / group) * M + k) * group + (p % group)];
+      const uint32_t e = group == kAtomGroup ? cm[atom_slot((int)p, k, M)]
+                                             : cm[((p / group) * M + k) * group + (p % group)];
       const uint32_t idx = e & 0x7FFFFFFu;
       const uint64_t img = (uint64_t)(e >> 27) << 27;
       if (idx < n) {

@@ -29,15 +29,22 @@ constexpr uint32_t kAtomSite = 0xFFFFFFFFu;
 // 8 lanes each reads 64 consecutive entries per step.
 constexpr int kLanes = 8;
 constexpr int kRowGroup = 8;
-// Atom lists use groups of 64 rows: entry k of atom i lives at
-// ((i/64)*M + k)*64 + i%64, so the 64 threads of a wavefront (one atom each)
-// read 64 consecutive entries per step.
+// Atom lists use groups of 64 rows and runs of 4 entries: entry k of atom i
+// lives at ((i/64)*M + (k & ~3))*64 + (i%64)*4 + (k & 3), so the 64 threads
+// of a wavefront (one atom each) read 4 entries each with one 16-byte load,
+// 1 KB contiguous per step (M is a multiple of 8).
 constexpr int kAtomGroup = 64;
+constexpr int kAtomRun = 4;
 // Vertex clusters of integration points: at most 8 node slots per cluster.
 constexpr int kClMax = 8;
 constexpr int kUSmall = 256;
 __host__ __device__ __forceinline__ size_t atom_slot(int i, int k, int M) {
-  return ((size_t)(i >> 6) * M + k) * kAtomGroup + (i & 63);
+  return ((size_t)(i >> 6) * M + (k & ~(kAtomRun - 1))) * kAtomGroup + (size_t)(i & 63) * kAtomRun +
+         (k & (kAtomRun - 1));
+}
+// offset of entry k from atom_slot(i, 0, M)
+__host__ __device__ __forceinline__ size_t atom_off(int k) {
+  return (size_t)(k & ~(kAtomRun - 1)) * kAtomGroup + (k & (kAtomRun - 1));
 }
 
 // Workgroups are handed to the 8 XCDs of an MI355X round-robin.  Kernels

@@ -88,7 +88,7 @@ __global__ __launch_bounds__(kCentroBlock) void k_centro(int n, const double *__
     uint32_t eb[kCU];
     double xb[kCU], yb[kCU], zb[kCU];
 #pragma unroll
-    for (int u = 0; u < kCU; ++u) eb[u] = k0 + u < jn ? row[(size_t)(k0 + u) * kAtomGroup] : 0xFFFFFFFFu;
+    for (int u = 0; u < kCU; ++u) eb[u] = k0 + u < jn ? row[atom_off(k0 + u)] : 0xFFFFFFFFu;
 #pragma unroll
     for (int u = 0; u < kCU; ++u)
       if (eb[u] != 0xFFFFFFFFu) entry_xyz(x4, eb[u], prd, &xb[u], &yb[u], &zb[u]);

@@ -1275,9 +1275,9 @@ __global__ void k_validate_list(int rows, int M, const uint32_t *__restrict__ li
     }
     return;
   }
-  const uint32_t *row = list + (size_t)(p / G) * M * G + (p % G);
+  const uint32_t *row = list + (G == kAtomGroup ? atom_slot(p, 0, M) : (size_t)(p / G) * M * G + (p % G));
   for (int k = 0; k < c; ++k) {
-    const uint32_t e = row[(size_t)k * G];
+    const uint32_t e = row[G == kAtomGroup ? atom_off(k) : (size_t)k * G];
     if ((e & 0x7FFFFFFu) >= nrow || (e >> 27) >= 27u) {
       if (atomicCAS(&flags[7], 0, 1) == 0) {
         flags[8] = p;

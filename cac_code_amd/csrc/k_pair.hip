@@ -390,10 +390,23 @@ __host__ __device__ __forceinline__ size_t mask_slot(int i, int w, int W) {
   return ((size_t)(i >> 6) * W + w) * kAtomGroup + (i & 63);
 }
 
+// U entries from k0 (a multiple of 4): one 16-byte load per run of 4 (the
+// run lies inside the row, whose length is a multiple of 8; entries past jn
+// are dropped)
 template <int U>
 __device__ __forceinline__ void fetch_atom_entries(const uint32_t *__restrict__ row, int k0, int jn, uint32_t e[U]) {
+  static_assert(U % kAtomRun == 0, "whole runs");
 #pragma unroll
-  for (int u = 0; u < U; ++u) e[u] = k0 + u < jn ? get_entry(row + (size_t)(k0 + u) * kAtomGroup) : kNoEntry;
+  for (int c = 0; c < U / kAtomRun; ++c) {
+    const int k = k0 + kAtomRun * c;
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    u32x4 v = {kNoEntry, kNoEntry, kNoEntry, kNoEntry};
+    if (k < jn) v = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(row + atom_off(k)));
+    e[kAtomRun * c] = v.x;
+    e[kAtomRun * c + 1] = k + 1 < jn ? v.y : kNoEntry;
+    e[kAtomRun * c + 2] = k + 2 < jn ? v.z : kNoEntry;
+    e[kAtomRun * c + 3] = k + 3 < jn ? v.w : kNoEntry;
+  }
 }
 
 // rho pass + fp pass fused (pair_eam.cpp:629-680); fp stored in x4.w, which
