@@ -42,6 +42,16 @@ __host__ __device__ __forceinline__ size_t atom_slot(int i, int k, int M) {
   return ((size_t)(i >> 6) * M + (k & ~(kAtomRun - 1))) * kAtomGroup + (size_t)(i & 63) * kAtomRun +
          (k & (kAtomRun - 1));
 }
+// Cluster-local integration-point lists (uint16 union indices): row p at
+// stride M rounded to 32; entry k, taken by lane k % 8 at its step k / 8,
+// lives in runs of 4 steps per lane, so a lane reads 4 consecutive entries
+// of its own with one 8-byte load and the 8 lanes of a point read 64
+// contiguous bytes.
+__host__ __device__ __forceinline__ size_t local_stride(int M) { return (size_t)((M + 31) & ~31); }
+__host__ __device__ __forceinline__ size_t local_slot(int p, int k, int M) {
+  const int l = k & 7, u = k >> 3;
+  return (size_t)p * local_stride(M) + (size_t)(u >> 2) * 32 + (size_t)l * 4 + (u & 3);
+}
 // offset of entry k from atom_slot(i, 0, M)
 __host__ __device__ __forceinline__ size_t atom_off(int k) {
   return (size_t)(k & ~(kAtomRun - 1)) * kAtomGroup + (k & (kAtomRun - 1));

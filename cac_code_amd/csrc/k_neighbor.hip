@@ -1775,7 +1775,7 @@ __global__ __launch_bounds__(kClBuildThreads) void k_cl_build(
       const int mid = (lo + hi) >> 1;
       if (uni[mid] < v) lo = mid + 1; else hi = mid;
     }
-    iplocal[q] = (uint16_t)lo;
+    iplocal[local_slot(pid[i], t - off[i], M)] = (uint16_t)lo;
   }
 }
 
@@ -1845,7 +1845,7 @@ int launch_clusters(cac_ctx *ctx) {
   if (lds > ctx->lds_bytes) return CAC_OK;  // lists too long for the cluster passes
   if ((rc = ensure(ctx, ctx->b_cl_union, (size_t)ctx->ncl * cap2, &ctx->cl_union)) ||
       (rc = ensure(ctx, ctx->b_cl_nu, (size_t)ctx->ncl, &ctx->cl_nu)) ||
-      (rc = ensure(ctx, ctx->b_iplocal, (size_t)M * round_up(nip, kRowGroup), &ctx->iplocal)))
+      (rc = ensure(ctx, ctx->b_iplocal, local_stride(M) * (size_t)nip, &ctx->iplocal)))
     return rc;
   ctx->cl_stride = cap2;
   NodeList8 nl;

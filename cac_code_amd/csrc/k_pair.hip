@@ -932,15 +932,18 @@ __device__ __forceinline__ void site_shape(const ElemArgs &a, int s, double N[8]
   N[7] = 0.125 * (1 - px) * (1 + py) * (1 + pz);
 }
 
-// Union indices of a lane's next kU list entries (k, k+8, ...), -1 past
-// the end or where the selection bits are clear.
+// Union indices of a lane's next kU list entries k, k+8, ... (steps
+// u0 .. u0+kU-1 of the lane, u0 a multiple of 4: one 8-byte load from the
+// lane's run), -1 past the end or where the selection bits are clear.
 template <int U>
-__device__ __forceinline__ void cl_indices(const uint16_t *__restrict__ row, int k, int jn, uint32_t sel, int q[U]) {
+__device__ __forceinline__ void cl_indices(const uint16_t *__restrict__ lrow, int u0, int k, int jn, uint32_t sel,
+                                           int q[U]) {
+  static_assert(U == 4, "one run of 4 steps");
+  typedef uint16_t u16x4 __attribute__((ext_vector_type(4)));
+  u16x4 v = {0, 0, 0, 0};
+  if (k < jn) v = *reinterpret_cast<const u16x4 *>(lrow + (size_t)(u0 >> 2) * 32);
 #pragma unroll
-  for (int u = 0; u < U; ++u) {
-    const int kk = k + u * kLanes;
-    q[u] = (kk < jn && ((sel >> u) & 1u)) ? (int)row[(size_t)kk * kRowGroup] : -1;
-  }
+  for (int u = 0; u < U; ++u) q[u] = (k + u * kLanes < jn && ((sel >> u) & 1u)) ? (int)v[u] : -1;
 }
 __device__ __forceinline__ uint32_t k0_mask(const uint32_t *__restrict__ mrow, int jn, int l) {
   return l < jn ? mrow[0] : 0u;
@@ -1054,19 +1057,19 @@ __device__ __forceinline__ void cl_erho_body(const ClArgs &c, const ElemArgs &a,
                  iz = a.nodex[24 * (size_t)i + 3 * k + 2];
     const int p = i * a.nint + a.node_list[k];
     const int jn = a.ipnum[p];
-    const uint16_t *row = c.local + (size_t)(p >> 3) * a.M * kRowGroup + (p & 7);
+    const uint16_t *lrow = c.local + local_slot(p, l, a.M);
     const int Wl = (a.M + 8 * 32 - 1) / (8 * 32);
     uint32_t *mrow = mask + ((size_t)p * kLanes + l) * Wl;
     uint32_t bits = 0;
     // software pipeline: the next chunk's union indices are requested
     // before this chunk's spline rows
     int qn[kU];
-    cl_indices<kU>(row, l, jn, 0xFFFFFFFFu, qn);
+    cl_indices<kU>(lrow, 0, l, jn, 0xFFFFFFFFu, qn);
     for (int k0 = l, j0 = 0; k0 < jn; k0 += kU * kLanes, j0 += kU) {
       int q[kU];
 #pragma unroll
       for (int u = 0; u < kU; ++u) q[u] = qn[u];
-      cl_indices<kU>(row, k0 + kU * kLanes, jn, 0xFFFFFFFFu, qn);
+      cl_indices<kU>(lrow, j0 + kU, k0 + kU * kLanes, jn, 0xFFFFFFFFu, qn);
       // distances and knots, then the rows (all in flight), then the sums
       P pp[kU];
       int mm[kU];
@@ -1152,7 +1155,7 @@ __device__ __forceinline__ void cl_force_body(const ClArgs &c, const ElemArgs &a
     rhoi = enode[8 * gi + k];
     fpi = embed_fp<P>(t, rhoi);
     const int jn = a.ipnum[p];
-    const uint16_t *row = c.local + (size_t)(p >> 3) * a.M * kRowGroup + (p & 7);
+    const uint16_t *lrow = c.local + local_slot(p, l, a.M);
     const int Wl = (a.M + 8 * 32 - 1) / (8 * 32);
     const uint32_t *mrow = mask + ((size_t)p * kLanes + l) * Wl;
     // software pipeline: the union indices and mask word of the next chunk
@@ -1160,7 +1163,7 @@ __device__ __forceinline__ void cl_force_body(const ClArgs &c, const ElemArgs &a
     // this chunk computes
     uint32_t bits = k0_mask(mrow, jn, l);
     int q[kU];
-    cl_indices<kU>(row, l, jn, bits, q);
+    cl_indices<kU>(lrow, 0, l, jn, bits, q);
     for (int k0 = l, j0 = 0; k0 < jn; k0 += kU * kLanes, j0 += kU) {
       int qc[kU];
 #pragma unroll
@@ -1168,7 +1171,7 @@ __device__ __forceinline__ void cl_force_body(const ClArgs &c, const ElemArgs &a
       {
         const int j1 = j0 + kU, k1 = k0 + kU * kLanes;
         if ((j1 & 31) == 0 && k1 < jn) bits = mrow[j1 >> 5];
-        cl_indices<kU>(row, k1, jn, bits >> (j1 & 31), q);
+        cl_indices<kU>(lrow, j1, k1, jn, bits >> (j1 & 31), q);
       }
 #pragma unroll
       for (int u = 0; u < kU; ++u) {
