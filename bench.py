@@ -392,6 +392,33 @@ def drift(s, pair, cutneigh, skin, every, delay, nsteps, precision, chunk=1000, 
             "every": chunk, "series_eV_per_particle": series, "seconds": round(time.perf_counter() - t0, 1)}
 
 
+def drift_regions(pair, cutneigh, skin, args):
+    """The NVE drift of each half of the method on its own, FP64 and mixed:
+    a periodic 300 K crystal of 500,000 atoms (the finest mesh, where the
+    forces are the gradient of the energy) and the workload's whole box
+    meshed into nae = 11 elements (23,328), whose force-based element forces
+    (pair_eam.cpp:1013-1134) are not.  The mixed-minus-FP64 energy difference
+    is what the precision mode adds."""
+    from cac_code_amd import _lib
+    from cac_code_amd.lattice import build_cac_system
+    regions = {"atoms_only": build_cac_system(50, 5, 1.0e6, temperature=300.0, seed=args.seed),
+               "elements_only": build_cac_system(198, 11, 0.0, temperature=300.0, seed=args.seed)}
+    out = {}
+    for name, s in regions.items():
+        rec = {"atoms": s.n_atoms, "elements": s.n_elements}
+        for tag, prec in (("fp64", _lib.PREC_FP64), ("mixed", _lib.PREC_MIXED)):
+            d = drift(s, pair, cutneigh, skin, args.every, args.delay, args.drift_steps, prec)
+            rec[tag] = {k: d[k] for k in ("steps", "diverged_at", "drift_eV_per_particle", "max_abs_eV_per_particle",
+                                          "series_eV_per_particle")}
+        a, b = rec["mixed"]["series_eV_per_particle"], rec["fp64"]["series_eV_per_particle"]
+        rec["mixed_minus_fp64_max_eV_per_particle"] = max(abs(x - y) for x, y in zip(a, b))
+        log(f"[bench] drift {name}: fp64 {rec['fp64']['drift_eV_per_particle']:+.3e}, mixed "
+            f"{rec['mixed']['drift_eV_per_particle']:+.3e} eV/particle after {rec['fp64']['steps']} steps; "
+            f"|mixed - fp64| <= {rec['mixed_minus_fp64_max_eV_per_particle']:.2e}")
+        out[name] = rec
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -518,6 +545,8 @@ def main():
                 end = (f"diverged (|dE| > {d['limit_eV_per_particle']} eV/particle) at step {d['diverged_at']}"
                        if d["diverged_at"] else f"after {d['steps']} steps")
                 log(f"[bench] {k}: {d['drift_eV_per_particle']:+.3e} eV/particle, {end}")
+            if args.workload == "eam_c1":
+                acc["drift_regions"] = drift_regions(pair, cutneigh, skin, args)
 
     cpu = None
     if not args.no_cpu:
