@@ -1625,14 +1625,6 @@ int site_reactions(cac_ctx *ctx, bool eam, const ElemArgs &a) {
   return CAC_OK;
 }
 
-// One EAM force call (PairEAM::compute + ecompute, pair_eam.cpp:584-746,
-// 927-1137) in the order the coupled lists need: site positions, atom
-// density + F', node densities, site F', atom forces, integration-point
-// forces + scatter.  Without coupling the atom and element halves are
-// independent and the order changes nothing.  The element passes run over
-// optional subsets (surrogate mode): site positions over `spos`, node
-// densities over `erho_el`, site F' over `sfp`, forces and scatter over
-// `force_el`.
 // LDS window of a per-knot table with rows of `rowbytes`: the last rows up to
 // the device's shared-memory limit per workgroup (rows m0 .. nr-1).
 struct TabWindow {
@@ -1786,6 +1778,14 @@ int eam_clusters(cac_ctx *ctx, const ElemArgs &base) {
   return site_reactions<P>(ctx, true, base);
 }
 
+// One EAM force call (PairEAM::compute + ecompute, pair_eam.cpp:584-746,
+// 927-1137) in the order the coupled lists need: site positions, atom
+// density + F', node densities, site F', atom forces, integration-point
+// forces + scatter.  Without coupling the atom and element halves are
+// independent and the order changes nothing.  The element passes run over
+// optional subsets (surrogate mode): site positions over `spos`, node
+// densities over `erho_el`, site F' over `sfp`, forces and scatter over
+// `force_el`.
 template <class P, bool EV>
 int eam_all(cac_ctx *ctx, const ElemArgs &base, const ElemArgs &spos, const ElemArgs &erho_el, const ElemArgs &sfp,
             const ElemArgs &force_el) {
