@@ -217,15 +217,29 @@ __global__ __launch_bounds__(kBlock) void k_cell_maxcand(const int *__restrict__
 }
 
 constexpr int kNeighWaves = 4;
+constexpr int kCodeWords = 64;  // presence bitmap of the lattice-rounded displacements, per wavefront
+
+// Dense code of a list key's lattice-rounded displacement (k_x, k_y, k_z),
+// each in [-qk, qk]: ((k_z + qk) * D + k_y + qk) * D + k_x + qk with
+// D = 2 qk + 1, which orders codes as the keys' high words order them; -1
+// outside that box.
+__device__ __forceinline__ int list_code(uint64_t key, int qk, int D) {
+  const uint32_t hi = (uint32_t)(key >> 32);
+  const int kx = (int)(hi & 1023u) - 512, ky = (int)((hi >> 10) & 1023u) - 512, kz = (int)(hi >> 20) - 512;
+  if (kx < -qk || kx > qk || ky < -qk || ky > qk || kz < -qk || kz > qk) return -1;
+  return ((kz + qk) * D + (ky + qk)) * D + (kx + qk);
+}
 
 __global__ __launch_bounds__(64 * kNeighWaves) void k_atom_neigh(
     int ncells, const int *__restrict__ cells, const double *__restrict__ x4, Box3 b, Grid3 g,
     const uint32_t *__restrict__ sorted, const int *__restrict__ cs, const int *__restrict__ ce, double cutsq,
-    int maxneigh, int cap2, int ccap, double qinv, int n, uint32_t *__restrict__ neigh, int *__restrict__ num,
-    double *__restrict__ xhold, int *__restrict__ flags) {
+    int maxneigh, int cap2, int ccap, double qinv, int qk, int n, uint32_t *__restrict__ neigh,
+    int *__restrict__ num, double *__restrict__ xhold, int *__restrict__ flags) {
   // LDS: keys[kNeighWaves][cap2] (u64), then sx, sy, sz [ccap] (f64), se [ccap] (u32)
   extern __shared__ uint64_t smem[];
   __shared__ int pre[28], rstart[27], rimg[27];
+  __shared__ uint32_t cbits_all[kNeighWaves][kCodeWords];
+  __shared__ int cpre_all[kNeighWaves][kCodeWords];
   const int w = xcd_block();
   if (w >= ncells) return;  // whole workgroup
   const int tid = (int)threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -270,6 +284,9 @@ __global__ __launch_bounds__(64 * kNeighWaves) void k_atom_neigh(
     for (int f = tid; f < T; f += 64 * kNeighWaves) fetch(f, sx[f], sy[f], sz[f], se[f]);
   __syncthreads();
   uint64_t *keys = smem + wave * cap2;
+  uint32_t *cbits = cbits_all[wave];
+  int *cpre = cpre_all[wave];
+  const int D = 2 * qk + 1, nwords = qk > 0 ? (D * D * D + 31) >> 5 : 0;
   for (int h = cs[c] + wave; h < ce[c]; h += kNeighWaves) {
     const int i = (int)sorted[h];
     const double xi = x4[4 * (size_t)i], yi = x4[4 * (size_t)i + 1], zi = x4[4 * (size_t)i + 2];
@@ -324,6 +341,44 @@ __global__ __launch_bounds__(64 * kNeighWaves) void k_atom_neigh(
     }
     wave_sync();
     const int kept = cnt < cap2 ? cnt : cap2;
+    // On a lattice the rounded displacements of one row are distinct: mark
+    // each in a presence bitmap, and a key's rank is the number of marked
+    // codes below its own (a prefix popcount).  A row where two keys share a
+    // code (or one lies outside the code box) is ranked by the sweep below,
+    // which also breaks such ties by entry; both give the same row.
+    bool fast = nwords > 0;
+    if (fast) {
+      if (lane < nwords) cbits[lane] = 0u;
+      wave_sync();
+      bool clash = false;
+      for (int a = lane; a < kept; a += 64) {
+        const int code = list_code(keys[a], qk, D);
+        if (code < 0) {
+          clash = true;
+        } else {
+          const uint32_t bit = 1u << (code & 31);
+          clash |= (atomicOr(&cbits[code >> 5], bit) & bit) != 0u;
+        }
+      }
+      fast = __ballot(clash) == 0ull;
+      wave_sync();
+    }
+    if (fast) {
+      const uint32_t wv = lane < nwords ? cbits[lane] : 0u;
+      int incl = __popc(wv);
+      for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(incl, o);
+        incl += lane >= o ? t : 0;
+      }
+      cpre[lane] = incl - __popc(wv);
+      wave_sync();
+      for (int a = lane; a < kept; a += 64) {
+        const uint64_t ka = keys[a];
+        const int code = list_code(ka, qk, D);
+        const int r = cpre[code >> 5] + __popc(cbits[code >> 5] & ((1u << (code & 31)) - 1u));
+        if (r < maxneigh) neigh[atom_slot(i, r, maxneigh)] = (uint32_t)ka;
+      }
+    } else {
     // rank of keys a and a + 64 in one sweep over the row (broadcast reads,
     // unrolled so that several are in flight)
     for (int a = lane; a < ((kept + 127) & ~127); a += 128) {
@@ -337,6 +392,7 @@ __global__ __launch_bounds__(64 * kNeighWaves) void k_atom_neigh(
       }
       if (a < kept && ra < maxneigh) neigh[atom_slot(i, ra, maxneigh)] = (uint32_t)ka;
       if (a + 64 < kept && rb < maxneigh) neigh[atom_slot(i, rb, maxneigh)] = (uint32_t)kb;
+    }
     }
     if (lane == 0) {
       num[i] = cnt;
@@ -1471,9 +1527,17 @@ int launch_atom_neighbors(cac_ctx *ctx) {
   ccap = ccap < ccap_max ? ccap : ccap_max;
   const size_t lds = key_bytes + (size_t)ccap * 28;
   const double qinv = ctx->list_spacing > 0.0 ? 1.0 / ctx->list_spacing : 0.0;
+  // code box of the rounded displacements within cutneigh (one spare unit
+  // each way); 0 = rank every row by the sweep
+  int qk = 0;
+  if (qinv > 0.0) {
+    const double kmax = ceil(ctx->cutneigh * qinv) + 1.0;
+    if (kmax <= 16.0 && (2.0 * kmax + 1.0) * (2.0 * kmax + 1.0) * (2.0 * kmax + 1.0) <= 32.0 * kCodeWords)
+      qk = (int)kmax;
+  }
   hipLaunchKernelGGL(k_atom_neigh, dim3(xcd_grid(ncells, 1)), dim3(64 * kNeighWaves), lds, ctx->stream, ncells, cells,
                      ctx->x4, box_of(ctx), g, sorted, cs, ce, ctx->cutneigh * ctx->cutneigh, ctx->maxneigh, cap2, ccap,
-                     qinv, n, ctx->aneigh, ctx->anum, ctx->xhold, ctx->dflags);
+                     qinv, qk, n, ctx->aneigh, ctx->anum, ctx->xhold, ctx->dflags);
   timer_end(ctx, "atom_neigh", t0);
   CAC_HIP(ctx, hipGetLastError());
   return CAC_OK;
