@@ -185,6 +185,13 @@ struct cac_ctx {
   uint64_t *atmp = nullptr;  // [niface][maxneigh] raw (element | img<<27 | site<<32)
   int *etouch = nullptr;     // [ne] element has an atom in a list or a site in an atom list
   cac::DevBuf b_iflag, b_ipos, b_iface, b_asnum, b_atmp, b_etouch;
+  // Every atom-site pair of the coupled atom lists once, as (active site
+  // a) << 32 | (atom | img << 27): interface atom r's site entries at
+  // sent[sbase[r] ..], nsent in all.  Sorted, they are the reaction lists.
+  long nsent = 0;
+  int *sbase = nullptr;
+  uint64_t *sent = nullptr, *sent2 = nullptr;  // sent2: the sorted copy
+  cac::DevBuf b_sbase, b_sent, b_sent2;
   // Reactions of the atom-site pair forces (coupling): per active site the
   // atoms whose lists hold it, as sorted entries (atom | img << 27) in
   // rkey[rstart[a] .. rstart[a+1]); the sites with at least one, ascending

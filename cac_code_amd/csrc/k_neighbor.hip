@@ -1293,18 +1293,24 @@ __global__ void k_ip_translate(int nip, int maxnip, const uint64_t *__restrict__
 }
 
 // Coupled atom lists: the raw site entries of flagged atom r go after the
-// atom's atom entries.
+// atom's atom entries, and each pair also to the compact array:
+// sent[sbase[r] + k] = (active site) << 32 | (i | img << 27).
 __global__ void k_atom_site_translate(int niface, int maxneigh, const int *__restrict__ iface,
                                       const uint64_t *__restrict__ atmp, const int *__restrict__ asnum,
                                       const int *__restrict__ anum, int n, int nwords,
                                       const uint32_t *__restrict__ bitmap, const int *__restrict__ wprefix,
-                                      uint32_t *__restrict__ aneigh) {
+                                      uint32_t *__restrict__ aneigh, const int *__restrict__ sbase,
+                                      uint64_t *__restrict__ sent) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= niface) return;
   const int i = iface[r], a = anum[i];
   const uint64_t *row = atmp + (size_t)r * maxneigh;
-  for (int k = 0; k < asnum[r] && a + k < maxneigh; ++k)
-    aneigh[atom_slot(i, a + k, maxneigh)] = particle_entry(row[k], n, nwords, bitmap, wprefix);
+  uint64_t *out = sent + sbase[r];
+  for (int k = 0; k < asnum[r]; ++k) {
+    const uint32_t e = particle_entry(row[k], n, nwords, bitmap, wprefix);
+    if (a + k < maxneigh) aneigh[atom_slot(i, a + k, maxneigh)] = e;
+    out[k] = ((uint64_t)((e & 0x7FFFFFFu) - (uint32_t)n) << 32) | ((uint32_t)i | (e & ~0x7FFFFFFu));
+  }
 }
 
 __global__ void k_atom_site_count(int niface, const int *__restrict__ iface, const int *__restrict__ asnum,
@@ -1675,7 +1681,7 @@ int launch_atom_sites(cac_ctx *ctx) {
     return CAC_OK;
   }
   if ((rc = ensure(ctx, ctx->b_iface, (size_t)niface, &ctx->iface)) ||
-      (rc = ensure(ctx, ctx->b_asnum, (size_t)niface, &ctx->asnum)) ||
+      (rc = ensure(ctx, ctx->b_asnum, (size_t)niface + 1, &ctx->asnum)) ||
       (rc = ensure(ctx, ctx->b_atmp, (size_t)niface * ctx->maxneigh, &ctx->atmp)))
     return rc;
   hipLaunchKernelGGL(k_iface_compact, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, n, ctx->iflag, ctx->ipos,
@@ -1695,6 +1701,7 @@ int launch_atom_sites(cac_ctx *ctx) {
 // the lists as particle entries.
 int launch_active_sites(cac_ctx *ctx) {
   const int ne = ctx->ne;
+  ctx->nsent = 0;
   if (ne == 0) return CAC_OK;
   const int nip = ne * ctx->nint;
   const int nw = ctx->nwords;
@@ -1730,9 +1737,22 @@ int launch_active_sites(cac_ctx *ctx) {
                      ctx->stream, nip, ctx->maxnip, ctx->iptmp,
                      ctx->ipnum, n, nw, ctx->bitmap, ctx->wprefix, ctx->ipneigh);
   if (ctx->niface > 0) {
+    // bases of the interface atoms' pairs in the compact array (asnum[niface] = 0: the total)
+    const int ni = ctx->niface;
+    if ((rc = ensure(ctx, ctx->b_sbase, (size_t)ni + 1, &ctx->sbase))) return rc;
+    CAC_HIP(ctx, hipMemsetAsync(ctx->asnum + ni, 0, sizeof(int), ctx->stream));
+    size_t sb = 0;
+    CAC_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, sb, ctx->asnum, ctx->sbase, ni + 1, ctx->stream));
+    if ((rc = ensure(ctx, ctx->b_scantmp, sb, &tmp))) return rc;
+    CAC_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(tmp, sb, ctx->asnum, ctx->sbase, ni + 1, ctx->stream));
+    int ns = 0;
+    CAC_HIP(ctx, hipMemcpyAsync(&ns, ctx->sbase + ni, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->nsent = ns;
+    if ((rc = ensure(ctx, ctx->b_sent, (size_t)(ns > 0 ? ns : 1), &ctx->sent))) return rc;
     hipLaunchKernelGGL(k_atom_site_translate, dim3(grid_for(ctx->niface)), dim3(kBlock), 0, ctx->stream, ctx->niface,
                        ctx->maxneigh, ctx->iface, ctx->atmp, ctx->asnum, ctx->anum, n, nw, ctx->bitmap, ctx->wprefix,
-                       ctx->aneigh);
+                       ctx->aneigh, ctx->sbase, ctx->sent);
     hipLaunchKernelGGL(k_atom_site_count, dim3(grid_for(ctx->niface)), dim3(kBlock), 0, ctx->stream, ctx->niface,
                        ctx->iface, ctx->asnum, ctx->anum);
   }
@@ -1845,17 +1865,9 @@ __global__ __launch_bounds__(kClBuildThreads) void k_cl_build(
 
 // Active sites that atom lists refer to (atom-element coupling): marks, then
 // the ascending list of marked ids.
-__global__ void k_mark_aref(int niface, int maxneigh, const int *__restrict__ iface, const int *__restrict__ anum,
-                            const uint32_t *__restrict__ aneigh, int n, int *__restrict__ mark) {
+__global__ void k_mark_sent(long ns, const uint64_t *__restrict__ sent, int *__restrict__ mark) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)niface * maxneigh) return;
-  // consecutive threads take consecutive interface atoms at one entry k: the
-  // atoms of a 64-row group share the line that holds their k-th entries
-  const int k = (int)(t / niface), r = (int)(t % niface);
-  const int i = iface[r];
-  if (k >= anum[i]) return;
-  const uint32_t j = aneigh[atom_slot(i, k, maxneigh)] & 0x7FFFFFFu;
-  if ((int)j >= n) mark[j - n] = 1;
+  if (t < ns) mark[sent[t] >> 32] = 1;
 }
 
 __global__ void k_compact_marks(int na, const int *__restrict__ mark, const int *__restrict__ pos,
@@ -1931,9 +1943,9 @@ int launch_clusters(cac_ctx *ctx) {
       return rc;
     pos = ctx->amark + na + 1;
     CAC_HIP(ctx, hipMemsetAsync(ctx->amark, 0, ((size_t)na + 1) * sizeof(int), ctx->stream));
-    const long tot = (long)ctx->niface * ctx->maxneigh;
-    hipLaunchKernelGGL(k_mark_aref, dim3(grid_for(tot)), dim3(kBlock), 0, ctx->stream, ctx->niface, ctx->maxneigh,
-                       ctx->iface, ctx->anum, ctx->aneigh, ctx->n, ctx->amark);
+    if (ctx->nsent > 0)
+      hipLaunchKernelGGL(k_mark_sent, dim3(grid_for(ctx->nsent)), dim3(kBlock), 0, ctx->stream, ctx->nsent, ctx->sent,
+                         ctx->amark);
     size_t tmp_bytes = 0;
     CAC_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, ctx->amark, pos, na + 1, ctx->stream));
     char *tmp;
@@ -1989,56 +2001,23 @@ int launch_clusters(cac_ctx *ctx) {
 }
 
 // ------------------------------------------------------------ reactions
-// Reverse lists of the atom-site pairs (coupling): every atom list entry
-// that names an active site (particle row >= n) is counted for that site,
-// the counts are scanned into ranges, the ranges filled with the entries
-// (atom | img << 27, the image the atom's list gives the site) and each range
-// sorted, so the per-step reaction sum has a fixed order.  The traversal is
-// k_mark_aref's: consecutive threads take consecutive interface atoms at one
-// entry k.
-__global__ void k_rev_count(int niface, int maxneigh, const int *__restrict__ iface, const int *__restrict__ anum,
-                            const uint32_t *__restrict__ aneigh, int n, int *__restrict__ rcount) {
+// Reverse lists of the atom-site pairs (coupling): per active site the
+// atoms whose lists name it (atom | img << 27, the image the atom's list
+// gives the site), ascending, so the per-step reaction sum has a fixed
+// order.  The compact pair array sent (site << 32 | atom entry) is counted
+// per site, the counts scanned into ranges, and the array radix-sorted:
+// sorted, its low words are the ranges' entries in order.
+__global__ void k_rev_count(long ns, const uint64_t *__restrict__ sent, int *__restrict__ rcount) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)niface * maxneigh) return;
-  const int k = (int)(t / niface), r = (int)(t % niface);
-  const int i = iface[r];
-  if (k >= anum[i]) return;
-  const uint32_t j = aneigh[atom_slot(i, k, maxneigh)] & 0x7FFFFFFu;
-  if ((int)j >= n) atomicAdd(&rcount[j - n], 1);
+  if (t < ns) atomicAdd(&rcount[sent[t] >> 32], 1);
 }
 
-__global__ void k_rev_fill(int niface, int maxneigh, const int *__restrict__ iface, const int *__restrict__ anum,
-                           const uint32_t *__restrict__ aneigh, int n, const int *__restrict__ rstart,
-                           int *__restrict__ rfill, uint32_t *__restrict__ rkey) {
+// rkey from the sorted pairs; rcount[a] becomes the "has entries" flag.
+__global__ void k_rev_keys(long ns, int na, const uint64_t *__restrict__ sorted, uint32_t *__restrict__ rkey,
+                           int *__restrict__ rcount) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)niface * maxneigh) return;
-  const int k = (int)(t / niface), r = (int)(t % niface);
-  const int i = iface[r];
-  if (k >= anum[i]) return;
-  const uint32_t e = aneigh[atom_slot(i, k, maxneigh)];
-  const uint32_t j = e & 0x7FFFFFFu;
-  if ((int)j < n) return;
-  const int slot = rstart[j - n] + atomicAdd(&rfill[j - n], 1);
-  rkey[slot] = (uint32_t)i | (e & ~0x7FFFFFFu);
-}
-
-// One thread per active site: insertion sort of its range (a few tens of
-// entries), then flag[a] = 1 if the site has any.
-__global__ void k_rev_sort(int na, const int *__restrict__ rstart, uint32_t *__restrict__ rkey,
-                           int *__restrict__ flag) {
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= na) return;
-  const int b = rstart[a], e = rstart[a + 1];
-  for (int q = b + 1; q < e; ++q) {
-    const uint32_t v = rkey[q];
-    int p = q - 1;
-    while (p >= b && rkey[p] > v) {
-      rkey[p + 1] = rkey[p];
-      --p;
-    }
-    rkey[p + 1] = v;
-  }
-  flag[a] = e > b ? 1 : 0;
+  if (t < ns) rkey[t] = (uint32_t)sorted[t];
+  if (t < na) rcount[t] = rcount[t] > 0 ? 1 : 0;
 }
 
 // Elements of the reaction sites (ascending site ids are grouped by
@@ -2068,36 +2047,34 @@ __global__ void k_rel_compact(int nrs, const int *__restrict__ rsite, const int2
 int launch_reaction_lists(cac_ctx *ctx) {
   ctx->nrs = ctx->nre = 0;
   ctx->nrev = 0;
-  const int na = ctx->nactive, n = ctx->n;
-  if (!ctx->coupled || ctx->niface == 0 || na == 0) return CAC_OK;
+  const int na = ctx->nactive;
+  const long ns = ctx->nsent;
+  if (!ctx->coupled || ctx->niface == 0 || na == 0 || ns == 0) return CAC_OK;
   int rc;
   if ((rc = ensure(ctx, ctx->b_rcount, (size_t)na + 1, &ctx->rcount)) ||
       (rc = ensure(ctx, ctx->b_rstart, (size_t)na + 1, &ctx->rstart)) ||
-      (rc = ensure(ctx, ctx->b_rsite, (size_t)na, &ctx->rsite)))
+      (rc = ensure(ctx, ctx->b_rsite, (size_t)na, &ctx->rsite)) ||
+      (rc = ensure(ctx, ctx->b_rkey, (size_t)ns, &ctx->rkey)) ||
+      (rc = ensure(ctx, ctx->b_sent2, (size_t)ns, &ctx->sent2)))
     return rc;
   int *tmpi;
   if ((rc = ensure(ctx, ctx->b_rtmp, 2 * ((size_t)na + 1), &tmpi))) return rc;
-  const long tot = (long)ctx->niface * ctx->maxneigh;
+  uint64_t *sorted = ctx->sent2;
   CAC_HIP(ctx, hipMemsetAsync(ctx->rcount, 0, ((size_t)na + 1) * sizeof(int), ctx->stream));
-  hipLaunchKernelGGL(k_rev_count, dim3(grid_for(tot)), dim3(kBlock), 0, ctx->stream, ctx->niface, ctx->maxneigh,
-                     ctx->iface, ctx->anum, ctx->aneigh, n, ctx->rcount);
-  size_t tmp_bytes = 0;
+  hipLaunchKernelGGL(k_rev_count, dim3(grid_for(ns)), dim3(kBlock), 0, ctx->stream, ns, ctx->sent, ctx->rcount);
+  int bits = 32;
+  while ((1L << (bits - 32)) < na) ++bits;
+  size_t tmp_bytes = 0, sort_bytes = 0;
   CAC_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, ctx->rcount, ctx->rstart, na + 1, ctx->stream));
+  CAC_HIP(ctx, hipcub::DeviceRadixSort::SortKeys(nullptr, sort_bytes, ctx->sent, sorted, (int)ns, 0, bits,
+                                                 ctx->stream));
   char *tmp;
-  if ((rc = ensure(ctx, ctx->b_scantmp, tmp_bytes, &tmp))) return rc;
+  if ((rc = ensure(ctx, ctx->b_scantmp, std::max(tmp_bytes, sort_bytes), &tmp))) return rc;
   CAC_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, ctx->rcount, ctx->rstart, na + 1, ctx->stream));
-  int nrev = 0;
-  CAC_HIP(ctx, hipMemcpyAsync(&nrev, ctx->rstart + na, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-  CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  ctx->nrev = nrev;
-  if (nrev == 0) return CAC_OK;
-  if ((rc = ensure(ctx, ctx->b_rkey, (size_t)nrev, &ctx->rkey))) return rc;
-  CAC_HIP(ctx, hipMemsetAsync(ctx->rcount, 0, ((size_t)na + 1) * sizeof(int), ctx->stream));
-  hipLaunchKernelGGL(k_rev_fill, dim3(grid_for(tot)), dim3(kBlock), 0, ctx->stream, ctx->niface, ctx->maxneigh,
-                     ctx->iface, ctx->anum, ctx->aneigh, n, ctx->rstart, ctx->rcount, ctx->rkey);
-  // rcount[a] becomes the "has entries" flag, rcount[na] stays 0
-  hipLaunchKernelGGL(k_rev_sort, dim3(grid_for(na)), dim3(kBlock), 0, ctx->stream, na, ctx->rstart, ctx->rkey,
-                     ctx->rcount);
+  CAC_HIP(ctx, hipcub::DeviceRadixSort::SortKeys(tmp, sort_bytes, ctx->sent, sorted, (int)ns, 0, bits, ctx->stream));
+  ctx->nrev = ns;
+  hipLaunchKernelGGL(k_rev_keys, dim3(grid_for(std::max(ns, (long)na))), dim3(kBlock), 0, ctx->stream, ns, na, sorted,
+                     ctx->rkey, ctx->rcount);
   int *pos = tmpi;
   CAC_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, ctx->rcount, pos, na + 1, ctx->stream));
   hipLaunchKernelGGL(k_compact_marks, dim3(grid_for(na)), dim3(kBlock), 0, ctx->stream, na, ctx->rcount, pos,
