@@ -361,14 +361,30 @@ struct Virial {
   }
 };
 
-// EAM pair terms shared by the atom and integration-point force passes,
-// pair_eam.cpp:706-738 / 1083-1120.
+// EAM pair terms of the atom force pass, pair_eam.cpp:706-738 (the
+// integration-point passes evaluate the same terms, :1083-1120).
 template <class P>
 __device__ __forceinline__ double eam_fpair(const Tabs<P> &t, const Win<P> &w, double rsq, P fpi, P fpj,
                                             P *phi_out) {
   const RInvP<P> rr = r_and_inv<P>(rsq);
   P r = rr.r, rhoip, z2p, z2;
   rz_of_r_w<P>(t, w, r, &rhoip, &z2p, &z2);
+  P recip = rr.rinv;
+  P phi = z2 * recip;
+  P phip = z2p * recip - phi * recip;
+  P psip = fpi * rhoip + fpj * rhoip + phip;
+  *phi_out = phi;
+  return (double)(-psip * recip);
+}
+
+// The same from the global rows (the same bits as the knot form), for the
+// reactions, which have no LDS window: the rows need no coefficient
+// recompute.
+template <class P>
+__device__ __forceinline__ double eam_fpair_global(const Tabs<P> &t, double rsq, P fpi, P fpj, P *phi_out) {
+  const RInvP<P> rr = r_and_inv<P>(rsq);
+  P rhoip, z2p, z2;
+  rz_of_r<P>(t, rr.r, &rhoip, &z2p, &z2);
   P recip = rr.rinv;
   P phi = z2 * recip;
   P phip = z2p * recip - phi * recip;
@@ -1338,7 +1354,8 @@ __global__ void k_scatter(ElemArgs a, const double *__restrict__ ipf, double *__
 // equal and opposite totals (sum_k N_k = 1).  k_site_react_* recompute f_is
 // for the atoms of each site's reverse list (rkey, ascending) with the atom
 // pass's own operations: del = x_i - (x_s + img*prd), the same pair
-// function and arguments in the same order.
+// arithmetic and arguments in the same order (the global rows, whose
+// coefficients the atom pass's knot form reproduces bit for bit).
 struct ReactArgs {
   int nrs, n;
   const int *rsite, *rstart;
@@ -1358,7 +1375,6 @@ __global__ __launch_bounds__(kBlock) void k_site_react_eam(ReactArgs r, const do
   const int q = gt >> 3, l = gt & 7;
   if (*r.skip) return;
   const bool live = q < r.nrs;
-  const Win<P> w = Win<P>{nullptr, t.nr + 2};  // no LDS window: every row from the tables
   double rx = 0.0, ry = 0.0, rz = 0.0;
   if (live) {
     const int a = r.rsite[q];
@@ -1395,7 +1411,7 @@ __global__ __launch_bounds__(kBlock) void k_site_react_eam(ReactArgs r, const do
         const double rsq = delx * delx + dely * dely + delz * delz;
         if (rsq < t.cutforcesq) {
           P phi;
-          const double fpair = eam_fpair<P>(t, w, rsq, (P)izw[u].y, fps, &phi);
+          const double fpair = eam_fpair_global<P>(t, rsq, (P)izw[u].y, fps, &phi);
           rx -= delx * fpair;
           ry -= dely * fpair;
           rz -= delz * fpair;
