@@ -260,12 +260,14 @@ struct cac_ctx {
   uint16_t *iplocal = nullptr;
   int cl_stride = 0, cl_umax = 0;
   // launch order of the cluster passes: the clusters with unions of more
-  // than kUSmall entries (cl_nbig, rows in cl_scratch slot cl_big[cl]) at the
-  // start of every XCD's share, the rest (largest union cl_umax_small) after
-  int *cl_order = nullptr, *cl_big = nullptr;
+  // than kUSmall entries (cl_nbig, rows in a cl_scratch slot) at the start of
+  // every XCD's share, the rest (largest union cl_umax_small) after.
+  // cl_order[o]: launch slot o's cluster, slot start, union size and
+  // (scratch slot or -1) * 16 + slot count, one load for a wavefront's start
+  int4 *cl_order = nullptr;
   int cl_nbig = 0, cl_umax_small = 0;
   double *cl_scratch = nullptr;
-  cac::DevBuf b_cl_big, b_cl_scratch;
+  cac::DevBuf b_cl_scratch;
   // centro-symmetry values of the last cac_centro_atoms call
   double *centro = nullptr;
   cac::DevBuf b_centro;

@@ -1991,11 +1991,16 @@ int launch_clusters(cac_ctx *ctx) {
   ctx->cl_umax_small = small_max;
   const size_t sstride = (size_t)4 * ctx->cl_umax + kClMax * (24 + 8);
   if ((rc = ensure(ctx, ctx->b_cl_order, (size_t)ctx->ncl, &ctx->cl_order)) ||
-      (rc = ensure(ctx, ctx->b_cl_big, (size_t)ctx->ncl, &ctx->cl_big)) ||
       (rc = ensure(ctx, ctx->b_cl_scratch, sstride * (nbig > 0 ? nbig : 1), &ctx->cl_scratch)))
     return rc;
-  CAC_HIP(ctx, hipMemcpy(ctx->cl_order, order.data(), (size_t)ctx->ncl * sizeof(int), hipMemcpyHostToDevice));
-  CAC_HIP(ctx, hipMemcpy(ctx->cl_big, big.data(), (size_t)ctx->ncl * sizeof(int), hipMemcpyHostToDevice));
+  std::vector<int> start((size_t)ctx->ncl + 1);
+  CAC_HIP(ctx, hipMemcpy(start.data(), ctx->cl_start, start.size() * sizeof(int), hipMemcpyDeviceToHost));
+  std::vector<int4> meta(ctx->ncl);
+  for (int o = 0; o < ctx->ncl; ++o) {
+    const int c = order[o];
+    meta[o] = make_int4(c, start[c], nu[c], big[c] * 16 + (start[c + 1] - start[c]));
+  }
+  CAC_HIP(ctx, hipMemcpy(ctx->cl_order, meta.data(), (size_t)ctx->ncl * sizeof(int4), hipMemcpyHostToDevice));
   ctx->cl_ok = true;
   return CAC_OK;
 }

@@ -913,17 +913,17 @@ __global__ __launch_bounds__(kBlock) void k_eam_ip_force(ElemArgs a, Prd prd, Ta
 // gathering x4 rows from HBM, and sum in the same order: the same bits.
 struct ClArgs {
   int ncl, n, stride;
-  const int *start, *slot;
+  const int *slot;
   const uint32_t *uni;     // [ncl][stride]
-  const int *nu;
   const uint16_t *local;   // ipneigh layout
-  const int *order;        // clusters of this launch
+  // per launch slot: cluster, slot start, union size, (scratch slot or -1)
+  // * 16 + slot count
+  const int4 *order;
   int count;
   int rows;                // LDS rows of this launch (its largest small union)
   // clusters whose union exceeds the LDS rows keep their rows in global
-  // memory instead: big[cl] = slot in scratch (-1: LDS), slot stride sstride
+  // memory instead (a scratch slot in order[o].w), slot stride sstride
   // doubles, srows rows
-  const int *big;
   double *scratch;
   size_t sstride;
   int srows;
@@ -971,6 +971,7 @@ template <bool FP, class PX>
 __device__ __forceinline__ void site_interp(PX X, PX E, const double N[8], double *x, double *y, double *z,
                                             double *rho) {
   double xx = 0.0, yy = 0.0, zz = 0.0;
+#pragma unroll
   for (int k = 0; k < 8; ++k) {
     xx += N[k] * X[3 * k];
     yy += N[k] * X[3 * k + 1];
@@ -981,6 +982,7 @@ __device__ __forceinline__ void site_interp(PX X, PX E, const double N[8], doubl
   *z = zz;
   if (FP) {
     double r = 0.0;
+#pragma unroll
     for (int k = 0; k < 8; ++k) r += N[k] * E[k];
     *rho = r;
   }
@@ -990,67 +992,133 @@ __device__ __forceinline__ void site_interp(PX X, PX E, const double N[8], doubl
 // entry), then the nodes [8][24] and node densities [8][8] of the cluster's
 // own elements (the elements whose nodes meet at the vertex: nearly every
 // union site lies in one of them).
+//
+// A cluster pass is a chain of dependent loads (cluster bounds, slots and
+// union entries, then the node copy, the atom rows and the sites' elements,
+// then the lists), each a full memory latency for a wavefront that has
+// nothing else to do.  The fill is therefore written in stages that issue
+// the loads of one level together (a lane's first three union rows, the node
+// copy and the integration point's list header) before waiting on any.
+
+// One union row's sources, requested before the node copy lands in LDS: the
+// union entry and, for a site, its (element, site) pair (js.x = -1 for an
+// atom, whose x4 row is read when the row is written).
+struct ClRow {
+  uint32_t e;
+  int2 js;
+};
+
+__device__ __forceinline__ void cl_row_src(const ClArgs &c, const ElemArgs &a, int cl, int u, int nu, ClRow &r) {
+  r.e = 0;
+  r.js = make_int2(-1, 0);
+  if (u >= nu) return;
+  r.e = c.uni[(size_t)cl * c.stride + u];
+  const int j = (int)(r.e & 0x7FFFFFFu);
+  if (j >= c.n) r.js = a.asite[j - c.n];
+}
+
+// The row itself: a site is interpolated from its element's nodes (from the
+// LDS copy when the element is one of the cluster's), then shifted by its
+// periodic image.
 template <class P, bool FP, bool LDS>
-__device__ __forceinline__ void cl_fill(const ClArgs &c, int cl, int s0, int ns, const ElemArgs &a, const Prd &prd,
-                                        const Tabs<P> &t, const double *__restrict__ x4,
-                                        const double *__restrict__ erho, double *rows, int cap, int lane) {
-  double *nodes = rows + 4 * cap, *enode = nodes + kClMax * 24;
+__device__ __forceinline__ void cl_row_put(const ClArgs &c, const ElemArgs &a, const Prd &prd, const Tabs<P> &t,
+                                           const double *__restrict__ x4, const double *__restrict__ erho,
+                                           const int (&el)[kClMax], const double *nodes, const double *enode,
+                                           const ClRow &r, double *rows, int u) {
+  double x, y, z, w = 0.0;
+  if (r.js.x < 0) {
+    const int j = (int)(r.e & 0x7FFFFFFu);
+    const double2 xy = reinterpret_cast<const double2 *>(x4)[2 * (size_t)j];
+    const double2 zw = reinterpret_cast<const double2 *>(x4)[2 * (size_t)j + 1];
+    x = xy.x;
+    y = xy.y;
+    z = zw.x;
+    // the force pass reads the atom's F'; the density pass marks atom rows
+    w = FP ? zw.y : 1.0;
+  } else {
+    double N[8];
+    site_shape(a, r.js.y, N);
+    int li = -1;
+#pragma unroll
+    for (int i = 0; i < kClMax; ++i)
+      if (el[i] == r.js.x) li = i;
+    // one branch per memory space (the cluster's own elements from LDS,
+    // any other from global memory): a pointer selected between the two
+    // would be read with flat loads, which take the texture-address path
+    // even for LDS data
+    double rho = 0.0;
+    using gptr = const __attribute__((address_space(1))) double *;
+    if (li < 0)
+      site_interp<FP>((gptr)(a.nodex + 24 * (size_t)r.js.x), (gptr)(FP ? erho + 8 * (size_t)r.js.x : nullptr), N, &x,
+                      &y, &z, &rho);
+    else if (LDS)
+      site_interp<FP>((lds_cptr<double>)(nodes + 24 * li), (lds_cptr<double>)(enode + 8 * li), N, &x, &y, &z, &rho);
+    else
+      site_interp<FP>((gptr)(nodes + 24 * li), (gptr)(enode + 8 * li), N, &x, &y, &z, &rho);
+    if (FP) w = (double)embed_fp<P>(t, rho);
+  }
+  const int img = (int)(r.e >> 27);
+  if (img != kImgSelf) {
+    int s[3];
+    img_shift(img, s);
+    x = shifted(x, s[0], prd.v[0]);
+    y = shifted(y, s[1], prd.v[1]);
+    z = shifted(z, s[2], prd.v[2]);
+  }
+  double *o = rows + 4 * u;
+  o[0] = x;
+  o[1] = y;
+  o[2] = z;
+  o[3] = w;
+}
+
+// Stage 1: the cluster's elements, the lane's first three union rows (most
+// unions have fewer than 192 entries) and the node copy are requested
+// together; the node copy is then stored to LDS.
+struct ClFill {
   int el[kClMax];
+  ClRow r0, r1, r2;
+};
+
+
+template <bool FP>
+__device__ __forceinline__ void cl_fill_issue(const ClArgs &c, int cl, int s0, int ns, int nu, const ElemArgs &a,
+                                              const double *__restrict__ x4, const double *__restrict__ erho,
+                                              double *nodes, double *enode, ClFill &f, int lane) {
 #pragma unroll
-  for (int i = 0; i < kClMax; ++i) el[i] = i < ns ? (c.slot[s0 + i] >> 3) : -1;
-  for (int q = lane; q < ns * 24; q += 64) nodes[q] = a.nodex[24 * (size_t)el[q / 24] + q % 24];
-  if (FP)
-    for (int q = lane; q < ns * 8; q += 64) enode[q] = erho[8 * (size_t)el[q / 8] + q % 8];
-  __syncthreads();
-  const int nu = c.nu[cl];
-  for (int u = lane; u < nu; u += 64) {
-    const uint32_t e = c.uni[(size_t)cl * c.stride + u];
-    const int j = (int)(e & 0x7FFFFFFu);
-    double x, y, z, w = 0.0;
-    if (j < c.n) {
-      const double2 xy = reinterpret_cast<const double2 *>(x4)[2 * (size_t)j];
-      const double2 zw = reinterpret_cast<const double2 *>(x4)[2 * (size_t)j + 1];
-      x = xy.x;
-      y = xy.y;
-      z = zw.x;
-      // the force pass reads the atom's F'; the density pass marks atom rows
-      w = FP ? zw.y : 1.0;
-    } else {
-      const int2 js = a.asite[j - c.n];
-      double N[8];
-      site_shape(a, js.y, N);
-      int li = -1;
+  for (int i = 0; i < kClMax; ++i) f.el[i] = i < ns ? (c.slot[s0 + i] >> 3) : -1;
+  cl_row_src(c, a, cl, lane, nu, f.r0);
+  cl_row_src(c, a, cl, lane + 64, nu, f.r1);
+  cl_row_src(c, a, cl, lane + 128, nu, f.r2);
+  // ns * 24 <= 3 * 64 node coordinates and ns * 8 <= 64 node densities
+  double nv[3], ev = 0.0;
 #pragma unroll
-      for (int i = 0; i < kClMax; ++i)
-        if (el[i] == js.x) li = i;
-      // one branch per memory space (the cluster's own elements from LDS,
-      // any other from global memory): a pointer selected between the two
-      // would be read with flat loads, which take the texture-address path
-      // even for LDS data
-      double rho = 0.0;
-      using gptr = const __attribute__((address_space(1))) double *;
-      if (li < 0)
-        site_interp<FP>((gptr)(a.nodex + 24 * (size_t)js.x), (gptr)(FP ? erho + 8 * (size_t)js.x : nullptr), N, &x, &y,
-                        &z, &rho);
-      else if (LDS)
-        site_interp<FP>((lds_cptr<double>)(nodes + 24 * li), (lds_cptr<double>)(enode + 8 * li), N, &x, &y, &z, &rho);
-      else
-        site_interp<FP>((gptr)(nodes + 24 * li), (gptr)(enode + 8 * li), N, &x, &y, &z, &rho);
-      if (FP) w = (double)embed_fp<P>(t, rho);
-    }
-    const int img = (int)(e >> 27);
-    if (img != kImgSelf) {
-      int s[3];
-      img_shift(img, s);
-      x = shifted(x, s[0], prd.v[0]);
-      y = shifted(y, s[1], prd.v[1]);
-      z = shifted(z, s[2], prd.v[2]);
-    }
-    double *r = rows + 4 * u;
-    r[0] = x;
-    r[1] = y;
-    r[2] = z;
-    r[3] = w;
+  for (int m = 0; m < 3; ++m) {
+    const int q = lane + 64 * m;
+    nv[m] = q < ns * 24 ? a.nodex[24 * (size_t)(c.slot[s0 + q / 24] >> 3) + q % 24] : 0.0;
+  }
+  if (FP && lane < ns * 8) ev = erho[8 * (size_t)(c.slot[s0 + lane / 8] >> 3) + lane % 8];
+#pragma unroll
+  for (int m = 0; m < 3; ++m)
+    if (lane + 64 * m < ns * 24) nodes[lane + 64 * m] = nv[m];
+  if (FP && lane < ns * 8) enode[lane] = ev;
+}
+
+// Stage 2 (after the barrier that publishes the node copy): every row.
+template <class P, bool FP, bool LDS>
+__device__ __forceinline__ void cl_fill_rows(const ClArgs &c, int cl, int nu, const ElemArgs &a, const Prd &prd,
+                                             const Tabs<P> &t, const double *__restrict__ x4,
+                                             const double *__restrict__ erho, double *rows, const double *nodes,
+                                             const double *enode, const ClFill &f, int lane) {
+  for (int u = lane, h = 0; u < nu; u += 64, ++h) {
+    // the prefetched rows field by field (a selected struct would be
+    // copied through scratch memory)
+    ClRow r;
+    r.e = h == 0 ? f.r0.e : h == 1 ? f.r1.e : f.r2.e;
+    r.js.x = h == 0 ? f.r0.js.x : h == 1 ? f.r1.js.x : f.r2.js.x;
+    r.js.y = h == 0 ? f.r0.js.y : h == 1 ? f.r1.js.y : f.r2.js.y;
+    if (h > 2) cl_row_src(c, a, cl, u, nu, r);
+    cl_row_put<P, FP, LDS>(c, a, prd, t, x4, erho, f.el, nodes, enode, r, rows, u);
   }
 }
 
@@ -1059,28 +1127,35 @@ __device__ __forceinline__ void cl_fill(const ClArgs &c, int cl, int s0, int ns,
 template <class P, bool LDS>
 __device__ __forceinline__ void cl_erho_body(const ClArgs &c, const ElemArgs &a, const Prd &prd, const Tabs<P> &t,
                                              const double *__restrict__ px4, double *__restrict__ erho,
-                                             uint32_t *__restrict__ mask, double *rows, int cap, int cl, int lane) {
-  const int s0 = c.start[cl], ns = c.start[cl + 1] - s0;
-  cl_fill<P, false, LDS>(c, cl, s0, ns, a, prd, t, px4, nullptr, rows, cap, lane);
-  __syncthreads();
+                                             uint32_t *__restrict__ mask, double *rows, int cap, int cl, int s0,
+                                             int ns, int nu, int lane) {
+  double *nodes = rows + 4 * cap, *enode = nodes + kClMax * 24;
   const int gi = lane >> 3, l = lane & 7;
+  // the integration point of the lane's group: node k of element i (the
+  // cluster's gi-th slot)
+  const bool live = gi < ns;
+  const int slot = live ? c.slot[s0 + gi] : 0;
+  ClFill f;
+  cl_fill_issue<false>(c, cl, s0, ns, nu, a, px4, nullptr, nodes, enode, f, lane);
+  const int i = slot >> 3, k = slot & 7;
+  const int p = i * a.nint + a.node_list[k];
+  const int jn = live ? a.ipnum[p] : 0;
+  const uint16_t *lrow = c.local + local_slot(p, l, a.M);
+  int qn[kU];
+  cl_indices<kU>(lrow, 0, l, jn, 0xFFFFFFFFu, qn);
+  __syncthreads();
+  cl_fill_rows<P, false, LDS>(c, cl, nu, a, prd, t, px4, nullptr, rows, nodes, enode, f, lane);
+  __syncthreads();
   double acc = 0.0;
-  int slot = 0;
-  if (gi < ns) {
-    slot = c.slot[s0 + gi];
-    const int i = slot >> 3, k = slot & 7;
-    const double ix = a.nodex[24 * (size_t)i + 3 * k], iy = a.nodex[24 * (size_t)i + 3 * k + 1],
-                 iz = a.nodex[24 * (size_t)i + 3 * k + 2];
-    const int p = i * a.nint + a.node_list[k];
-    const int jn = a.ipnum[p];
-    const uint16_t *lrow = c.local + local_slot(p, l, a.M);
+  if (live) {
+    // the node's position from the LDS node copy: the shape-function sums
+    // over its unit shape row give exactly the node's coordinates
+    const double ix = nodes[24 * gi + 3 * k], iy = nodes[24 * gi + 3 * k + 1], iz = nodes[24 * gi + 3 * k + 2];
     const int Wl = (a.M + 8 * 32 - 1) / (8 * 32);
     uint32_t *mrow = mask + ((size_t)p * kLanes + l) * Wl;
     uint32_t bits = 0;
     // software pipeline: the next chunk's union indices are requested
     // before this chunk's spline rows
-    int qn[kU];
-    cl_indices<kU>(lrow, 0, l, jn, 0xFFFFFFFFu, qn);
     for (int k0 = l, j0 = 0; k0 < jn; k0 += kU * kLanes, j0 += kU) {
       int q[kU];
 #pragma unroll
@@ -1123,7 +1198,7 @@ __device__ __forceinline__ void cl_erho_body(const ClArgs &c, const ElemArgs &a,
     }
   }
   acc = group_sum(acc);
-  if (gi < ns && l == 0) erho[slot] = acc;
+  if (live && l == 0) erho[slot] = acc;
 }
 
 template <class P>
@@ -1132,14 +1207,17 @@ __global__ __launch_bounds__(64) void k_cl_erho(ClArgs c, ElemArgs a, Prd prd, T
                                                 uint32_t *__restrict__ mask) {
   extern __shared__ __align__(16) double lds_rows[];
   const int o = xcd_block();
-  if (o >= c.count || *c.skip) return;  // whole workgroup
-  const int cl = c.order[o];
-  const int big = c.big[cl];
+  if (o >= c.count) return;  // whole workgroup
+  // the skip flag and the slot's cluster are requested together
+  const int skip = *c.skip;
+  const int4 m = c.order[o];
+  if (skip) return;
+  const int big = m.w >> 4, ns = m.w & 15;
   if (big >= 0)
-    cl_erho_body<P, false>(c, a, prd, t, px4, erho, mask, c.scratch + (size_t)big * c.sstride, c.srows, cl,
-                    (int)threadIdx.x);
+    cl_erho_body<P, false>(c, a, prd, t, px4, erho, mask, c.scratch + (size_t)big * c.sstride, c.srows, m.x, m.y, ns,
+                           m.z, (int)threadIdx.x);
   else
-    cl_erho_body<P, true>(c, a, prd, t, px4, erho, mask, lds_rows, c.rows, cl, (int)threadIdx.x);
+    cl_erho_body<P, true>(c, a, prd, t, px4, erho, mask, lds_rows, c.rows, m.x, m.y, ns, m.z, (int)threadIdx.x);
 }
 
 template <class P, bool EV, bool LDS>
@@ -1147,39 +1225,40 @@ __device__ __forceinline__ void cl_force_body(const ClArgs &c, const ElemArgs &a
                                               const double *__restrict__ erho, const double *__restrict__ px4,
                                               double *__restrict__ ipf, double *__restrict__ eip,
                                               double *__restrict__ vip, const uint32_t *__restrict__ mask,
-                                              double *__restrict__ nodef, double *rows, int cap, int cl,
-                                              int lane) {
-  const int s0 = c.start[cl], ns = c.start[cl + 1] - s0;
-  cl_fill<P, true, LDS>(c, cl, s0, ns, a, prd, t, px4, erho, rows, cap, lane);
-  __syncthreads();
+                                              double *__restrict__ nodef, double *rows, int cap, int cl, int s0,
+                                              int ns, int nu, int lane) {
+  double *nodes = rows + 4 * cap, *enode = nodes + kClMax * 24;
   const int gi = lane >> 3, l = lane & 7;
-  double fx = 0.0, fy = 0.0, fz = 0.0, ep = 0.0, rhoi = 0.0;
-  P fpi = P(0);
+  // the integration point of the lane's group is node k of element i (the
+  // cluster's gi-th slot, whose nodes and node densities the fill puts in
+  // LDS): the shape-function sums over its unit shape row give exactly the
+  // node's position and density
+  const bool live = gi < ns;
+  const int slot = live ? c.slot[s0 + gi] : 0;
+  ClFill f;
+  cl_fill_issue<true>(c, cl, s0, ns, nu, a, px4, erho, nodes, enode, f, lane);
+  const int i = slot >> 3, k = slot & 7;
+  const int p = i * a.nint + a.node_list[k];
+  const int jn = live ? a.ipnum[p] : 0;
+  const uint16_t *lrow = c.local + local_slot(p, l, a.M);
+  const int Wl = (a.M + 8 * 32 - 1) / (8 * 32);
+  const uint32_t *mrow = mask + ((size_t)p * kLanes + l) * Wl;
+  // software pipeline: the union indices and mask word of the next chunk
+  // are requested before this chunk's spline rows, so they arrive while
+  // this chunk computes
+  uint32_t bits = k0_mask(mrow, jn, l);
+  int q[kU];
+  cl_indices<kU>(lrow, 0, l, jn, bits, q);
+  __syncthreads();
+  const double rhoi = live ? enode[8 * gi + k] : 0.0;
+  const P fpi = live ? embed_fp<P>(t, rhoi) : P(0);
+  cl_fill_rows<P, true, LDS>(c, cl, nu, a, prd, t, px4, erho, rows, nodes, enode, f, lane);
+  __syncthreads();
+  double fx = 0.0, fy = 0.0, fz = 0.0, ep = 0.0;
   Virial vir;
   vir.zero();
-  int p = 0, slot = 0;
-  if (gi < ns) {
-    slot = c.slot[s0 + gi];
-    const int i = slot >> 3, k = slot & 7;
-    p = i * a.nint + a.node_list[k];
-    // the integration point is node k of element i (the cluster's gi-th
-    // slot, whose nodes and node densities cl_fill put in LDS): the
-    // shape-function sums over its unit shape row give exactly the node's
-    // position and density
-    const double *nodes = rows + 4 * cap, *enode = nodes + kClMax * 24;
+  if (live) {
     const double ix = nodes[24 * gi + 3 * k], iy = nodes[24 * gi + 3 * k + 1], iz = nodes[24 * gi + 3 * k + 2];
-    rhoi = enode[8 * gi + k];
-    fpi = embed_fp<P>(t, rhoi);
-    const int jn = a.ipnum[p];
-    const uint16_t *lrow = c.local + local_slot(p, l, a.M);
-    const int Wl = (a.M + 8 * 32 - 1) / (8 * 32);
-    const uint32_t *mrow = mask + ((size_t)p * kLanes + l) * Wl;
-    // software pipeline: the union indices and mask word of the next chunk
-    // are requested before this chunk's spline rows, so they arrive while
-    // this chunk computes
-    uint32_t bits = k0_mask(mrow, jn, l);
-    int q[kU];
-    cl_indices<kU>(lrow, 0, l, jn, bits, q);
     for (int k0 = l, j0 = 0; k0 < jn; k0 += kU * kLanes, j0 += kU) {
       int qc[kU];
 #pragma unroll
@@ -1223,7 +1302,7 @@ __device__ __forceinline__ void cl_force_body(const ClArgs &c, const ElemArgs &a
     ep = group_sum(ep);
     vir.reduce();
   }
-  if (gi < ns && l == 0) {
+  if (live && l == 0) {
     ipf[p] = fx;
     ipf[(size_t)a.nip + p] = fy;
     ipf[2 * (size_t)a.nip + p] = fz;
@@ -1250,15 +1329,17 @@ __global__ __launch_bounds__(64) void k_cl_force(ClArgs c, ElemArgs a, Prd prd, 
                                                  double *__restrict__ nodef) {
   extern __shared__ __align__(16) double lds_rows[];
   const int o = xcd_block();
-  if (o >= c.count || *c.skip) return;  // whole workgroup
-  const int cl = c.order[o];
-  const int big = c.big[cl];
+  if (o >= c.count) return;  // whole workgroup
+  const int skip = *c.skip;
+  const int4 m = c.order[o];
+  if (skip) return;
+  const int big = m.w >> 4, ns = m.w & 15;
   if (big >= 0)
     cl_force_body<P, EV, false>(c, a, prd, t, erho, px4, ipf, eip, vip, mask, nodef,
-                                c.scratch + (size_t)big * c.sstride, c.srows, cl, (int)threadIdx.x);
+                                c.scratch + (size_t)big * c.sstride, c.srows, m.x, m.y, ns, m.z, (int)threadIdx.x);
   else
-    cl_force_body<P, EV, true>(c, a, prd, t, erho, px4, ipf, eip, vip, mask, nodef, lds_rows, c.rows, cl,
-                         (int)threadIdx.x);
+    cl_force_body<P, EV, true>(c, a, prd, t, erho, px4, ipf, eip, vip, mask, nodef, lds_rows, c.rows, m.x, m.y, ns,
+                               m.z, (int)threadIdx.x);
 }
 
 // LJ integration-point force, pair_lj_cut.cpp:355-422.
@@ -1717,10 +1798,8 @@ ClArgs cl_args(const cac_ctx *ctx) {
   c.ncl = ctx->ncl;
   c.n = ctx->n;
   c.stride = ctx->cl_stride;
-  c.start = ctx->cl_start;
   c.slot = ctx->cl_slot;
   c.uni = ctx->cl_union;
-  c.nu = ctx->cl_nu;
   c.local = ctx->iplocal;
   // one launch: LDS rows for the small unions (most of them), so the few
   // large unions at interfaces do not lower the occupancy of all; those keep
@@ -1729,7 +1808,6 @@ ClArgs cl_args(const cac_ctx *ctx) {
   c.order = ctx->cl_order;
   c.count = ctx->ncl;
   c.rows = ctx->cl_umax_small > 0 ? ctx->cl_umax_small : 1;
-  c.big = ctx->cl_big;
   c.scratch = ctx->cl_scratch;
   c.srows = ctx->cl_umax;
   c.sstride = (size_t)4 * ctx->cl_umax + kClMax * (24 + 8);
