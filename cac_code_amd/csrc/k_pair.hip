@@ -930,12 +930,26 @@ struct ClArgs {
   const int *skip;  // nonzero: this step rebuilds first, return at once
 };
 
+// s / d for 0 <= s < 2^16 (an element has fewer than 65536 sites) and
+// d >= 1: a float reciprocal estimate, at these sizes within one of the
+// quotient, and one correction with 24-bit multiplies (full rate; a 32-bit
+// integer division takes about 25 instructions, six of them quarter-rate
+// multiplies)
+__device__ __forceinline__ int site_div(int s, int d) {
+  const int q = (int)((float)s * __builtin_amdgcn_rcpf((float)d));
+  const int r = s - (int)__umul24((unsigned)q, (unsigned)d);
+  return q + (r >= d ? 1 : 0) - (r < 0 ? 1 : 0);
+}
+
 // Shape functions of site s, computed as the host computes the site table
 // (Element::inter_ele + shape_array_inter_ele, element.cpp:1361-1423: the
-// same operations in the same order, so the same bits as spa_site).
+// same operations in the same order, unfused, so the same bits as
+// spa_site).
 __device__ __forceinline__ void site_shape(const ElemArgs &a, int s, double N[8]) {
+#pragma clang fp contract(off)
   const int n12 = a.nae[1] * a.nae[2];
-  const int ii = s / n12, rem = s - ii * n12, jj = rem / a.nae[2], kk = rem - jj * a.nae[2];
+  const int ii = site_div(s, n12), rem = s - (int)__umul24((unsigned)ii, (unsigned)n12);
+  const int jj = site_div(rem, a.nae[2]), kk = rem - (int)__umul24((unsigned)jj, (unsigned)a.nae[2]);
   const double px = a.pstep[0] * (double)ii - 1, py = a.pstep[1] * (double)jj - 1,
                pz = a.pstep[2] * (double)kk - 1;
   N[0] = 0.125 * (1 - px) * (1 - py) * (1 - pz);
