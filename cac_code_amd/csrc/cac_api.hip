@@ -699,7 +699,7 @@ int cac_ctx_destroy(cac_ctx *ctx) {
                     &ctx->b_scantmp, &ctx->b_iflag, &ctx->b_ipos, &ctx->b_iface, &ctx->b_asnum, &ctx->b_atmp,
                     &ctx->b_sbase, &ctx->b_sent, &ctx->b_sent2,
                     &ctx->b_etouch, &ctx->b_rcount, &ctx->b_rstart, &ctx->b_rkey, &ctx->b_rsite,
-                    &ctx->b_relem, &ctx->b_rel_start, &ctx->b_rforce, &ctx->b_rtmp, &ctx->b_rsl, &ctx->b_amask, &ctx->b_ipmask, &ctx->b_cl_start, &ctx->b_cl_slot,
+                    &ctx->b_relem, &ctx->b_rel_start, &ctx->b_rforce, &ctx->b_rtmp, &ctx->b_rsl, &ctx->b_rmeta, &ctx->b_amask, &ctx->b_ipmask, &ctx->b_cl_start, &ctx->b_cl_slot,
                     &ctx->b_cl_union, &ctx->b_cl_nu, &ctx->b_iplocal, &ctx->b_aref, &ctx->b_amark, &ctx->b_cl_order,
                     &ctx->b_ipf, &ctx->b_eip, &ctx->b_vip, &ctx->b_red, &ctx->b_dflags, &ctx->eam.b_rz,
                     &ctx->eam.b_fr, &ctx->eam.b_rz_f, &ctx->eam.b_fr_f, &ctx->eam.b_rv, &ctx->eam.b_rv_f, &ctx->eam.b_rk};

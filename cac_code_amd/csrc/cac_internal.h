@@ -202,9 +202,12 @@ struct cac_ctx {
   long nrev = 0;
   int *rcount = nullptr, *rstart = nullptr, *rsite = nullptr, *relem = nullptr, *rel_start = nullptr;
   int *rsl = nullptr;  // [nrs] site index of each reaction site in its element
+  // [nrs] per reaction site: active site a, rstart[a], rstart[a+1] (one load
+  // where the reaction pass would chain rsite and rstart)
+  int4 *rmeta = nullptr;
   uint32_t *rkey = nullptr;
   double *rforce = nullptr;
-  cac::DevBuf b_rcount, b_rstart, b_rkey, b_rsite, b_relem, b_rel_start, b_rforce, b_rtmp, b_rsl;
+  cac::DevBuf b_rcount, b_rstart, b_rkey, b_rsite, b_relem, b_rel_start, b_rforce, b_rtmp, b_rsl, b_rmeta;
 
   // ---- elements (one type)
   int ne = 0, nae[3] = {0, 0, 0}, nsite = 0;

@@ -2026,14 +2026,18 @@ __global__ void k_rev_keys(long ns, int na, const uint64_t *__restrict__ sorted,
 }
 
 // Elements of the reaction sites (ascending site ids are grouped by
-// element): head flags, then the element list and its ranges in rsite.
+// element): head flags, then the element list and its ranges in rsite; and
+// each reaction site's entry range in rkey.
 __global__ void k_rel_heads(int nrs, const int *__restrict__ rsite, const int2 *__restrict__ asite,
-                            int *__restrict__ head, int *__restrict__ rsl) {
+                            const int *__restrict__ rstart, int *__restrict__ head, int *__restrict__ rsl,
+                            int4 *__restrict__ rmeta) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q > nrs) return;
   if (q == nrs) { head[q] = 0; return; }
-  const int2 js = asite[rsite[q]];
+  const int a = rsite[q];
+  const int2 js = asite[a];
   rsl[q] = js.y;
+  rmeta[q] = make_int4(a, rstart[a], rstart[a + 1], 0);
   head[q] = (q == 0 || js.x != asite[rsite[q - 1]].x) ? 1 : 0;
 }
 
@@ -2091,12 +2095,13 @@ int launch_reaction_lists(cac_ctx *ctx) {
   if ((rc = ensure(ctx, ctx->b_relem, (size_t)nrs, &ctx->relem)) ||
       (rc = ensure(ctx, ctx->b_rel_start, (size_t)nrs + 1, &ctx->rel_start)) ||
       (rc = ensure(ctx, ctx->b_rforce, 3 * (size_t)nrs, &ctx->rforce)) ||
-      (rc = ensure(ctx, ctx->b_rsl, (size_t)nrs, &ctx->rsl)))
+      (rc = ensure(ctx, ctx->b_rsl, (size_t)nrs, &ctx->rsl)) ||
+      (rc = ensure(ctx, ctx->b_rmeta, (size_t)nrs, &ctx->rmeta)))
     return rc;
   // head flags [nrs + 1] in tmpi[0 ..], their exclusive sums after them
   int *head = tmpi, *hpos = tmpi + (nrs + 1);
   hipLaunchKernelGGL(k_rel_heads, dim3(grid_for((long)nrs + 1)), dim3(kBlock), 0, ctx->stream, nrs, ctx->rsite,
-                     ctx->asite, head, ctx->rsl);
+                     ctx->asite, ctx->rstart, head, ctx->rsl, ctx->rmeta);
   tmp_bytes = 0;
   CAC_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, head, hpos, nrs + 1, ctx->stream));
   if ((rc = ensure(ctx, ctx->b_scantmp, tmp_bytes, &tmp))) return rc;

@@ -1453,7 +1453,7 @@ __global__ void k_scatter(ElemArgs a, const double *__restrict__ ipf, double *__
 // coefficients the atom pass's knot form reproduces bit for bit).
 struct ReactArgs {
   int nrs, n;
-  const int *rsite, *rstart;
+  const int4 *meta;  // [nrs] active site a, rkey range [rstart[a], rstart[a+1])
   const uint32_t *rkey;
   double *R;  // [3][nrs]
   const int *skip;
@@ -1468,17 +1468,19 @@ __global__ __launch_bounds__(kBlock) void k_site_react_eam(ReactArgs r, const do
                                                            Tabs<P> t) {
   const int gt = xcd_block() * kBlock + (int)threadIdx.x;
   const int q = gt >> 3, l = gt & 7;
-  if (*r.skip) return;
   const bool live = q < r.nrs;
+  // the skip flag and the site's entry range are requested together
+  const int skip = *r.skip;
+  const int4 m = live ? r.meta[q] : make_int4(0, 0, 0, 0);
+  if (skip) return;
   double rx = 0.0, ry = 0.0, rz = 0.0;
   if (live) {
-    const int a = r.rsite[q];
-    const size_t row = (size_t)r.n + a;
+    const size_t row = (size_t)r.n + m.x;
     const double2 sxy = reinterpret_cast<const double2 *>(x4)[2 * row];
     const double2 szw = reinterpret_cast<const double2 *>(x4)[2 * row + 1];
     const P fps = (P)szw.y;
-    const int k1 = r.rstart[a + 1];
-    for (int k0 = r.rstart[a] + l; k0 < k1; k0 += kU * kLanes) {
+    const int k1 = m.z;
+    for (int k0 = m.y + l; k0 < k1; k0 += kU * kLanes) {
       uint32_t key[kU];
 #pragma unroll
       for (int u = 0; u < kU; ++u) key[u] = k0 + u * kLanes < k1 ? r.rkey[k0 + u * kLanes] : kNoEntry;
@@ -1533,10 +1535,10 @@ __global__ __launch_bounds__(kBlock) void k_site_react_lj(ReactArgs r, const dou
   const P lj1 = (P)lj.lj1, lj2 = (P)lj.lj2;
   double rx = 0.0, ry = 0.0, rz = 0.0;
   if (live) {
-    const int a = r.rsite[q];
-    const size_t row = (size_t)r.n + a;
+    const int4 m = r.meta[q];
+    const size_t row = (size_t)r.n + m.x;
     const double sx0 = x4[4 * row], sy0 = x4[4 * row + 1], sz0 = x4[4 * row + 2];
-    for (int k = r.rstart[a] + l; k < r.rstart[a + 1]; k += kLanes) {
+    for (int k = m.y + l; k < m.z; k += kLanes) {
       const uint32_t key = r.rkey[k];
       const size_t i = key & 0x7FFFFFFu;
       const int img = (int)(key >> 27);
@@ -1576,25 +1578,48 @@ __global__ __launch_bounds__(kBlock) void k_site_react_lj(ReactArgs r, const dou
 // (24 partial sums); a fixed xor butterfly combines the lanes (every lane ends
 // with the same bits), and lanes 0..23 add node k = l/3, component l%3 to the
 // node force written by the integration-point pass.  rsl[q]: the site index
-// of reaction site q in its element.
+// of reaction site q in its element.  The site indices and reactions of four
+// steps are requested before the first is summed (an element has a few
+// hundred reaction sites, so this is the whole range), in the same order.
 __global__ __launch_bounds__(64) void k_site_react_nodes(ReactArgs r, int nre, const int *__restrict__ relem,
                                                          const int *__restrict__ rel_start,
                                                          const int *__restrict__ rsl, ElemArgs a,
                                                          double *__restrict__ nodef) {
   const int ri = xcd_block();
-  if (ri >= nre || *r.skip) return;  // whole wavefront
+  if (ri >= nre) return;  // whole wavefront
+  const int skip = *r.skip;
+  const int q0 = rel_start[ri], q1 = rel_start[ri + 1], e = relem[ri];
+  if (skip) return;
   const int lane = (int)threadIdx.x;
   double acc[24];
 #pragma unroll
   for (int c = 0; c < 24; ++c) acc[c] = 0.0;
-  for (int q = rel_start[ri] + lane; q < rel_start[ri + 1]; q += 64) {
-    double N[8];
-    site_shape(a, rsl[q], N);
-    const double R[3] = {r.R[q], r.R[(size_t)r.nrs + q], r.R[2 * (size_t)r.nrs + q]};
+  constexpr int kSteps = 4;
+  for (int qb = q0 + lane; qb < q1; qb += kSteps * 64) {
+    int si[kSteps];
+    double R[kSteps][3];
 #pragma unroll
-    for (int k = 0; k < 8; ++k)
+    for (int h = 0; h < kSteps; ++h) {
+      const int q = qb + 64 * h;
+      si[h] = 0;
+      R[h][0] = R[h][1] = R[h][2] = 0.0;
+      if (q < q1) {
+        si[h] = rsl[q];
+        R[h][0] = r.R[q];
+        R[h][1] = r.R[(size_t)r.nrs + q];
+        R[h][2] = r.R[2 * (size_t)r.nrs + q];
+      }
+    }
 #pragma unroll
-      for (int d = 0; d < 3; ++d) acc[3 * k + d] += N[k] * R[d];
+    for (int h = 0; h < kSteps; ++h) {
+      if (qb + 64 * h >= q1) break;
+      double N[8];
+      site_shape(a, si[h], N);
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) acc[3 * k + d] += N[k] * R[h][d];
+    }
   }
 #pragma unroll
   for (int c = 0; c < 24; ++c)
@@ -1604,7 +1629,7 @@ __global__ __launch_bounds__(64) void k_site_react_nodes(ReactArgs r, int nre, c
 #pragma unroll
   for (int c = 0; c < 24; ++c)
     if (lane == c) mine = acc[c];
-  if (lane < 24) nodef[24 * (size_t)relem[ri] + lane] += mine;
+  if (lane < 24) nodef[24 * (size_t)e + lane] += mine;
 }
 
 inline unsigned grid_for(long n, int block = kBlock) { return (unsigned)((n + block - 1) / block); }
@@ -1715,8 +1740,7 @@ int site_reactions(cac_ctx *ctx, bool eam, const ElemArgs &a) {
   ReactArgs r;
   r.nrs = ctx->nrs;
   r.n = ctx->n;
-  r.rsite = ctx->rsite;
-  r.rstart = ctx->rstart;
+  r.meta = ctx->rmeta;
   r.rkey = ctx->rkey;
   r.R = ctx->rforce;
   r.skip = ctx->dflags + 15;
