@@ -1257,12 +1257,12 @@ __device__ __forceinline__ void cl_force_body(const ClArgs &c, const ElemArgs &a
   const uint16_t *lrow = c.local + local_slot(p, l, a.M);
   const int Wl = (a.M + 8 * 32 - 1) / (8 * 32);
   const uint32_t *mrow = mask + ((size_t)p * kLanes + l) * Wl;
-  // software pipeline: the union indices and mask word of the next chunk
-  // are requested before this chunk's spline rows, so they arrive while
-  // this chunk computes
+  // the lane's first three runs of union indices (steps 0..11) and its
+  // first mask word
+  const int steps = jn > l ? (jn - l + kLanes - 1) / kLanes : 0;
   uint32_t bits = k0_mask(mrow, jn, l);
-  int q[kU];
-  cl_indices<kU>(lrow, 0, l, jn, bits, q);
+  const uint64_t *runs = reinterpret_cast<const uint64_t *>(lrow);
+  const uint64_t w0 = steps > 0 ? runs[0] : 0, w1 = steps > 4 ? runs[8] : 0, w2 = steps > 8 ? runs[16] : 0;
   __syncthreads();
   const double rhoi = live ? enode[8 * gi + k] : 0.0;
   const P fpi = live ? embed_fp<P>(t, rhoi) : P(0);
@@ -1273,15 +1273,8 @@ __device__ __forceinline__ void cl_force_body(const ClArgs &c, const ElemArgs &a
   vir.zero();
   if (live) {
     const double ix = nodes[24 * gi + 3 * k], iy = nodes[24 * gi + 3 * k + 1], iz = nodes[24 * gi + 3 * k + 2];
-    for (int k0 = l, j0 = 0; k0 < jn; k0 += kU * kLanes, j0 += kU) {
-      int qc[kU];
-#pragma unroll
-      for (int u = 0; u < kU; ++u) qc[u] = q[u];
-      {
-        const int j1 = j0 + kU, k1 = k0 + kU * kLanes;
-        if ((j1 & 31) == 0 && k1 < jn) bits = mrow[j1 >> 5];
-        cl_indices<kU>(lrow, j1, k1, jn, bits >> (j1 & 31), q);
-      }
+    // the pair terms of up to kU union rows (-1: none), in order
+    auto pairs = [&](const int (&qc)[kU]) {
 #pragma unroll
       for (int u = 0; u < kU; ++u) {
         if (qc[u] < 0) continue;
@@ -1306,6 +1299,42 @@ __device__ __forceinline__ void cl_force_body(const ClArgs &c, const ElemArgs &a
             vir.add(delx, dely, delz, fpair);
           }
         }
+      }
+    };
+    if (steps <= 12) {
+      // the whole walk is in registers: only the kept entries (the mask's
+      // set bits: in range, element sites), four at a time in step order,
+      // so the sums run in the order of the full walk without its skipped
+      // entries and in fewer rounds
+      while (bits) {
+        int qc[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+          qc[u] = -1;
+          if (bits) {
+            const int s = __builtin_ctz(bits);
+            bits &= bits - 1;
+            const uint64_t w = s < 4 ? w0 : s < 8 ? w1 : w2;
+            qc[u] = (int)((w >> (16 * (s & 3))) & 0xFFFFu);
+          }
+        }
+        pairs(qc);
+      }
+    } else {
+      // a longer walk: every step, the skipped entries masked out; the union
+      // indices and mask word of the next chunk are requested before this
+      // chunk's spline rows, so they arrive while this chunk computes
+      int q[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) q[u] = ((bits >> u) & 1u) ? (int)((w0 >> (16 * u)) & 0xFFFFu) : -1;
+      for (int k0 = l, j0 = 0; k0 < jn; k0 += kU * kLanes, j0 += kU) {
+        int qc[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) qc[u] = q[u];
+        const int j1 = j0 + kU, k1 = k0 + kU * kLanes;
+        if ((j1 & 31) == 0 && k1 < jn) bits = mrow[j1 >> 5];
+        cl_indices<kU>(lrow, j1, k1, jn, bits >> (j1 & 31), q);
+        pairs(qc);
       }
     }
   }

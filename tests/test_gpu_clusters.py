@@ -31,14 +31,14 @@ def eam_pair(cu_funcfl):
     return p
 
 
-def forces(s, pair, clusters, coupled, precision, ev):
+def forces(s, pair, clusters, coupled, precision, ev, cutneigh=CUTNEIGH_EAM):
     eng = Engine(0)
     try:
         eng.load_system(s)
         if not clusters:
             eng.set_ip_clusters(np.zeros(1, np.int32), np.zeros(0, np.int32))
         eng.set_pair(pair)
-        eng.set_neighbor(CUTNEIGH_EAM, every=1, delay=0, check=True)
+        eng.set_neighbor(cutneigh, every=1, delay=0, check=True)
         eng.set_coupling(coupled)
         eng.build()
         ncl, active, umax = eng.cluster_info()
@@ -69,6 +69,28 @@ def test_clusters_equal_per_point_passes(name, coupled, precision, cu_funcfl):
     ev = precision == _lib.PREC_FP64
     a = forces(s, pair, True, coupled, precision, ev)
     b = forces(s, pair, False, coupled, precision, ev)
+    names = ["f", "nodef", "erho", "ipf", "energy", "virial", "x", "v", "f3", "nodex", "nodev", "nodef3"]
+    names = names if len(a) == len(names) else names[:4] + names[6:]
+    diff = {n: rel_err(qa, qb) for n, qa, qb in zip(names, a, b)}
+    print("  max |cluster - per point| / max |per point|:", {k: f"{v:.1e}" for k, v in diff.items()}, flush=True)
+    bad = [k for k, v in diff.items() if not v <= TOL]
+    assert not bad, f"differ: {bad}"
+
+
+@pytest.mark.parametrize("precision", [_lib.PREC_FP64, _lib.PREC_MIXED], ids=["fp64", "mixed"])
+@pytest.mark.parametrize("coupled", [False, True], ids=["plain", "coupled"])
+def test_clusters_long_lists_equal_per_point_passes(coupled, precision, cu_funcfl):
+    """A 7.5 A list cutoff gives integration-point lists of about 150 entries,
+    more than 12 steps per lane: the cluster force pass then walks every step
+    with the skipped entries masked, instead of the kept entries from
+    registers that the 5.95 A lists use."""
+    s = SYSTEMS["small"]()
+    print(f"[clusters] long lists coupled={coupled} prec={precision}: atoms={s.n_atoms} elements={s.n_elements}",
+          flush=True)
+    pair = eam_pair(cu_funcfl)
+    ev = precision == _lib.PREC_FP64
+    a = forces(s, pair, True, coupled, precision, ev, cutneigh=7.5)
+    b = forces(s, pair, False, coupled, precision, ev, cutneigh=7.5)
     names = ["f", "nodef", "erho", "ipf", "energy", "virial", "x", "v", "f3", "nodex", "nodev", "nodef3"]
     names = names if len(a) == len(names) else names[:4] + names[6:]
     diff = {n: rel_err(qa, qb) for n, qa, qb in zip(names, a, b)}
