@@ -1155,8 +1155,14 @@ __device__ __forceinline__ void cl_erho_body(const ClArgs &c, const ElemArgs &a,
   const int p = i * a.nint + a.node_list[k];
   const int jn = live ? a.ipnum[p] : 0;
   const uint16_t *lrow = c.local + local_slot(p, l, a.M);
+  // the lane's first three runs of union indices (its steps 0..11): a walk
+  // of at most 12 steps needs no index loads in its loop
+  const int steps = jn > l ? (jn - l + kLanes - 1) / kLanes : 0;
+  const uint64_t *runs = reinterpret_cast<const uint64_t *>(lrow);
+  const uint64_t w0 = steps > 0 ? runs[0] : 0, w1 = steps > 4 ? runs[8] : 0, w2 = steps > 8 ? runs[16] : 0;
   int qn[kU];
-  cl_indices<kU>(lrow, 0, l, jn, 0xFFFFFFFFu, qn);
+#pragma unroll
+  for (int u = 0; u < kU; ++u) qn[u] = u < steps ? (int)((w0 >> (16 * u)) & 0xFFFFu) : -1;
   __syncthreads();
   cl_fill_rows<P, false, LDS>(c, cl, nu, a, prd, t, px4, nullptr, rows, nodes, enode, f, lane);
   __syncthreads();
@@ -1174,7 +1180,14 @@ __device__ __forceinline__ void cl_erho_body(const ClArgs &c, const ElemArgs &a,
       int q[kU];
 #pragma unroll
       for (int u = 0; u < kU; ++u) q[u] = qn[u];
-      cl_indices<kU>(lrow, j0 + kU, k0 + kU * kLanes, jn, 0xFFFFFFFFu, qn);
+      if (steps <= 12) {
+        const int j1 = j0 + kU;
+        const uint64_t w = j1 == 4 ? w1 : w2;
+#pragma unroll
+        for (int u = 0; u < kU; ++u) qn[u] = j1 + u < steps ? (int)((w >> (16 * u)) & 0xFFFFu) : -1;
+      } else {
+        cl_indices<kU>(lrow, j0 + kU, k0 + kU * kLanes, jn, 0xFFFFFFFFu, qn);
+      }
       // distances and knots, then the rows (all in flight), then the sums
       P pp[kU];
       int mm[kU];
