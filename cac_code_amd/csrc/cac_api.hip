@@ -689,7 +689,7 @@ int cac_ctx_destroy(cac_ctx *ctx) {
   mlp_destroy(ctx);
   surrogate_destroy(ctx);
   DevBuf *bufs[] = {&ctx->b_x4, &ctx->b_v, &ctx->b_f, &ctx->b_xhold, &ctx->b_rho, &ctx->b_eatom, &ctx->b_vatom,
-                    &ctx->b_aneigh, &ctx->b_anum, &ctx->b_akey, &ctx->b_aval, &ctx->b_akey2, &ctx->b_aval2,
+                    &ctx->b_aneigh, &ctx->b_anum, &ctx->b_crank, &ctx->b_akey, &ctx->b_aval, &ctx->b_akey2, &ctx->b_aval2,
                     &ctx->b_ahead, &ctx->b_acells, &ctx->b_cl_scratch, &ctx->b_centro, &ctx->b_sitex, &ctx->b_acs, &ctx->b_ace, &ctx->b_sorttmp, &ctx->b_nodex,
                     &ctx->b_nodev, &ctx->b_nodef, &ctx->b_amask_in, &ctx->b_xhold_in,
                     &ctx->b_nodexhold, &ctx->b_xc, &ctx->b_spa_site, &ctx->b_spa_int, &ctx->b_weight,

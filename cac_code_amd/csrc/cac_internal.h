@@ -146,13 +146,20 @@ struct cac_ctx {
   cac::DevBuf b_x4, b_v, b_f, b_xhold, b_rho, b_eatom, b_vatom;
   // atom neighbor list, blocked: entry k of atom i at atom_slot(i, k, maxneigh).
   // Each row is ordered by its displacement vectors rounded to list_spacing
-  // (lattice order; by entry value when 0), so on a lattice every atom lists
-  // its neighbors in the same relative order.
+  // (shell by shell, then lattice order; by entry value when 0), so on a
+  // lattice every atom lists its neighbors in the same relative order, the
+  // nearest shells first.
   int maxneigh = 128;
   double list_spacing = 0.0;
   uint32_t *aneigh = nullptr;
   int *anum = nullptr;
   cac::DevBuf b_aneigh, b_anum;
+  // rank of each rounded displacement of the build's code box in list-key
+  // order (box half-width crank_qk; the table is made when it changes)
+  int16_t *crank = nullptr;
+  int crank_qk = 0;
+  std::vector<int16_t> crank_host;
+  cac::DevBuf b_crank;
   // in-range bits of each atom row (r^2 < cutforcesq), written by the
   // density pass of every EAM force call for its force pass
   uint32_t *amask = nullptr;

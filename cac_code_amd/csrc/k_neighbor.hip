@@ -10,6 +10,8 @@
 // compared with <= (neighbor.cpp:609, full_node_neighbor.cpp:167,216).
 // Search pruning (cells, element radii, element hulls) is conservative by
 // construction, so the sets equal the brute-force sets.
+#include <algorithm>
+
 #include <hipcub/hipcub.hpp>
 
 #include "cac_internal.h"
@@ -130,22 +132,30 @@ __global__ void k_cell_bounds(int n, const uint32_t *__restrict__ key, int *__re
 // atoms in turn, tests the staged candidates 64 at a time and compacts the
 // accepted entries into LDS.  Each entry is written straight to its rank in
 // the row (rank = number of smaller keys), i.e. the row sorted by a key that
-// puts the lattice-rounded displacement x_j' - x_i (components rounded to
-// multiples of `spacing`) before the entry value, in the atom_slot layout.
-// On a lattice of that spacing every atom lists its neighbors in the same
-// relative order, so at step k the 64 atoms of a force-pass wavefront gather
-// neighboring rows of x4 instead of 64 unrelated ones.  spacing = 0 sorts by
-// entry value.  Keys are unique (they end in the entry), so the row is
-// independent of the candidate order.  A cell with more candidates than the
-// LDS holds reads them from global memory instead.
+// puts the lattice-rounded displacement k = rint((x_j' - x_i) / spacing)
+// before the entry value, in the atom_slot layout: ordered by |k|^2 first,
+// then k_z, k_y, k_x.  On a lattice of that spacing every atom lists its
+// neighbors in the same relative order, so at step k the 64 atoms of a
+// force-pass wavefront gather neighboring rows of x4 instead of 64
+// unrelated ones; and the rows run shell by shell, so the entries within
+// the force cutoff fill the front of every row and the pair passes stop at
+// the last one.  spacing = 0 sorts by entry value.  Keys are unique (they
+// end in the entry), so the row is independent of the candidate order.  A
+// cell with more candidates than the LDS holds reads them from global memory
+// instead.  High word: min(|k|^2, 255) << 24 | (k_z + 128) << 16 |
+// (k_y + 128) << 8 | (k_x + 128), components clamped to [-128, 127].
+__host__ __device__ __forceinline__ uint32_t list_key_hi(int kx, int ky, int kz) {
+  const int k2 = kx * kx + ky * ky + kz * kz;
+  return ((uint32_t)(k2 < 255 ? k2 : 255) << 24) | ((uint32_t)(kz + 128) << 16) | ((uint32_t)(ky + 128) << 8) |
+         (uint32_t)(kx + 128);
+}
 __device__ __forceinline__ uint64_t list_key(double dx, double dy, double dz, double qinv, uint32_t entry) {
   if (qinv <= 0.0) return entry;
   auto q = [&](double d) {
-    int k = (int)rint(d * qinv);
-    k = k < -512 ? -512 : (k > 511 ? 511 : k);
-    return (uint64_t)(k + 512);
+    const int k = (int)rint(d * qinv);
+    return k < -128 ? -128 : (k > 127 ? 127 : k);
   };
-  return (((q(dz) << 20) | (q(dy) << 10) | q(dx)) << 32) | entry;
+  return ((uint64_t)list_key_hi(q(dx), q(dy), q(dz)) << 32) | entry;
 }
 
 // LDS written by some lanes of a wavefront is read by others after this.
@@ -220,20 +230,21 @@ constexpr int kNeighWaves = 4;
 constexpr int kCodeWords = 64;  // presence bitmap of the lattice-rounded displacements, per wavefront
 
 // Dense code of a list key's lattice-rounded displacement (k_x, k_y, k_z),
-// each in [-qk, qk]: ((k_z + qk) * D + k_y + qk) * D + k_x + qk with
-// D = 2 qk + 1, which orders codes as the keys' high words order them; -1
-// outside that box.
-__device__ __forceinline__ int list_code(uint64_t key, int qk, int D) {
+// each in [-qk, qk]: the rank crank[((k_z + qk) * D + k_y + qk) * D + k_x +
+// qk] (D = 2 qk + 1) of its high word among the box's high words, so codes
+// order as the keys' high words order them; -1 outside that box.
+__device__ __forceinline__ int list_code(uint64_t key, int qk, int D, const int16_t *__restrict__ crank) {
   const uint32_t hi = (uint32_t)(key >> 32);
-  const int kx = (int)(hi & 1023u) - 512, ky = (int)((hi >> 10) & 1023u) - 512, kz = (int)(hi >> 20) - 512;
+  const int kx = (int)(hi & 255u) - 128, ky = (int)((hi >> 8) & 255u) - 128, kz = (int)((hi >> 16) & 255u) - 128;
   if (kx < -qk || kx > qk || ky < -qk || ky > qk || kz < -qk || kz > qk) return -1;
-  return ((kz + qk) * D + (ky + qk)) * D + (kx + qk);
+  return crank[((kz + qk) * D + (ky + qk)) * D + (kx + qk)];
 }
 
 __global__ __launch_bounds__(64 * kNeighWaves) void k_atom_neigh(
     int ncells, const int *__restrict__ cells, const double *__restrict__ x4, Box3 b, Grid3 g,
     const uint32_t *__restrict__ sorted, const int *__restrict__ cs, const int *__restrict__ ce, double cutsq,
-    int maxneigh, int cap2, int ccap, double qinv, int qk, int n, uint32_t *__restrict__ neigh,
+    int maxneigh, int cap2, int ccap, double qinv, int qk, const int16_t *__restrict__ crank, int n,
+    uint32_t *__restrict__ neigh,
     int *__restrict__ num, double *__restrict__ xhold, int *__restrict__ flags) {
   // LDS: keys[kNeighWaves][cap2] (u64), then sx, sy, sz [ccap] (f64), se [ccap] (u32)
   extern __shared__ uint64_t smem[];
@@ -352,7 +363,7 @@ __global__ __launch_bounds__(64 * kNeighWaves) void k_atom_neigh(
       wave_sync();
       bool clash = false;
       for (int a = lane; a < kept; a += 64) {
-        const int code = list_code(keys[a], qk, D);
+        const int code = list_code(keys[a], qk, D, crank);
         if (code < 0) {
           clash = true;
         } else {
@@ -374,7 +385,7 @@ __global__ __launch_bounds__(64 * kNeighWaves) void k_atom_neigh(
       wave_sync();
       for (int a = lane; a < kept; a += 64) {
         const uint64_t ka = keys[a];
-        const int code = list_code(ka, qk, D);
+        const int code = list_code(ka, qk, D, crank);
         const int r = cpre[code >> 5] + __popc(cbits[code >> 5] & ((1u << (code & 31)) - 1u));
         if (r < maxneigh) neigh[atom_slot(i, r, maxneigh)] = (uint32_t)ka;
       }
@@ -1541,9 +1552,27 @@ int launch_atom_neighbors(cac_ctx *ctx) {
     if (kmax <= 16.0 && (2.0 * kmax + 1.0) * (2.0 * kmax + 1.0) * (2.0 * kmax + 1.0) <= 32.0 * kCodeWords)
       qk = (int)kmax;
   }
+  if (qk > 0 && ctx->crank_qk != qk) {
+    // the codes' ranks in key order (a table per box size, made once)
+    const int D = 2 * qk + 1, nc = D * D * D;
+    std::vector<uint32_t> hiw(nc);
+    std::vector<int> idx(nc);
+    for (int c = 0; c < nc; ++c) {
+      hiw[c] = list_key_hi(c % D - qk, (c / D) % D - qk, c / (D * D) - qk);
+      idx[c] = c;
+    }
+    std::sort(idx.begin(), idx.end(), [&](int u, int v) { return hiw[u] < hiw[v]; });
+    ctx->crank_host.assign(nc, 0);
+    for (int r = 0; r < nc; ++r) ctx->crank_host[idx[r]] = (int16_t)r;
+    if ((rc = ensure(ctx, ctx->b_crank, (size_t)nc, &ctx->crank))) return rc;
+    CAC_HIP(ctx, hipMemcpyAsync(ctx->crank, ctx->crank_host.data(), (size_t)nc * sizeof(int16_t),
+                                hipMemcpyHostToDevice, ctx->stream));
+    CAC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->crank_qk = qk;
+  }
   hipLaunchKernelGGL(k_atom_neigh, dim3(xcd_grid(ncells, 1)), dim3(64 * kNeighWaves), lds, ctx->stream, ncells, cells,
                      ctx->x4, box_of(ctx), g, sorted, cs, ce, ctx->cutneigh * ctx->cutneigh, ctx->maxneigh, cap2, ccap,
-                     qinv, qk, n, ctx->aneigh, ctx->anum, ctx->xhold, ctx->dflags);
+                     qinv, qk, qk > 0 ? ctx->crank : nullptr, n, ctx->aneigh, ctx->anum, ctx->xhold, ctx->dflags);
   timer_end(ctx, "atom_neigh", t0);
   CAC_HIP(ctx, hipGetLastError());
   return CAC_OK;

@@ -406,6 +406,21 @@ __host__ __device__ __forceinline__ size_t mask_slot(int i, int w, int W) {
   return ((size_t)(i >> 6) * W + w) * kAtomGroup + (i & 63);
 }
 
+// One past the last set bit of atom i's mask words over its jn entries (0:
+// none).  The rows run shell by shell (k_atom_neigh's key), so the entries
+// a pass needs fill the front of the row and the pass stops here instead
+// of walking the skin shells to the end.
+__device__ __forceinline__ int mask_end(const uint32_t *__restrict__ m, int i, int W, int jn) {
+  int end = 0;
+  const int nw = (jn + 31) >> 5;
+#pragma unroll 4
+  for (int w = 0; w < nw; ++w) {
+    const uint32_t v = m[mask_slot(i, w, W)];
+    if (v) end = 32 * w + 32 - __clz(v);
+  }
+  return end;
+}
+
 // U entries from k0 (a multiple of 4): one 16-byte load per run of 4 (the
 // run lies inside the row, whose length is a multiple of 8; entries past jn
 // are dropped)
@@ -439,11 +454,12 @@ __device__ __forceinline__ void fetch_atom_entries(const uint32_t *__restrict__ 
 //
 // Inner mask (a second, shorter Verlet list inside the neighbor list): a
 // full density pass (*ibad != 0) gathers every entry and also records the
-// entries with r^2 < cutinsq = (cutforce + margin)^2, and the positions
-// (xhold_in).  While no atom has moved more than margin/2 since, no atom pair
-// outside that set can be within cutforce, so the next density passes
-// (*ibad == 0) gather only the inner entries (and every element site, which
-// moves with its nodes): the same pairs in the same order, so the same bits.
+// entries with r^2 < cutinsq = (cutforce + margin)^2 and every element site
+// (which moves with its nodes), and the positions (xhold_in).  While no atom
+// has moved more than margin/2 since, no atom pair outside that set can be
+// within cutforce, so the next density passes (*ibad == 0) gather only the
+// entries of that set: the same pairs in the same order, so the same bits.
+// Both passes stop after the last entry their mask keeps (mask_end).
 //
 // The table window fills a CU's LDS, so one workgroup runs per CU at a time.
 // A large system runs one thread per atom (many rounds of workgroups; the
@@ -497,7 +513,8 @@ __global__ __launch_bounds__(kAtomWG) void k_eam_atom_rho(int n, double *__restr
     uint32_t e[kUA];
     fetch_atom_entries<kUA>(row, 0, jn, e);
     uint32_t in_w = (inner && jn > 0) ? mask_in[mask_slot(i, 0, W)] : 0u;
-    for (int k0 = 0; k0 < jn; k0 += kUA) {
+    const int kend = inner ? min(jn, mask_end(mask_in, i, W, jn)) : jn;
+    for (int k0 = 0; k0 < kend; k0 += kUA) {
       if (inner && (k0 & 31) == 0) bits_in = in_w;
       if (inner) {  // element sites (rows >= n, moved by their nodes) are always gathered
         const uint32_t sel = bits_in >> (k0 & 31);
@@ -523,9 +540,9 @@ __global__ __launch_bounds__(kAtomWG) void k_eam_atom_rho(int n, double *__restr
           rho += (double)rho_of_r_w<P>(t, w, sqrt_p<P>(rsq));
           bits |= 1u << ((k0 + u) & 31);
         }
-        if (!inner && rsq < cutinsq) bits_in |= 1u << ((k0 + u) & 31);
+        if (!inner && (rsq < cutinsq || (e[u] & 0x7FFFFFFu) >= (uint32_t)n)) bits_in |= 1u << ((k0 + u) & 31);
       }
-      if (((k0 + kUA) & 31) == 0 || k0 + kUA >= jn) {
+      if (((k0 + kUA) & 31) == 0 || k0 + kUA >= kend) {
         mask[mask_slot(i, k0 >> 5, W)] = bits;
         if (!inner) mask_in[mask_slot(i, k0 >> 5, W)] = bits_in;
         bits = 0;
@@ -534,6 +551,8 @@ __global__ __launch_bounds__(kAtomWG) void k_eam_atom_rho(int n, double *__restr
   #pragma unroll
       for (int u = 0; u < kUA; ++u) e[u] = en[u];
     }
+    // the words past the walk keep no entry
+    for (int wd = (kend + 31) >> 5; wd < ((jn + 31) >> 5); ++wd) mask[mask_slot(i, wd, W)] = 0u;
     if (!inner) {
       xhold_in[i] = xi;
       xhold_in[n + i] = yi;
@@ -582,7 +601,8 @@ __global__ __launch_bounds__(kAtomWG) void k_eam_atom_force(int n, const double 
     uint32_t e[kUA];
     fetch_atom_entries<kUA>(row, 0, jn, e);
     uint32_t bits = jn > 0 ? mask[mask_slot(i, 0, W)] : 0u;
-    for (int k0 = 0; k0 < jn; k0 += kUA) {
+    const int kend = min(jn, mask_end(mask, i, W, jn));
+    for (int k0 = 0; k0 < kend; k0 += kUA) {
       const uint32_t sel = bits >> (k0 & 31);
   #pragma unroll
       for (int u = 0; u < kUA; ++u)

@@ -1,8 +1,11 @@
 """Order of the atom lists (k_atom_neigh): every row is sorted by the key
-(lattice-rounded displacement x_j' - x_i, components rounded to multiples of
-the list spacing, ordered z, y, x; then the entry j | img << 27).  The key is
-recomputed on the host from the downloaded positions and each row must be
-strictly increasing in it.
+(lattice-rounded displacement k = rint((x_j' - x_i) / spacing), components
+clamped to [-128, 127], ordered by min(|k|^2, 255), then k_z, k_y, k_x; then
+the entry j | img << 27).  The key is recomputed on the host from the
+downloaded positions and each row must be strictly increasing in it.  On an
+fcc lattice at a/2 the rows therefore run shell by shell: the test also
+checks that the first three shells (the entries within the EAM cutoff)
+fill the front of every row of a perfect lattice.
 
 Rows whose rounded displacements are all distinct (an fcc lattice at a/2) are
 ranked through a presence bitmap of the codes; rows where two neighbors share
@@ -26,7 +29,7 @@ pytestmark = pytest.mark.gpu
 def host_keys(x, lo, hi, num, nb, spacing):
     """The device's list key of every stored entry, from the same positions
     and the same operations (x + s*prd for an image, rint of d * (1/spacing)
-    clamped to [-512, 511])."""
+    clamped to [-128, 127], |k|^2 capped at 255 in front)."""
     prd = hi - lo
     keys = []
     qinv = 1.0 / spacing if spacing > 0.0 else 0.0
@@ -42,8 +45,10 @@ def host_keys(x, lo, hi, num, nb, spacing):
         for d in range(3):
             m = s[:, d] != 0
             xj[m, d] = xj[m, d] + s[m, d].astype(np.float64) * prd[d]
-        q = np.clip(np.rint((xj - x[i]) * qinv), -512, 511).astype(np.int64) + 512
-        hiw = (q[:, 2] << 20) | (q[:, 1] << 10) | q[:, 0]
+        k = np.clip(np.rint((xj - x[i]) * qinv), -128, 127).astype(np.int64)
+        k2 = np.minimum((k * k).sum(axis=1), 255)
+        q = k + 128
+        hiw = (k2 << 24) | (q[:, 2] << 16) | (q[:, 1] << 8) | q[:, 0]
         keys.append((hiw.astype(np.uint64) << np.uint64(32)) | e)
     return keys
 
@@ -86,6 +91,13 @@ def test_rows_sorted_by_key(name, cu_funcfl):
         keys = host_keys(x, s.box_lo, s.box_hi, num, nb, spacing)
         bad = [i for i, k in enumerate(keys) if k.size > 1 and not np.all(k[1:] > k[:-1])]
         assert not bad, f"spacing {spacing}: rows not in key order, e.g. atoms {bad[:10]}"
+        if spacing == half and name == "jittered":
+            # fcc shells at a/2: |k|^2 = 2, 4, 6 (12 + 6 + 24 entries, within
+            # the 4.95 A cutoff), then 8 and 10 (12 + 24, the skin)
+            k2 = [(k >> np.uint64(56)).astype(np.int64) for k in keys]
+            full = [r for r in k2 if r.size == 78]
+            assert len(full) > len(k2) // 2
+            assert all(np.all(r[:42] <= 6) and np.all(r[42:] > 6) for r in full)
         rows = sorted_rows(num, nb)
         if ref is None:
             ref = (num.copy(), rows)
