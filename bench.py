@@ -52,8 +52,8 @@ B_ELEM = 7640
 # counts (fetch x2.000, write x1.000).  A profile-derived constant, not
 # measured by this script: TRAFFIC_SOURCE names the summary it comes from.
 TRAFFIC_SOURCE = "profiles/eam_c1_kernels.md (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)"
-TRAFFIC = {"atom_force": 207.9e6 + 22.2e6, "atom_rho": 200.8e6 + 26.9e6, "ip_force": 135.3e6 + 18.9e6,
-           "ip_erho": 120.3e6 + 12.2e6}
+TRAFFIC = {"atom_force": 189.9e6 + 28.3e6, "atom_rho": 184.8e6 + 27.2e6, "ip_force": 135.2e6 + 18.8e6,
+           "ip_erho": 120.2e6 + 12.2e6}
 
 
 def log(msg):
