@@ -123,10 +123,12 @@ int cac_set_neighbor(cac_ctx *ctx, double cutneigh, double triggersq, int every,
 /* Order of the entries within each atom list (no counterpart in the
  * reference, whose lists are in bin order, neighbor.cpp:583-613): entries are
  * sorted by their displacement x_j' - x_i rounded to multiples of `spacing`
- * (Angstrom), then by value.  On a lattice whose neighbor vectors are
- * multiples of `spacing` (a/2 for fcc) every atom then lists its neighbors in
- * the same relative order, which makes the force passes' gathers coalesce.
- * 0 (the default) sorts by value.  The neighbor sets do not change. */
+ * (Angstrom), k, ordered by |k|^2 and then k_z, k_y, k_x, then by value.  On a
+ * lattice whose neighbor vectors are multiples of `spacing` (a/2 for fcc)
+ * every atom then lists its neighbors in the same relative order, nearest
+ * shells first, which makes the force passes' gathers coalesce and lets them
+ * stop at the last shell within the cutoff.  0 (the default) sorts by value.
+ * The neighbor sets do not change. */
 int cac_set_list_order(cac_ctx *ctx, double spacing);
 
 /* Atom-element coupling (no counterpart in the reference, where atoms see
